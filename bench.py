@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Benchmark: RK3 substeps/s of the InterPartS substep on one MI355X.
+
+A "step" here is one RK3 substep (src/main.f90:268-414): the fused RK stage,
+the 3-pass IBM, the Poisson solve, the projection and the FP64 Newton-Euler /
+collision update, on the whole domain resident in HBM.  The workload is the
+256^3 channel with 1,000 spheres (D = 12h, density ratio 1.5, Re_bulk = 5000),
+FP64 path, synthetic seeded inputs.
+
+Prints one JSON line.  `value` is substeps/s with all inputs resident in HBM.
+`roofline` is for the dominant kernel: its algorithmic HBM bytes per launch
+divided by its average duration, measured here with HIP events on the
+library's stream over the K timed substeps.  `cpu_baseline` is the FP64 CPU
+oracle timed on this host on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(case, name, rkiter):
+    """Algorithmic HBM bytes of one launch of a kernel (FP64 words x 8)."""
+    N = float(case.itot * case.jtot * case.ktot)
+    S = (case.itot // 2 + 1) * 2.0 / case.itot * N  # half-spectrum words
+    words = {
+        "rk_stage": {1: 13.0, 2: 17.0, 3: 13.0}[rkiter] * N,  # reads u v w c p (+4 rk) ; writes 4 dt (+4 rk)
+        "fillps": 4.0 * N,
+        "fft_x_fwd": N + S,
+        "fft_y_fwd": 2.0 * S,
+        "thomas": 5.0 * S,            # spec r/w twice + d w/r (d is half a complex word)
+        "fft_y_inv": 2.0 * S,
+        "fft_x_inv": S + N,
+        "correc": 11.0 * N,           # dudt dvdt dwdt p pnew dcdt -> u v w pnew cnew
+    }
+    w = words.get(name)
+    return None if w is None else 8.0 * w
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300, help="timed substeps")
+    ap.add_argument("--warmup", type=int, default=30, help="untimed substeps")
+    ap.add_argument("--config", default="c2_256")
+    ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--cpu-substeps", type=int, default=3, help="oracle substeps in the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from interparts_mpich_amd.inputs import CONFIGS, initial_fields, place_spheres, markers_for
+    from interparts_mpich_amd.driver import Simulation
+    from interparts_mpich_amd import _lib
+
+    case = CONFIGS[args.config]
+    fields, spheres, markers = initial_fields(case), place_spheres(case), markers_for(case)
+    print(f"[bench] {case.name}: {case.itot}x{case.jtot}x{case.ktot}, np={case.np}, nl={case.nl}", file=sys.stderr, flush=True)
+    sim = Simulation(case, use_graphs=not args.no_graphs).initialize(fields, spheres, markers)
+    for s in range(args.warmup):
+        sim.substep(s % 3 + 1)
+    sim.sync()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        sim.substep(s % 3 + 1)
+    sim.sync()
+    t1 = time.perf_counter()
+    ms_per = (t1 - t0) * 1e3 / args.steps
+    value = args.steps / (t1 - t0)
+    print(f"[bench] {args.steps} substeps in {t1 - t0:.3f} s", file=sys.stderr, flush=True)
+
+    # per-kernel durations with HIP events over the same number of substeps
+    lib = _lib.load()
+    maxn = 64
+    ms = (C.c_double * maxn)()
+    names = (C.c_char_p * maxn)()
+    n = C.c_int()
+    acc = {}
+    for s in range(args.steps):
+        rk = s % 3 + 1
+        _lib.check(lib.ipg_substep_phases(sim.ctx, rk, maxn, ms, names, C.byref(n)), "substep_phases")
+        for q in range(n.value):
+            key = names[q].decode()
+            a = acc.setdefault(key, {"ms": 0.0, "launches": 0, "bytes": 0.0})
+            a["ms"] += ms[q]
+            a["launches"] += 1
+            b = kernel_bytes(case, key, rk)
+            if b is not None:
+                a["bytes"] += b
+    per_substep = {k: v["ms"] / args.steps for k, v in acc.items()}
+    dominant = max((k for k in acc if acc[k]["bytes"] > 0), key=lambda k: acc[k]["ms"])
+    d = acc[dominant]
+    avg_ms = d["ms"] / d["launches"]
+    bytes_per_launch = d["bytes"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    total_ms_events = sum(v["ms"] for v in acc.values()) / args.steps
+    print("[bench] per-substep kernel time (ms): " +
+          ", ".join(f"{k}={v:.4f}" for k, v in sorted(per_substep.items(), key=lambda kv: -kv[1])),
+          file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "RK3 substeps/s (256^3, 1k spheres) FP64",
+        "value": round(value, 3),
+        "unit": "substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Poiseuille + 0.1 U_bulk noise, seeded sphere placement)",
+        "config": {"workload": f"{case.name}: {case.itot}x{case.jtot}x{case.ktot} channel, {case.np} spheres "
+                               f"D={case.diameter_h:g}h nl={case.nl}, Re_bulk={case.re_bulk:g}, FP64 path",
+                   "graphs": not args.no_graphs,
+                   "gcell_updates_per_s": round(value * case.cells / 1e9, 4)},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch},
+        "kernel_ms_per_substep": {k: round(v, 5) for k, v in per_substep.items()},
+        "event_sum_ms_per_substep": round(total_ms_events, 4),
+    }
+    sim.close()
+
+    if not args.no_cpu_baseline:
+        from oracle.pyoracle import Oracle
+        orc = Oracle(case).initialize(fields, spheres, markers)
+        nsub = args.cpu_substeps
+        t0 = time.perf_counter()
+        for s in range(nsub):
+            orc.substep(s % 3 + 1)
+        t1 = time.perf_counter()
+        orc.close()
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": round(nsub / (t1 - t0), 5), "unit": "substeps/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"{nsub} FP64 substeps of {case.name} by the C oracle (OpenMP over "
+                                         f"field loops, {cores} threads), {t1 - t0:.1f} s"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+/*
+ * ipg.h -- C-ABI of the MI355X-native InterPartS RK3 substep (libipg.so).
+ *
+ * One context holds one case resident in HBM: the staggered fields with one
+ * ghost layer, the IBM force buffers, the Poisson spectra, the particle and
+ * marker state.  Every entry point returns 0 on success and a non-zero code on
+ * failure (ipg_last_error() gives the message); the Fortran layer turns a
+ * non-zero code into `error stop`, as the reference aborts inside the routine.
+ *
+ * The entry points replace the Fortran module procedures of
+ * ajayhudge/InterPartS_MPICH named beside each declaration (file:line of the
+ * reference).  The reference's procedures work on module globals; here the
+ * globals live in the context, so the array arguments of the reference become
+ * the context handle plus field selectors.  Host arrays handed over at
+ * upload/download use the reference's layout: real(8) (0:i1,0:j1,0:k1),
+ * i fastest.
+ */
+#ifndef IPG_H
+#define IPG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ipg_ctx ipg_ctx;
+
+/* &PARAMETERS namelist (src/param.f90:66-86) plus the compile-time sizes of
+ * src/gridsize.f90:6-26, which are runtime values here. */
+typedef struct ipg_params {
+  int itot, jtot, ktot;          /* grid (src/gridsize.f90:6-8) */
+  int np;                        /* number of spheres (src/gridsize.f90:18) */
+  int nl;                        /* markers per sphere; must equal nl of src/init.f90:129 */
+  double lx, ly, lz;
+  double visc, pran, beta, radius, ratiorho;
+  int particle_rotation;         /* logical particle_rotation */
+  int bc_uvw_top, bc_uvw_bot;    /* 0 'BC_NOSLIP', 1 'BC_FREESLIP' */
+  int bc_c_top, bc_c_bot;        /* 0 'BC_DIRICHLET', 1 'BC_NEUMANN' */
+  double bc_c_top_val, bc_c_bot_val;
+  int precision;                 /* 0 FP64 path, 1 mixed precision */
+} ipg_params;
+
+/* field selectors for ipg_field_put / ipg_field_get */
+enum {
+  IPG_U = 0, IPG_V, IPG_W, IPG_PNEW, IPG_C,         /* unew vnew wnew pnew cnew (src/common.f90:4) */
+  IPG_DUDT, IPG_DVDT, IPG_DWDT, IPG_DCDT,           /* dudt dvdt dwdt dcdt (src/common.f90:7) */
+  IPG_DURK, IPG_DVRK, IPG_DWRK, IPG_DCRK,           /* durkold.. (src/main.f90:66) */
+  IPG_P,                                            /* correction pressure p (src/main.f90:67) */
+  IPG_NFIELDS
+};
+
+/* particle quantity selectors (type particle, src/common.f90:35-71) */
+enum {
+  IPG_PX = 0, IPG_PY, IPG_PZ, IPG_PTHETA, IPG_PPHI, IPG_PU, IPG_PV, IPG_PW, IPG_POMX, IPG_POMY, IPG_POMZ,
+  IPG_POMTHETA, IPG_PINTU, IPG_PINTV, IPG_PINTW, IPG_PINTOMX, IPG_PINTOMY, IPG_PINTOMZ, IPG_PCOLFX,
+  IPG_PCOLFY, IPG_PCOLFZ, IPG_PCOLTX, IPG_PCOLTY, IPG_PCOLTZ, IPG_PFXLTOT, IPG_PFYLTOT, IPG_PFZLTOT,
+  IPG_PTORQX, IPG_PTORQY, IPG_PTORQZ, IPG_PTORQTHETA,
+  IPG_NPART
+};
+
+/* ---- context -------------------------------------------------------- */
+/* Replaces initmpi + input_parameters + derived_parameters
+ * (src/initmpi.f90:12, src/init.f90:10-243) for one rank holding the domain. */
+int ipg_init(const ipg_params *prm, ipg_ctx **ctx);
+int ipg_finalize(ipg_ctx *ctx);
+const char *ipg_last_error(void);
+/* nl = nint((pi/3)(12((radius-0.3/dxi)dxi)^2+1)), src/init.f90:127-133 */
+int ipg_nl_formula(int itot, double lx, double radius);
+
+/* ---- state transfer (host <-> HBM) ----------------------------------- */
+/* host: real(8) (0:i1,0:j1,0:k1) Fortran order, (itot+2)(jtot+2)(ktot+2) values */
+int ipg_field_put(ipg_ctx *ctx, int field, const double *host);
+int ipg_field_get(ipg_ctx *ctx, int field, double *host);
+/* host: np values of one particle quantity */
+int ipg_particles_put(ipg_ctx *ctx, int quantity, const double *host);
+int ipg_particles_get(ipg_ctx *ctx, int quantity, double *host);
+/* dt and time (src/common.f90:9) */
+int ipg_set_dt(ipg_ctx *ctx, double dt);
+int ipg_get_dt(ipg_ctx *ctx, double *dt);
+
+/* ---- set-up ---------------------------------------------------------- */
+/* src/initsolver.f90:10-67: tridiagonal coefficients and eigenvalues */
+int ipg_initsolver(ipg_ctx *ctx);
+/* src/initparticles.f90:159-241: positions given, velocities zero, neighbour flags */
+int ipg_initparticles(ipg_ctx *ctx, const double *x, const double *y, const double *z);
+/* src/coordsfp.f90:10-106: marker angles (thetarc, phirc, radfp columns of
+ * lagrangianforcepoints2), dVlagr, dVeul */
+int ipg_coordsfp(ipg_ctx *ctx, const double *thetarc, const double *phirc, double radfp);
+
+/* ---- the RK3 substep, one entry per reference procedure -------------- */
+int ipg_rk(ipg_ctx *ctx, int rkiter);                 /* rk1/rk2/rk3, src/rk3.f90:13,108,224 */
+int ipg_ibm(ipg_ctx *ctx);                            /* IBM block, src/main.f90:286-338:
+                                                         eulr2lagr src/interp_spread.f90:12,
+                                                         complagrforces/updtlagrforces src/forcing.f90:9,189,
+                                                         lagr2eulr src/interp_spread.f90:230 */
+int ipg_bounduvw(ipg_ctx *ctx, int which);            /* src/bound.f90:11; which 0 = unew.., 1 = dudt.. */
+int ipg_boundp(ipg_ctx *ctx, int which);              /* src/bound.f90:82; which 0 = pnew, 1 = p */
+int ipg_boundscal(ipg_ctx *ctx);                      /* src/bound.f90:109 on cnew */
+int ipg_fillps(ipg_ctx *ctx, int rkiter);             /* src/fillps.f90:8 */
+int ipg_solver2d(ipg_ctx *ctx);                       /* src/solver.f90:43 */
+int ipg_correc(ipg_ctx *ctx, int rkiter);             /* src/correc.f90:11 */
+int ipg_pnew_update(ipg_ctx *ctx);                    /* src/main.f90:368-376 */
+int ipg_intgr_over_sphere(ipg_ctx *ctx, int type);    /* src/intgr_over_sphere.f90:12 */
+int ipg_intgr_nwtn_eulr(ipg_ctx *ctx, int rkiter);    /* src/intgr_nwtn_eulr.f90:12 */
+int ipg_substep(ipg_ctx *ctx, int rkiter);            /* src/main.f90:268-414, whole substep */
+int ipg_chkdt(ipg_ctx *ctx, double *dtmax);           /* src/chkdt.f90:6 */
+int ipg_chkdiv(ipg_ctx *ctx, double *divtot, double *divmax); /* src/chkdiv.f90:7 */
+
+/* ---- runtime controls (no reference counterpart) --------------------- */
+int ipg_sync(ipg_ctx *ctx);                           /* wait for the context's stream */
+int ipg_use_graphs(ipg_ctx *ctx, int on);             /* capture each substep as a hipGraph */
+/* HBM bytes one substep must move by the algorithmic model, for roofline */
+int ipg_substep_bytes(ipg_ctx *ctx, double *bytes);
+/* one eager substep with a HIP event after every kernel launch: per-launch
+ * durations (ms) and kernel names, at most maxn, count in *n */
+int ipg_substep_phases(ipg_ctx *ctx, int rkiter, int maxn, double *ms, const char **names, int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

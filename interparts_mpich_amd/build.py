@@ -1,0 +1,65 @@
+"""Build steps: libipg.so (HIP, gfx950), the Fortran host driver, and the
+test-side oracle libraries.
+
+    python -m interparts_mpich_amd.build
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HIP_SOURCES = ["ipg_api.hip", "ipg_fields.hip", "ipg_poisson.hip", "ipg_ibm.hip", "ipg_particles.hip"]
+LIB = os.path.join(HERE, "libipg.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+             # FP64 parity: no contraction of a*b+c into fma, the reference's
+             # rounding sequence is kept
+             "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "unknown target feature" not in l and "packed-fp32" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build_hip():
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + srcs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran():
+    """The Fortran host driver (fortran/) against libipg.so, with amdflang."""
+    fdir = os.path.join(HERE, "fortran")
+    if not os.path.exists(os.path.join(fdir, "Makefile")):
+        return None
+    _run(["make", "-s", "-C", fdir])
+    return os.path.join(fdir, "interpt_gpu")
+
+
+def build_oracle():
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", odir, "liboracle.so"])
+    if os.path.isdir("/root/reference/src"):
+        _run(["make", "-s", "-C", odir, "ref"])
+
+
+def build_all():
+    build_hip()
+    build_fortran()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB)

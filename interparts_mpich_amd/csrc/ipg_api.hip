@@ -1,0 +1,450 @@
+// ipg_api.hip -- the C-ABI of libipg.so (declared in include/ipg.h): context
+// set-up with the reference's derived parameters, host <-> HBM transfers, and
+// the substep sequence of src/main.f90:268-414, optionally captured as one
+// hipGraph per RK stage.
+#include "ipg_internal.h"
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+static thread_local std::string g_err;
+void ipg_set_error(const std::string &msg) { g_err = msg; }
+
+extern "C" const char *ipg_last_error(void) { return g_err.c_str(); }
+
+int ipg_launch_correc_fused(ipg_ctx *c, int rkiter);
+
+namespace {
+
+double sq(double x) { return x * x; }
+
+// src/init.f90:74-243 derived_parameters and src/param.f90:95-120 constants.
+// x**2. is written x*x and x**(-1.) 1/x (gfortran folds both), x**3. pow().
+void derived(ipg_ctx *c) {
+  const ipg_params &p = c->prm;
+  Grid &g = c->g;
+  PartConst &pc = c->pc;
+  const double pi = std::acos(-1.0);
+  g.nx = p.itot; g.ny = p.jtot; g.nz = p.ktot;
+  g.lx = p.lx; g.ly = p.ly; g.lz = p.lz;
+  g.dxi = p.itot / p.lx; g.dyi = p.jtot / p.ly; g.dzi = p.ktot / p.lz;
+  g.dx = 1.0 / g.dxi; g.dy = 1.0 / g.dyi; g.dz = 1.0 / g.dzi;
+  g.visc = p.visc; g.kappa = p.visc / p.pran; g.beta = p.beta;
+  g.bc_uvw_top = p.bc_uvw_top; g.bc_uvw_bot = p.bc_uvw_bot; g.bc_c_top = p.bc_c_top; g.bc_c_bot = p.bc_c_bot;
+  g.bc_c_top_val = p.bc_c_top_val; g.bc_c_bot_val = p.bc_c_bot_val;
+  g.sx = ((long)p.itot + 2 + 15) / 16 * 16;
+  g.sxy = g.sx * (p.jtot + 2);
+  pc.np = p.np; pc.nl = p.nl; pc.rotation = p.particle_rotation;
+  pc.radius = p.radius; pc.ratiorho = p.ratiorho;
+  pc.offset = (std::sqrt(3.0 * (1.5 * 1.5))) / g.dxi + 0.01 / g.dxi;
+  pc.volp = ((4.0 / 3.0) * pi) * std::pow(p.radius, 3.0);
+  pc.mominert = ((2.0 / 5.0) * pc.volp) * sq(p.radius);
+  pc.gaccx = -9.81 * std::cos(pi / 2.0);
+  pc.gaccy = -9.81 * std::cos(pi / 2.0);
+  pc.gaccz = -9.81 * std::sin(pi / 2.0);
+  pc.rkab[0] = 32.0 / 60.0 + 0.0;
+  pc.rkab[1] = 25.0 / 60.0 + (-17.0 / 60.0);
+  pc.rkab[2] = 45.0 / 60.0 + (-25.0 / 60.0);
+  const double Nstretch = 8.0, dt_estim = 0.05, en = 0.97;
+  const double eps_ini_pp = 0.025, eps_sat_pp = 0.001, eps_ini_pw = 0.075, eps_sat_pw = 0.001;
+  const double meffn_ss = p.ratiorho * pc.volp / 2.0, meffn_sw = p.ratiorho * pc.volp;
+  pc.kn_ss = (sq(pi) + sq(std::fabs(std::log(en)))) * meffn_ss / sq(Nstretch * dt_estim);
+  pc.etan_ss = -2.0 * (std::log(en)) * meffn_ss / (Nstretch * dt_estim);
+  pc.kn_sw = (sq(pi) + sq(std::fabs(std::log(en)))) * meffn_sw / sq(Nstretch * dt_estim);
+  pc.etan_sw = -2.0 * (std::log(en)) * meffn_sw / (Nstretch * dt_estim);
+  pc.coeff_f = 6.0 * pi * p.visc * p.radius;
+  double a11 = -(1.0 / 4.0 * (1.0 / eps_ini_pp)) + 9.0 / 40.0 * std::log(eps_ini_pp) + 3.0 / 112.0 * eps_ini_pp * std::log(eps_ini_pp);
+  pc.a11a_ini_pp = a11 - .995; pc.a11b_ini_pp = -a11 + .350;
+  a11 = -(1.0 / 4.0 * (1.0 / eps_sat_pp)) + 9.0 / 40.0 * std::log(eps_sat_pp) + 3.0 / 112.0 * eps_sat_pp * std::log(eps_sat_pp);
+  pc.a11a_sat_pp = a11 - .995; pc.a11b_sat_pp = -a11 + .350;
+  pc.a11_ini_pw = -(1.0 / eps_ini_pw) + 1.0 / 5.0 * std::log(eps_ini_pw) + 1.0 / 21.0 * eps_ini_pw * std::log(eps_ini_pw) - .9713;
+  pc.a11_sat_pw = -(1.0 / eps_sat_pw) + 1.0 / 5.0 * std::log(eps_sat_pw) + 1.0 / 21.0 * eps_sat_pw * std::log(eps_sat_pw) - .9713;
+  pc.dVeul = 1.0 / (g.dxi * g.dyi * g.dzi);
+}
+
+bool check_ctx(ipg_ctx *c) {
+  if (!c) { ipg_set_error("null context"); return false; }
+  return true;
+}
+
+int release(ipg_ctx *c) {
+  for (int r = 0; r < 3; r++)
+    if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
+  for (int q = 0; q < 64; q++)
+    if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
+  ipg_poisson_free(c);
+  void *ptrs[] = {c->slab, c->mk, c->fl, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1]};
+  for (void *q : ptrs) if (q) (void)hipFree(q);
+  for (int b = 0; b < 2; b++)
+    for (int d = 0; d < 3; d++) if (c->G[b][d]) (void)hipFree(c->G[b][d]);
+  for (int q = 0; q < IPG_NPART; q++) if (c->ps.q[q]) (void)hipFree(c->ps.q[q]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  return 0;
+}
+
+int substep_launches(ipg_ctx *c, int rkiter) {
+  if (ipg_launch_rk(c, rkiter)) return 1;
+  if (c->pc.np > 0 && ipg_launch_ibm(c)) return 1;
+  if (ipg_launch_bound(c, BND_UVW_DT)) return 1;
+  if (ipg_launch_fillps(c, rkiter)) return 1;
+  if (ipg_launch_poisson(c)) return 1;
+  if (ipg_launch_bound(c, BND_P)) return 1;
+  if (ipg_launch_correc_fused(c, rkiter)) return 1;
+  if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C)) return 1;
+  if (c->pc.np > 0 && ipg_launch_nwtn_eulr(c, rkiter)) return 1;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ipg_nl_formula(int itot, double lx, double radius) {
+  const double pi = std::acos(-1.0), dxi = itot / lx, retrac = 0.3 / dxi;
+  return (int)std::lround((pi / 3.0) * (12.0 * sq((radius - retrac) * dxi) + 1.0));
+}
+
+int ipg_init(const ipg_params *prm, ipg_ctx **out) {
+  if (!prm || !out) { ipg_set_error("ipg_init: null argument"); return 1; }
+  *out = nullptr;
+  if (prm->itot < 4 || prm->jtot < 4 || prm->ktot < 4) { ipg_set_error("ipg_init: grid too small"); return 1; }
+  if (prm->np < 0 || (prm->np > 0 && prm->nl <= 0)) { ipg_set_error("ipg_init: bad np/nl"); return 1; }
+  if (prm->np > 0 && prm->nl != ipg_nl_formula(prm->itot, prm->lx, prm->radius)) {
+    // src/main.f90:117-133: the run stops when nl /= nlmax
+    ipg_set_error("ipg_init: nl does not match the marker count of src/init.f90:129 (nl /= nlmax)");
+    return 2;
+  }
+  if (prm->precision != 0) { ipg_set_error("ipg_init: only precision 0 (FP64) is built"); return 1; }
+  ipg_ctx *c = new ipg_ctx();
+  std::memset((void *)c, 0, sizeof(ipg_ctx));
+  c->prm = *prm;
+  derived(c);
+  auto fail = [&](int code) { release(c); delete c; return code; };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { ipg_set_error("stream create failed"); return fail(1); }
+  const Grid &g = c->g;
+  c->nfield = 16 + (size_t)g.sxy * (g.nz + 2) + 16;
+  c->nfield = (c->nfield + 15) / 16 * 16;
+  if (hipMalloc(&c->slab, c->nfield * IPG_NFIELDS * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory (fields)"); return fail(1); }
+  if (hipMemset(c->slab, 0, c->nfield * IPG_NFIELDS * sizeof(double)) != hipSuccess) { ipg_set_error("memset failed"); return fail(1); }
+  for (int q = 0; q < IPG_NFIELDS; q++) c->f[q] = c->slab + (size_t)q * c->nfield + 16;
+  const size_t ncell = (size_t)g.nx * g.ny * g.nz, nplane = (size_t)g.nx * g.ny;
+  const int np = prm->np;
+  if (np > 0) {
+    for (int b = 0; b < 2; b++)
+      for (int d = 0; d < 3; d++) {
+        if (hipMalloc(&c->G[b][d], ncell * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory (G)"); return fail(1); }
+        if (hipMemset(c->G[b][d], 0, ncell * sizeof(double)) != hipSuccess) { ipg_set_error("memset failed"); return fail(1); }
+      }
+    for (int b = 0; b < 2; b++) {
+      if (hipMalloc(&c->K1[b], nplane * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory"); return fail(1); }
+      if (hipMemset(c->K1[b], 0, nplane * sizeof(double)) != hipSuccess) { ipg_set_error("memset failed"); return fail(1); }
+    }
+    const size_t M = (size_t)np * prm->nl;
+    if (hipMalloc(&c->mk, 3 * (size_t)prm->nl * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->fl, 3 * M * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->op, 24 * (size_t)np * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->ps.nbf, (size_t)np * IPG_NB * sizeof(int)) != hipSuccess) {
+      ipg_set_error("out of device memory (particles)"); return fail(1);
+    }
+    (void)hipMemset(c->fl, 0, 3 * M * sizeof(double));
+    (void)hipMemset(c->ps.nbf, 0, (size_t)np * IPG_NB * sizeof(int));
+    for (int q = 0; q < IPG_NPART; q++) {
+      if (hipMalloc(&c->ps.q[q], (size_t)np * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory"); return fail(1); }
+      (void)hipMemset(c->ps.q[q], 0, (size_t)np * sizeof(double));
+    }
+  }
+  size_t nred = 4096 + 3 * (size_t)np + 64;
+  if (hipMalloc(&c->red, nred * sizeof(double)) != hipSuccess || hipMalloc(&c->dscal, 16 * sizeof(double)) != hipSuccess) {
+    ipg_set_error("out of device memory (scratch)"); return fail(1);
+  }
+  (void)hipMemset(c->red, 0, nred * sizeof(double));
+  (void)hipMemset(c->dscal, 0, 16 * sizeof(double));
+  if (hipDeviceSynchronize() != hipSuccess) { ipg_set_error("device sync failed at init"); return fail(1); }
+  *out = c;
+  return 0;
+}
+
+int ipg_finalize(ipg_ctx *c) {
+  if (!c) return 0;
+  (void)hipStreamSynchronize(c->stream);
+  release(c);
+  delete c;
+  return 0;
+}
+
+int ipg_field_put(ipg_ctx *c, int field, const double *host) {
+  if (!check_ctx(c)) return 1;
+  if (field < 0 || field >= IPG_NFIELDS || !host) { ipg_set_error("field_put: bad field"); return 1; }
+  const Grid &g = c->g;
+  IPG_HIP(hipMemcpy2DAsync(c->f[field] - 1, g.sx * sizeof(double), host, (g.nx + 2) * sizeof(double),
+                           (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyHostToDevice, c->stream));
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_field_get(ipg_ctx *c, int field, double *host) {
+  if (!check_ctx(c)) return 1;
+  if (field < 0 || field >= IPG_NFIELDS || !host) { ipg_set_error("field_get: bad field"); return 1; }
+  const Grid &g = c->g;
+  IPG_HIP(hipMemcpy2DAsync(host, (g.nx + 2) * sizeof(double), c->f[field] - 1, g.sx * sizeof(double),
+                           (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyDeviceToHost, c->stream));
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_particles_put(ipg_ctx *c, int quantity, const double *host) {
+  if (!check_ctx(c)) return 1;
+  if (quantity < 0 || quantity >= IPG_NPART || !host) { ipg_set_error("particles_put: bad quantity"); return 1; }
+  if (c->pc.np == 0) return 0;
+  IPG_HIP(hipMemcpyAsync(c->ps.q[quantity], host, c->pc.np * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_particles_get(ipg_ctx *c, int quantity, double *host) {
+  if (!check_ctx(c)) return 1;
+  if (quantity < 0 || quantity >= IPG_NPART || !host) { ipg_set_error("particles_get: bad quantity"); return 1; }
+  if (c->pc.np == 0) return 0;
+  IPG_HIP(hipMemcpyAsync(host, c->ps.q[quantity], c->pc.np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_set_dt(ipg_ctx *c, double dt) {
+  if (!check_ctx(c)) return 1;
+  c->dt_host = dt;
+  IPG_HIP(hipMemcpyAsync(c->dscal, &c->dt_host, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_get_dt(ipg_ctx *c, double *dt) {
+  if (!check_ctx(c) || !dt) return 1;
+  *dt = c->dt_host;
+  return 0;
+}
+
+int ipg_initsolver(ipg_ctx *c) {
+  if (!check_ctx(c)) return 1;
+  if (c->solver_ready) return 0;
+  if (ipg_poisson_setup(c)) return 1;
+  c->solver_ready = true;
+  return 0;
+}
+
+int ipg_initparticles(ipg_ctx *c, const double *x, const double *y, const double *z) {
+  if (!check_ctx(c)) return 1;
+  const int np = c->pc.np;
+  if (np == 0) return 0;
+  std::vector<double> zero(np, 0.0);
+  for (int q = 0; q < IPG_NPART; q++)
+    if (ipg_particles_put(c, q, zero.data())) return 1;
+  for (int p = 0; p < np; p++) {
+    // src/initparticles.f90:173-184: particle outside the domain aborts
+    if (x[p] < 0.0 || x[p] > c->g.lx || y[p] < 0.0 || y[p] > c->g.ly || z[p] < c->pc.radius || z[p] > c->g.lz - c->pc.radius) {
+      ipg_set_error("initparticles: particle outside the domain");
+      return 2;
+    }
+  }
+  if (ipg_particles_put(c, IPG_PX, x) || ipg_particles_put(c, IPG_PY, y) || ipg_particles_put(c, IPG_PZ, z)) return 1;
+  if (ipg_launch_nbflags(c)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_coordsfp(ipg_ctx *c, const double *thetarc, const double *phirc, double radfp) {
+  if (!check_ctx(c)) return 1;
+  const int nl = c->pc.nl;
+  if (c->pc.np == 0) return 0;
+  std::vector<double> mk(3 * (size_t)nl);
+  for (int l = 0; l < nl; l++) {
+    // src/coordsfp.f90:49-51 with theta*0. = phi*0. = 0
+    mk[l] = radfp * std::sin(0.0 + thetarc[l]) * std::cos(0.0 + phirc[l]);
+    mk[nl + l] = radfp * std::sin(0.0 + thetarc[l]) * std::sin(0.0 + phirc[l]);
+    mk[2 * nl + l] = radfp * std::cos(0.0 + thetarc[l]);
+  }
+  IPG_HIP(hipMemcpy(c->mk, mk.data(), mk.size() * sizeof(double), hipMemcpyHostToDevice));
+  const double pi = std::acos(-1.0), dxi = c->g.dxi;
+  c->pc.radfp = radfp;
+  // src/coordsfp.f90:84-89
+  c->pc.dVlagr = ((4.0 / 3.0) * pi * std::pow(radfp + 0.5 / dxi, 3.0) - (4.0 / 3.0) * pi * std::pow(radfp - 0.5 / dxi, 3.0)) / (1.0 * nl);
+  c->pc.dVeul = 1.0 / (c->g.dxi * c->g.dyi * c->g.dzi);
+  c->markers_ready = true;
+  return 0;
+}
+
+int ipg_rk(ipg_ctx *c, int rkiter) {
+  if (!check_ctx(c)) return 1;
+  if (rkiter < 1 || rkiter > 3) { ipg_set_error("rk: rkiter must be 1..3"); return 1; }
+  if (ipg_launch_rk(c, rkiter)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_ibm(ipg_ctx *c) {
+  if (!check_ctx(c)) return 1;
+  if (ipg_launch_ibm(c)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_bounduvw(ipg_ctx *c, int which) {
+  if (!check_ctx(c)) return 1;
+  if (ipg_launch_bound(c, which == 0 ? BND_UVW_NEW : BND_UVW_DT)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_boundp(ipg_ctx *c, int which) {
+  if (!check_ctx(c)) return 1;
+  if (ipg_launch_bound(c, which == 0 ? BND_PNEW : BND_P)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_boundscal(ipg_ctx *c) {
+  if (!check_ctx(c)) return 1;
+  if (ipg_launch_bound(c, BND_C)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_fillps(ipg_ctx *c, int rkiter) {
+  if (!check_ctx(c)) return 1;
+  if (rkiter < 1 || rkiter > 3) { ipg_set_error("fillps: rkiter must be 1..3"); return 1; }
+  if (ipg_launch_fillps(c, rkiter)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_solver2d(ipg_ctx *c) {
+  if (!check_ctx(c)) return 1;
+  if (!c->solver_ready) { ipg_set_error("solver2d: ipg_initsolver has not been called"); return 1; }
+  if (ipg_launch_poisson(c)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_correc(ipg_ctx *c, int rkiter) {
+  if (!check_ctx(c)) return 1;
+  if (rkiter < 1 || rkiter > 3) { ipg_set_error("correc: rkiter must be 1..3"); return 1; }
+  if (ipg_launch_correc(c, rkiter)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_pnew_update(ipg_ctx *c) {
+  if (!check_ctx(c)) return 1;
+  if (ipg_launch_pnew(c)) return 1;
+  if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_intgr_over_sphere(ipg_ctx *c, int type) {
+  if (!check_ctx(c)) return 1;
+  if (type < 1 || type > 3) { ipg_set_error("intgr_over_sphere: type must be 1..3"); return 1; }
+  if (ipg_launch_sphere_integrals(c, type)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_intgr_nwtn_eulr(ipg_ctx *c, int rkiter) {
+  if (!check_ctx(c)) return 1;
+  if (rkiter < 1 || rkiter > 3) { ipg_set_error("intgr_nwtn_eulr: rkiter must be 1..3"); return 1; }
+  if (ipg_launch_nwtn_eulr(c, rkiter)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_substep(ipg_ctx *c, int rkiter) {
+  if (!check_ctx(c)) return 1;
+  if (rkiter < 1 || rkiter > 3) { ipg_set_error("substep: rkiter must be 1..3"); return 1; }
+  if (!c->solver_ready) { ipg_set_error("substep: ipg_initsolver has not been called"); return 1; }
+  if (c->pc.np > 0 && !c->markers_ready) { ipg_set_error("substep: ipg_coordsfp has not been called"); return 1; }
+  const int r = rkiter - 1;
+  if (c->use_graphs) {
+    if (!c->graph_ok[r]) {
+      hipGraph_t graph;
+      IPG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
+      int rc = substep_launches(c, rkiter);
+      hipError_t e = hipStreamEndCapture(c->stream, &graph);
+      if (rc) return rc;
+      if (e != hipSuccess) { ipg_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return 1; }
+      IPG_HIP(hipGraphInstantiate(&c->graph[r], graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      c->graph_ok[r] = true;
+    }
+    IPG_HIP(hipGraphLaunch(c->graph[r], c->stream));
+    return 0;
+  }
+  return substep_launches(c, rkiter);
+}
+
+int ipg_chkdt(ipg_ctx *c, double *dtmax) {
+  if (!check_ctx(c) || !dtmax) return 1;
+  return ipg_launch_chkdt(c, dtmax);
+}
+
+int ipg_chkdiv(ipg_ctx *c, double *tot, double *mx) {
+  if (!check_ctx(c) || !tot || !mx) return 1;
+  return ipg_launch_chkdiv(c, tot, mx);
+}
+
+int ipg_sync(ipg_ctx *c) {
+  if (!check_ctx(c)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_substep_phases(ipg_ctx *c, int rkiter, int maxn, double *ms, const char **names, int *n) {
+  if (!check_ctx(c) || !ms || !names || !n) return 1;
+  if (rkiter < 1 || rkiter > 3) { ipg_set_error("substep_phases: rkiter must be 1..3"); return 1; }
+  for (int q = 0; q < 64; q++)
+    if (!c->ev[q]) IPG_HIP(hipEventCreate(&c->ev[q]));
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  c->timing = true;
+  c->nev = 0;
+  IPG_HIP(hipEventRecord(c->ev[0], c->stream));
+  int rc = substep_launches(c, rkiter);
+  c->timing = false;
+  if (rc) return rc;
+  IPG_HIP(hipEventSynchronize(c->ev[c->nev]));
+  int k = c->nev < maxn ? c->nev : maxn;
+  for (int q = 0; q < k; q++) {
+    float t = 0.f;
+    IPG_HIP(hipEventElapsedTime(&t, c->ev[q], c->ev[q + 1]));
+    ms[q] = t;
+    names[q] = c->evname[q];
+  }
+  *n = k;
+  return 0;
+}
+
+int ipg_use_graphs(ipg_ctx *c, int on) {
+  if (!check_ctx(c)) return 1;
+  c->use_graphs = on != 0;
+  return 0;
+}
+
+// Algorithmic HBM bytes of one substep (FP64), the roofline numerator:
+// fused RK stage (stage-averaged 14.33 N words), fillps 4 N, Poisson
+// x-fwd 2 + y-fwd 2 + Thomas (2 + 0.5 d writes + 2 + 0.5 d reads) + y-inv 2 +
+// x-inv 2 words per cell (half spectrum of nx/2+1 complex = N words), correc
+// with pnew/cnew 11 N, ghost fills O(n^2) omitted; IBM per pass 27-point
+// stencils x3 components of marker-unique cells counted once, markers 15 words.
+int ipg_substep_bytes(ipg_ctx *c, double *bytes) {
+  if (!check_ctx(c) || !bytes) return 1;
+  const double N = (double)c->g.nx * c->g.ny * c->g.nz;
+  const double s = 8.0;
+  double fluid = (14.0 + 1.0 / 3.0) * N + 4.0 * N + 13.0 * N + 11.0 * N;
+  const double np = c->pc.np, R = c->pc.radius, h = c->g.dx;
+  double cells = np * 4.0 * 3.14159265358979 * (R - 0.3 * h) * (R - 0.3 * h) * 3.0 * h / (h * h * h);
+  double ibm = 3.0 * (3.0 * cells * 3.0 + 15.0 * np * c->pc.nl);
+  double box = np * 3.0 * std::pow(2.0 * R / h + 5.0, 3.0);
+  *bytes = s * (fluid + ibm + box);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,400 @@
+// ipg_fields.hip -- field kernels of the RK3 substep: fused RK stage
+// (momentum + scalar + pressure gradient + RK combination), boundary/ghost
+// fill, Poisson right-hand side, projection, divergence and time-step checks.
+// FP64, with the reference's operation order kept term by term (the library
+// is compiled with -ffp-contract=off so no multiply-add is fused).
+#include "ipg_internal.h"
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+// One RK stage, src/rk3.f90:13-106 (STAGE 1), 108-222 (2), 224-329 (3), with
+// momxad/momyad/momzad/momxp/momyp/momzp/momzbuoy (src/mom.f90) and scalad
+// (src/scal.f90:11-60) evaluated in registers: one read of each input and one
+// write of each output per cell.
+template <int STAGE>
+__global__ __launch_bounds__(BX *BY) void k_rk(Grid g, const double *__restrict__ U, const double *__restrict__ V,
+                                               const double *__restrict__ W, const double *__restrict__ C,
+                                               const double *__restrict__ P, double *__restrict__ dudt,
+                                               double *__restrict__ dvdt, double *__restrict__ dwdt,
+                                               double *__restrict__ dcdt, double *__restrict__ durk,
+                                               double *__restrict__ dvrk, double *__restrict__ dwrk,
+                                               double *__restrict__ dcrk, const double *__restrict__ dtp) {
+  const int i = blockIdx.x * BX + threadIdx.x + 1;
+  const int j = blockIdx.y * BY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long sx = g.sx, sxy = g.sxy;
+  const long c = g.at(i, j, k);
+  const long ip = c + 1, im = c - 1, jp = c + sx, jm = c - sx, kp = c + sxy, km = c - sxy;
+  const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, visc = g.visc, kappa = g.kappa;
+  const double dt = *dtp;
+  double f1, f2 = 0.0, f3 = 0.0;
+  if (STAGE == 1) f1 = dt * (32.0 / 60.0);
+  else if (STAGE == 2) { f1 = dt * (8.0 / 60.0); f2 = dt * (25.0 / 60.0); f3 = dt * (17.0 / 60.0); }
+  else { f1 = dt * (20.0 / 60.0); f2 = dt * (45.0 / 60.0); f3 = dt * (25.0 / 60.0); }
+
+  const double Uc = U[c], Vc = V[c], Wc = W[c];
+  // momxp/momyp/momzp (src/mom.f90:56-79,129-155,205-231)
+  const double gpx = -(dxi * (P[ip] - P[c]));
+  const double gpy = -(dyi * (P[jp] - P[c]));
+  const double gpz = -(dzi * (P[kp] - P[c]));
+  const double du0 = Uc + f1 * gpx, dv0 = Vc + f1 * gpy, dw0 = Wc + f1 * gpz;
+  double ax, ay, az, ac;
+  {  // momxad, src/mom.f90:30-47
+    const double Uip = U[ip], Uim = U[im], Ujp = U[jp], Ujm = U[jm], Ukp = U[kp], Ukm = U[km];
+    double uuip = 0.25 * (Uip + Uc) * (Uip + Uc);
+    double uuim = 0.25 * (Uim + Uc) * (Uim + Uc);
+    double uvjp = 0.25 * (Ujp + Uc) * (V[ip] + Vc);
+    double uvjm = 0.25 * (Ujm + Uc) * (V[ip - sx] + V[jm]);
+    double uwkp = 0.25 * (Ukp + Uc) * (W[ip] + Wc);
+    double uwkm = 0.25 * (Ukm + Uc) * (W[ip - sxy] + W[km]);
+    double dudxp = (Uip - Uc) * dxi, dudxm = (Uc - Uim) * dxi;
+    double dudyp = (Ujp - Uc) * dyi, dudym = (Uc - Ujm) * dyi;
+    double dudzp = (Ukp - Uc) * dzi, dudzm = (Uc - Ukm) * dzi;
+    ax = dxi * (-uuip + uuim) + visc * (dudxp - dudxm) * dxi + dyi * (-uvjp + uvjm) + visc * (dudyp - dudym) * dyi +
+         dzi * (-uwkp + uwkm) + visc * (dudzp - dudzm) * dzi;
+  }
+  {  // momyad, src/mom.f90:103-120
+    const double Vip = V[ip], Vim = V[im], Vjp = V[jp], Vjm = V[jm], Vkp = V[kp], Vkm = V[km];
+    double uvip = 0.25 * (Uc + U[jp]) * (Vc + Vip);
+    double uvim = 0.25 * (U[im] + U[im + sx]) * (Vc + Vim);
+    double vvjp = 0.25 * (Vc + Vjp) * (Vc + Vjp);
+    double vvjm = 0.25 * (Vc + Vjm) * (Vc + Vjm);
+    double wvkp = 0.25 * (Wc + W[jp]) * (Vkp + Vc);
+    double wvkm = 0.25 * (W[km] + W[jp - sxy]) * (Vkm + Vc);
+    double dvdxp = (Vip - Vc) * dxi, dvdxm = (Vc - Vim) * dxi;
+    double dvdyp = (Vjp - Vc) * dyi, dvdym = (Vc - Vjm) * dyi;
+    double dvdzp = (Vkp - Vc) * dzi, dvdzm = (Vc - Vkm) * dzi;
+    ay = dxi * (-uvip + uvim) + visc * (dvdxp - dvdxm) * dxi + dyi * (-vvjp + vvjm) + visc * (dvdyp - dvdym) * dyi +
+         dzi * (-wvkp + wvkm) + visc * (dvdzp - dvdzm) * dzi;
+  }
+  {  // momzad, src/mom.f90:179-196; momzbuoy src/mom.f90:252 added first (src/rk3.f90:56-58)
+    const double Wip = W[ip], Wim = W[im], Wjp = W[jp], Wjm = W[jm], Wkp = W[kp], Wkm = W[km];
+    double uwip = 0.25 * (Wc + Wip) * (Uc + U[kp]);
+    double uwim = 0.25 * (Wc + Wim) * (U[im] + U[im + sxy]);
+    double vwjp = 0.25 * (Wc + Wjp) * (Vc + V[kp]);
+    double vwjm = 0.25 * (Wc + Wjm) * (V[jm] + V[jm + sxy]);
+    double wwkp = 0.25 * (Wc + Wkp) * (Wc + Wkp);
+    double wwkm = 0.25 * (Wc + Wkm) * (Wc + Wkm);
+    double dwdxp = (Wip - Wc) * dxi, dwdxm = (Wc - Wim) * dxi;
+    double dwdyp = (Wjp - Wc) * dyi, dwdym = (Wc - Wjm) * dyi;
+    double dwdzp = (Wkp - Wc) * dzi, dwdzm = (Wc - Wkm) * dzi;
+    double adv = dxi * (-uwip + uwim) + visc * (dwdxp - dwdxm) * dxi + dyi * (-vwjp + vwjm) +
+                 visc * (dwdyp - dwdym) * dyi + dzi * (-wwkp + wwkm) + visc * (dwdzp - dwdzm) * dzi;
+    double buoy = 0.5 * g.beta * 9.81 * (C[c] + C[kp]);
+    az = buoy + adv;
+  }
+  {  // scalad, src/scal.f90:35-53
+    const double Cc = C[c], Cip = C[ip], Cim = C[im], Cjp = C[jp], Cjm = C[jm], Ckp = C[kp], Ckm = C[km];
+    double ucip = 0.5 * Uc * (Cip + Cc);
+    double ucim = 0.5 * U[im] * (Cc + Cim);
+    double vcjp = 0.5 * Vc * (Cjp + Cc);
+    double vcjm = 0.5 * V[jm] * (Cc + Cjm);
+    double wckp = 0.5 * Wc * (Ckp + Cc);
+    double wckm = 0.5 * W[km] * (Cc + Ckm);
+    double dcdxp = (Cip - Cc) * dxi, dcdxm = (Cc - Cim) * dxi;
+    double dcdyp = (Cjp - Cc) * dyi, dcdym = (Cc - Cjm) * dyi;
+    double dcdzp = (Ckp - Cc) * dzi, dcdzm = (Cc - Ckm) * dzi;
+    ac = dxi * (-ucip + ucim) + kappa * (dcdxp - dcdxm) * dxi + dyi * (-vcjp + vcjm) + kappa * (dcdyp - dcdym) * dyi +
+         dzi * (-wckp + wckm) + kappa * (dcdzp - dcdzm) * dzi;
+  }
+  if (STAGE == 1) {
+    dudt[c] = du0 + f1 * ax;
+    dvdt[c] = dv0 + f1 * ay;
+    dwdt[c] = dw0 + f1 * az;
+    dcdt[c] = C[c] + f1 * ac;
+    durk[c] = ax; dvrk[c] = ay; dwrk[c] = az; dcrk[c] = ac;
+  } else {
+    const double ru = durk[c], rv = dvrk[c], rw = dwrk[c], rc = dcrk[c];
+    dudt[c] = du0 + (f2 * ax - f3 * ru);
+    dvdt[c] = dv0 + (f2 * ay - f3 * rv);
+    dwdt[c] = dw0 + (f2 * az - f3 * rw);
+    dcdt[c] = C[c] + (f2 * ac - f3 * rc);
+    if (STAGE == 2) { durk[c] = ax; dvrk[c] = ay; dwrk[c] = az; dcrk[c] = ac; }
+  }
+}
+
+// Ghost fill for up to five fields in one launch: src/bound.f90:11-80
+// (bounduvw), 82-107 (boundp), 109-171 (boundscal) followed by updthalos
+// (173-221) on one rank.  Each ghost takes its final value directly: the z
+// rule applied to the periodic image (i',j') of its column, which is what the
+// reference's sequence z-rule -> x-halo -> y-halo produces.
+enum { KIND_UV = 0, KIND_W = 1, KIND_P = 2, KIND_C = 3 };
+struct BoundJob { double *f[5]; int kind[5]; int nf; };
+
+__global__ void k_bound(Grid g, BoundJob job) {
+  const long nx = g.nx, ny = g.ny, nz = g.nz;
+  const long nplane = (nx + 2) * (ny + 2), nrow = (nx + 2) * nz, ncol = ny * nz, nwall = nx * ny;
+  const long total = 2 * nplane + 2 * nrow + 2 * ncol + nwall;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    int i, j, k; bool wallplane = false;
+    long r = t;
+    if (r < 2 * nplane) { k = r < nplane ? 0 : (int)nz + 1; r %= nplane; i = (int)(r % (nx + 2)); j = (int)(r / (nx + 2)); }
+    else if ((r -= 2 * nplane) < 2 * nrow) {
+      j = r < nrow ? 0 : (int)ny + 1; r %= nrow; i = (int)(r % (nx + 2)); k = (int)(r / (nx + 2)) + 1;
+    } else if ((r -= 2 * nrow) < 2 * ncol) {
+      i = r < ncol ? 0 : (int)nx + 1; r %= ncol; j = (int)(r % ny) + 1; k = (int)(r / ny) + 1;
+    } else {
+      r -= 2 * ncol; i = (int)(r % nx) + 1; j = (int)(r / nx) + 1; k = (int)nz; wallplane = true;
+    }
+    const int ii = i == 0 ? (int)nx : (i == nx + 1 ? 1 : i);
+    const int jj = j == 0 ? (int)ny : (j == ny + 1 ? 1 : j);
+    const long dst = g.at(i, j, k);
+    for (int q = 0; q < job.nf; q++) {
+      double *f = job.f[q];
+      const int kind = job.kind[q];
+      if (wallplane) { if (kind == KIND_W) f[dst] = 0.0; continue; }
+      double val;
+      if (kind == KIND_W) {
+        if (k == 0 || k == nz) val = 0.0;
+        else val = f[g.at(ii, jj, k)];
+      } else if (k == nz + 1) {
+        const double in = f[g.at(ii, jj, (int)nz)];
+        if (kind == KIND_UV) val = g.bc_uvw_top == 0 ? -in : in;
+        else if (kind == KIND_P) val = in;
+        else val = g.bc_c_top == 0 ? 2.0 * g.bc_c_top_val - in : in;
+      } else if (k == 0) {
+        const double in = f[g.at(ii, jj, 1)];
+        if (kind == KIND_UV) val = g.bc_uvw_bot == 0 ? -in : in;
+        else if (kind == KIND_P) val = in;
+        else val = g.bc_c_bot == 0 ? 2.0 * g.bc_c_bot_val - in : in;
+      } else {
+        val = f[g.at(ii, jj, k)];
+      }
+      f[dst] = val;
+    }
+  }
+}
+
+// src/fillps.f90:8-53
+__global__ __launch_bounds__(BX *BY) void k_fillps(Grid g, const double *__restrict__ dudt, const double *__restrict__ dvdt,
+                                                   const double *__restrict__ dwdt, double *__restrict__ p,
+                                                   const double *__restrict__ dtp, double rkab) {
+  const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const double dti = 1.0 / (rkab * *dtp);
+  const double dtidxi = dti * g.dxi, dtidyi = dti * g.dyi, dtidzi = dti * g.dzi;
+  const long c = g.at(i, j, k);
+  p[c] = (dwdt[c] - dwdt[c - g.sxy]) * dtidzi + (dvdt[c] - dvdt[c - g.sx]) * dtidyi + (dudt[c] - dudt[c - 1]) * dtidxi;
+}
+
+// src/correc.f90:11-42, fused with src/main.f90:370 (pnew += p) and 375
+// (cnew = dcdt) on the interior; the ghosts are filled by k_bound afterwards.
+__global__ __launch_bounds__(BX *BY) void k_correc(Grid g, const double *__restrict__ dudt, const double *__restrict__ dvdt,
+                                                   const double *__restrict__ dwdt, const double *__restrict__ p,
+                                                   double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+                                                   double *__restrict__ pnew, const double *__restrict__ dcdt,
+                                                   double *__restrict__ cnew, const double *__restrict__ dtp, double rkab,
+                                                   int fuse_pnew) {
+  const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const double factor = rkab * *dtp;
+  const double fi = factor * g.dxi, fj = factor * g.dyi, fk = factor * g.dzi;
+  const long c = g.at(i, j, k);
+  const double pc = p[c];
+  u[c] = dudt[c] - fi * (p[c + 1] - pc);
+  v[c] = dvdt[c] - fj * (p[c + g.sx] - pc);
+  w[c] = dwdt[c] - fk * (p[c + g.sxy] - pc);
+  if (fuse_pnew) {
+    pnew[c] = pnew[c] + pc;
+    cnew[c] = dcdt[c];
+  }
+}
+
+__global__ __launch_bounds__(BX *BY) void k_pnew(Grid g, const double *__restrict__ p, double *__restrict__ pnew,
+                                                 const double *__restrict__ dcdt, double *__restrict__ cnew) {
+  const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long c = g.at(i, j, k);
+  pnew[c] = pnew[c] + p[c];
+  cnew[c] = dcdt[c];
+}
+
+// ---- reductions -------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// src/chkdt.f90:59-86 (u), 121-148 (v), 182-209 (w): per cell the three
+// components' min(dt1, dt2); the minimum over the domain is order-free.
+__device__ __forceinline__ double chkdt_dt2(double u2, double v2, double w2, double dxi, double dyi, double dzi) {
+  double velo = sqrt(u2 + v2 + w2);
+  if (fabs(1.0 * velo) > 1.e-12) return sqrt(3.0) / (sqrt(u2) * dxi + sqrt(v2) * dyi + sqrt(w2) * dzi);
+  return 99999.0;
+}
+__global__ __launch_bounds__(256) void k_chkdt(Grid g, const double *__restrict__ U, const double *__restrict__ V,
+                                                const double *__restrict__ W, double *__restrict__ part) {
+  __shared__ double sm[4];
+  const long n = (long)g.nx * g.ny * g.nz;
+  double m = 899999.0;
+  const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
+  const double dxi2 = dxi * dxi + dyi * dyi + dzi * dzi;
+  const double dt1 = 1.65 / (4.0 * g.visc * dxi2);
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i = (int)(t % g.nx) + 1, j = (int)((t / g.nx) % g.ny) + 1, k = (int)(t / ((long)g.nx * g.ny)) + 1;
+    long c = g.at(i, j, k), sx = g.sx, sxy = g.sxy;
+    double u2 = sqd(U[c]);
+    double v2 = sqd(0.25 * (V[c] + V[c - sx] + V[c + 1] + V[c + 1 - sx]));
+    double w2 = sqd(0.25 * (W[c] + W[c - sxy] + W[c + 1] + W[c + 1 - sxy]));
+    m = fmin(m, fmin(dt1, chkdt_dt2(u2, v2, w2, dxi, dyi, dzi)));
+    u2 = sqd(0.25 * (U[c] + U[c + sx] + U[c - 1 + sx] + U[c - 1]));
+    v2 = sqd(V[c]);
+    w2 = sqd(0.25 * (W[c] + W[c + sx] + W[c + sx - sxy] + W[c - sxy]));
+    m = fmin(m, fmin(dt1, chkdt_dt2(u2, v2, w2, dxi, dyi, dzi)));
+    u2 = sqd(0.25 * (U[c] + U[c - 1] + U[c - 1 + sxy] + U[c + sxy]));
+    v2 = sqd(0.25 * (V[c] + V[c - sx] + V[c - sx + sxy] + V[c + sxy]));
+    w2 = sqd(W[c]);
+    m = fmin(m, fmin(dt1, chkdt_dt2(u2, v2, w2, dxi, dyi, dzi)));
+  }
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = sm[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) r = fmin(r, sm[q]);
+    part[blockIdx.x] = r;
+  }
+}
+
+// src/chkdiv.f90:23-39
+__global__ __launch_bounds__(256) void k_chkdiv(Grid g, const double *__restrict__ U, const double *__restrict__ V,
+                                                 const double *__restrict__ W, double *__restrict__ part) {
+  __shared__ double s1[4], s2[4];
+  const long n = (long)g.nx * g.ny * g.nz;
+  double tot = 0.0, mx = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    int i = (int)(t % g.nx) + 1, j = (int)((t / g.nx) % g.ny) + 1, k = (int)(t / ((long)g.nx * g.ny)) + 1;
+    long c = g.at(i, j, k);
+    double div = (W[c] - W[c - g.sxy]) * g.dzi + (V[c] - V[c - g.sx]) * g.dyi + (U[c] - U[c - 1]) * g.dxi;
+    tot += div;
+    mx = fmax(mx, fabs(div));
+  }
+  tot = wave_sum(tot); mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) { s1[threadIdx.x >> 6] = tot; s2[threadIdx.x >> 6] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) { a += s1[q]; b = fmax(b, s2[q]); }
+    part[2 * blockIdx.x] = a; part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+}  // namespace
+
+static dim3 cell_grid(const Grid &g) { return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, g.nz); }
+
+int ipg_launch_rk(ipg_ctx *c, int rkiter) {
+  const Grid &g = c->g;
+  double **f = c->f;
+  dim3 blk(BX, BY);
+  if (rkiter == 1)
+    hipLaunchKernelGGL(k_rk<1>, cell_grid(g), blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],
+                       f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK], f[IPG_DCRK], c->dscal);
+  else if (rkiter == 2)
+    hipLaunchKernelGGL(k_rk<2>, cell_grid(g), blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],
+                       f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK], f[IPG_DCRK], c->dscal);
+  else
+    hipLaunchKernelGGL(k_rk<3>, cell_grid(g), blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],
+                       f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK], f[IPG_DCRK], c->dscal);
+  IPG_HIP(hipGetLastError());
+  ipg_mark(c, "rk_stage");
+  return 0;
+}
+
+int ipg_launch_bound(ipg_ctx *c, int mask) {
+  BoundJob job{};
+  auto add = [&](double *f, int kind) { job.f[job.nf] = f; job.kind[job.nf] = kind; job.nf++; };
+  if (mask & BND_UVW_NEW) { add(c->f[IPG_U], KIND_UV); add(c->f[IPG_V], KIND_UV); add(c->f[IPG_W], KIND_W); }
+  if (mask & BND_PNEW) add(c->f[IPG_PNEW], KIND_P);
+  if (mask & BND_C) add(c->f[IPG_C], KIND_C);
+  if (mask & BND_UVW_DT) {
+    if (job.nf > 2) { ipg_set_error("bound: too many fields"); return 1; }
+    add(c->f[IPG_DUDT], KIND_UV); add(c->f[IPG_DVDT], KIND_UV); add(c->f[IPG_DWDT], KIND_W);
+  }
+  if (mask & BND_P) {
+    if (job.nf > 4) { ipg_set_error("bound: too many fields"); return 1; }
+    add(c->f[IPG_P], KIND_P);
+  }
+  if (job.nf == 0) return 0;
+  const Grid &g = c->g;
+  long total = 2L * (g.nx + 2) * (g.ny + 2) + 2L * (g.nx + 2) * g.nz + 2L * g.ny * g.nz + (long)g.nx * g.ny;
+  int blocks = (int)((total + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_bound, dim3(blocks), dim3(256), 0, c->stream, g, job);
+  IPG_HIP(hipGetLastError());
+  ipg_mark(c, "bound");
+  return 0;
+}
+
+int ipg_launch_fillps(ipg_ctx *c, int rkiter) {
+  const Grid &g = c->g;
+  hipLaunchKernelGGL(k_fillps, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT],
+                     c->f[IPG_P], c->dscal, c->pc.rkab[rkiter - 1]);
+  IPG_HIP(hipGetLastError());
+  ipg_mark(c, "fillps");
+  return 0;
+}
+
+int ipg_launch_correc(ipg_ctx *c, int rkiter) {
+  const Grid &g = c->g;
+  hipLaunchKernelGGL(k_correc, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT],
+                     c->f[IPG_P], c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->f[IPG_DCDT], c->f[IPG_C], c->dscal,
+                     c->pc.rkab[rkiter - 1], 0);
+  IPG_HIP(hipGetLastError());
+  ipg_mark(c, "correc");
+  return 0;
+}
+
+int ipg_launch_correc_fused(ipg_ctx *c, int rkiter) {
+  const Grid &g = c->g;
+  hipLaunchKernelGGL(k_correc, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT],
+                     c->f[IPG_P], c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->f[IPG_DCDT], c->f[IPG_C], c->dscal,
+                     c->pc.rkab[rkiter - 1], 1);
+  IPG_HIP(hipGetLastError());
+  ipg_mark(c, "correc");
+  return 0;
+}
+
+int ipg_launch_pnew(ipg_ctx *c) {
+  const Grid &g = c->g;
+  hipLaunchKernelGGL(k_pnew, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_P], c->f[IPG_PNEW], c->f[IPG_DCDT], c->f[IPG_C]);
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+int ipg_launch_chkdt(ipg_ctx *c, double *dtmax) {
+  const int nb = 1024;
+  hipLaunchKernelGGL(k_chkdt, dim3(nb), dim3(256), 0, c->stream, c->g, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->red);
+  IPG_HIP(hipGetLastError());
+  double h[nb];
+  IPG_HIP(hipMemcpyAsync(h, c->red, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  double m = h[0];
+  for (int q = 1; q < nb; q++) m = h[q] < m ? h[q] : m;
+  *dtmax = m;
+  return 0;
+}
+
+int ipg_launch_chkdiv(ipg_ctx *c, double *tot, double *mx) {
+  const int nb = 1024;
+  hipLaunchKernelGGL(k_chkdiv, dim3(nb), dim3(256), 0, c->stream, c->g, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->red);
+  IPG_HIP(hipGetLastError());
+  double h[2 * nb];
+  IPG_HIP(hipMemcpyAsync(h, c->red, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  double a = 0.0, b = 0.0;
+  for (int q = 0; q < nb; q++) { a += h[2 * q]; b = h[2 * q + 1] > b ? h[2 * q + 1] : b; }
+  *tot = a; *mx = b;
+  return 0;
+}

@@ -1,0 +1,144 @@
+// ipg_internal.h -- context layout, grid indexing and device helpers shared by
+// the HIP translation units of libipg.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include "../../include/ipg.h"
+
+#define IPG_NB 9
+
+// Grid description passed by value to every field kernel.
+// Field layout in HBM: element (i,j,k), i in 0..nx+1, j in 0..ny+1,
+// k in 0..nz+1, lives at base[(i-1) + sx*j + sxy*k].  sx is a multiple of 16
+// doubles, so each row's first interior element i=1 is 128-byte aligned; the
+// ghost i=0 is the last slot of the previous row's padding (base[-1] exists).
+struct Grid {
+  int nx, ny, nz;
+  long sx, sxy;
+  double lx, ly, lz, dxi, dyi, dzi, dx, dy, dz;
+  double visc, kappa, beta;
+  int bc_uvw_top, bc_uvw_bot, bc_c_top, bc_c_bot;
+  double bc_c_top_val, bc_c_bot_val;
+  __host__ __device__ inline long at(int i, int j, int k) const { return (long)(i - 1) + sx * j + sxy * k; }
+};
+
+// Constants of the particle model (src/init.f90:74-243, src/param.f90:95-120).
+struct PartConst {
+  int np, nl, rotation;
+  double radius, ratiorho, volp, mominert, offset, radfp, dVlagr, dVeul;
+  double gaccx, gaccy, gaccz;
+  double kn_ss, etan_ss, kn_sw, etan_sw, coeff_f;
+  double a11a_ini_pp, a11b_ini_pp, a11a_sat_pp, a11b_sat_pp, a11_ini_pw, a11_sat_pw;
+  double rkab[3];
+};
+
+// Particle state, structure of arrays (np each), quantity order = IPG_P*.
+struct PartState {
+  double *q[IPG_NPART];
+  int *nbf;        // np*9 neighbour flags (slot 0 unused)
+};
+
+// Poisson plan (src/initsolver.f90, src/solver.f90).
+struct Poisson {
+  int nmx;          // nx/2+1 complex x-modes
+  long sp;          // spectrum row pitch (complex elements)
+  double2 *spec;    // [nz][ny][sp] complex spectrum
+  double *dtri;     // [nz][ny][sp] Thomas multipliers d
+  double *a, *b, *c;// [nz+1] tridiagonal coefficients (1-based)
+  double *xrt, *yrt;// eigenvalues by x-mode m (nmx) and y-mode my (ny)
+  double2 *twx, *twy; // e^{-2 pi i t/n} tables
+  int radx[16], nradx, rady[16], nrady;
+};
+
+struct ipg_ctx {
+  ipg_params prm;
+  Grid g;
+  PartConst pc;
+  hipStream_t stream;
+  size_t nfield;                  // doubles per field allocation
+  double *slab;                   // one allocation for all fields
+  double *f[IPG_NFIELDS];         // field base pointers (see Grid)
+  double *tmpf[4];                // rk2/rk3 temporaries are not needed: fused
+  // IBM
+  double *G[2][3];                // spread force buffers, interior (nx*ny*nz), modular x,y
+  double *K1[2];                  // dwdtf(.,.,k1) plane (nx*ny), src/interp_spread.f90:100 (no upper clamp)
+  int gcur, kcur;
+  double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
+  double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl
+  // particles
+  PartState ps;
+  double *op;                     // [24][np] state at substep start (src/intgr_nwtn_eulr.f90:77-118)
+  double *red;                    // reduction scratch
+  double *dscal;                  // device scalars: [0] dt
+  double dt_host;
+  Poisson ps2;
+  bool solver_ready, markers_ready, use_graphs;
+  hipGraphExec_t graph[3];
+  bool graph_ok[3];
+  // kernel timing (ipg_substep_phases): an event is recorded after every
+  // launch while timing is on
+  bool timing;
+  int nev;
+  hipEvent_t ev[64];
+  const char *evname[64];
+};
+
+// record a timing event named for the kernel just launched
+inline void ipg_mark(ipg_ctx *c, const char *name) {
+  if (!c->timing || c->nev >= 63) return;
+  (void)hipEventRecord(c->ev[c->nev + 1], c->stream);
+  c->evname[c->nev] = name;
+  c->nev++;
+}
+
+// error plumbing
+void ipg_set_error(const std::string &msg);
+#define IPG_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      ipg_set_error(std::string(#call) + ": " + hipGetErrorString(e_));                 \
+      return 1;                                                                         \
+    }                                                                                   \
+  } while (0)
+
+// ---- device helpers --------------------------------------------------------
+// Fortran nint: round half away from zero.
+__device__ __forceinline__ int nint_f(double x) { return (int)llround(x); }
+__device__ __forceinline__ double sqd(double x) { return x * x; }
+
+// src/kernel.f90:7-25, Roma et al. 3-point regularized delta times h.
+__device__ __forceinline__ double roma_kernel(double r) {
+  double ar = fabs(r);
+  if (ar > 1.5) return 0.0;
+  if (ar > 0.5) return (1.0 / 6.0) * ((5.0 - 3.0 * ar) - sqrt(-(3.0 * sqd(1.0 - ar)) + 1.0));
+  return (1.0 / 3.0) * (1.0 + sqrt(-(3.0 * sqd(r)) + 1.0));
+}
+
+// periodic wrap of an index in [-1, n+2] onto 1..n
+__device__ __forceinline__ int wrapi(int i, int n) {
+  int r = i - 1;
+  r = r < 0 ? r + n : (r >= n ? r - n : r);
+  return r + 1;
+}
+
+// launch helpers implemented per translation unit
+int ipg_launch_rk(ipg_ctx *c, int rkiter);
+int ipg_launch_bound(ipg_ctx *c, int mask);
+int ipg_launch_fillps(ipg_ctx *c, int rkiter);
+int ipg_launch_correc(ipg_ctx *c, int rkiter);
+int ipg_launch_pnew(ipg_ctx *c);
+int ipg_launch_chkdt(ipg_ctx *c, double *dtmax);
+int ipg_launch_chkdiv(ipg_ctx *c, double *tot, double *mx);
+int ipg_poisson_setup(ipg_ctx *c);
+int ipg_poisson_free(ipg_ctx *c);
+int ipg_launch_poisson(ipg_ctx *c);
+int ipg_launch_ibm(ipg_ctx *c);
+int ipg_launch_sphere_integrals(ipg_ctx *c, int type);
+int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter);
+int ipg_launch_nbflags(ipg_ctx *c);
+
+// bound masks for ipg_launch_bound
+enum { BND_UVW_NEW = 1, BND_UVW_DT = 2, BND_PNEW = 4, BND_P = 8, BND_C = 16 };

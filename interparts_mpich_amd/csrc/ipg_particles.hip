@@ -1,0 +1,385 @@
+// ipg_particles.hip -- FP64 particle update of one substep:
+//  * rigid-body integrals over the spheres, src/intgr_over_sphere.f90:12-625
+//    (one block per sphere, periodic images as in the reference's neighbour
+//    slots);
+//  * Newton-Euler with Breugem's inner-fluid correction and the soft-sphere
+//    collision / lubrication forces, src/intgr_nwtn_eulr.f90:12-714 and
+//    src/collisions.f90 (one thread per sphere);
+//  * periodic wrap and neighbour flags, src/intgr_nwtn_eulr.f90:702-714,
+//    1076-1111.
+// All arithmetic is FP64 in the reference's operation order.
+#include "ipg_internal.h"
+
+namespace {
+
+constexpr int NOP = 24;  // saved quantities (src/intgr_nwtn_eulr.f90:80-103)
+// op layout: x y z theta phi u v w omx omy omz omtheta intu intv intw intomx intomy intomz colfx colfy colfz coltx colty coltz
+__constant__ int c_op_src[NOP] = {IPG_PX, IPG_PY, IPG_PZ, IPG_PTHETA, IPG_PPHI, IPG_PU, IPG_PV, IPG_PW,
+                                  IPG_POMX, IPG_POMY, IPG_POMZ, IPG_POMTHETA, IPG_PINTU, IPG_PINTV, IPG_PINTW,
+                                  IPG_PINTOMX, IPG_PINTOMY, IPG_PINTOMZ, IPG_PCOLFX, IPG_PCOLFY, IPG_PCOLFZ,
+                                  IPG_PCOLTX, IPG_PCOLTY, IPG_PCOLTZ};
+
+struct PQ { double *q[IPG_NPART]; };
+
+__global__ void k_ne_save(PQ P, double *op, int np) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  for (int s = 0; s < NOP; s++) op[(long)s * np + p] = P.q[c_op_src[s]][p];
+}
+
+// image centre of neighbour slot nbrecv (src/intgr_over_sphere.f90:83-109)
+__device__ __forceinline__ void image_shift(const Grid &g, int nbrecv, double &x, double &y) {
+  switch (nbrecv) {
+    case 1: x = x + g.lx; break;
+    case 2: x = x + g.lx; y = y - g.ly; break;
+    case 3: y = y - g.ly; break;
+    case 4: x = x - g.lx; y = y - g.ly; break;
+    case 5: x = x - g.lx; break;
+    case 6: x = x - g.lx; y = y + g.ly; break;
+    case 7: y = y + g.ly; break;
+    case 8: x = x + g.lx; y = y + g.ly; break;
+    default: break;
+  }
+}
+
+// types: 1 volume, 2 int u dV, 3 int r x u dV, 23 both 2 and 3 from one sweep
+__global__ __launch_bounds__(256) void k_sphere_int(Grid g, PartConst pc, const double *__restrict__ U,
+                                                    const double *__restrict__ V, const double *__restrict__ W, PQ P,
+                                                    const int *__restrict__ nbf, int type) {
+  const int p = blockIdx.x;
+  __shared__ double acc[IPG_NB][6];
+  __shared__ double red[6][4];
+  if (threadIdx.x < IPG_NB * 6) acc[threadIdx.x / 6][threadIdx.x % 6] = 0.0;
+  __syncthreads();
+  const double R = pc.radius, dVeul = pc.dVeul;
+  const double dx = 1.0 / g.dxi, dy = 1.0 / g.dyi, dz = 1.0 / g.dzi;
+  const double radin = R - 2.0 * dx, radin2 = radin * radin;
+  const bool t1 = type == 1, t2 = type == 2 || type == 23, t3 = type == 3 || type == 23;
+  for (int nb = 0; nb < IPG_NB; nb++) {
+    if (nb > 0 && nbf[p * IPG_NB + nb] != 1) continue;
+    int nbrecv = nb;
+    if (nb > 0) { nbrecv = nb + 4; if (nbrecv > 8) nbrecv -= 8; }
+    double X = P.q[IPG_PX][p], Y = P.q[IPG_PY][p];
+    const double Z = P.q[IPG_PZ][p];
+    image_shift(g, nbrecv, X, Y);
+    const double cxc = X - 0.0, cyc = Y - 0.0, czc = Z;
+    int ilow = nint_f((cxc - R) * g.dxi - 2.0), ihigh = nint_f((cxc + R) * g.dxi + 2.0);
+    int jlow = nint_f((cyc - R) * g.dyi - 2.0), jhigh = nint_f((cyc + R) * g.dyi + 2.0);
+    int klow = nint_f((czc - R) * g.dzi - 2.0), khigh = nint_f((czc + R) * g.dzi + 2.0);
+    ilow = max(ilow, 1); jlow = max(jlow, 1); klow = max(klow, 1);
+    ihigh = min(ihigh, g.nx); jhigh = min(jhigh, g.ny); khigh = min(khigh, g.nz);
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    const int ni = ihigh - ilow + 1, nj = jhigh - jlow + 1, nk = khigh - klow + 1;
+    if (ni > 0 && nj > 0 && nk > 0) {
+      const long tot = (long)ni * nj * nk;
+      for (int comp = 0; comp < 3; comp++) {
+        const double *F = comp == 0 ? U : (comp == 1 ? V : W);
+        for (long t = threadIdx.x; t < tot; t += blockDim.x) {
+          const int i = ilow + (int)(t % ni), j = jlow + (int)((t / ni) % nj), k = klow + (int)(t / ((long)ni * nj));
+          double zmin, zplus, ymin, yplus, xmin, xplus;
+          if (comp == 2) { zmin = ((double)k - 0.5) * dz; zplus = ((double)k + 0.5) * dz; }
+          else { zmin = (double)(k - 1) * dz; zplus = (double)k * dz; }
+          if (comp == 1) { ymin = 0.0 + ((double)j - 0.5) * dy; yplus = 0.0 + ((double)j + 0.5) * dy; }
+          else { ymin = 0.0 + (double)(j - 1) * dy; yplus = 0.0 + (double)j * dy; }
+          if (comp == 0) { xmin = 0.0 + ((double)i - 0.5) * dx; xplus = 0.0 + ((double)i + 0.5) * dx; }
+          else { xmin = 0.0 + (double)(i - 1) * dx; xplus = 0.0 + (double)i * dx; }
+          const double dzp2 = sqd(zplus - Z), dyp2 = sqd(yplus - Y), dxp2 = sqd(xplus - X);
+          const double dist2 = dxp2 + dyp2 + dzp2;
+          const double fval = F[g.at(i, j, k)];
+          bool full = dist2 < radin2;
+          double wfrac = 1.0;
+          if (!full) {
+            const double dxm2 = sqd(xmin - X), dym2 = sqd(ymin - Y), dzm2 = sqd(zmin - Z);
+            double sg[8];
+            sg[0] = R - sqrt(dxm2 + dym2 + dzm2); sg[1] = R - sqrt(dxm2 + dyp2 + dzm2);
+            sg[2] = R - sqrt(dxp2 + dyp2 + dzm2); sg[3] = R - sqrt(dxp2 + dym2 + dzm2);
+            sg[4] = R - sqrt(dxm2 + dym2 + dzp2); sg[5] = R - sqrt(dxm2 + dyp2 + dzp2);
+            sg[6] = R - sqrt(dxp2 + dyp2 + dzp2); sg[7] = R - sqrt(dxp2 + dym2 + dzp2);
+            double s1 = 0.0, s2 = 0.0;
+            for (int cp = 0; cp < 8; cp++) { if (sg[cp] > 0.0) s1 = s1 + sg[cp]; s2 = s2 + fabs(sg[cp]); }
+            wfrac = s1 / s2;
+          }
+          if (t1) s[comp] += full ? dVeul : wfrac * dVeul;
+          if (t2) s[comp] += full ? fval * dVeul : fval * wfrac * dVeul;
+          if (t3) {
+            const double rx = 0.0 + ((double)i - 0.5) * dx - X;
+            const double ry = 0.0 + ((double)j - 0.5) * dy - Y;
+            const double rz = ((double)k - 0.5) * dz - Z;
+            if (comp == 0) {
+              s[4] += full ? rz * fval * dVeul : rz * fval * wfrac * dVeul;
+              s[5] -= full ? ry * fval * dVeul : ry * fval * wfrac * dVeul;
+            } else if (comp == 1) {
+              s[3] -= full ? rz * fval * dVeul : rz * fval * wfrac * dVeul;
+              s[5] += full ? rx * fval * dVeul : rx * fval * wfrac * dVeul;
+            } else {
+              s[3] += full ? ry * fval * dVeul : ry * fval * wfrac * dVeul;
+              s[4] -= full ? rx * fval * dVeul : rx * fval * wfrac * dVeul;
+            }
+          }
+        }
+      }
+    }
+    for (int q = 0; q < 6; q++) {
+      double v = s[q];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      double v = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) v += red[threadIdx.x][w];
+      acc[nbrecv][threadIdx.x] = v;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) {
+    // src/intgr_over_sphere.f90:564-599: sum of the nine slots in order
+    double t = 0.0;
+    for (int nb = 0; nb < IPG_NB; nb++) t = t + acc[nb][threadIdx.x];
+    const int q = threadIdx.x;
+    if (q < 3) {
+      if (t1 || t2) P.q[IPG_PINTU + q][p] = t;
+    } else if (t3) {
+      P.q[IPG_PINTOMX + (q - 3)][p] = t;
+    }
+  }
+}
+
+// src/collisions.f90:222-438, normal squeeze force only (lines 430-432)
+__device__ __forceinline__ void lubrication(const PartConst &pc, double up, double vp, double wp, bool wall, double nx, double ny,
+                                            double nz, double eps, double un, double vn, double wn, double &fx, double &fy,
+                                            double &fz) {
+  const double u1a = up * nx + vp * ny + wp * nz;
+  const double u1b = un * nx + vn * ny + wn * nz;
+  double f1;
+  if (wall) {
+    double a11;
+    if (eps < 0.001) a11 = pc.a11_sat_pw;
+    else a11 = -(1.0 / eps) + 1.0 / 5.0 * log(eps) + 1.0 / 21.0 * eps * log(eps) - .9713;
+    f1 = (a11 * u1a);
+    f1 = pc.coeff_f * (f1 - (pc.a11_ini_pw * u1a));
+  } else {
+    double a11a, a11b;
+    if (eps < 0.001) { a11a = pc.a11a_sat_pp; a11b = pc.a11b_sat_pp; }
+    else {
+      const double a11 = -(1.0 / 4.0 * (1.0 / eps)) + 9.0 / 40.0 * log(eps) + 3.0 / 112.0 * eps * log(eps);
+      a11a = a11 - .995; a11b = -a11 + .350;
+    }
+    f1 = (a11a * u1a + a11b * u1b);
+    f1 = pc.coeff_f * (f1 - (pc.a11a_ini_pp * u1a + pc.a11b_ini_pp * u1b));
+  }
+  fx = fx + f1 * nx; fy = fy + f1 * ny; fz = fz + f1 * nz;
+}
+
+// src/collisions.f90:10-220, normal spring-dashpot force only (lines 199-201)
+__device__ __forceinline__ void collision(const PartConst &pc, double up, double vp, double wp, double omx, double omy, double omz,
+                                          bool wall, double dist, double deltax, double deltay, double deltaz, double un,
+                                          double vn, double wn, double omxn, double omyn, double omzn, double &fx,
+                                          double &fy, double &fz) {
+  const double nx = deltax / dist, ny = deltay / dist, nz = deltaz / dist;
+  const double R = pc.radius;
+  double deltan, kn, etan;
+  if (!wall) { deltan = 2.0 * R - dist; kn = pc.kn_ss; etan = pc.etan_ss; }
+  else { deltan = R - dist; kn = pc.kn_sw; etan = pc.etan_sw; }
+  const double dun = (up + (omy * R * nz) - (omz * R * ny)) - (un + (omyn * R * (-nz)) - (omzn * R * (-ny)));
+  const double dvn = (vp - (omx * R * nz) + (omz * R * nx)) - (vn - (omxn * R * (-nz)) + (omzn * R * (-nx)));
+  const double dwn = (wp + (omx * R * ny) - (omy * R * nx)) - (wn + (omxn * R * (-ny)) - (omyn * R * (-nx)));
+  const double vabs = dun * nx + dvn * ny + dwn * nz;
+  fx = fx + (-(kn * (deltan * nx)) - etan * (vabs * nx));
+  fy = fy + (-(kn * (deltan * ny)) - etan * (vabs * ny));
+  fz = fz + (-(kn * (deltan * nz)) - etan * (vabs * nz));
+}
+
+// copy nb of particle q as seen at one rank (src/intgr_nwtn_eulr.f90:223-322)
+__device__ __forceinline__ void nb_copy(const Grid &g, int nb, double &x, double &y) {
+  switch (nb) {
+    case 1: if (x < g.lx) x = x + g.lx; break;
+    case 2: if (x < g.lx) x = x + g.lx; if (y > 0.0) y = y - g.ly; break;
+    case 3: if (y > 0.0) y = y - g.ly; break;
+    case 4: if (x > 0.0) x = x - g.lx; if (y > 0.0) y = y - g.ly; break;
+    case 5: if (x > 0.0) x = x - g.lx; break;
+    case 6: if (x > 0.0) x = x - g.lx; if (y < g.ly) y = y + g.ly; break;
+    case 7: if (y < g.ly) y = y + g.ly; break;
+    case 8: if (x < g.lx) x = x + g.lx; if (y < g.ly) y = y + g.ly; break;
+    default: break;
+  }
+}
+
+constexpr int MAXCAND = 48;
+
+// Collision model, src/intgr_nwtn_eulr.f90:338-606: for each sphere the
+// neighbour tables nb = 0..8 (at one rank: the sphere set and its 8 periodic
+// copies) are scanned in order; every copy of a partner in range adds its
+// lubrication and contact force (the reference's 9x multiplicity at one rank).
+// Partners are first screened with a margin so that only candidates enter the
+// ordered scan.
+__global__ __launch_bounds__(256) void k_collide(Grid g, PartConst pc, PQ P, double *cfx, double *cfy, double *cfz,
+                                                 int *err) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = pc.np;
+  if (p >= np) return;
+  const double *X = P.q[IPG_PX], *Y = P.q[IPG_PY], *Zq = P.q[IPG_PZ], *Uq = P.q[IPG_PU], *Vq = P.q[IPG_PV], *Wq = P.q[IPG_PW];
+  const double xp = X[p], yp = Y[p], zp = Zq[p];
+  const double R = pc.radius, lx = g.lx, ly = g.ly;
+  const double reach = 2.0 * R * (1.0 + 0.025) + 1e-9 * (lx + ly + R);
+  int cand[MAXCAND];
+  int nc = 0;
+  for (int q = 0; q < np; q++) {
+    if (q == p) continue;
+    double dz = Zq[q] - zp;
+    if (fabs(dz) > reach) continue;
+    double dx = X[q] - xp, dy = Y[q] - yp;
+    dx = dx - lx * rint(dx / lx);
+    dy = dy - ly * rint(dy / ly);
+    if (dx * dx + dy * dy + dz * dz <= reach * reach) {
+      if (nc < MAXCAND) cand[nc++] = q; else atomicOr(err, 1);
+    }
+  }
+  const double up = Uq[p], vp = Vq[p], wp = Wq[p];
+  const double omx = P.q[IPG_POMX][p], omy = P.q[IPG_POMY][p], omz = P.q[IPG_POMZ][p];
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int nb = 0; nb < IPG_NB; nb++)
+    for (int c = 0; c < nc; c++) {
+      const int q = cand[c];
+      double xq = X[q], yq = Y[q];
+      nb_copy(g, nb, xq, yq);
+      const double deltaz = Zq[q] - zp;
+      const double ax0 = xq - xp, ax1 = xq - lx - xp, ax2 = xq + lx - xp;
+      double deltax = ax0;
+      if (fabs(ax1) < fabs(deltax)) deltax = ax1;
+      if (fabs(ax2) < fabs(deltax)) deltax = ax2;
+      const double ay0 = yq - yp, ay1 = yq - ly - yp, ay2 = yq + ly - yp;
+      double deltay = ay0;
+      if (fabs(ay1) < fabs(deltay)) deltay = ay1;
+      if (fabs(ay2) < fabs(deltay)) deltay = ay2;
+      const double dist = sqrt(sqd(deltax) + sqd(deltay) + sqd(deltaz));
+      const double nx = deltax / dist, ny = deltay / dist, nz = deltaz / dist;
+      const double eps = (dist - 2.0 * R) / R;
+      if ((eps < 0.025) && (eps > 0.0)) lubrication(pc, up, vp, wp, false, nx, ny, nz, eps, Uq[q], Vq[q], Wq[q], fx, fy, fz);
+      if (dist < (2.0 * R))
+        collision(pc, up, vp, wp, omx, omy, omz, false, dist, deltax, deltay, deltaz, Uq[q], Vq[q], Wq[q],
+                  P.q[IPG_POMX][q], P.q[IPG_POMY][q], P.q[IPG_POMZ][q], fx, fy, fz);
+    }
+  for (int wq = 0; wq < 2; wq++) {  // walls q = botw, topw (src/intgr_nwtn_eulr.f90:504-603)
+    const double deltax = 0.0, deltay = 0.0, deltaz = (double)wq * g.lz - zp;
+    const double dist = sqrt(sqd(deltax) + sqd(deltay) + sqd(deltaz));
+    const double deltan = R - dist;
+    const double nx = deltax / dist, ny = deltay / dist, nz = deltaz / dist;
+    const double eps = -deltan / R;
+    if ((eps < 0.075) && (eps > 0.0)) lubrication(pc, up, vp, wp, true, nx, ny, nz, eps, 0.0, 0.0, 0.0, fx, fy, fz);
+    if (dist < R) collision(pc, up, vp, wp, omx, omy, omz, true, dist, deltax, deltay, deltaz, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, fx, fy, fz);
+  }
+  cfx[p] = fx; cfy[p] = fy; cfz[p] = fz;
+}
+
+// src/intgr_nwtn_eulr.f90:611-667
+__global__ void k_ne_update(Grid g, PartConst pc, PQ P, const double *__restrict__ op, const double *cfx, const double *cfy,
+                            const double *cfz, const double *__restrict__ dtp, int rkiter) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = pc.np;
+  if (p >= np) return;
+  auto O = [&](int s) { return op[(long)s * np + p]; };
+  double **q = P.q;
+  const double dt = *dtp, rkab = pc.rkab[rkiter - 1];
+  q[IPG_PCOLFX][p] = cfx[p]; q[IPG_PCOLFY][p] = cfy[p]; q[IPG_PCOLFZ][p] = cfz[p];
+  q[IPG_PCOLTX][p] = 0.0; q[IPG_PCOLTY][p] = 0.0; q[IPG_PCOLTZ][p] = 0.0;
+  const double vr = pc.volp * pc.ratiorho, rr = pc.ratiorho;
+  const double u = O(5) + (1.0 - 0.0) * ((-1.0) * dt * (q[IPG_PFXLTOT][p]) / vr + (q[IPG_PINTU][p] - O(12)) / vr) +
+                   rkab * dt * pc.gaccx * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (cfx[p] + O(18)) / vr;
+  const double x = O(0) + rkab * 0.5 * dt * (u + O(5));
+  const double v = O(6) + (1.0 - 0.0) * ((-1.0) * dt * (q[IPG_PFYLTOT][p]) / vr + (q[IPG_PINTV][p] - O(13)) / vr) +
+                   rkab * dt * pc.gaccy * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (cfy[p] + O(19)) / vr;
+  const double y = O(1) + rkab * 0.5 * dt * (v + O(6));
+  const double w = O(7) + (1.0 - 0.0) * ((-1.0) * dt * (q[IPG_PFZLTOT][p]) / vr + (q[IPG_PINTW][p] - O(14)) / vr) +
+                   rkab * dt * pc.gaccz * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (cfz[p] + O(20)) / vr;
+  const double z = O(2) + rkab * 0.5 * dt * (w + O(7));
+  q[IPG_PU][p] = u; q[IPG_PX][p] = x; q[IPG_PV][p] = v; q[IPG_PY][p] = y; q[IPG_PW][p] = w; q[IPG_PZ][p] = z;
+  double ox = 0.0, oy = 0.0, oz = 0.0;
+  if (pc.rotation) {
+    const double mr = pc.mominert * pc.ratiorho;
+    ox = O(8) + (-1.0) * dt * q[IPG_PTORQX][p] / mr + (q[IPG_PINTOMX][p] - O(15)) / mr + rkab * 0.5 * dt * (0.0 + O(21)) / mr;
+    oy = O(9) + (-1.0) * dt * q[IPG_PTORQY][p] / mr + (q[IPG_PINTOMY][p] - O(16)) / mr + rkab * 0.5 * dt * (0.0 + O(22)) / mr;
+    oz = O(10) + (-1.0) * dt * q[IPG_PTORQZ][p] / mr + (q[IPG_PINTOMZ][p] - O(17)) / mr + rkab * 0.5 * dt * (0.0 + O(23)) / mr;
+  }
+  q[IPG_POMX][p] = ox; q[IPG_POMY][p] = oy; q[IPG_POMZ][p] = oz;
+  const double phi = O(4) + 0.5 * rkab * dt * (oz + O(10));
+  q[IPG_PPHI][p] = phi;
+  const double omt = (oy * cos(phi)) - (ox * sin(phi));
+  q[IPG_POMTHETA][p] = omt;
+  q[IPG_PTHETA][p] = O(3) + rkab * 0.5 * dt * (omt + O(11));
+}
+
+// src/intgr_nwtn_eulr.f90:702-714 periodic wrap, then neighbour flags of
+// src/intgr_nwtn_eulr.f90:1076-1111 (= src/initparticles.f90:200-241) at one rank.
+__global__ void k_wrap_flags(Grid g, PartConst pc, PQ P, int *nbf, int wrap) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pc.np) return;
+  double x = P.q[IPG_PX][p], y = P.q[IPG_PY][p];
+  if (wrap) {
+    if (x > g.lx) x = x - g.lx;
+    if (x < 0.0) x = x + g.lx;
+    if (y > g.ly) y = y - g.ly;
+    if (y < 0.0) y = y + g.ly;
+    P.q[IPG_PX][p] = x; P.q[IPG_PY][p] = y;
+  }
+  const double ro = pc.radius + pc.offset;
+  const double left = 0.0 * g.lx / 1.0, right = 1.0 * g.lx / 1.0, front = 0.0 * g.ly / 1.0, back = 1.0 * g.ly / 1.0;
+  int *f = nbf + p * IPG_NB;
+  f[0] = 0;
+  f[1] = x > (right - ro);
+  f[2] = fabs(sqrt(sqd(right - x) + sqd(front - y))) < ro;
+  f[3] = y < (front + ro);
+  f[4] = fabs(sqrt(sqd(left - x) + sqd(front - y))) < ro;
+  f[5] = x < (left + ro);
+  f[6] = fabs(sqrt(sqd(left - x) + sqd(back - y))) < ro;
+  f[7] = y > (back - ro);
+  f[8] = fabs(sqrt(sqd(right - x) + sqd(back - y))) < ro;
+}
+
+PQ make_pq(ipg_ctx *c) {
+  PQ P;
+  for (int q = 0; q < IPG_NPART; q++) P.q[q] = c->ps.q[q];
+  return P;
+}
+
+}  // namespace
+
+int ipg_launch_sphere_integrals(ipg_ctx *c, int type) {
+  if (c->pc.np <= 0) return 0;
+  hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(256), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W],
+                     make_pq(c), c->ps.nbf, type);
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+int ipg_launch_nbflags(ipg_ctx *c) {
+  if (c->pc.np <= 0) return 0;
+  hipLaunchKernelGGL(k_wrap_flags, dim3((c->pc.np + 255) / 256), dim3(256), 0, c->stream, c->g, c->pc, make_pq(c), c->ps.nbf, 0);
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter) {
+  const int np = c->pc.np;
+  if (np <= 0) return 0;
+  PQ P = make_pq(c);
+  const int nb = (np + 255) / 256;
+  double *cf = c->red;  // 3*np scratch after the error flag slot
+  int *err = (int *)(c->red + 3L * np);
+  hipLaunchKernelGGL(k_ne_save, dim3(nb), dim3(256), 0, c->stream, P, c->op, np);
+  ipg_mark(c, "ne_save");
+  hipLaunchKernelGGL(k_sphere_int, dim3(np), dim3(256), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], P,
+                     c->ps.nbf, 23);
+  ipg_mark(c, "sphere_int");
+  for (int iter = 0; iter <= 1; iter++) {
+    hipLaunchKernelGGL(k_collide, dim3(nb), dim3(256), 0, c->stream, c->g, c->pc, P, cf, cf + np, cf + 2 * np, err);
+    ipg_mark(c, "collide");
+    hipLaunchKernelGGL(k_ne_update, dim3(nb), dim3(256), 0, c->stream, c->g, c->pc, P, c->op, cf, cf + np, cf + 2 * np,
+                       c->dscal, rkiter);
+    ipg_mark(c, "ne_update");
+  }
+  hipLaunchKernelGGL(k_wrap_flags, dim3(nb), dim3(256), 0, c->stream, c->g, c->pc, P, c->ps.nbf, 1);
+  ipg_mark(c, "wrap_flags");
+  IPG_HIP(hipGetLastError());
+  return 0;
+}

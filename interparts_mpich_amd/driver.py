@@ -1,0 +1,191 @@
+"""Host driver of the GPU substep, mirroring program interpt
+(src/main.f90:30-541) for one rank holding the whole domain.
+
+The method names follow the reference's module procedures (rk1/rk2/rk3,
+ibm = the eulr2lagr/complagrforces/updtlagrforces/lagr2eulr block, bounduvw,
+boundp, boundscal, fillps, solver2d, correc, intgr_over_sphere,
+intgr_nwtn_eulr, chkdt, chkdiv); each forwards to the C-ABI of libipg.so.
+Fields stay resident on the GPU between calls; host copies are made only by
+get()/put().
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import F, PQ, check, load
+from .inputs import Case, initial_fields, place_spheres, markers_for
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Simulation:
+    def __init__(self, case: Case, use_graphs=False):
+        self.case = case
+        self.lib = load()
+        prm = _lib.IpgParams(itot=case.itot, jtot=case.jtot, ktot=case.ktot, np=case.np,
+                             nl=case.nl if case.np > 0 else 0, lx=case.lx, ly=case.ly, lz=case.lz,
+                             visc=case.visc, pran=case.pran, beta=case.beta, radius=case.radius,
+                             ratiorho=case.ratiorho, particle_rotation=int(case.particle_rotation),
+                             bc_uvw_top=case.bc_uvw_top, bc_uvw_bot=case.bc_uvw_bot, bc_c_top=case.bc_c_top,
+                             bc_c_bot=case.bc_c_bot, bc_c_top_val=case.bc_c_top_val,
+                             bc_c_bot_val=case.bc_c_bot_val, precision=0)
+        self.ctx = C.c_void_p()
+        check(self.lib.ipg_init(C.byref(prm), C.byref(self.ctx)), "ipg_init")
+        if use_graphs:
+            check(self.lib.ipg_use_graphs(self.ctx, 1), "ipg_use_graphs")
+        self.shape = (case.ktot + 2, case.jtot + 2, case.itot + 2)
+        self.time = 0.0
+        self.dt = 0.0
+        self.istep = 0
+
+    def close(self):
+        if self.ctx:
+            self.lib.ipg_finalize(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- transfers ---------------------------------------------------------
+    def put(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.shape
+        check(self.lib.ipg_field_put(self.ctx, F[name], _ptr(a)), f"put {name}")
+
+    def get(self, name):
+        a = np.empty(self.shape)
+        check(self.lib.ipg_field_get(self.ctx, F[name], _ptr(a)), f"get {name}")
+        return a
+
+    def pput(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == (self.case.np,)
+        check(self.lib.ipg_particles_put(self.ctx, PQ[name], _ptr(a)), f"pput {name}")
+
+    def pget(self, name):
+        a = np.empty(self.case.np)
+        check(self.lib.ipg_particles_get(self.ctx, PQ[name], _ptr(a)), f"pget {name}")
+        return a
+
+    def set_dt(self, dt):
+        self.dt = float(dt)
+        check(self.lib.ipg_set_dt(self.ctx, C.c_double(self.dt)), "set_dt")
+
+    # ---- reference procedures ----------------------------------------------
+    def rk(self, rkiter):
+        check(self.lib.ipg_rk(self.ctx, rkiter), "rk")
+
+    def rk1(self):
+        self.rk(1)
+
+    def rk2(self):
+        self.rk(2)
+
+    def rk3(self):
+        self.rk(3)
+
+    def ibm(self):
+        check(self.lib.ipg_ibm(self.ctx), "ibm")
+
+    def bounduvw(self, which=0):
+        check(self.lib.ipg_bounduvw(self.ctx, which), "bounduvw")
+
+    def boundp(self, which=0):
+        check(self.lib.ipg_boundp(self.ctx, which), "boundp")
+
+    def boundscal(self):
+        check(self.lib.ipg_boundscal(self.ctx), "boundscal")
+
+    def fillps(self, rkiter):
+        check(self.lib.ipg_fillps(self.ctx, rkiter), "fillps")
+
+    def initsolver(self):
+        check(self.lib.ipg_initsolver(self.ctx), "initsolver")
+
+    def solver2d(self):
+        check(self.lib.ipg_solver2d(self.ctx), "solver2d")
+
+    def correc(self, rkiter):
+        check(self.lib.ipg_correc(self.ctx, rkiter), "correc")
+
+    def pnew_update(self):
+        check(self.lib.ipg_pnew_update(self.ctx), "pnew_update")
+
+    def intgr_over_sphere(self, typ):
+        check(self.lib.ipg_intgr_over_sphere(self.ctx, typ), "intgr_over_sphere")
+
+    def intgr_nwtn_eulr(self, rkiter):
+        check(self.lib.ipg_intgr_nwtn_eulr(self.ctx, rkiter), "intgr_nwtn_eulr")
+
+    def chkdt(self):
+        d = C.c_double()
+        check(self.lib.ipg_chkdt(self.ctx, C.byref(d)), "chkdt")
+        return d.value
+
+    def chkdiv(self):
+        a, b = C.c_double(), C.c_double()
+        check(self.lib.ipg_chkdiv(self.ctx, C.byref(a), C.byref(b)), "chkdiv")
+        return a.value, b.value
+
+    def substep(self, rkiter):
+        check(self.lib.ipg_substep(self.ctx, rkiter), "substep")
+
+    def sync(self):
+        check(self.lib.ipg_sync(self.ctx), "sync")
+
+    def substep_bytes(self):
+        b = C.c_double()
+        check(self.lib.ipg_substep_bytes(self.ctx, C.byref(b)), "substep_bytes")
+        return b.value
+
+    # ---- program interpt ---------------------------------------------------
+    def initialize(self, fields=None, spheres=None, markers=None):
+        """src/main.f90:98-227 with begin = 0: initial conditions, particles,
+        boundary conditions, dt = 0.5 dtmax, solver and marker set-up, the
+        initial sphere integrals."""
+        case = self.case
+        u, v, w, p, c = fields if fields is not None else initial_fields(case)
+        for name, a in (("U", u), ("V", v), ("W", w), ("PNEW", p), ("C", c)):
+            self.put(name, a)
+        if case.np > 0:
+            x, y, z = spheres if spheres is not None else place_spheres(case)
+            x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+            check(self.lib.ipg_initparticles(self.ctx, _ptr(x), _ptr(y), _ptr(z)), "initparticles")
+        self.time = 0.0
+        self.bounduvw(0)
+        self.boundscal()
+        self.set_dt(0.5 * self.chkdt())
+        self.initsolver()
+        self.bounduvw(0)
+        self.boundscal()
+        self.boundp(0)
+        self.divergence = self.chkdiv()
+        if case.np > 0:
+            theta, phi, radfp = markers if markers is not None else markers_for(case)
+            theta = np.ascontiguousarray(theta, dtype=np.float64)
+            phi = np.ascontiguousarray(phi, dtype=np.float64)
+            check(self.lib.ipg_coordsfp(self.ctx, _ptr(theta), _ptr(phi), C.c_double(radfp)), "coordsfp")
+            self.intgr_over_sphere(1)
+            self.volume = self.pget("INTU").copy()
+            self.intgr_over_sphere(2)
+            self.intgr_over_sphere(3)
+        return self
+
+    def step(self):
+        """One time step: src/main.f90:253-441 (three substeps, then every
+        ioutchk steps chkdiv, chkdt and dt = 0.75 dtmax)."""
+        self.istep += 1
+        self.time += self.dt
+        for rkiter in (1, 2, 3):
+            self.substep(rkiter)
+        if self.case.ioutchk > 0 and self.istep % self.case.ioutchk == 0:
+            self.divergence = self.chkdiv()
+            self.set_dt(0.75 * self.chkdt())
+        else:
+            self.sync()

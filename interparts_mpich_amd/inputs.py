@@ -1,0 +1,287 @@
+"""Seeded case inputs: parameters, initial fields, sphere placement and the
+marker file.
+
+The reference seeds its initial noise, sphere placement and marker layout
+with Fortran's random_number (src/init.f90:287-305,
+src/initparticles.f90:36-47, src/poslfp/lfp.f:55-93), which is compiler
+specific.  This module replaces it with a documented counter-based generator
+so that every consumer of a case (the GPU path, the CPU oracle, the Fortran
+driver) sees bit-identical inputs:
+
+    u(seed, stream, n) = splitmix64(seed*K1 + stream*K2 + n) >> 11, times 2^-53
+
+Initial velocity: the 'poi' profile of src/init.f90:436-450 (v = 6 z (1-z)
+with z the cell-centre height over lz, u = w = p = 0 on all cells) plus, on
+interior cells, 0.1*(2u-1) per component keyed by the global cell index.
+Spheres: rejection sampling with centre gap >= D + h (minimum image in x, y)
+and wall gap >= R + h.  Markers: a golden-angle spiral of nl points, written
+in the lagrangianforcepoints2 text format of src/poslfp/lfp.f:326-337 and read
+back through that format (src/coordsfp.f90:21-29), so all consumers use the
+rounded values the reference would read.
+"""
+import math
+import os
+from dataclasses import dataclass, field, asdict
+
+import numpy as np
+
+_K1 = np.uint64(0x9E3779B97F4A7C15)
+_K2 = np.uint64(0xD1B54A32D192ED03)
+
+
+def _splitmix(x):
+    with np.errstate(over="ignore"):
+        x = x + _K1
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def uniform(seed, stream, idx):
+    """Counter-based U[0,1) doubles for integer counters idx (array)."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = np.uint64(seed) * _K1 + np.uint64(stream) * _K2 + idx
+    return (_splitmix(x) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+@dataclass
+class Case:
+    """One case: the &PARAMETERS namelist (src/param.f90:66-86) plus the sizes
+    src/gridsize.f90 fixes at compile time, the seed and the run length."""
+    name: str = "case"
+    itot: int = 64
+    jtot: int = 64
+    ktot: int = 64
+    lx: float = 1.0
+    ly: float = 1.0
+    lz: float = 1.0
+    np: int = 8
+    diameter_h: float = 8.0        # sphere diameter in grid spacings
+    ratiorho: float = 1.5
+    re_bulk: float = 2000.0        # visc = 1 / re_bulk (U_bulk = 1, lz = 1)
+    pran: float = 1.0
+    beta: float = 0.0
+    perturb: float = 0.1
+    particle_rotation: bool = False
+    bc_uvw_top: int = 0            # 0 BC_NOSLIP, 1 BC_FREESLIP
+    bc_uvw_bot: int = 0
+    bc_c_top: int = 0              # 0 BC_DIRICHLET, 1 BC_NEUMANN
+    bc_c_bot: int = 0
+    bc_c_top_val: float = 0.0
+    bc_c_bot_val: float = 0.0
+    seed: int = 1
+    steps: int = 100
+    ioutchk: int = 10
+    nu: float = 0.0                # explicit kinematic viscosity (overrides 1/re_bulk when > 0)
+
+    @property
+    def visc(self):
+        return self.nu if self.nu > 0.0 else 1.0 / self.re_bulk
+
+    @property
+    def radius(self):
+        return 0.5 * self.diameter_h * (self.lx / self.itot)
+
+    @property
+    def nl(self):
+        return nl_formula(self.itot, self.lx, self.radius)
+
+    @property
+    def cells(self):
+        return self.itot * self.jtot * self.ktot
+
+    def as_dict(self):
+        return asdict(self)
+
+
+def nl_formula(itot, lx, radius):
+    """src/init.f90:127-133: nl = nint((pi/3)*(12*((radius-retrac)*dxi)**2+1))."""
+    pi = math.acos(-1.0)
+    dxi = itot / lx
+    retrac = 0.3 / dxi
+    v = (pi / 3.0) * (12.0 * (((radius - retrac) * dxi) * ((radius - retrac) * dxi)) + 1.0)
+    return int(math.floor(v + 0.5)) if v >= 0 else -int(math.floor(-v + 0.5))
+
+
+# The configurations the benchmark and the parity tests run (sizes of the
+# reference method's channel: lx = ly = lz = 1, flow along y, walls in z).
+CONFIGS = {
+    "c1_64": Case(name="c1_64", itot=64, jtot=64, ktot=64, np=8, diameter_h=8.0, ratiorho=1.5, re_bulk=2000.0,
+                  seed=1, steps=100),
+    "c2_256": Case(name="c2_256", itot=256, jtot=256, ktot=256, np=1000, diameter_h=12.0, ratiorho=1.5,
+                   re_bulk=5000.0, seed=2, steps=1000),
+    "c4_512": Case(name="c4_512", itot=512, jtot=512, ktot=512, np=8000, diameter_h=16.0, ratiorho=2.0,
+                   re_bulk=10000.0, seed=4, steps=200),
+    # small parity cases
+    "t16": Case(name="t16", itot=16, jtot=16, ktot=16, np=2, diameter_h=5.0, ratiorho=1.5, re_bulk=200.0, seed=7,
+                steps=4, ioutchk=2),
+    "t32": Case(name="t32", itot=32, jtot=32, ktot=32, np=6, diameter_h=8.0, ratiorho=1.5, re_bulk=1000.0, seed=3,
+                steps=10, ioutchk=5),
+    "t24x16x32": Case(name="t24x16x32", itot=24, jtot=16, ktot=32, np=3, diameter_h=6.0, ratiorho=1.5,
+                      re_bulk=500.0, seed=5, steps=4, ioutchk=2),
+}
+
+
+def initial_fields(case):
+    """unew, vnew, wnew, pnew, cnew as (ktot+2, jtot+2, itot+2) arrays, i.e.
+    real(8) (0:i1,0:j1,0:k1) in Fortran order, before any boundary call."""
+    nx, ny, nz = case.itot, case.jtot, case.ktot
+    shape = (nz + 2, ny + 2, nx + 2)
+    u = np.zeros(shape)
+    v = np.zeros(shape)
+    w = np.zeros(shape)
+    p = np.zeros(shape)
+    c = np.zeros(shape)
+    dz = 1.0 / (nz / case.lz)
+    k = np.arange(nz + 2, dtype=np.float64)
+    coorz = (k - 0.5) * dz / case.lz
+    prof = 6.0 * 1.0 * coorz * (1.0 - coorz)
+    v[:, :, :] = prof[:, None, None]
+    if case.perturb != 0.0:
+        idx = np.arange(nx * ny * nz, dtype=np.uint64).reshape(nz, ny, nx)
+        for comp, arr in enumerate((u, v, w)):
+            r = uniform(case.seed, 100 + comp, idx)
+            arr[1:-1, 1:-1, 1:-1] = arr[1:-1, 1:-1, 1:-1] + case.perturb * (2.0 * r - 1.0)
+    return u, v, w, p, c
+
+
+def place_spheres(case, max_tries=2_000_000):
+    """Seeded rejection sampling: centre gap >= D + h (minimum image in the
+    periodic x, y directions), wall gap >= R + h.  Returns x, y, z arrays."""
+    n = case.np
+    if n == 0:
+        return np.zeros(0), np.zeros(0), np.zeros(0)
+    h = case.lx / case.itot
+    R = case.radius
+    gap = 2.0 * R + h
+    zlo, zhi = R + h, case.lz - R - h
+    ncx = max(1, int(case.lx / gap))
+    ncy = max(1, int(case.ly / gap))
+    ncz = max(1, int(case.lz / gap))
+    grid = {}
+    xs, ys, zs = [], [], []
+    counter = 0
+    block = 4096
+    while len(xs) < n:
+        r = uniform(case.seed, 200, np.arange(counter, counter + 3 * block, dtype=np.uint64)).reshape(block, 3)
+        counter += 3 * block
+        if counter > 3 * max_tries:
+            raise RuntimeError(f"could not place {n} spheres")
+        for a, b, cc in r:
+            x = case.lx * a
+            y = case.ly * b
+            z = zlo + (zhi - zlo) * cc
+            ix, iy, iz = int(x / case.lx * ncx) % ncx, int(y / case.ly * ncy) % ncy, min(int(z / case.lz * ncz), ncz - 1)
+            ok = True
+            for ddx in (-1, 0, 1):
+                for ddy in (-1, 0, 1):
+                    for ddz in (-1, 0, 1):
+                        key = ((ix + ddx) % ncx, (iy + ddy) % ncy, iz + ddz)
+                        for q in grid.get(key, ()):
+                            dx = x - xs[q]
+                            dx -= case.lx * round(dx / case.lx)
+                            dy = y - ys[q]
+                            dy -= case.ly * round(dy / case.ly)
+                            dzz = z - zs[q]
+                            if dx * dx + dy * dy + dzz * dzz < gap * gap:
+                                ok = False
+                                break
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                grid.setdefault((ix, iy, iz), []).append(len(xs))
+                xs.append(x)
+                ys.append(y)
+                zs.append(z)
+                if len(xs) == n:
+                    break
+    return np.array(xs), np.array(ys), np.array(zs)
+
+
+def fortran_e16_8(v):
+    """Fortran E16.8 edit descriptor: ' 0.12345678E+01' right-justified in 16."""
+    if v == 0.0 or not math.isfinite(v):
+        s = "0.00000000E+00"
+        return s.rjust(16)
+    sgn = "-" if v < 0 else ""
+    a = abs(v)
+    m, e = f"{a:.7E}".split("E")
+    digits = m.replace(".", "")
+    ex = int(e) + 1
+    s = f"{sgn}0.{digits}E{'+' if ex >= 0 else '-'}{abs(ex):02d}"
+    return s.rjust(16)
+
+
+def marker_angles(nl):
+    """Golden-angle spiral of nl points on the unit sphere: (theta, phi)."""
+    l = np.arange(nl, dtype=np.float64)
+    zc = 1.0 - (2.0 * l + 1.0) / nl
+    theta = np.arccos(zc)
+    phi = np.mod(l * (math.pi * (3.0 - math.sqrt(5.0))), 2.0 * math.pi)
+    return theta, phi
+
+
+def write_markers(path, case):
+    """Write lagrangianforcepoints2 (3 header lines, nl lines of 6E16.8:
+    x, y, z, theta, phi, radfp) as src/poslfp/lfp.f:326-337 does."""
+    nl = case.nl
+    radfp = case.radius - 0.3 / (case.itot / case.lx)
+    theta, phi = marker_angles(nl)
+    with open(path, "w") as f:
+        f.write(' VARIABLES = "lfpx","lfpy","lfpz","theta","phi","radfp"\n')
+        f.write(f' ZONE T="Zone1" I={nl:12d}, F=POINT\n')
+        f.write(" \n")
+        for t, p in zip(theta, phi):
+            vals = (radfp * math.sin(t) * math.cos(p), radfp * math.sin(t) * math.sin(p), radfp * math.cos(t), t, p, radfp)
+            f.write("".join(fortran_e16_8(x) for x in vals) + "\n")
+    return path
+
+
+def read_markers(path, nl):
+    """src/coordsfp.f90:21-29: skip 3 lines, read nl records (6E16.8) and keep
+    theta, phi and radfp."""
+    theta = np.zeros(nl)
+    phi = np.zeros(nl)
+    radfp = 0.0
+    with open(path) as f:
+        for _ in range(3):
+            f.readline()
+        for l in range(nl):
+            line = f.readline()
+            vals = [float(line[16 * q:16 * (q + 1)]) for q in range(6)]
+            theta[l], phi[l], radfp = vals[3], vals[4], vals[5]
+    return theta, phi, radfp
+
+
+def markers_for(case, workdir=None):
+    """Marker angles and radius as every consumer reads them from the file."""
+    import tempfile
+    d = workdir or tempfile.mkdtemp(prefix="ipg_lfp_")
+    path = os.path.join(d, "lagrangianforcepoints2")
+    write_markers(path, case)
+    return read_markers(path, case.nl)
+
+
+def write_namelist(path, case):
+    """parameters.inp in the &PARAMETERS namelist of src/param.f90:66-86."""
+    bcu = {0: "BC_NOSLIP", 1: "BC_FREESLIP"}
+    bcc = {0: "BC_DIRICHLET", 1: "BC_NEUMANN"}
+    with open(path, "w") as f:
+        f.write("&PARAMETERS\n")
+        f.write(f"  lx = {case.lx!r}, ly = {case.ly!r}, lz = {case.lz!r}\n")
+        f.write("  iniu = 'poi', inic = 'ZERC'\n")
+        f.write(f"  bc_uvw_top_type = '{bcu[case.bc_uvw_top]}', bc_uvw_bot_type = '{bcu[case.bc_uvw_bot]}'\n")
+        f.write(f"  bc_c_top_type = '{bcc[case.bc_c_top]}', bc_c_bot_type = '{bcc[case.bc_c_bot]}'\n")
+        f.write(f"  bc_c_top_val = {case.bc_c_top_val!r}, bc_c_bot_val = {case.bc_c_bot_val!r}\n")
+        f.write(f"  visc = {case.visc!r}, Pran = {case.pran!r}, beta = {case.beta!r}\n")
+        f.write(f"  radius = {case.radius!r}, ratiorho = {case.ratiorho!r}\n")
+        f.write(f"  particle_rotation = {'.true.' if case.particle_rotation else '.false.'}\n")
+        f.write(f"  ioutchk = {case.ioutchk}, iout1d = {case.ioutchk}, iout2d = 100000, ioutfld = 100000\n")
+        f.write("  runmode = 0\n")
+        f.write("/\n")
+    return path

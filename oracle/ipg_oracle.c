@@ -1,0 +1,1249 @@
+/*
+ * ipg_oracle.c -- FP64 CPU restatement of the InterPartS RK3 substep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is loaded by the
+ * tests, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never
+ * by the product path (interparts_mpich_amd/), which must run on the GPU.
+ *
+ * It restates, loop for loop and operation for operation, the reference
+ * routines of ajayhudge/InterPartS_MPICH run on a single MPI rank
+ * (dims = (1,1)); every function cites the file:line it follows.  Arrays keep
+ * the reference's Fortran layout: fields (0:i1,0:j1,0:k1), i fastest; IBM
+ * fields (-1:i1+1,-1:j1+1,-1:k1+1).  It is compiled with
+ * -ffp-contract=off so that no fused multiply-add changes a rounding.
+ *
+ * Where the reference writes x**2. (real exponent 2) this file writes x*x,
+ * and x**(-1.) becomes 1./x: gfortran folds both forms before they reach
+ * libm.  x**3. stays pow(x,3.0) (glibc), as gfortran calls pow for it.
+ *
+ * Pinning (see tests/test_oracle_*.py): delta kernel vs closed form and the
+ * reference's own kernel.f90 compiled into oracle/_ref; the real FFT vs
+ * numpy.fft and the reference's VFFTPACK (fft.f) compiled into oracle/_ref;
+ * stencils vs closed forms on polynomial / trigonometric fields; Poisson vs a
+ * dense least-squares solve on tiny grids; IBM interp/spread adjointness and
+ * partition of unity; sphere integrals vs 4/3 pi R^3 and rigid fields;
+ * Newton-Euler vs the free-fall closed form; collisions vs the closed-form
+ * spring-dashpot force; chkdt vs the dtmax known answer of src/terminate.txt.
+ * Not pinned by any reference output (parity unpinned beyond those checks):
+ * the full substep sequencing of src/main.f90:268-414 and the 9x contact
+ * multiplicity of src/intgr_nwtn_eulr.f90:358-499 at one rank.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NB 9
+
+typedef struct {
+  /* grid and physics (src/param.f90, src/init.f90:74-243) */
+  int nx, ny, nz;
+  double lx, ly, lz, dxi, dyi, dzi, dx, dy, dz;
+  double visc, pran, kappa, beta, radius, ratiorho;
+  int rotation;
+  int bc_uvw_top, bc_uvw_bot; /* 0 no-slip, 1 free-slip */
+  int bc_c_top, bc_c_bot;     /* 0 Dirichlet, 1 Neumann */
+  double bc_c_top_val, bc_c_bot_val;
+  double pi, offset, volp, mominert, retrac, gaccx, gaccy, gaccz;
+  double rkcoeffab[3];
+  /* collisions / lubrication constants actually reaching the state */
+  double meffn_ss, kn_ss, etan_ss, meffn_sw, kn_sw, etan_sw, coeff_f;
+  double a11_ini_pp, a11a_ini_pp, a11b_ini_pp, a11a_sat_pp, a11b_sat_pp, a11_sat_pp;
+  double a11_ini_pw, a11_sat_pw;
+  /* fields */
+  size_t sx, sy, n1;       /* (nx+2), (ny+2), total cells with 1 ghost */
+  size_t fx, fy, n2;       /* (nx+4), (ny+4), total cells with 2 ghosts */
+  double *u, *v, *w, *pnew, *c;
+  double *dudt, *dvdt, *dwdt, *dcdt;
+  double *durk, *dvrk, *dwrk, *dcrk;
+  double *dudtold, *dvdtold, *dwdtold;
+  double *p;
+  double *dudtf, *dvdtf, *dwdtf;
+  double *tmp1, *tmp2, *tmp3, *tmp4, *tmp5; /* rk stack temporaries */
+  /* Poisson solver */
+  double *a, *b, *cc, *xyrt, *dtri;
+  /* particles (SoA, np) */
+  int np, nl;
+  double *px, *py, *pz, *ptheta, *pphi, *pu, *pv, *pw, *pomx, *pomy, *pomz, *pomtheta;
+  double *intu, *intv, *intw, *intomx, *intomy, *intomz;
+  double *colfx, *colfy, *colfz, *coltx, *colty, *coltz;
+  double *fxltot, *fyltot, *fzltot, *torqx, *torqy, *torqz, *torqtheta;
+  int *nbf; /* np*9 neighbour flags, nbf[p*9+nb], nb=1..8 */
+  /* markers (np*nl) */
+  int *nla;
+  double *xfp, *yfp, *zfp, *ul, *vl, *wl, *dudtl, *dvdtl, *dwdtl, *fxl, *fyl, *fzl;
+  double *thetarc, *phirc, radfp, dVlagr, dVeul;
+  /* time */
+  double dt, time;
+  double forceytot, diag_fsum[3];
+} orc_state;
+
+static inline size_t I1(const orc_state *s, int i, int j, int k) {
+  return (size_t)i + s->sx * ((size_t)j + s->sy * (size_t)k);
+}
+static inline size_t I2(const orc_state *s, int i, int j, int k) {
+  return (size_t)(i + 1) + s->fx * ((size_t)(j + 1) + s->fy * (size_t)(k + 1));
+}
+static inline double sq(double x) { return x * x; }
+/* Fortran nint: round half away from zero (H3) */
+static inline int nint_f(double x) { return (int)lround(x); }
+
+static double *zalloc(size_t n) {
+  double *p = (double *)calloc(n ? n : 1, sizeof(double));
+  if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+  return p;
+}
+
+/* ---------------------------------------------------------------------- */
+/* src/kernel.f90:7-25 -- Roma et al. 3-point regularized delta (times h) */
+double orc_kernel(double r) {
+  double ar = fabs(r);
+  if (ar > 1.5) return 0.0;
+  if (ar > 0.5) return (1.0 / 6.0) * ((5.0 - 3.0 * ar) - sqrt(-(3.0 * sq(1.0 - ar)) + 1.0));
+  return (1.0 / 3.0) * (1.0 + sqrt(-(3.0 * sq(r)) + 1.0));
+}
+
+/* src/kernel.f90:43-63 -- kerneltest: integral of the 3-D kernel on a 1/12 grid */
+double orc_kerneltest(void) {
+  double sum = 0.0;
+  for (int k = -25; k <= 25; k++) {
+    double cz = k / 12.0;
+    for (int j = -25; j <= 25; j++) {
+      double cy = j / 12.0;
+      for (int i = -25; i <= 25; i++) {
+        double cx = i / 12.0;
+        sum = sum + ((((1.0 / 12.0) * (1.0 / 12.0)) * (1.0 / 12.0)) * orc_kernel(cx)) * orc_kernel(cy) * orc_kernel(cz);
+      }
+    }
+  }
+  return sum;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Real FFT with VFFTPACK semantics (src/libs/fft/fft.f:693-761 forward,
+ * 1-45 backward): orthonormal sqrt(1/N) scaling both ways, half-complex
+ * packing [r0, r1, i1, ..., r_{N/2}] with F_k = sqrt(1/N) sum x_j e^{-2pi i jk/N}.
+ * The transform is this file's own mixed-radix Stockham FFT, not a restatement
+ * of VFFTPACK's butterflies; it is pinned against the compiled fft.f.        */
+typedef struct { int n; int nr; int rad[32]; double *wr, *wi; double *ar, *ai, *br, *bi; } orc_fft;
+
+static void fft_plan(orc_fft *f, int n) {
+  f->n = n; f->nr = 0;
+  int m = n;
+  while (m % 4 == 0) { f->rad[f->nr++] = 4; m /= 4; }
+  while (m % 2 == 0) { f->rad[f->nr++] = 2; m /= 2; }
+  while (m % 3 == 0) { f->rad[f->nr++] = 3; m /= 3; }
+  while (m % 5 == 0) { f->rad[f->nr++] = 5; m /= 5; }
+  if (m != 1) { fprintf(stderr, "oracle fft: unsupported length %d\n", n); abort(); }
+  f->wr = zalloc(n); f->wi = zalloc(n);
+  for (int t = 0; t < n; t++) { double a = 2.0 * M_PI * (double)t / (double)n; f->wr[t] = cos(a); f->wi[t] = sin(a); }
+  f->ar = zalloc(n); f->ai = zalloc(n); f->br = zalloc(n); f->bi = zalloc(n);
+}
+static void fft_free(orc_fft *f) { free(f->wr); free(f->wi); free(f->ar); free(f->ai); free(f->br); free(f->bi); }
+
+/* in-place complex DFT of (f->ar, f->ai): sum_j x_j e^{sign*2 pi i jk/n}, sign -1 forward,
+ * +1 backward, no scaling.  Table: (wr, wi)[t] = e^{+2 pi i t/n}.                           */
+static void fft_exec(orc_fft *f, int sign) {
+  int n = f->n;
+  double *xr = f->ar, *xi = f->ai, *yr = f->br, *yi = f->bi;
+  int Ns = 1;
+  for (int s = 0; s < f->nr; s++) {
+    int R = f->rad[s];
+    for (int j = 0; j < n / R; j++) {
+      double vr[5], vi[5];
+      int jm = j % Ns;
+      for (int r = 0; r < R; r++) {
+        double a = xr[j + r * (n / R)], b = xi[j + r * (n / R)];
+        int t = (r * jm * (n / (Ns * R))) % n;
+        double c = f->wr[t], d = sign * f->wi[t];
+        vr[r] = a * c - b * d; vi[r] = a * d + b * c;
+      }
+      double or_[5], oi[5];
+      for (int k = 0; k < R; k++) {
+        double sr = 0.0, si = 0.0;
+        for (int r = 0; r < R; r++) {
+          int t = ((r * k) % R) * (n / R);
+          double c = f->wr[t], d = sign * f->wi[t];
+          sr += vr[r] * c - vi[r] * d; si += vr[r] * d + vi[r] * c;
+        }
+        or_[k] = sr; oi[k] = si;
+      }
+      int idxD = (j / Ns) * Ns * R + jm;
+      for (int r = 0; r < R; r++) { yr[idxD + r * Ns] = or_[r]; yi[idxD + r * Ns] = oi[r]; }
+    }
+    double *t1 = xr; xr = yr; yr = t1; t1 = xi; xi = yi; yi = t1;
+    Ns *= R;
+  }
+  if (xr != f->ar) { memcpy(f->ar, xr, n * sizeof(double)); memcpy(f->ai, xi, n * sizeof(double)); }
+}
+
+/* forward: x (length n, stride st) -> VFFTPACK half-complex packing, in place */
+static void orc_rfftf_plan(orc_fft *f, double *x, size_t st) {
+  int n = f->n;
+  for (int j = 0; j < n; j++) { f->ar[j] = x[j * st]; f->ai[j] = 0.0; }
+  fft_exec(f, -1);
+  double sc = sqrt(1.0 / n);
+  x[0] = sc * f->ar[0];
+  for (int k = 1; k < (n + 1) / 2; k++) { x[(2 * k - 1) * st] = sc * f->ar[k]; x[(2 * k) * st] = sc * f->ai[k]; }
+  if (n % 2 == 0) x[(n - 1) * st] = sc * f->ar[n / 2];
+}
+static void orc_rfftb_plan(orc_fft *f, double *x, size_t st) {
+  int n = f->n;
+  f->ar[0] = x[0]; f->ai[0] = 0.0;
+  for (int k = 1; k < (n + 1) / 2; k++) {
+    double re = x[(2 * k - 1) * st], im = x[(2 * k) * st];
+    f->ar[k] = re; f->ai[k] = im; f->ar[n - k] = re; f->ai[n - k] = -im;
+  }
+  if (n % 2 == 0) { f->ar[n / 2] = x[(n - 1) * st]; f->ai[n / 2] = 0.0; }
+  fft_exec(f, +1);
+  double sc = sqrt(1.0 / n);
+  for (int j = 0; j < n; j++) x[j * st] = sc * f->ar[j];
+}
+void orc_rfftf(int n, double *x) { orc_fft f; fft_plan(&f, n); orc_rfftf_plan(&f, x, 1); fft_free(&f); }
+void orc_rfftb(int n, double *x) { orc_fft f; fft_plan(&f, n); orc_rfftb_plan(&f, x, 1); fft_free(&f); }
+
+/* ---------------------------------------------------------------------- */
+/* src/init.f90:74-243 derived_parameters (the constants on the path)     */
+static void derived(orc_state *s) {
+  s->pi = acos(-1.0);
+  s->dxi = s->nx / s->lx; s->dyi = s->ny / s->ly; s->dzi = s->nz / s->lz;
+  s->dx = 1.0 / s->dxi; s->dy = 1.0 / s->dyi; s->dz = 1.0 / s->dzi;
+  s->kappa = s->visc / s->pran;
+  s->offset = (sqrt(3.0 * (1.5 * 1.5))) / s->dxi + 0.01 / s->dxi;
+  s->volp = ((4.0 / 3.0) * s->pi) * pow(s->radius, 3.0);
+  s->mominert = ((2.0 / 5.0) * s->volp) * sq(s->radius);
+  s->retrac = 0.3 / s->dxi;
+  /* src/param.f90:102-104,110-112,115-120 */
+  s->gaccx = -9.81 * cos(s->pi / 2.0);
+  s->gaccy = -9.81 * cos(s->pi / 2.0);
+  s->gaccz = -9.81 * sin(s->pi / 2.0);
+  s->rkcoeffab[0] = 32.0 / 60.0 + 0.0;
+  s->rkcoeffab[1] = 25.0 / 60.0 + (-17.0 / 60.0);
+  s->rkcoeffab[2] = 45.0 / 60.0 + (-25.0 / 60.0);
+  const double Nstretch = 8.0, dt_estim = 0.05, en = 0.97;
+  const double eps_ini_pp = 0.025, eps_sat_pp = 0.001, eps_ini_pw = 0.075, eps_sat_pw = 0.001;
+  double pi = s->pi;
+  s->meffn_ss = s->ratiorho * s->volp / 2.0;
+  s->kn_ss = (sq(pi) + sq(fabs(log(en)))) * s->meffn_ss / sq(Nstretch * dt_estim);
+  s->etan_ss = -2.0 * (log(en)) * s->meffn_ss / (Nstretch * dt_estim);
+  s->meffn_sw = s->ratiorho * s->volp;
+  s->kn_sw = (sq(pi) + sq(fabs(log(en)))) * s->meffn_sw / sq(Nstretch * dt_estim);
+  s->etan_sw = -2.0 * (log(en)) * s->meffn_sw / (Nstretch * dt_estim);
+  s->coeff_f = 6.0 * pi * s->visc * s->radius;
+  s->a11_ini_pp = -(1.0 / 4.0 * (1.0 / eps_ini_pp)) + 9.0 / 40.0 * log(eps_ini_pp) + 3.0 / 112.0 * eps_ini_pp * log(eps_ini_pp);
+  s->a11a_ini_pp = s->a11_ini_pp - .995;
+  s->a11b_ini_pp = -s->a11_ini_pp + .350;
+  s->a11_sat_pp = -(1.0 / 4.0 * (1.0 / eps_sat_pp)) + 9.0 / 40.0 * log(eps_sat_pp) + 3.0 / 112.0 * eps_sat_pp * log(eps_sat_pp);
+  s->a11a_sat_pp = s->a11_sat_pp - .995;
+  s->a11b_sat_pp = -s->a11_sat_pp + .350;
+  s->a11_ini_pw = -(1.0 / eps_ini_pw) + 1.0 / 5.0 * log(eps_ini_pw) + 1.0 / 21.0 * eps_ini_pw * log(eps_ini_pw) - .9713;
+  s->a11_sat_pw = -(1.0 / eps_sat_pw) + 1.0 / 5.0 * log(eps_sat_pw) + 1.0 / 21.0 * eps_sat_pw * log(eps_sat_pw) - .9713;
+}
+
+/* src/init.f90:129 -- number of Lagrangian force points on the outer shell */
+int orc_nl_formula(int nx, double lx, double radius) {
+  double pi = acos(-1.0), dxi = nx / lx, retrac = 0.3 / dxi;
+  return nint_f((pi / 3.0) * (12.0 * sq((radius - retrac) * dxi) + 1.0));
+}
+
+orc_state *orc_create(int nx, int ny, int nz, double lx, double ly, double lz, double visc, double pran,
+                      double beta, double radius, double ratiorho, int np, int nl, int rotation,
+                      int bc_uvw_top, int bc_uvw_bot, int bc_c_top, int bc_c_bot, double ctop, double cbot) {
+  orc_state *s = (orc_state *)calloc(1, sizeof(orc_state));
+  s->nx = nx; s->ny = ny; s->nz = nz; s->lx = lx; s->ly = ly; s->lz = lz;
+  s->visc = visc; s->pran = pran; s->beta = beta; s->radius = radius; s->ratiorho = ratiorho;
+  s->np = np; s->nl = nl; s->rotation = rotation;
+  s->bc_uvw_top = bc_uvw_top; s->bc_uvw_bot = bc_uvw_bot; s->bc_c_top = bc_c_top; s->bc_c_bot = bc_c_bot;
+  s->bc_c_top_val = ctop; s->bc_c_bot_val = cbot;
+  derived(s);
+  s->sx = nx + 2; s->sy = ny + 2; s->n1 = s->sx * s->sy * (size_t)(nz + 2);
+  s->fx = nx + 4; s->fy = ny + 4; s->n2 = s->fx * s->fy * (size_t)(nz + 4);
+  double **f1[] = {&s->u, &s->v, &s->w, &s->pnew, &s->c, &s->dudt, &s->dvdt, &s->dwdt, &s->dcdt,
+                   &s->durk, &s->dvrk, &s->dwrk, &s->dcrk, &s->dudtold, &s->dvdtold, &s->dwdtold, &s->p,
+                   &s->tmp1, &s->tmp2, &s->tmp3, &s->tmp4, &s->tmp5};
+  for (size_t q = 0; q < sizeof(f1) / sizeof(f1[0]); q++) *f1[q] = zalloc(s->n1);
+  s->dudtf = zalloc(s->n2); s->dvdtf = zalloc(s->n2); s->dwdtf = zalloc(s->n2);
+  s->a = zalloc(nz + 1); s->b = zalloc(nz + 1); s->cc = zalloc(nz + 1);
+  s->xyrt = zalloc((size_t)nx * ny); s->dtri = zalloc((size_t)nx * ny * nz);
+  int npp = np > 0 ? np : 1;
+  double **pa[] = {&s->px, &s->py, &s->pz, &s->ptheta, &s->pphi, &s->pu, &s->pv, &s->pw, &s->pomx, &s->pomy,
+                   &s->pomz, &s->pomtheta, &s->intu, &s->intv, &s->intw, &s->intomx, &s->intomy, &s->intomz,
+                   &s->colfx, &s->colfy, &s->colfz, &s->coltx, &s->colty, &s->coltz, &s->fxltot, &s->fyltot,
+                   &s->fzltot, &s->torqx, &s->torqy, &s->torqz, &s->torqtheta};
+  for (size_t q = 0; q < sizeof(pa) / sizeof(pa[0]); q++) *pa[q] = zalloc(npp);
+  s->nbf = (int *)calloc((size_t)npp * NB, sizeof(int));
+  s->nla = (int *)calloc(npp, sizeof(int));
+  size_t nm = (size_t)npp * (nl > 0 ? nl : 1);
+  double **ma[] = {&s->xfp, &s->yfp, &s->zfp, &s->ul, &s->vl, &s->wl, &s->dudtl, &s->dvdtl, &s->dwdtl, &s->fxl, &s->fyl, &s->fzl};
+  for (size_t q = 0; q < sizeof(ma) / sizeof(ma[0]); q++) *ma[q] = zalloc(nm);
+  s->thetarc = zalloc(nl > 0 ? nl : 1); s->phirc = zalloc(nl > 0 ? nl : 1);
+  s->dVeul = 1.0 / (s->dxi * s->dyi * s->dzi);
+  return s;
+}
+
+void orc_destroy(orc_state *s) {
+  if (!s) return;
+  double *all[] = {s->u, s->v, s->w, s->pnew, s->c, s->dudt, s->dvdt, s->dwdt, s->dcdt, s->durk, s->dvrk, s->dwrk,
+                   s->dcrk, s->dudtold, s->dvdtold, s->dwdtold, s->p, s->tmp1, s->tmp2, s->tmp3, s->tmp4, s->tmp5,
+                   s->dudtf, s->dvdtf, s->dwdtf, s->a, s->b, s->cc, s->xyrt, s->dtri, s->px, s->py, s->pz,
+                   s->ptheta, s->pphi, s->pu, s->pv, s->pw, s->pomx, s->pomy, s->pomz, s->pomtheta, s->intu,
+                   s->intv, s->intw, s->intomx, s->intomy, s->intomz, s->colfx, s->colfy, s->colfz, s->coltx,
+                   s->colty, s->coltz, s->fxltot, s->fyltot, s->fzltot, s->torqx, s->torqy, s->torqz,
+                   s->torqtheta, s->xfp, s->yfp, s->zfp, s->ul, s->vl, s->wl, s->dudtl, s->dvdtl, s->dwdtl,
+                   s->fxl, s->fyl, s->fzl, s->thetarc, s->phirc};
+  for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); q++) free(all[q]);
+  free(s->nbf); free(s->nla); free(s);
+}
+
+/* field access for the test harness: id order matches interparts_mpich_amd.fields */
+double *orc_field(orc_state *s, int id) {
+  double *t[] = {s->u, s->v, s->w, s->pnew, s->c, s->dudt, s->dvdt, s->dwdt, s->dcdt, s->durk, s->dvrk, s->dwrk,
+                 s->dcrk, s->p, s->dudtf, s->dvdtf, s->dwdtf, s->dudtold, s->dvdtold, s->dwdtold};
+  if (id < 0 || id >= (int)(sizeof(t) / sizeof(t[0]))) return NULL;
+  return t[id];
+}
+double *orc_particle(orc_state *s, int id) {
+  double *t[] = {s->px, s->py, s->pz, s->ptheta, s->pphi, s->pu, s->pv, s->pw, s->pomx, s->pomy, s->pomz,
+                 s->pomtheta, s->intu, s->intv, s->intw, s->intomx, s->intomy, s->intomz, s->colfx, s->colfy,
+                 s->colfz, s->coltx, s->colty, s->coltz, s->fxltot, s->fyltot, s->fzltot, s->torqx, s->torqy,
+                 s->torqz, s->torqtheta};
+  if (id < 0 || id >= (int)(sizeof(t) / sizeof(t[0]))) return NULL;
+  return t[id];
+}
+double *orc_marker(orc_state *s, int id) {
+  double *t[] = {s->xfp, s->yfp, s->zfp, s->ul, s->vl, s->wl, s->dudtl, s->dvdtl, s->dwdtl, s->fxl, s->fyl, s->fzl};
+  if (id < 0 || id >= (int)(sizeof(t) / sizeof(t[0]))) return NULL;
+  return t[id];
+}
+int *orc_nla(orc_state *s) { return s->nla; }
+int *orc_nbflags(orc_state *s) { return s->nbf; }
+void orc_set_dt(orc_state *s, double dt) { s->dt = dt; }
+double orc_get_dt(orc_state *s) { return s->dt; }
+double orc_scalar(orc_state *s, int id) {
+  switch (id) {
+    case 0: return s->dVlagr; case 1: return s->dVeul; case 2: return s->volp; case 3: return s->radfp;
+    case 4: return s->kn_ss; case 5: return s->etan_ss; case 6: return s->kn_sw; case 7: return s->etan_sw;
+    case 8: return s->offset; case 9: return s->gaccz; case 10: return s->gaccx; case 11: return s->forceytot;
+    case 12: return s->rkcoeffab[0]; case 13: return s->rkcoeffab[1]; case 14: return s->rkcoeffab[2];
+    case 15: return s->mominert; case 16: return s->coeff_f;
+  }
+  return NAN;
+}
+
+/* ---------------------------------------------------------------------- */
+/* src/bound.f90:173-221 updthalos, single rank: periodic self copies     */
+static void updthalos(orc_state *s, double *var, int dir) {
+  int nx = s->nx, ny = s->ny, nz = s->nz;
+  if (dir == 1) {
+    for (int k = 0; k <= nz + 1; k++)
+      for (int j = 0; j <= ny + 1; j++) var[I1(s, nx + 1, j, k)] = var[I1(s, 1, j, k)];
+    for (int k = 0; k <= nz + 1; k++)
+      for (int j = 0; j <= ny + 1; j++) var[I1(s, 0, j, k)] = var[I1(s, nx, j, k)];
+  } else {
+    for (int k = 0; k <= nz + 1; k++)
+      for (int i = 0; i <= nx + 1; i++) var[I1(s, i, ny + 1, k)] = var[I1(s, i, 1, k)];
+    for (int k = 0; k <= nz + 1; k++)
+      for (int i = 0; i <= nx + 1; i++) var[I1(s, i, 0, k)] = var[I1(s, i, ny, k)];
+  }
+}
+
+/* src/bound.f90:11-80 */
+static void bounduvw3(orc_state *s, double *u, double *v, double *w) {
+  int nx = s->nx, ny = s->ny, nz = s->nz;
+  for (int j = 0; j <= ny + 1; j++)
+    for (int i = 0; i <= nx + 1; i++) {
+      if (s->bc_uvw_top == 0) { u[I1(s, i, j, nz + 1)] = -u[I1(s, i, j, nz)]; v[I1(s, i, j, nz + 1)] = -v[I1(s, i, j, nz)]; }
+      else { u[I1(s, i, j, nz + 1)] = u[I1(s, i, j, nz)]; v[I1(s, i, j, nz + 1)] = v[I1(s, i, j, nz)]; }
+      w[I1(s, i, j, nz)] = 0.0;
+    }
+  for (int j = 0; j <= ny + 1; j++)
+    for (int i = 0; i <= nx + 1; i++) {
+      if (s->bc_uvw_bot == 0) { u[I1(s, i, j, 0)] = -u[I1(s, i, j, 1)]; v[I1(s, i, j, 0)] = -v[I1(s, i, j, 1)]; }
+      else { u[I1(s, i, j, 0)] = u[I1(s, i, j, 1)]; v[I1(s, i, j, 0)] = v[I1(s, i, j, 1)]; }
+      w[I1(s, i, j, 0)] = 0.0;
+    }
+  updthalos(s, u, 1); updthalos(s, v, 1); updthalos(s, w, 1);
+  updthalos(s, u, 2); updthalos(s, v, 2); updthalos(s, w, 2);
+}
+void orc_bounduvw(orc_state *s, int which) {
+  if (which == 0) bounduvw3(s, s->u, s->v, s->w); else bounduvw3(s, s->dudt, s->dvdt, s->dwdt);
+}
+
+/* src/bound.f90:82-107 */
+static void boundp1(orc_state *s, double *p) {
+  int nx = s->nx, ny = s->ny, nz = s->nz;
+  for (int j = 0; j <= ny + 1; j++)
+    for (int i = 0; i <= nx + 1; i++) {
+      p[I1(s, i, j, 0)] = p[I1(s, i, j, 1)];
+      p[I1(s, i, j, nz + 1)] = p[I1(s, i, j, nz)];
+    }
+  updthalos(s, p, 1); updthalos(s, p, 2);
+}
+void orc_boundp(orc_state *s, int which) { boundp1(s, which == 0 ? s->pnew : s->p); }
+
+/* src/bound.f90:109-171 */
+void orc_boundscal(orc_state *s) {
+  int nx = s->nx, ny = s->ny, nz = s->nz;
+  double *c = s->c;
+  for (int j = 0; j <= ny + 1; j++)
+    for (int i = 0; i <= nx + 1; i++) {
+      if (s->bc_c_top == 0) c[I1(s, i, j, nz + 1)] = 2.0 * s->bc_c_top_val - c[I1(s, i, j, nz)];
+      else c[I1(s, i, j, nz + 1)] = c[I1(s, i, j, nz)];
+    }
+  for (int j = 0; j <= ny + 1; j++)
+    for (int i = 0; i <= nx + 1; i++) {
+      if (s->bc_c_bot == 0) c[I1(s, i, j, 0)] = 2.0 * s->bc_c_bot_val - c[I1(s, i, j, 1)];
+      else c[I1(s, i, j, 0)] = c[I1(s, i, j, 1)];
+    }
+  updthalos(s, c, 1); updthalos(s, c, 2);
+}
+
+/* ---------------------------------------------------------------------- */
+/* src/mom.f90 -- momentum operators, interior loops k,j,i                 */
+#define FOR_INTERIOR(s) for (int k = 1; k <= (s)->nz; k++) for (int j = 1; j <= (s)->ny; j++) for (int i = 1; i <= (s)->nx; i++)
+
+/* src/mom.f90:8-54 */
+static void momxad(orc_state *s, double *dudt, const double *U, const double *V, const double *W) {
+  const double dxi = s->dxi, dyi = s->dyi, dzi = s->dzi, visc = s->visc;
+#pragma omp parallel for
+  for (int k = 1; k <= s->nz; k++)
+    for (int j = 1; j <= s->ny; j++)
+      for (int i = 1; i <= s->nx; i++) {
+        size_t c = I1(s, i, j, k), ip = I1(s, i + 1, j, k), im = I1(s, i - 1, j, k), jp = I1(s, i, j + 1, k),
+               jm = I1(s, i, j - 1, k), kp = I1(s, i, j, k + 1), km = I1(s, i, j, k - 1);
+        double uuip = 0.25 * (U[ip] + U[c]) * (U[ip] + U[c]);
+        double uuim = 0.25 * (U[im] + U[c]) * (U[im] + U[c]);
+        double uvjp = 0.25 * (U[jp] + U[c]) * (V[ip] + V[c]);
+        double uvjm = 0.25 * (U[jm] + U[c]) * (V[I1(s, i + 1, j - 1, k)] + V[jm]);
+        double uwkp = 0.25 * (U[kp] + U[c]) * (W[ip] + W[c]);
+        double uwkm = 0.25 * (U[km] + U[c]) * (W[I1(s, i + 1, j, k - 1)] + W[km]);
+        double dudxp = (U[ip] - U[c]) * dxi, dudxm = (U[c] - U[im]) * dxi;
+        double dudyp = (U[jp] - U[c]) * dyi, dudym = (U[c] - U[jm]) * dyi;
+        double dudzp = (U[kp] - U[c]) * dzi, dudzm = (U[c] - U[km]) * dzi;
+        dudt[c] = dxi * (-uuip + uuim) + visc * (dudxp - dudxm) * dxi + dyi * (-uvjp + uvjm) +
+                  visc * (dudyp - dudym) * dyi + dzi * (-uwkp + uwkm) + visc * (dudzp - dudzm) * dzi;
+      }
+}
+/* src/mom.f90:81-127 */
+static void momyad(orc_state *s, double *dvdt, const double *U, const double *V, const double *W) {
+  const double dxi = s->dxi, dyi = s->dyi, dzi = s->dzi, visc = s->visc;
+#pragma omp parallel for
+  for (int k = 1; k <= s->nz; k++)
+    for (int j = 1; j <= s->ny; j++)
+      for (int i = 1; i <= s->nx; i++) {
+        size_t c = I1(s, i, j, k), ip = I1(s, i + 1, j, k), im = I1(s, i - 1, j, k), jp = I1(s, i, j + 1, k),
+               jm = I1(s, i, j - 1, k), kp = I1(s, i, j, k + 1), km = I1(s, i, j, k - 1);
+        double uvip = 0.25 * (U[c] + U[jp]) * (V[c] + V[ip]);
+        double uvim = 0.25 * (U[im] + U[I1(s, i - 1, j + 1, k)]) * (V[c] + V[im]);
+        double vvjp = 0.25 * (V[c] + V[jp]) * (V[c] + V[jp]);
+        double vvjm = 0.25 * (V[c] + V[jm]) * (V[c] + V[jm]);
+        double wvkp = 0.25 * (W[c] + W[jp]) * (V[kp] + V[c]);
+        double wvkm = 0.25 * (W[km] + W[I1(s, i, j + 1, k - 1)]) * (V[km] + V[c]);
+        double dvdxp = (V[ip] - V[c]) * dxi, dvdxm = (V[c] - V[im]) * dxi;
+        double dvdyp = (V[jp] - V[c]) * dyi, dvdym = (V[c] - V[jm]) * dyi;
+        double dvdzp = (V[kp] - V[c]) * dzi, dvdzm = (V[c] - V[km]) * dzi;
+        dvdt[c] = dxi * (-uvip + uvim) + visc * (dvdxp - dvdxm) * dxi + dyi * (-vvjp + vvjm) +
+                  visc * (dvdyp - dvdym) * dyi + dzi * (-wvkp + wvkm) + visc * (dvdzp - dvdzm) * dzi;
+      }
+}
+/* src/mom.f90:157-203 */
+static void momzad(orc_state *s, double *dwdt, const double *U, const double *V, const double *W) {
+  const double dxi = s->dxi, dyi = s->dyi, dzi = s->dzi, visc = s->visc;
+#pragma omp parallel for
+  for (int k = 1; k <= s->nz; k++)
+    for (int j = 1; j <= s->ny; j++)
+      for (int i = 1; i <= s->nx; i++) {
+        size_t c = I1(s, i, j, k), ip = I1(s, i + 1, j, k), im = I1(s, i - 1, j, k), jp = I1(s, i, j + 1, k),
+               jm = I1(s, i, j - 1, k), kp = I1(s, i, j, k + 1), km = I1(s, i, j, k - 1);
+        double uwip = 0.25 * (W[c] + W[ip]) * (U[c] + U[kp]);
+        double uwim = 0.25 * (W[c] + W[im]) * (U[im] + U[I1(s, i - 1, j, k + 1)]);
+        double vwjp = 0.25 * (W[c] + W[jp]) * (V[c] + V[kp]);
+        double vwjm = 0.25 * (W[c] + W[jm]) * (V[jm] + V[I1(s, i, j - 1, k + 1)]);
+        double wwkp = 0.25 * (W[c] + W[kp]) * (W[c] + W[kp]);
+        double wwkm = 0.25 * (W[c] + W[km]) * (W[c] + W[km]);
+        double dwdxp = (W[ip] - W[c]) * dxi, dwdxm = (W[c] - W[im]) * dxi;
+        double dwdyp = (W[jp] - W[c]) * dyi, dwdym = (W[c] - W[jm]) * dyi;
+        double dwdzp = (W[kp] - W[c]) * dzi, dwdzm = (W[c] - W[km]) * dzi;
+        dwdt[c] = dxi * (-uwip + uwim) + visc * (dwdxp - dwdxm) * dxi + dyi * (-vwjp + vwjm) +
+                  visc * (dwdyp - dwdym) * dyi + dzi * (-wwkp + wwkm) + visc * (dwdzp - dwdzm) * dzi;
+      }
+}
+/* src/mom.f90:56-79, 129-155, 205-231 */
+static void momp(orc_state *s, double *dudt, double *dvdt, double *dwdt, const double *p) {
+#pragma omp parallel for
+  for (int k = 1; k <= s->nz; k++)
+    for (int j = 1; j <= s->ny; j++)
+      for (int i = 1; i <= s->nx; i++) {
+        size_t c = I1(s, i, j, k);
+        dudt[c] = -(s->dxi * (p[I1(s, i + 1, j, k)] - p[c]));
+        dvdt[c] = -(s->dyi * (p[I1(s, i, j + 1, k)] - p[c]));
+        dwdt[c] = -(s->dzi * (p[I1(s, i, j, k + 1)] - p[c]));
+      }
+}
+/* src/mom.f90:233-259 momzbuoy, then src/rk3.f90:56-58 dw = buoy + adv (all cells) */
+static void buoy_add(orc_state *s, double *dw, const double *c) {
+  double *old = s->tmp5;
+  memcpy(old, dw, s->n1 * sizeof(double));
+#pragma omp parallel for
+  for (int k = 1; k <= s->nz; k++)
+    for (int j = 1; j <= s->ny; j++)
+      for (int i = 1; i <= s->nx; i++) {
+        size_t q = I1(s, i, j, k);
+        dw[q] = 0.5 * s->beta * 9.81 * (c[q] + c[I1(s, i, j, k + 1)]);
+      }
+  for (size_t q = 0; q < s->n1; q++) dw[q] = dw[q] + old[q];
+}
+/* src/scal.f90:11-60 */
+static void scalad(orc_state *s, double *dcdt, const double *U, const double *V, const double *W, const double *C) {
+  const double dxi = s->dxi, dyi = s->dyi, dzi = s->dzi, kappa = s->kappa;
+#pragma omp parallel for
+  for (int k = 1; k <= s->nz; k++)
+    for (int j = 1; j <= s->ny; j++)
+      for (int i = 1; i <= s->nx; i++) {
+        size_t c = I1(s, i, j, k), ip = I1(s, i + 1, j, k), im = I1(s, i - 1, j, k), jp = I1(s, i, j + 1, k),
+               jm = I1(s, i, j - 1, k), kp = I1(s, i, j, k + 1), km = I1(s, i, j, k - 1);
+        double ucip = 0.5 * U[c] * (C[ip] + C[c]);
+        double ucim = 0.5 * U[im] * (C[c] + C[im]);
+        double vcjp = 0.5 * V[c] * (C[jp] + C[c]);
+        double vcjm = 0.5 * V[jm] * (C[c] + C[jm]);
+        double wckp = 0.5 * W[c] * (C[kp] + C[c]);
+        double wckm = 0.5 * W[km] * (C[c] + C[km]);
+        double dcdxp = (C[ip] - C[c]) * dxi, dcdxm = (C[c] - C[im]) * dxi;
+        double dcdyp = (C[jp] - C[c]) * dyi, dcdym = (C[c] - C[jm]) * dyi;
+        double dcdzp = (C[kp] - C[c]) * dzi, dcdzm = (C[c] - C[km]) * dzi;
+        dcdt[c] = dxi * (-ucip + ucim) + kappa * (dcdxp - dcdxm) * dxi + dyi * (-vcjp + vcjm) +
+                  kappa * (dcdyp - dcdym) * dyi + dzi * (-wckp + wckm) + kappa * (dcdzp - dcdzm) * dzi;
+      }
+}
+
+/* src/rk3.f90:13-106 (rk1), 108-222 (rk2), 224-329 (rk3) */
+void orc_rk(orc_state *s, int rkiter) {
+  double dt = s->dt;
+  double *du, *dv, *dw, *dc;
+  double f1, f2 = 0.0, f3 = 0.0;
+  if (rkiter == 1) { du = s->durk; dv = s->dvrk; dw = s->dwrk; dc = s->dcrk; f1 = dt * (32.0 / 60.0); }
+  else {
+    du = s->tmp1; dv = s->tmp2; dw = s->tmp3; dc = s->tmp4;
+    if (rkiter == 2) { f1 = dt * (8.0 / 60.0); f2 = dt * (25.0 / 60.0); f3 = dt * (17.0 / 60.0); }
+    else { f1 = dt * (20.0 / 60.0); f2 = dt * (45.0 / 60.0); f3 = dt * (25.0 / 60.0); }
+  }
+  momp(s, du, dv, dw, s->pnew);
+  FOR_INTERIOR(s) {
+    size_t q = I1(s, i, j, k);
+    s->dudt[q] = s->u[q] + f1 * du[q];
+    s->dvdt[q] = s->v[q] + f1 * dv[q];
+    s->dwdt[q] = s->w[q] + f1 * dw[q];
+    s->dcdt[q] = s->c[q];
+  }
+  momxad(s, du, s->u, s->v, s->w);
+  momyad(s, dv, s->u, s->v, s->w);
+  momzad(s, dw, s->u, s->v, s->w);
+  buoy_add(s, dw, s->c);
+  scalad(s, dc, s->u, s->v, s->w, s->c);
+  if (rkiter == 1) {
+    FOR_INTERIOR(s) {
+      size_t q = I1(s, i, j, k);
+      s->dudt[q] = s->dudt[q] + f1 * du[q];
+      s->dvdt[q] = s->dvdt[q] + f1 * dv[q];
+      s->dwdt[q] = s->dwdt[q] + f1 * dw[q];
+      s->dcdt[q] = s->dcdt[q] + f1 * dc[q];
+    }
+  } else {
+    FOR_INTERIOR(s) {
+      size_t q = I1(s, i, j, k);
+      s->dudt[q] = s->dudt[q] + (f2 * du[q] - f3 * s->durk[q]);
+      s->dvdt[q] = s->dvdt[q] + (f2 * dv[q] - f3 * s->dvrk[q]);
+      s->dwdt[q] = s->dwdt[q] + (f2 * dw[q] - f3 * s->dwrk[q]);
+      s->dcdt[q] = s->dcdt[q] + (f2 * dc[q] - f3 * s->dcrk[q]);
+      if (rkiter == 2) { s->durk[q] = du[q]; s->dvrk[q] = dv[q]; s->dwrk[q] = dw[q]; s->dcrk[q] = dc[q]; }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* IBM.  src/bound.f90:223-309 updthalos_ibm (icase 1 copy, icase 2 fold)  */
+static void updthalos_ibm(orc_state *s, double *var, int dir, int icase) {
+  int nx = s->nx, ny = s->ny, nz = s->nz;
+  if (dir == 1) {
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++)
+        for (int t = 0; t < 2; t++) {
+          if (icase == 1) var[I2(s, nx + 1 + t, j, k)] = var[I2(s, 1 + t, j, k)];
+          else var[I2(s, 1 + t, j, k)] = var[I2(s, 1 + t, j, k)] + var[I2(s, nx + 1 + t, j, k)];
+        }
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++)
+        for (int t = 0; t < 2; t++) {
+          if (icase == 1) var[I2(s, -1 + t, j, k)] = var[I2(s, nx - 1 + t, j, k)];
+          else var[I2(s, nx - 1 + t, j, k)] = var[I2(s, nx - 1 + t, j, k)] + var[I2(s, -1 + t, j, k)];
+        }
+  } else {
+    for (int k = -1; k <= nz + 2; k++)
+      for (int t = 0; t < 2; t++)
+        for (int i = -1; i <= nx + 2; i++) {
+          if (icase == 1) var[I2(s, i, ny + 1 + t, k)] = var[I2(s, i, 1 + t, k)];
+          else var[I2(s, i, 1 + t, k)] = var[I2(s, i, 1 + t, k)] + var[I2(s, i, ny + 1 + t, k)];
+        }
+    for (int k = -1; k <= nz + 2; k++)
+      for (int t = 0; t < 2; t++)
+        for (int i = -1; i <= nx + 2; i++) {
+          if (icase == 1) var[I2(s, i, -1 + t, k)] = var[I2(s, i, ny - 1 + t, k)];
+          else var[I2(s, i, ny - 1 + t, k)] = var[I2(s, i, ny - 1 + t, k)] + var[I2(s, i, -1 + t, k)];
+        }
+  }
+}
+
+typedef struct { int ilows, jlows, klows, ilow, jlow, klow; double phix[3], phiy[3], phiz[3], phixh[3], phiyh[3], phizh[3]; } stencil_t;
+
+/* src/interp_spread.f90:76-130 (identical in 291-327): stencil of one marker */
+static void marker_stencil(const orc_state *s, double xfp, double yfp, double zfp, stencil_t *t) {
+  double cx = xfp * s->dxi - (double)((int)floor((xfp - 0.5 * s->dx) / s->lx) * s->nx) - 0.0;
+  double cy = yfp * s->dyi - (double)((int)floor((yfp - 0.5 * s->dy) / s->ly) * s->ny) - 0.0;
+  double cz = zfp * s->dzi;
+  t->ilows = nint_f(cx) - 1;
+  t->jlows = nint_f(cy) - 1;
+  t->klows = nint_f(cz) - 1; if (t->klows < 1) t->klows = 1;
+  t->ilow = (int)floor(cx);
+  t->jlow = (int)floor(cy);
+  int kl = (int)floor(cz); if (kl > s->nz - 2) kl = s->nz - 2; if (kl < 1) kl = 1; t->klow = kl;
+  for (int q = 0; q < 3; q++) {
+    t->phix[q] = orc_kernel(((double)t->ilows + (double)q) - cx);
+    t->phiy[q] = orc_kernel(((double)t->jlows + (double)q) - cy);
+    t->phiz[q] = orc_kernel(((double)t->klows + (double)q) - cz);
+    t->phixh[q] = orc_kernel((((double)t->ilow + (double)q) - 0.5) - cx);
+    t->phiyh[q] = orc_kernel((((double)t->jlow + (double)q) - 0.5) - cy);
+    t->phizh[q] = orc_kernel((((double)t->klow + (double)q) - 0.5) - cz);
+  }
+}
+
+/* src/interp_spread.f90:12-228 eulr2lagr */
+void orc_eulr2lagr(orc_state *s) {
+  int nl = s->nl;
+  for (size_t q = 0; q < (size_t)s->np * nl; q++) { s->dudtl[q] = 0.0; s->dvdtl[q] = 0.0; s->dwdtl[q] = 0.0; }
+  updthalos_ibm(s, s->dudtf, 1, 1); updthalos_ibm(s, s->dvdtf, 1, 1); updthalos_ibm(s, s->dwdtf, 1, 1);
+  updthalos_ibm(s, s->dudtf, 2, 1); updthalos_ibm(s, s->dvdtf, 2, 1); updthalos_ibm(s, s->dwdtf, 2, 1);
+#pragma omp parallel for
+  for (int p = 0; p < s->np; p++)
+    for (int l = 0; l < s->nla[p]; l++) {
+      size_t m = (size_t)p * nl + l;
+      stencil_t t; marker_stencil(s, s->xfp[m], s->yfp[m], s->zfp[m], &t);
+      double au = s->dudtl[m], av = s->dvdtl[m], aw = s->dwdtl[m];
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+          for (int i = 0; i < 3; i++)
+            au = au + s->dudtf[I2(s, t.ilows + i, t.jlow + j, t.klow + k)] * t.phix[i] * t.phiyh[j] * t.phizh[k];
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+          for (int i = 0; i < 3; i++)
+            av = av + s->dvdtf[I2(s, t.ilow + i, t.jlows + j, t.klow + k)] * t.phixh[i] * t.phiy[j] * t.phizh[k];
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+          for (int i = 0; i < 3; i++)
+            aw = aw + s->dwdtf[I2(s, t.ilow + i, t.jlow + j, t.klows + k)] * t.phixh[i] * t.phiyh[j] * t.phiz[k];
+      s->dudtl[m] = au; s->dvdtl[m] = av; s->dwdtl[m] = aw;
+    }
+}
+
+/* src/forcing.f90:9-187 complagrforces (ibmiter 0) and 189-391 updtlagrforces */
+void orc_lagrforces(orc_state *s, int ibmiter) {
+  double dti = 1.0 / s->dt;
+  int nl = s->nl;
+  for (int p = 0; p < s->np; p++) {
+    double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
+    for (int l = 0; l < s->nla[p]; l++) {
+      size_t m = (size_t)p * nl + l;
+      if (ibmiter == 0) {
+        s->fxl[m] = (s->ul[m] - s->dudtl[m]) * dti;
+        s->fyl[m] = (s->vl[m] - s->dvdtl[m]) * dti;
+        s->fzl[m] = (s->wl[m] - s->dwdtl[m]) * dti;
+      } else {
+        s->fxl[m] = s->fxl[m] + (s->ul[m] - s->dudtl[m]) * dti;
+        s->fyl[m] = s->fyl[m] + (s->vl[m] - s->dvdtl[m]) * dti;
+        s->fzl[m] = s->fzl[m] + (s->wl[m] - s->dwdtl[m]) * dti;
+      }
+      fx = fx + s->fxl[m]; fy = fy + s->fyl[m]; fz = fz + s->fzl[m];
+      tx = tx + (s->yfp[m] - s->py[p]) * s->fzl[m] - (s->zfp[m] - s->pz[p]) * s->fyl[m];
+      ty = ty + (s->zfp[m] - s->pz[p]) * s->fxl[m] - (s->xfp[m] - s->px[p]) * s->fzl[m];
+      tz = tz + (s->xfp[m] - s->px[p]) * s->fyl[m] - (s->yfp[m] - s->py[p]) * s->fxl[m];
+    }
+    s->fxltot[p] = fx * s->dVlagr; s->fyltot[p] = fy * s->dVlagr; s->fzltot[p] = fz * s->dVlagr;
+    s->torqx[p] = tx * s->dVlagr; s->torqy[p] = ty * s->dVlagr; s->torqz[p] = tz * s->dVlagr;
+    s->torqtheta[p] = (s->torqy[p] * cos(s->pphi[p])) - (s->torqx[p] * sin(s->pphi[p]));
+  }
+}
+
+/* src/interp_spread.f90:230-474 lagr2eulr */
+void orc_lagr2eulr(orc_state *s) {
+  double dVlagrdVeuli = s->dVlagr / s->dVeul;
+  memset(s->dudtf, 0, s->n2 * sizeof(double));
+  memset(s->dvdtf, 0, s->n2 * sizeof(double));
+  memset(s->dwdtf, 0, s->n2 * sizeof(double));
+  double c0 = dVlagrdVeuli * s->dt;
+  double fyt = 0.0;
+  int nl = s->nl;
+  for (int p = 0; p < s->np; p++)
+    for (int l = 0; l < s->nla[p]; l++) {
+      size_t m = (size_t)p * nl + l;
+      stencil_t t; marker_stencil(s, s->xfp[m], s->yfp[m], s->zfp[m], &t);
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+          for (int i = 0; i < 3; i++) {
+            size_t q = I2(s, t.ilows + i, t.jlow + j, t.klow + k);
+            s->dudtf[q] = s->dudtf[q] + s->fxl[m] * c0 * t.phix[i] * t.phiyh[j] * t.phizh[k];
+          }
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+          for (int i = 0; i < 3; i++) {
+            size_t q = I2(s, t.ilow + i, t.jlows + j, t.klow + k);
+            s->dvdtf[q] = s->dvdtf[q] + s->fyl[m] * c0 * t.phixh[i] * t.phiy[j] * t.phizh[k];
+            fyt = fyt + s->fyl[m] * c0 * t.phixh[i] * t.phiy[j] * t.phizh[k];
+          }
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+          for (int i = 0; i < 3; i++) {
+            size_t q = I2(s, t.ilow + i, t.jlow + j, t.klows + k);
+            s->dwdtf[q] = s->dwdtf[q] + s->fzl[m] * c0 * t.phixh[i] * t.phiyh[j] * t.phiz[k];
+          }
+    }
+  updthalos_ibm(s, s->dudtf, 1, 2); updthalos_ibm(s, s->dvdtf, 1, 2); updthalos_ibm(s, s->dwdtf, 1, 2);
+  updthalos_ibm(s, s->dudtf, 2, 2); updthalos_ibm(s, s->dvdtf, 2, 2); updthalos_ibm(s, s->dwdtf, 2, 2);
+  s->forceytot = fyt / s->dt / (1.0 * s->nx * s->ny * s->nz);
+  FOR_INTERIOR(s) {
+    size_t q = I1(s, i, j, k), f = I2(s, i, j, k);
+    s->dudtf[f] = s->dudtf[f] + s->dudtold[q];
+    s->dvdtf[f] = s->dvdtf[f] + s->dvdtold[q];
+    s->dwdtf[f] = s->dwdtf[f] + s->dwdtold[q];
+  }
+}
+
+/* src/main.f90:286-338 -- the IBM block of one substep */
+void orc_ibm(orc_state *s) {
+  memcpy(s->dudtold, s->dudt, s->n1 * sizeof(double));
+  memcpy(s->dvdtold, s->dvdt, s->n1 * sizeof(double));
+  memcpy(s->dwdtold, s->dwdt, s->n1 * sizeof(double));
+  FOR_INTERIOR(s) {
+    size_t q = I1(s, i, j, k), f = I2(s, i, j, k);
+    s->dudtf[f] = s->dudt[q]; s->dvdtf[f] = s->dvdt[q]; s->dwdtf[f] = s->dwdt[q];
+  }
+  for (int ibmiter = 0; ibmiter <= 2; ibmiter++) {
+    orc_eulr2lagr(s);
+    orc_lagrforces(s, ibmiter);
+    orc_lagr2eulr(s);
+  }
+  FOR_INTERIOR(s) {
+    size_t q = I1(s, i, j, k), f = I2(s, i, j, k);
+    s->dudt[q] = s->dudtf[f]; s->dvdt[q] = s->dvdtf[f]; s->dwdt[q] = s->dwdtf[f];
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* src/fillps.f90:8-53 */
+void orc_fillps(orc_state *s, int rkiter) {
+  double dti = 1.0 / (s->rkcoeffab[rkiter - 1] * s->dt);
+  double dtidxi = dti * s->dxi, dtidyi = dti * s->dyi, dtidzi = dti * s->dzi;
+#pragma omp parallel for
+  for (int k = 1; k <= s->nz; k++)
+    for (int j = 1; j <= s->ny; j++)
+      for (int i = 1; i <= s->nx; i++) {
+        size_t q = I1(s, i, j, k);
+        s->p[q] = (s->dwdt[q] - s->dwdt[I1(s, i, j, k - 1)]) * dtidzi + (s->dvdt[q] - s->dvdt[I1(s, i, j - 1, k)]) * dtidyi +
+                  (s->dudt[q] - s->dudt[I1(s, i - 1, j, k)]) * dtidxi;
+      }
+}
+
+/* src/initsolver.f90:10-67 */
+void orc_initsolver(orc_state *s) {
+  int nx = s->nx, ny = s->ny, nz = s->nz;
+  double dzi = s->dzi, dxi = s->dxi, dyi = s->dyi, pi = s->pi;
+  for (int k = 1; k <= nz; k++) { s->a[k] = dzi * dzi; s->cc[k] = dzi * dzi; s->b[k] = -(s->a[k] + s->cc[k]); }
+  s->b[1] = s->b[1] + s->a[1];
+  s->b[nz] = s->b[nz] + s->cc[nz];
+  s->a[1] = 0.0;
+  s->cc[nz] = 0.0;
+  double *xrt = zalloc(nx + 1), *yrt = zalloc(ny + 1);
+  for (int i = 3; i <= nx; i += 2) { xrt[i - 1] = -4.0 * dxi * dxi * sq(sin((double)(i - 1) * pi / (2.0 * nx))); xrt[i] = xrt[i - 1]; }
+  xrt[1] = 0.0; xrt[nx] = -4.0 * dxi * dxi;
+  for (int j = 3; j <= ny; j += 2) { yrt[j - 1] = -4.0 * dyi * dyi * sq(sin((double)(j - 1) * pi / (2.0 * ny))); yrt[j] = yrt[j - 1]; }
+  yrt[1] = 0.0; yrt[ny] = -4.0 * dyi * dyi;
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) s->xyrt[(size_t)(i - 1) + (size_t)nx * (j - 1)] = xrt[i] + yrt[j];
+  free(xrt); free(yrt);
+}
+
+/* src/solver.f90:43-172 solver2d on pp => p(1:imax,1:jmax,1:kmax) */
+void orc_solver2d(orc_state *s) {
+  int nx = s->nx, ny = s->ny, nz = s->nz;
+  double *p = s->p;
+#pragma omp parallel
+  {
+    orc_fft fx, fy; fft_plan(&fx, nx); fft_plan(&fy, ny);
+#pragma omp for
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++) orc_rfftf_plan(&fx, &p[I1(s, 1, j, k)], 1);
+#pragma omp for
+    for (int k = 1; k <= nz; k++)
+      for (int i = 1; i <= nx; i++) orc_rfftf_plan(&fy, &p[I1(s, i, 1, k)], s->sx);
+#pragma omp for
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        size_t m = (size_t)(i - 1) + (size_t)nx * (j - 1);
+        double *d = &s->dtri[m * nz];
+        double xy = s->xyrt[m], z, bb;
+        z = 1.0 / (s->b[1] + xy);
+        d[0] = s->cc[1] * z;
+        p[I1(s, i, j, 1)] = p[I1(s, i, j, 1)] * z;
+        for (int k = 2; k <= nz - 1; k++) {
+          bb = s->b[k] + xy;
+          z = 1.0 / (bb - s->a[k] * d[k - 2]);
+          d[k - 1] = s->cc[k] * z;
+          p[I1(s, i, j, k)] = (p[I1(s, i, j, k)] - s->a[k] * p[I1(s, i, j, k - 1)]) * z;
+        }
+        bb = s->b[nz] + xy;
+        z = bb - s->a[nz] * d[nz - 2];
+        if (z != 0.0) p[I1(s, i, j, nz)] = (p[I1(s, i, j, nz)] - s->a[nz] * p[I1(s, i, j, nz - 1)]) / z;
+        else p[I1(s, i, j, nz)] = 0.0;
+        for (int k = nz - 1; k >= 1; k--) p[I1(s, i, j, k)] = p[I1(s, i, j, k)] - d[k - 1] * p[I1(s, i, j, k + 1)];
+      }
+#pragma omp for
+    for (int k = 1; k <= nz; k++)
+      for (int i = 1; i <= nx; i++) orc_rfftb_plan(&fy, &p[I1(s, i, 1, k)], s->sx);
+#pragma omp for
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++) orc_rfftb_plan(&fx, &p[I1(s, 1, j, k)], 1);
+    fft_free(&fx); fft_free(&fy);
+  }
+}
+
+/* src/correc.f90:11-42 */
+void orc_correc(orc_state *s, int rkiter) {
+  double factor = s->rkcoeffab[rkiter - 1] * s->dt;
+  double fi = factor * s->dxi, fj = factor * s->dyi, fk = factor * s->dzi;
+#pragma omp parallel for
+  for (int k = 1; k <= s->nz; k++)
+    for (int j = 1; j <= s->ny; j++)
+      for (int i = 1; i <= s->nx; i++) {
+        size_t q = I1(s, i, j, k);
+        s->u[q] = s->dudt[q] - fi * (s->p[I1(s, i + 1, j, k)] - s->p[q]);
+        s->v[q] = s->dvdt[q] - fj * (s->p[I1(s, i, j + 1, k)] - s->p[q]);
+        s->w[q] = s->dwdt[q] - fk * (s->p[I1(s, i, j, k + 1)] - s->p[q]);
+      }
+}
+
+/* src/main.f90:368-376 */
+void orc_pnew_update(orc_state *s) {
+  bounduvw3(s, s->u, s->v, s->w);
+  for (size_t q = 0; q < s->n1; q++) s->pnew[q] = s->pnew[q] + s->p[q];
+  boundp1(s, s->pnew);
+  memcpy(s->c, s->dcdt, s->n1 * sizeof(double));
+  orc_boundscal(s);
+}
+
+/* ---------------------------------------------------------------------- */
+/* src/chkdiv.f90:7-61 */
+void orc_chkdiv(orc_state *s, double *tot, double *mx) {
+  double divtot = 0.0, divmax = 0.0;
+  FOR_INTERIOR(s) {
+    size_t q = I1(s, i, j, k);
+    double div = (s->w[q] - s->w[I1(s, i, j, k - 1)]) * s->dzi + (s->v[q] - s->v[I1(s, i, j - 1, k)]) * s->dyi +
+                 (s->u[q] - s->u[I1(s, i - 1, j, k)]) * s->dxi;
+    divtot = divtot + div;
+    div = fabs(div);
+    if (div > divmax) divmax = div;
+  }
+  *tot = divtot; *mx = divmax;
+}
+
+/* src/chkdt.f90:6-271 */
+void orc_chkdt(orc_state *s, double *dtmax) {
+  double dxi = s->dxi, dyi = s->dyi, dzi = s->dzi;
+  double dmin[3] = {899999.0, 899999.0, 899999.0};
+  const double *U = s->u, *V = s->v, *W = s->w;
+  for (int comp = 0; comp < 3; comp++) {
+    double dm = 899999.0;
+    FOR_INTERIOR(s) {
+      double dxi2 = dxi * dxi + dyi * dyi + dzi * dzi;
+      double dt1 = 1.65 / (4.0 * s->visc * dxi2);
+      double u2, v2, w2;
+      if (comp == 0) {
+        u2 = sq(U[I1(s, i, j, k)]);
+        v2 = sq(0.25 * (V[I1(s, i, j, k)] + V[I1(s, i, j - 1, k)] + V[I1(s, i + 1, j, k)] + V[I1(s, i + 1, j - 1, k)]));
+        w2 = sq(0.25 * (W[I1(s, i, j, k)] + W[I1(s, i, j, k - 1)] + W[I1(s, i + 1, j, k)] + W[I1(s, i + 1, j, k - 1)]));
+      } else if (comp == 1) {
+        u2 = sq(0.25 * (U[I1(s, i, j, k)] + U[I1(s, i, j + 1, k)] + U[I1(s, i - 1, j + 1, k)] + U[I1(s, i - 1, j, k)]));
+        v2 = sq(V[I1(s, i, j, k)]);
+        w2 = sq(0.25 * (W[I1(s, i, j, k)] + W[I1(s, i, j + 1, k)] + W[I1(s, i, j + 1, k - 1)] + W[I1(s, i, j, k - 1)]));
+      } else {
+        u2 = sq(0.25 * (U[I1(s, i, j, k)] + U[I1(s, i - 1, j, k)] + U[I1(s, i - 1, j, k + 1)] + U[I1(s, i, j, k + 1)]));
+        v2 = sq(0.25 * (V[I1(s, i, j, k)] + V[I1(s, i, j - 1, k)] + V[I1(s, i, j - 1, k + 1)] + V[I1(s, i, j, k + 1)]));
+        w2 = sq(W[I1(s, i, j, k)]);
+      }
+      double velo = sqrt(u2 + v2 + w2), dt2;
+      if (fabs(1.0 * velo) > 1.e-12) dt2 = sqrt(3.0) / (sqrt(u2) * dxi + sqrt(v2) * dyi + sqrt(w2) * dzi);
+      else dt2 = 99999.0;
+      double m = dm;
+      if (dt1 < m) m = dt1;
+      if (dt2 < m) m = dt2;
+      dm = m;
+    }
+    dmin[comp] = dm;
+  }
+  double d = dmin[0];
+  if (dmin[1] < d) d = dmin[1];
+  if (dmin[2] < d) d = dmin[2];
+  *dtmax = d;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Particles.  src/initparticles.f90:200-241 and
+ * src/intgr_nwtn_eulr.f90:1076-1111: neighbour flags at one rank          */
+static void set_nb_flags(orc_state *s, int p) {
+  double x = s->px[p], y = s->py[p], ro = s->radius + s->offset;
+  double left = 0.0 * s->lx / 1.0, right = 1.0 * s->lx / 1.0, front = 0.0 * s->ly / 1.0, back = 1.0 * s->ly / 1.0;
+  int *f = &s->nbf[p * NB];
+  for (int q = 0; q < NB; q++) f[q] = 0;
+  if (x > (right - ro)) f[1] = 1;
+  if (fabs(sqrt(sq(right - x) + sq(front - y))) < ro) f[2] = 1;
+  if (y < (front + ro)) f[3] = 1;
+  if (fabs(sqrt(sq(left - x) + sq(front - y))) < ro) f[4] = 1;
+  if (x < (left + ro)) f[5] = 1;
+  if (fabs(sqrt(sq(left - x) + sq(back - y))) < ro) f[6] = 1;
+  if (y > (back - ro)) f[7] = 1;
+  if (fabs(sqrt(sq(right - x) + sq(back - y))) < ro) f[8] = 1;
+}
+void orc_update_nb_flags(orc_state *s) { for (int p = 0; p < s->np; p++) set_nb_flags(s, p); }
+
+/* markers: src/coordsfp.f90:47-82 (init) == src/intgr_nwtn_eulr.f90:1438-1485 (refresh) */
+static void refresh_markers(orc_state *s) {
+  int nl = s->nl;
+  for (int p = 0; p < s->np; p++) {
+    int ll = 0;
+    for (int l = 0; l < nl; l++) {
+      double xl = s->px[p] + s->radfp * sin(s->ptheta[p] * 0.0 + s->thetarc[l]) * cos(s->pphi[p] * 0.0 + s->phirc[l]);
+      double yl = s->py[p] + s->radfp * sin(s->ptheta[p] * 0.0 + s->thetarc[l]) * sin(s->pphi[p] * 0.0 + s->phirc[l]);
+      double zl = s->pz[p] + s->radfp * cos(s->ptheta[p] * 0.0 + s->thetarc[l]);
+      double isx = 0.0, isy = 0.0;
+      if (xl < 0.0 + 0.5 * s->dx) isx = 1.0;
+      if (xl >= s->lx + 0.5 * s->dx) isx = -1.0;
+      if (yl < 0.0 + 0.5 * s->dy) isy = 1.0;
+      if (yl >= s->ly + 0.5 * s->dy) isy = -1.0;
+      int isout = 0;
+      double cx = (xl + isx * s->lx - 0.0) * s->dxi;
+      if (nint_f(cx) < 1 || nint_f(cx) > s->nx) isout = 1;
+      double cy = (yl + isy * s->ly - 0.0) * s->dyi;
+      if (nint_f(cy) < 1 || nint_f(cy) > s->ny) isout = 1;
+      if (!isout) {
+        size_t m = (size_t)p * nl + ll;
+        s->xfp[m] = xl; s->yfp[m] = yl; s->zfp[m] = zl;
+        s->ul[m] = s->pu[p] + s->pomy[p] * (zl - s->pz[p]) - s->pomz[p] * (yl - s->py[p]);
+        s->vl[m] = s->pv[p] + s->pomz[p] * (xl - s->px[p]) - s->pomx[p] * (zl - s->pz[p]);
+        s->wl[m] = s->pw[p] + s->pomx[p] * (yl - s->py[p]) - s->pomy[p] * (xl - s->px[p]);
+        ll++;
+      }
+    }
+    s->nla[p] = ll;
+  }
+}
+
+/* src/coordsfp.f90:10-106 with the marker angles already read from
+ * lagrangianforcepoints2 (theta, phi, radfp columns)                       */
+void orc_coordsfp(orc_state *s, const double *theta, const double *phi, double radfp) {
+  for (int l = 0; l < s->nl; l++) { s->thetarc[l] = theta[l]; s->phirc[l] = phi[l]; }
+  s->radfp = radfp;
+  refresh_markers(s);
+  s->dVlagr = ((4.0 / 3.0) * s->pi * pow(radfp + 0.5 / s->dxi, 3.0) - (4.0 / 3.0) * s->pi * pow(radfp - 0.5 / s->dxi, 3.0)) / (1.0 * s->nl);
+  s->dVeul = 1.0 / (s->dxi * s->dyi * s->dzi);
+}
+
+/* image centre for neighbour slot nbrecv: src/intgr_over_sphere.f90:83-109 */
+static void image_shift(const orc_state *s, int nbrecv, double *x, double *y) {
+  switch (nbrecv) {
+    case 1: *x = *x + s->lx; break;
+    case 2: *x = *x + s->lx; *y = *y - s->ly; break;
+    case 3: *y = *y - s->ly; break;
+    case 4: *x = *x - s->lx; *y = *y - s->ly; break;
+    case 5: *x = *x - s->lx; break;
+    case 6: *x = *x - s->lx; *y = *y + s->ly; break;
+    case 7: *y = *y + s->ly; break;
+    case 8: *x = *x + s->lx; *y = *y + s->ly; break;
+  }
+}
+
+/* src/intgr_over_sphere.f90:12-625 -- type 1 volume, 2 int u dV, 3 int r x u dV */
+void orc_intgr_over_sphere(orc_state *s, int type) {
+  double dx = 1.0 / s->dxi, dy = 1.0 / s->dyi, dz = 1.0 / s->dzi;
+  double radin = s->radius - 2.0 * dx, radin2 = sq(radin), R = s->radius, dVeul = s->dVeul;
+#pragma omp parallel for
+  for (int p = 0; p < s->np; p++) {
+    double acc[NB][3];
+    memset(acc, 0, sizeof(acc));
+    for (int nb = 0; nb < NB; nb++) {
+      if (!(nb == 0 || s->nbf[p * NB + nb] == 1)) continue;
+      int nbrecv = nb;
+      if (nb > 0) { nbrecv = nb + 4; if (nbrecv > 8) nbrecv -= 8; }
+      double X = s->px[p], Y = s->py[p], Z = s->pz[p];
+      image_shift(s, nbrecv, &X, &Y);
+      double cxc = X - 0.0, cyc = Y - 0.0, czc = Z;
+      int ilow = nint_f((cxc - R) * s->dxi - 2.0), ihigh = nint_f((cxc + R) * s->dxi + 2.0);
+      int jlow = nint_f((cyc - R) * s->dyi - 2.0), jhigh = nint_f((cyc + R) * s->dyi + 2.0);
+      int klow = nint_f((czc - R) * s->dzi - 2.0), khigh = nint_f((czc + R) * s->dzi + 2.0);
+      if (ilow < 1) ilow = 1; if (jlow < 1) jlow = 1; if (klow < 1) klow = 1;
+      if (ihigh > s->nx) ihigh = s->nx; if (jhigh > s->ny) jhigh = s->ny; if (khigh > s->nz) khigh = s->nz;
+      double *a = acc[nbrecv];
+      for (int comp = 0; comp < 3; comp++) {
+        const double *F = comp == 0 ? s->u : (comp == 1 ? s->v : s->w);
+        for (int k = klow; k <= khigh; k++) {
+          double zmin, zplus;
+          if (comp == 2) { zmin = ((double)k - 0.5) * dz; zplus = ((double)k + 0.5) * dz; }
+          else { zmin = (double)(k - 1) * dz; zplus = (double)k * dz; }
+          double dzp2 = sq(zplus - Z);
+          for (int j = jlow; j <= jhigh; j++) {
+            double ymin, yplus;
+            if (comp == 1) { ymin = 0.0 + ((double)j - 0.5) * dy; yplus = 0.0 + ((double)j + 0.5) * dy; }
+            else { ymin = 0.0 + (double)(j - 1) * dy; yplus = 0.0 + (double)j * dy; }
+            double dyp2 = sq(yplus - Y);
+            for (int i = ilow; i <= ihigh; i++) {
+              double xmin, xplus;
+              if (comp == 0) { xmin = 0.0 + ((double)i - 0.5) * dx; xplus = 0.0 + ((double)i + 0.5) * dx; }
+              else { xmin = 0.0 + (double)(i - 1) * dx; xplus = 0.0 + (double)i * dx; }
+              double dxp2 = sq(xplus - X);
+              double dist2 = dxp2 + dyp2 + dzp2;
+              double fval = F[I1(s, i, j, k)];
+              double wfrac; int full;
+              if (dist2 < radin2) { full = 1; wfrac = 1.0; }
+              else {
+                full = 0;
+                double dxm2 = sq(xmin - X), dym2 = sq(ymin - Y), dzm2 = sq(zmin - Z);
+                double sg[8];
+                sg[0] = R - sqrt(dxm2 + dym2 + dzm2); sg[1] = R - sqrt(dxm2 + dyp2 + dzm2);
+                sg[2] = R - sqrt(dxp2 + dyp2 + dzm2); sg[3] = R - sqrt(dxp2 + dym2 + dzm2);
+                sg[4] = R - sqrt(dxm2 + dym2 + dzp2); sg[5] = R - sqrt(dxm2 + dyp2 + dzp2);
+                sg[6] = R - sqrt(dxp2 + dyp2 + dzp2); sg[7] = R - sqrt(dxp2 + dym2 + dzp2);
+                double s1 = 0.0, s2 = 0.0;
+                for (int cp = 0; cp < 8; cp++) { if (sg[cp] > 0.0) s1 = s1 + sg[cp]; s2 = s2 + fabs(sg[cp]); }
+                wfrac = s1 / s2;
+              }
+              if (type == 1) {
+                a[comp] = a[comp] + (full ? dVeul : wfrac * dVeul);
+              } else if (type == 2) {
+                a[comp] = a[comp] + (full ? fval * dVeul : fval * wfrac * dVeul);
+              } else {
+                double rx = 0.0 + ((double)i - 0.5) * dx - X;
+                double ry = 0.0 + ((double)j - 0.5) * dy - Y;
+                double rz = ((double)k - 0.5) * dz - Z;
+                if (comp == 0) {
+                  a[1] = a[1] + (full ? rz * fval * dVeul : rz * fval * wfrac * dVeul);
+                  a[2] = a[2] - (full ? ry * fval * dVeul : ry * fval * wfrac * dVeul);
+                } else if (comp == 1) {
+                  a[0] = a[0] - (full ? rz * fval * dVeul : rz * fval * wfrac * dVeul);
+                  a[2] = a[2] + (full ? rx * fval * dVeul : rx * fval * wfrac * dVeul);
+                } else {
+                  a[0] = a[0] + (full ? ry * fval * dVeul : ry * fval * wfrac * dVeul);
+                  a[1] = a[1] - (full ? rx * fval * dVeul : rx * fval * wfrac * dVeul);
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int nb = 0; nb < NB; nb++) { t[0] = t[0] + acc[nb][0]; t[1] = t[1] + acc[nb][1]; t[2] = t[2] + acc[nb][2]; }
+    if (type == 3) { s->intomx[p] = t[0]; s->intomy[p] = t[1]; s->intomz[p] = t[2]; }
+    else if (type == 2) { s->intu[p] = t[0]; s->intv[p] = t[1]; s->intw[p] = t[2]; }
+    else { s->intu[p] = t[0]; s->intv[p] = t[1]; s->intw[p] = t[2]; }
+  }
+}
+
+/* src/collisions.f90:222-438 lubrication: only the squeeze force f1 reaches
+ * the state (lines 430-432); wall partner velocities are zero.            */
+static void lubrication(orc_state *s, int p, int wall, double nx, double ny, double nz, double eps,
+                        double un, double vn, double wn, double *fx, double *fy, double *fz) {
+  double u1a = s->pu[p] * nx + s->pv[p] * ny + s->pw[p] * nz;
+  double u1b = un * nx + vn * ny + wn * nz;
+  double f1;
+  if (wall) {
+    double a11;
+    if (eps < 0.001) a11 = s->a11_sat_pw;
+    else a11 = -(1.0 / eps) + 1.0 / 5.0 * log(eps) + 1.0 / 21.0 * eps * log(eps) - .9713;
+    f1 = (a11 * u1a);
+    f1 = s->coeff_f * (f1 - (s->a11_ini_pw * u1a));
+  } else {
+    double a11a, a11b;
+    if (eps < 0.001) { a11a = s->a11a_sat_pp; a11b = s->a11b_sat_pp; }
+    else {
+      double a11 = -(1.0 / 4.0 * (1.0 / eps)) + 9.0 / 40.0 * log(eps) + 3.0 / 112.0 * eps * log(eps);
+      a11a = a11 - .995; a11b = -a11 + .350;
+    }
+    f1 = (a11a * u1a + a11b * u1b);
+    f1 = s->coeff_f * (f1 - (s->a11a_ini_pp * u1a + s->a11b_ini_pp * u1b));
+  }
+  *fx = *fx + f1 * nx; *fy = *fy + f1 * ny; *fz = *fz + f1 * nz;
+}
+
+/* src/collisions.f90:10-220: only the normal spring-dashpot force reaches the
+ * state (lines 199-201); the tangential history is computed and discarded. */
+static void collision(orc_state *s, int p, int wall, double dist, double deltax, double deltay, double deltaz,
+                      double un, double vn, double wn, double omxn, double omyn, double omzn,
+                      double *fx, double *fy, double *fz) {
+  double nx = deltax / dist, ny = deltay / dist, nz = deltaz / dist;
+  double R = s->radius, deltan, kn, etan;
+  if (!wall) { deltan = 2.0 * R - dist; kn = s->kn_ss; etan = s->etan_ss; }
+  else { deltan = R - dist; kn = s->kn_sw; etan = s->etan_sw; }
+  double dun = (s->pu[p] + (s->pomy[p] * R * nz) - (s->pomz[p] * R * ny)) - (un + (omyn * R * (-nz)) - (omzn * R * (-ny)));
+  double dvn = (s->pv[p] - (s->pomx[p] * R * nz) + (s->pomz[p] * R * nx)) - (vn - (omxn * R * (-nz)) + (omzn * R * (-nx)));
+  double dwn = (s->pw[p] + (s->pomx[p] * R * ny) - (s->pomy[p] * R * nx)) - (wn + (omxn * R * (-ny)) - (omyn * R * (-nx)));
+  double vabs = dun * nx + dvn * ny + dwn * nz;
+  double fxn = -(kn * (deltan * nx)) - etan * (vabs * nx);
+  double fyn = -(kn * (deltan * ny)) - etan * (vabs * ny);
+  double fzn = -(kn * (deltan * nz)) - etan * (vabs * nz);
+  *fx = *fx + fxn; *fy = *fy + fyn; *fz = *fz + fzn;
+}
+
+/* src/intgr_nwtn_eulr.f90:12-714 (one rank), marker refresh 1438-1485 */
+void orc_intgr_nwtn_eulr(orc_state *s, int rkiter) {
+  int np = s->np;
+  double dt = s->dt, rkab = s->rkcoeffab[rkiter - 1], R = s->radius, lx = s->lx, ly = s->ly;
+  /* op <- ap (77-118) */
+  double *o = (double *)malloc(sizeof(double) * 24 * (size_t)(np > 0 ? np : 1));
+  double *ox = o, *oy = o + np, *oz = o + 2 * np, *oth = o + 3 * np, *oph = o + 4 * np, *ou = o + 5 * np,
+         *ov = o + 6 * np, *ow = o + 7 * np, *oomx = o + 8 * np, *oomy = o + 9 * np, *oomz = o + 10 * np,
+         *oomt = o + 11 * np, *ointu = o + 12 * np, *ointv = o + 13 * np, *ointw = o + 14 * np,
+         *ointomx = o + 15 * np, *ointomy = o + 16 * np, *ointomz = o + 17 * np, *ocfx = o + 18 * np,
+         *ocfy = o + 19 * np, *ocfz = o + 20 * np, *octx = o + 21 * np, *octy = o + 22 * np, *octz = o + 23 * np;
+  for (int p = 0; p < np; p++) {
+    ox[p] = s->px[p]; oy[p] = s->py[p]; oz[p] = s->pz[p]; oth[p] = s->ptheta[p]; oph[p] = s->pphi[p];
+    ou[p] = s->pu[p]; ov[p] = s->pv[p]; ow[p] = s->pw[p]; oomx[p] = s->pomx[p]; oomy[p] = s->pomy[p];
+    oomz[p] = s->pomz[p]; oomt[p] = s->pomtheta[p]; ointu[p] = s->intu[p]; ointv[p] = s->intv[p];
+    ointw[p] = s->intw[p]; ointomx[p] = s->intomx[p]; ointomy[p] = s->intomy[p]; ointomz[p] = s->intomz[p];
+    ocfx[p] = s->colfx[p]; ocfy[p] = s->colfy[p]; ocfz[p] = s->colfz[p];
+    octx[p] = s->coltx[p]; octy[p] = s->colty[p]; octz[p] = s->coltz[p];
+  }
+  orc_intgr_over_sphere(s, 2);
+  orc_intgr_over_sphere(s, 3);
+  double *anb = (double *)malloc(sizeof(double) * 9 * NB * (size_t)(np > 0 ? np : 1));
+  for (int iter = 0; iter <= 1; iter++) {
+    /* neighbour tables (159-322): nb 0 = own, 1..8 = periodic images */
+    for (int nb = 0; nb < NB; nb++)
+      for (int q = 0; q < np; q++) {
+        double *e = &anb[((size_t)nb * np + q) * 9];
+        double x = s->px[q], y = s->py[q];
+        switch (nb) {
+          case 1: if (x < lx) x = x + lx; break;
+          case 2: if (x < lx) x = x + lx; if (y > 0.0) y = y - ly; break;
+          case 3: if (y > 0.0) y = y - ly; break;
+          case 4: if (x > 0.0) x = x - lx; if (y > 0.0) y = y - ly; break;
+          case 5: if (x > 0.0) x = x - lx; break;
+          case 6: if (x > 0.0) x = x - lx; if (y < ly) y = y + ly; break;
+          case 7: if (y < ly) y = y + ly; break;
+          case 8: if (x < lx) x = x + lx; if (y < ly) y = y + ly; break;
+        }
+        e[0] = x; e[1] = y; e[2] = s->pz[q]; e[3] = s->pu[q]; e[4] = s->pv[q]; e[5] = s->pw[q];
+        e[6] = s->pomx[q]; e[7] = s->pomy[q]; e[8] = s->pomz[q];
+      }
+    /* collision model (338-606) */
+    for (int p = 0; p < np; p++) {
+      double fx = 0.0, fy = 0.0, fz = 0.0;
+      for (int nb = 0; nb < NB; nb++)
+        for (int q = 0; q < np; q++) {
+          if (q == p) continue;
+          const double *e = &anb[((size_t)nb * np + q) * 9];
+          double deltaz = e[2] - s->pz[p];
+          double ax[3] = {e[0] - s->px[p], e[0] - lx - s->px[p], e[0] + lx - s->px[p]};
+          int ix = 0; if (fabs(ax[1]) < fabs(ax[ix])) ix = 1; if (fabs(ax[2]) < fabs(ax[ix])) ix = 2;
+          double deltax = ax[ix];
+          double ay[3] = {e[1] - s->py[p], e[1] - ly - s->py[p], e[1] + ly - s->py[p]};
+          int iy = 0; if (fabs(ay[1]) < fabs(ay[iy])) iy = 1; if (fabs(ay[2]) < fabs(ay[iy])) iy = 2;
+          double deltay = ay[iy];
+          double dist = sqrt(sq(deltax) + sq(deltay) + sq(deltaz));
+          double nx = deltax / dist, ny = deltay / dist, nz = deltaz / dist;
+          double eps = (dist - 2.0 * R) / R;
+          if ((eps < 0.025) && (eps > 0.0)) lubrication(s, p, 0, nx, ny, nz, eps, e[3], e[4], e[5], &fx, &fy, &fz);
+          if (dist < (2.0 * R)) collision(s, p, 0, dist, deltax, deltay, deltaz, e[3], e[4], e[5], e[6], e[7], e[8], &fx, &fy, &fz);
+        }
+      for (int wq = 0; wq < 2; wq++) {
+        double deltax = 0.0, deltay = 0.0, deltaz = (double)wq * s->lz - s->pz[p];
+        double dist = sqrt(sq(deltax) + sq(deltay) + sq(deltaz));
+        double deltan = R - dist;
+        double nx = deltax / dist, ny = deltay / dist, nz = deltaz / dist;
+        double eps = -deltan / R;
+        if ((eps < 0.075) && (eps > 0.0)) lubrication(s, p, 1, nx, ny, nz, eps, 0.0, 0.0, 0.0, &fx, &fy, &fz);
+        if (dist < R) collision(s, p, 1, dist, deltax, deltay, deltaz, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, &fx, &fy, &fz);
+      }
+      s->colfx[p] = fx; s->colfy[p] = fy; s->colfz[p] = fz;
+      s->coltx[p] = 0.0; s->colty[p] = 0.0; s->coltz[p] = 0.0;
+    }
+    /* new particle positions (611-667) */
+    for (int p = 0; p < np; p++) {
+      double vr = s->volp * s->ratiorho, rr = s->ratiorho;
+      s->pu[p] = ou[p] + (1.0 - 0.0) * ((-1.0) * dt * (s->fxltot[p]) / vr + (s->intu[p] - ointu[p]) / vr) +
+                 rkab * dt * s->gaccx * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (s->colfx[p] + ocfx[p]) / vr;
+      s->px[p] = ox[p] + rkab * 0.5 * dt * (s->pu[p] + ou[p]);
+      s->pv[p] = ov[p] + (1.0 - 0.0) * ((-1.0) * dt * (s->fyltot[p]) / vr + (s->intv[p] - ointv[p]) / vr) +
+                 rkab * dt * s->gaccy * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (s->colfy[p] + ocfy[p]) / vr;
+      s->py[p] = oy[p] + rkab * 0.5 * dt * (s->pv[p] + ov[p]);
+      s->pw[p] = ow[p] + (1.0 - 0.0) * ((-1.0) * dt * (s->fzltot[p]) / vr + (s->intw[p] - ointw[p]) / vr) +
+                 rkab * dt * s->gaccz * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (s->colfz[p] + ocfz[p]) / vr;
+      s->pz[p] = oz[p] + rkab * 0.5 * dt * (s->pw[p] + ow[p]);
+      if (s->rotation) {
+        double mr = s->mominert * s->ratiorho;
+        s->pomx[p] = oomx[p] + (-1.0) * dt * s->torqx[p] / mr + (s->intomx[p] - ointomx[p]) / mr + rkab * 0.5 * dt * (s->coltx[p] + octx[p]) / mr;
+        s->pomy[p] = oomy[p] + (-1.0) * dt * s->torqy[p] / mr + (s->intomy[p] - ointomy[p]) / mr + rkab * 0.5 * dt * (s->colty[p] + octy[p]) / mr;
+        s->pomz[p] = oomz[p] + (-1.0) * dt * s->torqz[p] / mr + (s->intomz[p] - ointomz[p]) / mr + rkab * 0.5 * dt * (s->coltz[p] + octz[p]) / mr;
+      } else {
+        s->pomx[p] = 0.0; s->pomy[p] = 0.0; s->pomz[p] = 0.0;
+      }
+      s->pphi[p] = oph[p] + 0.5 * rkab * dt * (s->pomz[p] + oomz[p]);
+      s->pomtheta[p] = (s->pomy[p] * cos(s->pphi[p])) - (s->pomx[p] * sin(s->pphi[p]));
+      s->ptheta[p] = oth[p] + rkab * 0.5 * dt * (s->pomtheta[p] + oomt[p]);
+    }
+  }
+  /* periodic correction (702-714) */
+  for (int p = 0; p < np; p++) {
+    if (s->px[p] > lx) s->px[p] = s->px[p] - lx;
+    if (s->px[p] < 0.0) s->px[p] = s->px[p] + lx;
+    if (s->py[p] > ly) s->py[p] = s->py[p] - ly;
+    if (s->py[p] < 0.0) s->py[p] = s->py[p] + ly;
+    set_nb_flags(s, p);
+  }
+  refresh_markers(s);
+  free(anb); free(o);
+}
+
+/* initial particle state (src/initparticles.f90:159-241): positions given,
+ * velocities, angles and collision forces zero                            */
+void orc_init_particles(orc_state *s, const double *x, const double *y, const double *z) {
+  for (int p = 0; p < s->np; p++) {
+    s->px[p] = x[p]; s->py[p] = y[p]; s->pz[p] = z[p];
+    s->ptheta[p] = 0.0; s->pphi[p] = 0.0;
+    s->pu[p] = 0.0; s->pv[p] = 0.0; s->pw[p] = 0.0; s->pomx[p] = 0.0; s->pomy[p] = 0.0; s->pomz[p] = 0.0;
+    s->pomtheta[p] = 0.0; s->colfx[p] = 0.0; s->colfy[p] = 0.0; s->colfz[p] = 0.0;
+    s->coltx[p] = 0.0; s->colty[p] = 0.0; s->coltz[p] = 0.0;
+    set_nb_flags(s, p);
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* src/main.f90:268-414 -- one RK3 substep                                 */
+void orc_substep(orc_state *s, int rkiter) {
+  orc_rk(s, rkiter);
+  if (s->np > 0) orc_ibm(s);
+  bounduvw3(s, s->dudt, s->dvdt, s->dwdt);
+  orc_fillps(s, rkiter);
+  orc_solver2d(s);
+  boundp1(s, s->p);
+  orc_correc(s, rkiter);
+  orc_pnew_update(s);
+  if (s->np > 0) orc_intgr_nwtn_eulr(s, rkiter);
+}
+
+/* src/main.f90:253-441 -- one time step: three substeps, then every
+ * ioutchk steps chkdiv + chkdt and dt = 0.75 dtmax                       */
+void orc_step(orc_state *s, int istep, int ioutchk) {
+  s->time = s->time + s->dt;
+  for (int rk = 1; rk <= 3; rk++) orc_substep(s, rk);
+  if (ioutchk > 0 && istep % ioutchk == 0) {
+    double dtmax; orc_chkdt(s, &dtmax);
+    s->dt = 0.75 * dtmax;
+  }
+}
