@@ -74,7 +74,7 @@ int release(ipg_ctx *c) {
   for (int q = 0; q < 64; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
   ipg_poisson_free(c);
-  void *ptrs[] = {c->slab, c->mk, c->fl, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1]};
+  void *ptrs[] = {c->slab, c->mk, c->fl, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   for (int b = 0; b < 2; b++)
     for (int d = 0; d < 3; d++) if (c->G[b][d]) (void)hipFree(c->G[b][d]);
@@ -136,7 +136,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
         if (hipMalloc(&c->G[b][d], ncell * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory (G)"); return fail(1); }
         if (hipMemset(c->G[b][d], 0, ncell * sizeof(double)) != hipSuccess) { ipg_set_error("memset failed"); return fail(1); }
       }
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < 3; b++) {
       if (hipMalloc(&c->K1[b], nplane * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory"); return fail(1); }
       if (hipMemset(c->K1[b], 0, nplane * sizeof(double)) != hipSuccess) { ipg_set_error("memset failed"); return fail(1); }
     }
@@ -270,6 +270,7 @@ int ipg_coordsfp(ipg_ctx *c, const double *thetarc, const double *phirc, double 
   // src/coordsfp.f90:84-89
   c->pc.dVlagr = ((4.0 / 3.0) * pi * std::pow(radfp + 0.5 / dxi, 3.0) - (4.0 / 3.0) * pi * std::pow(radfp - 0.5 / dxi, 3.0)) / (1.0 * nl);
   c->pc.dVeul = 1.0 / (c->g.dxi * c->g.dyi * c->g.dzi);
+  if (ipg_ibm_setup(c)) return 1;
   c->markers_ready = true;
   return 0;
 }

@@ -8,15 +8,17 @@
 // every spread, scatters with atomics, folds the periodic ghosts and adds the
 // saved dudt back (9 full-field passes per IBM pass).  Here the spread
 // contributions go, with periodic wrap folded into the index, into sparse
-// force buffers G (zero outside marker stencils), and the velocity the markers
-// see is dudt + G: the same sums, fl(G + dudtold), without the full-field
-// passes.  Two G buffers alternate so a pass reads one while spreading into
-// the other; cells are cleared by the markers that touched them.  The
-// w-stencil of a marker within h/2 of the top wall reaches the k1 = kmax+1
-// plane (src/interp_spread.f90:100 has no upper clamp); that plane never
-// receives dudt and keeps the last spread until the next one, so it is held
-// in its own double-buffered plane K1.
+// force buffers G (zero outside the spheres' stencils), and the velocity the
+// markers see is dudt + G: the same sums, fl(G + dudtold), without the
+// full-field passes.  The last pass spreads straight into dudt.  Work is
+// organised one workgroup per sphere: the sphere's box of cells is staged in
+// LDS for the gathers, and the spread is summed in LDS before one global
+// atomic per touched cell.  The w-stencil of a marker within h/2 of the top
+// wall reaches the k1 = kmax+1 plane (src/interp_spread.f90:100 has no upper
+// clamp); that plane never receives dudt and keeps the last spread until the
+// next one, so it is held in its own planes K.
 #include "ipg_internal.h"
+#include <cmath>
 
 namespace {
 
@@ -70,147 +72,204 @@ __device__ __forceinline__ void stencil(const Grid &g, double xfp, double yfp, d
 struct IbmArgs {
   Grid g;
   int np, nl;
-  double dVlagr, dVeul;
+  double dVlagr, dVeul, radfp;
+  int bmx, bmy, bmz;  // largest box extent per direction (LDS sizing)
   const double *px, *py, *pz, *pu, *pv, *pw, *pomx, *pomy, *pomz, *pphi;
-  const double *mk;  // [3][nl]
-  double *fl;        // [3][np*nl]
-  double *fld[3];    // dudt dvdt dwdt
-  double *Gr[3], *Gw[3];
-  double *K1r, *K1w;
+  const double *mk;   // [3][nl] marker offsets from the sphere centre
+  double *fl;         // [3][np*nl] marker forces fxl, fyl, fzl
+  double *fld[3];     // dudt dvdt dwdt
+  double *Gread[3];   // spread of the previous pass, added to fld when gathering (null at pass 0)
+  double *Gspread[3]; // spread target; null: spread straight into fld (last pass)
+  double *Gzero[3];   // buffer cleared on this sphere's cells (null: none)
+  double *Kread, *Kspread, *Kzero;  // dwdtf(.,.,k1) planes
   const double *dtp;
 };
 
 __device__ __forceinline__ long gidx(const Grid &g, int wi, int wj, int k) {
   return (long)(wi - 1) + (long)g.nx * ((long)(wj - 1) + (long)g.ny * (long)(k - 1));
 }
+// periodic wrap of any unwrapped index onto 1..n
+__device__ __forceinline__ int wrapany(int i, int n) {
+  int r = (i - 1) % n;
+  return (r < 0 ? r + n : r) + 1;
+}
 
-// Gather (src/interp_spread.f90:132-154) + force update (src/forcing.f90:63-65,
-// 255-257).  pass 2 also clears the spread buffer of pass 0; every pass
-// clears the K1 plane it will spread into.
-__global__ __launch_bounds__(256) void k_ibm_interp(IbmArgs A, int pass) {
+// The per-sphere box of one velocity component: cells [b0, b0+bn) per
+// direction in unwrapped index space (z clipped to the channel, 1..nz).  It
+// holds every stencil cell of the sphere's markers with one cell to spare.
+struct Box { int bi0, bj0, bk0, bx, by, bz; };
+__device__ __forceinline__ Box sphere_box(const IbmArgs &A, double xp, double yp, double zp) {
   const Grid &g = A.g;
-  const long nplane = (long)g.nx * g.ny;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nplane; t += (long)gridDim.x * blockDim.x) A.K1w[t] = 0.0;
-  const long M = (long)A.np * A.nl;
-  const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (m >= M) return;
-  const int p = (int)(m / A.nl), l = (int)(m - (long)p * A.nl);
-  const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
-  double xl, yl, zl;
-  if (!marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return;
+  Box b;
+  const int x0 = (int)floor((xp - A.radfp) * g.dxi), x1 = (int)floor((xp + A.radfp) * g.dxi);
+  const int y0 = (int)floor((yp - A.radfp) * g.dyi), y1 = (int)floor((yp + A.radfp) * g.dyi);
+  const int z0 = (int)floor((zp - A.radfp) * g.dzi), z1 = (int)floor((zp + A.radfp) * g.dzi);
+  b.bi0 = x0 - 2; b.bx = min(x1 + 3 - b.bi0 + 1, A.bmx);
+  b.bj0 = y0 - 2; b.by = min(y1 + 3 - b.bj0 + 1, A.bmy);
+  b.bk0 = max(z0 - 2, 1);
+  b.bz = min(min(z1 + 3, g.nz) - b.bk0 + 1, A.bmz);
+  if (b.bz < 0) b.bz = 0;
+  return b;
+}
+
+// local box index of an unwrapped cell, -1 when outside the box
+__device__ __forceinline__ int box_at(const Box &b, int iu, int ju, int k) {
+  const int li = iu - b.bi0, lj = ju - b.bj0, lk = k - b.bk0;
+  if ((unsigned)li >= (unsigned)b.bx || (unsigned)lj >= (unsigned)b.by || (unsigned)lk >= (unsigned)b.bz) return -1;
+  return li + b.bx * (lj + b.by * lk);
+}
+
+// Marker stencil of one component in unwrapped indices: base cell (i0,j0,k0)
+// and the three weights per direction, in the reference's order
+// (src/interp_spread.f90:98-130).
+struct CompStencil { int i0, j0, k0; double wx[3], wy[3], wz[3]; };
+__device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl, double zl, int comp, CompStencil &c) {
   Stencil s;
   stencil(g, xl, yl, zl, s);
-  double au = 0.0, av = 0.0, aw = 0.0;
-  for (int k = 0; k < 3; k++)
-    for (int j = 0; j < 3; j++)
-      for (int i = 0; i < 3; i++) {
-        const int wi = wrapi(s.ilows + i, g.nx), wj = wrapi(s.jlow + j, g.ny), kk = s.klow + k;
-        const double f = A.fld[0][g.at(wi, wj, kk)] + A.Gr[0][gidx(g, wi, wj, kk)];
-        au = au + f * s.phix[i] * s.phiyh[j] * s.phizh[k];
-      }
-  for (int k = 0; k < 3; k++)
-    for (int j = 0; j < 3; j++)
-      for (int i = 0; i < 3; i++) {
-        const int wi = wrapi(s.ilow + i, g.nx), wj = wrapi(s.jlows + j, g.ny), kk = s.klow + k;
-        const double f = A.fld[1][g.at(wi, wj, kk)] + A.Gr[1][gidx(g, wi, wj, kk)];
-        av = av + f * s.phixh[i] * s.phiy[j] * s.phizh[k];
-      }
-  for (int k = 0; k < 3; k++)
-    for (int j = 0; j < 3; j++)
-      for (int i = 0; i < 3; i++) {
-        const int wi = wrapi(s.ilow + i, g.nx), wj = wrapi(s.jlow + j, g.ny), kk = s.klows + k;
-        double f;
-        if (kk == g.nz + 1) f = A.K1r[(long)(wi - 1) + (long)g.nx * (wj - 1)];
-        else if (kk > g.nz + 1) f = 0.0;  // dwdtf(.,.,k1+1): never holds dudt (marker outside the channel)
-        else f = A.fld[2][g.at(wi, wj, kk)] + A.Gr[2][gidx(g, wi, wj, kk)];
-        aw = aw + f * s.phixh[i] * s.phiyh[j] * s.phiz[k];
-      }
-  // rigid-body velocity at the marker, src/intgr_nwtn_eulr.f90:1471-1476
-  const double ul = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
-  const double vl = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
-  const double wl = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
-  const double dti = 1.0 / *A.dtp;
-  double *fx = A.fl, *fy = A.fl + M, *fz = A.fl + 2 * M;
-  if (pass == 0) {
-    fx[m] = (ul - au) * dti; fy[m] = (vl - av) * dti; fz[m] = (wl - aw) * dti;
+  const int sx = (int)floor((xl - 0.5 * g.dx) / g.lx) * g.nx;
+  const int sy = (int)floor((yl - 0.5 * g.dy) / g.ly) * g.ny;
+  if (comp == 0) {
+    c.i0 = s.ilows + sx; c.j0 = s.jlow + sy; c.k0 = s.klow;
+    for (int q = 0; q < 3; q++) { c.wx[q] = s.phix[q]; c.wy[q] = s.phiyh[q]; c.wz[q] = s.phizh[q]; }
+  } else if (comp == 1) {
+    c.i0 = s.ilow + sx; c.j0 = s.jlows + sy; c.k0 = s.klow;
+    for (int q = 0; q < 3; q++) { c.wx[q] = s.phixh[q]; c.wy[q] = s.phiy[q]; c.wz[q] = s.phizh[q]; }
   } else {
-    fx[m] = fx[m] + (ul - au) * dti; fy[m] = fy[m] + (vl - av) * dti; fz[m] = fz[m] + (wl - aw) * dti;
-  }
-  if (pass == 2) {
-    for (int k = 0; k < 3; k++)
-      for (int j = 0; j < 3; j++)
-        for (int i = 0; i < 3; i++) {
-          A.Gw[0][gidx(g, wrapi(s.ilows + i, g.nx), wrapi(s.jlow + j, g.ny), s.klow + k)] = 0.0;
-          A.Gw[1][gidx(g, wrapi(s.ilow + i, g.nx), wrapi(s.jlows + j, g.ny), s.klow + k)] = 0.0;
-          const int kk = s.klows + k;
-          if (kk <= g.nz) A.Gw[2][gidx(g, wrapi(s.ilow + i, g.nx), wrapi(s.jlow + j, g.ny), kk)] = 0.0;
-        }
+    c.i0 = s.ilow + sx; c.j0 = s.jlow + sy; c.k0 = s.klows;
+    for (int q = 0; q < 3; q++) { c.wx[q] = s.phixh[q]; c.wy[q] = s.phiyh[q]; c.wz[q] = s.phiz[q]; }
   }
 }
 
-// Spread, src/interp_spread.f90:329-355 with the periodic fold of
-// updthalos_ibm icase 2 (src/bound.f90:276-305) folded into the index.
-__global__ __launch_bounds__(256) void k_ibm_spread(IbmArgs A) {
+// One IBM pass for one sphere (one block): for each velocity component the
+// sphere's box of (dudt + previous spread) is staged in LDS, every marker
+// gathers its 27-point stencil from it (src/interp_spread.f90:132-154) and
+// updates its force (src/forcing.f90:63-65 / 255-257); then the markers'
+// spread (src/interp_spread.f90:329-355) is summed in the same LDS box with
+// LDS atomics and the box is added to the global target once, cell by cell.
+// gather/spread select the halves; the last pass runs them as two launches
+// because its spread goes straight into dudt, which other spheres may still
+// be gathering from.
+__global__ __launch_bounds__(256) void k_ibm_pass(IbmArgs A, int pass, int gather, int spread) {
+  extern __shared__ double box[];
   const Grid &g = A.g;
+  const int p = blockIdx.x;
   const long M = (long)A.np * A.nl;
-  const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (m >= M) return;
-  const int p = (int)(m / A.nl), l = (int)(m - (long)p * A.nl);
-  double xl, yl, zl;
-  if (!marker_pos(g, A.px[p], A.py[p], A.pz[p], A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return;
-  Stencil s;
-  stencil(g, xl, yl, zl, s);
-  const double c0 = (A.dVlagr / A.dVeul) * *A.dtp;
-  const double fxc = A.fl[m] * c0, fyc = A.fl[M + m] * c0, fzc = A.fl[2 * M + m] * c0;
-  for (int k = 0; k < 3; k++)
-    for (int j = 0; j < 3; j++)
-      for (int i = 0; i < 3; i++) {
-        unsafeAtomicAdd(&A.Gw[0][gidx(g, wrapi(s.ilows + i, g.nx), wrapi(s.jlow + j, g.ny), s.klow + k)],
-                        fxc * s.phix[i] * s.phiyh[j] * s.phizh[k]);
-        unsafeAtomicAdd(&A.Gw[1][gidx(g, wrapi(s.ilow + i, g.nx), wrapi(s.jlows + j, g.ny), s.klow + k)],
-                        fyc * s.phixh[i] * s.phiy[j] * s.phizh[k]);
-        const int wi = wrapi(s.ilow + i, g.nx), wj = wrapi(s.jlow + j, g.ny), kk = s.klows + k;
-        const double cw = fzc * s.phixh[i] * s.phiyh[j] * s.phiz[k];
-        if (kk == g.nz + 1) unsafeAtomicAdd(&A.K1w[(long)(wi - 1) + (long)g.nx * (wj - 1)], cw);
-        else if (kk <= g.nz) unsafeAtomicAdd(&A.Gw[2][gidx(g, wi, wj, kk)], cw);
+  if (A.Kzero) {
+    const long nplane = (long)g.nx * g.ny;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nplane; t += (long)gridDim.x * blockDim.x) A.Kzero[t] = 0.0;
+  }
+  const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
+  const double up = A.pu[p], vp = A.pv[p], wp = A.pw[p], ox = A.pomx[p], oy = A.pomy[p], oz = A.pomz[p];
+  const double dt = *A.dtp, dti = 1.0 / dt;
+  const double c0 = (A.dVlagr / A.dVeul) * dt;
+  const Box b = sphere_box(A, xp, yp, zp);
+  const int nbox = b.bx * b.by * b.bz;
+  for (int comp = 0; comp < 3; comp++) {
+    const double *F = A.fld[comp];
+    double *fl = A.fl + comp * M;
+    if (gather) {
+      const double *Gr = A.Gread[comp];
+      for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+        const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
+        const int wi = wrapany(b.bi0 + li, g.nx), wj = wrapany(b.bj0 + lj, g.ny), k = b.bk0 + lk;
+        double v = F[g.at(wi, wj, k)];
+        if (Gr) v = v + Gr[gidx(g, wi, wj, k)];
+        box[t] = v;
       }
-}
-
-// End of the IBM block (src/main.f90:332-334): dudt(1:n) = dudtf(1:n), i.e.
-// dudt += G of the last pass on the touched cells.  The first marker to reach
-// a cell takes its G value (atomic exchange with zero) and adds it; the
-// spread buffer of the middle pass is cleared on the same cells.
-__global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs A) {
-  const Grid &g = A.g;
-  // the last spread's k1 plane becomes the plane the next substep reads
-  const long nplane = (long)g.nx * g.ny;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nplane; t += (long)gridDim.x * blockDim.x) A.K1w[t] = A.K1r[t];
-  const long M = (long)A.np * A.nl;
-  const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (m >= M) return;
-  const int p = (int)(m / A.nl), l = (int)(m - (long)p * A.nl);
-  double xl, yl, zl;
-  if (!marker_pos(g, A.px[p], A.py[p], A.pz[p], A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return;
-  Stencil s;
-  stencil(g, xl, yl, zl, s);
-  for (int k = 0; k < 3; k++)
-    for (int j = 0; j < 3; j++)
-      for (int i = 0; i < 3; i++) {
-        for (int comp = 0; comp < 3; comp++) {
-          int wi, wj, kk;
-          if (comp == 0) { wi = wrapi(s.ilows + i, g.nx); wj = wrapi(s.jlow + j, g.ny); kk = s.klow + k; }
-          else if (comp == 1) { wi = wrapi(s.ilow + i, g.nx); wj = wrapi(s.jlows + j, g.ny); kk = s.klow + k; }
-          else { wi = wrapi(s.ilow + i, g.nx); wj = wrapi(s.jlow + j, g.ny); kk = s.klows + k; }
-          if (kk > g.nz) continue;
-          const long q = gidx(g, wi, wj, kk);
-          A.Gw[comp][q] = 0.0;
-          const double f = atomicExch(&A.Gr[comp][q], 0.0);
-          if (f != 0.0) {
-            double *fp = &A.fld[comp][g.at(wi, wj, kk)];
-            *fp = *fp + f;
-          }
+      __syncthreads();
+      for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
+        double xl, yl, zl;
+        if (!marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
+        CompStencil c;
+        comp_stencil(g, xl, yl, zl, comp, c);
+        double acc = 0.0;
+        for (int k = 0; k < 3; k++)
+          for (int j = 0; j < 3; j++)
+            for (int i = 0; i < 3; i++) {
+              const int iu = c.i0 + i, ju = c.j0 + j, kk = c.k0 + k;
+              double f;
+              const int q = box_at(b, iu, ju, kk);
+              if (q >= 0) f = box[q];
+              else if (kk >= 1 && kk <= g.nz) {
+                const int wi = wrapany(iu, g.nx), wj = wrapany(ju, g.ny);
+                f = F[g.at(wi, wj, kk)];
+                if (A.Gread[comp]) f = f + A.Gread[comp][gidx(g, wi, wj, kk)];
+              } else if (kk == g.nz + 1 && comp == 2) {
+                f = A.Kread[(long)(wrapany(iu, g.nx) - 1) + (long)g.nx * (wrapany(ju, g.ny) - 1)];
+              } else f = 0.0;
+              acc = acc + f * c.wx[i] * c.wy[j] * c.wz[k];
+            }
+        // rigid-body velocity at the marker, src/intgr_nwtn_eulr.f90:1471-1476
+        double ub;
+        if (comp == 0) ub = up + oy * (zl - zp) - oz * (yl - yp);
+        else if (comp == 1) ub = vp + oz * (xl - xp) - ox * (zl - zp);
+        else ub = wp + ox * (yl - yp) - oy * (xl - xp);
+        const long m = (long)p * A.nl + l;
+        if (pass == 0) fl[m] = (ub - acc) * dti;
+        else fl[m] = fl[m] + (ub - acc) * dti;
+      }
+      __syncthreads();
+    }
+    if (spread) {
+      double *Gw = A.Gspread[comp];
+      for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
+      __syncthreads();
+      for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
+        double xl, yl, zl;
+        if (!marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
+        CompStencil c;
+        comp_stencil(g, xl, yl, zl, comp, c);
+        const double fc = fl[(long)p * A.nl + l] * c0;
+        for (int k = 0; k < 3; k++)
+          for (int j = 0; j < 3; j++)
+            for (int i = 0; i < 3; i++) {
+              const int iu = c.i0 + i, ju = c.j0 + j, kk = c.k0 + k;
+              const double v = fc * c.wx[i] * c.wy[j] * c.wz[k];
+              const int q = box_at(b, iu, ju, kk);
+              if (q >= 0) unsafeAtomicAdd(&box[q], v);
+              else if (kk >= 1 && kk <= g.nz) {
+                const int wi = wrapany(iu, g.nx), wj = wrapany(ju, g.ny);
+                if (Gw) unsafeAtomicAdd(&Gw[gidx(g, wi, wj, kk)], v);
+                else unsafeAtomicAdd(&A.fld[comp][g.at(wi, wj, kk)], v);
+              } else if (kk == g.nz + 1 && comp == 2) {
+                unsafeAtomicAdd(&A.Kspread[(long)(wrapany(iu, g.nx) - 1) + (long)g.nx * (wrapany(ju, g.ny) - 1)], v);
+              }
+            }
+      }
+      __syncthreads();
+      for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+        const double v = box[t];
+        if (v != 0.0) {
+          const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
+          const int wi = wrapany(b.bi0 + li, g.nx), wj = wrapany(b.bj0 + lj, g.ny), k = b.bk0 + lk;
+          if (Gw) unsafeAtomicAdd(&Gw[gidx(g, wi, wj, k)], v);
+          else unsafeAtomicAdd(&A.fld[comp][g.at(wi, wj, k)], v);
         }
       }
+      __syncthreads();
+    }
+    if (A.Gzero[comp]) {
+      double *Gz = A.Gzero[comp];
+      for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+        const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
+        Gz[gidx(g, wrapany(b.bi0 + li, g.nx), wrapany(b.bj0 + lj, g.ny), b.bk0 + lk)] = 0.0;
+      }
+      // stencil cells outside the box (not expected; kept exact)
+      for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
+        double xl, yl, zl;
+        if (!marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
+        CompStencil c;
+        comp_stencil(g, xl, yl, zl, comp, c);
+        for (int k = 0; k < 3; k++)
+          for (int j = 0; j < 3; j++)
+            for (int i = 0; i < 3; i++) {
+              const int iu = c.i0 + i, ju = c.j0 + j, kk = c.k0 + k;
+              if (box_at(b, iu, ju, kk) < 0 && kk >= 1 && kk <= g.nz)
+                Gz[gidx(g, wrapany(iu, g.nx), wrapany(ju, g.ny), kk)] = 0.0;
+            }
+      }
+    }
+  }
 }
 
 // Per-sphere force and torque sums after the last pass, src/forcing.f90:258-287,
@@ -218,6 +277,10 @@ __global__ __launch_bounds__(256) void k_ibm_apply(IbmArgs A) {
 __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, double *fyt, double *fzt, double *tqx, double *tqy,
                                                     double *tqz, double *tqt) {
   const Grid &g = A.g;
+  if (A.Kzero) {
+    const long nplane = (long)g.nx * g.ny;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nplane; t += (long)gridDim.x * blockDim.x) A.Kzero[t] = 0.0;
+  }
   const int p = blockIdx.x;
   const long M = (long)A.np * A.nl;
   const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
@@ -250,44 +313,65 @@ __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, doub
 
 }  // namespace
 
+// LDS of the per-sphere box (one velocity component at a time), set once at
+// marker set-up so that graph capture never meets an attribute change.
+int ipg_ibm_setup(ipg_ctx *c) {
+  const double r2 = 2.0 * c->pc.radfp;
+  const size_t lds = (size_t)((int)std::ceil(r2 * c->g.dxi) + 7) * ((int)std::ceil(r2 * c->g.dyi) + 7) *
+                     ((int)std::ceil(r2 * c->g.dzi) + 7) * sizeof(double);
+  if (lds > 160 * 1024) { ipg_set_error("ibm: sphere box exceeds the 160 KiB LDS of a CU"); return 1; }
+  IPG_HIP(hipFuncSetAttribute((const void *)k_ibm_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  c->ibm_lds_set = true;
+  return 0;
+}
+
 int ipg_launch_ibm(ipg_ctx *c) {
   if (c->pc.np <= 0) return 0;
   if (!c->markers_ready) { ipg_set_error("ibm: ipg_coordsfp has not been called"); return 1; }
   IbmArgs A{};
-  A.g = c->g; A.np = c->pc.np; A.nl = c->pc.nl; A.dVlagr = c->pc.dVlagr; A.dVeul = c->pc.dVeul;
+  A.g = c->g; A.np = c->pc.np; A.nl = c->pc.nl; A.dVlagr = c->pc.dVlagr; A.dVeul = c->pc.dVeul; A.radfp = c->pc.radfp;
+  const double r2 = 2.0 * c->pc.radfp;
+  A.bmx = (int)std::ceil(r2 * c->g.dxi) + 7;
+  A.bmy = (int)std::ceil(r2 * c->g.dyi) + 7;
+  A.bmz = (int)std::ceil(r2 * c->g.dzi) + 7;
+  const size_t lds = (size_t)A.bmx * A.bmy * A.bmz * sizeof(double);
+  if (!c->ibm_lds_set) { ipg_set_error("ibm: ipg_coordsfp has not set up the sphere boxes"); return 1; }
   double **q = c->ps.q;
   A.px = q[IPG_PX]; A.py = q[IPG_PY]; A.pz = q[IPG_PZ]; A.pu = q[IPG_PU]; A.pv = q[IPG_PV]; A.pw = q[IPG_PW];
   A.pomx = q[IPG_POMX]; A.pomy = q[IPG_POMY]; A.pomz = q[IPG_POMZ]; A.pphi = q[IPG_PPHI];
   A.mk = c->mk; A.fl = c->fl;
   A.fld[0] = c->f[IPG_DUDT]; A.fld[1] = c->f[IPG_DVDT]; A.fld[2] = c->f[IPG_DWDT];
   A.dtp = c->dscal;
-  const long M = (long)A.np * A.nl;
-  const int blocks = (int)((M + 255) / 256);
-  // Fixed buffer roles every substep (so a captured graph replays correctly):
-  // pass 0 reads G0 (all zero) and spreads into G1, pass 1 reads G1 into G0,
-  // pass 2 reads G0 (clearing G1) into G1; the apply step consumes G1 and
-  // clears G0.  K1[0] holds the k1 plane between substeps.
-  c->gcur = 0;
-  c->kcur = 0;
-  for (int pass = 0; pass < 3; pass++) {
-    for (int d = 0; d < 3; d++) { A.Gr[d] = c->G[c->gcur][d]; A.Gw[d] = c->G[1 - c->gcur][d]; }
-    A.K1r = c->K1[c->kcur]; A.K1w = c->K1[1 - c->kcur];
-    hipLaunchKernelGGL(k_ibm_interp, dim3(blocks), dim3(256), 0, c->stream, A, pass);
-    ipg_mark(c, "ibm_interp");
-    hipLaunchKernelGGL(k_ibm_spread, dim3(blocks), dim3(256), 0, c->stream, A);
-    ipg_mark(c, "ibm_spread");
-    c->gcur = 1 - c->gcur;
-    c->kcur = 1 - c->kcur;
-  }
-  for (int d = 0; d < 3; d++) { A.Gr[d] = c->G[c->gcur][d]; A.Gw[d] = c->G[1 - c->gcur][d]; }
-  A.K1r = c->K1[c->kcur]; A.K1w = c->K1[1 - c->kcur];
-  hipLaunchKernelGGL(k_ibm_totals, dim3(A.np), dim3(256), 0, c->stream, A, q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
+  double **GA = c->G[0], **GB = c->G[1];
+  auto set = [&](double **rd, double **sp, double **zr, double *kr, double *ks, double *kz) {
+    for (int d = 0; d < 3; d++) {
+      A.Gread[d] = rd ? rd[d] : nullptr;
+      A.Gspread[d] = sp ? sp[d] : nullptr;
+      A.Gzero[d] = zr ? zr[d] : nullptr;
+    }
+    A.Kread = kr; A.Kspread = ks; A.Kzero = kz;
+  };
+  // Fixed roles (a captured graph replays them): pass 0 gathers dudt and
+  // spreads into GA; pass 1 gathers dudt+GA and spreads into GB; pass 2
+  // gathers dudt+GB (clearing GA), then spreads into dudt (clearing GB).
+  // The k1 plane: K0 carries it between substeps, K1/K2 hold passes 0/1.
+  const dim3 grid(A.np), blk(256);
+  set(nullptr, GA, nullptr, c->K1[0], c->K1[1], nullptr);
+  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, 0, 1, 1);
+  ipg_mark(c, "ibm_pass0");
+  set(GA, GB, nullptr, c->K1[1], c->K1[2], nullptr);
+  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, 1, 1, 1);
+  ipg_mark(c, "ibm_pass1");
+  set(GB, nullptr, GA, c->K1[2], nullptr, c->K1[0]);
+  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, 2, 1, 0);
+  ipg_mark(c, "ibm_pass2_gather");
+  set(nullptr, nullptr, GB, nullptr, c->K1[0], c->K1[1]);
+  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, 2, 0, 1);
+  ipg_mark(c, "ibm_pass2_spread");
+  set(nullptr, nullptr, nullptr, nullptr, nullptr, c->K1[2]);
+  hipLaunchKernelGGL(k_ibm_totals, grid, blk, 0, c->stream, A, q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
                      q[IPG_PTORQX], q[IPG_PTORQY], q[IPG_PTORQZ], q[IPG_PTORQTHETA]);
   ipg_mark(c, "ibm_totals");
-  hipLaunchKernelGGL(k_ibm_apply, dim3(blocks), dim3(256), 0, c->stream, A);
-  ipg_mark(c, "ibm_apply");
-  c->gcur = 0;
-  c->kcur = 0;
   IPG_HIP(hipGetLastError());
   return 0;
 }

@@ -63,8 +63,8 @@ struct ipg_ctx {
   double *tmpf[4];                // rk2/rk3 temporaries are not needed: fused
   // IBM
   double *G[2][3];                // spread force buffers, interior (nx*ny*nz), modular x,y
-  double *K1[2];                  // dwdtf(.,.,k1) plane (nx*ny), src/interp_spread.f90:100 (no upper clamp)
-  int gcur, kcur;
+  double *K1[3];                  // dwdtf(.,.,k1) planes (nx*ny), src/interp_spread.f90:100 (no upper clamp)
+  bool ibm_lds_set;
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
   double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl
   // particles
@@ -136,6 +136,7 @@ int ipg_poisson_setup(ipg_ctx *c);
 int ipg_poisson_free(ipg_ctx *c);
 int ipg_launch_poisson(ipg_ctx *c);
 int ipg_launch_ibm(ipg_ctx *c);
+int ipg_ibm_setup(ipg_ctx *c);
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type);
 int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter);
 int ipg_launch_nbflags(ipg_ctx *c);

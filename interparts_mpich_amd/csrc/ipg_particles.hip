@@ -211,11 +211,13 @@ constexpr int MAXCAND = 48;
 // neighbour tables nb = 0..8 (at one rank: the sphere set and its 8 periodic
 // copies) are scanned in order; every copy of a partner in range adds its
 // lubrication and contact force (the reference's 9x multiplicity at one rank).
-// Partners are first screened with a margin so that only candidates enter the
-// ordered scan.
+// One wavefront per sphere screens all partners (with a margin), keeping the
+// candidates in ascending order; lane 0 then runs the reference's ordered scan
+// over the candidates only.
 __global__ __launch_bounds__(256) void k_collide(Grid g, PartConst pc, PQ P, double *cfx, double *cfy, double *cfz,
                                                  int *err) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int np = pc.np;
   if (p >= np) return;
   const double *X = P.q[IPG_PX], *Y = P.q[IPG_PY], *Zq = P.q[IPG_PZ], *Uq = P.q[IPG_PU], *Vq = P.q[IPG_PV], *Wq = P.q[IPG_PW];
@@ -224,17 +226,26 @@ __global__ __launch_bounds__(256) void k_collide(Grid g, PartConst pc, PQ P, dou
   const double reach = 2.0 * R * (1.0 + 0.025) + 1e-9 * (lx + ly + R);
   int cand[MAXCAND];
   int nc = 0;
-  for (int q = 0; q < np; q++) {
-    if (q == p) continue;
-    double dz = Zq[q] - zp;
-    if (fabs(dz) > reach) continue;
-    double dx = X[q] - xp, dy = Y[q] - yp;
-    dx = dx - lx * rint(dx / lx);
-    dy = dy - ly * rint(dy / ly);
-    if (dx * dx + dy * dy + dz * dz <= reach * reach) {
-      if (nc < MAXCAND) cand[nc++] = q; else atomicOr(err, 1);
+  for (int q0 = 0; q0 < np; q0 += 64) {
+    const int q = q0 + lane;
+    bool hit = false;
+    if (q < np && q != p) {
+      const double dz = Zq[q] - zp;
+      double dx = X[q] - xp, dy = Y[q] - yp;
+      dx = dx - lx * rint(dx / lx);
+      dy = dy - ly * rint(dy / ly);
+      hit = dx * dx + dy * dy + dz * dz <= reach * reach;
+    }
+    unsigned long long mask = __ballot(hit);
+    while (mask) {
+      const int b = __ffsll((long long)mask) - 1;
+      if (nc < MAXCAND) cand[nc] = q0 + b;
+      nc++;
+      mask &= mask - 1;
     }
   }
+  if (lane != 0) return;
+  if (nc > MAXCAND) { atomicOr(err, 1); nc = MAXCAND; }
   const double up = Uq[p], vp = Vq[p], wp = Wq[p];
   const double omx = P.q[IPG_POMX][p], omy = P.q[IPG_POMY][p], omz = P.q[IPG_POMZ][p];
   double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -372,7 +383,7 @@ int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter) {
                      c->ps.nbf, 23);
   ipg_mark(c, "sphere_int");
   for (int iter = 0; iter <= 1; iter++) {
-    hipLaunchKernelGGL(k_collide, dim3(nb), dim3(256), 0, c->stream, c->g, c->pc, P, cf, cf + np, cf + 2 * np, err);
+    hipLaunchKernelGGL(k_collide, dim3((np + 3) / 4), dim3(256), 0, c->stream, c->g, c->pc, P, cf, cf + np, cf + 2 * np, err);
     ipg_mark(c, "collide");
     hipLaunchKernelGGL(k_ne_update, dim3(nb), dim3(256), 0, c->stream, c->g, c->pc, P, c->op, cf, cf + np, cf + 2 * np,
                        c->dscal, rkiter);
