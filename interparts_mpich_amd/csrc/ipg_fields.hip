@@ -12,7 +12,10 @@ constexpr int BX = 64, BY = 4;
 // One RK stage, src/rk3.f90:13-106 (STAGE 1), 108-222 (2), 224-329 (3), with
 // momxad/momyad/momzad/momxp/momyp/momzp/momzbuoy (src/mom.f90) and scalad
 // (src/scal.f90:11-60) evaluated in registers: one read of each input and one
-// write of each output per cell.
+// write of each output per cell.  A workgroup owns a 64 x 4 column tile and
+// marches up KB planes in z, keeping each column's k-1, k, k+1 values in
+// registers; tiles are assigned so that each XCD works on one contiguous slab
+// (its L2 then holds the halos its neighbours share).
 template <int STAGE>
 __global__ __launch_bounds__(BX *BY) void k_rk(Grid g, const double *__restrict__ U, const double *__restrict__ V,
                                                const double *__restrict__ W, const double *__restrict__ C,
@@ -20,99 +23,118 @@ __global__ __launch_bounds__(BX *BY) void k_rk(Grid g, const double *__restrict_
                                                double *__restrict__ dvdt, double *__restrict__ dwdt,
                                                double *__restrict__ dcdt, double *__restrict__ durk,
                                                double *__restrict__ dvrk, double *__restrict__ dwrk,
-                                               double *__restrict__ dcrk, const double *__restrict__ dtp) {
-  const int i = blockIdx.x * BX + threadIdx.x + 1;
-  const int j = blockIdx.y * BY + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
+                                               double *__restrict__ dcrk, const double *__restrict__ dtp, int ntx,
+                                               int nty, int kb) {
+  const int nblk = gridDim.x, L = blockIdx.x;
+  const int t = (nblk % 8 == 0) ? (L % 8) * (nblk / 8) + L / 8 : L;
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  const int i = tx * BX + (int)(threadIdx.x % BX) + 1;
+  const int j = ty * BY + (int)(threadIdx.x / BX) + 1;
+  const int k0 = tz * kb + 1, k1 = min(k0 + kb - 1, g.nz);
   if (i > g.nx || j > g.ny) return;
   const long sx = g.sx, sxy = g.sxy;
-  const long c = g.at(i, j, k);
-  const long ip = c + 1, im = c - 1, jp = c + sx, jm = c - sx, kp = c + sxy, km = c - sxy;
   const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, visc = g.visc, kappa = g.kappa;
   const double dt = *dtp;
   double f1, f2 = 0.0, f3 = 0.0;
   if (STAGE == 1) f1 = dt * (32.0 / 60.0);
   else if (STAGE == 2) { f1 = dt * (8.0 / 60.0); f2 = dt * (25.0 / 60.0); f3 = dt * (17.0 / 60.0); }
   else { f1 = dt * (20.0 / 60.0); f2 = dt * (45.0 / 60.0); f3 = dt * (25.0 / 60.0); }
-
-  const double Uc = U[c], Vc = V[c], Wc = W[c];
-  // momxp/momyp/momzp (src/mom.f90:56-79,129-155,205-231)
-  const double gpx = -(dxi * (P[ip] - P[c]));
-  const double gpy = -(dyi * (P[jp] - P[c]));
-  const double gpz = -(dzi * (P[kp] - P[c]));
-  const double du0 = Uc + f1 * gpx, dv0 = Vc + f1 * gpy, dw0 = Wc + f1 * gpz;
-  double ax, ay, az, ac;
-  {  // momxad, src/mom.f90:30-47
-    const double Uip = U[ip], Uim = U[im], Ujp = U[jp], Ujm = U[jm], Ukp = U[kp], Ukm = U[km];
-    double uuip = 0.25 * (Uip + Uc) * (Uip + Uc);
-    double uuim = 0.25 * (Uim + Uc) * (Uim + Uc);
-    double uvjp = 0.25 * (Ujp + Uc) * (V[ip] + Vc);
-    double uvjm = 0.25 * (Ujm + Uc) * (V[ip - sx] + V[jm]);
-    double uwkp = 0.25 * (Ukp + Uc) * (W[ip] + Wc);
-    double uwkm = 0.25 * (Ukm + Uc) * (W[ip - sxy] + W[km]);
-    double dudxp = (Uip - Uc) * dxi, dudxm = (Uc - Uim) * dxi;
-    double dudyp = (Ujp - Uc) * dyi, dudym = (Uc - Ujm) * dyi;
-    double dudzp = (Ukp - Uc) * dzi, dudzm = (Uc - Ukm) * dzi;
-    ax = dxi * (-uuip + uuim) + visc * (dudxp - dudxm) * dxi + dyi * (-uvjp + uvjm) + visc * (dudyp - dudym) * dyi +
-         dzi * (-uwkp + uwkm) + visc * (dudzp - dudzm) * dzi;
-  }
-  {  // momyad, src/mom.f90:103-120
-    const double Vip = V[ip], Vim = V[im], Vjp = V[jp], Vjm = V[jm], Vkp = V[kp], Vkm = V[km];
-    double uvip = 0.25 * (Uc + U[jp]) * (Vc + Vip);
-    double uvim = 0.25 * (U[im] + U[im + sx]) * (Vc + Vim);
-    double vvjp = 0.25 * (Vc + Vjp) * (Vc + Vjp);
-    double vvjm = 0.25 * (Vc + Vjm) * (Vc + Vjm);
-    double wvkp = 0.25 * (Wc + W[jp]) * (Vkp + Vc);
-    double wvkm = 0.25 * (W[km] + W[jp - sxy]) * (Vkm + Vc);
-    double dvdxp = (Vip - Vc) * dxi, dvdxm = (Vc - Vim) * dxi;
-    double dvdyp = (Vjp - Vc) * dyi, dvdym = (Vc - Vjm) * dyi;
-    double dvdzp = (Vkp - Vc) * dzi, dvdzm = (Vc - Vkm) * dzi;
-    ay = dxi * (-uvip + uvim) + visc * (dvdxp - dvdxm) * dxi + dyi * (-vvjp + vvjm) + visc * (dvdyp - dvdym) * dyi +
-         dzi * (-wvkp + wvkm) + visc * (dvdzp - dvdzm) * dzi;
-  }
-  {  // momzad, src/mom.f90:179-196; momzbuoy src/mom.f90:252 added first (src/rk3.f90:56-58)
-    const double Wip = W[ip], Wim = W[im], Wjp = W[jp], Wjm = W[jm], Wkp = W[kp], Wkm = W[km];
-    double uwip = 0.25 * (Wc + Wip) * (Uc + U[kp]);
-    double uwim = 0.25 * (Wc + Wim) * (U[im] + U[im + sxy]);
-    double vwjp = 0.25 * (Wc + Wjp) * (Vc + V[kp]);
-    double vwjm = 0.25 * (Wc + Wjm) * (V[jm] + V[jm + sxy]);
-    double wwkp = 0.25 * (Wc + Wkp) * (Wc + Wkp);
-    double wwkm = 0.25 * (Wc + Wkm) * (Wc + Wkm);
-    double dwdxp = (Wip - Wc) * dxi, dwdxm = (Wc - Wim) * dxi;
-    double dwdyp = (Wjp - Wc) * dyi, dwdym = (Wc - Wjm) * dyi;
-    double dwdzp = (Wkp - Wc) * dzi, dwdzm = (Wc - Wkm) * dzi;
-    double adv = dxi * (-uwip + uwim) + visc * (dwdxp - dwdxm) * dxi + dyi * (-vwjp + vwjm) +
-                 visc * (dwdyp - dwdym) * dyi + dzi * (-wwkp + wwkm) + visc * (dwdzp - dwdzm) * dzi;
-    double buoy = 0.5 * g.beta * 9.81 * (C[c] + C[kp]);
-    az = buoy + adv;
-  }
-  {  // scalad, src/scal.f90:35-53
-    const double Cc = C[c], Cip = C[ip], Cim = C[im], Cjp = C[jp], Cjm = C[jm], Ckp = C[kp], Ckm = C[km];
-    double ucip = 0.5 * Uc * (Cip + Cc);
-    double ucim = 0.5 * U[im] * (Cc + Cim);
-    double vcjp = 0.5 * Vc * (Cjp + Cc);
-    double vcjm = 0.5 * V[jm] * (Cc + Cjm);
-    double wckp = 0.5 * Wc * (Ckp + Cc);
-    double wckm = 0.5 * W[km] * (Cc + Ckm);
-    double dcdxp = (Cip - Cc) * dxi, dcdxm = (Cc - Cim) * dxi;
-    double dcdyp = (Cjp - Cc) * dyi, dcdym = (Cc - Cjm) * dyi;
-    double dcdzp = (Ckp - Cc) * dzi, dcdzm = (Cc - Ckm) * dzi;
-    ac = dxi * (-ucip + ucim) + kappa * (dcdxp - dcdxm) * dxi + dyi * (-vcjp + vcjm) + kappa * (dcdyp - dcdym) * dyi +
-         dzi * (-wckp + wckm) + kappa * (dcdzp - dcdzm) * dzi;
-  }
-  if (STAGE == 1) {
-    dudt[c] = du0 + f1 * ax;
-    dvdt[c] = dv0 + f1 * ay;
-    dwdt[c] = dw0 + f1 * az;
-    dcdt[c] = C[c] + f1 * ac;
-    durk[c] = ax; dvrk[c] = ay; dwrk[c] = az; dcrk[c] = ac;
-  } else {
-    const double ru = durk[c], rv = dvrk[c], rw = dwrk[c], rc = dcrk[c];
-    dudt[c] = du0 + (f2 * ax - f3 * ru);
-    dvdt[c] = dv0 + (f2 * ay - f3 * rv);
-    dwdt[c] = dw0 + (f2 * az - f3 * rw);
-    dcdt[c] = C[c] + (f2 * ac - f3 * rc);
-    if (STAGE == 2) { durk[c] = ax; dvrk[c] = ay; dwrk[c] = az; dcrk[c] = ac; }
+  const double bq = 0.5 * g.beta * 9.81;
+  long c = g.at(i, j, k0);
+  // column window: values at k (c) and k-1 (m) carried up the column
+  double Uc = U[c], Ukm = U[c - sxy], Uim = U[c - 1];
+  double Vc = V[c], Vkm = V[c - sxy], Vjm = V[c - sx];
+  double Wc = W[c], Wkm = W[c - sxy], Wip_km = W[c + 1 - sxy], Wjp_km = W[c + sx - sxy];
+  double Cc = C[c], Ckm = C[c - sxy];
+  double Pc = P[c];
+  for (int k = k0; k <= k1; k++, c += sxy) {
+    const double Ukp = U[c + sxy], Uim_kp = U[c - 1 + sxy], Vkp = V[c + sxy], Vjm_kp = V[c - sx + sxy];
+    const double Wkp = W[c + sxy], Ckp = C[c + sxy], Pkp = P[c + sxy];
+    const double Uip = U[c + 1], Ujp = U[c + sx], Ujm = U[c - sx], Uimjp = U[c - 1 + sx];
+    const double Vip = V[c + 1], Vim = V[c - 1], Vjp = V[c + sx], Vipjm = V[c + 1 - sx];
+    const double Wip = W[c + 1], Wim = W[c - 1], Wjp = W[c + sx], Wjm = W[c - sx];
+    const double Cip = C[c + 1], Cim = C[c - 1], Cjp = C[c + sx], Cjm = C[c - sx];
+    const double Pip = P[c + 1], Pjp = P[c + sx];
+    // momxp/momyp/momzp (src/mom.f90:56-79,129-155,205-231)
+    const double gpx = -(dxi * (Pip - Pc));
+    const double gpy = -(dyi * (Pjp - Pc));
+    const double gpz = -(dzi * (Pkp - Pc));
+    const double du0 = Uc + f1 * gpx, dv0 = Vc + f1 * gpy, dw0 = Wc + f1 * gpz;
+    double ax, ay, az, ac;
+    {  // momxad, src/mom.f90:30-47
+      double uuip = 0.25 * (Uip + Uc) * (Uip + Uc);
+      double uuim = 0.25 * (Uim + Uc) * (Uim + Uc);
+      double uvjp = 0.25 * (Ujp + Uc) * (Vip + Vc);
+      double uvjm = 0.25 * (Ujm + Uc) * (Vipjm + Vjm);
+      double uwkp = 0.25 * (Ukp + Uc) * (Wip + Wc);
+      double uwkm = 0.25 * (Ukm + Uc) * (Wip_km + Wkm);
+      double dudxp = (Uip - Uc) * dxi, dudxm = (Uc - Uim) * dxi;
+      double dudyp = (Ujp - Uc) * dyi, dudym = (Uc - Ujm) * dyi;
+      double dudzp = (Ukp - Uc) * dzi, dudzm = (Uc - Ukm) * dzi;
+      ax = dxi * (-uuip + uuim) + visc * (dudxp - dudxm) * dxi + dyi * (-uvjp + uvjm) + visc * (dudyp - dudym) * dyi +
+           dzi * (-uwkp + uwkm) + visc * (dudzp - dudzm) * dzi;
+    }
+    {  // momyad, src/mom.f90:103-120
+      double uvip = 0.25 * (Uc + Ujp) * (Vc + Vip);
+      double uvim = 0.25 * (Uim + Uimjp) * (Vc + Vim);
+      double vvjp = 0.25 * (Vc + Vjp) * (Vc + Vjp);
+      double vvjm = 0.25 * (Vc + Vjm) * (Vc + Vjm);
+      double wvkp = 0.25 * (Wc + Wjp) * (Vkp + Vc);
+      double wvkm = 0.25 * (Wkm + Wjp_km) * (Vkm + Vc);
+      double dvdxp = (Vip - Vc) * dxi, dvdxm = (Vc - Vim) * dxi;
+      double dvdyp = (Vjp - Vc) * dyi, dvdym = (Vc - Vjm) * dyi;
+      double dvdzp = (Vkp - Vc) * dzi, dvdzm = (Vc - Vkm) * dzi;
+      ay = dxi * (-uvip + uvim) + visc * (dvdxp - dvdxm) * dxi + dyi * (-vvjp + vvjm) + visc * (dvdyp - dvdym) * dyi +
+           dzi * (-wvkp + wvkm) + visc * (dvdzp - dvdzm) * dzi;
+    }
+    {  // momzad, src/mom.f90:179-196; momzbuoy src/mom.f90:252 added first (src/rk3.f90:56-58)
+      double uwip = 0.25 * (Wc + Wip) * (Uc + Ukp);
+      double uwim = 0.25 * (Wc + Wim) * (Uim + Uim_kp);
+      double vwjp = 0.25 * (Wc + Wjp) * (Vc + Vkp);
+      double vwjm = 0.25 * (Wc + Wjm) * (Vjm + Vjm_kp);
+      double wwkp = 0.25 * (Wc + Wkp) * (Wc + Wkp);
+      double wwkm = 0.25 * (Wc + Wkm) * (Wc + Wkm);
+      double dwdxp = (Wip - Wc) * dxi, dwdxm = (Wc - Wim) * dxi;
+      double dwdyp = (Wjp - Wc) * dyi, dwdym = (Wc - Wjm) * dyi;
+      double dwdzp = (Wkp - Wc) * dzi, dwdzm = (Wc - Wkm) * dzi;
+      double adv = dxi * (-uwip + uwim) + visc * (dwdxp - dwdxm) * dxi + dyi * (-vwjp + vwjm) +
+                   visc * (dwdyp - dwdym) * dyi + dzi * (-wwkp + wwkm) + visc * (dwdzp - dwdzm) * dzi;
+      double buoy = bq * (Cc + Ckp);
+      az = buoy + adv;
+    }
+    {  // scalad, src/scal.f90:35-53
+      double ucip = 0.5 * Uc * (Cip + Cc);
+      double ucim = 0.5 * Uim * (Cc + Cim);
+      double vcjp = 0.5 * Vc * (Cjp + Cc);
+      double vcjm = 0.5 * Vjm * (Cc + Cjm);
+      double wckp = 0.5 * Wc * (Ckp + Cc);
+      double wckm = 0.5 * Wkm * (Cc + Ckm);
+      double dcdxp = (Cip - Cc) * dxi, dcdxm = (Cc - Cim) * dxi;
+      double dcdyp = (Cjp - Cc) * dyi, dcdym = (Cc - Cjm) * dyi;
+      double dcdzp = (Ckp - Cc) * dzi, dcdzm = (Cc - Ckm) * dzi;
+      ac = dxi * (-ucip + ucim) + kappa * (dcdxp - dcdxm) * dxi + dyi * (-vcjp + vcjm) + kappa * (dcdyp - dcdym) * dyi +
+           dzi * (-wckp + wckm) + kappa * (dcdzp - dcdzm) * dzi;
+    }
+    if (STAGE == 1) {
+      dudt[c] = du0 + f1 * ax;
+      dvdt[c] = dv0 + f1 * ay;
+      dwdt[c] = dw0 + f1 * az;
+      dcdt[c] = Cc + f1 * ac;
+      durk[c] = ax; dvrk[c] = ay; dwrk[c] = az; dcrk[c] = ac;
+    } else {
+      const double ru = durk[c], rv = dvrk[c], rw = dwrk[c], rc = dcrk[c];
+      dudt[c] = du0 + (f2 * ax - f3 * ru);
+      dvdt[c] = dv0 + (f2 * ay - f3 * rv);
+      dwdt[c] = dw0 + (f2 * az - f3 * rw);
+      dcdt[c] = Cc + (f2 * ac - f3 * rc);
+      if (STAGE == 2) { durk[c] = ax; dvrk[c] = ay; dwrk[c] = az; dcrk[c] = ac; }
+    }
+    // slide the column window up one plane
+    Ukm = Uc; Uc = Ukp; Uim = Uim_kp;
+    Vkm = Vc; Vc = Vkp; Vjm = Vjm_kp;
+    Wkm = Wc; Wc = Wkp; Wip_km = Wip; Wjp_km = Wjp;
+    Ckm = Cc; Cc = Ckp;
+    Pc = Pkp;
   }
 }
 
@@ -297,16 +319,18 @@ static dim3 cell_grid(const Grid &g) { return dim3((g.nx + BX - 1) / BX, (g.ny +
 int ipg_launch_rk(ipg_ctx *c, int rkiter) {
   const Grid &g = c->g;
   double **f = c->f;
-  dim3 blk(BX, BY);
-  if (rkiter == 1)
-    hipLaunchKernelGGL(k_rk<1>, cell_grid(g), blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],
-                       f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK], f[IPG_DCRK], c->dscal);
-  else if (rkiter == 2)
-    hipLaunchKernelGGL(k_rk<2>, cell_grid(g), blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],
-                       f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK], f[IPG_DCRK], c->dscal);
-  else
-    hipLaunchKernelGGL(k_rk<3>, cell_grid(g), blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],
-                       f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK], f[IPG_DCRK], c->dscal);
+  const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY;
+  const int kb = g.nz >= 128 ? 16 : (g.nz >= 32 ? 8 : 4);
+  const int ntz = (g.nz + kb - 1) / kb;
+  dim3 grid(ntx * nty * ntz), blk(BX * BY);
+#define IPG_RK_LAUNCH(S)                                                                                               \
+  hipLaunchKernelGGL(k_rk<S>, grid, blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],         \
+                     f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK],         \
+                     f[IPG_DCRK], c->dscal, ntx, nty, kb)
+  if (rkiter == 1) IPG_RK_LAUNCH(1);
+  else if (rkiter == 2) IPG_RK_LAUNCH(2);
+  else IPG_RK_LAUNCH(3);
+#undef IPG_RK_LAUNCH
   IPG_HIP(hipGetLastError());
   ipg_mark(c, "rk_stage");
   return 0;

@@ -140,134 +140,163 @@ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl
   }
 }
 
-// One IBM pass for one sphere (one block): for each velocity component the
-// sphere's box of (dudt + previous spread) is staged in LDS, every marker
-// gathers its 27-point stencil from it (src/interp_spread.f90:132-154) and
-// updates its force (src/forcing.f90:63-65 / 255-257); then the markers'
-// spread (src/interp_spread.f90:329-355) is summed in the same LDS box with
-// LDS atomics and the box is added to the global target once, cell by cell.
+// Marker preparation, once per substep (markers do not move during the IBM
+// block): the validity flag (the isout test of src/coordsfp.f90:55-65), the
+// unwrapped stencil base of each velocity component, the nine stencil weights
+// of each component and the rigid-body velocity at the marker
+// (src/intgr_nwtn_eulr.f90:1471-1476).  Layout (SoA over markers m):
+// mw[(comp*9 + q)*M + m] weights (q = 0..2 x, 3..5 y, 6..8 z), mb[(comp*3+d)*M + m]
+// bases, mv[comp*M + m] rigid velocity, mb[9*M + m] validity.
+__global__ __launch_bounds__(256) void k_ibm_prep(IbmArgs A, double *mw, int *mb, double *mv) {
+  const Grid &g = A.g;
+  const long M = (long)A.np * A.nl;
+  const long m = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const int p = (int)(m / A.nl), l = (int)(m - (long)p * A.nl);
+  const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
+  double xl, yl, zl;
+  const bool ok = marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl);
+  mb[9 * M + m] = ok ? 1 : 0;
+  if (!ok) return;
+  for (int comp = 0; comp < 3; comp++) {
+    CompStencil c;
+    comp_stencil(g, xl, yl, zl, comp, c);
+    mb[(comp * 3 + 0) * M + m] = c.i0;
+    mb[(comp * 3 + 1) * M + m] = c.j0;
+    mb[(comp * 3 + 2) * M + m] = c.k0;
+    for (int q = 0; q < 3; q++) {
+      mw[(comp * 9 + q) * M + m] = c.wx[q];
+      mw[(comp * 9 + 3 + q) * M + m] = c.wy[q];
+      mw[(comp * 9 + 6 + q) * M + m] = c.wz[q];
+    }
+  }
+  mv[m] = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
+  mv[M + m] = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
+  mv[2 * M + m] = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
+}
+
+// One IBM pass for one (sphere, velocity component) workgroup: the sphere's
+// box of (dudt + previous spread) is staged in LDS, every marker gathers its
+// 27-point stencil from it (src/interp_spread.f90:132-154) and updates its
+// force (src/forcing.f90:63-65 / 255-257); then the markers' spread
+// (src/interp_spread.f90:329-355) is summed in the same LDS box with LDS
+// atomics and the box is added to the global target once, cell by cell.
 // gather/spread select the halves; the last pass runs them as two launches
 // because its spread goes straight into dudt, which other spheres may still
 // be gathering from.
-__global__ __launch_bounds__(256) void k_ibm_pass(IbmArgs A, int pass, int gather, int spread) {
+__global__ __launch_bounds__(256) void k_ibm_pass(IbmArgs A, const double *__restrict__ mw, const int *__restrict__ mb,
+                                                  const double *__restrict__ mv, int pass, int gather, int spread) {
   extern __shared__ double box[];
   const Grid &g = A.g;
-  const int p = blockIdx.x;
+  const int p = blockIdx.x / 3, comp = blockIdx.x % 3;
   const long M = (long)A.np * A.nl;
   if (A.Kzero) {
     const long nplane = (long)g.nx * g.ny;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nplane; t += (long)gridDim.x * blockDim.x) A.Kzero[t] = 0.0;
   }
-  const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
-  const double up = A.pu[p], vp = A.pv[p], wp = A.pw[p], ox = A.pomx[p], oy = A.pomy[p], oz = A.pomz[p];
   const double dt = *A.dtp, dti = 1.0 / dt;
   const double c0 = (A.dVlagr / A.dVeul) * dt;
-  const Box b = sphere_box(A, xp, yp, zp);
+  const Box b = sphere_box(A, A.px[p], A.py[p], A.pz[p]);
   const int nbox = b.bx * b.by * b.bz;
-  for (int comp = 0; comp < 3; comp++) {
-    const double *F = A.fld[comp];
-    double *fl = A.fl + comp * M;
-    if (gather) {
-      const double *Gr = A.Gread[comp];
-      for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+  const double *F = A.fld[comp];
+  double *fl = A.fl + comp * M;
+  const long m0 = (long)p * A.nl;
+  const double *W = mw + (long)comp * 9 * M;
+  const int *I0 = mb + (long)comp * 3 * M, *J0 = I0 + M, *K0 = J0 + M, *OK = mb + 9 * M;
+  if (gather) {
+    const double *Gr = A.Gread[comp];
+    for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+      const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
+      const int wi = wrapany(b.bi0 + li, g.nx), wj = wrapany(b.bj0 + lj, g.ny), k = b.bk0 + lk;
+      double v = F[g.at(wi, wj, k)];
+      if (Gr) v = v + Gr[gidx(g, wi, wj, k)];
+      box[t] = v;
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
+      const long m = m0 + l;
+      if (!OK[m]) continue;
+      const int i0 = I0[m], j0 = J0[m], k0 = K0[m];
+      double wx[3], wy[3], wz[3];
+      for (int q = 0; q < 3; q++) { wx[q] = W[q * M + m]; wy[q] = W[(3 + q) * M + m]; wz[q] = W[(6 + q) * M + m]; }
+      double acc = 0.0;
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+          for (int i = 0; i < 3; i++) {
+            const int iu = i0 + i, ju = j0 + j, kk = k0 + k;
+            double f;
+            const int q = box_at(b, iu, ju, kk);
+            if (q >= 0) f = box[q];
+            else if (kk >= 1 && kk <= g.nz) {
+              const int wi = wrapany(iu, g.nx), wj = wrapany(ju, g.ny);
+              f = F[g.at(wi, wj, kk)];
+              if (A.Gread[comp]) f = f + A.Gread[comp][gidx(g, wi, wj, kk)];
+            } else if (kk == g.nz + 1 && comp == 2) {
+              f = A.Kread[(long)(wrapany(iu, g.nx) - 1) + (long)g.nx * (wrapany(ju, g.ny) - 1)];
+            } else f = 0.0;
+            acc = acc + f * wx[i] * wy[j] * wz[k];
+          }
+      const double ub = mv[comp * M + m];
+      if (pass == 0) fl[m] = (ub - acc) * dti;
+      else fl[m] = fl[m] + (ub - acc) * dti;
+    }
+    __syncthreads();
+  }
+  if (spread) {
+    double *Gw = A.Gspread[comp];
+    for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
+    __syncthreads();
+    for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
+      const long m = m0 + l;
+      if (!OK[m]) continue;
+      const int i0 = I0[m], j0 = J0[m], k0 = K0[m];
+      double wx[3], wy[3], wz[3];
+      for (int q = 0; q < 3; q++) { wx[q] = W[q * M + m]; wy[q] = W[(3 + q) * M + m]; wz[q] = W[(6 + q) * M + m]; }
+      const double fc = fl[m] * c0;
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+          for (int i = 0; i < 3; i++) {
+            const int iu = i0 + i, ju = j0 + j, kk = k0 + k;
+            const double v = fc * wx[i] * wy[j] * wz[k];
+            const int q = box_at(b, iu, ju, kk);
+            if (q >= 0) unsafeAtomicAdd(&box[q], v);
+            else if (kk >= 1 && kk <= g.nz) {
+              const int wi = wrapany(iu, g.nx), wj = wrapany(ju, g.ny);
+              if (Gw) unsafeAtomicAdd(&Gw[gidx(g, wi, wj, kk)], v);
+              else unsafeAtomicAdd(&A.fld[comp][g.at(wi, wj, kk)], v);
+            } else if (kk == g.nz + 1 && comp == 2) {
+              unsafeAtomicAdd(&A.Kspread[(long)(wrapany(iu, g.nx) - 1) + (long)g.nx * (wrapany(ju, g.ny) - 1)], v);
+            }
+          }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+      const double v = box[t];
+      if (v != 0.0) {
         const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
         const int wi = wrapany(b.bi0 + li, g.nx), wj = wrapany(b.bj0 + lj, g.ny), k = b.bk0 + lk;
-        double v = F[g.at(wi, wj, k)];
-        if (Gr) v = v + Gr[gidx(g, wi, wj, k)];
-        box[t] = v;
+        if (Gw) unsafeAtomicAdd(&Gw[gidx(g, wi, wj, k)], v);
+        else unsafeAtomicAdd(&A.fld[comp][g.at(wi, wj, k)], v);
       }
-      __syncthreads();
-      for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
-        double xl, yl, zl;
-        if (!marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
-        CompStencil c;
-        comp_stencil(g, xl, yl, zl, comp, c);
-        double acc = 0.0;
-        for (int k = 0; k < 3; k++)
-          for (int j = 0; j < 3; j++)
-            for (int i = 0; i < 3; i++) {
-              const int iu = c.i0 + i, ju = c.j0 + j, kk = c.k0 + k;
-              double f;
-              const int q = box_at(b, iu, ju, kk);
-              if (q >= 0) f = box[q];
-              else if (kk >= 1 && kk <= g.nz) {
-                const int wi = wrapany(iu, g.nx), wj = wrapany(ju, g.ny);
-                f = F[g.at(wi, wj, kk)];
-                if (A.Gread[comp]) f = f + A.Gread[comp][gidx(g, wi, wj, kk)];
-              } else if (kk == g.nz + 1 && comp == 2) {
-                f = A.Kread[(long)(wrapany(iu, g.nx) - 1) + (long)g.nx * (wrapany(ju, g.ny) - 1)];
-              } else f = 0.0;
-              acc = acc + f * c.wx[i] * c.wy[j] * c.wz[k];
-            }
-        // rigid-body velocity at the marker, src/intgr_nwtn_eulr.f90:1471-1476
-        double ub;
-        if (comp == 0) ub = up + oy * (zl - zp) - oz * (yl - yp);
-        else if (comp == 1) ub = vp + oz * (xl - xp) - ox * (zl - zp);
-        else ub = wp + ox * (yl - yp) - oy * (xl - xp);
-        const long m = (long)p * A.nl + l;
-        if (pass == 0) fl[m] = (ub - acc) * dti;
-        else fl[m] = fl[m] + (ub - acc) * dti;
-      }
-      __syncthreads();
     }
-    if (spread) {
-      double *Gw = A.Gspread[comp];
-      for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
-      __syncthreads();
-      for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
-        double xl, yl, zl;
-        if (!marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
-        CompStencil c;
-        comp_stencil(g, xl, yl, zl, comp, c);
-        const double fc = fl[(long)p * A.nl + l] * c0;
-        for (int k = 0; k < 3; k++)
-          for (int j = 0; j < 3; j++)
-            for (int i = 0; i < 3; i++) {
-              const int iu = c.i0 + i, ju = c.j0 + j, kk = c.k0 + k;
-              const double v = fc * c.wx[i] * c.wy[j] * c.wz[k];
-              const int q = box_at(b, iu, ju, kk);
-              if (q >= 0) unsafeAtomicAdd(&box[q], v);
-              else if (kk >= 1 && kk <= g.nz) {
-                const int wi = wrapany(iu, g.nx), wj = wrapany(ju, g.ny);
-                if (Gw) unsafeAtomicAdd(&Gw[gidx(g, wi, wj, kk)], v);
-                else unsafeAtomicAdd(&A.fld[comp][g.at(wi, wj, kk)], v);
-              } else if (kk == g.nz + 1 && comp == 2) {
-                unsafeAtomicAdd(&A.Kspread[(long)(wrapany(iu, g.nx) - 1) + (long)g.nx * (wrapany(ju, g.ny) - 1)], v);
-              }
-            }
-      }
-      __syncthreads();
-      for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
-        const double v = box[t];
-        if (v != 0.0) {
-          const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
-          const int wi = wrapany(b.bi0 + li, g.nx), wj = wrapany(b.bj0 + lj, g.ny), k = b.bk0 + lk;
-          if (Gw) unsafeAtomicAdd(&Gw[gidx(g, wi, wj, k)], v);
-          else unsafeAtomicAdd(&A.fld[comp][g.at(wi, wj, k)], v);
-        }
-      }
-      __syncthreads();
+  }
+  if (A.Gzero[comp]) {
+    double *Gz = A.Gzero[comp];
+    for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+      const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
+      Gz[gidx(g, wrapany(b.bi0 + li, g.nx), wrapany(b.bj0 + lj, g.ny), b.bk0 + lk)] = 0.0;
     }
-    if (A.Gzero[comp]) {
-      double *Gz = A.Gzero[comp];
-      for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
-        const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
-        Gz[gidx(g, wrapany(b.bi0 + li, g.nx), wrapany(b.bj0 + lj, g.ny), b.bk0 + lk)] = 0.0;
-      }
-      // stencil cells outside the box (not expected; kept exact)
-      for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
-        double xl, yl, zl;
-        if (!marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
-        CompStencil c;
-        comp_stencil(g, xl, yl, zl, comp, c);
-        for (int k = 0; k < 3; k++)
-          for (int j = 0; j < 3; j++)
-            for (int i = 0; i < 3; i++) {
-              const int iu = c.i0 + i, ju = c.j0 + j, kk = c.k0 + k;
-              if (box_at(b, iu, ju, kk) < 0 && kk >= 1 && kk <= g.nz)
-                Gz[gidx(g, wrapany(iu, g.nx), wrapany(ju, g.ny), kk)] = 0.0;
-            }
-      }
+    // stencil cells outside the box (not expected; kept exact)
+    for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
+      const long m = m0 + l;
+      if (!OK[m]) continue;
+      const int i0 = I0[m], j0 = J0[m], k0 = K0[m];
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+          for (int i = 0; i < 3; i++) {
+            const int iu = i0 + i, ju = j0 + j, kk = k0 + k;
+            if (box_at(b, iu, ju, kk) < 0 && kk >= 1 && kk <= g.nz) Gz[gidx(g, wrapany(iu, g.nx), wrapany(ju, g.ny), kk)] = 0.0;
+          }
     }
   }
 }
@@ -355,21 +384,26 @@ int ipg_launch_ibm(ipg_ctx *c) {
   // spreads into GA; pass 1 gathers dudt+GA and spreads into GB; pass 2
   // gathers dudt+GB (clearing GA), then spreads into dudt (clearing GB).
   // The k1 plane: K0 carries it between substeps, K1/K2 hold passes 0/1.
-  const dim3 grid(A.np), blk(256);
+  const long M = (long)A.np * A.nl;
+  double *mw = c->mw, *mv = c->mw + 27 * M;
+  int *mb = c->mb;
+  hipLaunchKernelGGL(k_ibm_prep, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, A, mw, mb, mv);
+  ipg_mark(c, "ibm_prep");
+  const dim3 grid(3 * A.np), blk(256);
   set(nullptr, GA, nullptr, c->K1[0], c->K1[1], nullptr);
-  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, 0, 1, 1);
+  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, mw, mb, mv, 0, 1, 1);
   ipg_mark(c, "ibm_pass0");
   set(GA, GB, nullptr, c->K1[1], c->K1[2], nullptr);
-  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, 1, 1, 1);
+  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, mw, mb, mv, 1, 1, 1);
   ipg_mark(c, "ibm_pass1");
   set(GB, nullptr, GA, c->K1[2], nullptr, c->K1[0]);
-  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, 2, 1, 0);
+  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, mw, mb, mv, 2, 1, 0);
   ipg_mark(c, "ibm_pass2_gather");
   set(nullptr, nullptr, GB, nullptr, c->K1[0], c->K1[1]);
-  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, 2, 0, 1);
+  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, mw, mb, mv, 2, 0, 1);
   ipg_mark(c, "ibm_pass2_spread");
   set(nullptr, nullptr, nullptr, nullptr, nullptr, c->K1[2]);
-  hipLaunchKernelGGL(k_ibm_totals, grid, blk, 0, c->stream, A, q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
+  hipLaunchKernelGGL(k_ibm_totals, dim3(A.np), blk, 0, c->stream, A, q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
                      q[IPG_PTORQX], q[IPG_PTORQY], q[IPG_PTORQZ], q[IPG_PTORQTHETA]);
   ipg_mark(c, "ibm_totals");
   IPG_HIP(hipGetLastError());

@@ -67,6 +67,8 @@ struct ipg_ctx {
   bool ibm_lds_set;
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
   double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl
+  double *mw;                     // [27+3][np*nl] marker stencil weights and rigid velocities
+  int *mb;                        // [10][np*nl] marker stencil bases and validity
   // particles
   PartState ps;
   double *op;                     // [24][np] state at substep start (src/intgr_nwtn_eulr.f90:77-118)
@@ -135,6 +137,7 @@ int ipg_launch_chkdiv(ipg_ctx *c, double *tot, double *mx);
 int ipg_poisson_setup(ipg_ctx *c);
 int ipg_poisson_free(ipg_ctx *c);
 int ipg_launch_poisson(ipg_ctx *c);
+int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter);
 int ipg_launch_ibm(ipg_ctx *c);
 int ipg_ibm_setup(ipg_ctx *c);
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type);

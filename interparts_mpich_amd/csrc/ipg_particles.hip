@@ -85,6 +85,14 @@ __global__ __launch_bounds__(256) void k_sphere_int(Grid g, PartConst pc, const 
           else { xmin = 0.0 + (double)(i - 1) * dx; xplus = 0.0 + (double)i * dx; }
           const double dzp2 = sqd(zplus - Z), dyp2 = sqd(yplus - Y), dxp2 = sqd(xplus - X);
           const double dist2 = dxp2 + dyp2 + dzp2;
+          // A cell whose nearest point lies beyond R has every corner outside:
+          // its weight sum1/sum2 is exactly 0 and it adds nothing (skipped).
+          {
+            const double ex = xmin > X ? xmin - X : (xplus < X ? X - xplus : 0.0);
+            const double ey = ymin > Y ? ymin - Y : (yplus < Y ? Y - yplus : 0.0);
+            const double ez = zmin > Z ? zmin - Z : (zplus < Z ? Z - zplus : 0.0);
+            if (ex * ex + ey * ey + ez * ez > R * R * (1.0 + 1e-9)) continue;
+          }
           const double fval = F[g.at(i, j, k)];
           bool full = dist2 < radin2;
           double wfrac = 1.0;

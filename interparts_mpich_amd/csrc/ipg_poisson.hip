@@ -10,6 +10,15 @@
 // eigenvalues and the Thomas recurrence are the reference's
 // (src/initsolver.f90:17-64, src/solver.f90:94-135), including the
 // singular kx=ky=0 branch (z == 0 -> p(kmax) = 0).
+//
+// The transforms are Stockham mixed-radix (4, 2, 3) FFTs in LDS.  Lengths
+// 16..512 with radices 2 and 3 are compiled with the length and the batch as
+// constants (all index arithmetic folds); other lengths use a generic
+// runtime-plan kernel.  The x pass keeps each sequence contiguous in LDS
+// (global rows load coalesced); the y pass interleaves the sequences
+// (element r of sequence s at r*B+s) so that the tile loads and every
+// butterfly access are bank-conflict free.  The x forward pass can compute the
+// right-hand side of src/fillps.f90 on the fly from dudt, dvdt, dwdt.
 #include "ipg_internal.h"
 #include <cmath>
 #include <vector>
@@ -27,10 +36,27 @@ __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_doub
 // (sign * i) * z
 __device__ __forceinline__ double2 cmuli(double2 z, int sign) { return make_double2(-sign * z.y, sign * z.x); }
 
-// Stockham mixed-radix (4, 2, 3) FFT of B sequences of length n held
-// back-to-back in LDS buffer x (y is the ping-pong partner).  tw[t] =
-// e^{-2 pi i t / n}; sign -1 forward, +1 backward (conjugated twiddles).
-// Returns the buffer holding the result.
+template <int R>
+__device__ __forceinline__ void butterfly(double2 *v, int sign) {
+  if (R == 4) {
+    double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    double2 s13 = cadd(v[1], v[3]), d13 = cmuli(csub(v[1], v[3]), sign);
+    v[0] = cadd(s02, s13); v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13); v[3] = csub(d02, d13);
+  } else if (R == 2) {
+    double2 a0 = v[0];
+    v[0] = cadd(a0, v[1]); v[1] = csub(a0, v[1]);
+  } else {  // R == 3
+    const double h = 0.8660254037844386467637231707529362;  // sqrt(3)/2
+    double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    double2 a0 = v[0];
+    double2 m = make_double2(a0.x - 0.5 * t1.x, a0.y - 0.5 * t1.y);
+    double2 r2 = cmuli(make_double2(h * t2.x, h * t2.y), sign);
+    v[0] = cadd(a0, t1); v[1] = cadd(m, r2); v[2] = csub(m, r2);
+  }
+}
+
+// ---- generic runtime-plan Stockham FFT (any length of 2^a 3^b) -------------
 __device__ double2 *lds_fft(double2 *x, double2 *y, int n, int B, const Radices &rd, const double2 *__restrict__ tw, int sign) {
   int Ns = 1;
   for (int s = 0; s < rd.n; s++) {
@@ -52,22 +78,9 @@ __device__ double2 *lds_fft(double2 *x, double2 *y, int n, int B, const Radices 
         }
         v[r] = a;
       }
-      if (R == 4) {
-        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-        double2 s13 = cadd(v[1], v[3]), d13 = cmuli(csub(v[1], v[3]), sign);
-        v[0] = cadd(s02, s13); v[2] = csub(s02, s13);
-        v[1] = cadd(d02, d13); v[3] = csub(d02, d13);
-      } else if (R == 2) {
-        double2 a0 = v[0];
-        v[0] = cadd(a0, v[1]); v[1] = csub(a0, v[1]);
-      } else {  // R == 3
-        const double h = 0.8660254037844386467637231707529362;  // sqrt(3)/2
-        double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-        double2 a0 = v[0];
-        double2 m = make_double2(a0.x - 0.5 * t1.x, a0.y - 0.5 * t1.y);
-        double2 r2 = cmuli(make_double2(h * t2.x, h * t2.y), sign);
-        v[0] = cadd(a0, t1); v[1] = cadd(m, r2); v[2] = csub(m, r2);
-      }
+      if (R == 4) butterfly<4>(v, sign);
+      else if (R == 2) butterfly<2>(v, sign);
+      else butterfly<3>(v, sign);
       const int idxD = (j / Ns) * Ns * R + jm;
       for (int r = 0; r < R; r++) ys[idxD + r * Ns] = v[r];
     }
@@ -78,23 +91,201 @@ __device__ double2 *lds_fft(double2 *x, double2 *y, int n, int B, const Radices 
   return x;
 }
 
-// Forward x transform of the Poisson right-hand side held in p's interior:
-// rows (j, j+1) of plane k form one complex sequence; the two half spectra are
-// split after the transform.  spec[(k*ny + j)*sp + m], m = 0..nx/2.
+// ---- compile-time plans -----------------------------------------------------
+template <int N> struct Plan;
+template <> struct Plan<16> { static constexpr int n = 2; static constexpr int r[8] = {4, 4}; };
+template <> struct Plan<24> { static constexpr int n = 3; static constexpr int r[8] = {4, 2, 3}; };
+template <> struct Plan<32> { static constexpr int n = 3; static constexpr int r[8] = {4, 4, 2}; };
+template <> struct Plan<48> { static constexpr int n = 3; static constexpr int r[8] = {4, 4, 3}; };
+template <> struct Plan<64> { static constexpr int n = 3; static constexpr int r[8] = {4, 4, 4}; };
+template <> struct Plan<96> { static constexpr int n = 4; static constexpr int r[8] = {4, 4, 2, 3}; };
+template <> struct Plan<128> { static constexpr int n = 4; static constexpr int r[8] = {4, 4, 4, 2}; };
+template <> struct Plan<192> { static constexpr int n = 4; static constexpr int r[8] = {4, 4, 4, 3}; };
+template <> struct Plan<256> { static constexpr int n = 4; static constexpr int r[8] = {4, 4, 4, 4}; };
+template <> struct Plan<384> { static constexpr int n = 5; static constexpr int r[8] = {4, 4, 4, 2, 3}; };
+template <> struct Plan<512> { static constexpr int n = 5; static constexpr int r[8] = {4, 4, 4, 4, 2}; };
+
+// B sequences of length N; INTER: element e of sequence s at e*B+s, else s*N+e.
+template <int N, int B, bool INTER>
+__device__ __forceinline__ int lidx(int s, int e) { return INTER ? e * B + s : s * N + e; }
+
+template <int N, int B, bool INTER, int S, int NS>
+__device__ __forceinline__ void fft_stages(double2 *&x, double2 *&y, const double2 *__restrict__ tw, int sign) {
+  if constexpr (S < Plan<N>::n) {
+    constexpr int R = Plan<N>::r[S];
+    constexpr int NB = N / R;
+    constexpr int STEP = N / (NS * R);
+    for (int t = threadIdx.x; t < B * NB; t += FFT_THREADS) {
+      const int seq = INTER ? (t % B) : (t / NB);
+      const int j = INTER ? (t / B) : (t % NB);
+      const int jm = j % NS;
+      double2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        double2 a = x[lidx<N, B, INTER>(seq, j + r * NB)];
+        if (NS > 1 && r > 0) {
+          double2 w = tw[r * jm * STEP];
+          if (sign > 0) w.y = -w.y;
+          a = cmul(a, w);
+        }
+        v[r] = a;
+      }
+      butterfly<R>(v, sign);
+      const int idxD = (j / NS) * NS * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; r++) y[lidx<N, B, INTER>(seq, idxD + r * NS)] = v[r];
+    }
+    __syncthreads();
+    double2 *tmp = x; x = y; y = tmp;
+    fft_stages<N, B, INTER, S + 1, NS * R>(x, y, tw, sign);
+  }
+}
+
+template <int N>
+constexpr int batch_of() {
+  int b = 1;
+  while (2 * b * N <= LDS_COMPLEX) b *= 2;
+  return b;
+}
+
+// Right-hand side of the Poisson equation at (i,j,k), src/fillps.f90:43-46.
+__device__ __forceinline__ double rhs_at(const Grid &g, const double *__restrict__ du, const double *__restrict__ dv,
+                                         const double *__restrict__ dw, long c, double dtidxi, double dtidyi, double dtidzi) {
+  return (dw[c] - dw[c - g.sxy]) * dtidzi + (dv[c] - dv[c - g.sx]) * dtidyi + (du[c] - du[c - 1]) * dtidxi;
+}
+
+// Forward x transform: rows (j, j+1) of plane k form one complex sequence;
+// the two half spectra are split after the transform.  Input is p's interior
+// (fused = 0) or the divergence of dudt computed on the fly (fused = 1).
+// spec[(k*ny + j)*sp + m], m = 0..nx/2.
+template <int N>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, const double *__restrict__ p, double2 *__restrict__ spec,
+                                                             long sp, const double2 *__restrict__ tw, int fused,
+                                                             const double *__restrict__ du, const double *__restrict__ dv,
+                                                             const double *__restrict__ dw, const double *__restrict__ dtp,
+                                                             double rkab) {
+  constexpr int B = batch_of<N>();
+  __shared__ double2 buf[2 * B * N];
+  const long pairs = (long)(g.ny / 2) * g.nz;
+  const long pair0 = (long)blockIdx.x * B;
+  double dtidxi = 0.0, dtidyi = 0.0, dtidzi = 0.0;
+  if (fused) {
+    const double dti = 1.0 / (rkab * *dtp);
+    dtidxi = dti * g.dxi; dtidyi = dti * g.dyi; dtidzi = dti * g.dzi;
+  }
+  for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
+    const int b = t / N, i = t % N;
+    const long pr = pair0 + b;
+    double2 v = make_double2(0.0, 0.0);
+    if (pr < pairs) {
+      const int k = (int)(pr / (g.ny / 2)) + 1, j = 2 * (int)(pr % (g.ny / 2)) + 1;
+      const long c = g.at(i + 1, j, k);
+      if (fused) v = make_double2(rhs_at(g, du, dv, dw, c, dtidxi, dtidyi, dtidzi), rhs_at(g, du, dv, dw, c + g.sx, dtidxi, dtidyi, dtidzi));
+      else v = make_double2(p[c], p[c + g.sx]);
+    }
+    buf[b * N + i] = v;
+  }
+  __syncthreads();
+  double2 *x = buf, *y = buf + B * N;
+  fft_stages<N, B, false, 0, 1>(x, y, tw, -1);
+  constexpr int NM = N / 2 + 1;
+  for (int t = threadIdx.x; t < B * NM; t += FFT_THREADS) {
+    const int b = t / NM, m = t % NM;
+    const long pr = pair0 + b;
+    if (pr >= pairs) continue;
+    const int k = (int)(pr / (g.ny / 2)), j = 2 * (int)(pr % (g.ny / 2));
+    double2 za = x[b * N + m];
+    double2 zb = x[b * N + (N - m) % N];
+    zb.y = -zb.y;
+    spec[((long)k * g.ny + j) * sp + m] = make_double2(0.5 * (za.x + zb.x), 0.5 * (za.y + zb.y));
+    spec[((long)k * g.ny + j + 1) * sp + m] = make_double2(0.5 * (za.y - zb.y), -0.5 * (za.x - zb.x));
+  }
+}
+
+// y transform of a tile of B x-modes of one z-plane, in place in spec.
+template <int N>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_y_t(Grid g, double2 *__restrict__ spec, long sp, int nmx,
+                                                         const double2 *__restrict__ tw, int sign) {
+  constexpr int B = batch_of<N>();
+  __shared__ double2 buf[2 * B * N];
+  const int tiles = (nmx + B - 1) / B;
+  const int k = blockIdx.x / tiles, m0 = (blockIdx.x % tiles) * B;
+  double2 *plane = spec + (long)k * g.ny * sp;
+  for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
+    const int jj = t / B, b = t % B;
+    buf[jj * B + b] = (m0 + b < nmx) ? plane[(long)jj * sp + m0 + b] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double2 *x = buf, *y = buf + B * N;
+  fft_stages<N, B, true, 0, 1>(x, y, tw, sign);
+  for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
+    const int jj = t / B, b = t % B;
+    if (m0 + b < nmx) plane[(long)jj * sp + m0 + b] = x[jj * B + b];
+  }
+}
+
+// Inverse x transform of two half spectra (rows j, j+1) into p's interior,
+// with the overall 1/(nx*ny) normalisation.
+template <int N>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, double *__restrict__ p, const double2 *__restrict__ spec,
+                                                             long sp, const double2 *__restrict__ tw, double scale) {
+  constexpr int B = batch_of<N>();
+  __shared__ double2 buf[2 * B * N];
+  const long pairs = (long)(g.ny / 2) * g.nz;
+  const long pair0 = (long)blockIdx.x * B;
+  constexpr int half = N / 2;
+  for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
+    const int b = t / N, m = t % N;
+    const long pr = pair0 + b;
+    double2 z = make_double2(0.0, 0.0);
+    if (pr < pairs) {
+      const int k = (int)(pr / (g.ny / 2)), j = 2 * (int)(pr % (g.ny / 2));
+      const double2 *ra = spec + ((long)k * g.ny + j) * sp, *rb = ra + sp;
+      double2 xa, xb;
+      if (m <= half) { xa = ra[m]; xb = rb[m]; }
+      else { xa = ra[N - m]; xb = rb[N - m]; xa.y = -xa.y; xb.y = -xb.y; }
+      z = make_double2(xa.x - xb.y, xa.y + xb.x);
+    }
+    buf[b * N + m] = z;
+  }
+  __syncthreads();
+  double2 *x = buf, *y = buf + B * N;
+  fft_stages<N, B, false, 0, 1>(x, y, tw, +1);
+  for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
+    const int b = t / N, i = t % N;
+    const long pr = pair0 + b;
+    if (pr >= pairs) continue;
+    const int k = (int)(pr / (g.ny / 2)) + 1, j = 2 * (int)(pr % (g.ny / 2)) + 1;
+    const long c = g.at(i + 1, j, k);
+    p[c] = x[b * N + i].x * scale;
+    p[c + g.sx] = x[b * N + i].y * scale;
+  }
+}
+
+// ---- generic kernels (runtime plan) -----------------------------------------
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd(Grid g, const double *__restrict__ p, double2 *__restrict__ spec,
-                                                           long sp, Radices rd, const double2 *__restrict__ tw) {
+                                                           long sp, Radices rd, const double2 *__restrict__ tw, int fused,
+                                                           const double *__restrict__ du, const double *__restrict__ dv,
+                                                           const double *__restrict__ dw, const double *__restrict__ dtp,
+                                                           double rkab) {
   __shared__ double2 buf[2 * LDS_COMPLEX];
   const int n = g.nx;
   const int RP = LDS_COMPLEX / n;
   const long pairs = (long)(g.ny / 2) * g.nz;
   const long pair0 = (long)blockIdx.x * RP;
   const int B = (int)min((long)RP, pairs - pair0);
+  double dtidxi = 0.0, dtidyi = 0.0, dtidzi = 0.0;
+  if (fused) {
+    const double dti = 1.0 / (rkab * *dtp);
+    dtidxi = dti * g.dxi; dtidyi = dti * g.dyi; dtidzi = dti * g.dzi;
+  }
   for (int t = threadIdx.x; t < B * n; t += blockDim.x) {
     const int b = t / n, i = t - b * n;
     const long pr = pair0 + b;
     const int k = (int)(pr / (g.ny / 2)) + 1, j = 2 * (int)(pr % (g.ny / 2)) + 1;
     const long c = g.at(i + 1, j, k);
-    buf[b * n + i] = make_double2(p[c], p[c + g.sx]);
+    if (fused) buf[b * n + i] = make_double2(rhs_at(g, du, dv, dw, c, dtidxi, dtidyi, dtidzi), rhs_at(g, du, dv, dw, c + g.sx, dtidxi, dtidyi, dtidzi));
+    else buf[b * n + i] = make_double2(p[c], p[c + g.sx]);
   }
   __syncthreads();
   double2 *res = lds_fft(buf, buf + LDS_COMPLEX, n, B, rd, tw, -1);
@@ -106,15 +297,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd(Grid g, const double 
     double2 za = res[b * n + m];
     double2 zb = res[b * n + (n - m) % n];
     zb.y = -zb.y;
-    double2 xa = make_double2(0.5 * (za.x + zb.x), 0.5 * (za.y + zb.y));
-    double2 xb = make_double2(0.5 * (za.y - zb.y), -0.5 * (za.x - zb.x));
-    spec[((long)k * g.ny + j) * sp + m] = xa;
-    spec[((long)k * g.ny + j + 1) * sp + m] = xb;
+    spec[((long)k * g.ny + j) * sp + m] = make_double2(0.5 * (za.x + zb.x), 0.5 * (za.y + zb.y));
+    spec[((long)k * g.ny + j + 1) * sp + m] = make_double2(0.5 * (za.y - zb.y), -0.5 * (za.x - zb.x));
   }
 }
 
-// y transform (forward sign -1 / backward +1) of a tile of TM x-modes of one
-// z-plane, in place in spec.
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_y(Grid g, double2 *__restrict__ spec, long sp, int nmx, Radices rd,
                                                        const double2 *__restrict__ tw, int sign) {
   __shared__ double2 buf[2 * LDS_COMPLEX];
@@ -136,57 +323,6 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_y(Grid g, double2 *__restri
   }
 }
 
-// Batched Thomas solve along z, src/solver.f90:90-135, one thread per
-// wavenumber pair (m, my); real and imaginary parts share the recurrence.
-__global__ __launch_bounds__(256) void k_thomas(Grid g, double2 *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
-                                                const double *__restrict__ a, const double *__restrict__ b,
-                                                const double *__restrict__ cc, const double *__restrict__ xrt,
-                                                const double *__restrict__ yrt) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const int my = blockIdx.y;
-  if (m >= nmx) return;
-  const int nz = g.nz;
-  const long plane = (long)g.ny * sp;
-  const long base = (long)my * sp + m;
-  const double xy = xrt[m] + yrt[my];
-  double z = 1.0 / (b[1] + xy);
-  double dprev = cc[1] * z;
-  dtri[base] = dprev;
-  double2 pp = spec[base];
-  pp.x = pp.x * z; pp.y = pp.y * z;
-  spec[base] = pp;
-  for (int k = 2; k <= nz - 1; k++) {
-    const long q = base + (long)(k - 1) * plane;
-    const double bb = b[k] + xy;
-    z = 1.0 / (bb - a[k] * dprev);
-    dprev = cc[k] * z;
-    dtri[q] = dprev;
-    double2 v = spec[q];
-    v.x = (v.x - a[k] * pp.x) * z;
-    v.y = (v.y - a[k] * pp.y) * z;
-    spec[q] = v; pp = v;
-  }
-  {
-    const long q = base + (long)(nz - 1) * plane;
-    const double bb = b[nz] + xy;
-    z = bb - a[nz] * dprev;
-    double2 v = spec[q];
-    if (z != 0.0) { v.x = (v.x - a[nz] * pp.x) / z; v.y = (v.y - a[nz] * pp.y) / z; }
-    else { v.x = 0.0; v.y = 0.0; }
-    spec[q] = v; pp = v;
-  }
-  for (int k = nz - 1; k >= 1; k--) {
-    const long q = base + (long)(k - 1) * plane;
-    const double d = dtri[q];
-    double2 v = spec[q];
-    v.x = v.x - d * pp.x;
-    v.y = v.y - d * pp.y;
-    spec[q] = v; pp = v;
-  }
-}
-
-// Inverse x transform of two half spectra (rows j, j+1) into p's interior,
-// with the overall 1/(nx*ny) normalisation.
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv(Grid g, double *__restrict__ p, const double2 *__restrict__ spec,
                                                            long sp, Radices rd, const double2 *__restrict__ tw, double scale) {
   __shared__ double2 buf[2 * LDS_COMPLEX];
@@ -218,6 +354,62 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv(Grid g, double *__res
   }
 }
 
+// Batched Thomas solve along z, src/solver.f90:90-135, one thread per
+// wavenumber pair (m, my); real and imaginary parts share the recurrence.
+// Loads of the next plane are issued ahead of the recurrence.
+__global__ __launch_bounds__(256) void k_thomas(Grid g, double2 *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
+                                                const double *__restrict__ a, const double *__restrict__ b,
+                                                const double *__restrict__ cc, const double *__restrict__ xrt,
+                                                const double *__restrict__ yrt) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int my = blockIdx.y;
+  if (m >= nmx) return;
+  const int nz = g.nz;
+  const long plane = (long)g.ny * sp;
+  const long base = (long)my * sp + m;
+  const double xy = xrt[m] + yrt[my];
+  double z = 1.0 / (b[1] + xy);
+  double dprev = cc[1] * z;
+  dtri[base] = dprev;
+  double2 pp = spec[base];
+  pp.x = pp.x * z; pp.y = pp.y * z;
+  spec[base] = pp;
+  double2 nxt = spec[base + plane];
+  for (int k = 2; k <= nz - 1; k++) {
+    const long q = base + (long)(k - 1) * plane;
+    double2 v = nxt;
+    nxt = spec[q + plane];
+    const double bb = b[k] + xy;
+    z = 1.0 / (bb - a[k] * dprev);
+    dprev = cc[k] * z;
+    dtri[q] = dprev;
+    v.x = (v.x - a[k] * pp.x) * z;
+    v.y = (v.y - a[k] * pp.y) * z;
+    spec[q] = v; pp = v;
+  }
+  {
+    const long q = base + (long)(nz - 1) * plane;
+    const double bb = b[nz] + xy;
+    z = bb - a[nz] * dprev;
+    double2 v = nxt;
+    if (z != 0.0) { v.x = (v.x - a[nz] * pp.x) / z; v.y = (v.y - a[nz] * pp.y) / z; }
+    else { v.x = 0.0; v.y = 0.0; }
+    spec[q] = v; pp = v;
+  }
+  long q = base + (long)(nz - 2) * plane;
+  double2 vn = spec[q];
+  double dn = dtri[q];
+  for (int k = nz - 1; k >= 1; k--) {
+    double2 v = vn;
+    const double d = dn;
+    if (k > 1) { vn = spec[q - plane]; dn = dtri[q - plane]; }
+    v.x = v.x - d * pp.x;
+    v.y = v.y - d * pp.y;
+    spec[q] = v; pp = v;
+    q -= plane;
+  }
+}
+
 bool plan_radices(int n, Radices &rd) {
   rd.n = 0;
   int m = n;
@@ -226,6 +418,54 @@ bool plan_radices(int n, Radices &rd) {
   while (m % 3 == 0) { rd.r[rd.n++] = 3; m /= 3; }
   return m == 1 && rd.n <= 16;
 }
+
+bool specialised(int n) {
+  switch (n) {
+    case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256: case 384: case 512: return true;
+  }
+  return false;
+}
+
+struct XArgs {
+  Grid g; double *p; double2 *spec; long sp; const double2 *tw; int fused;
+  const double *du, *dv, *dw, *dtp; double rkab; double scale;
+};
+
+template <int N>
+void launch_x_fwd(const XArgs &a, hipStream_t s) {
+  constexpr int B = batch_of<N>();
+  const long pairs = (long)(a.g.ny / 2) * a.g.nz;
+  hipLaunchKernelGGL(k_fft_x_fwd_t<N>, dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g, a.p, a.spec,
+                     a.sp, a.tw, a.fused, a.du, a.dv, a.dw, a.dtp, a.rkab);
+}
+template <int N>
+void launch_x_inv(const XArgs &a, hipStream_t s) {
+  constexpr int B = batch_of<N>();
+  const long pairs = (long)(a.g.ny / 2) * a.g.nz;
+  hipLaunchKernelGGL(k_fft_x_inv_t<N>, dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g, a.p, a.spec,
+                     a.sp, a.tw, a.scale);
+}
+template <int N>
+void launch_y(const Grid &g, double2 *spec, long sp, int nmx, const double2 *tw, int sign, hipStream_t s) {
+  constexpr int B = batch_of<N>();
+  const int tiles = (nmx + B - 1) / B;
+  hipLaunchKernelGGL(k_fft_y_t<N>, dim3(g.nz * tiles), dim3(FFT_THREADS), 0, s, g, spec, sp, nmx, tw, sign);
+}
+
+#define IPG_FFT_DISPATCH(n, CALL) \
+  switch (n) {                    \
+    case 16: CALL(16); break;     \
+    case 24: CALL(24); break;     \
+    case 32: CALL(32); break;     \
+    case 48: CALL(48); break;     \
+    case 64: CALL(64); break;     \
+    case 96: CALL(96); break;     \
+    case 128: CALL(128); break;   \
+    case 192: CALL(192); break;   \
+    case 256: CALL(256); break;   \
+    case 384: CALL(384); break;   \
+    case 512: CALL(512); break;   \
+  }
 
 }  // namespace
 
@@ -300,30 +540,61 @@ int ipg_poisson_free(ipg_ctx *c) {
   return 0;
 }
 
-int ipg_launch_poisson(ipg_ctx *c) {
+// fused_rkiter > 0: the x forward pass computes the right-hand side of
+// fillps (src/fillps.f90) itself from dudt/dvdt/dwdt instead of reading p.
+int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter) {
   const Grid &g = c->g;
   Poisson &P = c->ps2;
   Radices rx, ry;
   rx.n = P.nradx; ry.n = P.nrady;
   for (int q = 0; q < rx.n; q++) rx.r[q] = P.radx[q];
   for (int q = 0; q < ry.n; q++) ry.r[q] = P.rady[q];
+  XArgs xa{g, c->f[IPG_P], P.spec, P.sp, P.twx, fused_rkiter > 0 ? 1 : 0, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT],
+           c->dscal, fused_rkiter > 0 ? c->pc.rkab[fused_rkiter - 1] : 1.0, 1.0 / ((double)g.nx * (double)g.ny)};
   const long pairs = (long)(g.ny / 2) * g.nz;
   const int RPx = LDS_COMPLEX / g.nx;
   const int gx = (int)((pairs + RPx - 1) / RPx);
   const int TM = LDS_COMPLEX / g.ny;
   const int gy = g.nz * ((P.nmx + TM - 1) / TM);
-  hipLaunchKernelGGL(k_fft_x_fwd, dim3(gx), dim3(FFT_THREADS), 0, c->stream, g, c->f[IPG_P], P.spec, P.sp, rx, P.twx);
+  if (specialised(g.nx)) {
+#define CALLX(n) launch_x_fwd<n>(xa, c->stream)
+    IPG_FFT_DISPATCH(g.nx, CALLX)
+#undef CALLX
+  } else {
+    hipLaunchKernelGGL(k_fft_x_fwd, dim3(gx), dim3(FFT_THREADS), 0, c->stream, g, c->f[IPG_P], P.spec, P.sp, rx, P.twx,
+                       xa.fused, xa.du, xa.dv, xa.dw, xa.dtp, xa.rkab);
+  }
   ipg_mark(c, "fft_x_fwd");
-  hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, -1);
+  if (specialised(g.ny)) {
+#define CALLY(n) launch_y<n>(g, P.spec, P.sp, P.nmx, P.twy, -1, c->stream)
+    IPG_FFT_DISPATCH(g.ny, CALLY)
+#undef CALLY
+  } else {
+    hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, -1);
+  }
   ipg_mark(c, "fft_y_fwd");
   hipLaunchKernelGGL(k_thomas, dim3((P.nmx + 255) / 256, g.ny), dim3(256), 0, c->stream, g, P.spec, P.dtri, P.sp, P.nmx,
                      P.a, P.b, P.c, P.xrt, P.yrt);
   ipg_mark(c, "thomas");
-  hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, +1);
+  if (specialised(g.ny)) {
+#define CALLY(n) launch_y<n>(g, P.spec, P.sp, P.nmx, P.twy, +1, c->stream)
+    IPG_FFT_DISPATCH(g.ny, CALLY)
+#undef CALLY
+  } else {
+    hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, +1);
+  }
   ipg_mark(c, "fft_y_inv");
-  hipLaunchKernelGGL(k_fft_x_inv, dim3(gx), dim3(FFT_THREADS), 0, c->stream, g, c->f[IPG_P], P.spec, P.sp, rx, P.twx,
-                     1.0 / ((double)g.nx * (double)g.ny));
+  if (specialised(g.nx)) {
+#define CALLX(n) launch_x_inv<n>(xa, c->stream)
+    IPG_FFT_DISPATCH(g.nx, CALLX)
+#undef CALLX
+  } else {
+    hipLaunchKernelGGL(k_fft_x_inv, dim3(gx), dim3(FFT_THREADS), 0, c->stream, g, c->f[IPG_P], P.spec, P.sp, rx, P.twx,
+                       xa.scale);
+  }
   ipg_mark(c, "fft_x_inv");
   IPG_HIP(hipGetLastError());
   return 0;
 }
+
+int ipg_launch_poisson(ipg_ctx *c) { return ipg_launch_poisson_ex(c, 0); }
