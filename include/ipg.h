@@ -92,6 +92,17 @@ int ipg_ibm(ipg_ctx *ctx);                            /* IBM block, src/main.f90
                                                          eulr2lagr src/interp_spread.f90:12,
                                                          complagrforces/updtlagrforces src/forcing.f90:9,189,
                                                          lagr2eulr src/interp_spread.f90:230 */
+/* the IBM block split as the reference calls it (src/main.f90:299-335):
+ * ipg_ibm_begin replaces the copy dudtf = dudt (main.f90:301-303);
+ * per pass ibmiter = 0..2: ipg_eulr2lagr (interp_spread.f90:12, which here
+ * also updates the marker forces), ipg_lagrforces (the per-sphere sums of
+ * complagrforces forcing.f90:9 / updtlagrforces forcing.f90:189),
+ * ipg_lagr2eulr (interp_spread.f90:230; the last pass writes dudt directly,
+ * replacing main.f90:332-334) */
+int ipg_ibm_begin(ipg_ctx *ctx);
+int ipg_eulr2lagr(ipg_ctx *ctx, int ibmiter);
+int ipg_lagrforces(ipg_ctx *ctx, int ibmiter);
+int ipg_lagr2eulr(ipg_ctx *ctx, int ibmiter);
 int ipg_bounduvw(ipg_ctx *ctx, int which);            /* src/bound.f90:11; which 0 = unew.., 1 = dudt.. */
 int ipg_boundp(ipg_ctx *ctx, int which);              /* src/bound.f90:82; which 0 = pnew, 1 = p */
 int ipg_boundscal(ipg_ctx *ctx);                      /* src/bound.f90:109 on cnew */
@@ -104,6 +115,12 @@ int ipg_intgr_nwtn_eulr(ipg_ctx *ctx, int rkiter);    /* src/intgr_nwtn_eulr.f90
 int ipg_substep(ipg_ctx *ctx, int rkiter);            /* src/main.f90:268-414, whole substep */
 int ipg_chkdt(ipg_ctx *ctx, double *dtmax);           /* src/chkdt.f90:6 */
 int ipg_chkdiv(ipg_ctx *ctx, double *divtot, double *divmax); /* src/chkdiv.f90:7 */
+
+/* ---- seeded inputs (replaces random_number in src/init.f90:287-305,
+ * src/initparticles.f90:36-47): U[0,1) = splitmix64(seed*K1 + stream*K2 + n)
+ * >> 11 * 2^-53 for n = first .. first+count-1 ---------------------------- */
+int ipg_uniform(unsigned long long seed, unsigned long long stream, unsigned long long first, long long count,
+                double *out);
 
 /* ---- runtime controls (no reference counterpart) --------------------- */
 int ipg_sync(ipg_ctx *ctx);                           /* wait for the context's stream */

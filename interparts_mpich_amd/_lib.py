@@ -22,7 +22,8 @@ PQ = {n: i for i, n in enumerate(PARTS)}
 # every symbol include/ipg.h declares
 EXPORTS = ["ipg_init", "ipg_finalize", "ipg_last_error", "ipg_nl_formula", "ipg_field_put", "ipg_field_get",
            "ipg_particles_put", "ipg_particles_get", "ipg_set_dt", "ipg_get_dt", "ipg_initsolver",
-           "ipg_initparticles", "ipg_coordsfp", "ipg_rk", "ipg_ibm", "ipg_bounduvw", "ipg_boundp",
+           "ipg_initparticles", "ipg_coordsfp", "ipg_rk", "ipg_ibm", "ipg_ibm_begin", "ipg_eulr2lagr",
+           "ipg_lagrforces", "ipg_lagr2eulr", "ipg_uniform", "ipg_bounduvw", "ipg_boundp",
            "ipg_boundscal", "ipg_fillps", "ipg_solver2d", "ipg_correc", "ipg_pnew_update",
            "ipg_intgr_over_sphere", "ipg_intgr_nwtn_eulr", "ipg_substep", "ipg_chkdt", "ipg_chkdiv",
            "ipg_sync", "ipg_use_graphs", "ipg_substep_bytes", "ipg_substep_phases"]
@@ -70,6 +71,11 @@ def load():
         "ipg_coordsfp": ([vp, vp, vp, d], ip),
         "ipg_rk": ([vp, ip], ip),
         "ipg_ibm": ([vp], ip),
+        "ipg_ibm_begin": ([vp], ip),
+        "ipg_eulr2lagr": ([vp, ip], ip),
+        "ipg_lagrforces": ([vp, ip], ip),
+        "ipg_lagr2eulr": ([vp, ip], ip),
+        "ipg_uniform": ([C.c_ulonglong, C.c_ulonglong, C.c_ulonglong, C.c_longlong, vp], ip),
         "ipg_bounduvw": ([vp, ip], ip),
         "ipg_boundp": ([vp, ip], ip),
         "ipg_boundscal": ([vp], ip),

@@ -4,6 +4,7 @@
 // hipGraph per RK stage.
 #include "ipg_internal.h"
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -120,6 +121,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
   std::memset((void *)c, 0, sizeof(ipg_ctx));
   c->prm = *prm;
   derived(c);
+  if (const char *e = std::getenv("IPG_TUNE_RK_KB")) c->tune_rk_kb = std::atoi(e);
   auto fail = [&](int code) { release(c); delete c; return code; };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { ipg_set_error("stream create failed"); return fail(1); }
   const Grid &g = c->g;
@@ -289,6 +291,54 @@ int ipg_ibm(ipg_ctx *c) {
   if (!check_ctx(c)) return 1;
   if (ipg_launch_ibm(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_ibm_begin(ipg_ctx *c) {
+  if (!check_ctx(c)) return 1;
+  if (ipg_launch_ibm_begin(c)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_eulr2lagr(ipg_ctx *c, int ibmiter) {
+  if (!check_ctx(c)) return 1;
+  if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("eulr2lagr: ibmiter must be 0..2"); return 1; }
+  if (ipg_launch_ibm_pass(c, ibmiter, 1, 0)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_lagrforces(ipg_ctx *c, int ibmiter) {
+  if (!check_ctx(c)) return 1;
+  if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("lagrforces: ibmiter must be 0..2"); return 1; }
+  if (ipg_launch_ibm_totals(c, ibmiter == 2)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ipg_lagr2eulr(ipg_ctx *c, int ibmiter) {
+  if (!check_ctx(c)) return 1;
+  if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("lagr2eulr: ibmiter must be 0..2"); return 1; }
+  if (ipg_launch_ibm_pass(c, ibmiter, 0, 1)) return 1;
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// splitmix64 counter generator of the seeded inputs (identical to
+// interparts_mpich_amd/inputs.py uniform())
+int ipg_uniform(unsigned long long seed, unsigned long long stream, unsigned long long first, long long count,
+                double *out) {
+  if (!out || count < 0) { ipg_set_error("uniform: bad arguments"); return 1; }
+  const unsigned long long K1 = 0x9E3779B97F4A7C15ULL, K2 = 0xD1B54A32D192ED03ULL;
+  for (long long n = 0; n < count; n++) {
+    unsigned long long x = seed * K1 + stream * K2 + (first + (unsigned long long)n);
+    x = x + K1;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    x = x ^ (x >> 31);
+    out[n] = (double)(x >> 11) * (1.0 / 9007199254740992.0);
+  }
   return 0;
 }
 

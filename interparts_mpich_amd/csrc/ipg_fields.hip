@@ -147,19 +147,19 @@ enum { KIND_UV = 0, KIND_W = 1, KIND_P = 2, KIND_C = 3 };
 struct BoundJob { double *f[5]; int kind[5]; int nf; };
 
 __global__ void k_bound(Grid g, BoundJob job) {
-  const long nx = g.nx, ny = g.ny, nz = g.nz;
-  const long nplane = (nx + 2) * (ny + 2), nrow = (nx + 2) * nz, ncol = ny * nz, nwall = nx * ny;
-  const long total = 2 * nplane + 2 * nrow + 2 * ncol + nwall;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int nplane = (nx + 2) * (ny + 2), nrow = (nx + 2) * nz, ncol = ny * nz, nwall = nx * ny;
+  const int total = 2 * nplane + 2 * nrow + 2 * ncol + nwall;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     int i, j, k; bool wallplane = false;
-    long r = t;
-    if (r < 2 * nplane) { k = r < nplane ? 0 : (int)nz + 1; r %= nplane; i = (int)(r % (nx + 2)); j = (int)(r / (nx + 2)); }
+    int r = t;
+    if (r < 2 * nplane) { k = r < nplane ? 0 : nz + 1; r = r < nplane ? r : r - nplane; i = r % (nx + 2); j = r / (nx + 2); }
     else if ((r -= 2 * nplane) < 2 * nrow) {
-      j = r < nrow ? 0 : (int)ny + 1; r %= nrow; i = (int)(r % (nx + 2)); k = (int)(r / (nx + 2)) + 1;
+      j = r < nrow ? 0 : ny + 1; r = r < nrow ? r : r - nrow; i = r % (nx + 2); k = r / (nx + 2) + 1;
     } else if ((r -= 2 * nrow) < 2 * ncol) {
-      i = r < ncol ? 0 : (int)nx + 1; r %= ncol; j = (int)(r % ny) + 1; k = (int)(r / ny) + 1;
+      i = r < ncol ? 0 : nx + 1; r = r < ncol ? r : r - ncol; j = r % ny + 1; k = r / ny + 1;
     } else {
-      r -= 2 * ncol; i = (int)(r % nx) + 1; j = (int)(r / nx) + 1; k = (int)nz; wallplane = true;
+      r -= 2 * ncol; i = r % nx + 1; j = r / nx + 1; k = nz; wallplane = true;
     }
     const int ii = i == 0 ? (int)nx : (i == nx + 1 ? 1 : i);
     const int jj = j == 0 ? (int)ny : (j == ny + 1 ? 1 : j);
@@ -320,7 +320,7 @@ int ipg_launch_rk(ipg_ctx *c, int rkiter) {
   const Grid &g = c->g;
   double **f = c->f;
   const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY;
-  const int kb = g.nz >= 128 ? 16 : (g.nz >= 32 ? 8 : 4);
+  const int kb = c->tune_rk_kb > 0 ? c->tune_rk_kb : 1;
   const int ntz = (g.nz + kb - 1) / kb;
   dim3 grid(ntx * nty * ntz), blk(BX * BY);
 #define IPG_RK_LAUNCH(S)                                                                                               \
@@ -352,7 +352,8 @@ int ipg_launch_bound(ipg_ctx *c, int mask) {
   }
   if (job.nf == 0) return 0;
   const Grid &g = c->g;
-  long total = 2L * (g.nx + 2) * (g.ny + 2) + 2L * (g.nx + 2) * g.nz + 2L * g.ny * g.nz + (long)g.nx * g.ny;
+  const long total = 2L * (g.nx + 2) * (g.ny + 2) + 2L * (g.nx + 2) * g.nz + 2L * g.ny * g.nz + (long)g.nx * g.ny;
+  if (total > 2000000000L) { ipg_set_error("bound: grid too large"); return 1; }
   int blocks = (int)((total + 255) / 256);
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(k_bound, dim3(blocks), dim3(256), 0, c->stream, g, job);

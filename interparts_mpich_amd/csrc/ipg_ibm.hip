@@ -354,58 +354,107 @@ int ipg_ibm_setup(ipg_ctx *c) {
   return 0;
 }
 
-int ipg_launch_ibm(ipg_ctx *c) {
-  if (c->pc.np <= 0) return 0;
-  if (!c->markers_ready) { ipg_set_error("ibm: ipg_coordsfp has not been called"); return 1; }
-  IbmArgs A{};
+namespace {
+
+// Common arguments of the IBM launches for this context.
+bool ibm_args(ipg_ctx *c, IbmArgs &A, size_t &lds) {
+  if (!c->markers_ready || !c->ibm_lds_set) { ipg_set_error("ibm: ipg_coordsfp has not been called"); return false; }
+  A = IbmArgs{};
   A.g = c->g; A.np = c->pc.np; A.nl = c->pc.nl; A.dVlagr = c->pc.dVlagr; A.dVeul = c->pc.dVeul; A.radfp = c->pc.radfp;
   const double r2 = 2.0 * c->pc.radfp;
   A.bmx = (int)std::ceil(r2 * c->g.dxi) + 7;
   A.bmy = (int)std::ceil(r2 * c->g.dyi) + 7;
   A.bmz = (int)std::ceil(r2 * c->g.dzi) + 7;
-  const size_t lds = (size_t)A.bmx * A.bmy * A.bmz * sizeof(double);
-  if (!c->ibm_lds_set) { ipg_set_error("ibm: ipg_coordsfp has not set up the sphere boxes"); return 1; }
+  lds = (size_t)A.bmx * A.bmy * A.bmz * sizeof(double);
   double **q = c->ps.q;
   A.px = q[IPG_PX]; A.py = q[IPG_PY]; A.pz = q[IPG_PZ]; A.pu = q[IPG_PU]; A.pv = q[IPG_PV]; A.pw = q[IPG_PW];
   A.pomx = q[IPG_POMX]; A.pomy = q[IPG_POMY]; A.pomz = q[IPG_POMZ]; A.pphi = q[IPG_PPHI];
   A.mk = c->mk; A.fl = c->fl;
   A.fld[0] = c->f[IPG_DUDT]; A.fld[1] = c->f[IPG_DVDT]; A.fld[2] = c->f[IPG_DWDT];
   A.dtp = c->dscal;
-  double **GA = c->G[0], **GB = c->G[1];
-  auto set = [&](double **rd, double **sp, double **zr, double *kr, double *ks, double *kz) {
-    for (int d = 0; d < 3; d++) {
-      A.Gread[d] = rd ? rd[d] : nullptr;
-      A.Gspread[d] = sp ? sp[d] : nullptr;
-      A.Gzero[d] = zr ? zr[d] : nullptr;
-    }
-    A.Kread = kr; A.Kspread = ks; A.Kzero = kz;
-  };
-  // Fixed roles (a captured graph replays them): pass 0 gathers dudt and
-  // spreads into GA; pass 1 gathers dudt+GA and spreads into GB; pass 2
-  // gathers dudt+GB (clearing GA), then spreads into dudt (clearing GB).
-  // The k1 plane: K0 carries it between substeps, K1/K2 hold passes 0/1.
+  return true;
+}
+
+// Buffer roles of pass `pass` (fixed, so a captured graph replays them):
+// pass 0 gathers dudt and spreads into GA; pass 1 gathers dudt+GA and spreads
+// into GB; pass 2 gathers dudt+GB (clearing GA), then spreads into dudt
+// (clearing GB).  The k1 plane: K0 carries it between substeps, K1/K2 hold
+// passes 0/1; K2 is cleared by the last force sums.
+void ibm_roles(ipg_ctx *c, IbmArgs &A, int pass, int gather, int spread) {
+  double **GA = c->G[0], **GB = c->G[1], **K = c->K1;
+  double **rd = nullptr, **sp = nullptr, **zr = nullptr;
+  double *kr = nullptr, *ks = nullptr, *kz = nullptr;
+  if (pass == 0) { kr = K[0]; sp = GA; ks = K[1]; }
+  else if (pass == 1) { rd = GA; kr = K[1]; sp = GB; ks = K[2]; }
+  else {
+    if (gather) { rd = GB; kr = K[2]; zr = GA; kz = K[0]; }
+    else { ks = K[0]; zr = GB; kz = K[1]; }
+  }
+  if (!gather) { rd = nullptr; kr = nullptr; }
+  if (!spread) { sp = nullptr; ks = nullptr; }
+  for (int d = 0; d < 3; d++) {
+    A.Gread[d] = rd ? rd[d] : nullptr;
+    A.Gspread[d] = sp ? sp[d] : nullptr;
+    A.Gzero[d] = zr ? zr[d] : nullptr;
+  }
+  A.Kread = kr; A.Kspread = ks; A.Kzero = kz;
+}
+
+}  // namespace
+
+// src/main.f90:301-303 (dudtf = dudt): the marker stencils of this substep.
+int ipg_launch_ibm_begin(ipg_ctx *c) {
+  if (c->pc.np <= 0) return 0;
+  IbmArgs A; size_t lds;
+  if (!ibm_args(c, A, lds)) return 1;
   const long M = (long)A.np * A.nl;
-  double *mw = c->mw, *mv = c->mw + 27 * M;
-  int *mb = c->mb;
-  hipLaunchKernelGGL(k_ibm_prep, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, A, mw, mb, mv);
+  hipLaunchKernelGGL(k_ibm_prep, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, c->stream, A, c->mw, c->mb, c->mw + 27 * M);
   ipg_mark(c, "ibm_prep");
-  const dim3 grid(3 * A.np), blk(256);
-  set(nullptr, GA, nullptr, c->K1[0], c->K1[1], nullptr);
-  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, mw, mb, mv, 0, 1, 1);
-  ipg_mark(c, "ibm_pass0");
-  set(GA, GB, nullptr, c->K1[1], c->K1[2], nullptr);
-  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, mw, mb, mv, 1, 1, 1);
-  ipg_mark(c, "ibm_pass1");
-  set(GB, nullptr, GA, c->K1[2], nullptr, c->K1[0]);
-  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, mw, mb, mv, 2, 1, 0);
-  ipg_mark(c, "ibm_pass2_gather");
-  set(nullptr, nullptr, GB, nullptr, c->K1[0], c->K1[1]);
-  hipLaunchKernelGGL(k_ibm_pass, grid, blk, lds, c->stream, A, mw, mb, mv, 2, 0, 1);
-  ipg_mark(c, "ibm_pass2_spread");
-  set(nullptr, nullptr, nullptr, nullptr, nullptr, c->K1[2]);
-  hipLaunchKernelGGL(k_ibm_totals, dim3(A.np), blk, 0, c->stream, A, q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+// One IBM pass or half pass: gather (+ marker force update) and/or spread.
+int ipg_launch_ibm_pass(ipg_ctx *c, int pass, int gather, int spread) {
+  if (c->pc.np <= 0) return 0;
+  IbmArgs A; size_t lds;
+  if (!ibm_args(c, A, lds)) return 1;
+  if (pass == 2 && gather && spread) { ipg_set_error("ibm: the last pass gathers and spreads in two launches"); return 1; }
+  ibm_roles(c, A, pass, gather, spread);
+  const long M = (long)A.np * A.nl;
+  hipLaunchKernelGGL(k_ibm_pass, dim3(3 * A.np), dim3(256), lds, c->stream, A, c->mw, c->mb, c->mw + 27 * M, pass, gather,
+                     spread);
+  static const char *names[3][2] = {{"ibm_pass0_spread", "ibm_pass0"}, {"ibm_pass1_spread", "ibm_pass1"},
+                                    {"ibm_pass2_spread", "ibm_pass2_gather"}};
+  ipg_mark(c, gather ? (spread ? names[pass][1] : (pass == 2 ? names[2][1] : "ibm_gather")) : names[pass][0]);
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+// Per-sphere force and torque sums (src/forcing.f90:66-89, 258-287); after the
+// last pass this also clears the pass-1 k1 plane.
+int ipg_launch_ibm_totals(ipg_ctx *c, int last) {
+  if (c->pc.np <= 0) return 0;
+  IbmArgs A; size_t lds;
+  if (!ibm_args(c, A, lds)) return 1;
+  for (int d = 0; d < 3; d++) { A.Gread[d] = A.Gspread[d] = A.Gzero[d] = nullptr; }
+  A.Kread = A.Kspread = nullptr;
+  A.Kzero = last ? c->K1[2] : nullptr;
+  double **q = c->ps.q;
+  hipLaunchKernelGGL(k_ibm_totals, dim3(A.np), dim3(256), 0, c->stream, A, q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
                      q[IPG_PTORQX], q[IPG_PTORQY], q[IPG_PTORQZ], q[IPG_PTORQTHETA]);
   ipg_mark(c, "ibm_totals");
   IPG_HIP(hipGetLastError());
   return 0;
+}
+
+// The whole IBM block of one substep, src/main.f90:296-338.
+int ipg_launch_ibm(ipg_ctx *c) {
+  if (c->pc.np <= 0) return 0;
+  if (ipg_launch_ibm_begin(c)) return 1;
+  if (ipg_launch_ibm_pass(c, 0, 1, 1)) return 1;
+  if (ipg_launch_ibm_pass(c, 1, 1, 1)) return 1;
+  if (ipg_launch_ibm_pass(c, 2, 1, 0)) return 1;
+  if (ipg_launch_ibm_pass(c, 2, 0, 1)) return 1;
+  return ipg_launch_ibm_totals(c, 1);
 }

@@ -65,6 +65,8 @@ struct ipg_ctx {
   double *G[2][3];                // spread force buffers, interior (nx*ny*nz), modular x,y
   double *K1[3];                  // dwdtf(.,.,k1) planes (nx*ny), src/interp_spread.f90:100 (no upper clamp)
   bool ibm_lds_set;
+  // launch shape: z planes per RK-stage workgroup (IPG_TUNE_RK_KB at init; default 1)
+  int tune_rk_kb;
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
   double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl
   double *mw;                     // [27+3][np*nl] marker stencil weights and rigid velocities
@@ -140,6 +142,9 @@ int ipg_launch_poisson(ipg_ctx *c);
 int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter);
 int ipg_launch_ibm(ipg_ctx *c);
 int ipg_ibm_setup(ipg_ctx *c);
+int ipg_launch_ibm_begin(ipg_ctx *c);
+int ipg_launch_ibm_pass(ipg_ctx *c, int pass, int gather, int spread);
+int ipg_launch_ibm_totals(ipg_ctx *c, int last);
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type);
 int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter);
 int ipg_launch_nbflags(ipg_ctx *c);

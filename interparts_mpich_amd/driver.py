@@ -93,6 +93,18 @@ class Simulation:
     def ibm(self):
         check(self.lib.ipg_ibm(self.ctx), "ibm")
 
+    def ibm_begin(self):
+        check(self.lib.ipg_ibm_begin(self.ctx), "ibm_begin")
+
+    def eulr2lagr(self, ibmiter):
+        check(self.lib.ipg_eulr2lagr(self.ctx, ibmiter), "eulr2lagr")
+
+    def lagrforces(self, ibmiter):
+        check(self.lib.ipg_lagrforces(self.ctx, ibmiter), "lagrforces")
+
+    def lagr2eulr(self, ibmiter):
+        check(self.lib.ipg_lagr2eulr(self.ctx, ibmiter), "lagr2eulr")
+
     def bounduvw(self, which=0):
         check(self.lib.ipg_bounduvw(self.ctx, which), "bounduvw")
 

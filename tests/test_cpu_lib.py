@@ -33,3 +33,16 @@ def test_nl_formula_matches_host_model():
     assert nl_formula(64, 1.0, 4 / 64) == 173
     assert nl_formula(256, 1.0, 6 / 256) == 409
     assert nl_formula(512, 1.0, 8 / 512) == 746
+
+
+def test_uniform_matches_host_generator():
+    """ipg_uniform (C) and inputs.uniform (numpy) draw the same doubles."""
+    import numpy as np
+    from interparts_mpich_amd.inputs import uniform
+    lib = _lib.load()
+    for seed, stream, first, count in ((1, 0, 0, 1000), (7, 101, 12345, 257), (2**40 + 3, 200, 2**33, 64)):
+        out = np.empty(count)
+        assert lib.ipg_uniform(seed, stream, first, count, out.ctypes.data_as(ctypes.c_void_p)) == 0
+        ref = uniform(seed, stream, np.arange(first, first + count, dtype=np.uint64))
+        assert np.array_equal(out, ref)
+    assert lib.ipg_uniform(1, 0, 0, -1, None) != 0
