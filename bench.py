@@ -44,6 +44,18 @@ def kernel_bytes(case, name, rkiter):
     return None if w is None else 8.0 * w
 
 
+def copy_probe(mib):
+    """A plain copy kernel of known size: reads and writes mib MiB per launch."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,15 +63,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=30, help="untimed substeps")
     ap.add_argument("--config", default="c2_256")
     ap.add_argument("--no-graphs", action="store_true")
-    ap.add_argument("--cpu-substeps", type=int, default=3, help="oracle substeps in the CPU-baseline sample")
+    ap.add_argument("--cpu-substeps", type=int, default=12, help="oracle substeps in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (calibrates the HBM "
+                         "byte counters of a profiler pass) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+
     from interparts_mpich_amd.inputs import CONFIGS, initial_fields, place_spheres, markers_for
     from interparts_mpich_amd.driver import Simulation
     from interparts_mpich_amd import _lib

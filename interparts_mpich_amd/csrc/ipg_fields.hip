@@ -12,27 +12,67 @@ constexpr int BX = 64, BY = 4;
 // One RK stage, src/rk3.f90:13-106 (STAGE 1), 108-222 (2), 224-329 (3), with
 // momxad/momyad/momzad/momxp/momyp/momzp/momzbuoy (src/mom.f90) and scalad
 // (src/scal.f90:11-60) evaluated in registers: one read of each input and one
-// write of each output per cell.  A workgroup owns a 64 x 4 column tile and
-// marches up KB planes in z, keeping each column's k-1, k, k+1 values in
-// registers; tiles are assigned so that each XCD works on one contiguous slab
-// (its L2 then holds the halos its neighbours share).
+// write of each output per cell.
+//
+// A workgroup owns a 64 x 8 (i, j) column tile and marches up KB planes in z.
+// The five stencil inputs (u v w c p) of planes k and k+1, each with a
+// one-cell halo in i and j, sit in a two-slot LDS ring; every plane is read
+// from HBM once per workgroup (coalesced 64-wide rows), the next plane is
+// prefetched into registers while the current one is computed, and the k-1
+// values a cell needs stay in registers.  Tiles are numbered i fastest, then
+// j, then z, and handed out so that each XCD works through one contiguous
+// range of them: neighbouring tiles, which share halo rows, then run on the
+// same XCD and meet in its L2.
+constexpr int RTX = 64, RTY = 8, RNT = RTX * RTY;
+constexpr int LW = RTX + 2, LH = RTY + 2, LPL = LW * LH;
+constexpr int NLD = (LPL + RNT - 1) / RNT;
+
 template <int STAGE>
-__global__ __launch_bounds__(BX *BY) void k_rk(Grid g, const double *__restrict__ U, const double *__restrict__ V,
-                                               const double *__restrict__ W, const double *__restrict__ C,
-                                               const double *__restrict__ P, double *__restrict__ dudt,
-                                               double *__restrict__ dvdt, double *__restrict__ dwdt,
-                                               double *__restrict__ dcdt, double *__restrict__ durk,
-                                               double *__restrict__ dvrk, double *__restrict__ dwrk,
-                                               double *__restrict__ dcrk, const double *__restrict__ dtp, int ntx,
-                                               int nty, int kb) {
+__global__ __launch_bounds__(RNT) void k_rk(Grid g, const double *__restrict__ U, const double *__restrict__ V,
+                                            const double *__restrict__ W, const double *__restrict__ C,
+                                            const double *__restrict__ P, double *__restrict__ dudt,
+                                            double *__restrict__ dvdt, double *__restrict__ dwdt,
+                                            double *__restrict__ dcdt, double *__restrict__ durk,
+                                            double *__restrict__ dvrk, double *__restrict__ dwrk,
+                                            double *__restrict__ dcrk, const double *__restrict__ dtp, int ntx,
+                                            int nty, int kb) {
+  __shared__ double sm[2][5][LPL];
   const int nblk = gridDim.x, L = blockIdx.x;
   const int t = (nblk % 8 == 0) ? (L % 8) * (nblk / 8) + L / 8 : L;
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
-  const int i = tx * BX + (int)(threadIdx.x % BX) + 1;
-  const int j = ty * BY + (int)(threadIdx.x / BX) + 1;
+  const int i0 = tx * RTX + 1, j0 = ty * RTY + 1;
   const int k0 = tz * kb + 1, k1 = min(k0 + kb - 1, g.nz);
-  if (i > g.nx || j > g.ny) return;
+  const int lx = (int)threadIdx.x % RTX, ly = (int)threadIdx.x / RTX;
+  const int i = i0 + lx, j = j0 + ly;
+  const bool active = i <= g.nx && j <= g.ny;
   const long sx = g.sx, sxy = g.sxy;
+  const double *src[5] = {U, V, W, C, P};
+  // this thread's share of a halo plane: element e = jj*LW + ii <-> cell (i0-1+ii, j0-1+jj)
+  long goff[NLD];
+  bool gok[NLD];
+#pragma unroll
+  for (int q = 0; q < NLD; q++) {
+    const int e = (int)threadIdx.x + q * RNT, ii = e % LW, jj = e / LW;
+    const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+    gok[q] = e < LPL && gi <= g.nx + 1 && gj <= g.ny + 1;
+    goff[q] = (long)(gi - 1) + sx * gj;
+  }
+  double pre[5][NLD];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int f = 0; f < 5; f++)
+#pragma unroll
+      for (int q = 0; q < NLD; q++) pre[f][q] = gok[q] ? src[f][goff[q] + sxy * k] : 0.0;
+  };
+  auto stash = [&](int slot) {
+#pragma unroll
+    for (int f = 0; f < 5; f++)
+#pragma unroll
+      for (int q = 0; q < NLD; q++) {
+        const int e = (int)threadIdx.x + q * RNT;
+        if (e < LPL) sm[slot][f][e] = pre[f][q];
+      }
+  };
   const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, visc = g.visc, kappa = g.kappa;
   const double dt = *dtp;
   double f1, f2 = 0.0, f3 = 0.0;
@@ -41,20 +81,32 @@ __global__ __launch_bounds__(BX *BY) void k_rk(Grid g, const double *__restrict_
   else { f1 = dt * (20.0 / 60.0); f2 = dt * (45.0 / 60.0); f3 = dt * (25.0 / 60.0); }
   const double bq = 0.5 * g.beta * 9.81;
   long c = g.at(i, j, k0);
-  // column window: values at k (c) and k-1 (m) carried up the column
-  double Uc = U[c], Ukm = U[c - sxy], Uim = U[c - 1];
-  double Vc = V[c], Vkm = V[c - sxy], Vjm = V[c - sx];
-  double Wc = W[c], Wkm = W[c - sxy], Wip_km = W[c + 1 - sxy], Wjp_km = W[c + sx - sxy];
-  double Cc = C[c], Ckm = C[c - sxy];
-  double Pc = P[c];
+  // k-1 values of the first plane
+  double Ukm = 0.0, Vkm = 0.0, Wkm = 0.0, Wip_km = 0.0, Wjp_km = 0.0, Ckm = 0.0;
+  if (active) { Ukm = U[c - sxy]; Vkm = V[c - sxy]; Wkm = W[c - sxy]; Wip_km = W[c + 1 - sxy]; Wjp_km = W[c + sx - sxy]; Ckm = C[c - sxy]; }
+  fetch(k0);
+  stash(0);
+  fetch(k0 + 1);
+  stash(1);
+  __syncthreads();
+  const int e0 = (ly + 1) * LW + lx + 1;  // this cell in a halo plane
+  int a = 0;
   for (int k = k0; k <= k1; k++, c += sxy) {
-    const double Ukp = U[c + sxy], Uim_kp = U[c - 1 + sxy], Vkp = V[c + sxy], Vjm_kp = V[c - sx + sxy];
-    const double Wkp = W[c + sxy], Ckp = C[c + sxy], Pkp = P[c + sxy];
-    const double Uip = U[c + 1], Ujp = U[c + sx], Ujm = U[c - sx], Uimjp = U[c - 1 + sx];
-    const double Vip = V[c + 1], Vim = V[c - 1], Vjp = V[c + sx], Vipjm = V[c + 1 - sx];
-    const double Wip = W[c + 1], Wim = W[c - 1], Wjp = W[c + sx], Wjm = W[c - sx];
-    const double Cip = C[c + 1], Cim = C[c - 1], Cjp = C[c + sx], Cjm = C[c - sx];
-    const double Pip = P[c + 1], Pjp = P[c + sx];
+    const bool more = k + 2 <= k1 + 1;
+    if (more) fetch(k + 2);
+    const double(*pk)[LPL] = sm[a];
+    const double(*pn)[LPL] = sm[a ^ 1];
+    if (active) {
+    const double Uc = pk[0][e0], Vc = pk[1][e0], Wc = pk[2][e0], Cc = pk[3][e0], Pc = pk[4][e0];
+    const double Ukp = pn[0][e0], Uim_kp = pn[0][e0 - 1], Vkp = pn[1][e0], Vjm_kp = pn[1][e0 - LW];
+    const double Wkp = pn[2][e0], Ckp = pn[3][e0], Pkp = pn[4][e0];
+    const double Uim = pk[0][e0 - 1], Uip = pk[0][e0 + 1], Ujp = pk[0][e0 + LW], Ujm = pk[0][e0 - LW];
+    const double Uimjp = pk[0][e0 - 1 + LW];
+    const double Vip = pk[1][e0 + 1], Vim = pk[1][e0 - 1], Vjp = pk[1][e0 + LW], Vjm = pk[1][e0 - LW];
+    const double Vipjm = pk[1][e0 + 1 - LW];
+    const double Wip = pk[2][e0 + 1], Wim = pk[2][e0 - 1], Wjp = pk[2][e0 + LW], Wjm = pk[2][e0 - LW];
+    const double Cip = pk[3][e0 + 1], Cim = pk[3][e0 - 1], Cjp = pk[3][e0 + LW], Cjm = pk[3][e0 - LW];
+    const double Pip = pk[4][e0 + 1], Pjp = pk[4][e0 + LW];
     // momxp/momyp/momzp (src/mom.f90:56-79,129-155,205-231)
     const double gpx = -(dxi * (Pip - Pc));
     const double gpy = -(dyi * (Pjp - Pc));
@@ -129,12 +181,13 @@ __global__ __launch_bounds__(BX *BY) void k_rk(Grid g, const double *__restrict_
       dcdt[c] = Cc + (f2 * ac - f3 * rc);
       if (STAGE == 2) { durk[c] = ax; dvrk[c] = ay; dwrk[c] = az; dcrk[c] = ac; }
     }
-    // slide the column window up one plane
-    Ukm = Uc; Uc = Ukp; Uim = Uim_kp;
-    Vkm = Vc; Vc = Vkp; Vjm = Vjm_kp;
-    Wkm = Wc; Wc = Wkp; Wip_km = Wip; Wjp_km = Wjp;
-    Ckm = Cc; Cc = Ckp;
-    Pc = Pkp;
+    // the k-1 values of the next plane
+    Ukm = Uc; Vkm = Vc; Wkm = Wc; Wip_km = Wip; Wjp_km = Wjp; Ckm = Cc;
+    }
+    __syncthreads();          // every wave is done with plane k
+    if (more) stash(a);       // plane k+2 takes its slot
+    __syncthreads();
+    a ^= 1;
   }
 }
 
@@ -319,10 +372,16 @@ static dim3 cell_grid(const Grid &g) { return dim3((g.nx + BX - 1) / BX, (g.ny +
 int ipg_launch_rk(ipg_ctx *c, int rkiter) {
   const Grid &g = c->g;
   double **f = c->f;
-  const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY;
-  const int kb = c->tune_rk_kb > 0 ? c->tune_rk_kb : 1;
+  const int ntx = (g.nx + RTX - 1) / RTX, nty = (g.ny + RTY - 1) / RTY;
+  // z planes per workgroup: enough workgroups (>= 2048) to fill 256 CUs a few
+  // times over, but long columns so the LDS ring pays for its prologue
+  int kb = c->tune_rk_kb;
+  if (kb <= 0) {
+    kb = 64;
+    while (kb > 4 && (long)ntx * nty * ((g.nz + kb - 1) / kb) < 2048) kb /= 2;
+  }
   const int ntz = (g.nz + kb - 1) / kb;
-  dim3 grid(ntx * nty * ntz), blk(BX * BY);
+  dim3 grid(ntx * nty * ntz), blk(RNT);
 #define IPG_RK_LAUNCH(S)                                                                                               \
   hipLaunchKernelGGL(k_rk<S>, grid, blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],         \
                      f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK],         \

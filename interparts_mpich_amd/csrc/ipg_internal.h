@@ -65,7 +65,7 @@ struct ipg_ctx {
   double *G[2][3];                // spread force buffers, interior (nx*ny*nz), modular x,y
   double *K1[3];                  // dwdtf(.,.,k1) planes (nx*ny), src/interp_spread.f90:100 (no upper clamp)
   bool ibm_lds_set;
-  // launch shape: z planes per RK-stage workgroup (IPG_TUNE_RK_KB at init; default 1)
+  // launch shape: z planes per RK-stage workgroup (IPG_TUNE_RK_KB at init; 0 = chosen per grid)
   int tune_rk_kb;
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
   double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl

@@ -56,10 +56,14 @@ def test_initialize_parity(name):
         assert r <= 1e-13, (n, r)
 
 
-@pytest.mark.parametrize("name", ["t16", "t24x16x32"])
+# several RK-stage tiles in i and j, a partial tile in i, several z blocks
+RK_CASES = {"r192x48x40": Case(name="r192x48x40", itot=192, jtot=48, ktot=40, np=0, re_bulk=700.0, seed=11)}
+
+
+@pytest.mark.parametrize("name", ["t16", "t24x16x32", "r192x48x40"])
 @pytest.mark.parametrize("rkiter", [1, 2, 3])
 def test_rk_stage_bitwise(name, rkiter):
-    case = CONFIGS[name]
+    case = CONFIGS.get(name) or RK_CASES[name]
     g, o = pair(case)
     rng = np.random.default_rng(rkiter)
     for n in ("DURK", "DVRK", "DWRK", "DCRK", "C"):
