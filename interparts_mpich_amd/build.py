@@ -43,15 +43,15 @@ def build_fortran():
     fdir = os.path.join(HERE, "fortran")
     if not os.path.exists(os.path.join(fdir, "Makefile")):
         return None
-    _run(["make", "-s", "-C", fdir])
-    return os.path.join(fdir, "interpt_gpu")
+    _run(["make", "-s", "-B", "-C", fdir])
+    return os.path.join(fdir, "interpt")
 
 
 def build_oracle():
     odir = os.path.join(ROOT, "oracle")
-    _run(["make", "-s", "-C", odir, "liboracle.so"])
+    _run(["make", "-s", "-B", "-C", odir, "liboracle.so"])
     if os.path.isdir("/root/reference/src"):
-        _run(["make", "-s", "-C", odir, "ref"])
+        _run(["make", "-s", "-B", "-C", odir, "ref"])
 
 
 def build_all():

@@ -1,0 +1,135 @@
+! main.f90 -- program interpt on one MI355X: the time loop of the reference's
+! src/main.f90 (start-up 98-227, substep 268-414, dt policy 425-441) over the
+! MI355X library.  Inputs: parameters.inp (&PARAMETERS of src/param.f90),
+! run.nml (&GRIDSIZE: the compile-time sizes of src/gridsize.f90, and
+! &RUNCONTROL), lagrangianforcepoints2 when np > 0, fld%07d when begin > 0.
+! Outputs: timestep.txt (src/main.f90:425-431); fld%07d (src/loadd.f90
+! format) and, with particles, partstate%07d every ioutfld steps and at the
+! end of the run.
+program interpt
+  use mod_param
+  use mod_common
+  use mod_initmpi
+  use mod_init
+  use mod_initparticles
+  use mod_coordsfp
+  use mod_initsolver
+  use mod_bound
+  use mod_rk
+  use mod_interp_spread
+  use mod_forcing
+  use mod_fillps
+  use mod_solver
+  use mod_correc
+  use mod_intgr_over_sphere
+  use mod_intgr_nwtn_eulr
+  use mod_chkdt
+  use mod_chkdiv
+  use mod_loadflds
+  use mod_loadpart
+  implicit none
+  real :: dtmax, maxerror, avererror, t0, t1
+  integer :: ibmiter, u, last
+  integer(8) :: c0, c1, crate
+
+  call input_parameters
+  call initmpi
+  if (np > 0) then
+    write(6, *) 'Number of Lagrangian Force Points of first shell = ', nl
+  end if
+  if (begin == 0) then
+    call init_ics
+    if (np > 0) call initparticles
+    time = 0.
+    call bounduvw(FLD_NEW)
+    call boundscal
+    call chkdt(dtmax)
+    dt = 0.5 * dtmax
+  else
+    if (np > 0) call loadpart(0, begin)
+    call loadflds(0, begin)
+    call chkdt(dtmax)
+    write(6, *) 'nr steps at beginning simulation = ', begin
+  end if
+  call ipg_ok(ipg_set_dt(ctx, dt), 'set_dt')
+  call initsolver
+  call bounduvw(FLD_NEW)
+  call boundscal
+  call boundp(FLD_NEW)
+  call chkdiv
+  if (np > 0) then
+    call coordsfp
+    if (begin == 0) then
+      call intgr_over_sphere(1)
+      call intgr_over_sphere(2)
+      call intgr_over_sphere(3)
+    end if
+  end if
+  write(6, *) 'dtmax = ', dtmax, ' dt = ', dt
+
+  call system_clock(c0, crate)
+  last = begin
+  do istep = begin + 1, begin + nstep
+    if (time + dt > t_end) then
+      dt = t_end - time
+      call ipg_ok(ipg_set_dt(ctx, dt), 'set_dt')
+    end if
+    time = time + dt
+    do rkiter = 1, 3
+      if (fused_substep) then
+        ! the whole substep below as one launch sequence (one hipGraph per stage)
+        call ipg_ok(ipg_substep(ctx, int(rkiter, c_int)), 'substep')
+        cycle
+      end if
+      if (rkiter == 1) call rk1
+      if (rkiter == 2) call rk2
+      if (rkiter == 3) call rk3
+      if (np > 0) then
+        call ibm_begin
+        do ibmiter = 0, 2
+          call eulr2lagr(ibmiter)
+          if (ibmiter == 0) then
+            call complagrforces
+          else
+            call updtlagrforces(maxerror, avererror)
+          end if
+          call lagr2eulr
+        end do
+      end if
+      call bounduvw(FLD_DT)
+      call fillps(rkiter)
+      call solver2d
+      call boundp(FLD_DT)
+      call correc(rkiter)
+      call bounduvw(FLD_NEW)
+      call pnew_update
+      if (np > 0) call intgr_nwtn_eulr
+    end do
+    if (mod(istep, ioutchk) == 0) then
+      call chkdiv
+      call chkdt(dtmax)
+      write(6, *) 'dtmax = ', dtmax, ' dt = ', dt
+      open(newunit=u, file='timestep.txt', position='append')
+      write(u, '(3E16.8)') 1. * istep, time, dt
+      close(u)
+      dt = 0.75 * dtmax
+      call ipg_ok(ipg_set_dt(ctx, dt), 'set_dt')
+    end if
+    last = istep
+    if (mod(istep, ioutfld) == 0) call dump(last)
+  end do
+  call ipg_ok(ipg_sync(ctx), 'sync')
+  call system_clock(c1)
+  t0 = 0.
+  t1 = real(c1 - c0) / real(crate)
+  if (nstep > 0) write(6, '(A,I0,A,F10.3,A,F12.3,A)') 'time loop: ', 3 * nstep, ' substeps in ', t1, ' s (', &
+       3 * nstep / max(t1, 1.e-9), ' substeps/s)'
+  if (nstep == 0 .or. mod(last, ioutfld) /= 0) call dump(last)
+  call ipg_ok(ipg_finalize(ctx), 'finalize')
+contains
+  subroutine dump(nr)
+    integer, intent(inout) :: nr
+    call loadflds(1, nr)
+    if (np > 0) call loadpart(1, nr)
+  end subroutine dump
+end program interpt

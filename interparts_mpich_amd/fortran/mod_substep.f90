@@ -6,6 +6,9 @@
 ! and the markers stay resident in HBM inside one context: the reference's
 ! array arguments (unew.., dudt.., p) become selectors, and host copies of a
 ! field exist only for input and output (mod_init, mod_loadflds).
+!
+! Compiled, as the reference is, with -fdefault-real-8: every real and every
+! real literal is FP64.
 
 ! ---------------------------------------------------------------------------
 ! src/param.f90:19-86 (&PARAMETERS) and src/gridsize.f90:6-26, which are
@@ -415,9 +418,58 @@ contains
         call ipg_ok(ipg_field_get(ctx, sel(q), hbuf), 'field_get')
         write(u) hbuf(1:imax, 1:jmax, 1:kmax)
       end do
-      fldinfo = [time, 1. * nr, dt]
+      fldinfo(1) = time
+      fldinfo(2) = 1. * nr
+      fldinfo(3) = dt
       write(u) fldinfo
       close(u)
     end if
   end subroutine loadflds
 end module mod_loadflds
+
+! Particle state dump and restart: partstate%07d holds the IPG_NPART
+! quantities of type particle kept on the device (x y z theta phi u v w omx
+! omy omz omtheta intu intv intw intomx intomy intomz colfx colfy colfz coltx
+! colty coltz fxltot fyltot fzltot torqx torqy torqz torqtheta), each as np
+! FP64 values, quantity after quantity, then (time, istep, dt).  It stands in
+! for allpartdata%07d of src/loadd.f90:121-692, whose contact-history arrays
+! (dx.., psi, firstc) have no counterpart here: the tangential contact force
+! they feed never reaches the state (src/collisions.f90:199-205).
+module mod_loadpart
+  use mod_param
+  use mod_common
+  implicit none
+contains
+  subroutine loadpart(in, nr)
+    integer, intent(in) :: in
+    integer, intent(inout) :: nr
+    character(len=7) :: istepchar
+    real(8), allocatable :: q(:, :)
+    real(8) :: info(3)
+    integer :: u, m
+    allocate(q(np, 0:IPG_NPART - 1))
+    write(istepchar, '(i7.7)') nr
+    if (in == 0) then
+      open(newunit=u, file='partstate'//istepchar, access='stream', form='unformatted', status='old', action='read')
+      read(u) q
+      read(u) info
+      close(u)
+      call ipg_ok(ipg_initparticles(ctx, q(:, IPG_PX), q(:, IPG_PY), q(:, IPG_PZ)), 'ipg_initparticles')
+      do m = 0, IPG_NPART - 1
+        call ipg_ok(ipg_particles_put(ctx, int(m, c_int), q(:, m)), 'particles_put')
+      end do
+    else
+      do m = 0, IPG_NPART - 1
+        call ipg_ok(ipg_particles_get(ctx, int(m, c_int), q(:, m)), 'particles_get')
+      end do
+      info(1) = time
+      info(2) = 1. * nr
+      info(3) = dt
+      open(newunit=u, file='partstate'//istepchar, access='stream', form='unformatted', status='replace', &
+           action='write')
+      write(u) q
+      write(u) info
+      close(u)
+    end if
+  end subroutine loadpart
+end module mod_loadpart

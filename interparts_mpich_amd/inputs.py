@@ -285,3 +285,64 @@ def write_namelist(path, case):
         f.write("  runmode = 0\n")
         f.write("/\n")
     return path
+
+
+def write_run_nml(path, case, nstep, begin=0, use_graphs=True, fused_substep=True, t_end=1.0e30):
+    """run.nml of the Fortran driver: &GRIDSIZE (the compile-time sizes of
+    src/gridsize.f90:6-26) and &RUNCONTROL (start step, step count, seed)."""
+    tf = {True: ".true.", False: ".false."}
+    with open(path, "w") as f:
+        f.write(f"&GRIDSIZE itot = {case.itot}, jtot = {case.jtot}, ktot = {case.ktot}, np = {case.np}, "
+                f"nlmax = {case.nl if case.np > 0 else 0}, t_end = {t_end!r} /\n")
+        f.write(f"&RUNCONTROL begin = {begin}, nstep = {nstep}, seed = {case.seed}, perturb = {case.perturb!r}, "
+                f"use_graphs = {tf[use_graphs]}, fused_substep = {tf[fused_substep]} /\n")
+    return path
+
+
+def write_case_dir(d, case, nstep, **kw):
+    """parameters.inp, run.nml and (with spheres) lagrangianforcepoints2 in d."""
+    os.makedirs(d, exist_ok=True)
+    write_namelist(os.path.join(d, "parameters.inp"), case)
+    write_run_nml(os.path.join(d, "run.nml"), case, nstep, **kw)
+    if case.np > 0:
+        write_markers(os.path.join(d, "lagrangianforcepoints2"), case)
+    return d
+
+
+def fld_path(d, nr):
+    return os.path.join(d, f"fld{nr:07d}")
+
+
+def read_fld(path, case):
+    """fld%07d (src/loadd.f90:30-67): unew vnew wnew pnew cnew over the interior,
+    i fastest, then (time, istep, dt).  Returns ({name: (ktot, jtot, itot)}, info)."""
+    n = case.itot * case.jtot * case.ktot
+    a = np.fromfile(path, dtype="<f8")
+    if a.size != 5 * n + 3:
+        raise ValueError(f"{path}: {a.size} values, expected {5 * n + 3}")
+    shape = (case.ktot, case.jtot, case.itot)
+    out = {name: a[q * n:(q + 1) * n].reshape(shape) for q, name in enumerate(("U", "V", "W", "PNEW", "C"))}
+    return out, tuple(a[5 * n:])
+
+
+def write_fld(path, case, fields, time, istep, dt):
+    """Inverse of read_fld; fields maps U V W PNEW C to interior (or ghosted) arrays."""
+    parts = []
+    for name in ("U", "V", "W", "PNEW", "C"):
+        x = np.asarray(fields[name], dtype=np.float64)
+        if x.shape == (case.ktot + 2, case.jtot + 2, case.itot + 2):
+            x = x[1:-1, 1:-1, 1:-1]
+        parts.append(np.ascontiguousarray(x).ravel())
+    parts.append(np.array([time, float(istep), dt]))
+    np.concatenate(parts).astype("<f8").tofile(path)
+
+
+def read_partstate(path, case):
+    """partstate%07d of the Fortran driver: 31 quantities x np FP64, then
+    (time, istep, dt)."""
+    from ._lib import PARTS
+    a = np.fromfile(path, dtype="<f8")
+    n = case.np
+    if a.size != len(PARTS) * n + 3:
+        raise ValueError(f"{path}: {a.size} values")
+    return {q: a[m * n:(m + 1) * n] for m, q in enumerate(PARTS)}, tuple(a[-3:])

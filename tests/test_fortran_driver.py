@@ -1,0 +1,136 @@
+"""The Fortran host driver (program interpt, interparts_mpich_amd/fortran/)
+calling libipg.so through iso_c_binding: seeded start-up, the time loop of
+src/main.f90 (fused substep or routine by routine), the dt policy, and the
+fld%07d / partstate%07d files, checked against the FP64 oracle."""
+import dataclasses
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from interparts_mpich_amd.inputs import (CONFIGS, fld_path, initial_fields, markers_for, place_spheres,
+                                         read_fld, read_partstate, write_case_dir, write_fld)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "interparts_mpich_amd", "fortran", "interpt")
+FLUID = ("U", "V", "W", "PNEW", "C")
+PART = ("X", "Y", "Z", "U", "V", "W", "OMX", "OMY", "OMZ", "INTU", "INTV", "INTW", "FXLTOT", "FYLTOT", "FZLTOT",
+        "COLFX", "COLFY", "COLFZ")
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300)
+
+
+def run(d, timeout=180):
+    print(f"[interpt] {d}", flush=True)
+    r = subprocess.run([EXE], cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=timeout)
+    print(r.stdout[-3000:], flush=True)
+    return r
+
+
+def needs_exe():
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built (make -C interparts_mpich_amd/fortran)")
+
+
+def test_fld_round_trip(tmp_path):
+    case = CONFIGS["t24x16x32"]
+    u, v, w, p, c = initial_fields(case)
+    path = str(tmp_path / "fld0000007")
+    write_fld(path, case, {"U": u, "V": v, "W": w, "PNEW": p, "C": c}, 0.25, 7, 1e-3)
+    f, info = read_fld(path, case)
+    assert info == (0.25, 7.0, 1e-3)
+    assert np.array_equal(f["V"], v[1:-1, 1:-1, 1:-1]) and np.array_equal(f["U"], u[1:-1, 1:-1, 1:-1])
+    assert os.path.getsize(path) == 8 * (5 * case.itot * case.jtot * case.ktot + 3)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_driver_fails_loudly_without_a_gpu(tmp_path):
+    """No CPU fallback: with no device, ipg_init fails and interpt stops with
+    the library's message and a non-zero status."""
+    needs_exe()
+    d = write_case_dir(str(tmp_path), CONFIGS["t16"], nstep=1)
+    r = run(d, timeout=60)
+    assert r.returncode != 0
+    assert "ipg_init" in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_start_up_matches_host_inputs(tmp_path):
+    """nstep = 0: the fields and spheres the driver seeds itself are the ones
+    inputs.py draws, and the start-up (boundaries, sphere integrals) matches
+    the oracle."""
+    from oracle.pyoracle import Oracle
+    needs_exe()
+    case = CONFIGS["t32"]
+    d = write_case_dir(str(tmp_path), case, nstep=0)
+    assert run(d).returncode == 0
+    f, info = read_fld(fld_path(d, 0), case)
+    fields, spheres = initial_fields(case), place_spheres(case)
+    o = Oracle(case).initialize(fields, spheres, markers_for(case))
+    for n in FLUID:
+        assert np.array_equal(f[n], o.get(n)[1:-1, 1:-1, 1:-1]), n
+    assert info[2] == o.dt
+    q, _ = read_partstate(os.path.join(d, "partstate0000000"), case)
+    for a, b in zip(("X", "Y", "Z"), spheres):
+        assert np.array_equal(q[a], b), a
+    for n in ("INTU", "INTV", "INTW", "INTOMX", "INTOMY", "INTOMZ"):
+        assert rel(q[n], o.pget(n)) <= 1e-13, n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fused", [True, False])
+def test_driver_steps_match_oracle(tmp_path, fused):
+    """Three steps with a dt update after every step (ioutchk = 1): the fused
+    substep and the routine-by-routine loop (rk1.., eulr2lagr/complagrforces/
+    lagr2eulr, bounduvw, fillps, solver2d, correc, intgr_nwtn_eulr) both match
+    the oracle to <= 1e-11 relative."""
+    from oracle.pyoracle import Oracle
+    needs_exe()
+    case = dataclasses.replace(CONFIGS["t32"], ioutchk=1)
+    d = write_case_dir(str(tmp_path), case, nstep=3, fused_substep=fused)
+    assert run(d).returncode == 0
+    o = Oracle(case).initialize(initial_fields(case), place_spheres(case), markers_for(case))
+    for _ in range(3):
+        o.step()
+    f, info = read_fld(fld_path(d, 3), case)
+    assert info[1] == 3.0
+    for n in FLUID:
+        r = rel(f[n], o.get(n)[1:-1, 1:-1, 1:-1])
+        print(n, r)
+        assert r <= 1e-11, n
+    q, _ = read_partstate(os.path.join(d, "partstate0000003"), case)
+    for n in PART:
+        r = rel(q[n], o.pget(n))
+        print(n, r)
+        assert r <= 1e-11, n
+    rows = np.loadtxt(os.path.join(d, "timestep.txt"), ndmin=2)
+    assert rows.shape == (3, 3) and list(rows[:, 0]) == [1.0, 2.0, 3.0]
+
+
+@pytest.mark.gpu
+def test_driver_restart_round_trip(tmp_path):
+    """begin = 2, nstep = 0 reads fld0000002 / partstate0000002 and writes them
+    back unchanged (src/loadd.f90 read and write paths)."""
+    needs_exe()
+    case = CONFIGS["t32"]
+    d = write_case_dir(str(tmp_path), case, nstep=2)
+    assert run(d).returncode == 0
+    keep = str(tmp_path / "keep")
+    os.makedirs(keep)
+    for name in ("fld0000002", "partstate0000002"):
+        shutil.copy(os.path.join(d, name), os.path.join(keep, name))
+    write_case_dir(d, case, nstep=0, begin=2)
+    assert run(d).returncode == 0
+    a, ia = read_fld(fld_path(d, 2), case)
+    b, ib = read_fld(os.path.join(keep, "fld0000002"), case)
+    assert ia == ib
+    for n in FLUID:
+        assert np.array_equal(a[n], b[n]), n
+    qa, _ = read_partstate(os.path.join(d, "partstate0000002"), case)
+    qb, _ = read_partstate(os.path.join(keep, "partstate0000002"), case)
+    for n in qa:
+        assert np.array_equal(qa[n], qb[n]), n
