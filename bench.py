@@ -27,21 +27,56 @@ HBM_PEAK_GBS = 8000.0
 
 
 def kernel_bytes(case, name, rkiter):
-    """Algorithmic HBM bytes of one launch of a kernel (FP64 words x 8)."""
+    """Algorithmic HBM bytes of one launch of a kernel (FP64 words x 8).
+    N cells; Sw = FP64 words of the half spectrum, 2 (nx/2+1) ny nz."""
     N = float(case.itot * case.jtot * case.ktot)
-    S = (case.itot // 2 + 1) * 2.0 / case.itot * N  # half-spectrum words
+    Sw = 2.0 * (case.itot // 2 + 1) * case.jtot * case.ktot
     words = {
-        "rk_stage": {1: 13.0, 2: 17.0, 3: 13.0}[rkiter] * N,  # reads u v w c p (+4 rk) ; writes 4 dt (+4 rk)
+        "rk_stage": {1: 13.0, 2: 17.0, 3: 13.0}[rkiter] * N,  # reads u v w c p (+4 rk); writes 4 d?dt (+4 rk)
         "fillps": 4.0 * N,
-        "fft_x_fwd": N + S,
-        "fft_y_fwd": 2.0 * S,
-        "thomas": 5.0 * S,            # spec r/w twice + d w/r (d is half a complex word)
-        "fft_y_inv": 2.0 * S,
-        "fft_x_inv": S + N,
+        "fft_x_fwd": 3.0 * N + Sw,    # fillps fused: reads dudt dvdt dwdt, writes the half spectrum
+        "fft_y_fwd": 2.0 * Sw,
+        "thomas": 5.0 * Sw,           # spectrum read + written in each sweep, d written then read
+        "fft_y_inv": 2.0 * Sw,
+        "fft_x_inv": Sw + N,
         "correc": 11.0 * N,           # dudt dvdt dwdt p pnew dcdt -> u v w pnew cnew
     }
     w = words.get(name)
     return None if w is None else 8.0 * w
+
+
+def traffic_from_profile(kernel, workload):
+    """HBM bytes per launch of `kernel` measured by the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes summarised in profiles/hbm_traffic_<kernel>_<workload>.txt
+    (null when no such pass exists for this workload)."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{kernel}_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def copy_bandwidth(mib=1024, reps=10):
+    """A plain device-to-device copy of mib MiB, timed with HIP events: the
+    practical HBM ceiling next to the 8 TB/s nominal peak (read + write bytes)."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del x, y
+    torch.cuda.empty_cache()
+    return 2.0 * n * 8 / (ms * 1e-3) / 1e9
 
 
 def copy_probe(mib):
@@ -117,6 +152,7 @@ def main():
             if b is not None:
                 a["bytes"] += b
     per_substep = {k: v["ms"] / args.steps for k, v in acc.items()}
+    kernel_gbs = {k: round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) for k, v in acc.items() if v["bytes"] > 0}
     dominant = max((k for k in acc if acc[k]["bytes"] > 0), key=lambda k: acc[k]["ms"])
     d = acc[dominant]
     avg_ms = d["ms"] / d["launches"]
@@ -145,12 +181,20 @@ def main():
                    "graphs": not args.no_graphs,
                    "gcell_updates_per_s": round(value * case.cells / 1e9, 4)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile(dominant, case.name),
                      "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch},
         "kernel_ms_per_substep": {k: round(v, 5) for k, v in per_substep.items()},
+        "kernel_algorithmic_gbs": kernel_gbs,
         "event_sum_ms_per_substep": round(total_ms_events, 4),
     }
     sim.close()
+    try:
+        copy_gbs = copy_bandwidth()
+    except Exception as e:  # torch without a device: the number is simply absent
+        print(f"[bench] copy bandwidth not measured: {e}", file=sys.stderr)
+        copy_gbs = None
+    out["roofline"]["copy_gbs_measured"] = round(copy_gbs, 1) if copy_gbs else None
 
     if not args.no_cpu_baseline:
         from oracle.pyoracle import Oracle

@@ -11,9 +11,9 @@
 // force buffers G (zero outside the spheres' stencils), and the velocity the
 // markers see is dudt + G: the same sums, fl(G + dudtold), without the
 // full-field passes.  The last pass spreads straight into dudt.  Work is
-// organised one workgroup per sphere: the sphere's box of cells is staged in
-// LDS for the gathers, and the spread is summed in LDS before one global
-// atomic per touched cell.  The w-stencil of a marker within h/2 of the top
+// organised one workgroup per (sphere, velocity component): the stencil cells
+// of the sphere's box are staged in LDS for the gathers, and the spread is
+// summed in LDS before one global atomic per touched cell.  The w-stencil of a marker within h/2 of the top
 // wall reaches the k1 = kmax+1 plane (src/interp_spread.f90:100 has no upper
 // clamp); that plane never receives dudt and keeps the last spread until the
 // next one, so it is held in its own planes K.
@@ -90,8 +90,12 @@ __device__ __forceinline__ long gidx(const Grid &g, int wi, int wj, int k) {
 }
 // periodic wrap of any unwrapped index onto 1..n
 __device__ __forceinline__ int wrapany(int i, int n) {
-  int r = (i - 1) % n;
-  return (r < 0 ? r + n : r) + 1;
+  int r = i - 1;
+  if ((unsigned)r >= (unsigned)n) {
+    r %= n;
+    if (r < 0) r += n;
+  }
+  return r + 1;
 }
 
 // The per-sphere box of one velocity component: cells [b0, b0+bn) per
@@ -175,12 +179,14 @@ __global__ __launch_bounds__(256) void k_ibm_prep(IbmArgs A, double *mw, int *mb
   mv[2 * M + m] = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
 }
 
-// One IBM pass for one (sphere, velocity component) workgroup: the sphere's
-// box of (dudt + previous spread) is staged in LDS, every marker gathers its
-// 27-point stencil from it (src/interp_spread.f90:132-154) and updates its
-// force (src/forcing.f90:63-65 / 255-257); then the markers' spread
-// (src/interp_spread.f90:329-355) is summed in the same LDS box with LDS
-// atomics and the box is added to the global target once, cell by cell.
+// One IBM pass for one (sphere, velocity component) workgroup.  The cells of
+// the sphere's box that some marker's stencil touches are flagged first (one
+// bit per cell in LDS); only those are staged from (dudt + previous spread),
+// which is a shell about 2.5 cells thick, not the whole box.  Every marker
+// gathers its 27-point stencil from LDS (src/interp_spread.f90:132-154) and
+// updates its force (src/forcing.f90:63-65 / 255-257); then the markers'
+// spread (src/interp_spread.f90:329-355) is summed in the same LDS box with
+// LDS atomics and the flagged cells are added to the global target once.
 // gather/spread select the halves; the last pass runs them as two launches
 // because its spread goes straight into dudt, which other spheres may still
 // be gathering from.
@@ -197,17 +203,39 @@ __global__ __launch_bounds__(256) void k_ibm_pass(IbmArgs A, const double *__res
   const double dt = *A.dtp, dti = 1.0 / dt;
   const double c0 = (A.dVlagr / A.dVeul) * dt;
   const Box b = sphere_box(A, A.px[p], A.py[p], A.pz[p]);
-  const int nbox = b.bx * b.by * b.bz;
+  const int nbox = b.bx * b.by * b.bz, bxy = b.bx * b.by;
+  unsigned *flag = (unsigned *)(box + A.bmx * A.bmy * A.bmz);
+  const int nflag = (nbox + 31) >> 5;
   const double *F = A.fld[comp];
   double *fl = A.fl + comp * M;
   const long m0 = (long)p * A.nl;
   const double *W = mw + (long)comp * 9 * M;
   const int *I0 = mb + (long)comp * 3 * M, *J0 = I0 + M, *K0 = J0 + M, *OK = mb + 9 * M;
+  for (int t = threadIdx.x; t < nflag; t += blockDim.x) flag[t] = 0u;
+  __syncthreads();
+  for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
+    const long m = m0 + l;
+    if (!OK[m]) continue;
+    const int i0 = I0[m], j0 = J0[m], k0 = K0[m];
+    for (int k = 0; k < 3; k++)
+      for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+          const int q = box_at(b, i0 + i, j0 + j, k0 + k);
+          if (q >= 0) atomicOr(&flag[q >> 5], 1u << (q & 31));
+        }
+  }
+  __syncthreads();
+  auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
+  auto cell_of = [&](int t, int &wi, int &wj, int &k) {
+    const int lk = t / bxy, r = t - lk * bxy, lj = r / b.bx, li = r - lj * b.bx;
+    wi = wrapany(b.bi0 + li, g.nx); wj = wrapany(b.bj0 + lj, g.ny); k = b.bk0 + lk;
+  };
   if (gather) {
     const double *Gr = A.Gread[comp];
     for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
-      const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
-      const int wi = wrapany(b.bi0 + li, g.nx), wj = wrapany(b.bj0 + lj, g.ny), k = b.bk0 + lk;
+      if (!flagged(t)) continue;
+      int wi, wj, k;
+      cell_of(t, wi, wj, k);
       double v = F[g.at(wi, wj, k)];
       if (Gr) v = v + Gr[gidx(g, wi, wj, k)];
       box[t] = v;
@@ -271,22 +299,26 @@ __global__ __launch_bounds__(256) void k_ibm_pass(IbmArgs A, const double *__res
     }
     __syncthreads();
     for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+      if (!flagged(t)) continue;
       const double v = box[t];
       if (v != 0.0) {
-        const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
-        const int wi = wrapany(b.bi0 + li, g.nx), wj = wrapany(b.bj0 + lj, g.ny), k = b.bk0 + lk;
+        int wi, wj, k;
+        cell_of(t, wi, wj, k);
         if (Gw) unsafeAtomicAdd(&Gw[gidx(g, wi, wj, k)], v);
         else unsafeAtomicAdd(&A.fld[comp][g.at(wi, wj, k)], v);
       }
     }
   }
   if (A.Gzero[comp]) {
+    // the cells this sphere spread into in an earlier pass: its flagged cells
+    // (the markers have not moved) and any stencil cell outside the box
     double *Gz = A.Gzero[comp];
     for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
-      const int li = t % b.bx, lj = (t / b.bx) % b.by, lk = t / (b.bx * b.by);
-      Gz[gidx(g, wrapany(b.bi0 + li, g.nx), wrapany(b.bj0 + lj, g.ny), b.bk0 + lk)] = 0.0;
+      if (!flagged(t)) continue;
+      int wi, wj, k;
+      cell_of(t, wi, wj, k);
+      Gz[gidx(g, wi, wj, k)] = 0.0;
     }
-    // stencil cells outside the box (not expected; kept exact)
     for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
       const long m = m0 + l;
       if (!OK[m]) continue;
@@ -346,8 +378,9 @@ __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, doub
 // marker set-up so that graph capture never meets an attribute change.
 int ipg_ibm_setup(ipg_ctx *c) {
   const double r2 = 2.0 * c->pc.radfp;
-  const size_t lds = (size_t)((int)std::ceil(r2 * c->g.dxi) + 7) * ((int)std::ceil(r2 * c->g.dyi) + 7) *
-                     ((int)std::ceil(r2 * c->g.dzi) + 7) * sizeof(double);
+  const size_t cells = (size_t)((int)std::ceil(r2 * c->g.dxi) + 7) * ((int)std::ceil(r2 * c->g.dyi) + 7) *
+                       ((int)std::ceil(r2 * c->g.dzi) + 7);
+  const size_t lds = cells * sizeof(double) + (cells + 31) / 32 * sizeof(unsigned);
   if (lds > 160 * 1024) { ipg_set_error("ibm: sphere box exceeds the 160 KiB LDS of a CU"); return 1; }
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   c->ibm_lds_set = true;
@@ -365,7 +398,8 @@ bool ibm_args(ipg_ctx *c, IbmArgs &A, size_t &lds) {
   A.bmx = (int)std::ceil(r2 * c->g.dxi) + 7;
   A.bmy = (int)std::ceil(r2 * c->g.dyi) + 7;
   A.bmz = (int)std::ceil(r2 * c->g.dzi) + 7;
-  lds = (size_t)A.bmx * A.bmy * A.bmz * sizeof(double);
+  const size_t cells = (size_t)A.bmx * A.bmy * A.bmz;
+  lds = cells * sizeof(double) + (cells + 31) / 32 * sizeof(unsigned);
   double **q = c->ps.q;
   A.px = q[IPG_PX]; A.py = q[IPG_PY]; A.pz = q[IPG_PZ]; A.pu = q[IPG_PU]; A.pv = q[IPG_PV]; A.pw = q[IPG_PW];
   A.pomx = q[IPG_POMX]; A.pomy = q[IPG_POMY]; A.pomz = q[IPG_POMZ]; A.pphi = q[IPG_PPHI];
