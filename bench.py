@@ -58,9 +58,10 @@ def traffic_from_profile(kernel, workload):
     return None
 
 
-def copy_bandwidth(mib=1024, reps=10):
-    """A plain device-to-device copy of mib MiB, timed with HIP events: the
-    practical HBM ceiling next to the 8 TB/s nominal peak (read + write bytes)."""
+def copy_probe(mib, reps=10):
+    """A plain copy kernel of known size: reads and writes mib MiB per launch,
+    timed with HIP events.  Runs in a process of its own: torch carries its own
+    HIP runtime, which must not share a process with libipg.so's."""
     import torch
     n = mib * (1 << 20) // 8
     x = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -74,21 +75,20 @@ def copy_bandwidth(mib=1024, reps=10):
     b.record()
     torch.cuda.synchronize()
     ms = a.elapsed_time(b) / reps
-    del x, y
-    torch.cuda.empty_cache()
-    return 2.0 * n * 8 / (ms * 1e-3) / 1e9
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": reps + 1,
+                      "ms_per_copy": ms, "copy_gbs": 2.0 * n * 8 / (ms * 1e-3) / 1e9}))
 
 
-def copy_probe(mib):
-    """A plain copy kernel of known size: reads and writes mib MiB per launch."""
-    import torch
-    n = mib * (1 << 20) // 8
-    x = torch.ones(n, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    for _ in range(5):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 5}))
+def copy_bandwidth(mib=1024):
+    """Measured read+write bandwidth of a plain device copy (GB/s), from a
+    child process, or None when it cannot run."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-probe", str(mib)], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=300)
+    if r.returncode != 0:
+        print(f"[bench] copy probe failed: {r.stderr[-500:]}", file=sys.stderr)
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])["copy_gbs"]
 
 
 def main():
@@ -101,8 +101,9 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=12, help="oracle substeps in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run a plain device-to-device copy of MIB MiB five times (calibrates the HBM "
-                         "byte counters of a profiler pass) and exit")
+                    help="only run a plain device-to-device copy of MIB MiB (calibrates the HBM byte counters of "
+                         "a profiler pass; measures the practical copy bandwidth) and exit")
+    ap.add_argument("--no-copy-probe", action="store_true", help="skip the copy-bandwidth child process")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
@@ -189,12 +190,9 @@ def main():
         "event_sum_ms_per_substep": round(total_ms_events, 4),
     }
     sim.close()
-    try:
+    if not args.no_copy_probe:
         copy_gbs = copy_bandwidth()
-    except Exception as e:  # torch without a device: the number is simply absent
-        print(f"[bench] copy bandwidth not measured: {e}", file=sys.stderr)
-        copy_gbs = None
-    out["roofline"]["copy_gbs_measured"] = round(copy_gbs, 1) if copy_gbs else None
+        out["roofline"]["copy_gbs_measured"] = round(copy_gbs, 1) if copy_gbs else None
 
     if not args.no_cpu_baseline:
         from oracle.pyoracle import Oracle

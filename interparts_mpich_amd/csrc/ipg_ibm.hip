@@ -231,14 +231,28 @@ __global__ __launch_bounds__(256) void k_ibm_pass(IbmArgs A, const double *__res
     wi = wrapany(b.bi0 + li, g.nx); wj = wrapany(b.bj0 + lj, g.ny); k = b.bk0 + lk;
   };
   if (gather) {
+    // staged in chunks of 8 cells per thread so that the loads of a chunk are
+    // all in flight together
     const double *Gr = A.Gread[comp];
-    for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
-      if (!flagged(t)) continue;
-      int wi, wj, k;
-      cell_of(t, wi, wj, k);
-      double v = F[g.at(wi, wj, k)];
-      if (Gr) v = v + Gr[gidx(g, wi, wj, k)];
-      box[t] = v;
+    constexpr int CH = 8;
+    for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * (int)blockDim.x) {
+      double v[CH];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int t = t0 + u * (int)blockDim.x;
+        v[u] = 0.0;
+        if (t < nbox && flagged(t)) {
+          int wi, wj, k;
+          cell_of(t, wi, wj, k);
+          v[u] = F[g.at(wi, wj, k)];
+          if (Gr) v[u] = v[u] + Gr[gidx(g, wi, wj, k)];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int t = t0 + u * (int)blockDim.x;
+        if (t < nbox) box[t] = v[u];
+      }
     }
     __syncthreads();
     for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {

@@ -71,11 +71,12 @@ __global__ __launch_bounds__(256) void k_sphere_int(Grid g, PartConst pc, const 
     double s[6] = {0, 0, 0, 0, 0, 0};
     const int ni = ihigh - ilow + 1, nj = jhigh - jlow + 1, nk = khigh - klow + 1;
     if (ni > 0 && nj > 0 && nk > 0) {
-      const long tot = (long)ni * nj * nk;
+      const int nij = ni * nj, tot = nij * nk;
       for (int comp = 0; comp < 3; comp++) {
         const double *F = comp == 0 ? U : (comp == 1 ? V : W);
-        for (long t = threadIdx.x; t < tot; t += blockDim.x) {
-          const int i = ilow + (int)(t % ni), j = jlow + (int)((t / ni) % nj), k = klow + (int)(t / ((long)ni * nj));
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+          const int kk = t / nij, r = t - kk * nij, jj = r / ni;
+          const int i = ilow + (r - jj * ni), j = jlow + jj, k = klow + kk;
           double zmin, zplus, ymin, yplus, xmin, xplus;
           if (comp == 2) { zmin = ((double)k - 0.5) * dz; zplus = ((double)k + 0.5) * dz; }
           else { zmin = (double)(k - 1) * dz; zplus = (double)k * dz; }
