@@ -69,8 +69,6 @@ struct ipg_ctx {
   int tune_rk_kb;
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
   double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl
-  double *mw;                     // [27+3][np*nl] marker stencil weights and rigid velocities
-  int *mb;                        // [10][np*nl] marker stencil bases and validity
   // particles
   PartState ps;
   double *op;                     // [24][np] state at substep start (src/intgr_nwtn_eulr.f90:77-118)

@@ -346,3 +346,25 @@ def read_partstate(path, case):
     if a.size != len(PARTS) * n + 3:
         raise ValueError(f"{path}: {a.size} values")
     return {q: a[m * n:(m + 1) * n] for m, q in enumerate(PARTS)}, tuple(a[-3:])
+
+
+def ensemble_cases(n=64, seed=5):
+    """The batched ensemble: n independent cases with their own seeds.  Grids
+    from {64^3, 96x96x64, 128^3, 128x128x192}; D uniform in [8h, 12h]; Re_bulk
+    uniform in [1000, 5000]; sphere count uniform in [8, 200], capped so that
+    the solid fraction stays <= 10 % (seeded rejection sampling with a gap of
+    D + h cannot pack more).  All draws come from stream 300 of `seed`."""
+    grids = [(64, 64, 64), (96, 96, 64), (128, 128, 128), (128, 128, 192)]
+    r = uniform(seed, 300, np.arange(5 * n, dtype=np.uint64)).reshape(n, 5)
+    out = []
+    for q in range(n):
+        nx, ny, nz = grids[min(int(r[q, 0] * 4), 3)]
+        d_h = 8.0 + 4.0 * r[q, 1]
+        re = 1000.0 + 4000.0 * r[q, 2]
+        npart = 8 + int(r[q, 3] * 193)
+        vol_sphere = 4.0 / 3.0 * math.pi * (0.5 * d_h) ** 3
+        cap = int(0.10 * nx * ny * nz / vol_sphere)
+        out.append(Case(name=f"e{q:02d}_{nx}x{ny}x{nz}", itot=nx, jtot=ny, ktot=nz, lx=1.0, ly=ny / nx,
+                        lz=nz / nx, np=max(1, min(npart, cap)), diameter_h=d_h, ratiorho=1.5, re_bulk=re,
+                        seed=1000 + q, steps=10))
+    return out
