@@ -1,0 +1,73 @@
+"""Full-size checks (256^3 with 1 000 spheres; 512^3 with 8 000 spheres)
+where the oracle would take too long: properties that hold at any size.
+
+- IBM momentum balance: the three passes leave sum(dudt_after - dudt_before)
+  * dVeul = dt * F_total per component, F_total = sum_p FXLTOT_p (the spread
+  weights of every marker sum to one away from the walls; spheres are placed
+  at least R + h from a wall, so no stencil is clipped).  Relative <= 1e-11.
+- The projection: after a full substep max |div u| <= 1e-10 (src/chkdiv.f90
+  definition), the north-star bound for the FP64 path.
+- Graph replay equals the eager launch sequence to <= 1e-13 relative.
+"""
+import numpy as np
+import pytest
+
+from interparts_mpich_amd.inputs import CONFIGS, initial_fields, markers_for, place_spheres
+from interparts_mpich_amd.driver import Simulation
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300)
+
+
+def make(case, graphs=False):
+    print(f"[case {case.name}] {case.itot}x{case.jtot}x{case.ktot} np={case.np} nl={case.nl}", flush=True)
+    return Simulation(case, use_graphs=graphs).initialize(initial_fields(case), place_spheres(case), markers_for(case))
+
+
+@pytest.mark.parametrize("name", ["c2_256", "c4_512"])
+def test_ibm_momentum_balance_and_divergence(name):
+    case = CONFIGS[name]
+    g = make(case)
+    g.rk(1)
+    before = {n: g.get(n)[1:-1, 1:-1, 1:-1].copy() for n in ("DUDT", "DVDT", "DWDT")}
+    g.ibm()
+    dVeul = (case.lx / case.itot) * (case.ly / case.jtot) * (case.lz / case.ktot)
+    for n, q in (("DUDT", "FXLTOT"), ("DVDT", "FYLTOT"), ("DWDT", "FZLTOT")):
+        spread = (g.get(n)[1:-1, 1:-1, 1:-1] - before[n]).sum(dtype=np.float64) * dVeul
+        before[n] = None
+        total = g.dt * g.pget(q).sum()
+        scale = g.dt * np.abs(g.pget(q)).sum()
+        print(n, spread, total, scale)
+        assert abs(spread - total) <= 1e-11 * scale, (n, spread, total)
+    # finish the substep (bounds, Poisson, projection, particles) and check the divergence
+    g.bounduvw(1)
+    g.fillps(1)
+    g.solver2d()
+    g.boundp(1)
+    g.correc(1)
+    g.bounduvw(0)
+    g.pnew_update()
+    g.intgr_nwtn_eulr(1)
+    tot, mx = g.chkdiv()
+    print("div", tot, mx)
+    assert mx <= 1e-10
+    g.close()
+
+
+def test_graph_replay_full_size():
+    case = CONFIGS["c2_256"]
+    a = make(case)
+    b = make(case, graphs=True)
+    for s in range(3):
+        a.substep(s + 1)
+        b.substep(s + 1)
+    b.sync()
+    for n in ("U", "V", "W", "PNEW"):
+        r = rel(b.get(n), a.get(n))
+        print(n, r)
+        assert r <= 1e-13, n
+    for q in ("X", "U", "FXLTOT"):
+        assert rel(b.pget(q), a.pget(q)) <= 1e-13, q
