@@ -75,7 +75,7 @@ int release(ipg_ctx *c) {
   for (int q = 0; q < 64; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
   ipg_poisson_free(c);
-  void *ptrs[] = {c->slab, c->mk, c->fl, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
+  void *ptrs[] = {c->slab, c->mk, c->fl, c->iso, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   for (int b = 0; b < 2; b++)
     for (int d = 0; d < 3; d++) if (c->G[b][d]) (void)hipFree(c->G[b][d]);
@@ -145,6 +145,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
     const size_t M = (size_t)np * prm->nl;
     if (hipMalloc(&c->mk, 3 * (size_t)prm->nl * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->fl, 3 * M * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->iso, (size_t)np * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->op, 24 * (size_t)np * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->ps.nbf, (size_t)np * IPG_NB * sizeof(int)) != hipSuccess) {
       ipg_set_error("out of device memory (particles)"); return fail(1);

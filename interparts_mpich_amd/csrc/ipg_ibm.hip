@@ -144,7 +144,10 @@ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl
 // be gathering from.
 constexpr int IBM_THREADS = 512, IBM_MPT = 2;
 
-__global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_ibm_pass(IbmArgs A, int pass, int gather, int spread) {
+// iso (may be null) marks the isolated spheres, which k_ibm_local handles:
+// their workgroups return at once.
+__global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_ibm_pass(IbmArgs A, int pass, int gather, int spread,
+                                                                                                 const int *__restrict__ iso) {
   extern __shared__ double box[];
   const Grid &g = A.g;
   const int p = blockIdx.x / 3, comp = blockIdx.x % 3;
@@ -153,6 +156,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     const long nplane = (long)g.nx * g.ny;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nplane; t += (long)gridDim.x * blockDim.x) A.Kzero[t] = 0.0;
   }
+  if (iso && iso[p]) return;
   const double dt = *A.dtp, dti = 1.0 / dt;
   const double c0 = (A.dVlagr / A.dVeul) * dt;
   const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
@@ -324,6 +328,161 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   }
 }
 
+// The three IBM passes (src/main.f90:299-334) of an isolated sphere
+// (k_ibm_isolate): its spreads touch no cell any other sphere reads or writes,
+// so the workgroup keeps them in LDS between the passes and adds the last
+// one to dudt at the end.  Workgroups of the other spheres return at once.
+// Every stencil of an isolated sphere lies inside its box (the box spans
+// the markers' cells with two cells to spare and stops below the k1 plane),
+// so a marker's 27 cells are e0 + i + bx (j + by k) in LDS.
+__global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_ibm_local(IbmArgs A, const int *__restrict__ iso) {
+  extern __shared__ double box[];
+  const Grid &g = A.g;
+  const int p = blockIdx.x / 3, comp = blockIdx.x % 3;
+  const long M = (long)A.np * A.nl;
+  if (!iso[p]) return;
+  const double dt = *A.dtp, dti = 1.0 / dt;
+  const double c0 = (A.dVlagr / A.dVeul) * dt;
+  const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
+  const Box b = sphere_box(A, xp, yp, zp);
+  const int nbox = b.bx * b.by * b.bz, bxy = b.bx * b.by;
+  unsigned *flag = (unsigned *)(box + A.bmx * A.bmy * A.bmz);
+  const int nflag = (nbox + 31) >> 5;
+  const double *F = A.fld[comp];
+  double *Fw = A.fld[comp];
+  double *fl = A.fl + comp * M;
+  const long m0 = (long)p * A.nl;
+  double wx[IBM_MPT][3], wy[IBM_MPT][3], wz[IBM_MPT][3], ub[IBM_MPT];
+  int e0[IBM_MPT];
+#pragma unroll
+  for (int q = 0; q < IBM_MPT; q++) {
+    const int l = (int)threadIdx.x + q * IBM_THREADS;
+    double xl = 0.0, yl = 0.0, zl = 0.0;
+    e0[q] = -1;
+    ub[q] = 0.0;
+    if (l < A.nl && marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) {
+      CompStencil c;
+      comp_stencil(g, xl, yl, zl, comp, c);
+      for (int r = 0; r < 3; r++) { wx[q][r] = c.wx[r]; wy[q][r] = c.wy[r]; wz[q][r] = c.wz[r]; }
+      // inside by construction for an isolated sphere; a marker that were not
+      // would be left out rather than index outside the box
+      if (box_at(b, c.i0, c.j0, c.k0) >= 0 && box_at(b, c.i0 + 2, c.j0 + 2, c.k0 + 2) >= 0)
+        e0[q] = box_at(b, c.i0, c.j0, c.k0);
+      if (comp == 0) ub[q] = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
+      else if (comp == 1) ub[q] = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
+      else ub[q] = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
+    }
+  }
+  for (int t = threadIdx.x; t < nflag; t += blockDim.x) flag[t] = 0u;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < IBM_MPT; q++) {
+    if (e0[q] < 0) continue;
+    for (int k = 0; k < 3; k++)
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          const int e = e0[q] + i + b.bx * j + bxy * k;
+          atomicOr(&flag[e >> 5], 1u << (e & 31));
+        }
+  }
+  __syncthreads();
+  auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
+  auto cell_of = [&](int t) {
+    const int lk = t / bxy, r = t - lk * bxy, lj = r / b.bx, li = r - lj * b.bx;
+    return g.at(wrapany(b.bi0 + li, g.nx), wrapany(b.bj0 + lj, g.ny), b.bk0 + lk);
+  };
+  // Before each gather the box holds the previous pass's spread S (zero at
+  // pass 0) and becomes fl(dudt + S) on the flagged cells, the dudtf the
+  // reference gathers from; after each spread it holds the new S.
+  constexpr int CH = 8;
+#pragma unroll 1
+  for (int it = 0; it < 3; it++) {
+    for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * (int)blockDim.x) {
+      double v[CH];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int t = t0 + u * (int)blockDim.x;
+        v[u] = 0.0;
+        if (t < nbox && flagged(t)) {
+          v[u] = F[cell_of(t)];
+          if (it > 0) v[u] = v[u] + box[t];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int t = t0 + u * (int)blockDim.x;
+        if (t < nbox) box[t] = v[u];
+      }
+    }
+    __syncthreads();
+    double fnew[IBM_MPT];
+#pragma unroll
+    for (int q = 0; q < IBM_MPT; q++) {
+      fnew[q] = 0.0;
+      if (e0[q] < 0) continue;
+      const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
+      double acc = 0.0;
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) acc = acc + box[e0[q] + i + b.bx * j + bxy * k] * wx[q][i] * wy[q][j] * wz[q][k];
+      fnew[q] = it == 0 ? (ub[q] - acc) * dti : fl[m] + (ub[q] - acc) * dti;
+      fl[m] = fnew[q];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < IBM_MPT; q++) {
+      if (e0[q] < 0) continue;
+      const double fc = fnew[q] * c0;
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) unsafeAtomicAdd(&box[e0[q] + i + b.bx * j + bxy * k], fc * wx[q][i] * wy[q][j] * wz[q][k]);
+    }
+    __syncthreads();
+  }
+  // dudt = dudtf = fl(dudt + S) on this sphere's cells (src/main.f90:332-334)
+  for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+    if (!flagged(t)) continue;
+    const long cc = cell_of(t);
+    Fw[cc] = Fw[cc] + box[t];
+  }
+}
+
+// a box [a0, a0+an) and a box [b0, b0+bn) of one periodic direction of n
+// cells overlap once wrapped
+__device__ __forceinline__ bool overlap_periodic(int a0, int an, int b0, int bn, int n) {
+  for (int s = -2; s <= 2; s++) {
+    const int c0 = b0 + s * n;
+    if (a0 < c0 + bn && c0 < a0 + an) return true;
+  }
+  return false;
+}
+
+// iso[p] = 1 when sphere p's box overlaps no other sphere's box (periodic in
+// x and y), does not wrap onto itself, and no w stencil of its markers can
+// reach the k1 = kmax+1 plane (src/interp_spread.f90:100); its IBM passes
+// then touch cells no other sphere touches.
+__global__ __launch_bounds__(256) void k_ibm_isolate(IbmArgs A, int *iso) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.np) return;
+  const Grid &g = A.g;
+  const Box bp = sphere_box(A, A.px[p], A.py[p], A.pz[p]);
+  bool ok = bp.bx < g.nx && bp.by < g.ny && (A.pz[p] + A.radfp) * g.dzi + 2.0 < (double)g.nz;
+  for (int q = 0; q < A.np && ok; q++) {
+    if (q == p) continue;
+    const Box bq = sphere_box(A, A.px[q], A.py[q], A.pz[q]);
+    if (bq.bk0 + bq.bz <= bp.bk0 || bp.bk0 + bp.bz <= bq.bk0) continue;
+    if (!overlap_periodic(bp.bi0, bp.bx, bq.bi0, bq.bx, g.nx)) continue;
+    if (!overlap_periodic(bp.bj0, bp.by, bq.bj0, bq.by, g.ny)) continue;
+    ok = false;
+  }
+  iso[p] = ok ? 1 : 0;
+}
+
 // Per-sphere force and torque sums after the last pass, src/forcing.f90:258-287,
 // one block per sphere.
 __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, double *fyt, double *fzt, double *tqx, double *tqy,
@@ -375,6 +534,7 @@ int ipg_ibm_setup(ipg_ctx *c) {
   if (lds > 160 * 1024) { ipg_set_error("ibm: sphere box exceeds the 160 KiB LDS of a CU"); return 1; }
   if (c->pc.nl > IBM_THREADS * IBM_MPT) { ipg_set_error("ibm: more than 1024 markers per sphere"); return 1; }
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  IPG_HIP(hipFuncSetAttribute((const void *)k_ibm_local, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   c->ibm_lds_set = true;
   return 0;
 }
@@ -446,7 +606,8 @@ int ipg_launch_ibm_pass(ipg_ctx *c, int pass, int gather, int spread) {
   if (!ibm_args(c, A, lds)) return 1;
   if (pass == 2 && gather && spread) { ipg_set_error("ibm: the last pass gathers and spreads in two launches"); return 1; }
   ibm_roles(c, A, pass, gather, spread);
-  hipLaunchKernelGGL(k_ibm_pass, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, pass, gather, spread);
+  hipLaunchKernelGGL(k_ibm_pass, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, pass, gather, spread,
+                     (const int *)nullptr);
   static const char *names[3][2] = {{"ibm_pass0_spread", "ibm_pass0"}, {"ibm_pass1_spread", "ibm_pass1"},
                                     {"ibm_pass2_spread", "ibm_pass2_gather"}};
   ipg_mark(c, gather ? (spread ? names[pass][1] : (pass == 2 ? names[2][1] : "ibm_gather")) : names[pass][0]);
@@ -471,13 +632,27 @@ int ipg_launch_ibm_totals(ipg_ctx *c, int last) {
   return 0;
 }
 
-// The whole IBM block of one substep, src/main.f90:296-338.
+// The whole IBM block of one substep, src/main.f90:296-338.  Isolated
+// spheres run their three passes in the first launch; the passes through the
+// global force buffers then serve only the spheres near another sphere or the
+// top wall.
 int ipg_launch_ibm(ipg_ctx *c) {
   if (c->pc.np <= 0) return 0;
-  if (ipg_launch_ibm_begin(c)) return 1;
-  if (ipg_launch_ibm_pass(c, 0, 1, 1)) return 1;
-  if (ipg_launch_ibm_pass(c, 1, 1, 1)) return 1;
-  if (ipg_launch_ibm_pass(c, 2, 1, 0)) return 1;
-  if (ipg_launch_ibm_pass(c, 2, 0, 1)) return 1;
+  IbmArgs A; size_t lds;
+  if (!ibm_args(c, A, lds)) return 1;
+  hipLaunchKernelGGL(k_ibm_isolate, dim3((A.np + 255) / 256), dim3(256), 0, c->stream, A, c->iso);
+  ipg_mark(c, "ibm_isolate");
+  hipLaunchKernelGGL(k_ibm_local, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, (const int *)c->iso);
+  ipg_mark(c, "ibm_local");
+  const int roles[4][3] = {{0, 1, 1}, {1, 1, 1}, {2, 1, 0}, {2, 0, 1}};
+  static const char *names[4] = {"ibm_pass0", "ibm_pass1", "ibm_pass2_gather", "ibm_pass2_spread"};
+  for (int r = 0; r < 4; r++) {
+    IbmArgs B = A;
+    ibm_roles(c, B, roles[r][0], roles[r][1], roles[r][2]);
+    hipLaunchKernelGGL(k_ibm_pass, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, B, roles[r][0], roles[r][1],
+                       roles[r][2], (const int *)c->iso);
+    ipg_mark(c, names[r]);
+  }
+  IPG_HIP(hipGetLastError());
   return ipg_launch_ibm_totals(c, 1);
 }

@@ -69,6 +69,7 @@ struct ipg_ctx {
   int tune_rk_kb;
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
   double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl
+  int *iso;                       // [np] sphere isolated for the IBM (see k_ibm_isolate)
   // particles
   PartState ps;
   double *op;                     // [24][np] state at substep start (src/intgr_nwtn_eulr.f90:77-118)

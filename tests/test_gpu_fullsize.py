@@ -2,9 +2,11 @@
 where the oracle would take too long: properties that hold at any size.
 
 - IBM momentum balance: the three passes leave sum(dudt_after - dudt_before)
-  * dVeul = dt * F_total per component, F_total = sum_p FXLTOT_p (the spread
-  weights of every marker sum to one away from the walls; spheres are placed
-  at least R + h from a wall, so no stencil is clipped).  Relative <= 1e-11.
+  * dVeul = dt * F_total per component, F_total = sum_p FXLTOT_p: the spread
+  weights of a marker sum to one (relative <= 1e-11).  u and v always; w only
+  when no marker sits within 1.5 h of the bottom wall, where the reference
+  clamps the w stencil to k >= 1 (src/interp_spread.f90:100) and the weight of
+  the wall plane is dropped.
 - The projection: after a full substep max |div u| <= 1e-10 (src/chkdiv.f90
   definition), the north-star bound for the FP64 path.
 - Graph replay equals the eager launch sequence to <= 1e-13 relative.
@@ -35,7 +37,13 @@ def test_ibm_momentum_balance_and_divergence(name):
     before = {n: g.get(n)[1:-1, 1:-1, 1:-1].copy() for n in ("DUDT", "DVDT", "DWDT")}
     g.ibm()
     dVeul = (case.lx / case.itot) * (case.ly / case.jtot) * (case.lz / case.ktot)
+    theta, phi, radfp = markers_for(case)
+    zl = place_spheres(case)[2][:, None] + radfp * np.cos(theta)[None, :]
+    w_clamped = bool(np.any(np.floor(zl * case.ktot / case.lz + 0.5) - 1 < 1))
     for n, q in (("DUDT", "FXLTOT"), ("DVDT", "FYLTOT"), ("DWDT", "FZLTOT")):
+        if n == "DWDT" and w_clamped:
+            print("w stencils clamped at the bottom wall: w balance not exact")
+            continue
         spread = (g.get(n)[1:-1, 1:-1, 1:-1] - before[n]).sum(dtype=np.float64) * dVeul
         before[n] = None
         total = g.dt * g.pget(q).sum()
