@@ -3,6 +3,7 @@
 // the substep sequence of src/main.f90:268-414, optionally captured as one
 // hipGraph per RK stage.
 #include "ipg_internal.h"
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -75,10 +76,9 @@ int release(ipg_ctx *c) {
   for (int q = 0; q < 64; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
   ipg_poisson_free(c);
-  void *ptrs[] = {c->slab, c->mk, c->fl, c->iso, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
+  void *ptrs[] = {c->slab, c->mk, c->fl, c->nbl, c->nbc, c->ibm_err, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   for (int b = 0; b < 2; b++)
-    for (int d = 0; d < 3; d++) if (c->G[b][d]) (void)hipFree(c->G[b][d]);
   for (int q = 0; q < IPG_NPART; q++) if (c->ps.q[q]) (void)hipFree(c->ps.q[q]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   return 0;
@@ -133,11 +133,6 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
   const size_t ncell = (size_t)g.nx * g.ny * g.nz, nplane = (size_t)g.nx * g.ny;
   const int np = prm->np;
   if (np > 0) {
-    for (int b = 0; b < 2; b++)
-      for (int d = 0; d < 3; d++) {
-        if (hipMalloc(&c->G[b][d], ncell * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory (G)"); return fail(1); }
-        if (hipMemset(c->G[b][d], 0, ncell * sizeof(double)) != hipSuccess) { ipg_set_error("memset failed"); return fail(1); }
-      }
     for (int b = 0; b < 3; b++) {
       if (hipMalloc(&c->K1[b], nplane * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory"); return fail(1); }
       if (hipMemset(c->K1[b], 0, nplane * sizeof(double)) != hipSuccess) { ipg_set_error("memset failed"); return fail(1); }
@@ -145,12 +140,16 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
     const size_t M = (size_t)np * prm->nl;
     if (hipMalloc(&c->mk, 3 * (size_t)prm->nl * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->fl, 3 * M * sizeof(double)) != hipSuccess ||
-        hipMalloc(&c->iso, (size_t)np * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->nbl, (size_t)np * 64 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->nbc, (size_t)np * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->ibm_err, sizeof(int)) != hipSuccess ||
         hipMalloc(&c->op, 24 * (size_t)np * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->ps.nbf, (size_t)np * IPG_NB * sizeof(int)) != hipSuccess) {
       ipg_set_error("out of device memory (particles)"); return fail(1);
     }
     (void)hipMemset(c->fl, 0, 3 * M * sizeof(double));
+    (void)hipMemset(c->nbc, 0, (size_t)np * sizeof(int));
+    (void)hipMemset(c->ibm_err, 0, sizeof(int));
     (void)hipMemset(c->ps.nbf, 0, (size_t)np * IPG_NB * sizeof(int));
     for (int q = 0; q < IPG_NPART; q++) {
       if (hipMalloc(&c->ps.q[q], (size_t)np * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory"); return fail(1); }
@@ -290,20 +289,20 @@ int ipg_ibm(ipg_ctx *c) {
   if (!check_ctx(c)) return 1;
   if (ipg_launch_ibm(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
-  return 0;
+  return ipg_ibm_check(c);
 }
 
 int ipg_ibm_begin(ipg_ctx *c) {
   if (!check_ctx(c)) return 1;
   if (ipg_launch_ibm_begin(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
-  return 0;
+  return ipg_ibm_check(c);
 }
 
 int ipg_eulr2lagr(ipg_ctx *c, int ibmiter) {
   if (!check_ctx(c)) return 1;
   if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("eulr2lagr: ibmiter must be 0..2"); return 1; }
-  if (ipg_launch_ibm_pass(c, ibmiter, 1, 0)) return 1;
+  if (ipg_launch_ibm_gather(c, ibmiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -311,7 +310,7 @@ int ipg_eulr2lagr(ipg_ctx *c, int ibmiter) {
 int ipg_lagrforces(ipg_ctx *c, int ibmiter) {
   if (!check_ctx(c)) return 1;
   if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("lagrforces: ibmiter must be 0..2"); return 1; }
-  if (ipg_launch_ibm_totals(c, ibmiter == 2)) return 1;
+  if (ipg_launch_ibm_totals(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -319,7 +318,8 @@ int ipg_lagrforces(ipg_ctx *c, int ibmiter) {
 int ipg_lagr2eulr(ipg_ctx *c, int ibmiter) {
   if (!check_ctx(c)) return 1;
   if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("lagr2eulr: ibmiter must be 0..2"); return 1; }
-  if (ipg_launch_ibm_pass(c, ibmiter, 0, 1)) return 1;
+  // the spreads of passes 0 and 1 are rebuilt by the next eulr2lagr
+  if (ibmiter == 2 && ipg_launch_ibm_final(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -447,7 +447,7 @@ int ipg_chkdiv(ipg_ctx *c, double *tot, double *mx) {
 int ipg_sync(ipg_ctx *c) {
   if (!check_ctx(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
-  return 0;
+  return ipg_ibm_check(c);
 }
 
 int ipg_substep_phases(ipg_ctx *c, int rkiter, int maxn, double *ms, const char **names, int *n) {
@@ -488,14 +488,20 @@ int ipg_use_graphs(ipg_ctx *c, int on) {
 // stencils x3 components of marker-unique cells counted once, markers 15 words.
 int ipg_substep_bytes(ipg_ctx *c, double *bytes) {
   if (!check_ctx(c) || !bytes) return 1;
+  // FP64 words per substep, by kernel: RK stage 43/3 N (13, 17, 13 by stage),
+  // forward x FFT with fillps 3 N + Sw, two y passes 2 Sw each, Thomas 5 Sw,
+  // inverse x FFT Sw + N, correc 11 N; Sw = 2 (nx/2+1) ny nz.
   const double N = (double)c->g.nx * c->g.ny * c->g.nz;
-  const double s = 8.0;
-  double fluid = (14.0 + 1.0 / 3.0) * N + 4.0 * N + 13.0 * N + 11.0 * N;
-  const double np = c->pc.np, R = c->pc.radius, h = c->g.dx;
-  double cells = np * 4.0 * 3.14159265358979 * (R - 0.3 * h) * (R - 0.3 * h) * 3.0 * h / (h * h * h);
-  double ibm = 3.0 * (3.0 * cells * 3.0 + 15.0 * np * c->pc.nl);
-  double box = np * 3.0 * std::pow(2.0 * R / h + 5.0, 3.0);
-  *bytes = s * (fluid + ibm + box);
+  const double Sw = 2.0 * (c->g.nx / 2 + 1) * (double)c->g.ny * c->g.nz;
+  const double fluid = (43.0 / 3.0 + 3.0 + 1.0 + 11.0) * N + (1.0 + 4.0 + 5.0 + 1.0) * Sw;
+  // IBM: per pass and component the stencil shell (radfp +- 2.5 h) is read
+  // once from dudt (+ G) and receives one atomic add; sphere integrals read
+  // each sphere's box once per component
+  const double np = c->pc.np, R = c->pc.radius, rf = c->pc.radfp, h = c->g.dx;
+  const double shell = (4.0 / 3.0) * 3.14159265358979 * (std::pow(rf / h + 2.5, 3.0) - std::pow(std::max(rf / h - 2.5, 0.0), 3.0));
+  const double ibm = np * 3.0 * 3.0 * 3.0 * shell;
+  const double box = np * 3.0 * std::pow(2.0 * R / h + 5.0, 3.0);
+  *bytes = 8.0 * (fluid + ibm + box);
   return 0;
 }
 

@@ -4,23 +4,30 @@
 // (complagrforces / updtlagrforces, src/forcing.f90:9-391) and spreading
 // (lagr2eulr, src/interp_spread.f90:230-474).
 //
-// The reference copies dudt into a 2-ghost scratch field, zeroes it before
-// every spread, scatters with atomics, folds the periodic ghosts and adds the
-// saved dudt back (9 full-field passes per IBM pass).  Here the spread
-// contributions go, with periodic wrap folded into the index, into sparse
-// force buffers G (zero outside the spheres' stencils), and the velocity the
-// markers see is dudt + G: the same sums, fl(G + dudtold), without the
-// full-field passes.  The last pass spreads straight into dudt.  Work is
-// organised one workgroup per (sphere, velocity component): the stencil cells
-// of the sphere's box are staged in LDS for the gathers, and the spread is
-// summed in LDS before one global atomic per touched cell.  The w-stencil of a marker within h/2 of the top
-// wall reaches the k1 = kmax+1 plane (src/interp_spread.f90:100 has no upper
-// clamp); that plane never receives dudt and keeps the last spread until the
-// next one, so it is held in its own planes K.
+// The reference copies dudt into a 2-ghost scratch field dudtf, zeroes it
+// before every spread, scatters the marker forces with atomics, folds the
+// periodic ghosts and adds the saved dudt back: 9 full-field passes per IBM
+// pass.  Here nothing full-field is touched.  One workgroup works on one
+// (sphere, velocity component): the cells its markers' 27-point stencils touch
+// lie in a box around the sphere, and that box is held in LDS.  The spread S
+// of a pass at those cells is the sum over the sphere's own markers and the
+// markers of the spheres whose boxes overlap its box (a neighbour list built
+// once per substep), so each workgroup rebuilds S in LDS from the marker
+// forces of the previous pass and gathers from fl(dudt + S), the dudtf of the
+// reference.  After the last pass each cell's owner (the lowest-numbered
+// sphere whose stencils touch it) adds S to dudt, once.  No force field is
+// kept, and no global atomics are needed except on the top-wall plane: the
+// w-stencil of a marker within h/2 of the top wall reaches the k1 = kmax+1
+// plane (src/interp_spread.f90:100 has no upper clamp), which never receives
+// dudt and keeps the last spread until the next one (planes K).
 #include "ipg_internal.h"
 #include <cmath>
 
 namespace {
+
+constexpr int IBM_THREADS = 512;  // threads per (sphere, component) workgroup
+constexpr int IBM_MPT = 2;        // markers per thread: nl <= 1024
+constexpr int IBM_MAXNB = 64;     // neighbour images per sphere
 
 // src/coordsfp.f90:49-68 / src/intgr_nwtn_eulr.f90:1451-1465 marker position
 // and the isout test; false when the marker is not kept.
@@ -51,16 +58,12 @@ struct IbmArgs {
   const double *mk;   // [3][nl] marker offsets from the sphere centre
   double *fl;         // [3][np*nl] marker forces fxl, fyl, fzl
   double *fld[3];     // dudt dvdt dwdt
-  double *Gread[3];   // spread of the previous pass, added to fld when gathering (null at pass 0)
-  double *Gspread[3]; // spread target; null: spread straight into fld (last pass)
-  double *Gzero[3];   // buffer cleared on this sphere's cells (null: none)
+  const int *nbl;     // [np][IBM_MAXNB] neighbour images: q*25 + (sx+2)*5 + (sy+2)
+  const int *nbc;     // [np] neighbour image count
   double *Kread, *Kspread, *Kzero;  // dwdtf(.,.,k1) planes
   const double *dtp;
 };
 
-__device__ __forceinline__ long gidx(const Grid &g, int wi, int wj, int k) {
-  return (long)(wi - 1) + (long)g.nx * ((long)(wj - 1) + (long)g.ny * (long)(k - 1));
-}
 // periodic wrap of any unwrapped index onto 1..n
 __device__ __forceinline__ int wrapany(int i, int n) {
   int r = i - 1;
@@ -142,12 +145,62 @@ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl
 // gather/spread select the halves; the last pass runs them as two launches
 // because its spread goes straight into dudt, which other spheres may still
 // be gathering from.
-constexpr int IBM_THREADS = 512, IBM_MPT = 2;
 
-// iso (may be null) marks the isolated spheres, which k_ibm_local handles:
-// their workgroups return at once.
-__global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_ibm_pass(IbmArgs A, int pass, int gather, int spread,
-                                                                                                 const int *__restrict__ iso) {
+// a box [a0, a0+an) and a box [b0 + s n, b0 + s n + bn) overlap
+__device__ __forceinline__ bool overlap(int a0, int an, int b0, int bn) { return a0 < b0 + bn && b0 < a0 + an; }
+
+// Neighbour images of each sphere, once per substep: every (q, sx, sy) with
+// q's box, shifted by (sx nx, sy ny), overlapping p's box; q != p (a box
+// never meets its own image: it is narrower than the domain).  One wavefront
+// per sphere, entries in increasing q.
+__global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int *nbc, int *err) {
+  const Grid &g = A.g;
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (p >= A.np) return;
+  const Box bp = sphere_box(A, A.px[p], A.py[p], A.pz[p]);
+  int count = 0;
+  for (int q0 = 0; q0 < A.np; q0 += 64) {
+    const int q = q0 + lane;
+    Box bq{};
+    bool zok = false;
+    if (q < A.np && q != p) {
+      bq = sphere_box(A, A.px[q], A.py[q], A.pz[q]);
+      zok = overlap(bp.bk0, bp.bz, bq.bk0, bq.bz);
+    }
+    for (int sx = -2; sx <= 2; sx++)
+      for (int sy = -2; sy <= 2; sy++) {
+        const bool hit = zok && overlap(bp.bi0, bp.bx, bq.bi0 + sx * g.nx, bq.bx) &&
+                         overlap(bp.bj0, bp.by, bq.bj0 + sy * g.ny, bq.by);
+        const unsigned long long mask = __ballot(hit);
+        if (hit) {
+          const int slot = count + (int)__popcll(mask & ((1ull << lane) - 1ull));
+          if (slot < IBM_MAXNB) nbl[(long)p * IBM_MAXNB + slot] = q * 25 + (sx + 2) * 5 + (sy + 2);
+        }
+        count += (int)__popcll(mask);
+      }
+  }
+  if (lane == 0) {
+    nbc[p] = min(count, IBM_MAXNB);
+    if (count > IBM_MAXNB) atomicOr(err, 1);
+  }
+}
+
+// One IBM launch for one (sphere, velocity component) workgroup.
+//   MODE 0 (pass 0): gather from dudt, set the marker forces.
+//   MODE 1 (passes 1, 2): rebuild the previous pass's spread S in LDS, gather
+//     from fl(dudt + S), add to the marker forces.
+//   MODE 2 (after pass 2): rebuild the last spread S, add it to dudt on the
+//     cells this sphere owns.
+// Thread t owns markers t, t + IBM_THREADS; their positions
+// (src/coordsfp.f90:49-68), stencils and weights (src/interp_spread.f90:76-130)
+// and rigid-body velocities (src/intgr_nwtn_eulr.f90:1471-1476) are computed in
+// registers, as the markers do not move inside the IBM block.  Gathers follow
+// src/interp_spread.f90:132-154 (((f phi_x) phi_y) phi_z); the force update
+// src/forcing.f90:63-65 / 255-257; spreads src/interp_spread.f90:329-355
+// ((((F c0) phi_x) phi_y) phi_z, c0 = dVlagr / dVeul dt).
+template <int MODE>
+__global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_ibm(IbmArgs A, int pass) {
   extern __shared__ double box[];
   const Grid &g = A.g;
   const int p = blockIdx.x / 3, comp = blockIdx.x % 3;
@@ -156,7 +209,6 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     const long nplane = (long)g.nx * g.ny;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nplane; t += (long)gridDim.x * blockDim.x) A.Kzero[t] = 0.0;
   }
-  if (iso && iso[p]) return;
   const double dt = *A.dtp, dti = 1.0 / dt;
   const double c0 = (A.dVlagr / A.dVeul) * dt;
   const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
@@ -164,227 +216,87 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   const int nbox = b.bx * b.by * b.bz, bxy = b.bx * b.by;
   unsigned *flag = (unsigned *)(box + A.bmx * A.bmy * A.bmz);
   const int nflag = (nbox + 31) >> 5;
-  const double *F = A.fld[comp];
+  unsigned *taken = flag + nflag;  // MODE 2: cells a lower-numbered sphere owns
+  double *F = A.fld[comp];
   double *fl = A.fl + comp * M;
   const long m0 = (long)p * A.nl;
-  // this thread's markers
-  CompStencil cs[IBM_MPT];
-  bool ok[IBM_MPT];
-  double ub[IBM_MPT];
-#pragma unroll
-  for (int q = 0; q < IBM_MPT; q++) {
-    const int l = (int)threadIdx.x + q * IBM_THREADS;
-    double xl = 0.0, yl = 0.0, zl = 0.0;
-    ok[q] = l < A.nl && marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl);
-    ub[q] = 0.0;
-    if (ok[q]) {
-      comp_stencil(g, xl, yl, zl, comp, cs[q]);
-      if (comp == 0) ub[q] = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
-      else if (comp == 1) ub[q] = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
-      else ub[q] = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
-    }
-  }
-  for (int t = threadIdx.x; t < nflag; t += blockDim.x) flag[t] = 0u;
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < IBM_MPT; q++) {
-    if (!ok[q]) continue;
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          const int e = box_at(b, cs[q].i0 + i, cs[q].j0 + j, cs[q].k0 + k);
-          if (e >= 0) atomicOr(&flag[e >> 5], 1u << (e & 31));
-        }
-  }
-  __syncthreads();
-  auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
-  auto cell_of = [&](int t, int &wi, int &wj, int &k) {
-    const int lk = t / bxy, r = t - lk * bxy, lj = r / b.bx, li = r - lj * b.bx;
-    wi = wrapany(b.bi0 + li, g.nx); wj = wrapany(b.bj0 + lj, g.ny); k = b.bk0 + lk;
-  };
-  if (gather) {
-    // staged in chunks of 8 cells per thread so that the loads of a chunk are
-    // all in flight together
-    const double *Gr = A.Gread[comp];
-    constexpr int CH = 8;
-    for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * (int)blockDim.x) {
-      double v[CH];
-#pragma unroll
-      for (int u = 0; u < CH; u++) {
-        const int t = t0 + u * (int)blockDim.x;
-        v[u] = 0.0;
-        if (t < nbox && flagged(t)) {
-          int wi, wj, k;
-          cell_of(t, wi, wj, k);
-          v[u] = F[g.at(wi, wj, k)];
-          if (Gr) v[u] = v[u] + Gr[gidx(g, wi, wj, k)];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < CH; u++) {
-        const int t = t0 + u * (int)blockDim.x;
-        if (t < nbox) box[t] = v[u];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < IBM_MPT; q++) {
-      if (!ok[q]) continue;
-      const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
-      const CompStencil &c = cs[q];
-      double acc = 0.0;
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            const int iu = c.i0 + i, ju = c.j0 + j, kk = c.k0 + k;
-            double f;
-            const int e = box_at(b, iu, ju, kk);
-            if (e >= 0) f = box[e];
-            else if (kk >= 1 && kk <= g.nz) {
-              const int wi = wrapany(iu, g.nx), wj = wrapany(ju, g.ny);
-              f = F[g.at(wi, wj, kk)];
-              if (A.Gread[comp]) f = f + A.Gread[comp][gidx(g, wi, wj, kk)];
-            } else if (kk == g.nz + 1 && comp == 2) {
-              f = A.Kread[(long)(wrapany(iu, g.nx) - 1) + (long)g.nx * (wrapany(ju, g.ny) - 1)];
-            } else f = 0.0;
-            acc = acc + f * c.wx[i] * c.wy[j] * c.wz[k];
-          }
-      if (pass == 0) fl[m] = (ub[q] - acc) * dti;
-      else fl[m] = fl[m] + (ub[q] - acc) * dti;
-    }
-    __syncthreads();
-  }
-  if (spread) {
-    double *Gw = A.Gspread[comp];
-    for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < IBM_MPT; q++) {
-      if (!ok[q]) continue;
-      const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
-      const CompStencil &c = cs[q];
-      const double fc = fl[m] * c0;
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            const int iu = c.i0 + i, ju = c.j0 + j, kk = c.k0 + k;
-            const double v = fc * c.wx[i] * c.wy[j] * c.wz[k];
-            const int e = box_at(b, iu, ju, kk);
-            if (e >= 0) unsafeAtomicAdd(&box[e], v);
-            else if (kk >= 1 && kk <= g.nz) {
-              const int wi = wrapany(iu, g.nx), wj = wrapany(ju, g.ny);
-              if (Gw) unsafeAtomicAdd(&Gw[gidx(g, wi, wj, kk)], v);
-              else unsafeAtomicAdd(&A.fld[comp][g.at(wi, wj, kk)], v);
-            } else if (kk == g.nz + 1 && comp == 2) {
-              unsafeAtomicAdd(&A.Kspread[(long)(wrapany(iu, g.nx) - 1) + (long)g.nx * (wrapany(ju, g.ny) - 1)], v);
-            }
-          }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
-      if (!flagged(t)) continue;
-      const double v = box[t];
-      if (v != 0.0) {
-        int wi, wj, k;
-        cell_of(t, wi, wj, k);
-        if (Gw) unsafeAtomicAdd(&Gw[gidx(g, wi, wj, k)], v);
-        else unsafeAtomicAdd(&A.fld[comp][g.at(wi, wj, k)], v);
-      }
-    }
-  }
-  if (A.Gzero[comp]) {
-    // the cells this sphere spread into in an earlier pass: its flagged cells
-    // (the markers have not moved) and any stencil cell outside the box
-    double *Gz = A.Gzero[comp];
-    for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
-      if (!flagged(t)) continue;
-      int wi, wj, k;
-      cell_of(t, wi, wj, k);
-      Gz[gidx(g, wi, wj, k)] = 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < IBM_MPT; q++) {
-      if (!ok[q]) continue;
-      const CompStencil &c = cs[q];
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            const int iu = c.i0 + i, ju = c.j0 + j, kk = c.k0 + k;
-            if (box_at(b, iu, ju, kk) < 0 && kk >= 1 && kk <= g.nz) Gz[gidx(g, wrapany(iu, g.nx), wrapany(ju, g.ny), kk)] = 0.0;
-          }
-    }
-  }
-}
-
-// The three IBM passes (src/main.f90:299-334) of an isolated sphere
-// (k_ibm_isolate): its spreads touch no cell any other sphere reads or writes,
-// so the workgroup keeps them in LDS between the passes and adds the last
-// one to dudt at the end.  Workgroups of the other spheres return at once.
-// Every stencil of an isolated sphere lies inside its box (the box spans
-// the markers' cells with two cells to spare and stops below the k1 plane),
-// so a marker's 27 cells are e0 + i + bx (j + by k) in LDS.
-__global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_ibm_local(IbmArgs A, const int *__restrict__ iso) {
-  extern __shared__ double box[];
-  const Grid &g = A.g;
-  const int p = blockIdx.x / 3, comp = blockIdx.x % 3;
-  const long M = (long)A.np * A.nl;
-  if (!iso[p]) return;
-  const double dt = *A.dtp, dti = 1.0 / dt;
-  const double c0 = (A.dVlagr / A.dVeul) * dt;
-  const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
-  const Box b = sphere_box(A, xp, yp, zp);
-  const int nbox = b.bx * b.by * b.bz, bxy = b.bx * b.by;
-  unsigned *flag = (unsigned *)(box + A.bmx * A.bmy * A.bmz);
-  const int nflag = (nbox + 31) >> 5;
-  const double *F = A.fld[comp];
-  double *Fw = A.fld[comp];
-  double *fl = A.fl + comp * M;
-  const long m0 = (long)p * A.nl;
+  // own markers: stencil weights, LDS index of the stencil's first cell, and
+  // how many of its z planes lie in the box (the rest is the k1 plane)
   double wx[IBM_MPT][3], wy[IBM_MPT][3], wz[IBM_MPT][3], ub[IBM_MPT];
-  int e0[IBM_MPT];
+  int e0[IBM_MPT], nzin[IBM_MPT], kx0[IBM_MPT], ky0[IBM_MPT];
 #pragma unroll
   for (int q = 0; q < IBM_MPT; q++) {
     const int l = (int)threadIdx.x + q * IBM_THREADS;
     double xl = 0.0, yl = 0.0, zl = 0.0;
-    e0[q] = -1;
-    ub[q] = 0.0;
+    e0[q] = -1; nzin[q] = 0; kx0[q] = 0; ky0[q] = 0; ub[q] = 0.0;
     if (l < A.nl && marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) {
       CompStencil c;
       comp_stencil(g, xl, yl, zl, comp, c);
+#pragma unroll
       for (int r = 0; r < 3; r++) { wx[q][r] = c.wx[r]; wy[q][r] = c.wy[r]; wz[q][r] = c.wz[r]; }
-      // inside by construction for an isolated sphere; a marker that were not
-      // would be left out rather than index outside the box
-      if (box_at(b, c.i0, c.j0, c.k0) >= 0 && box_at(b, c.i0 + 2, c.j0 + 2, c.k0 + 2) >= 0)
-        e0[q] = box_at(b, c.i0, c.j0, c.k0);
+      // the box holds the stencil in x and y, and in z up to kmax
+      // (sphere_box); a stencil that were not held would be left out rather
+      // than index outside the box
+      const int ein = box_at(b, c.i0, c.j0, c.k0);
+      if (ein >= 0 && box_at(b, c.i0 + 2, c.j0 + 2, c.k0) >= 0) {
+        e0[q] = ein;
+        nzin[q] = min(3, b.bk0 + b.bz - c.k0);
+        kx0[q] = c.i0; ky0[q] = c.j0;
+      }
       if (comp == 0) ub[q] = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
       else if (comp == 1) ub[q] = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
       else ub[q] = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
     }
   }
-  for (int t = threadIdx.x; t < nflag; t += blockDim.x) flag[t] = 0u;
+  for (int t = threadIdx.x; t < nflag; t += blockDim.x) {
+    flag[t] = 0u;
+    if (MODE == 2) taken[t] = 0u;
+  }
+  if (MODE >= 1)
+    for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
   __syncthreads();
+  // flag the own stencil cells; with MODE >= 1 also spread the own forces of
+  // the previous pass into S
 #pragma unroll
   for (int q = 0; q < IBM_MPT; q++) {
     if (e0[q] < 0) continue;
-    for (int k = 0; k < 3; k++)
+    const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
+    const double fc = MODE >= 1 ? fl[m] * c0 : 0.0;
+    for (int k = 0; k < nzin[q]; k++)
       for (int j = 0; j < 3; j++)
 #pragma unroll
         for (int i = 0; i < 3; i++) {
           const int e = e0[q] + i + b.bx * j + bxy * k;
           atomicOr(&flag[e >> 5], 1u << (e & 31));
+          if (MODE >= 1) unsafeAtomicAdd(&box[e], fc * wx[q][i] * wy[q][j] * wz[q][k]);
         }
+  }
+  if (MODE >= 1) {
+    // the neighbours' markers: their contributions to this box
+    const int nn = A.nbc[p];
+    const long tot = (long)nn * A.nl;
+    for (long idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+      const int n = (int)(idx / A.nl), l = (int)(idx - (long)n * A.nl);
+      const int code = A.nbl[(long)p * IBM_MAXNB + n];
+      const int qs = code / 25, sxs = (code / 5) % 5 - 2, sys = code % 5 - 2;
+      const double xq = A.px[qs], yq = A.py[qs], zq = A.pz[qs];
+      double xl, yl, zl;
+      if (!marker_pos(g, xq, yq, zq, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
+      CompStencil c;
+      comp_stencil(g, xl, yl, zl, comp, c);
+      const int i0 = c.i0 + sxs * g.nx, j0 = c.j0 + sys * g.ny;
+      if (i0 + 2 < b.bi0 || i0 >= b.bi0 + b.bx || j0 + 2 < b.bj0 || j0 >= b.bj0 + b.by) continue;
+      const double fc = fl[(long)qs * A.nl + l] * c0;
+      for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            const int e = box_at(b, i0 + i, j0 + j, c.k0 + k);
+            if (e < 0) continue;
+            unsafeAtomicAdd(&box[e], fc * c.wx[i] * c.wy[j] * c.wz[k]);
+            if (MODE == 2 && qs < p) atomicOr(&taken[e >> 5], 1u << (e & 31));
+          }
+    }
   }
   __syncthreads();
   auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
@@ -392,95 +304,64 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     const int lk = t / bxy, r = t - lk * bxy, lj = r / b.bx, li = r - lj * b.bx;
     return g.at(wrapany(b.bi0 + li, g.nx), wrapany(b.bj0 + lj, g.ny), b.bk0 + lk);
   };
-  // Before each gather the box holds the previous pass's spread S (zero at
-  // pass 0) and becomes fl(dudt + S) on the flagged cells, the dudtf the
-  // reference gathers from; after each spread it holds the new S.
+  if (MODE == 2) {
+    // dudt = dudtf = fl(dudt + S) (src/main.f90:332-334), once per cell
+    for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+      if (!flagged(t) || ((taken[t >> 5] >> (t & 31)) & 1u)) continue;
+      const long cc = cell_of(t);
+      F[cc] = F[cc] + box[t];
+    }
+    return;
+  }
+  // stage fl(dudt + S) (MODE 1) or dudt (MODE 0) on the flagged cells, in
+  // chunks of 8 cells per thread so that the loads of a chunk are all in
+  // flight together
   constexpr int CH = 8;
-#pragma unroll 1
-  for (int it = 0; it < 3; it++) {
-    for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * (int)blockDim.x) {
-      double v[CH];
+  for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * (int)blockDim.x) {
+    double v[CH];
 #pragma unroll
-      for (int u = 0; u < CH; u++) {
-        const int t = t0 + u * (int)blockDim.x;
-        v[u] = 0.0;
-        if (t < nbox && flagged(t)) {
-          v[u] = F[cell_of(t)];
-          if (it > 0) v[u] = v[u] + box[t];
+    for (int u = 0; u < CH; u++) {
+      const int t = t0 + u * (int)blockDim.x;
+      v[u] = 0.0;
+      if (t < nbox && flagged(t)) {
+        v[u] = F[cell_of(t)];
+        if (MODE == 1) v[u] = v[u] + box[t];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const int t = t0 + u * (int)blockDim.x;
+      if (t < nbox) box[t] = v[u];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < IBM_MPT; q++) {
+    if (e0[q] < 0) continue;
+    const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
+    double acc = 0.0;
+    for (int k = 0; k < 3; k++)
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          double f;
+          if (k < nzin[q]) f = box[e0[q] + i + b.bx * j + bxy * k];
+          else f = A.Kread[(long)(wrapany(kx0[q] + i, g.nx) - 1) + (long)g.nx * (wrapany(ky0[q] + j, g.ny) - 1)];
+          acc = acc + f * wx[q][i] * wy[q][j] * wz[q][k];
         }
-      }
-#pragma unroll
-      for (int u = 0; u < CH; u++) {
-        const int t = t0 + u * (int)blockDim.x;
-        if (t < nbox) box[t] = v[u];
-      }
-    }
-    __syncthreads();
-    double fnew[IBM_MPT];
-#pragma unroll
-    for (int q = 0; q < IBM_MPT; q++) {
-      fnew[q] = 0.0;
-      if (e0[q] < 0) continue;
-      const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
-      double acc = 0.0;
-      for (int k = 0; k < 3; k++)
+    const double fnew = MODE == 0 ? (ub[q] - acc) * dti : fl[m] + (ub[q] - acc) * dti;
+    fl[m] = fnew;
+    // the part of this pass's spread that falls on the k1 plane
+    if (nzin[q] < 3) {
+      const double fc = fnew * c0;
+      for (int k = nzin[q]; k < 3; k++)
         for (int j = 0; j < 3; j++)
 #pragma unroll
-          for (int i = 0; i < 3; i++) acc = acc + box[e0[q] + i + b.bx * j + bxy * k] * wx[q][i] * wy[q][j] * wz[q][k];
-      fnew[q] = it == 0 ? (ub[q] - acc) * dti : fl[m] + (ub[q] - acc) * dti;
-      fl[m] = fnew[q];
+          for (int i = 0; i < 3; i++)
+            unsafeAtomicAdd(&A.Kspread[(long)(wrapany(kx0[q] + i, g.nx) - 1) + (long)g.nx * (wrapany(ky0[q] + j, g.ny) - 1)],
+                            fc * wx[q][i] * wy[q][j] * wz[q][k]);
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < IBM_MPT; q++) {
-      if (e0[q] < 0) continue;
-      const double fc = fnew[q] * c0;
-      for (int k = 0; k < 3; k++)
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-          for (int i = 0; i < 3; i++) unsafeAtomicAdd(&box[e0[q] + i + b.bx * j + bxy * k], fc * wx[q][i] * wy[q][j] * wz[q][k]);
-    }
-    __syncthreads();
   }
-  // dudt = dudtf = fl(dudt + S) on this sphere's cells (src/main.f90:332-334)
-  for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
-    if (!flagged(t)) continue;
-    const long cc = cell_of(t);
-    Fw[cc] = Fw[cc] + box[t];
-  }
-}
-
-// a box [a0, a0+an) and a box [b0, b0+bn) of one periodic direction of n
-// cells overlap once wrapped
-__device__ __forceinline__ bool overlap_periodic(int a0, int an, int b0, int bn, int n) {
-  for (int s = -2; s <= 2; s++) {
-    const int c0 = b0 + s * n;
-    if (a0 < c0 + bn && c0 < a0 + an) return true;
-  }
-  return false;
-}
-
-// iso[p] = 1 when sphere p's box overlaps no other sphere's box (periodic in
-// x and y), does not wrap onto itself, and no w stencil of its markers can
-// reach the k1 = kmax+1 plane (src/interp_spread.f90:100); its IBM passes
-// then touch cells no other sphere touches.
-__global__ __launch_bounds__(256) void k_ibm_isolate(IbmArgs A, int *iso) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A.np) return;
-  const Grid &g = A.g;
-  const Box bp = sphere_box(A, A.px[p], A.py[p], A.pz[p]);
-  bool ok = bp.bx < g.nx && bp.by < g.ny && (A.pz[p] + A.radfp) * g.dzi + 2.0 < (double)g.nz;
-  for (int q = 0; q < A.np && ok; q++) {
-    if (q == p) continue;
-    const Box bq = sphere_box(A, A.px[q], A.py[q], A.pz[q]);
-    if (bq.bk0 + bq.bz <= bp.bk0 || bp.bk0 + bp.bz <= bq.bk0) continue;
-    if (!overlap_periodic(bp.bi0, bp.bx, bq.bi0, bq.bx, g.nx)) continue;
-    if (!overlap_periodic(bp.bj0, bp.by, bq.bj0, bq.by, g.ny)) continue;
-    ok = false;
-  }
-  iso[p] = ok ? 1 : 0;
 }
 
 // Per-sphere force and torque sums after the last pass, src/forcing.f90:258-287,
@@ -488,10 +369,6 @@ __global__ __launch_bounds__(256) void k_ibm_isolate(IbmArgs A, int *iso) {
 __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, double *fyt, double *fzt, double *tqx, double *tqy,
                                                     double *tqz, double *tqt) {
   const Grid &g = A.g;
-  if (A.Kzero) {
-    const long nplane = (long)g.nx * g.ny;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nplane; t += (long)gridDim.x * blockDim.x) A.Kzero[t] = 0.0;
-  }
   const int p = blockIdx.x;
   const long M = (long)A.np * A.nl;
   const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
@@ -524,17 +401,20 @@ __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, doub
 
 }  // namespace
 
-// LDS of the per-sphere box (one velocity component at a time), set once at
-// marker set-up so that graph capture never meets an attribute change.
+// LDS of the per-sphere box (one velocity component at a time) and its two
+// cell masks, set once at marker set-up so that graph capture never meets an
+// attribute change.
 int ipg_ibm_setup(ipg_ctx *c) {
   const double r2 = 2.0 * c->pc.radfp;
-  const size_t cells = (size_t)((int)std::ceil(r2 * c->g.dxi) + 7) * ((int)std::ceil(r2 * c->g.dyi) + 7) *
-                       ((int)std::ceil(r2 * c->g.dzi) + 7);
-  const size_t lds = cells * sizeof(double) + (cells + 31) / 32 * sizeof(unsigned);
+  const int bmx = (int)std::ceil(r2 * c->g.dxi) + 7, bmy = (int)std::ceil(r2 * c->g.dyi) + 7;
+  const size_t cells = (size_t)bmx * bmy * ((int)std::ceil(r2 * c->g.dzi) + 7);
+  const size_t lds = cells * sizeof(double) + 2 * ((cells + 31) / 32) * sizeof(unsigned);
   if (lds > 160 * 1024) { ipg_set_error("ibm: sphere box exceeds the 160 KiB LDS of a CU"); return 1; }
+  if (bmx > c->g.nx || bmy > c->g.ny) { ipg_set_error("ibm: sphere box wider than the domain"); return 1; }
   if (c->pc.nl > IBM_THREADS * IBM_MPT) { ipg_set_error("ibm: more than 1024 markers per sphere"); return 1; }
-  IPG_HIP(hipFuncSetAttribute((const void *)k_ibm_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  IPG_HIP(hipFuncSetAttribute((const void *)k_ibm_local, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   c->ibm_lds_set = true;
   return 0;
 }
@@ -551,79 +431,70 @@ bool ibm_args(ipg_ctx *c, IbmArgs &A, size_t &lds) {
   A.bmy = (int)std::ceil(r2 * c->g.dyi) + 7;
   A.bmz = (int)std::ceil(r2 * c->g.dzi) + 7;
   const size_t cells = (size_t)A.bmx * A.bmy * A.bmz;
-  lds = cells * sizeof(double) + (cells + 31) / 32 * sizeof(unsigned);
+  lds = cells * sizeof(double) + 2 * ((cells + 31) / 32) * sizeof(unsigned);
   double **q = c->ps.q;
   A.px = q[IPG_PX]; A.py = q[IPG_PY]; A.pz = q[IPG_PZ]; A.pu = q[IPG_PU]; A.pv = q[IPG_PV]; A.pw = q[IPG_PW];
   A.pomx = q[IPG_POMX]; A.pomy = q[IPG_POMY]; A.pomz = q[IPG_POMZ]; A.pphi = q[IPG_PPHI];
   A.mk = c->mk; A.fl = c->fl;
   A.fld[0] = c->f[IPG_DUDT]; A.fld[1] = c->f[IPG_DVDT]; A.fld[2] = c->f[IPG_DWDT];
+  A.nbl = c->nbl; A.nbc = c->nbc;
   A.dtp = c->dscal;
   return true;
 }
 
-// Buffer roles of pass `pass` (fixed, so a captured graph replays them):
-// pass 0 gathers dudt and spreads into GA; pass 1 gathers dudt+GA and spreads
-// into GB; pass 2 gathers dudt+GB (clearing GA), then spreads into dudt
-// (clearing GB).  The k1 plane: K0 carries it between substeps, K1/K2 hold
-// passes 0/1; K2 is cleared by the last force sums.
-void ibm_roles(ipg_ctx *c, IbmArgs &A, int pass, int gather, int spread) {
-  double **GA = c->G[0], **GB = c->G[1], **K = c->K1;
-  double **rd = nullptr, **sp = nullptr, **zr = nullptr;
-  double *kr = nullptr, *ks = nullptr, *kz = nullptr;
-  if (pass == 0) { kr = K[0]; sp = GA; ks = K[1]; }
-  else if (pass == 1) { rd = GA; kr = K[1]; sp = GB; ks = K[2]; }
-  else {
-    if (gather) { rd = GB; kr = K[2]; zr = GA; kz = K[0]; }
-    else { ks = K[0]; zr = GB; kz = K[1]; }
-  }
-  if (!gather) { rd = nullptr; kr = nullptr; }
-  if (!spread) { sp = nullptr; ks = nullptr; }
-  for (int d = 0; d < 3; d++) {
-    A.Gread[d] = rd ? rd[d] : nullptr;
-    A.Gspread[d] = sp ? sp[d] : nullptr;
-    A.Gzero[d] = zr ? zr[d] : nullptr;
-  }
-  A.Kread = kr; A.Kspread = ks; A.Kzero = kz;
-}
-
 }  // namespace
 
-// src/main.f90:301-303 (dudtf = dudt): the marker stencils of this substep.
-// The reference copies dudtf = dudt before the passes (src/main.f90:301-303);
-// here the passes read dudt and the force buffers directly, so there is
-// nothing to launch.  Kept as the IBM block's entry for the split interface.
+// Neighbour lists of this substep (the markers do not move inside the IBM
+// block).  This is where the reference copies dudtf = dudt
+// (src/main.f90:301-303), which is not needed here.
 int ipg_launch_ibm_begin(ipg_ctx *c) {
   if (c->pc.np <= 0) return 0;
   IbmArgs A; size_t lds;
   if (!ibm_args(c, A, lds)) return 1;
-  return 0;
-}
-
-// One IBM pass or half pass: gather (+ marker force update) and/or spread.
-int ipg_launch_ibm_pass(ipg_ctx *c, int pass, int gather, int spread) {
-  if (c->pc.np <= 0) return 0;
-  IbmArgs A; size_t lds;
-  if (!ibm_args(c, A, lds)) return 1;
-  if (pass == 2 && gather && spread) { ipg_set_error("ibm: the last pass gathers and spreads in two launches"); return 1; }
-  ibm_roles(c, A, pass, gather, spread);
-  hipLaunchKernelGGL(k_ibm_pass, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, pass, gather, spread,
-                     (const int *)nullptr);
-  static const char *names[3][2] = {{"ibm_pass0_spread", "ibm_pass0"}, {"ibm_pass1_spread", "ibm_pass1"},
-                                    {"ibm_pass2_spread", "ibm_pass2_gather"}};
-  ipg_mark(c, gather ? (spread ? names[pass][1] : (pass == 2 ? names[2][1] : "ibm_gather")) : names[pass][0]);
+  hipLaunchKernelGGL(k_ibm_neighbours, dim3((A.np + 3) / 4), dim3(256), 0, c->stream, A, c->nbl, c->nbc, c->ibm_err);
+  ipg_mark(c, "ibm_neighbours");
   IPG_HIP(hipGetLastError());
   return 0;
 }
 
-// Per-sphere force and torque sums (src/forcing.f90:66-89, 258-287); after the
-// last pass this also clears the pass-1 k1 plane.
-int ipg_launch_ibm_totals(ipg_ctx *c, int last) {
+// Pass `pass` (0..2): eulr2lagr and the marker force update.  The k1 planes
+// (fixed roles, so a captured graph replays them): K0 carries the last
+// spread of the previous substep into pass 0, pass 0 spreads into K1, pass 1
+// into K2, pass 2 into K0; each plane is cleared once nothing reads it.
+int ipg_launch_ibm_gather(ipg_ctx *c, int pass) {
   if (c->pc.np <= 0) return 0;
   IbmArgs A; size_t lds;
   if (!ibm_args(c, A, lds)) return 1;
-  for (int d = 0; d < 3; d++) { A.Gread[d] = A.Gspread[d] = A.Gzero[d] = nullptr; }
-  A.Kread = A.Kspread = nullptr;
-  A.Kzero = last ? c->K1[2] : nullptr;
+  double **K = c->K1;
+  A.Kread = K[pass];
+  A.Kspread = K[(pass + 1) % 3];
+  A.Kzero = pass == 0 ? nullptr : K[pass - 1];
+  static const char *names[3] = {"ibm_pass0", "ibm_pass1", "ibm_pass2"};
+  if (pass == 0) hipLaunchKernelGGL(k_ibm<0>, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, pass);
+  else hipLaunchKernelGGL(k_ibm<1>, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, pass);
+  ipg_mark(c, names[pass]);
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+// lagr2eulr of the last pass: dudt += S, once per cell.  The spreads of
+// passes 0 and 1 need no launch of their own: the next gather rebuilds them.
+int ipg_launch_ibm_final(ipg_ctx *c) {
+  if (c->pc.np <= 0) return 0;
+  IbmArgs A; size_t lds;
+  if (!ibm_args(c, A, lds)) return 1;
+  A.Kzero = c->K1[2];
+  hipLaunchKernelGGL(k_ibm<2>, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, 2);
+  ipg_mark(c, "ibm_spread");
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+// Per-sphere force and torque sums (src/forcing.f90:66-89, 258-287).
+int ipg_launch_ibm_totals(ipg_ctx *c) {
+  if (c->pc.np <= 0) return 0;
+  IbmArgs A; size_t lds;
+  if (!ibm_args(c, A, lds)) return 1;
   double **q = c->ps.q;
   hipLaunchKernelGGL(k_ibm_totals, dim3(A.np), dim3(256), 0, c->stream, A, q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
                      q[IPG_PTORQX], q[IPG_PTORQY], q[IPG_PTORQZ], q[IPG_PTORQTHETA]);
@@ -632,27 +503,22 @@ int ipg_launch_ibm_totals(ipg_ctx *c, int last) {
   return 0;
 }
 
-// The whole IBM block of one substep, src/main.f90:296-338.  Isolated
-// spheres run their three passes in the first launch; the passes through the
-// global force buffers then serve only the spheres near another sphere or the
-// top wall.
+// A neighbour list that overflowed (more than 64 neighbour images: a cluster
+// far denser than random packing) would have dropped contributions: fail.
+int ipg_ibm_check(ipg_ctx *c) {
+  if (c->pc.np <= 0 || !c->ibm_err) return 0;
+  int e = 0;
+  IPG_HIP(hipMemcpy(&e, c->ibm_err, sizeof(int), hipMemcpyDeviceToHost));
+  if (e) { ipg_set_error("ibm: more than 64 neighbour images around a sphere"); return 3; }
+  return 0;
+}
+
+// The whole IBM block of one substep, src/main.f90:296-338.
 int ipg_launch_ibm(ipg_ctx *c) {
   if (c->pc.np <= 0) return 0;
-  IbmArgs A; size_t lds;
-  if (!ibm_args(c, A, lds)) return 1;
-  hipLaunchKernelGGL(k_ibm_isolate, dim3((A.np + 255) / 256), dim3(256), 0, c->stream, A, c->iso);
-  ipg_mark(c, "ibm_isolate");
-  hipLaunchKernelGGL(k_ibm_local, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, (const int *)c->iso);
-  ipg_mark(c, "ibm_local");
-  const int roles[4][3] = {{0, 1, 1}, {1, 1, 1}, {2, 1, 0}, {2, 0, 1}};
-  static const char *names[4] = {"ibm_pass0", "ibm_pass1", "ibm_pass2_gather", "ibm_pass2_spread"};
-  for (int r = 0; r < 4; r++) {
-    IbmArgs B = A;
-    ibm_roles(c, B, roles[r][0], roles[r][1], roles[r][2]);
-    hipLaunchKernelGGL(k_ibm_pass, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, B, roles[r][0], roles[r][1],
-                       roles[r][2], (const int *)c->iso);
-    ipg_mark(c, names[r]);
-  }
-  IPG_HIP(hipGetLastError());
-  return ipg_launch_ibm_totals(c, 1);
+  if (ipg_launch_ibm_begin(c)) return 1;
+  for (int pass = 0; pass < 3; pass++)
+    if (ipg_launch_ibm_gather(c, pass)) return 1;
+  if (ipg_launch_ibm_final(c)) return 1;
+  return ipg_launch_ibm_totals(c);
 }

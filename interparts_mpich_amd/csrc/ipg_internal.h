@@ -60,16 +60,15 @@ struct ipg_ctx {
   size_t nfield;                  // doubles per field allocation
   double *slab;                   // one allocation for all fields
   double *f[IPG_NFIELDS];         // field base pointers (see Grid)
-  double *tmpf[4];                // rk2/rk3 temporaries are not needed: fused
   // IBM
-  double *G[2][3];                // spread force buffers, interior (nx*ny*nz), modular x,y
   double *K1[3];                  // dwdtf(.,.,k1) planes (nx*ny), src/interp_spread.f90:100 (no upper clamp)
   bool ibm_lds_set;
   // launch shape: z planes per RK-stage workgroup (IPG_TUNE_RK_KB at init; 0 = chosen per grid)
   int tune_rk_kb;
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
   double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl
-  int *iso;                       // [np] sphere isolated for the IBM (see k_ibm_isolate)
+  int *nbl, *nbc;                 // IBM neighbour images: [np][64] codes, [np] counts
+  int *ibm_err;                   // set on the device when a neighbour list overflows
   // particles
   PartState ps;
   double *op;                     // [24][np] state at substep start (src/intgr_nwtn_eulr.f90:77-118)
@@ -142,8 +141,10 @@ int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter);
 int ipg_launch_ibm(ipg_ctx *c);
 int ipg_ibm_setup(ipg_ctx *c);
 int ipg_launch_ibm_begin(ipg_ctx *c);
-int ipg_launch_ibm_pass(ipg_ctx *c, int pass, int gather, int spread);
-int ipg_launch_ibm_totals(ipg_ctx *c, int last);
+int ipg_launch_ibm_gather(ipg_ctx *c, int pass);
+int ipg_launch_ibm_final(ipg_ctx *c);
+int ipg_launch_ibm_totals(ipg_ctx *c);
+int ipg_ibm_check(ipg_ctx *c);
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type);
 int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter);
 int ipg_launch_nbflags(ipg_ctx *c);
