@@ -78,7 +78,6 @@ int release(ipg_ctx *c) {
   ipg_poisson_free(c);
   void *ptrs[] = {c->slab, c->mk, c->fl, c->nbl, c->nbc, c->ibm_err, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
   for (void *q : ptrs) if (q) (void)hipFree(q);
-  for (int b = 0; b < 2; b++)
   for (int q = 0; q < IPG_NPART; q++) if (c->ps.q[q]) (void)hipFree(c->ps.q[q]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   return 0;

@@ -355,58 +355,78 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv(Grid g, double *__res
 }
 
 // Batched Thomas solve along z, src/solver.f90:90-135, one thread per
-// wavenumber pair (m, my); real and imaginary parts share the recurrence.
-// Loads of the next plane are issued ahead of the recurrence.
-__global__ __launch_bounds__(256) void k_thomas(Grid g, double2 *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
+// (wavenumber pair, real or imaginary part): the two parts share the
+// recurrence of the pivots and multipliers (both compute and store the same
+// values), which doubles the number of independent lanes over one thread per
+// pair.  Adjacent lanes hold the two parts of one complex value, so
+// every load and store is coalesced.  Planes are read TD at a time ahead of
+// the recurrence, so TD loads per lane are in flight together.
+constexpr int TD = 8;
+__global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
                                                 const double *__restrict__ a, const double *__restrict__ b,
                                                 const double *__restrict__ cc, const double *__restrict__ xrt,
                                                 const double *__restrict__ yrt) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const int my = blockIdx.y;
-  if (m >= nmx) return;
+  const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long nmode = (long)nmx * g.ny;
+  if (u >= 2 * nmode) return;
+  const long t = u >> 1;
+  const int part = (int)(u & 1);
+  const int my = (int)(t / nmx), m = (int)(t - (long)my * nmx);
   const int nz = g.nz;
-  const long plane = (long)g.ny * sp;
-  const long base = (long)my * sp + m;
+  const long plane = (long)g.ny * sp;                 // complex elements per z plane
+  const long base = (long)my * sp + m;                // complex index in plane 0
+  double *S = spec + part;                            // element (k, base): S[2 * (base + k plane)]
   const double xy = xrt[m] + yrt[my];
-  double z = 1.0 / (b[1] + xy);
-  double dprev = cc[1] * z;
-  dtri[base] = dprev;
-  double2 pp = spec[base];
-  pp.x = pp.x * z; pp.y = pp.y * z;
-  spec[base] = pp;
-  double2 nxt = spec[base + plane];
-  for (int k = 2; k <= nz - 1; k++) {
-    const long q = base + (long)(k - 1) * plane;
-    double2 v = nxt;
-    nxt = spec[q + plane];
-    const double bb = b[k] + xy;
-    z = 1.0 / (bb - a[k] * dprev);
-    dprev = cc[k] * z;
-    dtri[q] = dprev;
-    v.x = (v.x - a[k] * pp.x) * z;
-    v.y = (v.y - a[k] * pp.y) * z;
-    spec[q] = v; pp = v;
+  // forward elimination, k = 1 .. nz-1 (z = 1/(b + xy - a dprev))
+  double dprev = 0.0, pp = 0.0;
+  for (int k0 = 1; k0 <= nz - 1; k0 += TD) {
+    double v[TD];
+#pragma unroll
+    for (int r = 0; r < TD; r++) {
+      const int k = k0 + r;
+      v[r] = k <= nz - 1 ? S[2 * (base + (long)(k - 1) * plane)] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < TD; r++) {
+      const int k = k0 + r;
+      if (k > nz - 1) break;
+      const long q = base + (long)(k - 1) * plane;
+      double z;
+      if (k == 1) z = 1.0 / (b[1] + xy);
+      else z = 1.0 / ((b[k] + xy) - a[k] * dprev);
+      dprev = cc[k] * z;
+      dtri[q] = dprev;  // both parts store the same value; each reads back its own store
+      const double w = k == 1 ? v[r] * z : (v[r] - a[k] * pp) * z;
+      S[2 * q] = w;
+      pp = w;
+    }
   }
-  {
+  {  // last row (singular kx = ky = 0 guard)
     const long q = base + (long)(nz - 1) * plane;
-    const double bb = b[nz] + xy;
-    z = bb - a[nz] * dprev;
-    double2 v = nxt;
-    if (z != 0.0) { v.x = (v.x - a[nz] * pp.x) / z; v.y = (v.y - a[nz] * pp.y) / z; }
-    else { v.x = 0.0; v.y = 0.0; }
-    spec[q] = v; pp = v;
+    const double z = (b[nz] + xy) - a[nz] * dprev;
+    const double v = S[2 * q];
+    pp = z != 0.0 ? (v - a[nz] * pp) / z : 0.0;
+    S[2 * q] = pp;
   }
-  long q = base + (long)(nz - 2) * plane;
-  double2 vn = spec[q];
-  double dn = dtri[q];
-  for (int k = nz - 1; k >= 1; k--) {
-    double2 v = vn;
-    const double d = dn;
-    if (k > 1) { vn = spec[q - plane]; dn = dtri[q - plane]; }
-    v.x = v.x - d * pp.x;
-    v.y = v.y - d * pp.y;
-    spec[q] = v; pp = v;
-    q -= plane;
+  // back substitution, k = nz-1 .. 1
+  for (int k0 = nz - 1; k0 >= 1; k0 -= TD) {
+    double v[TD], d[TD];
+#pragma unroll
+    for (int r = 0; r < TD; r++) {
+      const int k = k0 - r;
+      const long q = base + (long)(k - 1) * plane;
+      v[r] = k >= 1 ? S[2 * q] : 0.0;
+      d[r] = k >= 1 ? dtri[q] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < TD; r++) {
+      const int k = k0 - r;
+      if (k < 1) break;
+      const long q = base + (long)(k - 1) * plane;
+      const double w = v[r] - d[r] * pp;
+      S[2 * q] = w;
+      pp = w;
+    }
   }
 }
 
@@ -573,7 +593,8 @@ int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter) {
     hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, -1);
   }
   ipg_mark(c, "fft_y_fwd");
-  hipLaunchKernelGGL(k_thomas, dim3((P.nmx + 255) / 256, g.ny), dim3(256), 0, c->stream, g, P.spec, P.dtri, P.sp, P.nmx,
+  const long nthomas = 2L * P.nmx * g.ny;
+  hipLaunchKernelGGL(k_thomas, dim3((unsigned)((nthomas + 255) / 256)), dim3(256), 0, c->stream, g, (double *)P.spec, P.dtri, P.sp, P.nmx,
                      P.a, P.b, P.c, P.xrt, P.yrt);
   ipg_mark(c, "thomas");
   if (specialised(g.ny)) {
