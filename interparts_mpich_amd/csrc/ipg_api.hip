@@ -76,7 +76,7 @@ int release(ipg_ctx *c) {
   for (int q = 0; q < 64; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
   ipg_poisson_free(c);
-  void *ptrs[] = {c->slab, c->mk, c->fl, c->nbl, c->nbc, c->ibm_err, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
+  void *ptrs[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->ibm_err, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   for (int q = 0; q < IPG_NPART; q++) if (c->ps.q[q]) (void)hipFree(c->ps.q[q]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -139,6 +139,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
     const size_t M = (size_t)np * prm->nl;
     if (hipMalloc(&c->mk, 3 * (size_t)prm->nl * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->fl, 3 * M * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->fl2, 3 * M * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->nbl, (size_t)np * 64 * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->nbc, (size_t)np * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->ibm_err, sizeof(int)) != hipSuccess ||
@@ -147,6 +148,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
       ipg_set_error("out of device memory (particles)"); return fail(1);
     }
     (void)hipMemset(c->fl, 0, 3 * M * sizeof(double));
+    (void)hipMemset(c->fl2, 0, 3 * M * sizeof(double));
     (void)hipMemset(c->nbc, 0, (size_t)np * sizeof(int));
     (void)hipMemset(c->ibm_err, 0, sizeof(int));
     (void)hipMemset(c->ps.nbf, 0, (size_t)np * IPG_NB * sizeof(int));
@@ -309,7 +311,7 @@ int ipg_eulr2lagr(ipg_ctx *c, int ibmiter) {
 int ipg_lagrforces(ipg_ctx *c, int ibmiter) {
   if (!check_ctx(c)) return 1;
   if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("lagrforces: ibmiter must be 0..2"); return 1; }
-  if (ipg_launch_ibm_totals(c)) return 1;
+  if (ipg_launch_ibm_totals(c, ibmiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -56,7 +56,8 @@ struct IbmArgs {
   int bmx, bmy, bmz;  // largest box extent per direction (LDS sizing)
   const double *px, *py, *pz, *pu, *pv, *pw, *pomx, *pomy, *pomz, *pphi;
   const double *mk;   // [3][nl] marker offsets from the sphere centre
-  double *fl;         // [3][np*nl] marker forces fxl, fyl, fzl
+  double *fl;         // [3][np*nl] marker forces fxl, fyl, fzl written by this launch
+  const double *flp;  // [3][np*nl] marker forces of the previous pass (read; another buffer)
   double *fld[3];     // dudt dvdt dwdt
   const int *nbl;     // [np][IBM_MAXNB] neighbour images: q*25 + (sx+2)*5 + (sy+2)
   const int *nbc;     // [np] neighbour image count
@@ -219,6 +220,9 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   unsigned *taken = flag + nflag;  // MODE 2: cells a lower-numbered sphere owns
   double *F = A.fld[comp];
   double *fl = A.fl + comp * M;
+  // the previous pass's forces: a buffer no workgroup of this launch writes,
+  // as neighbours read them while their owners compute the new ones
+  const double *flp = A.flp + comp * M;
   const long m0 = (long)p * A.nl;
   // own markers: stencil weights, LDS index of the stencil's first cell, and
   // how many of its z planes lie in the box (the rest is the k1 plane)
@@ -261,7 +265,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   for (int q = 0; q < IBM_MPT; q++) {
     if (e0[q] < 0) continue;
     const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
-    const double fc = MODE >= 1 ? fl[m] * c0 : 0.0;
+    const double fc = MODE >= 1 ? flp[m] * c0 : 0.0;
     for (int k = 0; k < nzin[q]; k++)
       for (int j = 0; j < 3; j++)
 #pragma unroll
@@ -286,7 +290,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
       comp_stencil(g, xl, yl, zl, comp, c);
       const int i0 = c.i0 + sxs * g.nx, j0 = c.j0 + sys * g.ny;
       if (i0 + 2 < b.bi0 || i0 >= b.bi0 + b.bx || j0 + 2 < b.bj0 || j0 >= b.bj0 + b.by) continue;
-      const double fc = fl[(long)qs * A.nl + l] * c0;
+      const double fc = flp[(long)qs * A.nl + l] * c0;
       for (int k = 0; k < 3; k++)
         for (int j = 0; j < 3; j++)
 #pragma unroll
@@ -349,7 +353,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
           else f = A.Kread[(long)(wrapany(kx0[q] + i, g.nx) - 1) + (long)g.nx * (wrapany(ky0[q] + j, g.ny) - 1)];
           acc = acc + f * wx[q][i] * wy[q][j] * wz[q][k];
         }
-    const double fnew = MODE == 0 ? (ub[q] - acc) * dti : fl[m] + (ub[q] - acc) * dti;
+    const double fnew = MODE == 0 ? (ub[q] - acc) * dti : flp[m] + (ub[q] - acc) * dti;
     fl[m] = fnew;
     // the part of this pass's spread that falls on the k1 plane
     if (nzin[q] < 3) {
@@ -466,6 +470,9 @@ int ipg_launch_ibm_gather(ipg_ctx *c, int pass) {
   IbmArgs A; size_t lds;
   if (!ibm_args(c, A, lds)) return 1;
   double **K = c->K1;
+  // forces: pass 0 -> fl, pass 1 fl -> fl2, pass 2 fl2 -> fl
+  A.fl = pass == 1 ? c->fl2 : c->fl;
+  A.flp = pass == 2 ? c->fl2 : c->fl;
   A.Kread = K[pass];
   A.Kspread = K[(pass + 1) % 3];
   A.Kzero = pass == 0 ? nullptr : K[pass - 1];
@@ -484,17 +491,20 @@ int ipg_launch_ibm_final(ipg_ctx *c) {
   IbmArgs A; size_t lds;
   if (!ibm_args(c, A, lds)) return 1;
   A.Kzero = c->K1[2];
+  A.flp = c->fl;
   hipLaunchKernelGGL(k_ibm<2>, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, 2);
   ipg_mark(c, "ibm_spread");
   IPG_HIP(hipGetLastError());
   return 0;
 }
 
-// Per-sphere force and torque sums (src/forcing.f90:66-89, 258-287).
-int ipg_launch_ibm_totals(ipg_ctx *c) {
+// Per-sphere force and torque sums (src/forcing.f90:66-89, 258-287) of the
+// forces after pass `pass`.
+int ipg_launch_ibm_totals(ipg_ctx *c, int pass) {
   if (c->pc.np <= 0) return 0;
   IbmArgs A; size_t lds;
   if (!ibm_args(c, A, lds)) return 1;
+  if (pass == 1) A.fl = c->fl2;
   double **q = c->ps.q;
   hipLaunchKernelGGL(k_ibm_totals, dim3(A.np), dim3(256), 0, c->stream, A, q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
                      q[IPG_PTORQX], q[IPG_PTORQY], q[IPG_PTORQZ], q[IPG_PTORQTHETA]);
@@ -520,5 +530,5 @@ int ipg_launch_ibm(ipg_ctx *c) {
   for (int pass = 0; pass < 3; pass++)
     if (ipg_launch_ibm_gather(c, pass)) return 1;
   if (ipg_launch_ibm_final(c)) return 1;
-  return ipg_launch_ibm_totals(c);
+  return ipg_launch_ibm_totals(c, 2);
 }

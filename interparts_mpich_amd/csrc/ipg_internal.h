@@ -66,7 +66,8 @@ struct ipg_ctx {
   // launch shape: z planes per RK-stage workgroup (IPG_TUNE_RK_KB at init; 0 = chosen per grid)
   int tune_rk_kb;
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
-  double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl
+  double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl (after passes 0 and 2)
+  double *fl2;                    // [3][np*nl] marker forces after pass 1
   int *nbl, *nbc;                 // IBM neighbour images: [np][64] codes, [np] counts
   int *ibm_err;                   // set on the device when a neighbour list overflows
   // particles
@@ -143,7 +144,7 @@ int ipg_ibm_setup(ipg_ctx *c);
 int ipg_launch_ibm_begin(ipg_ctx *c);
 int ipg_launch_ibm_gather(ipg_ctx *c, int pass);
 int ipg_launch_ibm_final(ipg_ctx *c);
-int ipg_launch_ibm_totals(ipg_ctx *c);
+int ipg_launch_ibm_totals(ipg_ctx *c, int pass);
 int ipg_ibm_check(ipg_ctx *c);
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type);
 int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter);
