@@ -278,12 +278,21 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   if (MODE >= 1) {
     // the neighbours' markers: their contributions to this box
     const int nn = A.nbc[p];
-    const long tot = (long)nn * A.nl;
-    for (long idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-      const int n = (int)(idx / A.nl), l = (int)(idx - (long)n * A.nl);
+    const int tot = nn * A.nl;
+    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+      const int n = idx / A.nl, l = idx - n * A.nl;
       const int code = A.nbl[(long)p * IBM_MAXNB + n];
       const int qs = code / 25, sxs = (code / 5) % 5 - 2, sys = code % 5 - 2;
       const double xq = A.px[qs], yq = A.py[qs], zq = A.pz[qs];
+      // markers on the far side of the neighbour cannot reach this box: a
+      // stencil spans less than two cells either side of its marker
+      {
+        const double xs = (xq + A.mk[l]) * g.dxi + sxs * g.nx, ys = (yq + A.mk[A.nl + l]) * g.dyi + sys * g.ny;
+        const double zs = (zq + A.mk[2 * A.nl + l]) * g.dzi;
+        if (xs < b.bi0 - 3.0 || xs > b.bi0 + b.bx + 2.0 || ys < b.bj0 - 3.0 || ys > b.bj0 + b.by + 2.0 ||
+            zs < b.bk0 - 3.0 || zs > b.bk0 + b.bz + 2.0)
+          continue;
+      }
       double xl, yl, zl;
       if (!marker_pos(g, xq, yq, zq, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
       CompStencil c;
