@@ -86,8 +86,9 @@ int release(ipg_ctx *c) {
 int substep_launches(ipg_ctx *c, int rkiter) {
   if (ipg_launch_rk(c, rkiter)) return 1;
   if (c->pc.np > 0 && ipg_launch_ibm(c)) return 1;
-  if (ipg_launch_bound(c, BND_UVW_DT)) return 1;
-  // fillps (src/fillps.f90) computed inside the forward x transform
+  // bounduvw(dudt, dvdt, dwdt) (src/main.f90:340) feeds only fillps, and
+  // fillps (src/fillps.f90), computed inside the forward x transform, takes
+  // the boundary values at the source: no launch of its own
   if (ipg_launch_poisson_ex(c, rkiter)) return 1;
   if (ipg_launch_bound(c, BND_P)) return 1;
   if (ipg_launch_correc_fused(c, rkiter)) return 1;

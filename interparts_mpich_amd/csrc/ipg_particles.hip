@@ -71,12 +71,26 @@ __global__ __launch_bounds__(256) void k_sphere_int(Grid g, PartConst pc, const 
     double s[6] = {0, 0, 0, 0, 0, 0};
     const int ni = ihigh - ilow + 1, nj = jhigh - jlow + 1, nk = khigh - klow + 1;
     if (ni > 0 && nj > 0 && nk > 0) {
-      const int nij = ni * nj, tot = nij * nk;
-      for (int comp = 0; comp < 3; comp++) {
+      // rows (component, j, k) over groups of 32 lanes, the lanes of a group
+      // along i: loads are coalesced, and a row that misses the sphere is
+      // skipped whole (its nearest point in y, z is already beyond R)
+      const int lane = (int)(threadIdx.x & 31), grp = (int)(threadIdx.x >> 5), ngrp = (int)(blockDim.x >> 5);
+      const int njk = nj * nk, nrow = 3 * njk;
+      for (int r0 = grp; r0 < nrow; r0 += ngrp) {
+        const int comp = r0 / njk, rr = r0 - comp * njk, kk = rr / nj;
+        const int j = jlow + (rr - kk * nj), k = klow + kk;
         const double *F = comp == 0 ? U : (comp == 1 ? V : W);
-        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-          const int kk = t / nij, r = t - kk * nij, jj = r / ni;
-          const int i = ilow + (r - jj * ni), j = jlow + jj, k = klow + kk;
+        {
+          double zmin, zplus, ymin, yplus;
+          if (comp == 2) { zmin = ((double)k - 0.5) * dz; zplus = ((double)k + 0.5) * dz; }
+          else { zmin = (double)(k - 1) * dz; zplus = (double)k * dz; }
+          if (comp == 1) { ymin = 0.0 + ((double)j - 0.5) * dy; yplus = 0.0 + ((double)j + 0.5) * dy; }
+          else { ymin = 0.0 + (double)(j - 1) * dy; yplus = 0.0 + (double)j * dy; }
+          const double ey = ymin > Y ? ymin - Y : (yplus < Y ? Y - yplus : 0.0);
+          const double ez = zmin > Z ? zmin - Z : (zplus < Z ? Z - zplus : 0.0);
+          if (ey * ey + ez * ez > R * R * (1.0 + 1e-9)) continue;
+        }
+        for (int i = ilow + lane; i <= ihigh; i += 32) {
           double zmin, zplus, ymin, yplus, xmin, xplus;
           if (comp == 2) { zmin = ((double)k - 0.5) * dz; zplus = ((double)k + 0.5) * dz; }
           else { zmin = (double)(k - 1) * dz; zplus = (double)k * dz; }

@@ -148,10 +148,18 @@ constexpr int batch_of() {
   return b;
 }
 
-// Right-hand side of the Poisson equation at (i,j,k), src/fillps.f90:43-46.
+// Right-hand side of the Poisson equation at (i,j,k), src/fillps.f90:43-46,
+// from dudt, dvdt, dwdt whose ghosts have not been filled: the values
+// bounduvw (src/bound.f90:11-80) would put there are taken at the source --
+// the periodic image in x and y, and dwdt = 0 on both walls (k = 0 and the
+// wall face k = kmax), which bounduvw sets for either velocity condition.
 __device__ __forceinline__ double rhs_at(const Grid &g, const double *__restrict__ du, const double *__restrict__ dv,
-                                         const double *__restrict__ dw, long c, double dtidxi, double dtidyi, double dtidzi) {
-  return (dw[c] - dw[c - g.sxy]) * dtidzi + (dv[c] - dv[c - g.sx]) * dtidyi + (du[c] - du[c - 1]) * dtidxi;
+                                         const double *__restrict__ dw, long c, int i, int j, int k, double dtidxi,
+                                         double dtidyi, double dtidzi) {
+  const double wk = k == g.nz ? 0.0 : dw[c], wkm = k == 1 ? 0.0 : dw[c - g.sxy];
+  const double vjm = j == 1 ? dv[c + (long)(g.ny - 1) * g.sx] : dv[c - g.sx];
+  const double uim = i == 1 ? du[c + (g.nx - 1)] : du[c - 1];
+  return (wk - wkm) * dtidzi + (dv[c] - vjm) * dtidyi + (du[c] - uim) * dtidxi;
 }
 
 // Forward x transform: rows (j, j+1) of plane k form one complex sequence;
@@ -180,7 +188,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, const doubl
     if (pr < pairs) {
       const int k = (int)(pr / (g.ny / 2)) + 1, j = 2 * (int)(pr % (g.ny / 2)) + 1;
       const long c = g.at(i + 1, j, k);
-      if (fused) v = make_double2(rhs_at(g, du, dv, dw, c, dtidxi, dtidyi, dtidzi), rhs_at(g, du, dv, dw, c + g.sx, dtidxi, dtidyi, dtidzi));
+      if (fused) v = make_double2(rhs_at(g, du, dv, dw, c, i + 1, j, k, dtidxi, dtidyi, dtidzi),
+                                  rhs_at(g, du, dv, dw, c + g.sx, i + 1, j + 1, k, dtidxi, dtidyi, dtidzi));
       else v = make_double2(p[c], p[c + g.sx]);
     }
     buf[b * N + i] = v;
@@ -284,7 +293,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd(Grid g, const double 
     const long pr = pair0 + b;
     const int k = (int)(pr / (g.ny / 2)) + 1, j = 2 * (int)(pr % (g.ny / 2)) + 1;
     const long c = g.at(i + 1, j, k);
-    if (fused) buf[b * n + i] = make_double2(rhs_at(g, du, dv, dw, c, dtidxi, dtidyi, dtidzi), rhs_at(g, du, dv, dw, c + g.sx, dtidxi, dtidyi, dtidzi));
+    if (fused) buf[b * n + i] = make_double2(rhs_at(g, du, dv, dw, c, i + 1, j, k, dtidxi, dtidyi, dtidzi),
+                                             rhs_at(g, du, dv, dw, c + g.sx, i + 1, j + 1, k, dtidxi, dtidyi, dtidzi));
     else buf[b * n + i] = make_double2(p[c], p[c + g.sx]);
   }
   __syncthreads();
