@@ -43,12 +43,12 @@ __device__ __forceinline__ void image_shift(const Grid &g, int nbrecv, double &x
 }
 
 // types: 1 volume, 2 int u dV, 3 int r x u dV, 23 both 2 and 3 from one sweep
-__global__ __launch_bounds__(256) void k_sphere_int(Grid g, PartConst pc, const double *__restrict__ U,
+__global__ __launch_bounds__(512) void k_sphere_int(Grid g, PartConst pc, const double *__restrict__ U,
                                                     const double *__restrict__ V, const double *__restrict__ W, PQ P,
                                                     const int *__restrict__ nbf, int type) {
   const int p = blockIdx.x;
   __shared__ double acc[IPG_NB][6];
-  __shared__ double red[6][4];
+  __shared__ double red[6][8];
   if (threadIdx.x < IPG_NB * 6) acc[threadIdx.x / 6][threadIdx.x % 6] = 0.0;
   __syncthreads();
   const double R = pc.radius, dVeul = pc.dVeul;
@@ -71,26 +71,12 @@ __global__ __launch_bounds__(256) void k_sphere_int(Grid g, PartConst pc, const 
     double s[6] = {0, 0, 0, 0, 0, 0};
     const int ni = ihigh - ilow + 1, nj = jhigh - jlow + 1, nk = khigh - klow + 1;
     if (ni > 0 && nj > 0 && nk > 0) {
-      // rows (component, j, k) over groups of 32 lanes, the lanes of a group
-      // along i: loads are coalesced, and a row that misses the sphere is
-      // skipped whole (its nearest point in y, z is already beyond R)
-      const int lane = (int)(threadIdx.x & 31), grp = (int)(threadIdx.x >> 5), ngrp = (int)(blockDim.x >> 5);
-      const int njk = nj * nk, nrow = 3 * njk;
-      for (int r0 = grp; r0 < nrow; r0 += ngrp) {
-        const int comp = r0 / njk, rr = r0 - comp * njk, kk = rr / nj;
-        const int j = jlow + (rr - kk * nj), k = klow + kk;
+      const int nij = ni * nj, tot = nij * nk;
+      for (int comp = 0; comp < 3; comp++) {
         const double *F = comp == 0 ? U : (comp == 1 ? V : W);
-        {
-          double zmin, zplus, ymin, yplus;
-          if (comp == 2) { zmin = ((double)k - 0.5) * dz; zplus = ((double)k + 0.5) * dz; }
-          else { zmin = (double)(k - 1) * dz; zplus = (double)k * dz; }
-          if (comp == 1) { ymin = 0.0 + ((double)j - 0.5) * dy; yplus = 0.0 + ((double)j + 0.5) * dy; }
-          else { ymin = 0.0 + (double)(j - 1) * dy; yplus = 0.0 + (double)j * dy; }
-          const double ey = ymin > Y ? ymin - Y : (yplus < Y ? Y - yplus : 0.0);
-          const double ez = zmin > Z ? zmin - Z : (zplus < Z ? Z - zplus : 0.0);
-          if (ey * ey + ez * ez > R * R * (1.0 + 1e-9)) continue;
-        }
-        for (int i = ilow + lane; i <= ihigh; i += 32) {
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+          const int kk = t / nij, r = t - kk * nij, jj = r / ni;
+          const int i = ilow + (r - jj * ni), j = jlow + jj, k = klow + kk;
           double zmin, zplus, ymin, yplus, xmin, xplus;
           if (comp == 2) { zmin = ((double)k - 0.5) * dz; zplus = ((double)k + 0.5) * dz; }
           else { zmin = (double)(k - 1) * dz; zplus = (double)k * dz; }
@@ -380,7 +366,7 @@ PQ make_pq(ipg_ctx *c) {
 
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type) {
   if (c->pc.np <= 0) return 0;
-  hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(256), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W],
+  hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(512), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W],
                      make_pq(c), c->ps.nbf, type);
   IPG_HIP(hipGetLastError());
   return 0;
@@ -402,7 +388,7 @@ int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter) {
   int *err = (int *)(c->red + 3L * np);
   hipLaunchKernelGGL(k_ne_save, dim3(nb), dim3(256), 0, c->stream, P, c->op, np);
   ipg_mark(c, "ne_save");
-  hipLaunchKernelGGL(k_sphere_int, dim3(np), dim3(256), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], P,
+  hipLaunchKernelGGL(k_sphere_int, dim3(np), dim3(512), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], P,
                      c->ps.nbf, 23);
   ipg_mark(c, "sphere_int");
   for (int iter = 0; iter <= 1; iter++) {

@@ -156,10 +156,11 @@ constexpr int batch_of() {
 __device__ __forceinline__ double rhs_at(const Grid &g, const double *__restrict__ du, const double *__restrict__ dv,
                                          const double *__restrict__ dw, long c, int i, int j, int k, double dtidxi,
                                          double dtidyi, double dtidzi) {
-  const double wk = k == g.nz ? 0.0 : dw[c], wkm = k == 1 ? 0.0 : dw[c - g.sxy];
-  const double vjm = j == 1 ? dv[c + (long)(g.ny - 1) * g.sx] : dv[c - g.sx];
-  const double uim = i == 1 ? du[c + (g.nx - 1)] : du[c - 1];
-  return (wk - wkm) * dtidzi + (dv[c] - vjm) * dtidyi + (du[c] - uim) * dtidxi;
+  const long cjm = j == 1 ? c + (long)(g.ny - 1) * g.sx : c - g.sx;
+  const long cim = i == 1 ? c + (g.nx - 1) : c - 1;
+  const double w0 = dw[c], w1 = dw[c - g.sxy];
+  const double wk = k == g.nz ? 0.0 : w0, wkm = k == 1 ? 0.0 : w1;
+  return (wk - wkm) * dtidzi + (dv[c] - dv[cjm]) * dtidyi + (du[c] - du[cim]) * dtidxi;
 }
 
 // Forward x transform: rows (j, j+1) of plane k form one complex sequence;
