@@ -76,7 +76,7 @@ int release(ipg_ctx *c) {
   for (int q = 0; q < 64; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
   ipg_poisson_free(c);
-  void *ptrs[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->ibm_err, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
+  void *ptrs[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->nml, c->nmc, c->ibm_err, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   for (int q = 0; q < IPG_NPART; q++) if (c->ps.q[q]) (void)hipFree(c->ps.q[q]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -143,6 +143,8 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
         hipMalloc(&c->fl2, 3 * M * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->nbl, (size_t)np * 64 * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->nbc, (size_t)np * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->nml, (size_t)np * 3 * 4096 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->nmc, (size_t)np * 3 * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->ibm_err, sizeof(int)) != hipSuccess ||
         hipMalloc(&c->op, 24 * (size_t)np * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->ps.nbf, (size_t)np * IPG_NB * sizeof(int)) != hipSuccess) {
@@ -151,6 +153,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
     (void)hipMemset(c->fl, 0, 3 * M * sizeof(double));
     (void)hipMemset(c->fl2, 0, 3 * M * sizeof(double));
     (void)hipMemset(c->nbc, 0, (size_t)np * sizeof(int));
+    (void)hipMemset(c->nmc, 0, (size_t)np * 3 * sizeof(int));
     (void)hipMemset(c->ibm_err, 0, sizeof(int));
     (void)hipMemset(c->ps.nbf, 0, (size_t)np * IPG_NB * sizeof(int));
     for (int q = 0; q < IPG_NPART; q++) {

@@ -28,6 +28,7 @@ namespace {
 constexpr int IBM_THREADS = 512;  // threads per (sphere, component) workgroup
 constexpr int IBM_MPT = 2;        // markers per thread: nl <= 1024
 constexpr int IBM_MAXNB = 64;     // neighbour images per sphere
+constexpr int IBM_MAXNM = 4096;   // neighbour markers touching one (sphere, component) box
 
 // src/coordsfp.f90:49-68 / src/intgr_nwtn_eulr.f90:1451-1465 marker position
 // and the isout test; false when the marker is not kept.
@@ -61,6 +62,9 @@ struct IbmArgs {
   double *fld[3];     // dudt dvdt dwdt
   const int *nbl;     // [np][IBM_MAXNB] neighbour images: q*25 + (sx+2)*5 + (sy+2)
   const int *nbc;     // [np] neighbour image count
+  int *nml;           // [np*3][IBM_MAXNM] neighbour markers touching the box: (q*nl + l)*25 + shift
+  int *nmc;           // [np*3] their count
+  int *err;           // bit 1: a neighbour list overflowed
   double *Kread, *Kspread, *Kzero;  // dwdtf(.,.,k1) planes
   const double *dtp;
 };
@@ -187,8 +191,30 @@ __global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int
   }
 }
 
+// Stencil of marker l of sphere q, seen through the periodic shift (sxs nx,
+// sys ny) in the unwrapped frame of box b; false when it misses the box.
+__device__ __forceinline__ bool neighbour_stencil(const IbmArgs &A, const Box &b, int comp, int qs, int l, int sxs, int sys,
+                                                  CompStencil &c) {
+  const Grid &g = A.g;
+  const double xq = A.px[qs], yq = A.py[qs], zq = A.pz[qs];
+  // a stencil spans less than two cells either side of its marker
+  const double xs = (xq + A.mk[l]) * g.dxi + sxs * g.nx, ys = (yq + A.mk[A.nl + l]) * g.dyi + sys * g.ny;
+  const double zs = (zq + A.mk[2 * A.nl + l]) * g.dzi;
+  if (xs < b.bi0 - 3.0 || xs > b.bi0 + b.bx + 2.0 || ys < b.bj0 - 3.0 || ys > b.bj0 + b.by + 2.0 ||
+      zs < b.bk0 - 3.0 || zs > b.bk0 + b.bz + 2.0)
+    return false;
+  double xl, yl, zl;
+  if (!marker_pos(g, xq, yq, zq, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return false;
+  comp_stencil(g, xl, yl, zl, comp, c);
+  c.i0 += sxs * g.nx;
+  c.j0 += sys * g.ny;
+  return !(c.i0 + 2 < b.bi0 || c.i0 >= b.bi0 + b.bx || c.j0 + 2 < b.bj0 || c.j0 >= b.bj0 + b.by ||
+           c.k0 + 2 < b.bk0 || c.k0 >= b.bk0 + b.bz);
+}
+
 // One IBM launch for one (sphere, velocity component) workgroup.
-//   MODE 0 (pass 0): gather from dudt, set the marker forces.
+//   MODE 0 (pass 0): gather from dudt, set the marker forces; list the
+//     neighbours' markers whose stencils touch this box (for the next launches).
 //   MODE 1 (passes 1, 2): rebuild the previous pass's spread S in LDS, gather
 //     from fl(dudt + S), add to the marker forces.
 //   MODE 2 (after pass 2): rebuild the last spread S, add it to dudt on the
@@ -224,34 +250,25 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   // as neighbours read them while their owners compute the new ones
   const double *flp = A.flp + comp * M;
   const long m0 = (long)p * A.nl;
-  // own markers: stencil weights, LDS index of the stencil's first cell, and
-  // how many of its z planes lie in the box (the rest is the k1 plane)
-  double wx[IBM_MPT][3], wy[IBM_MPT][3], wz[IBM_MPT][3], ub[IBM_MPT];
-  int e0[IBM_MPT], nzin[IBM_MPT], kx0[IBM_MPT], ky0[IBM_MPT];
-#pragma unroll
-  for (int q = 0; q < IBM_MPT; q++) {
+  // Own marker q of this thread: stencil, LDS index e0 of the stencil's first
+  // cell, how many of its z planes lie in the box (the rest is the k1 plane),
+  // and the rigid-body velocity ub.  Recomputed in each phase that needs it
+  // rather than kept live across the launch.  The box holds the stencil in x
+  // and y, and in z up to kmax (sphere_box); a stencil that were not held
+  // would be left out rather than index outside the box.
+  auto own = [&](int q, CompStencil &c, int &e0, int &nzin, double &ub) -> bool {
     const int l = (int)threadIdx.x + q * IBM_THREADS;
-    double xl = 0.0, yl = 0.0, zl = 0.0;
-    e0[q] = -1; nzin[q] = 0; kx0[q] = 0; ky0[q] = 0; ub[q] = 0.0;
-    if (l < A.nl && marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) {
-      CompStencil c;
-      comp_stencil(g, xl, yl, zl, comp, c);
-#pragma unroll
-      for (int r = 0; r < 3; r++) { wx[q][r] = c.wx[r]; wy[q][r] = c.wy[r]; wz[q][r] = c.wz[r]; }
-      // the box holds the stencil in x and y, and in z up to kmax
-      // (sphere_box); a stencil that were not held would be left out rather
-      // than index outside the box
-      const int ein = box_at(b, c.i0, c.j0, c.k0);
-      if (ein >= 0 && box_at(b, c.i0 + 2, c.j0 + 2, c.k0) >= 0) {
-        e0[q] = ein;
-        nzin[q] = min(3, b.bk0 + b.bz - c.k0);
-        kx0[q] = c.i0; ky0[q] = c.j0;
-      }
-      if (comp == 0) ub[q] = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
-      else if (comp == 1) ub[q] = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
-      else ub[q] = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
-    }
-  }
+    double xl, yl, zl;
+    if (l >= A.nl || !marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return false;
+    comp_stencil(g, xl, yl, zl, comp, c);
+    e0 = box_at(b, c.i0, c.j0, c.k0);
+    if (e0 < 0 || box_at(b, c.i0 + 2, c.j0 + 2, c.k0) < 0) return false;
+    nzin = min(3, b.bk0 + b.bz - c.k0);
+    if (comp == 0) ub = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
+    else if (comp == 1) ub = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
+    else ub = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
+    return true;
+  };
   for (int t = threadIdx.x; t < nflag; t += blockDim.x) {
     flag[t] = 0u;
     if (MODE == 2) taken[t] = 0u;
@@ -263,48 +280,60 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   // the previous pass into S
 #pragma unroll
   for (int q = 0; q < IBM_MPT; q++) {
-    if (e0[q] < 0) continue;
+    CompStencil c;
+    int e0, nzin;
+    double ub;
+    if (!own(q, c, e0, nzin, ub)) continue;
     const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
     const double fc = MODE >= 1 ? flp[m] * c0 : 0.0;
-    for (int k = 0; k < nzin[q]; k++)
+    for (int k = 0; k < nzin; k++)
       for (int j = 0; j < 3; j++)
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-          const int e = e0[q] + i + b.bx * j + bxy * k;
+          const int e = e0 + i + b.bx * j + bxy * k;
           atomicOr(&flag[e >> 5], 1u << (e & 31));
-          if (MODE >= 1) unsafeAtomicAdd(&box[e], fc * wx[q][i] * wy[q][j] * wz[q][k]);
+          if (MODE >= 1) unsafeAtomicAdd(&box[e], fc * c.wx[i] * c.wy[j] * c.wz[k]);
         }
   }
-  if (MODE >= 1) {
-    // the neighbours' markers: their contributions to this box
+  __shared__ int lcount;
+  const long lbase = (long)blockIdx.x * IBM_MAXNM;
+  if (MODE == 0) {
+    // list the neighbours' markers whose stencils touch this box; the
+    // markers do not move inside the IBM block, so the three launches that
+    // spread them reuse the list
+    if (threadIdx.x == 0) lcount = 0;
+    __syncthreads();
     const int nn = A.nbc[p];
     const int tot = nn * A.nl;
     for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
       const int n = idx / A.nl, l = idx - n * A.nl;
       const int code = A.nbl[(long)p * IBM_MAXNB + n];
-      const int qs = code / 25, sxs = (code / 5) % 5 - 2, sys = code % 5 - 2;
-      const double xq = A.px[qs], yq = A.py[qs], zq = A.pz[qs];
-      // markers on the far side of the neighbour cannot reach this box: a
-      // stencil spans less than two cells either side of its marker
-      {
-        const double xs = (xq + A.mk[l]) * g.dxi + sxs * g.nx, ys = (yq + A.mk[A.nl + l]) * g.dyi + sys * g.ny;
-        const double zs = (zq + A.mk[2 * A.nl + l]) * g.dzi;
-        if (xs < b.bi0 - 3.0 || xs > b.bi0 + b.bx + 2.0 || ys < b.bj0 - 3.0 || ys > b.bj0 + b.by + 2.0 ||
-            zs < b.bk0 - 3.0 || zs > b.bk0 + b.bz + 2.0)
-          continue;
-      }
-      double xl, yl, zl;
-      if (!marker_pos(g, xq, yq, zq, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
+      const int qs = code / 25, sh = code % 25;
       CompStencil c;
-      comp_stencil(g, xl, yl, zl, comp, c);
-      const int i0 = c.i0 + sxs * g.nx, j0 = c.j0 + sys * g.ny;
-      if (i0 + 2 < b.bi0 || i0 >= b.bi0 + b.bx || j0 + 2 < b.bj0 || j0 >= b.bj0 + b.by) continue;
-      const double fc = flp[(long)qs * A.nl + l] * c0;
+      if (!neighbour_stencil(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c)) continue;
+      const int slot = atomicAdd(&lcount, 1);
+      if (slot < IBM_MAXNM) A.nml[lbase + slot] = (qs * A.nl + l) * 25 + sh;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      A.nmc[blockIdx.x] = min(lcount, IBM_MAXNM);
+      if (lcount > IBM_MAXNM) atomicOr(A.err, 2);
+    }
+  } else {
+    // the listed neighbour markers' contributions to this box
+    const int cnt = A.nmc[blockIdx.x];
+    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+      const int code = A.nml[lbase + idx];
+      const int mq = code / 25, sh = code % 25;
+      const int qs = mq / A.nl, l = mq - qs * A.nl;
+      CompStencil c;
+      if (!neighbour_stencil(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c)) continue;
+      const double fc = flp[mq] * c0;
       for (int k = 0; k < 3; k++)
         for (int j = 0; j < 3; j++)
 #pragma unroll
           for (int i = 0; i < 3; i++) {
-            const int e = box_at(b, i0 + i, j0 + j, c.k0 + k);
+            const int e = box_at(b, c.i0 + i, c.j0 + j, c.k0 + k);
             if (e < 0) continue;
             unsafeAtomicAdd(&box[e], fc * c.wx[i] * c.wy[j] * c.wz[k]);
             if (MODE == 2 && qs < p) atomicOr(&taken[e >> 5], 1u << (e & 31));
@@ -350,7 +379,10 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < IBM_MPT; q++) {
-    if (e0[q] < 0) continue;
+    CompStencil c;
+    int e0, nzin;
+    double ub;
+    if (!own(q, c, e0, nzin, ub)) continue;
     const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
     double acc = 0.0;
     for (int k = 0; k < 3; k++)
@@ -358,21 +390,21 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
         for (int i = 0; i < 3; i++) {
           double f;
-          if (k < nzin[q]) f = box[e0[q] + i + b.bx * j + bxy * k];
-          else f = A.Kread[(long)(wrapany(kx0[q] + i, g.nx) - 1) + (long)g.nx * (wrapany(ky0[q] + j, g.ny) - 1)];
-          acc = acc + f * wx[q][i] * wy[q][j] * wz[q][k];
+          if (k < nzin) f = box[e0 + i + b.bx * j + bxy * k];
+          else f = A.Kread[(long)(wrapany(c.i0 + i, g.nx) - 1) + (long)g.nx * (wrapany(c.j0 + j, g.ny) - 1)];
+          acc = acc + f * c.wx[i] * c.wy[j] * c.wz[k];
         }
-    const double fnew = MODE == 0 ? (ub[q] - acc) * dti : flp[m] + (ub[q] - acc) * dti;
+    const double fnew = MODE == 0 ? (ub - acc) * dti : flp[m] + (ub - acc) * dti;
     fl[m] = fnew;
     // the part of this pass's spread that falls on the k1 plane
-    if (nzin[q] < 3) {
+    if (nzin < 3) {
       const double fc = fnew * c0;
-      for (int k = nzin[q]; k < 3; k++)
+      for (int k = nzin; k < 3; k++)
         for (int j = 0; j < 3; j++)
 #pragma unroll
           for (int i = 0; i < 3; i++)
-            unsafeAtomicAdd(&A.Kspread[(long)(wrapany(kx0[q] + i, g.nx) - 1) + (long)g.nx * (wrapany(ky0[q] + j, g.ny) - 1)],
-                            fc * wx[q][i] * wy[q][j] * wz[q][k]);
+            unsafeAtomicAdd(&A.Kspread[(long)(wrapany(c.i0 + i, g.nx) - 1) + (long)g.nx * (wrapany(c.j0 + j, g.ny) - 1)],
+                            fc * c.wx[i] * c.wy[j] * c.wz[k]);
     }
   }
 }
@@ -450,7 +482,7 @@ bool ibm_args(ipg_ctx *c, IbmArgs &A, size_t &lds) {
   A.pomx = q[IPG_POMX]; A.pomy = q[IPG_POMY]; A.pomz = q[IPG_POMZ]; A.pphi = q[IPG_PPHI];
   A.mk = c->mk; A.fl = c->fl;
   A.fld[0] = c->f[IPG_DUDT]; A.fld[1] = c->f[IPG_DVDT]; A.fld[2] = c->f[IPG_DWDT];
-  A.nbl = c->nbl; A.nbc = c->nbc;
+  A.nbl = c->nbl; A.nbc = c->nbc; A.nml = c->nml; A.nmc = c->nmc; A.err = c->ibm_err;
   A.dtp = c->dscal;
   return true;
 }
@@ -528,7 +560,11 @@ int ipg_ibm_check(ipg_ctx *c) {
   if (c->pc.np <= 0 || !c->ibm_err) return 0;
   int e = 0;
   IPG_HIP(hipMemcpy(&e, c->ibm_err, sizeof(int), hipMemcpyDeviceToHost));
-  if (e) { ipg_set_error("ibm: more than 64 neighbour images around a sphere"); return 3; }
+  if (e) {
+    ipg_set_error(e & 1 ? "ibm: more than 64 neighbour images around a sphere"
+                        : "ibm: more than 4096 neighbour markers touching one sphere's stencils");
+    return 3;
+  }
   return 0;
 }
 

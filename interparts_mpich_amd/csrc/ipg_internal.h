@@ -69,6 +69,7 @@ struct ipg_ctx {
   double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl (after passes 0 and 2)
   double *fl2;                    // [3][np*nl] marker forces after pass 1
   int *nbl, *nbc;                 // IBM neighbour images: [np][64] codes, [np] counts
+  int *nml, *nmc;                 // IBM neighbour markers per (sphere, component): [3 np][4096], [3 np]
   int *ibm_err;                   // set on the device when a neighbour list overflows
   // particles
   PartState ps;
