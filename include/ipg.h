@@ -93,12 +93,16 @@ int ipg_ibm(ipg_ctx *ctx);                            /* IBM block, src/main.f90
                                                          complagrforces/updtlagrforces src/forcing.f90:9,189,
                                                          lagr2eulr src/interp_spread.f90:230 */
 /* the IBM block split as the reference calls it (src/main.f90:299-335):
- * ipg_ibm_begin replaces the copy dudtf = dudt (main.f90:301-303);
- * per pass ibmiter = 0..2: ipg_eulr2lagr (interp_spread.f90:12, which here
- * also updates the marker forces), ipg_lagrforces (the per-sphere sums of
+ * ipg_ibm_begin stands where the copy dudtf = dudt is (main.f90:301-303) and
+ * builds the sphere neighbour lists of this substep; per pass
+ * ibmiter = 0..2: ipg_eulr2lagr (interp_spread.f90:12; for ibmiter > 0 it
+ * first rebuilds the previous pass's spread around each sphere, and it also
+ * updates the marker forces), ipg_lagrforces (the per-sphere sums of
  * complagrforces forcing.f90:9 / updtlagrforces forcing.f90:189),
- * ipg_lagr2eulr (interp_spread.f90:230; the last pass writes dudt directly,
- * replacing main.f90:332-334) */
+ * ipg_lagr2eulr (interp_spread.f90:230: for ibmiter 0 and 1 nothing is
+ * launched, as the next ipg_eulr2lagr rebuilds that spread; for ibmiter 2 the
+ * spread is added to dudt, replacing main.f90:332-334).  Calls must come in
+ * this order within one substep. */
 int ipg_ibm_begin(ipg_ctx *ctx);
 int ipg_eulr2lagr(ipg_ctx *ctx, int ibmiter);
 int ipg_lagrforces(ipg_ctx *ctx, int ibmiter);
