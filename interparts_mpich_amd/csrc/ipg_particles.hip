@@ -42,19 +42,6 @@ __device__ __forceinline__ void image_shift(const Grid &g, int nbrecv, double &x
   }
 }
 
-// Bounds of cell (i,j,k) of velocity component comp, src/intgr_over_sphere.f90:336-345,
-// 395-404, 454-463:
-// the staggered direction is centred on the face, the others span the cell.
-__device__ __forceinline__ void cell_bounds(int comp, int i, int j, int k, double dx, double dy, double dz, double lo[3],
-                                            double hi[3]) {
-  if (comp == 2) { lo[2] = ((double)k - 0.5) * dz; hi[2] = ((double)k + 0.5) * dz; }
-  else { lo[2] = (double)(k - 1) * dz; hi[2] = (double)k * dz; }
-  if (comp == 1) { lo[1] = 0.0 + ((double)j - 0.5) * dy; hi[1] = 0.0 + ((double)j + 0.5) * dy; }
-  else { lo[1] = 0.0 + (double)(j - 1) * dy; hi[1] = 0.0 + (double)j * dy; }
-  if (comp == 0) { lo[0] = 0.0 + ((double)i - 0.5) * dx; hi[0] = 0.0 + ((double)i + 0.5) * dx; }
-  else { lo[0] = 0.0 + (double)(i - 1) * dx; hi[0] = 0.0 + (double)i * dx; }
-}
-
 // types: 1 volume, 2 int u dV, 3 int r x u dV, 23 both 2 and 3 from one sweep
 __global__ __launch_bounds__(512) void k_sphere_int(Grid g, PartConst pc, const double *__restrict__ U,
                                                     const double *__restrict__ V, const double *__restrict__ W, PQ P,
@@ -87,72 +74,55 @@ __global__ __launch_bounds__(512) void k_sphere_int(Grid g, PartConst pc, const 
       const int nij = ni * nj, tot = nij * nk;
       for (int comp = 0; comp < 3; comp++) {
         const double *F = comp == 0 ? U : (comp == 1 ? V : W);
-        // four cells per thread at a time: their loads are issued together,
-        // then the cells are accumulated in the same order as one at a time
-        constexpr int UNR = 4;
-        for (int t0 = threadIdx.x; t0 < tot; t0 += UNR * (int)blockDim.x) {
-          double fvals[UNR];
-          bool live[UNR];
-#pragma unroll
-          for (int u = 0; u < UNR; u++) {
-            const int t = t0 + u * (int)blockDim.x;
-            live[u] = false;
-            fvals[u] = 0.0;
-            if (t >= tot) continue;
-            const int kk = t / nij, r = t - kk * nij, jj = r / ni;
-            const int i = ilow + (r - jj * ni), j = jlow + jj, k = klow + kk;
-            double lo[3], hi[3];
-            cell_bounds(comp, i, j, k, dx, dy, dz, lo, hi);
-            // A cell whose nearest point lies beyond R has every corner
-            // outside: its weight sum1/sum2 is exactly 0 and it adds nothing.
-            const double ex = lo[0] > X ? lo[0] - X : (hi[0] < X ? X - hi[0] : 0.0);
-            const double ey = lo[1] > Y ? lo[1] - Y : (hi[1] < Y ? Y - hi[1] : 0.0);
-            const double ez = lo[2] > Z ? lo[2] - Z : (hi[2] < Z ? Z - hi[2] : 0.0);
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+          const int kk = t / nij, r = t - kk * nij, jj = r / ni;
+          const int i = ilow + (r - jj * ni), j = jlow + jj, k = klow + kk;
+          double zmin, zplus, ymin, yplus, xmin, xplus;
+          if (comp == 2) { zmin = ((double)k - 0.5) * dz; zplus = ((double)k + 0.5) * dz; }
+          else { zmin = (double)(k - 1) * dz; zplus = (double)k * dz; }
+          if (comp == 1) { ymin = 0.0 + ((double)j - 0.5) * dy; yplus = 0.0 + ((double)j + 0.5) * dy; }
+          else { ymin = 0.0 + (double)(j - 1) * dy; yplus = 0.0 + (double)j * dy; }
+          if (comp == 0) { xmin = 0.0 + ((double)i - 0.5) * dx; xplus = 0.0 + ((double)i + 0.5) * dx; }
+          else { xmin = 0.0 + (double)(i - 1) * dx; xplus = 0.0 + (double)i * dx; }
+          const double dzp2 = sqd(zplus - Z), dyp2 = sqd(yplus - Y), dxp2 = sqd(xplus - X);
+          const double dist2 = dxp2 + dyp2 + dzp2;
+          // A cell whose nearest point lies beyond R has every corner outside:
+          // its weight sum1/sum2 is exactly 0 and it adds nothing (skipped).
+          {
+            const double ex = xmin > X ? xmin - X : (xplus < X ? X - xplus : 0.0);
+            const double ey = ymin > Y ? ymin - Y : (yplus < Y ? Y - yplus : 0.0);
+            const double ez = zmin > Z ? zmin - Z : (zplus < Z ? Z - zplus : 0.0);
             if (ex * ex + ey * ey + ez * ez > R * R * (1.0 + 1e-9)) continue;
-            live[u] = true;
-            fvals[u] = F[g.at(i, j, k)];
           }
-#pragma unroll
-          for (int u = 0; u < UNR; u++) {
-            if (!live[u]) continue;
-            const int t = t0 + u * (int)blockDim.x;
-            const int kk = t / nij, r = t - kk * nij, jj = r / ni;
-            const int i = ilow + (r - jj * ni), j = jlow + jj, k = klow + kk;
-            double lo[3], hi[3];
-            cell_bounds(comp, i, j, k, dx, dy, dz, lo, hi);
-            const double xmin = lo[0], xplus = hi[0], ymin = lo[1], yplus = hi[1], zmin = lo[2], zplus = hi[2];
-            const double dzp2 = sqd(zplus - Z), dyp2 = sqd(yplus - Y), dxp2 = sqd(xplus - X);
-            const double dist2 = dxp2 + dyp2 + dzp2;
-            const double fval = fvals[u];
-            const bool full = dist2 < radin2;
-            double wfrac = 1.0;
-            if (!full) {
-              const double dxm2 = sqd(xmin - X), dym2 = sqd(ymin - Y), dzm2 = sqd(zmin - Z);
-              double sg[8];
-              sg[0] = R - sqrt(dxm2 + dym2 + dzm2); sg[1] = R - sqrt(dxm2 + dyp2 + dzm2);
-              sg[2] = R - sqrt(dxp2 + dyp2 + dzm2); sg[3] = R - sqrt(dxp2 + dym2 + dzm2);
-              sg[4] = R - sqrt(dxm2 + dym2 + dzp2); sg[5] = R - sqrt(dxm2 + dyp2 + dzp2);
-              sg[6] = R - sqrt(dxp2 + dyp2 + dzp2); sg[7] = R - sqrt(dxp2 + dym2 + dzp2);
-              double s1 = 0.0, s2 = 0.0;
-              for (int cp = 0; cp < 8; cp++) { if (sg[cp] > 0.0) s1 = s1 + sg[cp]; s2 = s2 + fabs(sg[cp]); }
-              wfrac = s1 / s2;
-            }
-            if (t1) s[comp] += full ? dVeul : wfrac * dVeul;
-            if (t2) s[comp] += full ? fval * dVeul : fval * wfrac * dVeul;
-            if (t3) {
-              const double rx = 0.0 + ((double)i - 0.5) * dx - X;
-              const double ry = 0.0 + ((double)j - 0.5) * dy - Y;
-              const double rz = ((double)k - 0.5) * dz - Z;
-              if (comp == 0) {
-                s[4] += full ? rz * fval * dVeul : rz * fval * wfrac * dVeul;
-                s[5] -= full ? ry * fval * dVeul : ry * fval * wfrac * dVeul;
-              } else if (comp == 1) {
-                s[3] -= full ? rz * fval * dVeul : rz * fval * wfrac * dVeul;
-                s[5] += full ? rx * fval * dVeul : rx * fval * wfrac * dVeul;
-              } else {
-                s[3] += full ? ry * fval * dVeul : ry * fval * wfrac * dVeul;
-                s[4] -= full ? rx * fval * dVeul : rx * fval * wfrac * dVeul;
-              }
+          const double fval = F[g.at(i, j, k)];
+          bool full = dist2 < radin2;
+          double wfrac = 1.0;
+          if (!full) {
+            const double dxm2 = sqd(xmin - X), dym2 = sqd(ymin - Y), dzm2 = sqd(zmin - Z);
+            double sg[8];
+            sg[0] = R - sqrt(dxm2 + dym2 + dzm2); sg[1] = R - sqrt(dxm2 + dyp2 + dzm2);
+            sg[2] = R - sqrt(dxp2 + dyp2 + dzm2); sg[3] = R - sqrt(dxp2 + dym2 + dzm2);
+            sg[4] = R - sqrt(dxm2 + dym2 + dzp2); sg[5] = R - sqrt(dxm2 + dyp2 + dzp2);
+            sg[6] = R - sqrt(dxp2 + dyp2 + dzp2); sg[7] = R - sqrt(dxp2 + dym2 + dzp2);
+            double s1 = 0.0, s2 = 0.0;
+            for (int cp = 0; cp < 8; cp++) { if (sg[cp] > 0.0) s1 = s1 + sg[cp]; s2 = s2 + fabs(sg[cp]); }
+            wfrac = s1 / s2;
+          }
+          if (t1) s[comp] += full ? dVeul : wfrac * dVeul;
+          if (t2) s[comp] += full ? fval * dVeul : fval * wfrac * dVeul;
+          if (t3) {
+            const double rx = 0.0 + ((double)i - 0.5) * dx - X;
+            const double ry = 0.0 + ((double)j - 0.5) * dy - Y;
+            const double rz = ((double)k - 0.5) * dz - Z;
+            if (comp == 0) {
+              s[4] += full ? rz * fval * dVeul : rz * fval * wfrac * dVeul;
+              s[5] -= full ? ry * fval * dVeul : ry * fval * wfrac * dVeul;
+            } else if (comp == 1) {
+              s[3] -= full ? rz * fval * dVeul : rz * fval * wfrac * dVeul;
+              s[5] += full ? rx * fval * dVeul : rx * fval * wfrac * dVeul;
+            } else {
+              s[3] += full ? ry * fval * dVeul : ry * fval * wfrac * dVeul;
+              s[4] -= full ? rx * fval * dVeul : rx * fval * wfrac * dVeul;
             }
           }
         }
