@@ -145,6 +145,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=30, help="untimed substeps")
     ap.add_argument("--config", default="c2_256")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default RK-stage launch shape")
     ap.add_argument("--cpu-substeps", type=int, default=12, help="oracle substeps in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -176,6 +177,7 @@ def main():
     fields, spheres, markers = initial_fields(case), place_spheres(case), markers_for(case)
     print(f"[bench] {case.name}: {case.itot}x{case.jtot}x{case.ktot}, np={case.np}, nl={case.nl}", file=sys.stderr, flush=True)
     sim = Simulation(case, use_graphs=not args.no_graphs).initialize(fields, spheres, markers)
+    rk_kb = None if args.no_autotune else sim.autotune()
     for s in range(args.warmup):
         sim.substep(s % 3 + 1)
     sim.sync()
@@ -234,6 +236,7 @@ def main():
         "config": {"workload": f"{case.name}: {case.itot}x{case.jtot}x{case.ktot} channel, {case.np} spheres "
                                f"D={case.diameter_h:g}h nl={case.nl}, Re_bulk={case.re_bulk:g}, FP64 path",
                    "graphs": not args.no_graphs,
+                   "rk_z_planes_per_workgroup": rk_kb if rk_kb is not None else "default",
                    "gcell_updates_per_s": round(value * case.cells / 1e9, 4)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

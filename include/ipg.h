@@ -2,7 +2,7 @@
  * ipg.h -- C-ABI of the MI355X-native InterPartS RK3 substep (libipg.so).
  *
  * One context holds one case resident in HBM: the staggered fields with one
- * ghost layer, the IBM force buffers, the Poisson spectra, the particle and
+ * ghost layer, the IBM neighbour lists, the Poisson spectra, the particle and
  * marker state.  Every entry point returns 0 on success and a non-zero code on
  * failure (ipg_last_error() gives the message); the Fortran layer turns a
  * non-zero code into `error stop`, as the reference aborts inside the routine.
@@ -129,6 +129,9 @@ int ipg_uniform(unsigned long long seed, unsigned long long stream, unsigned lon
 /* ---- runtime controls (no reference counterpart) --------------------- */
 int ipg_sync(ipg_ctx *ctx);                           /* wait for the context's stream */
 int ipg_use_graphs(ipg_ctx *ctx, int on);             /* capture each substep as a hipGraph */
+/* time the RK stage for 8..64 z planes per workgroup and keep the fastest;
+ * overwrites dudt..dcdt (recomputed by every stage): call between substeps */
+int ipg_autotune(ipg_ctx *ctx, int *kb);
 /* HBM bytes one substep must move by the algorithmic model, for roofline */
 int ipg_substep_bytes(ipg_ctx *ctx, double *bytes);
 /* one eager substep with a HIP event after every kernel launch: per-launch

@@ -479,18 +479,47 @@ int ipg_substep_phases(ipg_ctx *c, int rkiter, int maxn, double *ms, const char 
   return 0;
 }
 
+// Launch-shape tuning: times the RK stage for z-block lengths 8..64 planes
+// per workgroup on this grid and keeps the fastest.  It runs stage 3, which
+// overwrites dudt..dcdt only (every RK stage recomputes them), so it may be
+// called before any substep; captured graphs are rebuilt afterwards.
+int ipg_autotune(ipg_ctx *c, int *kb_out) {
+  if (!check_ctx(c)) return 1;
+  for (int q = 0; q < 2; q++)
+    if (!c->ev[q]) IPG_HIP(hipEventCreate(&c->ev[q]));
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  const int saved = c->tune_rk_kb;
+  int best = 0;
+  float best_ms = 0.f;
+  for (int kb = 8; kb <= 64 && kb <= c->g.nz; kb *= 2) {
+    c->tune_rk_kb = kb;
+    if (ipg_launch_rk(c, 3)) { c->tune_rk_kb = saved; return 1; }  // warm-up
+    float t_min = 0.f;
+    for (int rep = 0; rep < 3; rep++) {
+      IPG_HIP(hipEventRecord(c->ev[0], c->stream));
+      if (ipg_launch_rk(c, 3)) { c->tune_rk_kb = saved; return 1; }
+      IPG_HIP(hipEventRecord(c->ev[1], c->stream));
+      IPG_HIP(hipEventSynchronize(c->ev[1]));
+      float t = 0.f;
+      IPG_HIP(hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+      if (rep == 0 || t < t_min) t_min = t;
+    }
+    if (best == 0 || t_min < best_ms) { best = kb; best_ms = t_min; }
+  }
+  c->tune_rk_kb = best > 0 ? best : saved;
+  for (int r = 0; r < 3; r++)
+    if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
+  if (kb_out) *kb_out = c->tune_rk_kb;
+  return 0;
+}
+
 int ipg_use_graphs(ipg_ctx *c, int on) {
   if (!check_ctx(c)) return 1;
   c->use_graphs = on != 0;
   return 0;
 }
 
-// Algorithmic HBM bytes of one substep (FP64), the roofline numerator:
-// fused RK stage (stage-averaged 14.33 N words), fillps 4 N, Poisson
-// x-fwd 2 + y-fwd 2 + Thomas (2 + 0.5 d writes + 2 + 0.5 d reads) + y-inv 2 +
-// x-inv 2 words per cell (half spectrum of nx/2+1 complex = N words), correc
-// with pnew/cnew 11 N, ghost fills O(n^2) omitted; IBM per pass 27-point
-// stencils x3 components of marker-unique cells counted once, markers 15 words.
+// Algorithmic HBM bytes of one substep (FP64), the roofline numerator.
 int ipg_substep_bytes(ipg_ctx *c, double *bytes) {
   if (!check_ctx(c) || !bytes) return 1;
   // FP64 words per substep, by kernel: RK stage 43/3 N (13, 17, 13 by stage),

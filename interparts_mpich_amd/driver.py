@@ -148,6 +148,12 @@ class Simulation:
     def substep(self, rkiter):
         check(self.lib.ipg_substep(self.ctx, rkiter), "substep")
 
+    def autotune(self):
+        """Pick the RK stage's z-block length on this grid (ipg_autotune)."""
+        kb = C.c_int()
+        check(self.lib.ipg_autotune(self.ctx, C.byref(kb)), "autotune")
+        return kb.value
+
     def sync(self):
         check(self.lib.ipg_sync(self.ctx), "sync")
 

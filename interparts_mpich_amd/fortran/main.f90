@@ -31,6 +31,7 @@ program interpt
   real :: dtmax, maxerror, avererror, t0, t1
   integer :: ibmiter, u, last
   integer(8) :: c0, c1, crate
+  integer(c_int) :: kbtune
 
   call input_parameters
   call initmpi
@@ -66,6 +67,8 @@ program interpt
     end if
   end if
   write(6, *) 'dtmax = ', dtmax, ' dt = ', dt
+  ! launch shape of the RK stage for this grid (overwrites dudt.. only)
+  call ipg_ok(ipg_autotune(ctx, kbtune), 'autotune')
 
   call system_clock(c0, crate)
   last = begin

@@ -12,7 +12,8 @@ module mod_ipg_capi
             ipg_particles_get, ipg_set_dt, ipg_get_dt, ipg_initsolver, ipg_initparticles, ipg_coordsfp, ipg_rk, &
             ipg_ibm, ipg_ibm_begin, ipg_eulr2lagr, ipg_lagrforces, ipg_lagr2eulr, ipg_bounduvw, ipg_boundp, &
             ipg_boundscal, ipg_fillps, ipg_solver2d, ipg_correc, ipg_pnew_update, ipg_intgr_over_sphere, &
-            ipg_intgr_nwtn_eulr, ipg_substep, ipg_chkdt, ipg_chkdiv, ipg_uniform, ipg_sync, ipg_use_graphs
+            ipg_intgr_nwtn_eulr, ipg_substep, ipg_chkdt, ipg_chkdiv, ipg_uniform, ipg_sync, ipg_use_graphs, &
+            ipg_autotune
 
   ! field selectors (enum of include/ipg.h)
   integer(c_int), parameter, public :: IPG_U = 0, IPG_V = 1, IPG_W = 2, IPG_PNEW = 3, IPG_C = 4, &
@@ -199,6 +200,11 @@ module mod_ipg_capi
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: on
+    end function
+    integer(c_int) function ipg_autotune(ctx, kb) bind(C, name="ipg_autotune")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: kb
     end function
   end interface
 
