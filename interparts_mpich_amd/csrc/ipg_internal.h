@@ -45,8 +45,7 @@ struct Poisson {
   int nmx;          // nx/2+1 complex x-modes
   long sp;          // spectrum row pitch (complex elements)
   double2 *spec;    // [nz][ny][sp] complex spectrum
-  double *tz, *td;  // [nz][ny/2+1][nmx] Thomas pivots 1/(b + xy - a d) and multipliers d, by |my|
-  double *zl;       // [ny/2+1][nmx] the last row's divisor
+  double *dtri;     // [nz][ny][sp] Thomas multipliers d
   double *a, *b, *c;// [nz+1] tridiagonal coefficients (1-based)
   double *xrt, *yrt;// eigenvalues by x-mode m (nmx) and y-mode my (ny)
   double2 *twx, *twy; // e^{-2 pi i t/n} tables

@@ -14,9 +14,10 @@
 // The transforms are Stockham mixed-radix (4, 2, 3) FFTs in LDS.  Lengths
 // 16..512 with radices 2 and 3 are compiled with the length and the batch as
 // constants (all index arithmetic folds); other lengths use a generic
-// runtime-plan kernel.  Both passes interleave the sequences in LDS
-// (element r of sequence s at r*B+s), which keeps every butterfly access
-// bank-conflict free.  The x forward pass can compute the
+// runtime-plan kernel.  The x pass keeps each sequence contiguous in LDS
+// (global rows load coalesced); the y pass interleaves the sequences
+// (element r of sequence s at r*B+s) so that the tile loads and every
+// butterfly access are bank-conflict free.  The x forward pass can compute the
 // right-hand side of src/fillps.f90 on the fly from dudt, dvdt, dwdt.
 #include "ipg_internal.h"
 #include <cmath>
@@ -181,11 +182,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, const doubl
     const double dti = 1.0 / (rkab * *dtp);
     dtidxi = dti * g.dxi; dtidyi = dti * g.dyi; dtidzi = dti * g.dzi;
   }
-  // LDS holds the B sequences interleaved (element i of sequence b at i*B+b):
-  // consecutive lanes take consecutive b, so LDS accesses are conflict-free
-  // and each row is still read in 128-byte runs
   for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
-    const int b = t % B, i = t / B;
+    const int b = t / N, i = t % N;
     const long pr = pair0 + b;
     double2 v = make_double2(0.0, 0.0);
     if (pr < pairs) {
@@ -195,19 +193,19 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, const doubl
                                   rhs_at(g, du, dv, dw, c + g.sx, i + 1, j + 1, k, dtidxi, dtidyi, dtidzi));
       else v = make_double2(p[c], p[c + g.sx]);
     }
-    buf[t] = v;
+    buf[b * N + i] = v;
   }
   __syncthreads();
   double2 *x = buf, *y = buf + B * N;
-  fft_stages<N, B, true, 0, 1>(x, y, tw, -1);
+  fft_stages<N, B, false, 0, 1>(x, y, tw, -1);
   constexpr int NM = N / 2 + 1;
   for (int t = threadIdx.x; t < B * NM; t += FFT_THREADS) {
     const int b = t / NM, m = t % NM;
     const long pr = pair0 + b;
     if (pr >= pairs) continue;
     const int k = (int)(pr / (g.ny / 2)), j = 2 * (int)(pr % (g.ny / 2));
-    double2 za = x[m * B + b];
-    double2 zb = x[((N - m) % N) * B + b];
+    double2 za = x[b * N + m];
+    double2 zb = x[b * N + (N - m) % N];
     zb.y = -zb.y;
     spec[((long)k * g.ny + j) * sp + m] = make_double2(0.5 * (za.x + zb.x), 0.5 * (za.y + zb.y));
     spec[((long)k * g.ny + j + 1) * sp + m] = make_double2(0.5 * (za.y - zb.y), -0.5 * (za.x - zb.x));
@@ -258,19 +256,19 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, double *__r
       else { xa = ra[N - m]; xb = rb[N - m]; xa.y = -xa.y; xb.y = -xb.y; }
       z = make_double2(xa.x - xb.y, xa.y + xb.x);
     }
-    buf[m * B + b] = z;
+    buf[b * N + m] = z;
   }
   __syncthreads();
   double2 *x = buf, *y = buf + B * N;
-  fft_stages<N, B, true, 0, 1>(x, y, tw, +1);
+  fft_stages<N, B, false, 0, 1>(x, y, tw, +1);
   for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
-    const int b = t % B, i = t / B;
+    const int b = t / N, i = t % N;
     const long pr = pair0 + b;
     if (pr >= pairs) continue;
     const int k = (int)(pr / (g.ny / 2)) + 1, j = 2 * (int)(pr % (g.ny / 2)) + 1;
     const long c = g.at(i + 1, j, k);
-    p[c] = x[t].x * scale;
-    p[c + g.sx] = x[t].y * scale;
+    p[c] = x[b * N + i].x * scale;
+    p[c + g.sx] = x[b * N + i].y * scale;
   }
 }
 
@@ -367,77 +365,56 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv(Grid g, double *__res
   }
 }
 
-// The elimination of the Thomas solve (src/solver.f90:90-135) depends only on
-// the eigenvalue sum xy = xrt(m) + yrt(my) and on k, not on the data: the
-// pivots z(k) = 1/(b(k) + xy - a(k) d(k-1)) and multipliers d(k) = c(k) z(k)
-// are computed once, at solver set-up, by this recurrence (one thread per
-// (m, |my| folded); y-modes my and ny-my share xy), with the reference's
-// operations in the reference's order.  zl is the last row's divisor
-// b(kmax) + xy - a(kmax) d(kmax-1), kept as a divisor.
-__global__ __launch_bounds__(256) void k_thomas_tables(int nz, int nmx, int nyh, const double *__restrict__ a,
-                                                       const double *__restrict__ b, const double *__restrict__ cc,
-                                                       const double *__restrict__ xrt, const double *__restrict__ yrt,
-                                                       double *__restrict__ tz, double *__restrict__ td,
-                                                       double *__restrict__ zl) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nmx * nyh) return;
-  const int myf = t / nmx, m = t - myf * nmx;
-  const long plane = (long)nmx * nyh;
-  const double xy = xrt[m] + yrt[myf];
-  double dprev = 0.0;
-  for (int k = 1; k <= nz - 1; k++) {
-    const double z = k == 1 ? 1.0 / (b[1] + xy) : 1.0 / ((b[k] + xy) - a[k] * dprev);
-    dprev = cc[k] * z;
-    tz[(long)(k - 1) * plane + t] = z;
-    td[(long)(k - 1) * plane + t] = dprev;
-  }
-  zl[t] = (b[nz] + xy) - a[nz] * dprev;
-}
-
-// Batched Thomas solve along z, src/solver.f90:90-135, with the pivots and
-// multipliers of k_thomas_tables.  One lane per (wavenumber pair, real or
-// imaginary part); adjacent lanes hold the two parts of one complex value, so
+// Batched Thomas solve along z, src/solver.f90:90-135, one thread per
+// (wavenumber pair, real or imaginary part): the two parts share the
+// recurrence of the pivots and multipliers (both compute and store the same
+// values), which doubles the number of independent lanes over one thread per
+// pair.  Adjacent lanes hold the two parts of one complex value, so
 // every load and store is coalesced.  Planes are read TD at a time ahead of
 // the recurrence, so TD loads per lane are in flight together.
 constexpr int TD = 8;
-__global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spec, long sp, int nmx,
-                                                const double *__restrict__ a, const double *__restrict__ tz,
-                                                const double *__restrict__ td, const double *__restrict__ zl) {
+__global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
+                                                const double *__restrict__ a, const double *__restrict__ b,
+                                                const double *__restrict__ cc, const double *__restrict__ xrt,
+                                                const double *__restrict__ yrt) {
   const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const long nmode = (long)nmx * g.ny;
   if (u >= 2 * nmode) return;
   const long t = u >> 1;
   const int part = (int)(u & 1);
   const int my = (int)(t / nmx), m = (int)(t - (long)my * nmx);
-  const int myf = my <= g.ny / 2 ? my : g.ny - my;
-  const int nyh = g.ny / 2 + 1;
-  const long tplane = (long)nmx * nyh, ti = (long)myf * nmx + m;   // table element (k, myf, m)
   const int nz = g.nz;
   const long plane = (long)g.ny * sp;                 // complex elements per z plane
   const long base = (long)my * sp + m;                // complex index in plane 0
   double *S = spec + part;                            // element (k, base): S[2 * (base + k plane)]
-  // forward elimination, k = 1 .. nz-1
-  double pp = 0.0;
+  const double xy = xrt[m] + yrt[my];
+  // forward elimination, k = 1 .. nz-1 (z = 1/(b + xy - a dprev))
+  double dprev = 0.0, pp = 0.0;
   for (int k0 = 1; k0 <= nz - 1; k0 += TD) {
-    double v[TD], z[TD];
+    double v[TD];
 #pragma unroll
     for (int r = 0; r < TD; r++) {
       const int k = k0 + r;
       v[r] = k <= nz - 1 ? S[2 * (base + (long)(k - 1) * plane)] : 0.0;
-      z[r] = k <= nz - 1 ? tz[(long)(k - 1) * tplane + ti] : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < TD; r++) {
       const int k = k0 + r;
       if (k > nz - 1) break;
-      const double w = k == 1 ? v[r] * z[r] : (v[r] - a[k] * pp) * z[r];
-      S[2 * (base + (long)(k - 1) * plane)] = w;
+      const long q = base + (long)(k - 1) * plane;
+      double z;
+      if (k == 1) z = 1.0 / (b[1] + xy);
+      else z = 1.0 / ((b[k] + xy) - a[k] * dprev);
+      dprev = cc[k] * z;
+      dtri[q] = dprev;  // both parts store the same value; each reads back its own store
+      const double w = k == 1 ? v[r] * z : (v[r] - a[k] * pp) * z;
+      S[2 * q] = w;
       pp = w;
     }
   }
   {  // last row (singular kx = ky = 0 guard)
     const long q = base + (long)(nz - 1) * plane;
-    const double z = zl[ti];
+    const double z = (b[nz] + xy) - a[nz] * dprev;
     const double v = S[2 * q];
     pp = z != 0.0 ? (v - a[nz] * pp) / z : 0.0;
     S[2 * q] = pp;
@@ -448,15 +425,17 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
 #pragma unroll
     for (int r = 0; r < TD; r++) {
       const int k = k0 - r;
-      v[r] = k >= 1 ? S[2 * (base + (long)(k - 1) * plane)] : 0.0;
-      d[r] = k >= 1 ? td[(long)(k - 1) * tplane + ti] : 0.0;
+      const long q = base + (long)(k - 1) * plane;
+      v[r] = k >= 1 ? S[2 * q] : 0.0;
+      d[r] = k >= 1 ? dtri[q] : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < TD; r++) {
       const int k = k0 - r;
       if (k < 1) break;
+      const long q = base + (long)(k - 1) * plane;
       const double w = v[r] - d[r] * pp;
-      S[2 * (base + (long)(k - 1) * plane)] = w;
+      S[2 * q] = w;
       pp = w;
     }
   }
@@ -540,11 +519,7 @@ int ipg_poisson_setup(ipg_ctx *c) {
   P.sp = (P.nmx + 7) / 8 * 8;
   size_t nspec = (size_t)P.sp * g.ny * g.nz;
   IPG_HIP(hipMalloc(&P.spec, nspec * sizeof(double2)));
-  const int nyh = g.ny / 2 + 1;
-  const size_t ntab = (size_t)P.nmx * nyh;
-  IPG_HIP(hipMalloc(&P.tz, ntab * g.nz * sizeof(double)));
-  IPG_HIP(hipMalloc(&P.td, ntab * g.nz * sizeof(double)));
-  IPG_HIP(hipMalloc(&P.zl, ntab * sizeof(double)));
+  IPG_HIP(hipMalloc(&P.dtri, nspec * sizeof(double)));
   // src/initsolver.f90:17-30
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi, pi = std::acos(-1.0);
@@ -574,11 +549,6 @@ int ipg_poisson_setup(ipg_ctx *c) {
     return 0;
   };
   if (up(&P.a, a) || up(&P.b, b) || up(&P.c, cc) || up(&P.xrt, xrt) || up(&P.yrt, yrt)) return 1;
-  const int ntab_threads = P.nmx * nyh;
-  hipLaunchKernelGGL(k_thomas_tables, dim3((ntab_threads + 255) / 256), dim3(256), 0, c->stream, g.nz, P.nmx, nyh, P.a, P.b,
-                     P.c, P.xrt, P.yrt, P.tz, P.td, P.zl);
-  IPG_HIP(hipGetLastError());
-  IPG_HIP(hipStreamSynchronize(c->stream));
   auto twid = [&](double2 **d, int n) -> int {
     std::vector<double2> h(n);
     for (int t = 0; t < n; t++) {
@@ -595,7 +565,7 @@ int ipg_poisson_setup(ipg_ctx *c) {
 
 int ipg_poisson_free(ipg_ctx *c) {
   Poisson &P = c->ps2;
-  void *ptrs[] = {P.spec, P.tz, P.td, P.zl, P.a, P.b, P.c, P.xrt, P.yrt, P.twx, P.twy};
+  void *ptrs[] = {P.spec, P.dtri, P.a, P.b, P.c, P.xrt, P.yrt, P.twx, P.twy};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   P = Poisson{};
   return 0;
@@ -635,8 +605,8 @@ int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter) {
   }
   ipg_mark(c, "fft_y_fwd");
   const long nthomas = 2L * P.nmx * g.ny;
-  hipLaunchKernelGGL(k_thomas, dim3((unsigned)((nthomas + 255) / 256)), dim3(256), 0, c->stream, g, (double *)P.spec, P.sp,
-                     P.nmx, P.a, P.tz, P.td, P.zl);
+  hipLaunchKernelGGL(k_thomas, dim3((unsigned)((nthomas + 255) / 256)), dim3(256), 0, c->stream, g, (double *)P.spec, P.dtri, P.sp, P.nmx,
+                     P.a, P.b, P.c, P.xrt, P.yrt);
   ipg_mark(c, "thomas");
   if (specialised(g.ny)) {
 #define CALLY(n) launch_y<n>(g, P.spec, P.sp, P.nmx, P.twy, +1, c->stream)
