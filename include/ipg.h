@@ -120,6 +120,12 @@ int ipg_substep(ipg_ctx *ctx, int rkiter);            /* src/main.f90:268-414, w
 int ipg_chkdt(ipg_ctx *ctx, double *dtmax);           /* src/chkdt.f90:6 */
 int ipg_chkdiv(ipg_ctx *ctx, double *divtot, double *divmax); /* src/chkdiv.f90:7 */
 
+/* ---- output ---------------------------------------------------------- */
+/* post3d quantities (src/output.f90:997-1030, vorticity src/post.f90:15-61) at
+ * the cell centres, interior only, i fastest (itot*jtot*ktot values): */
+enum { IPG_POST_PRE = 0, IPG_POST_VEZ, IPG_POST_CON, IPG_POST_VOX, IPG_POST_VOY, IPG_POST_VOZ };
+int ipg_post3d(ipg_ctx *ctx, int quantity, double *host);
+
 /* ---- seeded inputs (replaces random_number in src/init.f90:287-305,
  * src/initparticles.f90:36-47): U[0,1) = splitmix64(seed*K1 + stream*K2 + n)
  * >> 11 * 2^-53 for n = first .. first+count-1 ---------------------------- */

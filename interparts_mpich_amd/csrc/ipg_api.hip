@@ -449,6 +449,13 @@ int ipg_chkdiv(ipg_ctx *c, double *tot, double *mx) {
   return ipg_launch_chkdiv(c, tot, mx);
 }
 
+int ipg_post3d(ipg_ctx *c, int quantity, double *host) {
+  if (!check_ctx(c) || !host) return 1;
+  if (quantity < 0 || quantity > 5) { ipg_set_error("post3d: quantity must be 0..5"); return 1; }
+  if (!c->solver_ready) { ipg_set_error("post3d: ipg_initsolver has not been called"); return 1; }
+  return ipg_launch_post3d(c, quantity, host);
+}
+
 int ipg_sync(ipg_ctx *c) {
   if (!check_ctx(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));

@@ -365,6 +365,41 @@ __global__ __launch_bounds__(256) void k_chkdiv(Grid g, const double *__restrict
   }
 }
 
+// Cell-centred output quantities of post3d (src/output.f90:997-1030): pnew;
+// w and c averaged to the cell centre; the three vorticity components of
+// src/post.f90:15-61, term by term in the reference's order.  Interior only,
+// i fastest, into out[(i-1) + nx((j-1) + ny(k-1))].
+__global__ __launch_bounds__(BX *BY) void k_post3d(Grid g, int q, const double *__restrict__ U, const double *__restrict__ V,
+                                                   const double *__restrict__ W, const double *__restrict__ P,
+                                                   const double *__restrict__ C, double *__restrict__ out) {
+  const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long c = g.at(i, j, k), sx = g.sx, sz = g.sxy;
+  const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
+  double r;
+  switch (q) {
+    case 0: r = P[c]; break;
+    case 1: r = 0.5 * (W[c] + W[c - sz]); break;
+    case 2: r = 0.5 * (C[c] + C[c - sz]); break;
+    case 3:
+      r = 0.25 * ((W[c + sx] - W[c]) * dyi - (V[c + sz] - V[c]) * dzi + (W[c + sx - sz] - W[c - sz]) * dyi -
+                  (V[c] - V[c - sz]) * dzi + (W[c] - W[c - sx]) * dyi - (V[c - sx + sz] - V[c - sx]) * dzi +
+                  (W[c - sz] - W[c - sx - sz]) * dyi - (V[c - sx] - V[c - sx - sz]) * dzi);
+      break;
+    case 4:
+      r = 0.25 * ((U[c + sz] - U[c]) * dzi - (W[c + 1] - W[c]) * dxi + (U[c] - U[c - sz]) * dzi -
+                  (W[c + 1 - sz] - W[c - sz]) * dxi + (U[c - 1 + sz] - U[c - 1]) * dzi - (W[c] - W[c - 1]) * dxi +
+                  (U[c - 1] - U[c - 1 - sz]) * dzi - (W[c - sz] - W[c - 1 - sz]) * dxi);
+      break;
+    default:
+      r = 0.25 * ((V[c + 1] - V[c]) * dxi - (U[c + sx] - U[c]) * dyi + (V[c + 1 - sx] - V[c - sx]) * dxi -
+                  (U[c] - U[c - sx]) * dyi + (V[c] - V[c - 1]) * dxi - (U[c - 1 + sx] - U[c - 1]) * dyi +
+                  (V[c - sx] - V[c - 1 - sx]) * dxi - (U[c - 1] - U[c - 1 - sx]) * dyi);
+      break;
+  }
+  out[(long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (k - 1))] = r;
+}
+
 }  // namespace
 
 static dim3 cell_grid(const Grid &g) { return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, g.nz); }
@@ -480,5 +515,18 @@ int ipg_launch_chkdiv(ipg_ctx *c, double *tot, double *mx) {
   double a = 0.0, b = 0.0;
   for (int q = 0; q < nb; q++) { a += h[2 * q]; b = h[2 * q + 1] > b ? h[2 * q + 1] : b; }
   *tot = a; *mx = b;
+  return 0;
+}
+
+// One post3d quantity into host memory (interior, i fastest).  The Poisson
+// spectrum, dead between substeps, holds the result on the device.
+int ipg_launch_post3d(ipg_ctx *c, int q, double *host) {
+  const Grid &g = c->g;
+  double *out = (double *)c->ps2.spec;
+  hipLaunchKernelGGL(k_post3d, cell_grid(g), dim3(BX, BY), 0, c->stream, g, q, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W],
+                     c->f[IPG_PNEW], c->f[IPG_C], out);
+  IPG_HIP(hipGetLastError());
+  IPG_HIP(hipMemcpyAsync(host, out, (size_t)g.nx * g.ny * g.nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -147,6 +147,7 @@ int ipg_launch_ibm_gather(ipg_ctx *c, int pass);
 int ipg_launch_ibm_final(ipg_ctx *c);
 int ipg_launch_ibm_totals(ipg_ctx *c, int pass);
 int ipg_ibm_check(ipg_ctx *c);
+int ipg_launch_post3d(ipg_ctx *c, int q, double *host);
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type);
 int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter);
 int ipg_launch_nbflags(ipg_ctx *c);

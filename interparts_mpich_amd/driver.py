@@ -145,6 +145,13 @@ class Simulation:
         check(self.lib.ipg_chkdiv(self.ctx, C.byref(a), C.byref(b)), "chkdiv")
         return a.value, b.value
 
+    def post3d(self, name):
+        """One post3d quantity (pre vez con vox voy voz) as a (ktot, jtot, itot) array."""
+        q = ("pre", "vez", "con", "vox", "voy", "voz").index(name)
+        a = np.empty((self.case.ktot, self.case.jtot, self.case.itot))
+        check(self.lib.ipg_post3d(self.ctx, q, _ptr(a)), f"post3d {name}")
+        return a
+
     def substep(self, rkiter):
         check(self.lib.ipg_substep(self.ctx, rkiter), "substep")
 

@@ -3,9 +3,10 @@
 ! MI355X library.  Inputs: parameters.inp (&PARAMETERS of src/param.f90),
 ! run.nml (&GRIDSIZE: the compile-time sizes of src/gridsize.f90, and
 ! &RUNCONTROL), lagrangianforcepoints2 when np > 0, fld%07d when begin > 0.
-! Outputs: timestep.txt (src/main.f90:425-431); fld%07d (src/loadd.f90
-! format) and, with particles, partstate%07d every ioutfld steps and at the
-! end of the run.
+! Outputs: timestep.txt (src/main.f90:425-431); info_1dstat.out every iout1d
+! steps and the post3d fields name%07d_3d.bin every iout2d steps
+! (src/main.f90:454-472); fld%07d (src/loadd.f90 format) and, with
+! particles, partstate%07d every ioutfld steps and at the end of the run.
 program interpt
   use mod_param
   use mod_common
@@ -27,6 +28,7 @@ program interpt
   use mod_chkdiv
   use mod_loadflds
   use mod_loadpart
+  use mod_output
   implicit none
   real :: dtmax, maxerror, avererror, t0, t1
   integer :: ibmiter, u, last
@@ -119,6 +121,8 @@ program interpt
       call ipg_ok(ipg_set_dt(ctx, dt), 'set_dt')
     end if
     last = istep
+    if (mod(istep, iout1d) == 0) call post1d(time, istep)
+    if (mod(istep, iout2d) == 0) call post3d(istep)
     if (mod(istep, ioutfld) == 0) call dump(last)
   end do
   call ipg_ok(ipg_sync(ctx), 'sync')

@@ -13,7 +13,7 @@ module mod_ipg_capi
             ipg_ibm, ipg_ibm_begin, ipg_eulr2lagr, ipg_lagrforces, ipg_lagr2eulr, ipg_bounduvw, ipg_boundp, &
             ipg_boundscal, ipg_fillps, ipg_solver2d, ipg_correc, ipg_pnew_update, ipg_intgr_over_sphere, &
             ipg_intgr_nwtn_eulr, ipg_substep, ipg_chkdt, ipg_chkdiv, ipg_uniform, ipg_sync, ipg_use_graphs, &
-            ipg_autotune
+            ipg_autotune, ipg_post3d
 
   ! field selectors (enum of include/ipg.h)
   integer(c_int), parameter, public :: IPG_U = 0, IPG_V = 1, IPG_W = 2, IPG_PNEW = 3, IPG_C = 4, &
@@ -26,6 +26,9 @@ module mod_ipg_capi
        IPG_PCOLFX = 18, IPG_PCOLFY = 19, IPG_PCOLFZ = 20, IPG_PCOLTX = 21, IPG_PCOLTY = 22, IPG_PCOLTZ = 23, &
        IPG_PFXLTOT = 24, IPG_PFYLTOT = 25, IPG_PFZLTOT = 26, IPG_PTORQX = 27, IPG_PTORQY = 28, IPG_PTORQZ = 29, &
        IPG_PTORQTHETA = 30, IPG_NPART = 31
+  ! post3d quantities
+  integer(c_int), parameter, public :: IPG_POST_PRE = 0, IPG_POST_VEZ = 1, IPG_POST_CON = 2, IPG_POST_VOX = 3, &
+       IPG_POST_VOY = 4, IPG_POST_VOZ = 5
 
   type, bind(C) :: ipg_params
     integer(c_int) :: itot, jtot, ktot, np, nl
@@ -186,6 +189,12 @@ module mod_ipg_capi
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: divtot, divmax
+    end function
+    integer(c_int) function ipg_post3d(ctx, quantity, host) bind(C, name="ipg_post3d")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: quantity
+      real(c_double), intent(out) :: host(*)
     end function
     integer(c_int) function ipg_uniform(seed, stream, first, count, out) bind(C, name="ipg_uniform")
       import :: c_int, c_long_long, c_double

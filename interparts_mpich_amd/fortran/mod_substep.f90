@@ -473,3 +473,42 @@ contains
     end if
   end subroutine loadpart
 end module mod_loadpart
+
+! src/output.f90: post1d (13-593), whose only file is info_1dstat.out (its
+! profile statistics are computed and not written), and post3d (997-1030):
+! pre, vez, con, vox, voy as name%07d_3d.bin, the interior in Fortran order
+! (decomp_2d_write_every with no skipping), computed on the device.  The
+! phase indicator 'sol' of post3d (src/phase_indicator.f90) is not built.
+module mod_output
+  use mod_param
+  use mod_common
+  implicit none
+contains
+  subroutine post1d(tim, nr)
+    real(8), intent(in) :: tim
+    integer, intent(in) :: nr
+    integer :: u
+    real(8), parameter :: uref = 1., lref = 1., tref = 1.   ! src/init.f90:100-102
+    open(newunit=u, file='info_1dstat.out')
+    write(u, '(8E15.7)') 1. * nr, tim, visc, 1. * ktot, lx / itot, uref, lref, tref
+    close(u)
+  end subroutine post1d
+
+  subroutine post3d(nr)
+    integer, intent(in) :: nr
+    character(len=7) :: istepchar
+    character(len=3), parameter :: names(5) = ['pre', 'vez', 'con', 'vox', 'voy']
+    integer(c_int), parameter :: sel(5) = [IPG_POST_PRE, IPG_POST_VEZ, IPG_POST_CON, IPG_POST_VOX, IPG_POST_VOY]
+    real(8), allocatable :: var(:, :, :)
+    integer :: u, q
+    allocate(var(imax, jmax, kmax))
+    write(istepchar, '(i7.7)') nr
+    do q = 1, 5
+      call ipg_ok(ipg_post3d(ctx, sel(q), var), 'post3d')
+      open(newunit=u, file=names(q)//istepchar//'_3d.bin', access='stream', form='unformatted', status='replace', &
+           action='write')
+      write(u) var
+      close(u)
+    end do
+  end subroutine post3d
+end module mod_output

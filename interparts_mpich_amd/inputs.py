@@ -267,7 +267,7 @@ def markers_for(case, workdir=None):
     return read_markers(path, case.nl)
 
 
-def write_namelist(path, case):
+def write_namelist(path, case, iout2d=100000):
     """parameters.inp in the &PARAMETERS namelist of src/param.f90:66-86."""
     bcu = {0: "BC_NOSLIP", 1: "BC_FREESLIP"}
     bcc = {0: "BC_DIRICHLET", 1: "BC_NEUMANN"}
@@ -281,7 +281,7 @@ def write_namelist(path, case):
         f.write(f"  visc = {case.visc!r}, Pran = {case.pran!r}, beta = {case.beta!r}\n")
         f.write(f"  radius = {case.radius!r}, ratiorho = {case.ratiorho!r}\n")
         f.write(f"  particle_rotation = {'.true.' if case.particle_rotation else '.false.'}\n")
-        f.write(f"  ioutchk = {case.ioutchk}, iout1d = {case.ioutchk}, iout2d = 100000, ioutfld = 100000\n")
+        f.write(f"  ioutchk = {case.ioutchk}, iout1d = {case.ioutchk}, iout2d = {iout2d}, ioutfld = 100000\n")
         f.write("  runmode = 0\n")
         f.write("/\n")
     return path
@@ -299,10 +299,10 @@ def write_run_nml(path, case, nstep, begin=0, use_graphs=True, fused_substep=Tru
     return path
 
 
-def write_case_dir(d, case, nstep, **kw):
+def write_case_dir(d, case, nstep, iout2d=100000, **kw):
     """parameters.inp, run.nml and (with spheres) lagrangianforcepoints2 in d."""
     os.makedirs(d, exist_ok=True)
-    write_namelist(os.path.join(d, "parameters.inp"), case)
+    write_namelist(os.path.join(d, "parameters.inp"), case, iout2d=iout2d)
     write_run_nml(os.path.join(d, "run.nml"), case, nstep, **kw)
     if case.np > 0:
         write_markers(os.path.join(d, "lagrangianforcepoints2"), case)
@@ -368,3 +368,13 @@ def ensemble_cases(n=64, seed=5):
                         lz=nz / nx, np=max(1, min(npart, cap)), diameter_h=d_h, ratiorho=1.5, re_bulk=re,
                         seed=1000 + q, steps=10))
     return out
+
+
+POST3D = ("pre", "vez", "con", "vox", "voy")
+
+
+def read_post3d(path, case):
+    """name%07d_3d.bin of post3d (src/output.f90:1099-1125): the interior,
+    i fastest, as a (ktot, jtot, itot) array."""
+    a = np.fromfile(path, dtype="<f8")
+    return a.reshape(case.ktot, case.jtot, case.itot)

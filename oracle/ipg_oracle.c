@@ -854,6 +854,38 @@ void orc_chkdiv(orc_state *s, double *tot, double *mx) {
   *tot = divtot; *mx = divmax;
 }
 
+/* src/output.f90:997-1030 (post3d) and src/post.f90:15-61 (vorticity): one
+ * cell-centred output quantity, interior only, i fastest.
+ * q: 0 pre, 1 vez, 2 con, 3 vox, 4 voy, 5 voz */
+void orc_post3d(orc_state *s, int q, double *out) {
+  const double *ux = s->u, *uy = s->v, *uz = s->w;
+  const double dxi = s->dxi, dyi = s->dyi, dzi = s->dzi;
+#define A(f, i, j, k) (f)[I1(s, (i), (j), (k))]
+  FOR_INTERIOR(s) {
+    double r;
+    if (q == 0) r = A(s->pnew, i, j, k);
+    else if (q == 1) r = 0.5 * (A(uz, i, j, k) + A(uz, i, j, k - 1));
+    else if (q == 2) r = 0.5 * (A(s->c, i, j, k) + A(s->c, i, j, k - 1));
+    else if (q == 3)
+      r = 0.25 * ((A(uz, i, j + 1, k) - A(uz, i, j, k)) * dyi - (A(uy, i, j, k + 1) - A(uy, i, j, k)) * dzi +
+                  (A(uz, i, j + 1, k - 1) - A(uz, i, j, k - 1)) * dyi - (A(uy, i, j, k) - A(uy, i, j, k - 1)) * dzi +
+                  (A(uz, i, j, k) - A(uz, i, j - 1, k)) * dyi - (A(uy, i, j - 1, k + 1) - A(uy, i, j - 1, k)) * dzi +
+                  (A(uz, i, j, k - 1) - A(uz, i, j - 1, k - 1)) * dyi - (A(uy, i, j - 1, k) - A(uy, i, j - 1, k - 1)) * dzi);
+    else if (q == 4)
+      r = 0.25 * ((A(ux, i, j, k + 1) - A(ux, i, j, k)) * dzi - (A(uz, i + 1, j, k) - A(uz, i, j, k)) * dxi +
+                  (A(ux, i, j, k) - A(ux, i, j, k - 1)) * dzi - (A(uz, i + 1, j, k - 1) - A(uz, i, j, k - 1)) * dxi +
+                  (A(ux, i - 1, j, k + 1) - A(ux, i - 1, j, k)) * dzi - (A(uz, i, j, k) - A(uz, i - 1, j, k)) * dxi +
+                  (A(ux, i - 1, j, k) - A(ux, i - 1, j, k - 1)) * dzi - (A(uz, i, j, k - 1) - A(uz, i - 1, j, k - 1)) * dxi);
+    else
+      r = 0.25 * ((A(uy, i + 1, j, k) - A(uy, i, j, k)) * dxi - (A(ux, i, j + 1, k) - A(ux, i, j, k)) * dyi +
+                  (A(uy, i + 1, j - 1, k) - A(uy, i, j - 1, k)) * dxi - (A(ux, i, j, k) - A(ux, i, j - 1, k)) * dyi +
+                  (A(uy, i, j, k) - A(uy, i - 1, j, k)) * dxi - (A(ux, i - 1, j + 1, k) - A(ux, i - 1, j, k)) * dyi +
+                  (A(uy, i, j - 1, k) - A(uy, i - 1, j - 1, k)) * dxi - (A(ux, i - 1, j, k) - A(ux, i - 1, j - 1, k)) * dyi);
+    out[(size_t)(i - 1) + (size_t)s->nx * ((size_t)(j - 1) + (size_t)s->ny * (size_t)(k - 1))] = r;
+  }
+#undef A
+}
+
 /* src/chkdt.f90:6-271 */
 void orc_chkdt(orc_state *s, double *dtmax) {
   double dxi = s->dxi, dyi = s->dyi, dzi = s->dzi;

@@ -67,6 +67,7 @@ def load():
         "orc_correc": ([vp, ip], None),
         "orc_pnew_update": ([vp], None),
         "orc_chkdiv": ([vp, dp, dp], None),
+        "orc_post3d": ([vp, C.c_int, vp], None),
         "orc_chkdt": ([vp, dp], None),
         "orc_update_nb_flags": ([vp], None),
         "orc_coordsfp": ([vp, vp, vp, d], None),
@@ -170,6 +171,12 @@ class Oracle:
         d = C.c_double()
         self.lib.orc_chkdt(self.s, C.byref(d))
         return d.value
+
+    def post3d(self, name):
+        q = ("pre", "vez", "con", "vox", "voy", "voz").index(name)
+        a = np.empty((self.case.ktot, self.case.jtot, self.case.itot))
+        self.lib.orc_post3d(self.s, q, a.ctypes.data_as(C.c_void_p))
+        return a
 
     def chkdiv(self):
         a, b = C.c_double(), C.c_double()

@@ -10,8 +10,8 @@ import subprocess
 import numpy as np
 import pytest
 
-from interparts_mpich_amd.inputs import (CONFIGS, fld_path, initial_fields, markers_for, place_spheres,
-                                         read_fld, read_partstate, write_case_dir, write_fld)
+from interparts_mpich_amd.inputs import (CONFIGS, POST3D, fld_path, initial_fields, markers_for, place_spheres,
+                                         read_fld, read_partstate, read_post3d, write_case_dir, write_fld)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "interparts_mpich_amd", "fortran", "interpt")
@@ -134,3 +134,27 @@ def test_driver_restart_round_trip(tmp_path):
     qb, _ = read_partstate(os.path.join(keep, "partstate0000002"), case)
     for n in qa:
         assert np.array_equal(qa[n], qb[n]), n
+
+
+@pytest.mark.gpu
+def test_driver_output_files(tmp_path):
+    """post1d / post3d output (src/main.f90:454-472): info_1dstat.out every
+    iout1d steps and name%07d_3d.bin every iout2d steps, against the oracle."""
+    from oracle.pyoracle import Oracle
+    needs_exe()
+    case = dataclasses.replace(CONFIGS["t32"], ioutchk=1)
+    d = write_case_dir(str(tmp_path), case, nstep=2, iout2d=2)
+    assert run(d).returncode == 0
+    o = Oracle(case).initialize(initial_fields(case), place_spheres(case), markers_for(case))
+    for _ in range(2):
+        o.step()
+    f, _ = read_fld(fld_path(d, 2), case)
+    assert np.array_equal(read_post3d(os.path.join(d, "pre0000002_3d.bin"), case), f["PNEW"])
+    for n in POST3D:
+        a = read_post3d(os.path.join(d, f"{n}0000002_3d.bin"), case)
+        b = o.post3d(n)
+        scale = max(float(np.max(np.abs(b))), 1e-300)
+        assert float(np.max(np.abs(a - b))) <= 1e-11 * scale or not np.any(b), n
+    info = np.loadtxt(os.path.join(d, "info_1dstat.out"))
+    assert info.shape == (8,) and info[0] == 2.0 and info[3] == case.ktot
+    assert abs(info[2] - case.visc) <= 1e-7 * case.visc

@@ -234,3 +234,19 @@ def test_chkdt_known_answer():
     for n in ("U", "V", "W"):
         g.put(n, z)
     assert g.chkdt() == 0.37765502929687511
+
+
+def test_post3d_matches_oracle():
+    """post3d output quantities on the device: bit for bit against the oracle on
+    identical fields, and <= 1e-11 after two steps run separately."""
+    case = CONFIGS["t32"]
+    g, o = pair(case)
+    for _ in range(2):
+        g.step(); o.step()
+    for n in ("pre", "vez", "con", "vox", "voy", "voz"):
+        r = rel(g.post3d(n), o.post3d(n)) if np.any(o.post3d(n)) else float(np.max(np.abs(g.post3d(n))))
+        assert r <= 1e-11, (n, r)
+    for n in FLUID:
+        g.put(n, o.get(n))
+    for n in ("pre", "vez", "con", "vox", "voy", "voz"):
+        assert np.array_equal(g.post3d(n), o.post3d(n)), n

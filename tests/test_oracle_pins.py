@@ -372,3 +372,39 @@ def test_marker_count_formula():
     assert nl_formula(64, 1.0, 4 / 64) == 173 == pyoracle.load().orc_nl_formula(64, 1.0, 4 / 64)
     assert nl_formula(256, 1.0, 6 / 256) == 409
     assert nl_formula(512, 1.0, 8 / 512) == 746
+
+
+def test_post3d_closed_forms():
+    """post3d quantities (src/output.f90:997-1030, src/post.f90:15-61) of the
+    oracle on analytic fields: a linear shear u = a z has voy = a and vox =
+    voz = 0; a solid-body rotation u = -W (y - y0), v = W (x - x0) has
+    voz = 2 W; vez and con average w and c to the cell centre."""
+    from interparts_mpich_amd.inputs import Case
+    from oracle.pyoracle import Oracle
+    case = Case(name="p3", itot=12, jtot=10, ktot=8, lx=1.2, ly=1.0, lz=0.8, np=0)
+    o = Oracle(case)
+    nx, ny, nz = case.itot, case.jtot, case.ktot
+    dx, dy, dz = case.lx / nx, case.ly / ny, case.lz / nz
+    k = np.arange(nz + 2)[:, None, None]
+    j = np.arange(ny + 2)[None, :, None]
+    i = np.arange(nx + 2)[None, None, :]
+    shape = (nz + 2, ny + 2, nx + 2)
+    a, W = 0.7, 1.3
+    zc = (k - 0.5) * dz + 0 * i * j
+    o.put("U", a * zc)
+    o.put("V", np.zeros(shape))
+    o.put("W", np.zeros(shape))
+    for name, want in (("voy", a), ("vox", 0.0), ("voz", 0.0)):
+        got = o.post3d(name)
+        assert np.allclose(got, want, rtol=0, atol=1e-12), name
+    # solid-body rotation: u lives on x faces at (i dx, (j-0.5) dy), v on y faces
+    x0, y0 = 0.37, 0.41
+    o.put("U", -W * (((j - 0.5) * dy) - y0) + 0 * i * k)
+    o.put("V", W * ((i * dx) - x0) + 0 * j * k)
+    assert np.allclose(o.post3d("voz"), 2 * W, rtol=0, atol=1e-12)
+    rng = np.random.default_rng(3)
+    w, c, p = rng.standard_normal(shape), rng.standard_normal(shape), rng.standard_normal(shape)
+    o.put("W", w); o.put("C", c); o.put("PNEW", p)
+    assert np.array_equal(o.post3d("vez"), 0.5 * (w[1:-1, 1:-1, 1:-1] + w[:-2, 1:-1, 1:-1]))
+    assert np.array_equal(o.post3d("con"), 0.5 * (c[1:-1, 1:-1, 1:-1] + c[:-2, 1:-1, 1:-1]))
+    assert np.array_equal(o.post3d("pre"), p[1:-1, 1:-1, 1:-1])
