@@ -54,6 +54,10 @@ enum {
   IPG_POMTHETA, IPG_PINTU, IPG_PINTV, IPG_PINTW, IPG_PINTOMX, IPG_PINTOMY, IPG_PINTOMZ, IPG_PCOLFX,
   IPG_PCOLFY, IPG_PCOLFZ, IPG_PCOLTX, IPG_PCOLTY, IPG_PCOLTZ, IPG_PFXLTOT, IPG_PFYLTOT, IPG_PFZLTOT,
   IPG_PTORQX, IPG_PTORQY, IPG_PTORQZ, IPG_PTORQTHETA,
+  /* rkp%fxltot..fzltot, rkp%dudt..dwdt: the sums over one step's substeps that
+   * sumrk3 keeps for outputpart (src/intgr_nwtn_eulr.f90:1586-1669,
+   * src/output.f90:1184-1189) */
+  IPG_PRKFX, IPG_PRKFY, IPG_PRKFZ, IPG_PRKDUDT, IPG_PRKDVDT, IPG_PRKDWDT,
   IPG_NPART
 };
 

@@ -16,7 +16,7 @@ F = {n: i for i, n in enumerate(FIELDS)}
 # particle quantity selectors (include/ipg.h)
 PARTS = ["X", "Y", "Z", "THETA", "PHI", "U", "V", "W", "OMX", "OMY", "OMZ", "OMTHETA", "INTU", "INTV", "INTW",
          "INTOMX", "INTOMY", "INTOMZ", "COLFX", "COLFY", "COLFZ", "COLTX", "COLTY", "COLTZ", "FXLTOT", "FYLTOT",
-         "FZLTOT", "TORQX", "TORQY", "TORQZ", "TORQTHETA"]
+         "FZLTOT", "TORQX", "TORQY", "TORQZ", "TORQTHETA", "RKFX", "RKFY", "RKFZ", "RKDUDT", "RKDVDT", "RKDWDT"]
 PQ = {n: i for i, n in enumerate(PARTS)}
 
 # every symbol include/ipg.h declares

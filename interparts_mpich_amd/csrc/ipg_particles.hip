@@ -327,6 +327,15 @@ __global__ void k_ne_update(Grid g, PartConst pc, PQ P, const double *__restrict
   const double omt = (oy * cos(phi)) - (ox * sin(phi));
   q[IPG_POMTHETA][p] = omt;
   q[IPG_PTHETA][p] = O(3) + rkab * 0.5 * dt * (omt + O(11));
+  // sumrk3 (src/intgr_nwtn_eulr.f90:1586-1669), called on both passes of the
+  // collision loop: the rkiter = 1 reset happens on each pass.
+  const bool first = rkiter == 1;
+  q[IPG_PRKFX][p] = (first ? 0.0 : q[IPG_PRKFX][p]) + (-1.0) * q[IPG_PFXLTOT][p] + (q[IPG_PINTU][p] - O(12)) / dt;
+  q[IPG_PRKFY][p] = (first ? 0.0 : q[IPG_PRKFY][p]) + (-1.0) * q[IPG_PFYLTOT][p] + (q[IPG_PINTV][p] - O(13)) / dt;
+  q[IPG_PRKFZ][p] = (first ? 0.0 : q[IPG_PRKFZ][p]) + (-1.0) * q[IPG_PFZLTOT][p] + (q[IPG_PINTW][p] - O(14)) / dt;
+  q[IPG_PRKDUDT][p] = (first ? 0.0 : q[IPG_PRKDUDT][p]) + (u - O(5)) / dt;
+  q[IPG_PRKDVDT][p] = (first ? 0.0 : q[IPG_PRKDVDT][p]) + (v - O(6)) / dt;
+  q[IPG_PRKDWDT][p] = (first ? 0.0 : q[IPG_PRKDWDT][p]) + (w - O(7)) / dt;
 }
 
 // src/intgr_nwtn_eulr.f90:702-714 periodic wrap, then neighbour flags of

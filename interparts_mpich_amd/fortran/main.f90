@@ -3,10 +3,12 @@
 ! MI355X library.  Inputs: parameters.inp (&PARAMETERS of src/param.f90),
 ! run.nml (&GRIDSIZE: the compile-time sizes of src/gridsize.f90, and
 ! &RUNCONTROL), lagrangianforcepoints2 when np > 0, fld%07d when begin > 0.
-! Outputs: timestep.txt (src/main.f90:425-431); info_1dstat.out every iout1d
-! steps and the post3d fields name%07d_3d.bin every iout2d steps
-! (src/main.f90:454-472); fld%07d (src/loadd.f90 format) and, with
-! particles, partstate%07d every ioutfld steps and at the end of the run.
+! Outputs: timestep.txt (src/main.f90:425-431); info_1dstat.out and partvisu%07d.bin every iout1d
+! steps, partstate%07d every ioutchk steps with particles, the post3d fields
+! name%07d_3d.bin every iout2d steps (src/main.f90:443-472); fld%07d
+! (src/loadd.f90 format) and partstate%07d when the run stops (step nstep,
+! t_end or tmax hours, src/main.f90:498-512).  nstep is the last step index, as
+! in `do istep = begin+1,nstep` (src/main.f90:252).
 program interpt
   use mod_param
   use mod_common
@@ -30,11 +32,13 @@ program interpt
   use mod_loadpart
   use mod_output
   implicit none
-  real :: dtmax, maxerror, avererror, t0, t1
+  real :: dtmax, maxerror, avererror, t1
   integer :: ibmiter, u, last
-  integer(8) :: c0, c1, crate
+  integer(8) :: cs, c0, c1, crate
   integer(c_int) :: kbtune
+  real, parameter :: tmax = 23.5   ! hours, src/main.f90:140
 
+  call system_clock(cs, crate)
   call input_parameters
   call initmpi
   if (np > 0) then
@@ -74,7 +78,7 @@ program interpt
 
   call system_clock(c0, crate)
   last = begin
-  do istep = begin + 1, begin + nstep
+  do istep = begin + 1, nstep
     if (time + dt > t_end) then
       dt = t_end - time
       call ipg_ok(ipg_set_dt(ctx, dt), 'set_dt')
@@ -122,16 +126,23 @@ program interpt
     end if
     last = istep
     if (mod(istep, iout1d) == 0) call post1d(time, istep)
+    if (np > 0 .and. mod(istep, ioutchk) == 0) call loadpart(1, last)
     if (mod(istep, iout2d) == 0) call post3d(istep)
-    if (mod(istep, ioutfld) == 0) call dump(last)
+    ! src/main.f90:498-512: after tmax hours, at step nstep or at t_end, write
+    ! the particle and field state and leave the loop
+    call system_clock(c1)
+    if (real(c1 - cs) / real(crate) > tmax * 3600. .or. istep == nstep .or. time >= t_end) then
+      call ipg_ok(ipg_sync(ctx), 'sync')
+      call system_clock(c1)
+      call dump(last)
+      exit
+    end if
   end do
-  call ipg_ok(ipg_sync(ctx), 'sync')
-  call system_clock(c1)
-  t0 = 0.
   t1 = real(c1 - c0) / real(crate)
-  if (nstep > 0) write(6, '(A,I0,A,F10.3,A,F12.3,A)') 'time loop: ', 3 * nstep, ' substeps in ', t1, ' s (', &
-       3 * nstep / max(t1, 1.e-9), ' substeps/s)'
-  if (nstep == 0 .or. mod(last, ioutfld) /= 0) call dump(last)
+  if (last > begin) write(6, '(A,I0,A,F10.3,A,F12.3,A)') 'time loop: ', 3 * (last - begin), ' substeps in ', t1, &
+       ' s (', 3 * (last - begin) / max(t1, 1.e-9), ' substeps/s)'
+  ! no step ran: write the start-up state (the reference writes nothing then)
+  if (last == begin) call dump(last)
   call ipg_ok(ipg_finalize(ctx), 'finalize')
 contains
   subroutine dump(nr)

@@ -25,7 +25,8 @@ module mod_ipg_capi
        IPG_PINTU = 12, IPG_PINTV = 13, IPG_PINTW = 14, IPG_PINTOMX = 15, IPG_PINTOMY = 16, IPG_PINTOMZ = 17, &
        IPG_PCOLFX = 18, IPG_PCOLFY = 19, IPG_PCOLFZ = 20, IPG_PCOLTX = 21, IPG_PCOLTY = 22, IPG_PCOLTZ = 23, &
        IPG_PFXLTOT = 24, IPG_PFYLTOT = 25, IPG_PFZLTOT = 26, IPG_PTORQX = 27, IPG_PTORQY = 28, IPG_PTORQZ = 29, &
-       IPG_PTORQTHETA = 30, IPG_NPART = 31
+       IPG_PTORQTHETA = 30, IPG_PRKFX = 31, IPG_PRKFY = 32, IPG_PRKFZ = 33, IPG_PRKDUDT = 34, IPG_PRKDVDT = 35, &
+       IPG_PRKDWDT = 36, IPG_NPART = 37
   ! post3d quantities
   integer(c_int), parameter, public :: IPG_POST_PRE = 0, IPG_POST_VEZ = 1, IPG_POST_CON = 2, IPG_POST_VOX = 3, &
        IPG_POST_VOY = 4, IPG_POST_VOZ = 5

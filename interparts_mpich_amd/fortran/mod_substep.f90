@@ -474,8 +474,9 @@ contains
   end subroutine loadpart
 end module mod_loadpart
 
-! src/output.f90: post1d (13-593), whose only file is info_1dstat.out (its
-! profile statistics are computed and not written), and post3d (997-1030):
+! src/output.f90: post1d (13-593), whose files are info_1dstat.out and, through
+! outputpart, partvisu%07d.bin (its profile statistics are computed and not
+! written), and post3d (997-1030):
 ! pre, vez, con, vox, voy as name%07d_3d.bin, the interior in Fortran order
 ! (decomp_2d_write_every with no skipping), computed on the device.  The
 ! phase indicator 'sol' of post3d (src/phase_indicator.f90) is not built.
@@ -492,7 +493,33 @@ contains
     open(newunit=u, file='info_1dstat.out')
     write(u, '(8E15.7)') 1. * nr, tim, visc, 1. * ktot, lx / itot, uref, lref, tref
     close(u)
+    call outputpart(nr)
   end subroutine post1d
+
+  ! src/output.f90:1127-1273: partvisu%07d.bin holds fltot(3,np) and then
+  ! accel(3,np), the substep sums rkp%fxltot.. and rkp%dudt.. of the last step,
+  ! spheres in id order, native real(8).
+  subroutine outputpart(nr)
+    integer, intent(in) :: nr
+    character(len=7) :: istepchar
+    real(8), allocatable :: fltot(:, :), accel(:, :), a(:)
+    integer :: u, m
+    integer(c_int), parameter :: sf(3) = [IPG_PRKFX, IPG_PRKFY, IPG_PRKFZ], sa(3) = [IPG_PRKDUDT, IPG_PRKDVDT, IPG_PRKDWDT]
+    allocate(fltot(3, np), accel(3, np), a(max(np, 1)))
+    do m = 1, 3
+      if (np > 0) then
+        call ipg_ok(ipg_particles_get(ctx, sf(m), a), 'outputpart')
+        fltot(m, :) = a(1:np)
+        call ipg_ok(ipg_particles_get(ctx, sa(m), a), 'outputpart')
+        accel(m, :) = a(1:np)
+      end if
+    end do
+    write(istepchar, '(i7.7)') nr
+    open(newunit=u, file='partvisu'//istepchar//'.bin', access='stream', form='unformatted', status='replace', &
+         action='write')
+    write(u) fltot, accel
+    close(u)
+  end subroutine outputpart
 
   subroutine post3d(nr)
     integer, intent(in) :: nr

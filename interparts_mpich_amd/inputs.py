@@ -378,3 +378,11 @@ def read_post3d(path, case):
     i fastest, as a (ktot, jtot, itot) array."""
     a = np.fromfile(path, dtype="<f8")
     return a.reshape(case.ktot, case.jtot, case.itot)
+
+
+def read_partvisu(path, case):
+    """partvisu%07d.bin of outputpart (src/output.f90:1127-1273): fltot and
+    accel, each a (np, 3) array (x, y, z per sphere, spheres in id order)."""
+    a = np.fromfile(path, dtype="<f8")
+    assert a.size == 6 * case.np, (a.size, case.np)
+    return a[:3 * case.np].reshape(case.np, 3), a[3 * case.np:].reshape(case.np, 3)

@@ -69,6 +69,7 @@ typedef struct {
   double *intu, *intv, *intw, *intomx, *intomy, *intomz;
   double *colfx, *colfy, *colfz, *coltx, *colty, *coltz;
   double *fxltot, *fyltot, *fzltot, *torqx, *torqy, *torqz, *torqtheta;
+  double *rkfx, *rkfy, *rkfz, *rkdudt, *rkdvdt, *rkdwdt; /* rkp sums over one step's substeps */
   int *nbf; /* np*9 neighbour flags, nbf[p*9+nb], nb=1..8 */
   /* markers (np*nl) */
   int *nla;
@@ -270,7 +271,8 @@ orc_state *orc_create(int nx, int ny, int nz, double lx, double ly, double lz, d
   double **pa[] = {&s->px, &s->py, &s->pz, &s->ptheta, &s->pphi, &s->pu, &s->pv, &s->pw, &s->pomx, &s->pomy,
                    &s->pomz, &s->pomtheta, &s->intu, &s->intv, &s->intw, &s->intomx, &s->intomy, &s->intomz,
                    &s->colfx, &s->colfy, &s->colfz, &s->coltx, &s->colty, &s->coltz, &s->fxltot, &s->fyltot,
-                   &s->fzltot, &s->torqx, &s->torqy, &s->torqz, &s->torqtheta};
+                   &s->fzltot, &s->torqx, &s->torqy, &s->torqz, &s->torqtheta, &s->rkfx, &s->rkfy, &s->rkfz,
+                   &s->rkdudt, &s->rkdvdt, &s->rkdwdt};
   for (size_t q = 0; q < sizeof(pa) / sizeof(pa[0]); q++) *pa[q] = zalloc(npp);
   s->nbf = (int *)calloc((size_t)npp * NB, sizeof(int));
   s->nla = (int *)calloc(npp, sizeof(int));
@@ -290,7 +292,7 @@ void orc_destroy(orc_state *s) {
                    s->ptheta, s->pphi, s->pu, s->pv, s->pw, s->pomx, s->pomy, s->pomz, s->pomtheta, s->intu,
                    s->intv, s->intw, s->intomx, s->intomy, s->intomz, s->colfx, s->colfy, s->colfz, s->coltx,
                    s->colty, s->coltz, s->fxltot, s->fyltot, s->fzltot, s->torqx, s->torqy, s->torqz,
-                   s->torqtheta, s->xfp, s->yfp, s->zfp, s->ul, s->vl, s->wl, s->dudtl, s->dvdtl, s->dwdtl,
+                   s->torqtheta, s->rkfx, s->rkfy, s->rkfz, s->rkdudt, s->rkdvdt, s->rkdwdt, s->xfp, s->yfp, s->zfp, s->ul, s->vl, s->wl, s->dudtl, s->dvdtl, s->dwdtl,
                    s->fxl, s->fyl, s->fzl, s->thetarc, s->phirc};
   for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); q++) free(all[q]);
   free(s->nbf); free(s->nla); free(s);
@@ -307,7 +309,7 @@ double *orc_particle(orc_state *s, int id) {
   double *t[] = {s->px, s->py, s->pz, s->ptheta, s->pphi, s->pu, s->pv, s->pw, s->pomx, s->pomy, s->pomz,
                  s->pomtheta, s->intu, s->intv, s->intw, s->intomx, s->intomy, s->intomz, s->colfx, s->colfy,
                  s->colfz, s->coltx, s->colty, s->coltz, s->fxltot, s->fyltot, s->fzltot, s->torqx, s->torqy,
-                 s->torqz, s->torqtheta};
+                 s->torqz, s->torqtheta, s->rkfx, s->rkfy, s->rkfz, s->rkdudt, s->rkdvdt, s->rkdwdt};
   if (id < 0 || id >= (int)(sizeof(t) / sizeof(t[0]))) return NULL;
   return t[id];
 }
@@ -1228,6 +1230,21 @@ void orc_intgr_nwtn_eulr(orc_state *s, int rkiter) {
       s->pphi[p] = oph[p] + 0.5 * rkab * dt * (s->pomz[p] + oomz[p]);
       s->pomtheta[p] = (s->pomy[p] * cos(s->pphi[p])) - (s->pomx[p] * sin(s->pphi[p]));
       s->ptheta[p] = oth[p] + rkab * 0.5 * dt * (s->pomtheta[p] + oomt[p]);
+    }
+    /* sumrk3 (672, src/intgr_nwtn_eulr.f90:1586-1669): called inside the
+     * two-pass loop, so the rkiter = 1 reset happens on both passes and the
+     * other substeps add both passes. */
+    for (int p = 0; p < np; p++) {
+      if (rkiter == 1) {
+        s->rkfx[p] = 0.0; s->rkfy[p] = 0.0; s->rkfz[p] = 0.0;
+        s->rkdudt[p] = 0.0; s->rkdvdt[p] = 0.0; s->rkdwdt[p] = 0.0;
+      }
+      s->rkfx[p] = s->rkfx[p] + (-1.0) * s->fxltot[p] + (s->intu[p] - ointu[p]) / dt;
+      s->rkfy[p] = s->rkfy[p] + (-1.0) * s->fyltot[p] + (s->intv[p] - ointv[p]) / dt;
+      s->rkfz[p] = s->rkfz[p] + (-1.0) * s->fzltot[p] + (s->intw[p] - ointw[p]) / dt;
+      s->rkdudt[p] = s->rkdudt[p] + (s->pu[p] - ou[p]) / dt;
+      s->rkdvdt[p] = s->rkdvdt[p] + (s->pv[p] - ov[p]) / dt;
+      s->rkdwdt[p] = s->rkdwdt[p] + (s->pw[p] - ow[p]) / dt;
     }
   }
   /* periodic correction (702-714) */

@@ -19,7 +19,7 @@ FIELDS = ["U", "V", "W", "PNEW", "C", "DUDT", "DVDT", "DWDT", "DCDT", "DURK", "D
           "DUDTF", "DVDTF", "DWDTF", "DUDTOLD", "DVDTOLD", "DWDTOLD"]
 PARTS = ["X", "Y", "Z", "THETA", "PHI", "U", "V", "W", "OMX", "OMY", "OMZ", "OMTHETA", "INTU", "INTV", "INTW",
          "INTOMX", "INTOMY", "INTOMZ", "COLFX", "COLFY", "COLFZ", "COLTX", "COLTY", "COLTZ", "FXLTOT", "FYLTOT",
-         "FZLTOT", "TORQX", "TORQY", "TORQZ", "TORQTHETA"]
+         "FZLTOT", "TORQX", "TORQY", "TORQZ", "TORQTHETA", "RKFX", "RKFY", "RKFZ", "RKDUDT", "RKDVDT", "RKDWDT"]
 MARKERS = ["XFP", "YFP", "ZFP", "UL", "VL", "WL", "DUDTL", "DVDTL", "DWDTL", "FXL", "FYL", "FZL"]
 
 _lib = None

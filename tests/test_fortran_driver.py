@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 from interparts_mpich_amd.inputs import (CONFIGS, POST3D, fld_path, initial_fields, markers_for, place_spheres,
-                                         read_fld, read_partstate, read_post3d, write_case_dir, write_fld)
+                                         read_fld, read_partstate, read_partvisu, read_post3d, write_case_dir,
+                                         write_fld)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "interparts_mpich_amd", "fortran", "interpt")
@@ -138,8 +139,9 @@ def test_driver_restart_round_trip(tmp_path):
 
 @pytest.mark.gpu
 def test_driver_output_files(tmp_path):
-    """post1d / post3d output (src/main.f90:454-472): info_1dstat.out every
-    iout1d steps and name%07d_3d.bin every iout2d steps, against the oracle."""
+    """post1d / post3d output (src/main.f90:454-472): info_1dstat.out and
+    partvisu%07d.bin (outputpart) every iout1d steps and name%07d_3d.bin every
+    iout2d steps, against the oracle."""
     from oracle.pyoracle import Oracle
     needs_exe()
     case = dataclasses.replace(CONFIGS["t32"], ioutchk=1)
@@ -155,6 +157,13 @@ def test_driver_output_files(tmp_path):
         b = o.post3d(n)
         scale = max(float(np.max(np.abs(b))), 1e-300)
         assert float(np.max(np.abs(a - b))) <= 1e-11 * scale or not np.any(b), n
+    fltot, accel = read_partvisu(os.path.join(d, "partvisu0000002.bin"), case)
+    for k, ax in enumerate("XYZ"):
+        b = o.pget("RKF" + ax)
+        assert float(np.max(np.abs(fltot[:, k] - b))) <= 1e-10 * max(float(np.max(np.abs(b))), 1.0), ax
+    for k, ax in enumerate("UVW"):
+        b = o.pget("RKD" + ax + "DT")
+        assert float(np.max(np.abs(accel[:, k] - b))) <= 1e-10 * max(float(np.max(np.abs(b))), 1.0), ax
     info = np.loadtxt(os.path.join(d, "info_1dstat.out"))
     assert info.shape == (8,) and info[0] == 2.0 and info[3] == case.ktot
     assert abs(info[2] - case.visc) <= 1e-7 * case.visc

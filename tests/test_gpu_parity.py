@@ -250,3 +250,27 @@ def test_post3d_matches_oracle():
         g.put(n, o.get(n))
     for n in ("pre", "vez", "con", "vox", "voy", "voz"):
         assert np.array_equal(g.post3d(n), o.post3d(n)), n
+
+
+def sum_tolerance(o, dt):
+    """Scale for the substep sums of sumrk3: each term is a difference of
+    sphere integrals or velocities over dt, so rounding is bounded relative to
+    the size of the terms, not of the sum."""
+    fl = max(float(np.max(np.abs(o.pget(n)))) for n in ("INTU", "INTV", "INTW")) / dt + \
+        max(float(np.max(np.abs(o.pget(n)))) for n in ("FXLTOT", "FYLTOT", "FZLTOT"))
+    acc = max(float(np.max(np.abs(o.pget(n)))) for n in ("U", "V", "W")) / dt
+    return 5.0 * fl, 5.0 * acc
+
+
+@pytest.mark.parametrize("name", ["t32", "c1_64"])
+def test_substep_sums_match_oracle(name):
+    """rkp%fxltot.. and rkp%dudt.. (sumrk3, src/intgr_nwtn_eulr.f90:1586-1669)
+    after two steps, to 1e-12 of the size of the summed terms."""
+    g, o = pair(CONFIGS[name])
+    for _ in range(2):
+        g.step(); o.step()
+    sf, sa = sum_tolerance(o, o.dt)
+    for n, scale in (("RKFX", sf), ("RKFY", sf), ("RKFZ", sf), ("RKDUDT", sa), ("RKDVDT", sa), ("RKDWDT", sa)):
+        err = float(np.max(np.abs(g.pget(n) - o.pget(n))))
+        print(n, err / scale, float(np.max(np.abs(o.pget(n)))))
+        assert err <= 1e-12 * scale, (n, err, scale)

@@ -327,6 +327,34 @@ def test_free_fall_closed_form():
         np.testing.assert_allclose(o.pget("Z"), z0 + 0.5 * g * t * t, rtol=1e-12)
 
 
+def test_substep_sums_closed_form():
+    """sumrk3 (src/intgr_nwtn_eulr.f90:1586-1669) is called on both passes of
+    the collision loop (iter = -1..itermax-1 = 0..1, lines 149-152, 672) and
+    resets at rkiter = 1 on each: a sphere in fluid at rest with a fixed IBM
+    force f sees, per call, du/dt = -f/(V rho_r) + rkab g (1 - 1/rho_r) and
+    fltot = -f, summed with multiplicities 1, 2, 2 over the three substeps."""
+    case = Case(name="ff", itot=16, jtot=16, ktot=32, np=1, diameter_h=4.0, ratiorho=1.5, perturb=0.0)
+    o = Oracle(case).initialize([np.zeros((34, 18, 18))] * 5, (np.array([0.5]), np.array([0.5]), np.array([0.6])),
+                                markers_for(case))
+    o.set_dt(1e-3)
+    f = np.array([2e-4])
+    vr = o.scalar(2) * case.ratiorho
+    g = -9.81 * (1.0 - 1.0 / case.ratiorho)
+    rkab = (8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0)
+    for step in range(2):
+        for rk in (1, 2, 3):
+            o.pput("FZLTOT", f)
+            o.pput("FXLTOT", -f)
+            o.intgr_nwtn_eulr(rk)
+        mult = (1.0, 2.0, 2.0)
+        np.testing.assert_allclose(o.pget("RKFZ"), -5.0 * f, rtol=1e-13)
+        np.testing.assert_allclose(o.pget("RKFX"), 5.0 * f, rtol=1e-13)
+        assert o.pget("RKFY")[0] == 0.0
+        want_w = sum(m * (-f[0] / vr + a * g) for m, a in zip(mult, rkab))
+        np.testing.assert_allclose(o.pget("RKDWDT"), want_w, rtol=1e-9)
+        np.testing.assert_allclose(o.pget("RKDUDT"), sum(m * (f[0] / vr) for m in mult), rtol=1e-9)
+
+
 def test_collision_spring_force_and_multiplicity():
     """Two spheres at rest overlapping by delta: the normal spring force
     -kn*delta*n is added once per neighbour-table copy, i.e. 9 times at one
