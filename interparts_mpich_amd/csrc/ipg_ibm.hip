@@ -126,9 +126,9 @@ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl
   }
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    c.wx[q] = comp == 0 ? roma_kernel(((double)c.i0 + (double)q) - cx) : roma_kernel((((double)c.i0 + (double)q) - 0.5) - cx);
-    c.wy[q] = comp == 1 ? roma_kernel(((double)c.j0 + (double)q) - cy) : roma_kernel((((double)c.j0 + (double)q) - 0.5) - cy);
-    c.wz[q] = comp == 2 ? roma_kernel(((double)c.k0 + (double)q) - cz) : roma_kernel((((double)c.k0 + (double)q) - 0.5) - cz);
+    c.wx[q] = roma_kernel(comp == 0 ? ((double)c.i0 + (double)q) - cx : (((double)c.i0 + (double)q) - 0.5) - cx);
+    c.wy[q] = roma_kernel(comp == 1 ? ((double)c.j0 + (double)q) - cy : (((double)c.j0 + (double)q) - 0.5) - cy);
+    c.wz[q] = roma_kernel(comp == 2 ? ((double)c.k0 + (double)q) - cz : (((double)c.k0 + (double)q) - 0.5) - cz);
   }
   c.i0 += sxw;
   c.j0 += syw;
@@ -241,9 +241,13 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
   const Box b = sphere_box(A, xp, yp, zp);
   const int nbox = b.bx * b.by * b.bz, bxy = b.bx * b.by;
+  const int nflag = (nbox + 31) >> 5, nflagmax = (A.bmx * A.bmy * A.bmz + 31) >> 5;
   unsigned *flag = (unsigned *)(box + A.bmx * A.bmy * A.bmz);
-  const int nflag = (nbox + 31) >> 5;
-  unsigned *taken = flag + nflag;  // MODE 2: cells a lower-numbered sphere owns
+  unsigned *taken = flag + nflagmax;  // MODE 2: cells a lower-numbered sphere owns
+  // field offset of a box cell = rowoff[row] + icol[li], row = lj + by lk:
+  // the periodic wrap and the index arithmetic once per row and column
+  int *rowoff = (int *)(taken + nflagmax);
+  int *icol = rowoff + A.bmy * A.bmz;
   double *F = A.fld[comp];
   double *fl = A.fl + comp * M;
   // the previous pass's forces: a buffer no workgroup of this launch writes,
@@ -273,6 +277,11 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     flag[t] = 0u;
     if (MODE == 2) taken[t] = 0u;
   }
+  for (int t = threadIdx.x; t < b.by * b.bz; t += blockDim.x) {
+    const int lk = t / b.by, lj = t - lk * b.by;
+    rowoff[t] = (int)(g.sx * wrapany(b.bj0 + lj, g.ny) + g.sxy * (b.bk0 + lk));
+  }
+  for (int t = threadIdx.x; t < b.bx; t += blockDim.x) icol[t] = wrapany(b.bi0 + t, g.nx) - 1;
   if (MODE >= 1)
     for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
   __syncthreads();
@@ -286,10 +295,13 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     if (!own(q, c, e0, nzin, ub)) continue;
     const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
     const double fc = MODE >= 1 ? flp[m] * c0 : 0.0;
-    for (int k = 0; k < nzin; k++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
       for (int j = 0; j < 3; j++)
 #pragma unroll
         for (int i = 0; i < 3; i++) {
+          if (k >= nzin) continue;
           const int e = e0 + i + b.bx * j + bxy * k;
           atomicOr(&flag[e >> 5], 1u << (e & 31));
           if (MODE >= 1) unsafeAtomicAdd(&box[e], fc * c.wx[i] * c.wy[j] * c.wz[k]);
@@ -329,7 +341,9 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
       CompStencil c;
       if (!neighbour_stencil(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c)) continue;
       const double fc = flp[mq] * c0;
+#pragma unroll
       for (int k = 0; k < 3; k++)
+#pragma unroll
         for (int j = 0; j < 3; j++)
 #pragma unroll
           for (int i = 0; i < 3; i++) {
@@ -342,16 +356,23 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   }
   __syncthreads();
   auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
-  auto cell_of = [&](int t) {
-    const int lk = t / bxy, r = t - lk * bxy, lj = r / b.bx, li = r - lj * b.bx;
-    return g.at(wrapany(b.bi0 + li, g.nx), wrapany(b.bj0 + lj, g.ny), b.bk0 + lk);
+  // cell t = li + bx row of the box walks by IBM_THREADS per step: (li, row)
+  // advance by (rb, qb) with a carry instead of a division per cell
+  const int qb = IBM_THREADS / b.bx, rb = IBM_THREADS - qb * b.bx;
+  int row = (int)threadIdx.x / b.bx, li = (int)threadIdx.x - row * b.bx;
+  auto advance = [&]() {
+    li += rb;
+    row += qb;
+    if (li >= b.bx) { li -= b.bx; row++; }
   };
   if (MODE == 2) {
     // dudt = dudtf = fl(dudt + S) (src/main.f90:332-334), once per cell
-    for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
-      if (!flagged(t) || ((taken[t >> 5] >> (t & 31)) & 1u)) continue;
-      const long cc = cell_of(t);
-      F[cc] = F[cc] + box[t];
+    for (int t = threadIdx.x; t < nbox; t += IBM_THREADS) {
+      if (flagged(t) && !((taken[t >> 5] >> (t & 31)) & 1u)) {
+        const long cc = (long)rowoff[row] + icol[li];
+        F[cc] = F[cc] + box[t];
+      }
+      advance();
     }
     return;
   }
@@ -359,20 +380,21 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   // chunks of 8 cells per thread so that the loads of a chunk are all in
   // flight together
   constexpr int CH = 8;
-  for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * (int)blockDim.x) {
+  for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * IBM_THREADS) {
     double v[CH];
 #pragma unroll
     for (int u = 0; u < CH; u++) {
-      const int t = t0 + u * (int)blockDim.x;
+      const int t = t0 + u * IBM_THREADS;
       v[u] = 0.0;
       if (t < nbox && flagged(t)) {
-        v[u] = F[cell_of(t)];
+        v[u] = F[(long)rowoff[row] + icol[li]];
         if (MODE == 1) v[u] = v[u] + box[t];
       }
+      advance();
     }
 #pragma unroll
     for (int u = 0; u < CH; u++) {
-      const int t = t0 + u * (int)blockDim.x;
+      const int t = t0 + u * IBM_THREADS;
       if (t < nbox) box[t] = v[u];
     }
   }
@@ -385,7 +407,9 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     if (!own(q, c, e0, nzin, ub)) continue;
     const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
     double acc = 0.0;
+#pragma unroll
     for (int k = 0; k < 3; k++)
+#pragma unroll
       for (int j = 0; j < 3; j++)
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -399,11 +423,13 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     // the part of this pass's spread that falls on the k1 plane
     if (nzin < 3) {
       const double fc = fnew * c0;
-      for (int k = nzin; k < 3; k++)
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
         for (int j = 0; j < 3; j++)
 #pragma unroll
           for (int i = 0; i < 3; i++)
-            unsafeAtomicAdd(&A.Kspread[(long)(wrapany(c.i0 + i, g.nx) - 1) + (long)g.nx * (wrapany(c.j0 + j, g.ny) - 1)],
+            if (k >= nzin) unsafeAtomicAdd(&A.Kspread[(long)(wrapany(c.i0 + i, g.nx) - 1) + (long)g.nx * (wrapany(c.j0 + j, g.ny) - 1)],
                             fc * c.wx[i] * c.wy[j] * c.wz[k]);
     }
   }
@@ -444,6 +470,13 @@ __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, doub
   }
 }
 
+// LDS of one k_ibm workgroup: the box, its two cell masks, the row offsets
+// and the wrapped column indices.
+size_t ibm_lds_bytes(int bmx, int bmy, int bmz) {
+  const size_t cells = (size_t)bmx * bmy * bmz;
+  return cells * sizeof(double) + 2 * ((cells + 31) / 32) * sizeof(unsigned) + ((size_t)bmy * bmz + bmx) * sizeof(int);
+}
+
 }  // namespace
 
 // LDS of the per-sphere box (one velocity component at a time) and its two
@@ -452,9 +485,11 @@ __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, doub
 int ipg_ibm_setup(ipg_ctx *c) {
   const double r2 = 2.0 * c->pc.radfp;
   const int bmx = (int)std::ceil(r2 * c->g.dxi) + 7, bmy = (int)std::ceil(r2 * c->g.dyi) + 7;
-  const size_t cells = (size_t)bmx * bmy * ((int)std::ceil(r2 * c->g.dzi) + 7);
-  const size_t lds = cells * sizeof(double) + 2 * ((cells + 31) / 32) * sizeof(unsigned);
+  const int bmz = (int)std::ceil(r2 * c->g.dzi) + 7;
+  const size_t lds = ibm_lds_bytes(bmx, bmy, bmz);
   if (lds > 160 * 1024) { ipg_set_error("ibm: sphere box exceeds the 160 KiB LDS of a CU"); return 1; }
+  // the box's row offsets are kept as 32-bit integers
+  if ((double)c->g.sxy * (c->g.nz + 2) >= 2147483647.0) { ipg_set_error("ibm: field larger than 2^31 cells"); return 1; }
   if (bmx > c->g.nx || bmy > c->g.ny) { ipg_set_error("ibm: sphere box wider than the domain"); return 1; }
   if (c->pc.nl > IBM_THREADS * IBM_MPT) { ipg_set_error("ibm: more than 1024 markers per sphere"); return 1; }
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -475,8 +510,7 @@ bool ibm_args(ipg_ctx *c, IbmArgs &A, size_t &lds) {
   A.bmx = (int)std::ceil(r2 * c->g.dxi) + 7;
   A.bmy = (int)std::ceil(r2 * c->g.dyi) + 7;
   A.bmz = (int)std::ceil(r2 * c->g.dzi) + 7;
-  const size_t cells = (size_t)A.bmx * A.bmy * A.bmz;
-  lds = cells * sizeof(double) + 2 * ((cells + 31) / 32) * sizeof(unsigned);
+  lds = ibm_lds_bytes(A.bmx, A.bmy, A.bmz);
   double **q = c->ps.q;
   A.px = q[IPG_PX]; A.py = q[IPG_PY]; A.pz = q[IPG_PZ]; A.pu = q[IPG_PU]; A.pv = q[IPG_PV]; A.pw = q[IPG_PW];
   A.pomx = q[IPG_POMX]; A.pomy = q[IPG_POMY]; A.pomz = q[IPG_POMZ]; A.pphi = q[IPG_PPHI];

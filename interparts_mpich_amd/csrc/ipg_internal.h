@@ -114,11 +114,14 @@ __device__ __forceinline__ int nint_f(double x) { return (int)llround(x); }
 __device__ __forceinline__ double sqd(double x) { return x * x; }
 
 // src/kernel.f90:7-25, Roma et al. 3-point regularized delta times h.
+// Both branches share one square root: the argument is selected first, so a
+// wavefront whose lanes fall in both branches evaluates one sqrt, not two.
 __device__ __forceinline__ double roma_kernel(double r) {
-  double ar = fabs(r);
-  if (ar > 1.5) return 0.0;
-  if (ar > 0.5) return (1.0 / 6.0) * ((5.0 - 3.0 * ar) - sqrt(-(3.0 * sqd(1.0 - ar)) + 1.0));
-  return (1.0 / 3.0) * (1.0 + sqrt(-(3.0 * sqd(r)) + 1.0));
+  const double ar = fabs(r);
+  const bool outer = ar > 0.5;
+  const double s = sqrt(-(3.0 * sqd(outer ? 1.0 - ar : r)) + 1.0);
+  const double w = outer ? (1.0 / 6.0) * ((5.0 - 3.0 * ar) - s) : (1.0 / 3.0) * (1.0 + s);
+  return ar > 1.5 ? 0.0 : w;
 }
 
 // periodic wrap of an index in [-1, n+2] onto 1..n
