@@ -43,12 +43,23 @@ __device__ __forceinline__ void image_shift(const Grid &g, int nbrecv, double &x
 }
 
 // types: 1 volume, 2 int u dV, 3 int r x u dV, 23 both 2 and 3 from one sweep
-__global__ __launch_bounds__(512) void k_sphere_int(Grid g, PartConst pc, const double *__restrict__ U,
-                                                    const double *__restrict__ V, const double *__restrict__ W, PQ P,
-                                                    const int *__restrict__ nbf, int type) {
+//
+// One block per sphere.  Each (image, component) sweep over the bounding box
+// of src/intgr_over_sphere.f90:117-130 runs in slices of SPH_SLICE cells and
+// two phases per slice: every cell is classified with a few operations (no
+// corner inside: skipped, as its weight sum1/sum2 is exactly 0; all corners
+// inside the inner radius: weight 1), and the cut cells are queued in LDS; the
+// queue is then worked through densely, so that the eight corner square roots
+// are evaluated only by the lanes that need them.
+constexpr int SPH_THREADS = 512, SPH_SLICE = 16 * SPH_THREADS;
+__global__ __launch_bounds__(SPH_THREADS) void k_sphere_int(Grid g, PartConst pc, const double *__restrict__ U,
+                                                            const double *__restrict__ V, const double *__restrict__ W, PQ P,
+                                                            const int *__restrict__ nbf, int type) {
   const int p = blockIdx.x;
   __shared__ double acc[IPG_NB][6];
-  __shared__ double red[6][8];
+  __shared__ double red[6][SPH_THREADS / 64];
+  __shared__ int cut[SPH_SLICE];
+  __shared__ int ncut;
   if (threadIdx.x < IPG_NB * 6) acc[threadIdx.x / 6][threadIdx.x % 6] = 0.0;
   __syncthreads();
   const double R = pc.radius, dVeul = pc.dVeul;
@@ -72,42 +83,23 @@ __global__ __launch_bounds__(512) void k_sphere_int(Grid g, PartConst pc, const 
     const int ni = ihigh - ilow + 1, nj = jhigh - jlow + 1, nk = khigh - klow + 1;
     if (ni > 0 && nj > 0 && nk > 0) {
       const int nij = ni * nj, tot = nij * nk;
+      // a thread's cell t = ii + ni (jj + nj kk) advances by SPH_THREADS per
+      // step: (ii, jj, kk) by (si, sj, sk) with carries
+      const int sk = SPH_THREADS / nij, r0 = SPH_THREADS - sk * nij, sj = r0 / ni, si = r0 - sj * ni;
       for (int comp = 0; comp < 3; comp++) {
         const double *F = comp == 0 ? U : (comp == 1 ? V : W);
-        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-          const int kk = t / nij, r = t - kk * nij, jj = r / ni;
-          const int i = ilow + (r - jj * ni), j = jlow + jj, k = klow + kk;
-          double zmin, zplus, ymin, yplus, xmin, xplus;
+        // corner coordinates of cell (i, j, k) for this component's grid
+        auto corners = [&](int i, int j, int k, double &xmin, double &xplus, double &ymin, double &yplus, double &zmin,
+                           double &zplus) {
           if (comp == 2) { zmin = ((double)k - 0.5) * dz; zplus = ((double)k + 0.5) * dz; }
           else { zmin = (double)(k - 1) * dz; zplus = (double)k * dz; }
           if (comp == 1) { ymin = 0.0 + ((double)j - 0.5) * dy; yplus = 0.0 + ((double)j + 0.5) * dy; }
           else { ymin = 0.0 + (double)(j - 1) * dy; yplus = 0.0 + (double)j * dy; }
           if (comp == 0) { xmin = 0.0 + ((double)i - 0.5) * dx; xplus = 0.0 + ((double)i + 0.5) * dx; }
           else { xmin = 0.0 + (double)(i - 1) * dx; xplus = 0.0 + (double)i * dx; }
-          const double dzp2 = sqd(zplus - Z), dyp2 = sqd(yplus - Y), dxp2 = sqd(xplus - X);
-          const double dist2 = dxp2 + dyp2 + dzp2;
-          // A cell whose nearest point lies beyond R has every corner outside:
-          // its weight sum1/sum2 is exactly 0 and it adds nothing (skipped).
-          {
-            const double ex = xmin > X ? xmin - X : (xplus < X ? X - xplus : 0.0);
-            const double ey = ymin > Y ? ymin - Y : (yplus < Y ? Y - yplus : 0.0);
-            const double ez = zmin > Z ? zmin - Z : (zplus < Z ? Z - zplus : 0.0);
-            if (ex * ex + ey * ey + ez * ez > R * R * (1.0 + 1e-9)) continue;
-          }
-          const double fval = F[g.at(i, j, k)];
-          bool full = dist2 < radin2;
-          double wfrac = 1.0;
-          if (!full) {
-            const double dxm2 = sqd(xmin - X), dym2 = sqd(ymin - Y), dzm2 = sqd(zmin - Z);
-            double sg[8];
-            sg[0] = R - sqrt(dxm2 + dym2 + dzm2); sg[1] = R - sqrt(dxm2 + dyp2 + dzm2);
-            sg[2] = R - sqrt(dxp2 + dyp2 + dzm2); sg[3] = R - sqrt(dxp2 + dym2 + dzm2);
-            sg[4] = R - sqrt(dxm2 + dym2 + dzp2); sg[5] = R - sqrt(dxm2 + dyp2 + dzp2);
-            sg[6] = R - sqrt(dxp2 + dyp2 + dzp2); sg[7] = R - sqrt(dxp2 + dym2 + dzp2);
-            double s1 = 0.0, s2 = 0.0;
-            for (int cp = 0; cp < 8; cp++) { if (sg[cp] > 0.0) s1 = s1 + sg[cp]; s2 = s2 + fabs(sg[cp]); }
-            wfrac = s1 / s2;
-          }
+        };
+        // src/intgr_over_sphere.f90:131-300 contribution of one cell
+        auto add = [&](int i, int j, int k, double fval, bool full, double wfrac) {
           if (t1) s[comp] += full ? dVeul : wfrac * dVeul;
           if (t2) s[comp] += full ? fval * dVeul : fval * wfrac * dVeul;
           if (t3) {
@@ -125,6 +117,53 @@ __global__ __launch_bounds__(512) void k_sphere_int(Grid g, PartConst pc, const 
               s[4] -= full ? rx * fval * dVeul : rx * fval * wfrac * dVeul;
             }
           }
+        };
+        int t = threadIdx.x;
+        int kk = t / nij, ii = t - kk * nij, jj = ii / ni;
+        ii -= jj * ni;
+        for (int t0 = 0; t0 < tot; t0 += SPH_SLICE) {
+          if (threadIdx.x == 0) ncut = 0;
+          __syncthreads();
+          const int tend = min(t0 + SPH_SLICE, tot);
+          for (; t < tend; t += SPH_THREADS) {
+            const int i = ilow + ii, j = jlow + jj, k = klow + kk;
+            double xmin, xplus, ymin, yplus, zmin, zplus;
+            corners(i, j, k, xmin, xplus, ymin, yplus, zmin, zplus);
+            // a cell whose nearest point lies beyond R has every corner
+            // outside: its weight is exactly 0 and it adds nothing
+            const double ex = xmin > X ? xmin - X : (xplus < X ? X - xplus : 0.0);
+            const double ey = ymin > Y ? ymin - Y : (yplus < Y ? Y - yplus : 0.0);
+            const double ez = zmin > Z ? zmin - Z : (zplus < Z ? Z - zplus : 0.0);
+            if (ex * ex + ey * ey + ez * ez <= R * R * (1.0 + 1e-9)) {
+              const double dist2 = sqd(xplus - X) + sqd(yplus - Y) + sqd(zplus - Z);
+              if (dist2 < radin2) add(i, j, k, F[g.at(i, j, k)], true, 1.0);
+              else cut[atomicAdd(&ncut, 1)] = t;
+            }
+            ii += si; jj += sj; kk += sk;
+            if (ii >= ni) { ii -= ni; jj++; }
+            if (jj >= nj) { jj -= nj; kk++; }
+          }
+          __syncthreads();
+          const int n = ncut;
+          for (int q = threadIdx.x; q < n; q += SPH_THREADS) {
+            const int tc = cut[q];
+            const int ck = tc / nij, cr = tc - ck * nij, cj = cr / ni;
+            const int i = ilow + (cr - cj * ni), j = jlow + cj, k = klow + ck;
+            double xmin, xplus, ymin, yplus, zmin, zplus;
+            corners(i, j, k, xmin, xplus, ymin, yplus, zmin, zplus);
+            const double dzp2 = sqd(zplus - Z), dyp2 = sqd(yplus - Y), dxp2 = sqd(xplus - X);
+            const double dxm2 = sqd(xmin - X), dym2 = sqd(ymin - Y), dzm2 = sqd(zmin - Z);
+            double sg[8];
+            sg[0] = R - sqrt(dxm2 + dym2 + dzm2); sg[1] = R - sqrt(dxm2 + dyp2 + dzm2);
+            sg[2] = R - sqrt(dxp2 + dyp2 + dzm2); sg[3] = R - sqrt(dxp2 + dym2 + dzm2);
+            sg[4] = R - sqrt(dxm2 + dym2 + dzp2); sg[5] = R - sqrt(dxm2 + dyp2 + dzp2);
+            sg[6] = R - sqrt(dxp2 + dyp2 + dzp2); sg[7] = R - sqrt(dxp2 + dym2 + dzp2);
+            double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int cp = 0; cp < 8; cp++) { if (sg[cp] > 0.0) s1 = s1 + sg[cp]; s2 = s2 + fabs(sg[cp]); }
+            add(i, j, k, F[g.at(i, j, k)], false, s1 / s2);
+          }
+          __syncthreads();
         }
       }
     }
@@ -222,48 +261,78 @@ constexpr int MAXCAND = 48;
 // lubrication and contact force (the reference's 9x multiplicity at one rank).
 // One wavefront per sphere screens all partners (with a margin), keeping the
 // candidates in ascending order; lane 0 then runs the reference's ordered scan
-// over the candidates only.
-__global__ __launch_bounds__(256) void k_collide(Grid g, PartConst pc, PQ P, double *cfx, double *cfy, double *cfz,
-                                                 int *err) {
-  const int lane = threadIdx.x & 63;
-  const int p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+// over the candidates only.  Sphere centres are staged through LDS in chunks
+// (one latency per chunk rather than one per 64 partners), and the candidates'
+// state is gathered into LDS by the wave's lanes before the serial scan.
+constexpr int COL_WAVES = 4, COL_CHUNK = 1024;
+// LDS written by some lanes of a wavefront is visible to its other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc, PQ P, double *cfx, double *cfy, double *cfz,
+                                                            int *err) {
+  __shared__ double cx[COL_CHUNK], cy[COL_CHUNK], cz[COL_CHUNK];
+  __shared__ int cand[COL_WAVES][MAXCAND];
+  __shared__ double cd[COL_WAVES][MAXCAND][9];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * COL_WAVES + wv;
   const int np = pc.np;
-  if (p >= np) return;
   const double *X = P.q[IPG_PX], *Y = P.q[IPG_PY], *Zq = P.q[IPG_PZ], *Uq = P.q[IPG_PU], *Vq = P.q[IPG_PV], *Wq = P.q[IPG_PW];
-  const double xp = X[p], yp = Y[p], zp = Zq[p];
+  const bool live = p < np;
+  const double xp = live ? X[p] : 0.0, yp = live ? Y[p] : 0.0, zp = live ? Zq[p] : 0.0;
   const double R = pc.radius, lx = g.lx, ly = g.ly;
   const double reach = 2.0 * R * (1.0 + 0.025) + 1e-9 * (lx + ly + R);
-  int cand[MAXCAND];
   int nc = 0;
-  for (int q0 = 0; q0 < np; q0 += 64) {
-    const int q = q0 + lane;
-    bool hit = false;
-    if (q < np && q != p) {
-      const double dz = Zq[q] - zp;
-      double dx = X[q] - xp, dy = Y[q] - yp;
-      dx = dx - lx * rint(dx / lx);
-      dy = dy - ly * rint(dy / ly);
-      hit = dx * dx + dy * dy + dz * dz <= reach * reach;
-    }
-    unsigned long long mask = __ballot(hit);
-    while (mask) {
-      const int b = __ffsll((long long)mask) - 1;
-      if (nc < MAXCAND) cand[nc] = q0 + b;
-      nc++;
-      mask &= mask - 1;
+  for (int c0 = 0; c0 < np; c0 += COL_CHUNK) {
+    const int n = min(COL_CHUNK, np - c0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x) { cx[t] = X[c0 + t]; cy[t] = Y[c0 + t]; cz[t] = Zq[c0 + t]; }
+    __syncthreads();
+    if (!live) continue;
+    for (int q0 = 0; q0 < n; q0 += 64) {
+      const int ql = q0 + lane, q = c0 + ql;
+      bool hit = false;
+      if (ql < n && q != p) {
+        const double dz = cz[ql] - zp;
+        double dx = cx[ql] - xp, dy = cy[ql] - yp;
+        dx = dx - lx * rint(dx / lx);
+        dy = dy - ly * rint(dy / ly);
+        hit = dx * dx + dy * dy + dz * dz <= reach * reach;
+      }
+      unsigned long long mask = __ballot(hit);
+      if (hit) {
+        const int slot = nc + (int)__popcll(mask & ((1ull << lane) - 1ull));
+        if (slot < MAXCAND) cand[wv][slot] = q;
+      }
+      nc += (int)__popcll(mask);
     }
   }
+  if (!live) return;
+  wave_lds_sync();
+  if (nc > MAXCAND) {
+    if (lane == 0) atomicOr(err, 1);
+    nc = MAXCAND;
+  }
+  // the candidates' state, one lane per candidate
+  for (int c = lane; c < nc; c += 64) {
+    const int q = cand[wv][c];
+    double *e = cd[wv][c];
+    e[0] = X[q]; e[1] = Y[q]; e[2] = Zq[q]; e[3] = Uq[q]; e[4] = Vq[q]; e[5] = Wq[q];
+    e[6] = P.q[IPG_POMX][q]; e[7] = P.q[IPG_POMY][q]; e[8] = P.q[IPG_POMZ][q];
+  }
+  wave_lds_sync();
   if (lane != 0) return;
-  if (nc > MAXCAND) { atomicOr(err, 1); nc = MAXCAND; }
   const double up = Uq[p], vp = Vq[p], wp = Wq[p];
   const double omx = P.q[IPG_POMX][p], omy = P.q[IPG_POMY][p], omz = P.q[IPG_POMZ][p];
   double fx = 0.0, fy = 0.0, fz = 0.0;
   for (int nb = 0; nb < IPG_NB; nb++)
     for (int c = 0; c < nc; c++) {
-      const int q = cand[c];
-      double xq = X[q], yq = Y[q];
+      const double *e = cd[wv][c];
+      double xq = e[0], yq = e[1];
       nb_copy(g, nb, xq, yq);
-      const double deltaz = Zq[q] - zp;
+      const double deltaz = e[2] - zp;
       const double ax0 = xq - xp, ax1 = xq - lx - xp, ax2 = xq + lx - xp;
       double deltax = ax0;
       if (fabs(ax1) < fabs(deltax)) deltax = ax1;
@@ -275,10 +344,10 @@ __global__ __launch_bounds__(256) void k_collide(Grid g, PartConst pc, PQ P, dou
       const double dist = sqrt(sqd(deltax) + sqd(deltay) + sqd(deltaz));
       const double nx = deltax / dist, ny = deltay / dist, nz = deltaz / dist;
       const double eps = (dist - 2.0 * R) / R;
-      if ((eps < 0.025) && (eps > 0.0)) lubrication(pc, up, vp, wp, false, nx, ny, nz, eps, Uq[q], Vq[q], Wq[q], fx, fy, fz);
+      if ((eps < 0.025) && (eps > 0.0)) lubrication(pc, up, vp, wp, false, nx, ny, nz, eps, e[3], e[4], e[5], fx, fy, fz);
       if (dist < (2.0 * R))
-        collision(pc, up, vp, wp, omx, omy, omz, false, dist, deltax, deltay, deltaz, Uq[q], Vq[q], Wq[q],
-                  P.q[IPG_POMX][q], P.q[IPG_POMY][q], P.q[IPG_POMZ][q], fx, fy, fz);
+        collision(pc, up, vp, wp, omx, omy, omz, false, dist, deltax, deltay, deltaz, e[3], e[4], e[5], e[6], e[7], e[8],
+                  fx, fy, fz);
     }
   for (int wq = 0; wq < 2; wq++) {  // walls q = botw, topw (src/intgr_nwtn_eulr.f90:504-603)
     const double deltax = 0.0, deltay = 0.0, deltaz = (double)wq * g.lz - zp;
@@ -401,7 +470,7 @@ int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter) {
                      c->ps.nbf, 23);
   ipg_mark(c, "sphere_int");
   for (int iter = 0; iter <= 1; iter++) {
-    hipLaunchKernelGGL(k_collide, dim3((np + 3) / 4), dim3(256), 0, c->stream, c->g, c->pc, P, cf, cf + np, cf + 2 * np, err);
+    hipLaunchKernelGGL(k_collide, dim3((np + COL_WAVES - 1) / COL_WAVES), dim3(COL_WAVES * 64), 0, c->stream, c->g, c->pc, P, cf, cf + np, cf + 2 * np, err);
     ipg_mark(c, "collide");
     hipLaunchKernelGGL(k_ne_update, dim3(nb), dim3(256), 0, c->stream, c->g, c->pc, P, c->op, cf, cf + np, cf + 2 * np,
                        c->dscal, rkiter);

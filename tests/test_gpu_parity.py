@@ -274,3 +274,37 @@ def test_substep_sums_match_oracle(name):
         err = float(np.max(np.abs(g.pget(n) - o.pget(n))))
         print(n, err / scale, float(np.max(np.abs(o.pget(n)))))
         assert err <= 1e-12 * scale, (n, err, scale)
+
+
+def test_collisions_and_lubrication_match_oracle():
+    """Contacts the seeded placements never produce: an overlapping pair, a
+    pair inside the lubrication range (eps = 0.01 < 0.025), an overlap across
+    the periodic x boundary, a sphere in the wall lubrication range
+    (eps = 0.05 < 0.075) and one pressed into the bottom wall, all moving.
+    One intgr_nwtn_eulr call (src/intgr_nwtn_eulr.f90:338-667): collision
+    forces and the updated sphere state against the oracle."""
+    case = Case(name="col", itot=32, jtot=32, ktot=32, np=8, diameter_h=6.0, ratiorho=1.5, re_bulk=1000.0, seed=21,
+                perturb=0.0)
+    R = case.radius
+    xs = np.array([0.30, 0.30 + 1.98 * R, 0.30, 0.30, 0.01, 1.01 - 1.99 * R, 0.70, 0.60])
+    ys = np.array([0.50, 0.50, 0.20, 0.20 + 2.02 * R, 0.80, 0.80, 0.70, 0.30])
+    zs = np.array([0.50, 0.50, 0.25, 0.25, 0.60, 0.60, 1.05 * R, 0.50])
+    fields, markers = initial_fields(case), markers_for(case)
+    print(f"[case col] np={case.np} R={R}", flush=True)
+    g = Simulation(case).initialize(fields, (xs, ys, zs), markers)
+    o = Oracle(case).initialize(fields, (xs, ys, zs), markers)
+    rng = np.random.default_rng(3)
+    for n in ("U", "V", "W"):
+        a = 0.1 * rng.standard_normal(case.np)
+        g.pput(n, a); o.pput(n, a)
+    z = zs.copy()
+    z[7] = 0.995 * R
+    g.pput("Z", z); o.pput("Z", z)
+    for rk in (1, 2):
+        g.intgr_nwtn_eulr(rk); o.intgr_nwtn_eulr(rk)
+        cf = o.pget("COLFX")
+        assert np.count_nonzero(o.pget("COLFZ")) >= 2 and np.count_nonzero(cf) >= 4
+        for n in ("COLFX", "COLFY", "COLFZ", "U", "V", "W", "X", "Y", "Z"):
+            r = rel(g.pget(n), o.pget(n))
+            print(rk, n, r)
+            assert r <= 1e-12, (n, r)
