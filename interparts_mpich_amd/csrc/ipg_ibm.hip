@@ -156,36 +156,38 @@ __device__ __forceinline__ bool overlap(int a0, int an, int b0, int bn) { return
 
 // Neighbour images of each sphere, once per substep: every (q, sx, sy) with
 // q's box, shifted by (sx nx, sy ny), overlapping p's box; q != p (a box
-// never meets its own image: it is narrower than the domain).  One wavefront
-// per sphere, entries in increasing q.
+// never meets its own image: it is narrower than the domain).  One block per
+// sphere, one thread per partner; the order of the entries is immaterial, as
+// the spreads they feed are sums.
 __global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int *nbc, int *err) {
   const Grid &g = A.g;
-  const int lane = threadIdx.x & 63;
-  const int p = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-  if (p >= A.np) return;
+  const int p = blockIdx.x;
+  __shared__ int count;
+  if (threadIdx.x == 0) count = 0;
+  __syncthreads();
   const Box bp = sphere_box(A, A.px[p], A.py[p], A.pz[p]);
-  int count = 0;
-  for (int q0 = 0; q0 < A.np; q0 += 64) {
-    const int q = q0 + lane;
-    Box bq{};
-    bool zok = false;
-    if (q < A.np && q != p) {
-      bq = sphere_box(A, A.px[q], A.py[q], A.pz[q]);
-      zok = overlap(bp.bk0, bp.bz, bq.bk0, bq.bz);
+  for (int q = threadIdx.x; q < A.np; q += blockDim.x) {
+    if (q == p) continue;
+    const Box bq = sphere_box(A, A.px[q], A.py[q], A.pz[q]);
+    if (!overlap(bp.bk0, bp.bz, bq.bk0, bq.bz)) continue;
+    unsigned mx = 0u, my = 0u;
+#pragma unroll
+    for (int sh = -2; sh <= 2; sh++) {
+      if (overlap(bp.bi0, bp.bx, bq.bi0 + sh * g.nx, bq.bx)) mx |= 1u << (sh + 2);
+      if (overlap(bp.bj0, bp.by, bq.bj0 + sh * g.ny, bq.by)) my |= 1u << (sh + 2);
     }
-    for (int sx = -2; sx <= 2; sx++)
-      for (int sy = -2; sy <= 2; sy++) {
-        const bool hit = zok && overlap(bp.bi0, bp.bx, bq.bi0 + sx * g.nx, bq.bx) &&
-                         overlap(bp.bj0, bp.by, bq.bj0 + sy * g.ny, bq.by);
-        const unsigned long long mask = __ballot(hit);
-        if (hit) {
-          const int slot = count + (int)__popcll(mask & ((1ull << lane) - 1ull));
-          if (slot < IBM_MAXNB) nbl[(long)p * IBM_MAXNB + slot] = q * 25 + (sx + 2) * 5 + (sy + 2);
+    const int n = __popc(mx) * __popc(my);
+    if (n == 0) continue;
+    int slot = atomicAdd(&count, n);
+    for (int sx = 0; sx < 5; sx++)
+      for (int sy = 0; sy < 5; sy++)
+        if (((mx >> sx) & 1u) && ((my >> sy) & 1u)) {
+          if (slot < IBM_MAXNB) nbl[(long)p * IBM_MAXNB + slot] = q * 25 + sx * 5 + sy;
+          slot++;
         }
-        count += (int)__popcll(mask);
-      }
   }
-  if (lane == 0) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
     nbc[p] = min(count, IBM_MAXNB);
     if (count > IBM_MAXNB) atomicOr(err, 1);
   }
@@ -530,7 +532,7 @@ int ipg_launch_ibm_begin(ipg_ctx *c) {
   if (c->pc.np <= 0) return 0;
   IbmArgs A; size_t lds;
   if (!ibm_args(c, A, lds)) return 1;
-  hipLaunchKernelGGL(k_ibm_neighbours, dim3((A.np + 3) / 4), dim3(256), 0, c->stream, A, c->nbl, c->nbc, c->ibm_err);
+  hipLaunchKernelGGL(k_ibm_neighbours, dim3(A.np), dim3(256), 0, c->stream, A, c->nbl, c->nbc, c->ibm_err);
   ipg_mark(c, "ibm_neighbours");
   IPG_HIP(hipGetLastError());
   return 0;

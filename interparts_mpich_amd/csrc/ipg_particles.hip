@@ -14,17 +14,22 @@ namespace {
 
 constexpr int NOP = 24;  // saved quantities (src/intgr_nwtn_eulr.f90:80-103)
 // op layout: x y z theta phi u v w omx omy omz omtheta intu intv intw intomx intomy intomz colfx colfy colfz coltx colty coltz
-__constant__ int c_op_src[NOP] = {IPG_PX, IPG_PY, IPG_PZ, IPG_PTHETA, IPG_PPHI, IPG_PU, IPG_PV, IPG_PW,
-                                  IPG_POMX, IPG_POMY, IPG_POMZ, IPG_POMTHETA, IPG_PINTU, IPG_PINTV, IPG_PINTW,
-                                  IPG_PINTOMX, IPG_PINTOMY, IPG_PINTOMZ, IPG_PCOLFX, IPG_PCOLFY, IPG_PCOLFZ,
-                                  IPG_PCOLTX, IPG_PCOLTY, IPG_PCOLTZ};
+constexpr int kOpSrc[NOP] = {IPG_PX, IPG_PY, IPG_PZ, IPG_PTHETA, IPG_PPHI, IPG_PU, IPG_PV, IPG_PW,
+                             IPG_POMX, IPG_POMY, IPG_POMZ, IPG_POMTHETA, IPG_PINTU, IPG_PINTV, IPG_PINTW,
+                             IPG_PINTOMX, IPG_PINTOMY, IPG_PINTOMZ, IPG_PCOLFX, IPG_PCOLFY, IPG_PCOLFZ,
+                             IPG_PCOLTX, IPG_PCOLTY, IPG_PCOLTZ};
 
 struct PQ { double *q[IPG_NPART]; };
 
+// all 24 loads of a sphere in flight together
 __global__ void k_ne_save(PQ P, double *op, int np) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  for (int s = 0; s < NOP; s++) op[(long)s * np + p] = P.q[c_op_src[s]][p];
+  double v[NOP];
+#pragma unroll
+  for (int s = 0; s < NOP; s++) v[s] = P.q[kOpSrc[s]][p];
+#pragma unroll
+  for (int s = 0; s < NOP; s++) op[(long)s * np + p] = v[s];
 }
 
 // image centre of neighbour slot nbrecv (src/intgr_over_sphere.f90:83-109)
@@ -219,6 +224,15 @@ __device__ __forceinline__ void lubrication(const PartConst &pc, double up, doub
   fx = fx + f1 * nx; fy = fy + f1 * ny; fz = fz + f1 * nz;
 }
 
+// the force one lubrication call adds (fx = fx + term keeps its rounding)
+__device__ __forceinline__ void lubrication_term(const PartConst &pc, double up, double vp, double wp, bool wall, double nx,
+                                                 double ny, double nz, double eps, double un, double vn, double wn,
+                                                 double &tx, double &ty, double &tz) {
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  lubrication(pc, up, vp, wp, wall, nx, ny, nz, eps, un, vn, wn, fx, fy, fz);
+  tx = fx; ty = fy; tz = fz;  // 0 + t = t exactly
+}
+
 // src/collisions.f90:10-220, normal spring-dashpot force only (lines 199-201)
 __device__ __forceinline__ void collision(const PartConst &pc, double up, double vp, double wp, double omx, double omy, double omz,
                                           bool wall, double dist, double deltax, double deltay, double deltaz, double un,
@@ -276,6 +290,8 @@ __global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc
   __shared__ double cx[COL_CHUNK], cy[COL_CHUNK], cz[COL_CHUNK];
   __shared__ int cand[COL_WAVES][MAXCAND];
   __shared__ double cd[COL_WAVES][MAXCAND][9];
+  __shared__ double terms[COL_WAVES][64][6];
+  __shared__ int tuse[COL_WAVES][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * COL_WAVES + wv;
   const int np = pc.np;
@@ -323,12 +339,18 @@ __global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc
     e[6] = P.q[IPG_POMX][q]; e[7] = P.q[IPG_POMY][q]; e[8] = P.q[IPG_POMZ][q];
   }
   wave_lds_sync();
-  if (lane != 0) return;
   const double up = Uq[p], vp = Vq[p], wp = Wq[p];
   const double omx = P.q[IPG_POMX][p], omy = P.q[IPG_POMY][p], omz = P.q[IPG_POMZ][p];
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  for (int nb = 0; nb < IPG_NB; nb++)
-    for (int c = 0; c < nc; c++) {
+  // the terms of the ordered scan (nb outer, candidate inner) one per lane,
+  // 64 at a time; lane 0 then adds them in that order, as the reference does
+  const int nterm = IPG_NB * nc;
+  for (int l0 = 0; l0 < nterm; l0 += 64) {
+    const int l = l0 + lane;
+    double lt[3] = {0.0, 0.0, 0.0}, ct[3] = {0.0, 0.0, 0.0};
+    int use = 0;
+    if (l < nterm) {
+      const int nb = l / nc, c = l - nb * nc;
       const double *e = cd[wv][c];
       double xq = e[0], yq = e[1];
       nb_copy(g, nb, xq, yq);
@@ -344,11 +366,32 @@ __global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc
       const double dist = sqrt(sqd(deltax) + sqd(deltay) + sqd(deltaz));
       const double nx = deltax / dist, ny = deltay / dist, nz = deltaz / dist;
       const double eps = (dist - 2.0 * R) / R;
-      if ((eps < 0.025) && (eps > 0.0)) lubrication(pc, up, vp, wp, false, nx, ny, nz, eps, e[3], e[4], e[5], fx, fy, fz);
-      if (dist < (2.0 * R))
+      if ((eps < 0.025) && (eps > 0.0)) {
+        lubrication_term(pc, up, vp, wp, false, nx, ny, nz, eps, e[3], e[4], e[5], lt[0], lt[1], lt[2]);
+        use |= 1;
+      }
+      if (dist < (2.0 * R)) {
         collision(pc, up, vp, wp, omx, omy, omz, false, dist, deltax, deltay, deltaz, e[3], e[4], e[5], e[6], e[7], e[8],
-                  fx, fy, fz);
+                  ct[0], ct[1], ct[2]);
+        use |= 2;
+      }
     }
+    terms[wv][lane][0] = lt[0]; terms[wv][lane][1] = lt[1]; terms[wv][lane][2] = lt[2];
+    terms[wv][lane][3] = ct[0]; terms[wv][lane][4] = ct[1]; terms[wv][lane][5] = ct[2];
+    tuse[wv][lane] = use;
+    wave_lds_sync();
+    if (lane == 0) {
+      const int n = min(64, nterm - l0);
+      for (int k = 0; k < n; k++) {
+        const int u = tuse[wv][k];
+        const double *t = terms[wv][k];
+        if (u & 1) { fx = fx + t[0]; fy = fy + t[1]; fz = fz + t[2]; }
+        if (u & 2) { fx = fx + t[3]; fy = fy + t[4]; fz = fz + t[5]; }
+      }
+    }
+    wave_lds_sync();
+  }
+  if (lane != 0) return;
   for (int wq = 0; wq < 2; wq++) {  // walls q = botw, topw (src/intgr_nwtn_eulr.f90:504-603)
     const double deltax = 0.0, deltay = 0.0, deltaz = (double)wq * g.lz - zp;
     const double dist = sqrt(sqd(deltax) + sqd(deltay) + sqd(deltaz));
