@@ -182,18 +182,29 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, const doubl
     const double dti = 1.0 / (rkab * *dtp);
     dtidxi = dti * g.dxi; dtidyi = dti * g.dyi; dtidzi = dti * g.dzi;
   }
-  for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
-    const int b = t / N, i = t % N;
-    const long pr = pair0 + b;
-    double2 v = make_double2(0.0, 0.0);
-    if (pr < pairs) {
-      const int k = (int)(pr / (g.ny / 2)) + 1, j = 2 * (int)(pr % (g.ny / 2)) + 1;
-      const long c = g.at(i + 1, j, k);
-      if (fused) v = make_double2(rhs_at(g, du, dv, dw, c, i + 1, j, k, dtidxi, dtidyi, dtidzi),
-                                  rhs_at(g, du, dv, dw, c + g.sx, i + 1, j + 1, k, dtidxi, dtidyi, dtidzi));
-      else v = make_double2(p[c], p[c + g.sx]);
+  // the loads of CHK rows' worth of elements in flight before their LDS stores
+  constexpr int IT = (B * N + FFT_THREADS - 1) / FFT_THREADS, CHK = IT < 4 ? IT : 4;
+  for (int u0 = 0; u0 < IT; u0 += CHK) {
+    double2 v[CHK];
+#pragma unroll
+    for (int u = 0; u < CHK; u++) {
+      const int t = threadIdx.x + (u0 + u) * FFT_THREADS;
+      const int b = t / N, i = t % N;
+      const long pr = pair0 + b;
+      v[u] = make_double2(0.0, 0.0);
+      if (t < B * N && pr < pairs) {
+        const int k = (int)(pr / (g.ny / 2)) + 1, j = 2 * (int)(pr % (g.ny / 2)) + 1;
+        const long c = g.at(i + 1, j, k);
+        if (fused) v[u] = make_double2(rhs_at(g, du, dv, dw, c, i + 1, j, k, dtidxi, dtidyi, dtidzi),
+                                       rhs_at(g, du, dv, dw, c + g.sx, i + 1, j + 1, k, dtidxi, dtidyi, dtidzi));
+        else v[u] = make_double2(p[c], p[c + g.sx]);
+      }
     }
-    buf[b * N + i] = v;
+#pragma unroll
+    for (int u = 0; u < CHK; u++) {
+      const int t = threadIdx.x + (u0 + u) * FFT_THREADS;
+      if (t < B * N) buf[t] = v[u];
+    }
   }
   __syncthreads();
   double2 *x = buf, *y = buf + B * N;
@@ -221,9 +232,19 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_y_t(Grid g, double2 *__rest
   const int tiles = (nmx + B - 1) / B;
   const int k = blockIdx.x / tiles, m0 = (blockIdx.x % tiles) * B;
   double2 *plane = spec + (long)k * g.ny * sp;
-  for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
+  // every load of the tile in flight before the first LDS store
+  constexpr int IT = (B * N + FFT_THREADS - 1) / FFT_THREADS;
+  double2 v[IT];
+#pragma unroll
+  for (int u = 0; u < IT; u++) {
+    const int t = threadIdx.x + u * FFT_THREADS;
     const int jj = t / B, b = t % B;
-    buf[jj * B + b] = (m0 + b < nmx) ? plane[(long)jj * sp + m0 + b] : make_double2(0.0, 0.0);
+    v[u] = (t < B * N && m0 + b < nmx) ? plane[(long)jj * sp + m0 + b] : make_double2(0.0, 0.0);
+  }
+#pragma unroll
+  for (int u = 0; u < IT; u++) {
+    const int t = threadIdx.x + u * FFT_THREADS;
+    if (t < B * N) buf[t] = v[u];
   }
   __syncthreads();
   double2 *x = buf, *y = buf + B * N;
@@ -244,19 +265,32 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, double *__r
   const long pairs = (long)(g.ny / 2) * g.nz;
   const long pair0 = (long)blockIdx.x * B;
   constexpr int half = N / 2;
-  for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
+  // modes m > N/2 are the conjugates of N - m; every load of the block in
+  // flight before the first LDS store
+  constexpr int IT = (B * N + FFT_THREADS - 1) / FFT_THREADS;
+  double2 va[IT], vb[IT];
+#pragma unroll
+  for (int u = 0; u < IT; u++) {
+    const int t = threadIdx.x + u * FFT_THREADS;
     const int b = t / N, m = t % N;
     const long pr = pair0 + b;
-    double2 z = make_double2(0.0, 0.0);
-    if (pr < pairs) {
+    va[u] = make_double2(0.0, 0.0);
+    vb[u] = make_double2(0.0, 0.0);
+    if (t < B * N && pr < pairs) {
       const int k = (int)(pr / (g.ny / 2)), j = 2 * (int)(pr % (g.ny / 2));
-      const double2 *ra = spec + ((long)k * g.ny + j) * sp, *rb = ra + sp;
-      double2 xa, xb;
-      if (m <= half) { xa = ra[m]; xb = rb[m]; }
-      else { xa = ra[N - m]; xb = rb[N - m]; xa.y = -xa.y; xb.y = -xb.y; }
-      z = make_double2(xa.x - xb.y, xa.y + xb.x);
+      const double2 *ra = spec + ((long)k * g.ny + j) * sp;
+      const int mm = m <= half ? m : N - m;
+      va[u] = ra[mm];
+      vb[u] = ra[sp + mm];
     }
-    buf[b * N + m] = z;
+  }
+#pragma unroll
+  for (int u = 0; u < IT; u++) {
+    const int t = threadIdx.x + u * FFT_THREADS;
+    const int m = t % N;
+    double2 xa = va[u], xb = vb[u];
+    if (m > half) { xa.y = -xa.y; xb.y = -xb.y; }
+    if (t < B * N) buf[t] = make_double2(xa.x - xb.y, xa.y + xb.x);
   }
   __syncthreads();
   double2 *x = buf, *y = buf + B * N;
