@@ -141,12 +141,18 @@ __device__ __forceinline__ void fft_stages(double2 *&x, double2 *&y, const doubl
   }
 }
 
-template <int N>
+// sequences per block: the y pass takes tiles of up to 2048 complex values
+// (rows of B x-modes, 128-byte segments at B = 8); the x passes take half as
+// much, so that five blocks fit in a CU's LDS
+template <int N, int LIM = LDS_COMPLEX>
 constexpr int batch_of() {
   int b = 1;
-  while (2 * b * N <= LDS_COMPLEX) b *= 2;
+  while (2 * b * N <= LIM) b *= 2;
   return b;
 }
+constexpr int X_LDS_COMPLEX = LDS_COMPLEX / 2;
+template <int N>
+constexpr int xbatch_of() { return batch_of<N, X_LDS_COMPLEX>(); }
 
 // Right-hand side of the Poisson equation at (i,j,k), src/fillps.f90:43-46,
 // from dudt, dvdt, dwdt whose ghosts have not been filled: the values
@@ -173,7 +179,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, const doubl
                                                              const double *__restrict__ du, const double *__restrict__ dv,
                                                              const double *__restrict__ dw, const double *__restrict__ dtp,
                                                              double rkab) {
-  constexpr int B = batch_of<N>();
+  constexpr int B = xbatch_of<N>();
   __shared__ double2 buf[2 * B * N];
   const long pairs = (long)(g.ny / 2) * g.nz;
   const long pair0 = (long)blockIdx.x * B;
@@ -260,7 +266,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_y_t(Grid g, double2 *__rest
 template <int N>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, double *__restrict__ p, const double2 *__restrict__ spec,
                                                              long sp, const double2 *__restrict__ tw, double scale) {
-  constexpr int B = batch_of<N>();
+  constexpr int B = xbatch_of<N>();
   __shared__ double2 buf[2 * B * N];
   const long pairs = (long)(g.ny / 2) * g.nz;
   const long pair0 = (long)blockIdx.x * B;
@@ -498,14 +504,14 @@ struct XArgs {
 
 template <int N>
 void launch_x_fwd(const XArgs &a, hipStream_t s) {
-  constexpr int B = batch_of<N>();
+  constexpr int B = xbatch_of<N>();
   const long pairs = (long)(a.g.ny / 2) * a.g.nz;
   hipLaunchKernelGGL(k_fft_x_fwd_t<N>, dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g, a.p, a.spec,
                      a.sp, a.tw, a.fused, a.du, a.dv, a.dw, a.dtp, a.rkab);
 }
 template <int N>
 void launch_x_inv(const XArgs &a, hipStream_t s) {
-  constexpr int B = batch_of<N>();
+  constexpr int B = xbatch_of<N>();
   const long pairs = (long)(a.g.ny / 2) * a.g.nz;
   hipLaunchKernelGGL(k_fft_x_inv_t<N>, dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g, a.p, a.spec,
                      a.sp, a.tw, a.scale);
