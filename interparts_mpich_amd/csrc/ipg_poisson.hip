@@ -142,15 +142,15 @@ __device__ __forceinline__ void fft_stages(double2 *&x, double2 *&y, const doubl
 }
 
 // sequences per block: the y pass takes tiles of up to 2048 complex values
-// (rows of B x-modes, 128-byte segments at B = 8); the x passes take half as
-// much, so that five blocks fit in a CU's LDS
+// (rows of B x-modes, 128-byte segments at B = 8); the x passes take a
+// quarter of that (one row pair of 256), so that many blocks share a CU
 template <int N, int LIM = LDS_COMPLEX>
 constexpr int batch_of() {
   int b = 1;
   while (2 * b * N <= LIM) b *= 2;
   return b;
 }
-constexpr int X_LDS_COMPLEX = LDS_COMPLEX / 2;
+constexpr int X_LDS_COMPLEX = LDS_COMPLEX / 4;
 template <int N>
 constexpr int xbatch_of() { return batch_of<N, X_LDS_COMPLEX>(); }
 
