@@ -288,13 +288,19 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
   __syncthreads();
   // flag the own stencil cells; with MODE >= 1 also spread the own forces of
-  // the previous pass into S
+  // the previous pass into S.  The first marker's stencil is kept in
+  // registers for the gather below.
+  CompStencil cs0{};
+  int e00 = 0, nz00 = 0;
+  double ub00 = 0.0;
+  bool have0 = false;
 #pragma unroll
   for (int q = 0; q < IBM_MPT; q++) {
     CompStencil c;
     int e0, nzin;
     double ub;
     if (!own(q, c, e0, nzin, ub)) continue;
+    if (q == 0 && MODE <= 1) { cs0 = c; e00 = e0; nz00 = nzin; ub00 = ub; have0 = true; }
     const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
     const double fc = MODE >= 1 ? flp[m] * c0 : 0.0;
 #pragma unroll
@@ -406,7 +412,12 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     CompStencil c;
     int e0, nzin;
     double ub;
-    if (!own(q, c, e0, nzin, ub)) continue;
+    if (q == 0) {
+      if (!have0) continue;
+      c = cs0; e0 = e00; nzin = nz00; ub = ub00;
+    } else if (!own(q, c, e0, nzin, ub)) {
+      continue;
+    }
     const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
     double acc = 0.0;
 #pragma unroll
