@@ -108,7 +108,9 @@ __device__ __forceinline__ int box_at(const Box &b, int iu, int ju, int k) {
 // and the three weights per direction, in the reference's order
 // (src/interp_spread.f90:76-130).  Only the 9 weights the component uses are
 // evaluated; each is the same expression stencil() evaluates.
+// With WEIGHTS = false only the base cell is found.
 struct CompStencil { int i0, j0, k0; double wx[3], wy[3], wz[3]; };
+template <bool WEIGHTS = true>
 __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl, double zl, int comp, CompStencil &c) {
   const int sxw = (int)floor((xl - 0.5 * g.dx) / g.lx) * g.nx;
   const int syw = (int)floor((yl - 0.5 * g.dy) / g.ly) * g.ny;
@@ -125,7 +127,7 @@ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl
     c.i0 = (int)floor(cx); c.j0 = (int)floor(cy); c.k0 = max(nint_f(cz) - 1, 1);
   }
 #pragma unroll
-  for (int q = 0; q < 3; q++) {
+  for (int q = 0; q < 3 && WEIGHTS; q++) {
     c.wx[q] = roma_kernel(comp == 0 ? ((double)c.i0 + (double)q) - cx : (((double)c.i0 + (double)q) - 0.5) - cx);
     c.wy[q] = roma_kernel(comp == 1 ? ((double)c.j0 + (double)q) - cy : (((double)c.j0 + (double)q) - 0.5) - cy);
     c.wz[q] = roma_kernel(comp == 2 ? ((double)c.k0 + (double)q) - cz : (((double)c.k0 + (double)q) - 0.5) - cz);
@@ -195,6 +197,7 @@ __global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int
 
 // Stencil of marker l of sphere q, seen through the periodic shift (sxs nx,
 // sys ny) in the unwrapped frame of box b; false when it misses the box.
+template <bool WEIGHTS = true>
 __device__ __forceinline__ bool neighbour_stencil(const IbmArgs &A, const Box &b, int comp, int qs, int l, int sxs, int sys,
                                                   CompStencil &c) {
   const Grid &g = A.g;
@@ -207,7 +210,7 @@ __device__ __forceinline__ bool neighbour_stencil(const IbmArgs &A, const Box &b
     return false;
   double xl, yl, zl;
   if (!marker_pos(g, xq, yq, zq, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return false;
-  comp_stencil(g, xl, yl, zl, comp, c);
+  comp_stencil<WEIGHTS>(g, xl, yl, zl, comp, c);
   c.i0 += sxs * g.nx;
   c.j0 += sys * g.ny;
   return !(c.i0 + 2 < b.bi0 || c.i0 >= b.bi0 + b.bx || c.j0 + 2 < b.bj0 || c.j0 >= b.bj0 + b.by ||
@@ -330,8 +333,15 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
       const int code = A.nbl[(long)p * IBM_MAXNB + n];
       const int qs = code / 25, sh = code % 25;
       CompStencil c;
-      if (!neighbour_stencil(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c)) continue;
-      const int slot = atomicAdd(&lcount, 1);
+      const bool keep = neighbour_stencil<false>(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c);
+      // one LDS atomic per wavefront
+      const unsigned long long msk = __ballot(keep);
+      if (!keep) continue;
+      const int lead = __ffsll((long long)msk) - 1, lane = (int)(threadIdx.x & 63);
+      int base = 0;
+      if (lane == lead) base = atomicAdd(&lcount, (int)__popcll(msk));
+      base = __shfl(base, lead, 64);
+      const int slot = base + (int)__popcll(msk & ((1ull << lane) - 1ull));
       if (slot < IBM_MAXNM) A.nml[lbase + slot] = (qs * A.nl + l) * 25 + sh;
     }
     __syncthreads();
