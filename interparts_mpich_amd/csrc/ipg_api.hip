@@ -90,7 +90,8 @@ int substep_launches(ipg_ctx *c, int rkiter) {
   // fillps (src/fillps.f90), computed inside the forward x transform, takes
   // the boundary values at the source: no launch of its own
   if (ipg_launch_poisson_ex(c, rkiter)) return 1;
-  if (ipg_launch_bound(c, BND_P)) return 1;
+  // boundp(p) (src/main.f90:361) is read at the source by correc; p's own
+  // ghosts are filled only when p is read back (ipg_field_get)
   if (ipg_launch_correc_fused(c, rkiter)) return 1;
   if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C)) return 1;
   if (c->pc.np > 0 && ipg_launch_nwtn_eulr(c, rkiter)) return 1;
@@ -183,6 +184,7 @@ int ipg_finalize(ipg_ctx *c) {
 int ipg_field_put(ipg_ctx *c, int field, const double *host) {
   if (!check_ctx(c)) return 1;
   if (field < 0 || field >= IPG_NFIELDS || !host) { ipg_set_error("field_put: bad field"); return 1; }
+  if (field == IPG_P) c->p_ghosts_stale = false;
   const Grid &g = c->g;
   IPG_HIP(hipMemcpy2DAsync(c->f[field] - 1, g.sx * sizeof(double), host, (g.nx + 2) * sizeof(double),
                            (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyHostToDevice, c->stream));
@@ -193,6 +195,11 @@ int ipg_field_put(ipg_ctx *c, int field, const double *host) {
 int ipg_field_get(ipg_ctx *c, int field, double *host) {
   if (!check_ctx(c)) return 1;
   if (field < 0 || field >= IPG_NFIELDS || !host) { ipg_set_error("field_get: bad field"); return 1; }
+  if (field == IPG_P && c->p_ghosts_stale) {
+    // the fused substep leaves p's ghosts to the reader: boundp(p) now
+    if (ipg_launch_bound(c, BND_P)) return 1;
+    c->p_ghosts_stale = false;
+  }
   const Grid &g = c->g;
   IPG_HIP(hipMemcpy2DAsync(host, (g.nx + 2) * sizeof(double), c->f[field] - 1, g.sx * sizeof(double),
                            (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyDeviceToHost, c->stream));
@@ -356,6 +363,7 @@ int ipg_bounduvw(ipg_ctx *c, int which) {
 int ipg_boundp(ipg_ctx *c, int which) {
   if (!check_ctx(c)) return 1;
   if (ipg_launch_bound(c, which == 0 ? BND_PNEW : BND_P)) return 1;
+  if (which != 0) c->p_ghosts_stale = false;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -255,24 +255,34 @@ __global__ __launch_bounds__(BX *BY) void k_fillps(Grid g, const double *__restr
   p[c] = (dwdt[c] - dwdt[c - g.sxy]) * dtidzi + (dvdt[c] - dvdt[c - g.sx]) * dtidyi + (dudt[c] - dudt[c - 1]) * dtidxi;
 }
 
-// src/correc.f90:11-42, fused with src/main.f90:370 (pnew += p) and 375
-// (cnew = dcdt) on the interior; the ghosts are filled by k_bound afterwards.
+// src/correc.f90:11-42.  p is read at i+1, j+1, k+1 through the values
+// boundp (src/bound.f90:82-107) gives its ghosts -- the periodic image in x
+// and y, p(kmax) above the top wall -- so p's ghosts need not be filled.
+// FUSED adds src/main.f90:370 (pnew += p) and 375 (cnew = dcdt) on the
+// interior; the ghosts of u, v, w, pnew, cnew are filled by k_bound after.
+// (Writing those ghosts from the threads of their interior images was
+// measured: the scattered x-ghost writes and the divergent boundary lanes
+// cost as much as the separate ghost-fill launch.)
+template <bool FUSED>
 __global__ __launch_bounds__(BX *BY) void k_correc(Grid g, const double *__restrict__ dudt, const double *__restrict__ dvdt,
                                                    const double *__restrict__ dwdt, const double *__restrict__ p,
                                                    double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
                                                    double *__restrict__ pnew, const double *__restrict__ dcdt,
-                                                   double *__restrict__ cnew, const double *__restrict__ dtp, double rkab,
-                                                   int fuse_pnew) {
+                                                   double *__restrict__ cnew, const double *__restrict__ dtp, double rkab) {
   const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
-  if (i > g.nx || j > g.ny) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx || j > ny) return;
   const double factor = rkab * *dtp;
   const double fi = factor * g.dxi, fj = factor * g.dyi, fk = factor * g.dzi;
   const long c = g.at(i, j, k);
   const double pc = p[c];
-  u[c] = dudt[c] - fi * (p[c + 1] - pc);
-  v[c] = dvdt[c] - fj * (p[c + g.sx] - pc);
-  w[c] = dwdt[c] - fk * (p[c + g.sxy] - pc);
-  if (fuse_pnew) {
+  const double pi1 = p[i == nx ? c - (nx - 1) : c + 1];
+  const double pj1 = p[j == ny ? c - (long)(ny - 1) * g.sx : c + g.sx];
+  const double pk1 = k == nz ? pc : p[c + g.sxy];
+  u[c] = dudt[c] - fi * (pi1 - pc);
+  v[c] = dvdt[c] - fj * (pj1 - pc);
+  w[c] = dwdt[c] - fk * (pk1 - pc);
+  if (FUSED) {
     pnew[c] = pnew[c] + pc;
     cnew[c] = dcdt[c];
   }
@@ -467,9 +477,9 @@ int ipg_launch_fillps(ipg_ctx *c, int rkiter) {
 
 int ipg_launch_correc(ipg_ctx *c, int rkiter) {
   const Grid &g = c->g;
-  hipLaunchKernelGGL(k_correc, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT],
-                     c->f[IPG_P], c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->f[IPG_DCDT], c->f[IPG_C], c->dscal,
-                     c->pc.rkab[rkiter - 1], 0);
+  hipLaunchKernelGGL(k_correc<false>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT],
+                     c->f[IPG_DWDT], c->f[IPG_P], c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->f[IPG_DCDT],
+                     c->f[IPG_C], c->dscal, c->pc.rkab[rkiter - 1]);
   IPG_HIP(hipGetLastError());
   ipg_mark(c, "correc");
   return 0;
@@ -477,9 +487,10 @@ int ipg_launch_correc(ipg_ctx *c, int rkiter) {
 
 int ipg_launch_correc_fused(ipg_ctx *c, int rkiter) {
   const Grid &g = c->g;
-  hipLaunchKernelGGL(k_correc, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT],
-                     c->f[IPG_P], c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->f[IPG_DCDT], c->f[IPG_C], c->dscal,
-                     c->pc.rkab[rkiter - 1], 1);
+  hipLaunchKernelGGL(k_correc<true>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT],
+                     c->f[IPG_DWDT], c->f[IPG_P], c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->f[IPG_DCDT],
+                     c->f[IPG_C], c->dscal, c->pc.rkab[rkiter - 1]);
+  c->p_ghosts_stale = true;
   IPG_HIP(hipGetLastError());
   ipg_mark(c, "correc");
   return 0;

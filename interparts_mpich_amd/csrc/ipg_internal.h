@@ -63,6 +63,7 @@ struct ipg_ctx {
   // IBM
   double *K1[3];                  // dwdtf(.,.,k1) planes (nx*ny), src/interp_spread.f90:100 (no upper clamp)
   bool ibm_lds_set;
+  bool p_ghosts_stale;            // the fused substep did not fill p's ghosts
   // launch shape: z planes per RK-stage workgroup (IPG_TUNE_RK_KB at init; 0 = chosen per grid)
   int tune_rk_kb;
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
