@@ -410,9 +410,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv(Grid g, double *__res
 // recurrence of the pivots and multipliers (both compute and store the same
 // values), which doubles the number of independent lanes over one thread per
 // pair.  Adjacent lanes hold the two parts of one complex value, so
-// every load and store is coalesced.  Planes are read TD at a time ahead of
-// the recurrence, so TD loads per lane are in flight together.
-constexpr int TD = 8;
+// every load and store is coalesced.  With one wavefront per SIMD the sweep is
+// bound by memory latency, so both sweeps are software-pipelined: the next TD
+// planes are loaded while the current TD are eliminated.
+constexpr int TD = 16;
 __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
                                                 const double *__restrict__ a, const double *__restrict__ b,
                                                 const double *__restrict__ cc, const double *__restrict__ xrt,
@@ -430,26 +431,34 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
   const double xy = xrt[m] + yrt[my];
   // forward elimination, k = 1 .. nz-1 (z = 1/(b + xy - a dprev))
   double dprev = 0.0, pp = 0.0;
+  double vn[TD];
+#pragma unroll
+  for (int r = 0; r < TD; r++) vn[r] = 1 + r <= nz - 1 ? S[2 * (base + (long)r * plane)] : 0.0;
   for (int k0 = 1; k0 <= nz - 1; k0 += TD) {
     double v[TD];
 #pragma unroll
-    for (int r = 0; r < TD; r++) {
-      const int k = k0 + r;
-      v[r] = k <= nz - 1 ? S[2 * (base + (long)(k - 1) * plane)] : 0.0;
+    for (int r = 0; r < TD; r++) v[r] = vn[r];
+    if (k0 + TD <= nz - 1) {
+#pragma unroll
+      for (int r = 0; r < TD; r++) {
+        const int k = k0 + TD + r;
+        vn[r] = k <= nz - 1 ? S[2 * (base + (long)(k - 1) * plane)] : 0.0;
+      }
     }
 #pragma unroll
     for (int r = 0; r < TD; r++) {
       const int k = k0 + r;
-      if (k > nz - 1) break;
-      const long q = base + (long)(k - 1) * plane;
-      double z;
-      if (k == 1) z = 1.0 / (b[1] + xy);
-      else z = 1.0 / ((b[k] + xy) - a[k] * dprev);
-      dprev = cc[k] * z;
-      dtri[q] = dprev;  // both parts store the same value; each reads back its own store
-      const double w = k == 1 ? v[r] * z : (v[r] - a[k] * pp) * z;
-      S[2 * q] = w;
-      pp = w;
+      if (k <= nz - 1) {
+        const long q = base + (long)(k - 1) * plane;
+        double z;
+        if (k == 1) z = 1.0 / (b[1] + xy);
+        else z = 1.0 / ((b[k] + xy) - a[k] * dprev);
+        dprev = cc[k] * z;
+        dtri[q] = dprev;  // both parts store the same value; each reads back its own store
+        const double w = k == 1 ? v[r] * z : (v[r] - a[k] * pp) * z;
+        S[2 * q] = w;
+        pp = w;
+      }
     }
   }
   {  // last row (singular kx = ky = 0 guard)
@@ -460,23 +469,36 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
     S[2 * q] = pp;
   }
   // back substitution, k = nz-1 .. 1
+  double vb[TD], db[TD];
+#pragma unroll
+  for (int r = 0; r < TD; r++) {
+    const int k = nz - 1 - r;
+    const long q = base + (long)(k - 1) * plane;
+    vb[r] = k >= 1 ? S[2 * q] : 0.0;
+    db[r] = k >= 1 ? dtri[q] : 0.0;
+  }
   for (int k0 = nz - 1; k0 >= 1; k0 -= TD) {
     double v[TD], d[TD];
 #pragma unroll
-    for (int r = 0; r < TD; r++) {
-      const int k = k0 - r;
-      const long q = base + (long)(k - 1) * plane;
-      v[r] = k >= 1 ? S[2 * q] : 0.0;
-      d[r] = k >= 1 ? dtri[q] : 0.0;
+    for (int r = 0; r < TD; r++) { v[r] = vb[r]; d[r] = db[r]; }
+    if (k0 - TD >= 1) {
+#pragma unroll
+      for (int r = 0; r < TD; r++) {
+        const int k = k0 - TD - r;
+        const long q = base + (long)(k - 1) * plane;
+        vb[r] = k >= 1 ? S[2 * q] : 0.0;
+        db[r] = k >= 1 ? dtri[q] : 0.0;
+      }
     }
 #pragma unroll
     for (int r = 0; r < TD; r++) {
       const int k = k0 - r;
-      if (k < 1) break;
-      const long q = base + (long)(k - 1) * plane;
-      const double w = v[r] - d[r] * pp;
-      S[2 * q] = w;
-      pp = w;
+      if (k >= 1) {
+        const long q = base + (long)(k - 1) * plane;
+        const double w = v[r] - d[r] * pp;
+        S[2 * q] = w;
+        pp = w;
+      }
     }
   }
 }
