@@ -83,18 +83,23 @@ int release(ipg_ctx *c) {
   return 0;
 }
 
+// One substep, src/main.f90:268-414.  Every launch reads its inputs at the
+// source where the reference fills ghosts first:
+//  * bounduvw(dudt, dvdt, dwdt) (src/main.f90:340) feeds only fillps, which
+//    the forward x transform computes with the boundary values at the source;
+//  * boundp(p) (src/main.f90:361) is read at the source by correc; p's own
+//    ghosts are filled only when p is read back (ipg_field_get).
+// (Running the particle kernels that do not touch the fields on a second
+// stream, forked and joined with events, was measured slower in a captured
+// graph than this single-stream order.)
 int substep_launches(ipg_ctx *c, int rkiter) {
+  const bool np = c->pc.np > 0;
   if (ipg_launch_rk(c, rkiter)) return 1;
-  if (c->pc.np > 0 && ipg_launch_ibm(c)) return 1;
-  // bounduvw(dudt, dvdt, dwdt) (src/main.f90:340) feeds only fillps, and
-  // fillps (src/fillps.f90), computed inside the forward x transform, takes
-  // the boundary values at the source: no launch of its own
+  if (np && ipg_launch_ibm(c)) return 1;
   if (ipg_launch_poisson_ex(c, rkiter)) return 1;
-  // boundp(p) (src/main.f90:361) is read at the source by correc; p's own
-  // ghosts are filled only when p is read back (ipg_field_get)
   if (ipg_launch_correc_fused(c, rkiter)) return 1;
   if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C)) return 1;
-  if (c->pc.np > 0 && ipg_launch_nwtn_eulr(c, rkiter)) return 1;
+  if (np && ipg_launch_nwtn_eulr(c, rkiter)) return 1;
   return 0;
 }
 

@@ -625,12 +625,19 @@ int ipg_ibm_check(ipg_ctx *c) {
   return 0;
 }
 
-// The whole IBM block of one substep, src/main.f90:296-338.
-int ipg_launch_ibm(ipg_ctx *c) {
+// The three passes, the final spread and the force sums (after the
+// neighbour lists of ipg_launch_ibm_begin).
+int ipg_launch_ibm_passes(ipg_ctx *c) {
   if (c->pc.np <= 0) return 0;
-  if (ipg_launch_ibm_begin(c)) return 1;
   for (int pass = 0; pass < 3; pass++)
     if (ipg_launch_ibm_gather(c, pass)) return 1;
   if (ipg_launch_ibm_final(c)) return 1;
   return ipg_launch_ibm_totals(c, 2);
+}
+
+// The whole IBM block of one substep, src/main.f90:296-338.
+int ipg_launch_ibm(ipg_ctx *c) {
+  if (c->pc.np <= 0) return 0;
+  if (ipg_launch_ibm_begin(c)) return 1;
+  return ipg_launch_ibm_passes(c);
 }

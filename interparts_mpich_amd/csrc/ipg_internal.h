@@ -147,6 +147,7 @@ int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter);
 int ipg_launch_ibm(ipg_ctx *c);
 int ipg_ibm_setup(ipg_ctx *c);
 int ipg_launch_ibm_begin(ipg_ctx *c);
+int ipg_launch_ibm_passes(ipg_ctx *c);
 int ipg_launch_ibm_gather(ipg_ctx *c, int pass);
 int ipg_launch_ibm_final(ipg_ctx *c);
 int ipg_launch_ibm_totals(ipg_ctx *c, int pass);
@@ -154,6 +155,11 @@ int ipg_ibm_check(ipg_ctx *c);
 int ipg_launch_post3d(ipg_ctx *c, int q, double *host);
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type);
 int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter);
+int ipg_launch_ne_save(ipg_ctx *c);
+int ipg_launch_ne_sphere_int(ipg_ctx *c);
+int ipg_launch_ne_collide(ipg_ctx *c);
+int ipg_launch_ne_update(ipg_ctx *c, int rkiter);
+int ipg_launch_ne_wrap(ipg_ctx *c);
 int ipg_launch_nbflags(ipg_ctx *c);
 
 // bound masks for ipg_launch_bound

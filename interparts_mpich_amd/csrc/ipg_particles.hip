@@ -487,8 +487,8 @@ PQ make_pq(ipg_ctx *c) {
 
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type) {
   if (c->pc.np <= 0) return 0;
-  hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(512), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W],
-                     make_pq(c), c->ps.nbf, type);
+  hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(SPH_THREADS), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V],
+                     c->f[IPG_W], make_pq(c), c->ps.nbf, type);
   IPG_HIP(hipGetLastError());
   return 0;
 }
@@ -500,27 +500,67 @@ int ipg_launch_nbflags(ipg_ctx *c) {
   return 0;
 }
 
-int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter) {
+// The pieces of intgr_nwtn_eulr (src/intgr_nwtn_eulr.f90:12-714), each on
+// c->stream: the saved state op (77-118), the sphere integrals of types 2 and
+// 3 (120-121), one collision pass (338-606) and one position update
+// (611-672), the periodic wrap and neighbour flags (702-714);
+// ipg_launch_nwtn_eulr runs them in the reference's order.
+int ipg_launch_ne_save(ipg_ctx *c) {
   const int np = c->pc.np;
   if (np <= 0) return 0;
-  PQ P = make_pq(c);
-  const int nb = (np + 255) / 256;
-  double *cf = c->red;  // 3*np scratch after the error flag slot
-  int *err = (int *)(c->red + 3L * np);
-  hipLaunchKernelGGL(k_ne_save, dim3(nb), dim3(256), 0, c->stream, P, c->op, np);
+  hipLaunchKernelGGL(k_ne_save, dim3((np + 255) / 256), dim3(256), 0, c->stream, make_pq(c), c->op, np);
   ipg_mark(c, "ne_save");
-  hipLaunchKernelGGL(k_sphere_int, dim3(np), dim3(512), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], P,
-                     c->ps.nbf, 23);
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+int ipg_launch_ne_sphere_int(ipg_ctx *c) {
+  if (c->pc.np <= 0) return 0;
+  hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(SPH_THREADS), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V],
+                     c->f[IPG_W], make_pq(c), c->ps.nbf, 23);
   ipg_mark(c, "sphere_int");
-  for (int iter = 0; iter <= 1; iter++) {
-    hipLaunchKernelGGL(k_collide, dim3((np + COL_WAVES - 1) / COL_WAVES), dim3(COL_WAVES * 64), 0, c->stream, c->g, c->pc, P, cf, cf + np, cf + 2 * np, err);
-    ipg_mark(c, "collide");
-    hipLaunchKernelGGL(k_ne_update, dim3(nb), dim3(256), 0, c->stream, c->g, c->pc, P, c->op, cf, cf + np, cf + 2 * np,
-                       c->dscal, rkiter);
-    ipg_mark(c, "ne_update");
-  }
-  hipLaunchKernelGGL(k_wrap_flags, dim3(nb), dim3(256), 0, c->stream, c->g, c->pc, P, c->ps.nbf, 1);
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+// collision forces into the 3*np scratch after the reduction slots, with an
+// overflow flag after them
+int ipg_launch_ne_collide(ipg_ctx *c) {
+  const int np = c->pc.np;
+  if (np <= 0) return 0;
+  double *cf = c->red;
+  int *err = (int *)(c->red + 3L * np);
+  hipLaunchKernelGGL(k_collide, dim3((np + COL_WAVES - 1) / COL_WAVES), dim3(COL_WAVES * 64), 0, c->stream, c->g, c->pc,
+                     make_pq(c), cf, cf + np, cf + 2 * np, err);
+  ipg_mark(c, "collide");
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+int ipg_launch_ne_update(ipg_ctx *c, int rkiter) {
+  const int np = c->pc.np;
+  if (np <= 0) return 0;
+  double *cf = c->red;
+  hipLaunchKernelGGL(k_ne_update, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->g, c->pc, make_pq(c), c->op, cf, cf + np,
+                     cf + 2 * np, c->dscal, rkiter);
+  ipg_mark(c, "ne_update");
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+int ipg_launch_ne_wrap(ipg_ctx *c) {
+  const int np = c->pc.np;
+  if (np <= 0) return 0;
+  hipLaunchKernelGGL(k_wrap_flags, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->g, c->pc, make_pq(c), c->ps.nbf, 1);
   ipg_mark(c, "wrap_flags");
   IPG_HIP(hipGetLastError());
   return 0;
+}
+
+int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter) {
+  if (c->pc.np <= 0) return 0;
+  if (ipg_launch_ne_save(c) || ipg_launch_ne_sphere_int(c)) return 1;
+  for (int iter = 0; iter <= 1; iter++)
+    if (ipg_launch_ne_collide(c) || ipg_launch_ne_update(c, rkiter)) return 1;
+  return ipg_launch_ne_wrap(c);
 }
