@@ -69,6 +69,19 @@ struct IbmArgs {
   const double *dtp;
 };
 
+// offset in a k1 plane (nx*ny) of the unwrapped stencil cell (i, j).  A
+// marker lies within radfp of its sphere's centre, which is inside the
+// domain, and a box is narrower than the domain, so i is in [1-nx, 2nx] and
+// one conditional shift per direction wraps it.
+__device__ __forceinline__ long kplane(const Grid &g, int i, int j) {
+  int r = i - 1, s = j - 1;
+  r += r < 0 ? g.nx : 0;
+  r -= r >= g.nx ? g.nx : 0;
+  s += s < 0 ? g.ny : 0;
+  s -= s >= g.ny ? g.ny : 0;
+  return (long)r + (long)g.nx * s;
+}
+
 // periodic wrap of any unwrapped index onto 1..n
 __device__ __forceinline__ int wrapany(int i, int n) {
   int r = i - 1;
@@ -435,25 +448,30 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
       for (int j = 0; j < 3; j++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-          double f;
-          if (k < nzin) f = box[e0 + i + b.bx * j + bxy * k];
-          else f = A.Kread[(long)(wrapany(c.i0 + i, g.nx) - 1) + (long)g.nx * (wrapany(c.j0 + j, g.ny) - 1)];
-          acc = acc + f * c.wx[i] * c.wy[j] * c.wz[k];
-        }
-    const double fnew = MODE == 0 ? (ub - acc) * dti : flp[m] + (ub - acc) * dti;
-    fl[m] = fnew;
-    // the part of this pass's spread that falls on the k1 plane
+        for (int i = 0; i < 3; i++)
+          if (k < nzin) acc = acc + box[e0 + i + b.bx * j + bxy * k] * c.wx[i] * c.wy[j] * c.wz[k];
+    // the stencil planes above kmax: the k1 plane (the same k order, so the
+    // same sum), and this pass's spread into it
     if (nzin < 3) {
-      const double fc = fnew * c0;
 #pragma unroll
-      for (int k = 0; k < 3; k++)
+      for (int k = 1; k < 3; k++)
 #pragma unroll
         for (int j = 0; j < 3; j++)
 #pragma unroll
           for (int i = 0; i < 3; i++)
-            if (k >= nzin) unsafeAtomicAdd(&A.Kspread[(long)(wrapany(c.i0 + i, g.nx) - 1) + (long)g.nx * (wrapany(c.j0 + j, g.ny) - 1)],
-                            fc * c.wx[i] * c.wy[j] * c.wz[k]);
+            if (k >= nzin) acc = acc + A.Kread[kplane(g, c.i0 + i, c.j0 + j)] * c.wx[i] * c.wy[j] * c.wz[k];
+    }
+    const double fnew = MODE == 0 ? (ub - acc) * dti : flp[m] + (ub - acc) * dti;
+    fl[m] = fnew;
+    if (nzin < 3) {
+      const double fc = fnew * c0;
+#pragma unroll
+      for (int k = 1; k < 3; k++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+            if (k >= nzin) unsafeAtomicAdd(&A.Kspread[kplane(g, c.i0 + i, c.j0 + j)], fc * c.wx[i] * c.wy[j] * c.wz[k]);
     }
   }
 }
