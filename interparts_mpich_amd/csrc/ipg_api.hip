@@ -123,6 +123,12 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
     return 2;
   }
   if (prm->precision != 0) { ipg_set_error("ipg_init: only precision 0 (FP64) is built"); return 1; }
+  if (prm->np > 0) {
+    // the sphere-integral box (src/intgr_over_sphere.f90:117-130) is at most
+    // 2 R / h + 6 cells wide; its cells are indexed with ten bits per axis
+    const double w = 2.0 * prm->radius * std::max(prm->itot / prm->lx, std::max(prm->jtot / prm->ly, prm->ktot / prm->lz)) + 6.0;
+    if (!(w < 1024.0)) { ipg_set_error("ipg_init: spheres wider than 1017 grid cells"); return 1; }
+  }
   ipg_ctx *c = new ipg_ctx();
   std::memset((void *)c, 0, sizeof(ipg_ctx));
   c->prm = *prm;

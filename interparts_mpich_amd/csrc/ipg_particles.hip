@@ -142,7 +142,7 @@ __global__ __launch_bounds__(SPH_THREADS) void k_sphere_int(Grid g, PartConst pc
             if (ex * ex + ey * ey + ez * ez <= R * R * (1.0 + 1e-9)) {
               const double dist2 = sqd(xplus - X) + sqd(yplus - Y) + sqd(zplus - Z);
               if (dist2 < radin2) add(i, j, k, F[g.at(i, j, k)], true, 1.0);
-              else cut[atomicAdd(&ncut, 1)] = t;
+              else cut[atomicAdd(&ncut, 1)] = ii | (jj << 10) | (kk << 20);
             }
             ii += si; jj += sj; kk += sk;
             if (ii >= ni) { ii -= ni; jj++; }
@@ -151,9 +151,10 @@ __global__ __launch_bounds__(SPH_THREADS) void k_sphere_int(Grid g, PartConst pc
           __syncthreads();
           const int n = ncut;
           for (int q = threadIdx.x; q < n; q += SPH_THREADS) {
-            const int tc = cut[q];
-            const int ck = tc / nij, cr = tc - ck * nij, cj = cr / ni;
-            const int i = ilow + (cr - cj * ni), j = jlow + cj, k = klow + ck;
+            // box cell packed ten bits per axis (boxes are narrower than
+            // 1024 cells, checked at set-up)
+            const int e = cut[q];
+            const int i = ilow + (e & 1023), j = jlow + ((e >> 10) & 1023), k = klow + (e >> 20);
             double xmin, xplus, ymin, yplus, zmin, zplus;
             corners(i, j, k, xmin, xplus, ymin, yplus, zmin, zplus);
             const double dzp2 = sqd(zplus - Z), dyp2 = sqd(yplus - Y), dxp2 = sqd(xplus - X);
