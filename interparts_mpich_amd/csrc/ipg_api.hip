@@ -94,13 +94,30 @@ int release(ipg_ctx *c) {
 // graph than this single-stream order.)
 int substep_launches(ipg_ctx *c, int rkiter) {
   const bool np = c->pc.np > 0;
-  if (ipg_launch_rk(c, rkiter)) return 1;
+  if (ipg_launch_rk(c, rkiter, true)) return 1;
   if (np && ipg_launch_ibm(c)) return 1;
   if (ipg_launch_poisson_ex(c, rkiter)) return 1;
   if (ipg_launch_correc_fused(c, rkiter)) return 1;
-  if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C)) return 1;
+  if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C | BND_NO_XCOL)) return 1;
   if (np && ipg_launch_nwtn_eulr(c, rkiter)) return 1;
+  // left to the reader: p's ghosts and the x-ghost columns of the fields
+  c->p_ghosts_stale = true;
+  c->xcol_stale = true;
   return 0;
+}
+
+// The x-ghost columns of u, v, w, pnew, c that the fused substep leaves
+// unfilled (its readers take the periodic image): filled before any other
+// reader sees them.
+int fill_xcol(ipg_ctx *c) {
+  if (!c->xcol_stale) return 0;
+  if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C)) return 1;
+  c->xcol_stale = false;
+  return 0;
+}
+
+bool ghosted_field(int field) {
+  return field == IPG_U || field == IPG_V || field == IPG_W || field == IPG_PNEW || field == IPG_C;
 }
 
 }  // namespace
@@ -196,6 +213,7 @@ int ipg_field_put(ipg_ctx *c, int field, const double *host) {
   if (!check_ctx(c)) return 1;
   if (field < 0 || field >= IPG_NFIELDS || !host) { ipg_set_error("field_put: bad field"); return 1; }
   if (field == IPG_P) c->p_ghosts_stale = false;
+  if (ghosted_field(field) && fill_xcol(c)) return 1;
   const Grid &g = c->g;
   IPG_HIP(hipMemcpy2DAsync(c->f[field] - 1, g.sx * sizeof(double), host, (g.nx + 2) * sizeof(double),
                            (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyHostToDevice, c->stream));
@@ -211,6 +229,7 @@ int ipg_field_get(ipg_ctx *c, int field, double *host) {
     if (ipg_launch_bound(c, BND_P)) return 1;
     c->p_ghosts_stale = false;
   }
+  if (ghosted_field(field) && fill_xcol(c)) return 1;
   const Grid &g = c->g;
   IPG_HIP(hipMemcpy2DAsync(host, (g.nx + 2) * sizeof(double), c->f[field] - 1, g.sx * sizeof(double),
                            (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyDeviceToHost, c->stream));
@@ -303,7 +322,7 @@ int ipg_coordsfp(ipg_ctx *c, const double *thetarc, const double *phirc, double 
 int ipg_rk(ipg_ctx *c, int rkiter) {
   if (!check_ctx(c)) return 1;
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("rk: rkiter must be 1..3"); return 1; }
-  if (ipg_launch_rk(c, rkiter)) return 1;
+  if (fill_xcol(c) || ipg_launch_rk(c, rkiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -365,14 +384,14 @@ int ipg_uniform(unsigned long long seed, unsigned long long stream, unsigned lon
 }
 
 int ipg_bounduvw(ipg_ctx *c, int which) {
-  if (!check_ctx(c)) return 1;
+  if (!check_ctx(c) || fill_xcol(c)) return 1;
   if (ipg_launch_bound(c, which == 0 ? BND_UVW_NEW : BND_UVW_DT)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
 
 int ipg_boundp(ipg_ctx *c, int which) {
-  if (!check_ctx(c)) return 1;
+  if (!check_ctx(c) || fill_xcol(c)) return 1;
   if (ipg_launch_bound(c, which == 0 ? BND_PNEW : BND_P)) return 1;
   if (which != 0) c->p_ghosts_stale = false;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -380,7 +399,7 @@ int ipg_boundp(ipg_ctx *c, int which) {
 }
 
 int ipg_boundscal(ipg_ctx *c) {
-  if (!check_ctx(c)) return 1;
+  if (!check_ctx(c) || fill_xcol(c)) return 1;
   if (ipg_launch_bound(c, BND_C)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
@@ -414,6 +433,7 @@ int ipg_pnew_update(ipg_ctx *c) {
   if (!check_ctx(c)) return 1;
   if (ipg_launch_pnew(c)) return 1;
   if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C)) return 1;
+  c->xcol_stale = false;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -453,18 +473,20 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
       c->graph_ok[r] = true;
     }
     IPG_HIP(hipGraphLaunch(c->graph[r], c->stream));
+    c->p_ghosts_stale = true;
+    c->xcol_stale = true;
     return 0;
   }
   return substep_launches(c, rkiter);
 }
 
 int ipg_chkdt(ipg_ctx *c, double *dtmax) {
-  if (!check_ctx(c) || !dtmax) return 1;
+  if (!check_ctx(c) || !dtmax || fill_xcol(c)) return 1;
   return ipg_launch_chkdt(c, dtmax);
 }
 
 int ipg_chkdiv(ipg_ctx *c, double *tot, double *mx) {
-  if (!check_ctx(c) || !tot || !mx) return 1;
+  if (!check_ctx(c) || !tot || !mx || fill_xcol(c)) return 1;
   return ipg_launch_chkdiv(c, tot, mx);
 }
 
@@ -472,6 +494,7 @@ int ipg_post3d(ipg_ctx *c, int quantity, double *host) {
   if (!check_ctx(c) || !host) return 1;
   if (quantity < 0 || quantity > 5) { ipg_set_error("post3d: quantity must be 0..5"); return 1; }
   if (!c->solver_ready) { ipg_set_error("post3d: ipg_initsolver has not been called"); return 1; }
+  if (fill_xcol(c)) return 1;
   return ipg_launch_post3d(c, quantity, host);
 }
 
@@ -519,11 +542,11 @@ int ipg_autotune(ipg_ctx *c, int *kb_out) {
   float best_ms = 0.f;
   for (int kb = 8; kb <= 64 && kb <= c->g.nz; kb *= 2) {
     c->tune_rk_kb = kb;
-    if (ipg_launch_rk(c, 3)) { c->tune_rk_kb = saved; return 1; }  // warm-up
+    if (ipg_launch_rk(c, 3, true)) { c->tune_rk_kb = saved; return 1; }  // warm-up
     float t_min = 0.f;
     for (int rep = 0; rep < 3; rep++) {
       IPG_HIP(hipEventRecord(c->ev[0], c->stream));
-      if (ipg_launch_rk(c, 3)) { c->tune_rk_kb = saved; return 1; }
+      if (ipg_launch_rk(c, 3, true)) { c->tune_rk_kb = saved; return 1; }
       IPG_HIP(hipEventRecord(c->ev[1], c->stream));
       IPG_HIP(hipEventSynchronize(c->ev[1]));
       float t = 0.f;

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(RNT) void k_rk(Grid g, const double *__restrict__ U
                                             double *__restrict__ dcdt, double *__restrict__ durk,
                                             double *__restrict__ dvrk, double *__restrict__ dwrk,
                                             double *__restrict__ dcrk, const double *__restrict__ dtp, int ntx,
-                                            int nty, int kb) {
+                                            int nty, int kb, int wrapx) {
   __shared__ double sm[2][5][LPL];
   const int nblk = gridDim.x, L = blockIdx.x;
   const int t = (nblk % 8 == 0) ? (L % 8) * (nblk / 8) + L / 8 : L;
@@ -55,7 +55,11 @@ __global__ __launch_bounds__(RNT) void k_rk(Grid g, const double *__restrict__ U
     const int e = (int)threadIdx.x + q * RNT, ii = e % LW, jj = e / LW;
     const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
     gok[q] = e < LPL && gi <= g.nx + 1 && gj <= g.ny + 1;
-    goff[q] = (long)(gi - 1) + sx * gj;
+    // wrapx: the x halo is read at its periodic image, so the x-ghost
+    // columns of the inputs need not be filled (k_bound's plane and row parts
+    // fill the ghost rows and planes, x ghosts included); else at the ghosts
+    const int gw = !wrapx ? gi : (gi == 0 ? g.nx : (gi == g.nx + 1 ? 1 : gi));
+    goff[q] = (long)(gw - 1) + sx * gj;
   }
   double pre[5][NLD];
   auto fetch = [&](int k) {
@@ -83,7 +87,11 @@ __global__ __launch_bounds__(RNT) void k_rk(Grid g, const double *__restrict__ U
   long c = g.at(i, j, k0);
   // k-1 values of the first plane
   double Ukm = 0.0, Vkm = 0.0, Wkm = 0.0, Wip_km = 0.0, Wjp_km = 0.0, Ckm = 0.0;
-  if (active) { Ukm = U[c - sxy]; Vkm = V[c - sxy]; Wkm = W[c - sxy]; Wip_km = W[c + 1 - sxy]; Wjp_km = W[c + sx - sxy]; Ckm = C[c - sxy]; }
+  if (active) {
+    Ukm = U[c - sxy]; Vkm = V[c - sxy]; Wkm = W[c - sxy];
+    Wip_km = W[(wrapx && i == g.nx ? c - (g.nx - 1) : c + 1) - sxy];
+    Wjp_km = W[c + sx - sxy]; Ckm = C[c - sxy];
+  }
   fetch(k0);
   stash(0);
   fetch(k0 + 1);
@@ -197,11 +205,15 @@ __global__ __launch_bounds__(RNT) void k_rk(Grid g, const double *__restrict__ U
 // rule applied to the periodic image (i',j') of its column, which is what the
 // reference's sequence z-rule -> x-halo -> y-halo produces.
 enum { KIND_UV = 0, KIND_W = 1, KIND_P = 2, KIND_C = 3 };
-struct BoundJob { double *f[5]; int kind[5]; int nf; };
+// xcol = 0 leaves out the x-ghost columns (i = 0, nx+1 for j in 1..ny, k in
+// 1..nz), one scattered value per row: the fused substep's readers take the
+// periodic image instead, and the library fills them before anything else
+// reads them.
+struct BoundJob { double *f[5]; int kind[5]; int nf; int xcol; };
 
 __global__ void k_bound(Grid g, BoundJob job) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int nplane = (nx + 2) * (ny + 2), nrow = (nx + 2) * nz, ncol = ny * nz, nwall = nx * ny;
+  const int nplane = (nx + 2) * (ny + 2), nrow = (nx + 2) * nz, ncol = job.xcol ? ny * nz : 0, nwall = nx * ny;
   const int total = 2 * nplane + 2 * nrow + 2 * ncol + nwall;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     int i, j, k; bool wallplane = false;
@@ -414,7 +426,7 @@ __global__ __launch_bounds__(BX *BY) void k_post3d(Grid g, int q, const double *
 
 static dim3 cell_grid(const Grid &g) { return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, g.nz); }
 
-int ipg_launch_rk(ipg_ctx *c, int rkiter) {
+int ipg_launch_rk(ipg_ctx *c, int rkiter, bool wrapx) {
   const Grid &g = c->g;
   double **f = c->f;
   const int ntx = (g.nx + RTX - 1) / RTX, nty = (g.ny + RTY - 1) / RTY;
@@ -430,7 +442,7 @@ int ipg_launch_rk(ipg_ctx *c, int rkiter) {
 #define IPG_RK_LAUNCH(S)                                                                                               \
   hipLaunchKernelGGL(k_rk<S>, grid, blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],         \
                      f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK],         \
-                     f[IPG_DCRK], c->dscal, ntx, nty, kb)
+                     f[IPG_DCRK], c->dscal, ntx, nty, kb, wrapx ? 1 : 0)
   if (rkiter == 1) IPG_RK_LAUNCH(1);
   else if (rkiter == 2) IPG_RK_LAUNCH(2);
   else IPG_RK_LAUNCH(3);
@@ -442,6 +454,7 @@ int ipg_launch_rk(ipg_ctx *c, int rkiter) {
 
 int ipg_launch_bound(ipg_ctx *c, int mask) {
   BoundJob job{};
+  job.xcol = (mask & BND_NO_XCOL) ? 0 : 1;
   auto add = [&](double *f, int kind) { job.f[job.nf] = f; job.kind[job.nf] = kind; job.nf++; };
   if (mask & BND_UVW_NEW) { add(c->f[IPG_U], KIND_UV); add(c->f[IPG_V], KIND_UV); add(c->f[IPG_W], KIND_W); }
   if (mask & BND_PNEW) add(c->f[IPG_PNEW], KIND_P);
@@ -456,7 +469,7 @@ int ipg_launch_bound(ipg_ctx *c, int mask) {
   }
   if (job.nf == 0) return 0;
   const Grid &g = c->g;
-  const long total = 2L * (g.nx + 2) * (g.ny + 2) + 2L * (g.nx + 2) * g.nz + 2L * g.ny * g.nz + (long)g.nx * g.ny;
+  const long total = 2L * (g.nx + 2) * (g.ny + 2) + 2L * (g.nx + 2) * g.nz + (job.xcol ? 2L * g.ny * g.nz : 0L) + (long)g.nx * g.ny;
   if (total > 2000000000L) { ipg_set_error("bound: grid too large"); return 1; }
   int blocks = (int)((total + 255) / 256);
   if (blocks > 4096) blocks = 4096;
@@ -490,7 +503,6 @@ int ipg_launch_correc_fused(ipg_ctx *c, int rkiter) {
   hipLaunchKernelGGL(k_correc<true>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT],
                      c->f[IPG_DWDT], c->f[IPG_P], c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->f[IPG_DCDT],
                      c->f[IPG_C], c->dscal, c->pc.rkab[rkiter - 1]);
-  c->p_ghosts_stale = true;
   IPG_HIP(hipGetLastError());
   ipg_mark(c, "correc");
   return 0;

@@ -64,6 +64,7 @@ struct ipg_ctx {
   double *K1[3];                  // dwdtf(.,.,k1) planes (nx*ny), src/interp_spread.f90:100 (no upper clamp)
   bool ibm_lds_set;
   bool p_ghosts_stale;            // the fused substep did not fill p's ghosts
+  bool xcol_stale;                // nor the x-ghost columns of u, v, w, pnew, c
   // launch shape: z planes per RK-stage workgroup (IPG_TUNE_RK_KB at init; 0 = chosen per grid)
   int tune_rk_kb;
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
@@ -133,7 +134,9 @@ __device__ __forceinline__ int wrapi(int i, int n) {
 }
 
 // launch helpers implemented per translation unit
-int ipg_launch_rk(ipg_ctx *c, int rkiter);
+// wrapx: read the x halo at its periodic image (the fused substep, whose
+// ghost fill leaves the x-ghost columns out) rather than at the ghosts
+int ipg_launch_rk(ipg_ctx *c, int rkiter, bool wrapx = false);
 int ipg_launch_bound(ipg_ctx *c, int mask);
 int ipg_launch_fillps(ipg_ctx *c, int rkiter);
 int ipg_launch_correc(ipg_ctx *c, int rkiter);
@@ -163,4 +166,4 @@ int ipg_launch_ne_wrap(ipg_ctx *c);
 int ipg_launch_nbflags(ipg_ctx *c);
 
 // bound masks for ipg_launch_bound
-enum { BND_UVW_NEW = 1, BND_UVW_DT = 2, BND_PNEW = 4, BND_P = 8, BND_C = 16 };
+enum { BND_UVW_NEW = 1, BND_UVW_DT = 2, BND_PNEW = 4, BND_P = 8, BND_C = 16, BND_NO_XCOL = 32 };
