@@ -39,7 +39,7 @@ def kernel_bytes(case, name, rkiter):
         "thomas": 5.0 * Sw,           # spectrum read + written in each sweep, d written then read
         "fft_y_inv": 2.0 * Sw,
         "fft_x_inv": Sw + N,
-        "correc": 11.0 * N,           # dudt dvdt dwdt p pnew dcdt -> u v w pnew cnew
+        "correc": 9.0 * N,            # dudt dvdt dwdt p pnew -> u v w pnew (cnew = dcdt is a buffer swap)
     }
     w = words.get(name)
     return None if w is None else 8.0 * w

@@ -71,7 +71,7 @@ bool check_ctx(ipg_ctx *c) {
 }
 
 int release(ipg_ctx *c) {
-  for (int r = 0; r < 3; r++)
+  for (int r = 0; r < 6; r++)
     if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
   for (int q = 0; q < 64; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
@@ -98,11 +98,26 @@ int substep_launches(ipg_ctx *c, int rkiter) {
   if (np && ipg_launch_ibm(c)) return 1;
   if (ipg_launch_poisson_ex(c, rkiter)) return 1;
   if (ipg_launch_correc_fused(c, rkiter)) return 1;
+  // cnew = dcdt (src/main.f90:375): the buffers swap roles, no copy; dcdt's
+  // value (equal to cnew) is materialised only if it is read
+  std::swap(c->f[IPG_C], c->f[IPG_DCDT]);
+  c->cpar ^= 1;
   if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C | BND_NO_XCOL)) return 1;
   if (np && ipg_launch_nwtn_eulr(c, rkiter)) return 1;
-  // left to the reader: p's ghosts and the x-ghost columns of the fields
+  // left to the reader: p's ghosts, the x-ghost columns of the fields, dcdt
   c->p_ghosts_stale = true;
   c->xcol_stale = true;
+  c->dcdt_stale = true;
+  return 0;
+}
+
+// dcdt after a fused substep equals cnew (src/main.f90:375): copy it into
+// dcdt's buffer before dcdt is read, or before c changes
+int fill_dcdt(ipg_ctx *c) {
+  if (!c->dcdt_stale) return 0;
+  IPG_HIP(hipMemcpyAsync(c->f[IPG_DCDT] - 16, c->f[IPG_C] - 16, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
+                         c->stream));
+  c->dcdt_stale = false;
   return 0;
 }
 
@@ -214,6 +229,8 @@ int ipg_field_put(ipg_ctx *c, int field, const double *host) {
   if (field < 0 || field >= IPG_NFIELDS || !host) { ipg_set_error("field_put: bad field"); return 1; }
   if (field == IPG_P) c->p_ghosts_stale = false;
   if (ghosted_field(field) && fill_xcol(c)) return 1;
+  if (field == IPG_C && fill_dcdt(c)) return 1;
+  if (field == IPG_DCDT) c->dcdt_stale = false;
   const Grid &g = c->g;
   IPG_HIP(hipMemcpy2DAsync(c->f[field] - 1, g.sx * sizeof(double), host, (g.nx + 2) * sizeof(double),
                            (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyHostToDevice, c->stream));
@@ -230,6 +247,7 @@ int ipg_field_get(ipg_ctx *c, int field, double *host) {
     c->p_ghosts_stale = false;
   }
   if (ghosted_field(field) && fill_xcol(c)) return 1;
+  if (field == IPG_DCDT && fill_dcdt(c)) return 1;
   const Grid &g = c->g;
   IPG_HIP(hipMemcpy2DAsync(host, (g.nx + 2) * sizeof(double), c->f[field] - 1, g.sx * sizeof(double),
                            (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyDeviceToHost, c->stream));
@@ -322,7 +340,7 @@ int ipg_coordsfp(ipg_ctx *c, const double *thetarc, const double *phirc, double 
 int ipg_rk(ipg_ctx *c, int rkiter) {
   if (!check_ctx(c)) return 1;
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("rk: rkiter must be 1..3"); return 1; }
-  if (fill_xcol(c) || ipg_launch_rk(c, rkiter)) return 1;
+  if (fill_xcol(c) || fill_dcdt(c) || ipg_launch_rk(c, rkiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -430,7 +448,7 @@ int ipg_correc(ipg_ctx *c, int rkiter) {
 }
 
 int ipg_pnew_update(ipg_ctx *c) {
-  if (!check_ctx(c)) return 1;
+  if (!check_ctx(c) || fill_dcdt(c)) return 1;
   if (ipg_launch_pnew(c)) return 1;
   if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C)) return 1;
   c->xcol_stale = false;
@@ -459,9 +477,13 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("substep: rkiter must be 1..3"); return 1; }
   if (!c->solver_ready) { ipg_set_error("substep: ipg_initsolver has not been called"); return 1; }
   if (c->pc.np > 0 && !c->markers_ready) { ipg_set_error("substep: ipg_coordsfp has not been called"); return 1; }
-  const int r = rkiter - 1;
+  // the substep rewrites dcdt's interior (RK stage) before swapping it in as
+  // c, whose ghosts it then fills: the stale dcdt value is never needed
+  c->dcdt_stale = false;
   if (c->use_graphs) {
+    const int r = (rkiter - 1) + 3 * c->cpar;
     if (!c->graph_ok[r]) {
+      // capturing records the launches and makes the host-side buffer swap
       hipGraph_t graph;
       IPG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
       int rc = substep_launches(c, rkiter);
@@ -471,10 +493,14 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
       IPG_HIP(hipGraphInstantiate(&c->graph[r], graph, nullptr, nullptr, 0));
       (void)hipGraphDestroy(graph);
       c->graph_ok[r] = true;
+    } else {
+      std::swap(c->f[IPG_C], c->f[IPG_DCDT]);
+      c->cpar ^= 1;
     }
     IPG_HIP(hipGraphLaunch(c->graph[r], c->stream));
     c->p_ghosts_stale = true;
     c->xcol_stale = true;
+    c->dcdt_stale = true;
     return 0;
   }
   return substep_launches(c, rkiter);
@@ -556,7 +582,8 @@ int ipg_autotune(ipg_ctx *c, int *kb_out) {
     if (best == 0 || t_min < best_ms) { best = kb; best_ms = t_min; }
   }
   c->tune_rk_kb = best > 0 ? best : saved;
-  for (int r = 0; r < 3; r++)
+  c->dcdt_stale = false;  // the timed stages rewrote dcdt
+  for (int r = 0; r < 6; r++)
     if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
   if (kb_out) *kb_out = c->tune_rk_kb;
   return 0;
@@ -573,10 +600,10 @@ int ipg_substep_bytes(ipg_ctx *c, double *bytes) {
   if (!check_ctx(c) || !bytes) return 1;
   // FP64 words per substep, by kernel: RK stage 43/3 N (13, 17, 13 by stage),
   // forward x FFT with fillps 3 N + Sw, two y passes 2 Sw each, Thomas 5 Sw,
-  // inverse x FFT Sw + N, correc 11 N; Sw = 2 (nx/2+1) ny nz.
+  // inverse x FFT Sw + N, correc 9 N; Sw = 2 (nx/2+1) ny nz.
   const double N = (double)c->g.nx * c->g.ny * c->g.nz;
   const double Sw = 2.0 * (c->g.nx / 2 + 1) * (double)c->g.ny * c->g.nz;
-  const double fluid = (43.0 / 3.0 + 3.0 + 1.0 + 11.0) * N + (1.0 + 4.0 + 5.0 + 1.0) * Sw;
+  const double fluid = (43.0 / 3.0 + 3.0 + 1.0 + 9.0) * N + (1.0 + 4.0 + 5.0 + 1.0) * Sw;
   // IBM: per pass and component the stencil shell (radfp +- 2.5 h) is read
   // once from dudt (+ G) and receives one atomic add; sphere integrals read
   // each sphere's box once per component

@@ -270,8 +270,9 @@ __global__ __launch_bounds__(BX *BY) void k_fillps(Grid g, const double *__restr
 // src/correc.f90:11-42.  p is read at i+1, j+1, k+1 through the values
 // boundp (src/bound.f90:82-107) gives its ghosts -- the periodic image in x
 // and y, p(kmax) above the top wall -- so p's ghosts need not be filled.
-// FUSED adds src/main.f90:370 (pnew += p) and 375 (cnew = dcdt) on the
-// interior; the ghosts of u, v, w, pnew, cnew are filled by k_bound after.
+// FUSED adds src/main.f90:370 (pnew += p) on the interior; cnew = dcdt (375)
+// is a swap of the two buffers' roles made by the caller, and the ghosts of
+// u, v, w, pnew, cnew are filled by k_bound after.
 // (Writing those ghosts from the threads of their interior images was
 // measured: the scattered x-ghost writes and the divergent boundary lanes
 // cost as much as the separate ghost-fill launch.)
@@ -294,10 +295,7 @@ __global__ __launch_bounds__(BX *BY) void k_correc(Grid g, const double *__restr
   u[c] = dudt[c] - fi * (pi1 - pc);
   v[c] = dvdt[c] - fj * (pj1 - pc);
   w[c] = dwdt[c] - fk * (pk1 - pc);
-  if (FUSED) {
-    pnew[c] = pnew[c] + pc;
-    cnew[c] = dcdt[c];
-  }
+  if (FUSED) pnew[c] = pnew[c] + pc;
 }
 
 __global__ __launch_bounds__(BX *BY) void k_pnew(Grid g, const double *__restrict__ p, double *__restrict__ pnew,

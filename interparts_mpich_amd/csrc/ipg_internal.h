@@ -81,8 +81,13 @@ struct ipg_ctx {
   double dt_host;
   Poisson ps2;
   bool solver_ready, markers_ready, use_graphs;
-  hipGraphExec_t graph[3];
-  bool graph_ok[3];
+  // one graph per (rkiter, buffer parity): the fused substep swaps the roles
+  // of the c and dcdt buffers (cnew = dcdt, src/main.f90:375), so the
+  // captured pointers alternate
+  hipGraphExec_t graph[6];
+  bool graph_ok[6];
+  int cpar;                       // 1 when c and dcdt have swapped buffers an odd number of times
+  bool dcdt_stale;                // dcdt's buffer holds the previous c; its value is c's
   // kernel timing (ipg_substep_phases): an event is recorded after every
   // launch while timing is on
   bool timing;
