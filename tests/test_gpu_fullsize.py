@@ -66,16 +66,21 @@ def test_ibm_momentum_balance_and_divergence(name):
 
 
 def test_graph_replay_full_size():
+    """Four steps: each of the six graphs (rkiter x the parity of the c/dcdt
+    buffer swap) is captured once and replayed once; the result matches the
+    eager launches (IBM atomics reorder sums: <= 1e-11 after 12 substeps)."""
     case = CONFIGS["c2_256"]
     a = make(case)
     b = make(case, graphs=True)
-    for s in range(3):
-        a.substep(s + 1)
-        b.substep(s + 1)
+    for s in range(12):
+        a.substep(s % 3 + 1)
+        b.substep(s % 3 + 1)
     b.sync()
-    for n in ("U", "V", "W", "PNEW"):
+    for n in ("U", "V", "W", "PNEW", "C"):
         r = rel(b.get(n), a.get(n))
         print(n, r)
-        assert r <= 1e-13, n
+        assert r <= 1e-11, n
+    # dcdt after the substep equals cnew (src/main.f90:375)
+    assert np.array_equal(b.get("DCDT")[1:-1, 1:-1, 1:-1], b.get("C")[1:-1, 1:-1, 1:-1])
     for q in ("X", "U", "FXLTOT"):
-        assert rel(b.pget(q), a.pget(q)) <= 1e-13, q
+        assert rel(b.pget(q), a.pget(q)) <= 1e-11, q

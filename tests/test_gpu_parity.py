@@ -138,6 +138,9 @@ def test_step_parity(name):
     for n, r in list(wf.items()) + list(wp.items()):
         assert r <= 1e-12, (n, r)
     assert g.dt == o.dt or abs(g.dt - o.dt) <= 1e-14 * o.dt
+    # dcdt equals cnew after the substep (src/main.f90:375)
+    r = rel(g.get("DCDT")[1:-1, 1:-1, 1:-1], o.get("DCDT")[1:-1, 1:-1, 1:-1])
+    assert r <= 1e-12, ("DCDT", r)
     # the fused substep leaves p's ghosts to the reader: get() fills them as
     # boundp(p) would (compared without the solve's free constant)
     pg, po = g.get("P"), o.get("P")
