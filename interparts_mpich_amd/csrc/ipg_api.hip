@@ -76,7 +76,8 @@ int release(ipg_ctx *c) {
   for (int q = 0; q < 64; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
   ipg_poisson_free(c);
-  void *ptrs[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->nml, c->nmc, c->ibm_err, c->op, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
+  void *ptrs[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->nml, c->nmc, c->ibm_err, c->ibm_flg, c->op, c->red, c->dscal,
+                  c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   for (int q = 0; q < IPG_NPART; q++) if (c->ps.q[q]) (void)hipFree(c->ps.q[q]);
   if (c->stream) (void)hipStreamDestroy(c->stream);

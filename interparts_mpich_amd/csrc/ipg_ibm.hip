@@ -65,6 +65,7 @@ struct IbmArgs {
   int *nml;           // [np*3][IBM_MAXNM] neighbour markers touching the box: (q*nl + l)*25 + shift
   int *nmc;           // [np*3] their count
   int *err;           // bit 1: a neighbour list overflowed
+  unsigned *flg;      // [np*3][nflagmax] the box cells the own stencils touch, set by pass 0
   double *Kread, *Kspread, *Kzero;  // dwdtf(.,.,k1) planes
   const double *dtp;
 };
@@ -291,8 +292,11 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     else ub = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
     return true;
   };
+  // the cells the own stencils touch: flagged by pass 0, read back by the
+  // later launches (the markers do not move inside the IBM block)
+  unsigned *gflag = A.flg + (long)blockIdx.x * nflagmax;
   for (int t = threadIdx.x; t < nflag; t += blockDim.x) {
-    flag[t] = 0u;
+    flag[t] = MODE == 0 ? 0u : gflag[t];
     if (MODE == 2) taken[t] = 0u;
   }
   for (int t = threadIdx.x; t < b.by * b.bz; t += blockDim.x) {
@@ -327,8 +331,8 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
         for (int i = 0; i < 3; i++) {
           if (k >= nzin) continue;
           const int e = e0 + i + b.bx * j + bxy * k;
-          atomicOr(&flag[e >> 5], 1u << (e & 31));
-          if (MODE >= 1) unsafeAtomicAdd(&box[e], fc * c.wx[i] * c.wy[j] * c.wz[k]);
+          if (MODE == 0) atomicOr(&flag[e >> 5], 1u << (e & 31));
+          else unsafeAtomicAdd(&box[e], fc * c.wx[i] * c.wy[j] * c.wz[k]);
         }
   }
   __shared__ int lcount;
@@ -386,6 +390,8 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     }
   }
   __syncthreads();
+  if (MODE == 0)
+    for (int t = threadIdx.x; t < nflag; t += blockDim.x) gflag[t] = flag[t];
   auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
   // cell t = li + bx row of the box walks by IBM_THREADS per step: (li, row)
   // advance by (rb, qb) with a carry instead of a division per cell
@@ -536,6 +542,19 @@ int ipg_ibm_setup(ipg_ctx *c) {
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  // the pass-0 cell flags; a larger box (a second set-up with a larger
+  // marker radius) takes a new buffer, and the captured graphs that hold the
+  // old one are dropped
+  const size_t nflag = 3 * (size_t)c->pc.np * (((size_t)bmx * bmy * bmz + 31) / 32);
+  if (nflag > c->ibm_flg_n) {
+    for (int r = 0; r < 6; r++)
+      if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
+    if (c->ibm_flg) (void)hipFree(c->ibm_flg);
+    c->ibm_flg = nullptr;
+    c->ibm_flg_n = 0;
+    if (hipMalloc(&c->ibm_flg, nflag * sizeof(unsigned)) != hipSuccess) { ipg_set_error("ibm: out of device memory"); return 1; }
+    c->ibm_flg_n = nflag;
+  }
   c->ibm_lds_set = true;
   return 0;
 }
@@ -557,7 +576,7 @@ bool ibm_args(ipg_ctx *c, IbmArgs &A, size_t &lds) {
   A.pomx = q[IPG_POMX]; A.pomy = q[IPG_POMY]; A.pomz = q[IPG_POMZ]; A.pphi = q[IPG_PPHI];
   A.mk = c->mk; A.fl = c->fl;
   A.fld[0] = c->f[IPG_DUDT]; A.fld[1] = c->f[IPG_DVDT]; A.fld[2] = c->f[IPG_DWDT];
-  A.nbl = c->nbl; A.nbc = c->nbc; A.nml = c->nml; A.nmc = c->nmc; A.err = c->ibm_err;
+  A.nbl = c->nbl; A.nbc = c->nbc; A.nml = c->nml; A.nmc = c->nmc; A.err = c->ibm_err; A.flg = c->ibm_flg;
   A.dtp = c->dscal;
   return true;
 }

@@ -73,6 +73,8 @@ struct ipg_ctx {
   int *nbl, *nbc;                 // IBM neighbour images: [np][64] codes, [np] counts
   int *nml, *nmc;                 // IBM neighbour markers per (sphere, component): [3 np][4096], [3 np]
   int *ibm_err;                   // set on the device when a neighbour list overflows
+  unsigned *ibm_flg;              // per (sphere, component) box: cells its markers' stencils touch
+  size_t ibm_flg_n;               // its length in words
   // particles
   PartState ps;
   double *op;                     // [24][np] state at substep start (src/intgr_nwtn_eulr.f90:77-118)
