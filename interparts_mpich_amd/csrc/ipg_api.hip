@@ -76,7 +76,7 @@ int release(ipg_ctx *c) {
   for (int q = 0; q < 64; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
   ipg_poisson_free(c);
-  void *ptrs[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->nml, c->nmc, c->ibm_err, c->ibm_flg, c->op, c->red, c->dscal,
+  void *ptrs[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->nml, c->nmc, c->ibm_err, c->ibm_flg, c->op, c->motion2, c->red, c->dscal,
                   c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   for (int q = 0; q < IPG_NPART; q++) if (c->ps.q[q]) (void)hipFree(c->ps.q[q]);
@@ -192,6 +192,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
         hipMalloc(&c->nmc, (size_t)np * 3 * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->ibm_err, sizeof(int)) != hipSuccess ||
         hipMalloc(&c->op, 24 * (size_t)np * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->motion2, 9 * (size_t)np * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->ps.nbf, (size_t)np * IPG_NB * sizeof(int)) != hipSuccess) {
       ipg_set_error("out of device memory (particles)"); return fail(1);
     }

@@ -78,6 +78,7 @@ struct ipg_ctx {
   // particles
   PartState ps;
   double *op;                     // [24][np] state at substep start (src/intgr_nwtn_eulr.f90:77-118)
+  double *motion2;                // [9][np] positions, velocities, angular velocities after the first collision pass
   double *red;                    // reduction scratch
   double *dscal;                  // device scalars: [0] dt
   double dt_host;
@@ -165,11 +166,6 @@ int ipg_ibm_check(ipg_ctx *c);
 int ipg_launch_post3d(ipg_ctx *c, int q, double *host);
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type);
 int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter);
-int ipg_launch_ne_save(ipg_ctx *c);
-int ipg_launch_ne_sphere_int(ipg_ctx *c);
-int ipg_launch_ne_collide(ipg_ctx *c);
-int ipg_launch_ne_update(ipg_ctx *c, int rkiter);
-int ipg_launch_ne_wrap(ipg_ctx *c);
 int ipg_launch_nbflags(ipg_ctx *c);
 
 // bound masks for ipg_launch_bound

@@ -14,23 +14,11 @@ namespace {
 
 constexpr int NOP = 24;  // saved quantities (src/intgr_nwtn_eulr.f90:80-103)
 // op layout: x y z theta phi u v w omx omy omz omtheta intu intv intw intomx intomy intomz colfx colfy colfz coltx colty coltz
-constexpr int kOpSrc[NOP] = {IPG_PX, IPG_PY, IPG_PZ, IPG_PTHETA, IPG_PPHI, IPG_PU, IPG_PV, IPG_PW,
-                             IPG_POMX, IPG_POMY, IPG_POMZ, IPG_POMTHETA, IPG_PINTU, IPG_PINTV, IPG_PINTW,
-                             IPG_PINTOMX, IPG_PINTOMY, IPG_PINTOMZ, IPG_PCOLFX, IPG_PCOLFY, IPG_PCOLFZ,
-                             IPG_PCOLTX, IPG_PCOLTY, IPG_PCOLTZ};
-
+__constant__ int c_op_src[NOP] = {IPG_PX, IPG_PY, IPG_PZ, IPG_PTHETA, IPG_PPHI, IPG_PU, IPG_PV, IPG_PW,
+                                  IPG_POMX, IPG_POMY, IPG_POMZ, IPG_POMTHETA, IPG_PINTU, IPG_PINTV, IPG_PINTW,
+                                  IPG_PINTOMX, IPG_PINTOMY, IPG_PINTOMZ, IPG_PCOLFX, IPG_PCOLFY, IPG_PCOLFZ,
+                                  IPG_PCOLTX, IPG_PCOLTY, IPG_PCOLTZ};
 struct PQ { double *q[IPG_NPART]; };
-
-// all 24 loads of a sphere in flight together
-__global__ void k_ne_save(PQ P, double *op, int np) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  double v[NOP];
-#pragma unroll
-  for (int s = 0; s < NOP; s++) v[s] = P.q[kOpSrc[s]][p];
-#pragma unroll
-  for (int s = 0; s < NOP; s++) op[(long)s * np + p] = v[s];
-}
 
 // image centre of neighbour slot nbrecv (src/intgr_over_sphere.f90:83-109)
 __device__ __forceinline__ void image_shift(const Grid &g, int nbrecv, double &x, double &y) {
@@ -59,7 +47,7 @@ __device__ __forceinline__ void image_shift(const Grid &g, int nbrecv, double &x
 constexpr int SPH_THREADS = 512, SPH_SLICE = 16 * SPH_THREADS;
 __global__ __launch_bounds__(SPH_THREADS) void k_sphere_int(Grid g, PartConst pc, const double *__restrict__ U,
                                                             const double *__restrict__ V, const double *__restrict__ W, PQ P,
-                                                            const int *__restrict__ nbf, int type) {
+                                                            const int *__restrict__ nbf, int type, double *op_save) {
   const int p = blockIdx.x;
   __shared__ double acc[IPG_NB][6];
   __shared__ double red[6][SPH_THREADS / 64];
@@ -191,6 +179,9 @@ __global__ __launch_bounds__(SPH_THREADS) void k_sphere_int(Grid g, PartConst pc
     double t = 0.0;
     for (int nb = 0; nb < IPG_NB; nb++) t = t + acc[nb][threadIdx.x];
     const int q = threadIdx.x;
+    // with op_save: the integrals at the start of intgr_nwtn_eulr
+    // (src/intgr_nwtn_eulr.f90:77-118, op slots 12..17), before they change
+    if (op_save) op_save[(long)(12 + q) * pc.np + p] = P.q[IPG_PINTU + q][p];
     if (q < 3) {
       if (t1 || t2) P.q[IPG_PINTU + q][p] = t;
     } else if (t3) {
@@ -268,6 +259,85 @@ __device__ __forceinline__ void nb_copy(const Grid &g, int nb, double &x, double
   }
 }
 
+// Positions, velocities and angular velocities of the spheres: the state the
+// collision pass reads and the position update writes.  The first of the two
+// passes of src/intgr_nwtn_eulr.f90:149-680 writes into a second buffer, so
+// that the pass reads one state throughout and the next pass reads the
+// updated one.
+struct Motion { double *x, *y, *z, *u, *v, *w, *ox, *oy, *oz; };
+
+// src/intgr_nwtn_eulr.f90:611-667 for sphere p with collision force (fx, fy,
+// fz): the new position, velocity, angular velocity into dst; colf, phi,
+// theta, omtheta and the sumrk3 sums into the particle state.
+__device__ __forceinline__ void ne_update_one(const PartConst &pc, const PQ &P, const double *__restrict__ op, double fx,
+                                              double fy, double fz, double dt, int rkiter, int p, const Motion &dst) {
+  const int np = pc.np;
+  auto O = [&](int s) { return op[(long)s * np + p]; };
+  double *const *q = P.q;
+  const double rkab = pc.rkab[rkiter - 1];
+  q[IPG_PCOLFX][p] = fx; q[IPG_PCOLFY][p] = fy; q[IPG_PCOLFZ][p] = fz;
+  q[IPG_PCOLTX][p] = 0.0; q[IPG_PCOLTY][p] = 0.0; q[IPG_PCOLTZ][p] = 0.0;
+  const double vr = pc.volp * pc.ratiorho, rr = pc.ratiorho;
+  const double u = O(5) + (1.0 - 0.0) * ((-1.0) * dt * (q[IPG_PFXLTOT][p]) / vr + (q[IPG_PINTU][p] - O(12)) / vr) +
+                   rkab * dt * pc.gaccx * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (fx + O(18)) / vr;
+  const double x = O(0) + rkab * 0.5 * dt * (u + O(5));
+  const double v = O(6) + (1.0 - 0.0) * ((-1.0) * dt * (q[IPG_PFYLTOT][p]) / vr + (q[IPG_PINTV][p] - O(13)) / vr) +
+                   rkab * dt * pc.gaccy * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (fy + O(19)) / vr;
+  const double y = O(1) + rkab * 0.5 * dt * (v + O(6));
+  const double w = O(7) + (1.0 - 0.0) * ((-1.0) * dt * (q[IPG_PFZLTOT][p]) / vr + (q[IPG_PINTW][p] - O(14)) / vr) +
+                   rkab * dt * pc.gaccz * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (fz + O(20)) / vr;
+  const double z = O(2) + rkab * 0.5 * dt * (w + O(7));
+  dst.u[p] = u; dst.x[p] = x; dst.v[p] = v; dst.y[p] = y; dst.w[p] = w; dst.z[p] = z;
+  double ox = 0.0, oy = 0.0, oz = 0.0;
+  if (pc.rotation) {
+    const double mr = pc.mominert * pc.ratiorho;
+    ox = O(8) + (-1.0) * dt * q[IPG_PTORQX][p] / mr + (q[IPG_PINTOMX][p] - O(15)) / mr + rkab * 0.5 * dt * (0.0 + O(21)) / mr;
+    oy = O(9) + (-1.0) * dt * q[IPG_PTORQY][p] / mr + (q[IPG_PINTOMY][p] - O(16)) / mr + rkab * 0.5 * dt * (0.0 + O(22)) / mr;
+    oz = O(10) + (-1.0) * dt * q[IPG_PTORQZ][p] / mr + (q[IPG_PINTOMZ][p] - O(17)) / mr + rkab * 0.5 * dt * (0.0 + O(23)) / mr;
+  }
+  dst.ox[p] = ox; dst.oy[p] = oy; dst.oz[p] = oz;
+  const double phi = O(4) + 0.5 * rkab * dt * (oz + O(10));
+  q[IPG_PPHI][p] = phi;
+  const double omt = (oy * cos(phi)) - (ox * sin(phi));
+  q[IPG_POMTHETA][p] = omt;
+  q[IPG_PTHETA][p] = O(3) + rkab * 0.5 * dt * (omt + O(11));
+  // sumrk3 (src/intgr_nwtn_eulr.f90:1586-1669), called on both passes of the
+  // collision loop: the rkiter = 1 reset happens on each pass.
+  const bool first = rkiter == 1;
+  q[IPG_PRKFX][p] = (first ? 0.0 : q[IPG_PRKFX][p]) + (-1.0) * q[IPG_PFXLTOT][p] + (q[IPG_PINTU][p] - O(12)) / dt;
+  q[IPG_PRKFY][p] = (first ? 0.0 : q[IPG_PRKFY][p]) + (-1.0) * q[IPG_PFYLTOT][p] + (q[IPG_PINTV][p] - O(13)) / dt;
+  q[IPG_PRKFZ][p] = (first ? 0.0 : q[IPG_PRKFZ][p]) + (-1.0) * q[IPG_PFZLTOT][p] + (q[IPG_PINTW][p] - O(14)) / dt;
+  q[IPG_PRKDUDT][p] = (first ? 0.0 : q[IPG_PRKDUDT][p]) + (u - O(5)) / dt;
+  q[IPG_PRKDVDT][p] = (first ? 0.0 : q[IPG_PRKDVDT][p]) + (v - O(6)) / dt;
+  q[IPG_PRKDWDT][p] = (first ? 0.0 : q[IPG_PRKDWDT][p]) + (w - O(7)) / dt;
+}
+
+// src/intgr_nwtn_eulr.f90:702-714 periodic wrap (when wrap), then the
+// neighbour flags of src/intgr_nwtn_eulr.f90:1076-1111 (= src/initparticles.f90:200-241)
+// at one rank, for sphere p.
+__device__ __forceinline__ void wrap_flags_one(const Grid &g, const PartConst &pc, const PQ &P, int *nbf, int wrap, int p) {
+  double x = P.q[IPG_PX][p], y = P.q[IPG_PY][p];
+  if (wrap) {
+    if (x > g.lx) x = x - g.lx;
+    if (x < 0.0) x = x + g.lx;
+    if (y > g.ly) y = y - g.ly;
+    if (y < 0.0) y = y + g.ly;
+    P.q[IPG_PX][p] = x; P.q[IPG_PY][p] = y;
+  }
+  const double ro = pc.radius + pc.offset;
+  const double left = 0.0 * g.lx / 1.0, right = 1.0 * g.lx / 1.0, front = 0.0 * g.ly / 1.0, back = 1.0 * g.ly / 1.0;
+  int *f = nbf + p * IPG_NB;
+  f[0] = 0;
+  f[1] = x > (right - ro);
+  f[2] = fabs(sqrt(sqd(right - x) + sqd(front - y))) < ro;
+  f[3] = y < (front + ro);
+  f[4] = fabs(sqrt(sqd(left - x) + sqd(front - y))) < ro;
+  f[5] = x < (left + ro);
+  f[6] = fabs(sqrt(sqd(left - x) + sqd(back - y))) < ro;
+  f[7] = y > (back - ro);
+  f[8] = fabs(sqrt(sqd(right - x) + sqd(back - y))) < ro;
+}
+
 constexpr int MAXCAND = 48;
 
 // Collision model, src/intgr_nwtn_eulr.f90:338-606: for each sphere the
@@ -286,8 +356,15 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-__global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc, PQ P, double *cfx, double *cfy, double *cfz,
-                                                            int *err) {
+// One collision pass with the position update of src/intgr_nwtn_eulr.f90:
+// 611-672 fused: lane 0 of sphere p's wavefront updates p as soon as its
+// force is summed.  The pass reads the state src and writes the update to dst
+// (a different buffer), so no wavefront reads a state another one is
+// updating; the last pass also wraps the position and sets the neighbour
+// flags (src/intgr_nwtn_eulr.f90:702-714, 1076-1111).
+__global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc, PQ P, Motion src, Motion dst,
+                                                            double *__restrict__ op, const double *__restrict__ dtp,
+                                                            int rkiter, int last, int *nbf, int *err) {
   __shared__ double cx[COL_CHUNK], cy[COL_CHUNK], cz[COL_CHUNK];
   __shared__ int cand[COL_WAVES][MAXCAND];
   __shared__ double cd[COL_WAVES][MAXCAND][9];
@@ -296,7 +373,7 @@ __global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * COL_WAVES + wv;
   const int np = pc.np;
-  const double *X = P.q[IPG_PX], *Y = P.q[IPG_PY], *Zq = P.q[IPG_PZ], *Uq = P.q[IPG_PU], *Vq = P.q[IPG_PV], *Wq = P.q[IPG_PW];
+  const double *X = src.x, *Y = src.y, *Zq = src.z, *Uq = src.u, *Vq = src.v, *Wq = src.w;
   const bool live = p < np;
   const double xp = live ? X[p] : 0.0, yp = live ? Y[p] : 0.0, zp = live ? Zq[p] : 0.0;
   const double R = pc.radius, lx = g.lx, ly = g.ly;
@@ -337,11 +414,11 @@ __global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc
     const int q = cand[wv][c];
     double *e = cd[wv][c];
     e[0] = X[q]; e[1] = Y[q]; e[2] = Zq[q]; e[3] = Uq[q]; e[4] = Vq[q]; e[5] = Wq[q];
-    e[6] = P.q[IPG_POMX][q]; e[7] = P.q[IPG_POMY][q]; e[8] = P.q[IPG_POMZ][q];
+    e[6] = src.ox[q]; e[7] = src.oy[q]; e[8] = src.oz[q];
   }
   wave_lds_sync();
   const double up = Uq[p], vp = Vq[p], wp = Wq[p];
-  const double omx = P.q[IPG_POMX][p], omy = P.q[IPG_POMY][p], omz = P.q[IPG_POMZ][p];
+  const double omx = src.ox[p], omy = src.oy[p], omz = src.oz[p];
   double fx = 0.0, fy = 0.0, fz = 0.0;
   // the terms of the ordered scan (nb outer, candidate inner) one per lane,
   // 64 at a time; lane 0 then adds them in that order, as the reference does
@@ -402,80 +479,21 @@ __global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc
     if ((eps < 0.075) && (eps > 0.0)) lubrication(pc, up, vp, wp, true, nx, ny, nz, eps, 0.0, 0.0, 0.0, fx, fy, fz);
     if (dist < R) collision(pc, up, vp, wp, omx, omy, omz, true, dist, deltax, deltay, deltaz, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, fx, fy, fz);
   }
-  cfx[p] = fx; cfy[p] = fy; cfz[p] = fz;
-}
-
-// src/intgr_nwtn_eulr.f90:611-667
-__global__ void k_ne_update(Grid g, PartConst pc, PQ P, const double *__restrict__ op, const double *cfx, const double *cfy,
-                            const double *cfz, const double *__restrict__ dtp, int rkiter) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int np = pc.np;
-  if (p >= np) return;
-  auto O = [&](int s) { return op[(long)s * np + p]; };
-  double **q = P.q;
-  const double dt = *dtp, rkab = pc.rkab[rkiter - 1];
-  q[IPG_PCOLFX][p] = cfx[p]; q[IPG_PCOLFY][p] = cfy[p]; q[IPG_PCOLFZ][p] = cfz[p];
-  q[IPG_PCOLTX][p] = 0.0; q[IPG_PCOLTY][p] = 0.0; q[IPG_PCOLTZ][p] = 0.0;
-  const double vr = pc.volp * pc.ratiorho, rr = pc.ratiorho;
-  const double u = O(5) + (1.0 - 0.0) * ((-1.0) * dt * (q[IPG_PFXLTOT][p]) / vr + (q[IPG_PINTU][p] - O(12)) / vr) +
-                   rkab * dt * pc.gaccx * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (cfx[p] + O(18)) / vr;
-  const double x = O(0) + rkab * 0.5 * dt * (u + O(5));
-  const double v = O(6) + (1.0 - 0.0) * ((-1.0) * dt * (q[IPG_PFYLTOT][p]) / vr + (q[IPG_PINTV][p] - O(13)) / vr) +
-                   rkab * dt * pc.gaccy * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (cfy[p] + O(19)) / vr;
-  const double y = O(1) + rkab * 0.5 * dt * (v + O(6));
-  const double w = O(7) + (1.0 - 0.0) * ((-1.0) * dt * (q[IPG_PFZLTOT][p]) / vr + (q[IPG_PINTW][p] - O(14)) / vr) +
-                   rkab * dt * pc.gaccz * (1.0 - (1.0 / rr)) + rkab * 0.5 * dt * (cfz[p] + O(20)) / vr;
-  const double z = O(2) + rkab * 0.5 * dt * (w + O(7));
-  q[IPG_PU][p] = u; q[IPG_PX][p] = x; q[IPG_PV][p] = v; q[IPG_PY][p] = y; q[IPG_PW][p] = w; q[IPG_PZ][p] = z;
-  double ox = 0.0, oy = 0.0, oz = 0.0;
-  if (pc.rotation) {
-    const double mr = pc.mominert * pc.ratiorho;
-    ox = O(8) + (-1.0) * dt * q[IPG_PTORQX][p] / mr + (q[IPG_PINTOMX][p] - O(15)) / mr + rkab * 0.5 * dt * (0.0 + O(21)) / mr;
-    oy = O(9) + (-1.0) * dt * q[IPG_PTORQY][p] / mr + (q[IPG_PINTOMY][p] - O(16)) / mr + rkab * 0.5 * dt * (0.0 + O(22)) / mr;
-    oz = O(10) + (-1.0) * dt * q[IPG_PTORQZ][p] / mr + (q[IPG_PINTOMZ][p] - O(17)) / mr + rkab * 0.5 * dt * (0.0 + O(23)) / mr;
+  if (!last) {
+    // the first pass saves the rest of the state at the start of
+    // intgr_nwtn_eulr (src/intgr_nwtn_eulr.f90:77-118; the integrals, slots
+    // 12..17, were saved by the sphere-integral launch) before updating it
+    for (int s = 0; s < NOP; s++)
+      if (s < 12 || s > 17) op[(long)s * np + p] = P.q[c_op_src[s]][p];
   }
-  q[IPG_POMX][p] = ox; q[IPG_POMY][p] = oy; q[IPG_POMZ][p] = oz;
-  const double phi = O(4) + 0.5 * rkab * dt * (oz + O(10));
-  q[IPG_PPHI][p] = phi;
-  const double omt = (oy * cos(phi)) - (ox * sin(phi));
-  q[IPG_POMTHETA][p] = omt;
-  q[IPG_PTHETA][p] = O(3) + rkab * 0.5 * dt * (omt + O(11));
-  // sumrk3 (src/intgr_nwtn_eulr.f90:1586-1669), called on both passes of the
-  // collision loop: the rkiter = 1 reset happens on each pass.
-  const bool first = rkiter == 1;
-  q[IPG_PRKFX][p] = (first ? 0.0 : q[IPG_PRKFX][p]) + (-1.0) * q[IPG_PFXLTOT][p] + (q[IPG_PINTU][p] - O(12)) / dt;
-  q[IPG_PRKFY][p] = (first ? 0.0 : q[IPG_PRKFY][p]) + (-1.0) * q[IPG_PFYLTOT][p] + (q[IPG_PINTV][p] - O(13)) / dt;
-  q[IPG_PRKFZ][p] = (first ? 0.0 : q[IPG_PRKFZ][p]) + (-1.0) * q[IPG_PFZLTOT][p] + (q[IPG_PINTW][p] - O(14)) / dt;
-  q[IPG_PRKDUDT][p] = (first ? 0.0 : q[IPG_PRKDUDT][p]) + (u - O(5)) / dt;
-  q[IPG_PRKDVDT][p] = (first ? 0.0 : q[IPG_PRKDVDT][p]) + (v - O(6)) / dt;
-  q[IPG_PRKDWDT][p] = (first ? 0.0 : q[IPG_PRKDWDT][p]) + (w - O(7)) / dt;
+  ne_update_one(pc, P, op, fx, fy, fz, *dtp, rkiter, p, dst);
+  if (last) wrap_flags_one(g, pc, P, nbf, 1, p);
 }
 
-// src/intgr_nwtn_eulr.f90:702-714 periodic wrap, then neighbour flags of
-// src/intgr_nwtn_eulr.f90:1076-1111 (= src/initparticles.f90:200-241) at one rank.
 __global__ void k_wrap_flags(Grid g, PartConst pc, PQ P, int *nbf, int wrap) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= pc.np) return;
-  double x = P.q[IPG_PX][p], y = P.q[IPG_PY][p];
-  if (wrap) {
-    if (x > g.lx) x = x - g.lx;
-    if (x < 0.0) x = x + g.lx;
-    if (y > g.ly) y = y - g.ly;
-    if (y < 0.0) y = y + g.ly;
-    P.q[IPG_PX][p] = x; P.q[IPG_PY][p] = y;
-  }
-  const double ro = pc.radius + pc.offset;
-  const double left = 0.0 * g.lx / 1.0, right = 1.0 * g.lx / 1.0, front = 0.0 * g.ly / 1.0, back = 1.0 * g.ly / 1.0;
-  int *f = nbf + p * IPG_NB;
-  f[0] = 0;
-  f[1] = x > (right - ro);
-  f[2] = fabs(sqrt(sqd(right - x) + sqd(front - y))) < ro;
-  f[3] = y < (front + ro);
-  f[4] = fabs(sqrt(sqd(left - x) + sqd(front - y))) < ro;
-  f[5] = x < (left + ro);
-  f[6] = fabs(sqrt(sqd(left - x) + sqd(back - y))) < ro;
-  f[7] = y > (back - ro);
-  f[8] = fabs(sqrt(sqd(right - x) + sqd(back - y))) < ro;
+  wrap_flags_one(g, pc, P, nbf, wrap, p);
 }
 
 PQ make_pq(ipg_ctx *c) {
@@ -489,7 +507,7 @@ PQ make_pq(ipg_ctx *c) {
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type) {
   if (c->pc.np <= 0) return 0;
   hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(SPH_THREADS), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V],
-                     c->f[IPG_W], make_pq(c), c->ps.nbf, type);
+                     c->f[IPG_W], make_pq(c), c->ps.nbf, type, (double *)nullptr);
   IPG_HIP(hipGetLastError());
   return 0;
 }
@@ -501,67 +519,30 @@ int ipg_launch_nbflags(ipg_ctx *c) {
   return 0;
 }
 
-// The pieces of intgr_nwtn_eulr (src/intgr_nwtn_eulr.f90:12-714), each on
-// c->stream: the saved state op (77-118), the sphere integrals of types 2 and
-// 3 (120-121), one collision pass (338-606) and one position update
-// (611-672), the periodic wrap and neighbour flags (702-714);
-// ipg_launch_nwtn_eulr runs them in the reference's order.
-int ipg_launch_ne_save(ipg_ctx *c) {
-  const int np = c->pc.np;
-  if (np <= 0) return 0;
-  hipLaunchKernelGGL(k_ne_save, dim3((np + 255) / 256), dim3(256), 0, c->stream, make_pq(c), c->op, np);
-  ipg_mark(c, "ne_save");
-  IPG_HIP(hipGetLastError());
-  return 0;
-}
-
-int ipg_launch_ne_sphere_int(ipg_ctx *c) {
-  if (c->pc.np <= 0) return 0;
-  hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(SPH_THREADS), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V],
-                     c->f[IPG_W], make_pq(c), c->ps.nbf, 23);
-  ipg_mark(c, "sphere_int");
-  IPG_HIP(hipGetLastError());
-  return 0;
-}
-
-// collision forces into the 3*np scratch after the reduction slots, with an
-// overflow flag after them
-int ipg_launch_ne_collide(ipg_ctx *c) {
-  const int np = c->pc.np;
-  if (np <= 0) return 0;
-  double *cf = c->red;
-  int *err = (int *)(c->red + 3L * np);
-  hipLaunchKernelGGL(k_collide, dim3((np + COL_WAVES - 1) / COL_WAVES), dim3(COL_WAVES * 64), 0, c->stream, c->g, c->pc,
-                     make_pq(c), cf, cf + np, cf + 2 * np, err);
-  ipg_mark(c, "collide");
-  IPG_HIP(hipGetLastError());
-  return 0;
-}
-
-int ipg_launch_ne_update(ipg_ctx *c, int rkiter) {
-  const int np = c->pc.np;
-  if (np <= 0) return 0;
-  double *cf = c->red;
-  hipLaunchKernelGGL(k_ne_update, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->g, c->pc, make_pq(c), c->op, cf, cf + np,
-                     cf + 2 * np, c->dscal, rkiter);
-  ipg_mark(c, "ne_update");
-  IPG_HIP(hipGetLastError());
-  return 0;
-}
-
-int ipg_launch_ne_wrap(ipg_ctx *c) {
-  const int np = c->pc.np;
-  if (np <= 0) return 0;
-  hipLaunchKernelGGL(k_wrap_flags, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->g, c->pc, make_pq(c), c->ps.nbf, 1);
-  ipg_mark(c, "wrap_flags");
-  IPG_HIP(hipGetLastError());
-  return 0;
-}
-
+// intgr_nwtn_eulr (src/intgr_nwtn_eulr.f90:12-714) in three launches: the
+// sphere integrals of types 2 and 3 (120-121) with the saved state op
+// (77-118); the first collision pass with its position update (338-672),
+// into the second motion buffer; the second pass back into the particle
+// state, with the periodic wrap and the neighbour flags (702-714).
 int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter) {
-  if (c->pc.np <= 0) return 0;
-  if (ipg_launch_ne_save(c) || ipg_launch_ne_sphere_int(c)) return 1;
-  for (int iter = 0; iter <= 1; iter++)
-    if (ipg_launch_ne_collide(c) || ipg_launch_ne_update(c, rkiter)) return 1;
-  return ipg_launch_ne_wrap(c);
+  const int np = c->pc.np;
+  if (np <= 0) return 0;
+  const PQ P = make_pq(c);
+  hipLaunchKernelGGL(k_sphere_int, dim3(np), dim3(SPH_THREADS), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V],
+                     c->f[IPG_W], P, c->ps.nbf, 23, c->op);
+  ipg_mark(c, "sphere_int");
+  double **q = c->ps.q;
+  const Motion state{q[IPG_PX], q[IPG_PY], q[IPG_PZ], q[IPG_PU], q[IPG_PV], q[IPG_PW], q[IPG_POMX], q[IPG_POMY], q[IPG_POMZ]};
+  double *m = c->motion2;
+  const Motion second{m, m + np, m + 2L * np, m + 3L * np, m + 4L * np, m + 5L * np, m + 6L * np, m + 7L * np, m + 8L * np};
+  int *err = (int *)(c->red + 3L * np);
+  const dim3 grid((np + COL_WAVES - 1) / COL_WAVES), blk(COL_WAVES * 64);
+  hipLaunchKernelGGL(k_collide, grid, blk, 0, c->stream, c->g, c->pc, P, state, second, c->op, c->dscal, rkiter, 0,
+                     c->ps.nbf, err);
+  ipg_mark(c, "collide_update");
+  hipLaunchKernelGGL(k_collide, grid, blk, 0, c->stream, c->g, c->pc, P, second, state, c->op, c->dscal, rkiter, 1,
+                     c->ps.nbf, err);
+  ipg_mark(c, "collide_update");
+  IPG_HIP(hipGetLastError());
+  return 0;
 }
