@@ -36,7 +36,9 @@ def kernel_bytes(case, name, rkiter):
         "fillps": 4.0 * N,
         "fft_x_fwd": 3.0 * N + Sw,    # fillps fused: reads dudt dvdt dwdt, writes the half spectrum
         "fft_y_fwd": 2.0 * Sw,
-        "thomas": 5.0 * Sw,           # spectrum read + written in each sweep, d written then read
+        # spectrum read + written in each sweep; d (one word per mode and plane, shared by the
+        # y-modes +-ky) written then read
+        "thomas": 4.0 * Sw + Sw * (case.jtot // 2 + 1) / case.jtot,
         "fft_y_inv": 2.0 * Sw,
         "fft_x_inv": Sw + N,
         "correc": 9.0 * N,            # dudt dvdt dwdt p pnew -> u v w pnew (cnew = dcdt is a buffer swap)

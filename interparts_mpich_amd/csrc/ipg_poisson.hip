@@ -418,15 +418,24 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
                                                 const double *__restrict__ a, const double *__restrict__ b,
                                                 const double *__restrict__ cc, const double *__restrict__ xrt,
                                                 const double *__restrict__ yrt) {
+  // lane u: part (bit 0), x-mode m = 16 mb + (bits 1-4), side (bit 5), and
+  // the y-mode pair pr (u >> 6 = pr nmb + mb).  The y-modes pr and ny - pr
+  // have the same eigenvalue (both fold to |ky|), hence the same pivots: the
+  // two sides of a wavefront share one row of multipliers d, written by side
+  // 0 and read by both.
   const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long nmode = (long)nmx * g.ny;
-  if (u >= 2 * nmode) return;
-  const long t = u >> 1;
-  const int part = (int)(u & 1);
-  const int my = (int)(t / nmx), m = (int)(t - (long)my * nmx);
+  const int nmb = (nmx + 15) / 16;
+  const int part = (int)(u & 1), ml = (int)((u >> 1) & 15), side = (int)((u >> 5) & 1);
+  const long grp = u >> 6;
+  const int mb = (int)(grp % nmb), pr = (int)(grp / nmb);
+  const int m = mb * 16 + ml;
+  if (pr > g.ny / 2 || m >= nmx) return;
+  if (side == 1 && (pr == 0 || 2 * pr == g.ny)) return;  // self-paired y-modes
+  const int my = side == 0 ? pr : g.ny - pr;
   const int nz = g.nz;
   const long plane = (long)g.ny * sp;                 // complex elements per z plane
   const long base = (long)my * sp + m;                // complex index in plane 0
+  const long dbase = (long)pr * sp + m;               // the pair's row of d
   double *S = spec + part;                            // element (k, base): S[2 * (base + k plane)]
   const double xy = xrt[m] + yrt[my];
   // forward elimination, k = 1 .. nz-1 (z = 1/(b + xy - a dprev))
@@ -454,7 +463,9 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
         if (k == 1) z = 1.0 / (b[1] + xy);
         else z = 1.0 / ((b[k] + xy) - a[k] * dprev);
         dprev = cc[k] * z;
-        dtri[q] = dprev;  // both parts store the same value; each reads back its own store
+        // side 0's two parts store the same value; side 1 reads it back in
+        // the back substitution of this wavefront
+        if (side == 0) dtri[dbase + (long)(k - 1) * plane] = dprev;
         const double w = k == 1 ? v[r] * z : (v[r] - a[k] * pp) * z;
         S[2 * q] = w;
         pp = w;
@@ -468,14 +479,16 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
     pp = z != 0.0 ? (v - a[nz] * pp) / z : 0.0;
     S[2 * q] = pp;
   }
-  // back substitution, k = nz-1 .. 1
+  // back substitution, k = nz-1 .. 1; side 1 reads the d that side 0 of this
+  // wavefront stored
+  __threadfence_block();
   double vb[TD], db[TD];
 #pragma unroll
   for (int r = 0; r < TD; r++) {
     const int k = nz - 1 - r;
     const long q = base + (long)(k - 1) * plane;
     vb[r] = k >= 1 ? S[2 * q] : 0.0;
-    db[r] = k >= 1 ? dtri[q] : 0.0;
+    db[r] = k >= 1 ? dtri[dbase + (long)(k - 1) * plane] : 0.0;
   }
   for (int k0 = nz - 1; k0 >= 1; k0 -= TD) {
     double v[TD], d[TD];
@@ -487,7 +500,7 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
         const int k = k0 - TD - r;
         const long q = base + (long)(k - 1) * plane;
         vb[r] = k >= 1 ? S[2 * q] : 0.0;
-        db[r] = k >= 1 ? dtri[q] : 0.0;
+        db[r] = k >= 1 ? dtri[dbase + (long)(k - 1) * plane] : 0.0;
       }
     }
 #pragma unroll
@@ -666,7 +679,8 @@ int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter) {
     hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, -1);
   }
   ipg_mark(c, "fft_y_fwd");
-  const long nthomas = 2L * P.nmx * g.ny;
+  // lanes: 64 per (block of 16 x-modes, y-mode pair pr = 0..ny/2)
+  const long nthomas = 64L * ((P.nmx + 15) / 16) * (g.ny / 2 + 1);
   hipLaunchKernelGGL(k_thomas, dim3((unsigned)((nthomas + 255) / 256)), dim3(256), 0, c->stream, g, (double *)P.spec, P.dtri, P.sp, P.nmx,
                      P.a, P.b, P.c, P.xrt, P.yrt);
   ipg_mark(c, "thomas");
