@@ -64,7 +64,7 @@ struct IbmArgs {
   const int *nbc;     // [np] neighbour image count
   int *nml;           // [np*3][IBM_MAXNM] neighbour markers touching the box: (q*nl + l)*25 + shift
   int *nmc;           // [np*3] their count
-  int *err;           // bit 1: a neighbour list overflowed
+  int *err;           // bits 1, 2: a neighbour list overflowed; 4: a stencil outside its box
   unsigned *flg;      // [np*3][nflagmax] the box cells the own stencils touch, set by pass 0
   double *Kread, *Kspread, *Kzero;  // dwdtf(.,.,k1) planes
   const double *dtp;
@@ -93,21 +93,65 @@ __device__ __forceinline__ int wrapany(int i, int n) {
   return r + 1;
 }
 
-// The per-sphere box of one velocity component: cells [b0, b0+bn) per
-// direction in unwrapped index space (z clipped to the channel, 1..nz).  It
-// holds every stencil cell of the sphere's markers with one cell to spare.
+// Cells [b0, b0+bn) per direction in unwrapped index space (z clipped to the
+// channel, 1..nz).
 struct Box { int bi0, bj0, bk0, bx, by, bz; };
-__device__ __forceinline__ Box sphere_box(const IbmArgs &A, double xp, double yp, double zp) {
+
+// The reach of a sphere: a box holding every stencil cell of its markers, of
+// any component (cells floor(X) - 1 .. floor(X) + 2 for a marker at X cells),
+// with one cell to spare at each end.  Used to find the spheres whose markers
+// may reach another sphere's boxes.
+__device__ __forceinline__ Box reach_box(const IbmArgs &A, double xp, double yp, double zp) {
   const Grid &g = A.g;
   Box b;
   const int x0 = (int)floor((xp - A.radfp) * g.dxi), x1 = (int)floor((xp + A.radfp) * g.dxi);
   const int y0 = (int)floor((yp - A.radfp) * g.dyi), y1 = (int)floor((yp + A.radfp) * g.dyi);
   const int z0 = (int)floor((zp - A.radfp) * g.dzi), z1 = (int)floor((zp + A.radfp) * g.dzi);
-  b.bi0 = x0 - 2; b.bx = min(x1 + 3 - b.bi0 + 1, A.bmx);
-  b.bj0 = y0 - 2; b.by = min(y1 + 3 - b.bj0 + 1, A.bmy);
+  b.bi0 = x0 - 2; b.bx = x1 + 3 - b.bi0 + 1;
+  b.bj0 = y0 - 2; b.by = y1 + 3 - b.bj0 + 1;
   b.bk0 = max(z0 - 2, 1);
-  b.bz = min(min(z1 + 3, g.nz) - b.bk0 + 1, A.bmz);
-  if (b.bz < 0) b.bz = 0;
+  b.bz = max(min(z1 + 3, g.nz) - b.bk0 + 1, 0);
+  return b;
+}
+
+// First stencil cell of one direction over the markers of a sphere, [lo, hi].
+// A marker at X = x / h cells (x = centre + offset, offset in [-radfp,
+// radfp]) starts its stencil at floor(X) in a cell-centred direction and at
+// nint(X) - 1 in the component's own (face) direction.  Xa = (centre -
+// radfp) / h and Xb = (centre + radfp) / h bound X, but each of X, Xa and Xb
+// is rounded, and a rounding can push a floor or nint over an integer by one
+// (upward only for X, whose wrapped coordinate is positive); hence one cell
+// more than the exact bounds at each end:
+//   cell-centred: [floor(Xa) - 1, floor(Xb) + 1]
+//   face:         [floor(Xa + 1/2) - 2, floor(Xb + 1/2)]
+// so the stencils span at most ceil(2 radfp / h) + 5 cells (ibm_box_max).
+__device__ __forceinline__ void first_cell_range(double lo_pos, double hi_pos, double hinv, bool face, int &lo, int &hi) {
+  if (face) {
+    lo = (int)floor(lo_pos * hinv + 0.5) - 2;
+    hi = (int)floor(hi_pos * hinv + 0.5);
+  } else {
+    lo = (int)floor(lo_pos * hinv) - 1;
+    hi = (int)floor(hi_pos * hinv) + 1;
+  }
+}
+
+// The box of one (sphere, velocity component): the cells its markers'
+// stencils of that component can touch, z clipped to 1..nz (the planes above
+// kmax are the k1 plane, outside every box).  The first z plane of a stencil
+// is clamped to 1 (and to nz - 2 for u and v) as comp_stencil clamps it.
+__device__ __forceinline__ Box comp_box(const IbmArgs &A, double xp, double yp, double zp, int comp) {
+  const Grid &g = A.g;
+  int x0, x1, y0, y1, z0, z1;
+  first_cell_range(xp - A.radfp, xp + A.radfp, g.dxi, comp == 0, x0, x1);
+  first_cell_range(yp - A.radfp, yp + A.radfp, g.dyi, comp == 1, y0, y1);
+  first_cell_range(zp - A.radfp, zp + A.radfp, g.dzi, comp == 2, z0, z1);
+  if (comp != 2) { z0 = min(z0, g.nz - 2); z1 = min(z1, g.nz - 2); }
+  z0 = max(z0, 1);
+  z1 = max(z1, 1);
+  Box b;
+  b.bi0 = x0; b.bx = x1 + 2 - x0 + 1;
+  b.bj0 = y0; b.by = y1 + 2 - y0 + 1;
+  b.bk0 = z0; b.bz = max(min(z1 + 2, g.nz) - z0 + 1, 0);
   return b;
 }
 
@@ -181,10 +225,10 @@ __global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int
   __shared__ int count;
   if (threadIdx.x == 0) count = 0;
   __syncthreads();
-  const Box bp = sphere_box(A, A.px[p], A.py[p], A.pz[p]);
+  const Box bp = reach_box(A, A.px[p], A.py[p], A.pz[p]);
   for (int q = threadIdx.x; q < A.np; q += blockDim.x) {
     if (q == p) continue;
-    const Box bq = sphere_box(A, A.px[q], A.py[q], A.pz[q]);
+    const Box bq = reach_box(A, A.px[q], A.py[q], A.pz[q]);
     if (!overlap(bp.bk0, bp.bz, bq.bk0, bq.bz)) continue;
     unsigned mx = 0u, my = 0u;
 #pragma unroll
@@ -258,7 +302,12 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   const double dt = *A.dtp, dti = 1.0 / dt;
   const double c0 = (A.dVlagr / A.dVeul) * dt;
   const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
-  const Box b = sphere_box(A, xp, yp, zp);
+  Box b = comp_box(A, xp, yp, zp, comp);
+  if (b.bx > A.bmx || b.by > A.bmy || b.bz > A.bmz) {
+    // cannot happen for markers within radfp of the centre; kept as a check
+    if (MODE == 0 && threadIdx.x == 0) atomicOr(A.err, 4);
+    b.bx = min(b.bx, A.bmx); b.by = min(b.by, A.bmy); b.bz = min(b.bz, A.bmz);
+  }
   const int nbox = b.bx * b.by * b.bz, bxy = b.bx * b.by;
   const int nflag = (nbox + 31) >> 5, nflagmax = (A.bmx * A.bmy * A.bmz + 31) >> 5;
   unsigned *flag = (unsigned *)(box + A.bmx * A.bmy * A.bmz);
@@ -277,15 +326,18 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   // cell, how many of its z planes lie in the box (the rest is the k1 plane),
   // and the rigid-body velocity ub.  Recomputed in each phase that needs it
   // rather than kept live across the launch.  The box holds the stencil in x
-  // and y, and in z up to kmax (sphere_box); a stencil that were not held
-  // would be left out rather than index outside the box.
+  // and y, and in z up to kmax (comp_box); a stencil that were not held is
+  // reported (error bit 4) and left out rather than index outside the box.
   auto own = [&](int q, CompStencil &c, int &e0, int &nzin, double &ub) -> bool {
     const int l = (int)threadIdx.x + q * IBM_THREADS;
     double xl, yl, zl;
     if (l >= A.nl || !marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return false;
     comp_stencil(g, xl, yl, zl, comp, c);
     e0 = box_at(b, c.i0, c.j0, c.k0);
-    if (e0 < 0 || box_at(b, c.i0 + 2, c.j0 + 2, c.k0) < 0) return false;
+    if (e0 < 0 || box_at(b, c.i0 + 2, c.j0 + 2, c.k0) < 0) {
+      if (MODE == 0) atomicOr(A.err, 4);
+      return false;
+    }
     nzin = min(3, b.bk0 + b.bz - c.k0);
     if (comp == 0) ub = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
     else if (comp == 1) ub = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
@@ -306,10 +358,59 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   for (int t = threadIdx.x; t < b.bx; t += blockDim.x) icol[t] = wrapany(b.bi0 + t, g.nx) - 1;
   if (MODE >= 1)
     for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
+  __shared__ int lcount;
+  if (MODE == 0 && threadIdx.x == 0) lcount = 0;
   __syncthreads();
+  const long lbase = (long)blockIdx.x * IBM_MAXNM;
+  if (MODE == 0) {
+    // list the neighbours' markers whose stencils touch this box; the
+    // markers do not move inside the IBM block, so the three launches that
+    // spread them reuse the list
+    const int nn = A.nbc[p];
+    const int tot = nn * A.nl;
+    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+      const int n = idx / A.nl, l = idx - n * A.nl;
+      const int code = A.nbl[(long)p * IBM_MAXNB + n];
+      const int qs = code / 25, sh = code % 25;
+      CompStencil c;
+      const bool keep = neighbour_stencil<false>(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c);
+      // one LDS atomic per wavefront
+      const unsigned long long msk = __ballot(keep);
+      if (!keep) continue;
+      const int lead = __ffsll((long long)msk) - 1, lane = (int)(threadIdx.x & 63);
+      int base = 0;
+      if (lane == lead) base = atomicAdd(&lcount, (int)__popcll(msk));
+      base = __shfl(base, lead, 64);
+      const int slot = base + (int)__popcll(msk & ((1ull << lane) - 1ull));
+      if (slot < IBM_MAXNM) A.nml[lbase + slot] = (qs * A.nl + l) * 25 + sh;
+    }
+  } else {
+    // the listed neighbour markers' contributions to this box
+    const int cnt = A.nmc[blockIdx.x];
+    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+      const int code = A.nml[lbase + idx];
+      const int mq = code / 25, sh = code % 25;
+      const int qs = mq / A.nl, l = mq - qs * A.nl;
+      CompStencil c;
+      if (!neighbour_stencil(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c)) continue;
+      const double fc = flp[mq] * c0;
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            const int e = box_at(b, c.i0 + i, c.j0 + j, c.k0 + k);
+            if (e < 0) continue;
+            unsafeAtomicAdd(&box[e], fc * c.wx[i] * c.wy[j] * c.wz[k]);
+            if (MODE == 2 && qs < p) atomicOr(&taken[e >> 5], 1u << (e & 31));
+          }
+    }
+  }
   // flag the own stencil cells; with MODE >= 1 also spread the own forces of
-  // the previous pass into S.  The first marker's stencil is kept in
-  // registers for the gather below.
+  // the previous pass into S (after the neighbours' part, so that the first
+  // marker's stencil, kept in registers for the gather below, is not live
+  // across the neighbour loop).
   CompStencil cs0{};
   int e00 = 0, nz00 = 0;
   double ub00 = 0.0;
@@ -335,63 +436,14 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
           else unsafeAtomicAdd(&box[e], fc * c.wx[i] * c.wy[j] * c.wz[k]);
         }
   }
-  __shared__ int lcount;
-  const long lbase = (long)blockIdx.x * IBM_MAXNM;
+  __syncthreads();
   if (MODE == 0) {
-    // list the neighbours' markers whose stencils touch this box; the
-    // markers do not move inside the IBM block, so the three launches that
-    // spread them reuse the list
-    if (threadIdx.x == 0) lcount = 0;
-    __syncthreads();
-    const int nn = A.nbc[p];
-    const int tot = nn * A.nl;
-    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-      const int n = idx / A.nl, l = idx - n * A.nl;
-      const int code = A.nbl[(long)p * IBM_MAXNB + n];
-      const int qs = code / 25, sh = code % 25;
-      CompStencil c;
-      const bool keep = neighbour_stencil<false>(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c);
-      // one LDS atomic per wavefront
-      const unsigned long long msk = __ballot(keep);
-      if (!keep) continue;
-      const int lead = __ffsll((long long)msk) - 1, lane = (int)(threadIdx.x & 63);
-      int base = 0;
-      if (lane == lead) base = atomicAdd(&lcount, (int)__popcll(msk));
-      base = __shfl(base, lead, 64);
-      const int slot = base + (int)__popcll(msk & ((1ull << lane) - 1ull));
-      if (slot < IBM_MAXNM) A.nml[lbase + slot] = (qs * A.nl + l) * 25 + sh;
-    }
-    __syncthreads();
+    for (int t = threadIdx.x; t < nflag; t += blockDim.x) gflag[t] = flag[t];
     if (threadIdx.x == 0) {
       A.nmc[blockIdx.x] = min(lcount, IBM_MAXNM);
       if (lcount > IBM_MAXNM) atomicOr(A.err, 2);
     }
-  } else {
-    // the listed neighbour markers' contributions to this box
-    const int cnt = A.nmc[blockIdx.x];
-    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
-      const int code = A.nml[lbase + idx];
-      const int mq = code / 25, sh = code % 25;
-      const int qs = mq / A.nl, l = mq - qs * A.nl;
-      CompStencil c;
-      if (!neighbour_stencil(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c)) continue;
-      const double fc = flp[mq] * c0;
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            const int e = box_at(b, c.i0 + i, c.j0 + j, c.k0 + k);
-            if (e < 0) continue;
-            unsafeAtomicAdd(&box[e], fc * c.wx[i] * c.wy[j] * c.wz[k]);
-            if (MODE == 2 && qs < p) atomicOr(&taken[e >> 5], 1u << (e & 31));
-          }
-    }
   }
-  __syncthreads();
-  if (MODE == 0)
-    for (int t = threadIdx.x; t < nflag; t += blockDim.x) gflag[t] = flag[t];
   auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
   // cell t = li + bx row of the box walks by IBM_THREADS per step: (li, row)
   // advance by (rb, qb) with a carry instead of a division per cell
@@ -441,7 +493,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     CompStencil c;
     int e0, nzin;
     double ub;
-    if (q == 0) {
+    if (q == 0 && MODE <= 1) {
       if (!have0) continue;
       c = cs0; e0 = e00; nzin = nz00; ub = ub00;
     } else if (!own(q, c, e0, nzin, ub)) {
@@ -524,20 +576,28 @@ size_t ibm_lds_bytes(int bmx, int bmy, int bmz) {
   return cells * sizeof(double) + 2 * ((cells + 31) / 32) * sizeof(unsigned) + ((size_t)bmy * bmz + bmx) * sizeof(int);
 }
 
+// Largest comp_box extent in a direction of spacing 1/hinv: the first stencil
+// cells span floor(Xb) - floor(Xa) + 2 (face: the same with Xa, Xb + 1/2) <=
+// ceil(Xb - Xa) + 2, and a stencil is 3 cells; Xb - Xa exceeds 2 radfp hinv by
+// roundings only, which an exact integer 2 radfp hinv would carry over to the
+// next integer, hence floor(.. + 1e-9) + 1 rather than ceil.
+int ibm_box_max(double radfp, double hinv) { return (int)std::floor(2.0 * radfp * hinv + 1e-9) + 1 + 5; }
+
 }  // namespace
 
 // LDS of the per-sphere box (one velocity component at a time) and its two
 // cell masks, set once at marker set-up so that graph capture never meets an
 // attribute change.
 int ipg_ibm_setup(ipg_ctx *c) {
-  const double r2 = 2.0 * c->pc.radfp;
-  const int bmx = (int)std::ceil(r2 * c->g.dxi) + 7, bmy = (int)std::ceil(r2 * c->g.dyi) + 7;
-  const int bmz = (int)std::ceil(r2 * c->g.dzi) + 7;
+  const int bmx = ibm_box_max(c->pc.radfp, c->g.dxi), bmy = ibm_box_max(c->pc.radfp, c->g.dyi);
+  const int bmz = ibm_box_max(c->pc.radfp, c->g.dzi);
   const size_t lds = ibm_lds_bytes(bmx, bmy, bmz);
   if (lds > 160 * 1024) { ipg_set_error("ibm: sphere box exceeds the 160 KiB LDS of a CU"); return 1; }
   // the box's row offsets are kept as 32-bit integers
   if ((double)c->g.sxy * (c->g.nz + 2) >= 2147483647.0) { ipg_set_error("ibm: field larger than 2^31 cells"); return 1; }
-  if (bmx > c->g.nx || bmy > c->g.ny) { ipg_set_error("ibm: sphere box wider than the domain"); return 1; }
+  // the reach box (one cell wider than a component's box) must not meet its
+  // own periodic image
+  if (bmx + 1 > c->g.nx || bmy + 1 > c->g.ny) { ipg_set_error("ibm: sphere box wider than the domain"); return 1; }
   if (c->pc.nl > IBM_THREADS * IBM_MPT) { ipg_set_error("ibm: more than 1024 markers per sphere"); return 1; }
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -566,10 +626,9 @@ bool ibm_args(ipg_ctx *c, IbmArgs &A, size_t &lds) {
   if (!c->markers_ready || !c->ibm_lds_set) { ipg_set_error("ibm: ipg_coordsfp has not been called"); return false; }
   A = IbmArgs{};
   A.g = c->g; A.np = c->pc.np; A.nl = c->pc.nl; A.dVlagr = c->pc.dVlagr; A.dVeul = c->pc.dVeul; A.radfp = c->pc.radfp;
-  const double r2 = 2.0 * c->pc.radfp;
-  A.bmx = (int)std::ceil(r2 * c->g.dxi) + 7;
-  A.bmy = (int)std::ceil(r2 * c->g.dyi) + 7;
-  A.bmz = (int)std::ceil(r2 * c->g.dzi) + 7;
+  A.bmx = ibm_box_max(c->pc.radfp, c->g.dxi);
+  A.bmy = ibm_box_max(c->pc.radfp, c->g.dyi);
+  A.bmz = ibm_box_max(c->pc.radfp, c->g.dzi);
   lds = ibm_lds_bytes(A.bmx, A.bmy, A.bmz);
   double **q = c->ps.q;
   A.px = q[IPG_PX]; A.py = q[IPG_PY]; A.pz = q[IPG_PZ]; A.pu = q[IPG_PU]; A.pv = q[IPG_PV]; A.pw = q[IPG_PW];
@@ -649,14 +708,16 @@ int ipg_launch_ibm_totals(ipg_ctx *c, int pass) {
 }
 
 // A neighbour list that overflowed (more than 64 neighbour images: a cluster
-// far denser than random packing) would have dropped contributions: fail.
+// far denser than random packing) or a stencil outside its sphere's box would
+// have dropped contributions: fail.
 int ipg_ibm_check(ipg_ctx *c) {
   if (c->pc.np <= 0 || !c->ibm_err) return 0;
   int e = 0;
   IPG_HIP(hipMemcpy(&e, c->ibm_err, sizeof(int), hipMemcpyDeviceToHost));
   if (e) {
-    ipg_set_error(e & 1 ? "ibm: more than 64 neighbour images around a sphere"
-                        : "ibm: more than 4096 neighbour markers touching one sphere's stencils");
+    ipg_set_error(e & 1   ? "ibm: more than 64 neighbour images around a sphere"
+                  : e & 2 ? "ibm: more than 4096 neighbour markers touching one sphere's stencils"
+                          : "ibm: a marker stencil outside its sphere's box");
     return 3;
   }
   return 0;
