@@ -278,10 +278,10 @@ __device__ __forceinline__ bool neighbour_stencil(const IbmArgs &A, const Box &b
 // One IBM launch for one (sphere, velocity component) workgroup.
 //   MODE 0 (pass 0): gather from dudt, set the marker forces; list the
 //     neighbours' markers whose stencils touch this box (for the next launches).
-//   MODE 1 (passes 1, 2): rebuild the previous pass's spread S in LDS, gather
-//     from fl(dudt + S), add to the marker forces.
-//   MODE 2 (after pass 2): rebuild the last spread S, add it to dudt on the
-//     cells this sphere owns.
+//   MODE 1 (passes 1, 2): sum dudt and the previous pass's spread S in LDS
+//     (dudtf = dudt + S), gather from it, add to the marker forces.
+//   MODE 2 (after pass 2): sum dudt and the last spread S in LDS, and store
+//     dudt + S on the cells this sphere owns.
 // Thread t owns markers t, t + IBM_THREADS; their positions
 // (src/coordsfp.f90:49-68), stencils and weights (src/interp_spread.f90:76-130)
 // and rigid-body velocities (src/intgr_nwtn_eulr.f90:1471-1476) are computed in
@@ -361,6 +361,41 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   __shared__ int lcount;
   if (MODE == 0 && threadIdx.x == 0) lcount = 0;
   __syncthreads();
+  auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
+  // cell t = li + bx row of the box walks by IBM_THREADS per step: (li, row)
+  // advance by (rb, qb) with a carry instead of a division per cell
+  const int qb = IBM_THREADS / b.bx, rb = IBM_THREADS - qb * b.bx;
+  const int row0 = (int)threadIdx.x / b.bx, li0 = (int)threadIdx.x - row0 * b.bx;
+  auto advance = [&](int &li, int &row) {
+    li += rb;
+    row += qb;
+    if (li >= b.bx) { li -= b.bx; row++; }
+  };
+  constexpr int CH = 8;
+  if (MODE >= 1) {
+    // dudt on the cells the own stencils touch (flagged by pass 0), added
+    // into the box together with the spreads below, so that its loads overlap
+    // them: the box ends as dudt + S, the reference's dudtf, with the terms
+    // summed in another order (as the LDS atomics already order S's terms)
+    int li = li0, row = row0;
+    for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * IBM_THREADS) {
+      double v[CH];
+      unsigned fm = 0u;
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int t = t0 + u * IBM_THREADS;
+        v[u] = 0.0;
+        if (t < nbox && flagged(t)) {
+          v[u] = F[(long)rowoff[row] + icol[li]];
+          fm |= 1u << u;
+        }
+        advance(li, row);
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++)
+        if ((fm >> u) & 1u) unsafeAtomicAdd(&box[t0 + u * IBM_THREADS], v[u]);
+    }
+  }
   const long lbase = (long)blockIdx.x * IBM_MAXNM;
   if (MODE == 0) {
     // list the neighbours' markers whose stencils touch this box; the
@@ -444,50 +479,36 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
       if (lcount > IBM_MAXNM) atomicOr(A.err, 2);
     }
   }
-  auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
-  // cell t = li + bx row of the box walks by IBM_THREADS per step: (li, row)
-  // advance by (rb, qb) with a carry instead of a division per cell
-  const int qb = IBM_THREADS / b.bx, rb = IBM_THREADS - qb * b.bx;
-  int row = (int)threadIdx.x / b.bx, li = (int)threadIdx.x - row * b.bx;
-  auto advance = [&]() {
-    li += rb;
-    row += qb;
-    if (li >= b.bx) { li -= b.bx; row++; }
-  };
   if (MODE == 2) {
     // dudt = dudtf = fl(dudt + S) (src/main.f90:332-334), once per cell
+    int li = li0, row = row0;
     for (int t = threadIdx.x; t < nbox; t += IBM_THREADS) {
-      if (flagged(t) && !((taken[t >> 5] >> (t & 31)) & 1u)) {
-        const long cc = (long)rowoff[row] + icol[li];
-        F[cc] = F[cc] + box[t];
-      }
-      advance();
+      if (flagged(t) && !((taken[t >> 5] >> (t & 31)) & 1u)) F[(long)rowoff[row] + icol[li]] = box[t];
+      advance(li, row);
     }
     return;
   }
-  // stage fl(dudt + S) (MODE 1) or dudt (MODE 0) on the flagged cells, in
-  // chunks of 8 cells per thread so that the loads of a chunk are all in
-  // flight together
-  constexpr int CH = 8;
-  for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * IBM_THREADS) {
-    double v[CH];
+  if (MODE == 0) {
+    // stage dudt on the flagged cells, in chunks of 8 cells per thread so
+    // that the loads of a chunk are all in flight together
+    int li = li0, row = row0;
+    for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * IBM_THREADS) {
+      double v[CH];
 #pragma unroll
-    for (int u = 0; u < CH; u++) {
-      const int t = t0 + u * IBM_THREADS;
-      v[u] = 0.0;
-      if (t < nbox && flagged(t)) {
-        v[u] = F[(long)rowoff[row] + icol[li]];
-        if (MODE == 1) v[u] = v[u] + box[t];
+      for (int u = 0; u < CH; u++) {
+        const int t = t0 + u * IBM_THREADS;
+        v[u] = 0.0;
+        if (t < nbox && flagged(t)) v[u] = F[(long)rowoff[row] + icol[li]];
+        advance(li, row);
       }
-      advance();
-    }
 #pragma unroll
-    for (int u = 0; u < CH; u++) {
-      const int t = t0 + u * IBM_THREADS;
-      if (t < nbox) box[t] = v[u];
+      for (int u = 0; u < CH; u++) {
+        const int t = t0 + u * IBM_THREADS;
+        if (t < nbox) box[t] = v[u];
+      }
     }
+    __syncthreads();
   }
-  __syncthreads();
 #pragma unroll
   for (int q = 0; q < IBM_MPT; q++) {
     CompStencil c;
