@@ -120,7 +120,11 @@ int ipg_correc(ipg_ctx *ctx, int rkiter);             /* src/correc.f90:11 */
 int ipg_pnew_update(ipg_ctx *ctx);                    /* src/main.f90:368-376 */
 int ipg_intgr_over_sphere(ipg_ctx *ctx, int type);    /* src/intgr_over_sphere.f90:12 */
 int ipg_intgr_nwtn_eulr(ipg_ctx *ctx, int rkiter);    /* src/intgr_nwtn_eulr.f90:12 */
-int ipg_substep(ipg_ctx *ctx, int rkiter);            /* src/main.f90:268-414, whole substep */
+/* src/main.f90:268-414, whole substep.  Returns with the stream running.  The
+ * closing Newton-Euler update (src/main.f90:380-405) is left pending and runs
+ * beside the next substep's RK stage, or first thing in any other entry point
+ * that reads or writes the particle state, dt or u, v, w (ipg_sync included). */
+int ipg_substep(ipg_ctx *ctx, int rkiter);
 int ipg_chkdt(ipg_ctx *ctx, double *dtmax);           /* src/chkdt.f90:6 */
 int ipg_chkdiv(ipg_ctx *ctx, double *divtot, double *divmax); /* src/chkdiv.f90:7 */
 
@@ -137,7 +141,7 @@ int ipg_uniform(unsigned long long seed, unsigned long long stream, unsigned lon
                 double *out);
 
 /* ---- runtime controls (no reference counterpart) --------------------- */
-int ipg_sync(ipg_ctx *ctx);                           /* wait for the context's stream */
+int ipg_sync(ipg_ctx *ctx);                           /* finish all pending work and wait for it */
 int ipg_use_graphs(ipg_ctx *ctx, int on);             /* capture each substep as a hipGraph */
 /* time the RK stage for 8..64 z planes per workgroup and keep the fastest;
  * overwrites dudt..dcdt (recomputed by every stage): call between substeps */

@@ -70,9 +70,25 @@ bool check_ctx(ipg_ctx *c) {
   return true;
 }
 
+// Runs the Newton-Euler update a fused substep left pending, on the main
+// stream, in the order the reference runs it.
+int settle(ipg_ctx *c) {
+  if (!c->ne_pending) return 0;
+  const int rkiter = c->ne_pending;
+  c->ne_pending = 0;
+  return ipg_launch_nwtn_eulr(c, rkiter);
+}
+
+// check_ctx and settle: the entry points that read or write the particle
+// state, dt or u, v, w
+bool ready(ipg_ctx *c) { return check_ctx(c) && settle(c) == 0; }
+
 int release(ipg_ctx *c) {
-  for (int r = 0; r < 6; r++)
+  for (int r = 0; r < IPG_NGRAPH; r++)
     if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+  if (c->side) (void)hipStreamDestroy(c->side);
   for (int q = 0; q < 64; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
   ipg_poisson_free(c);
@@ -90,12 +106,23 @@ int release(ipg_ctx *c) {
 //    the forward x transform computes with the boundary values at the source;
 //  * boundp(p) (src/main.f90:361) is read at the source by correc; p's own
 //    ghosts are filled only when p is read back (ipg_field_get).
-// (Running the particle kernels that do not touch the fields on a second
-// stream, forked and joined with events, was measured slower in a captured
-// graph than this single-stream order.)
-int substep_launches(ipg_ctx *c, int rkiter) {
+// pend > 0: the previous substep's Newton-Euler update (its rkiter), run on
+// the side stream beside this substep's RK stage and joined before the IBM,
+// which needs the particles it moves.  defer: leave this substep's update
+// pending instead of launching it at the end.
+int substep_launches(ipg_ctx *c, int rkiter, int pend, bool defer) {
   const bool np = c->pc.np > 0;
+  if (pend) {
+    IPG_HIP(hipEventRecord(c->fork_ev, c->stream));
+    IPG_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    std::swap(c->stream, c->side);
+    const int rc = ipg_launch_nwtn_eulr(c, pend);
+    std::swap(c->stream, c->side);
+    if (rc) return rc;
+    IPG_HIP(hipEventRecord(c->join_ev, c->side));
+  }
   if (ipg_launch_rk(c, rkiter, true)) return 1;
+  if (pend) IPG_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
   if (np && ipg_launch_ibm(c)) return 1;
   if (ipg_launch_poisson_ex(c, rkiter)) return 1;
   if (ipg_launch_correc_fused(c, rkiter)) return 1;
@@ -104,7 +131,7 @@ int substep_launches(ipg_ctx *c, int rkiter) {
   std::swap(c->f[IPG_C], c->f[IPG_DCDT]);
   c->cpar ^= 1;
   if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C | BND_NO_XCOL)) return 1;
-  if (np && ipg_launch_nwtn_eulr(c, rkiter)) return 1;
+  if (np && !defer && ipg_launch_nwtn_eulr(c, rkiter)) return 1;
   // left to the reader: p's ghosts, the x-ghost columns of the fields, dcdt
   c->p_ghosts_stale = true;
   c->xcol_stale = true;
@@ -168,7 +195,13 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
   derived(c);
   if (const char *e = std::getenv("IPG_TUNE_RK_KB")) c->tune_rk_kb = std::atoi(e);
   auto fail = [&](int code) { release(c); delete c; return code; };
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { ipg_set_error("stream create failed"); return fail(1); }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+    ipg_set_error("stream create failed");
+    return fail(1);
+  }
   const Grid &g = c->g;
   c->nfield = 16 + (size_t)g.sxy * (g.nz + 2) + 16;
   c->nfield = (c->nfield + 15) / 16 * 16;
@@ -227,7 +260,7 @@ int ipg_finalize(ipg_ctx *c) {
 }
 
 int ipg_field_put(ipg_ctx *c, int field, const double *host) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (field < 0 || field >= IPG_NFIELDS || !host) { ipg_set_error("field_put: bad field"); return 1; }
   if (field == IPG_P) c->p_ghosts_stale = false;
   if (ghosted_field(field) && fill_xcol(c)) return 1;
@@ -258,7 +291,7 @@ int ipg_field_get(ipg_ctx *c, int field, double *host) {
 }
 
 int ipg_particles_put(ipg_ctx *c, int quantity, const double *host) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (quantity < 0 || quantity >= IPG_NPART || !host) { ipg_set_error("particles_put: bad quantity"); return 1; }
   if (c->pc.np == 0) return 0;
   IPG_HIP(hipMemcpyAsync(c->ps.q[quantity], host, c->pc.np * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -267,7 +300,7 @@ int ipg_particles_put(ipg_ctx *c, int quantity, const double *host) {
 }
 
 int ipg_particles_get(ipg_ctx *c, int quantity, double *host) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (quantity < 0 || quantity >= IPG_NPART || !host) { ipg_set_error("particles_get: bad quantity"); return 1; }
   if (c->pc.np == 0) return 0;
   IPG_HIP(hipMemcpyAsync(host, c->ps.q[quantity], c->pc.np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -276,7 +309,7 @@ int ipg_particles_get(ipg_ctx *c, int quantity, double *host) {
 }
 
 int ipg_set_dt(ipg_ctx *c, double dt) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   c->dt_host = dt;
   IPG_HIP(hipMemcpyAsync(c->dscal, &c->dt_host, sizeof(double), hipMemcpyHostToDevice, c->stream));
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -290,7 +323,7 @@ int ipg_get_dt(ipg_ctx *c, double *dt) {
 }
 
 int ipg_initsolver(ipg_ctx *c) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (c->solver_ready) return 0;
   if (ipg_poisson_setup(c)) return 1;
   c->solver_ready = true;
@@ -298,7 +331,7 @@ int ipg_initsolver(ipg_ctx *c) {
 }
 
 int ipg_initparticles(ipg_ctx *c, const double *x, const double *y, const double *z) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   const int np = c->pc.np;
   if (np == 0) return 0;
   std::vector<double> zero(np, 0.0);
@@ -318,7 +351,7 @@ int ipg_initparticles(ipg_ctx *c, const double *x, const double *y, const double
 }
 
 int ipg_coordsfp(ipg_ctx *c, const double *thetarc, const double *phirc, double radfp) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   const int nl = c->pc.nl;
   if (c->pc.np == 0) return 0;
   std::vector<double> mk(3 * (size_t)nl);
@@ -340,7 +373,7 @@ int ipg_coordsfp(ipg_ctx *c, const double *thetarc, const double *phirc, double 
 }
 
 int ipg_rk(ipg_ctx *c, int rkiter) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("rk: rkiter must be 1..3"); return 1; }
   if (fill_xcol(c) || fill_dcdt(c) || ipg_launch_rk(c, rkiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -348,21 +381,21 @@ int ipg_rk(ipg_ctx *c, int rkiter) {
 }
 
 int ipg_ibm(ipg_ctx *c) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (ipg_launch_ibm(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return ipg_ibm_check(c);
 }
 
 int ipg_ibm_begin(ipg_ctx *c) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (ipg_launch_ibm_begin(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return ipg_ibm_check(c);
 }
 
 int ipg_eulr2lagr(ipg_ctx *c, int ibmiter) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("eulr2lagr: ibmiter must be 0..2"); return 1; }
   if (ipg_launch_ibm_gather(c, ibmiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -370,7 +403,7 @@ int ipg_eulr2lagr(ipg_ctx *c, int ibmiter) {
 }
 
 int ipg_lagrforces(ipg_ctx *c, int ibmiter) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("lagrforces: ibmiter must be 0..2"); return 1; }
   if (ipg_launch_ibm_totals(c, ibmiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -378,7 +411,7 @@ int ipg_lagrforces(ipg_ctx *c, int ibmiter) {
 }
 
 int ipg_lagr2eulr(ipg_ctx *c, int ibmiter) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("lagr2eulr: ibmiter must be 0..2"); return 1; }
   // the spreads of passes 0 and 1 are rebuilt by the next eulr2lagr
   if (ibmiter == 2 && ipg_launch_ibm_final(c)) return 1;
@@ -404,14 +437,14 @@ int ipg_uniform(unsigned long long seed, unsigned long long stream, unsigned lon
 }
 
 int ipg_bounduvw(ipg_ctx *c, int which) {
-  if (!check_ctx(c) || fill_xcol(c)) return 1;
+  if (!ready(c) || fill_xcol(c)) return 1;
   if (ipg_launch_bound(c, which == 0 ? BND_UVW_NEW : BND_UVW_DT)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
 
 int ipg_boundp(ipg_ctx *c, int which) {
-  if (!check_ctx(c) || fill_xcol(c)) return 1;
+  if (!ready(c) || fill_xcol(c)) return 1;
   if (ipg_launch_bound(c, which == 0 ? BND_PNEW : BND_P)) return 1;
   if (which != 0) c->p_ghosts_stale = false;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -419,14 +452,14 @@ int ipg_boundp(ipg_ctx *c, int which) {
 }
 
 int ipg_boundscal(ipg_ctx *c) {
-  if (!check_ctx(c) || fill_xcol(c)) return 1;
+  if (!ready(c) || fill_xcol(c)) return 1;
   if (ipg_launch_bound(c, BND_C)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
 
 int ipg_fillps(ipg_ctx *c, int rkiter) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("fillps: rkiter must be 1..3"); return 1; }
   if (ipg_launch_fillps(c, rkiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -434,7 +467,7 @@ int ipg_fillps(ipg_ctx *c, int rkiter) {
 }
 
 int ipg_solver2d(ipg_ctx *c) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (!c->solver_ready) { ipg_set_error("solver2d: ipg_initsolver has not been called"); return 1; }
   if (ipg_launch_poisson(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -442,7 +475,7 @@ int ipg_solver2d(ipg_ctx *c) {
 }
 
 int ipg_correc(ipg_ctx *c, int rkiter) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("correc: rkiter must be 1..3"); return 1; }
   if (ipg_launch_correc(c, rkiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -450,7 +483,7 @@ int ipg_correc(ipg_ctx *c, int rkiter) {
 }
 
 int ipg_pnew_update(ipg_ctx *c) {
-  if (!check_ctx(c) || fill_dcdt(c)) return 1;
+  if (!ready(c) || fill_dcdt(c)) return 1;
   if (ipg_launch_pnew(c)) return 1;
   if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C)) return 1;
   c->xcol_stale = false;
@@ -459,7 +492,7 @@ int ipg_pnew_update(ipg_ctx *c) {
 }
 
 int ipg_intgr_over_sphere(ipg_ctx *c, int type) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (type < 1 || type > 3) { ipg_set_error("intgr_over_sphere: type must be 1..3"); return 1; }
   if (ipg_launch_sphere_integrals(c, type)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -467,7 +500,7 @@ int ipg_intgr_over_sphere(ipg_ctx *c, int type) {
 }
 
 int ipg_intgr_nwtn_eulr(ipg_ctx *c, int rkiter) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("intgr_nwtn_eulr: rkiter must be 1..3"); return 1; }
   if (ipg_launch_nwtn_eulr(c, rkiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -482,13 +515,23 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
   // the substep rewrites dcdt's interior (RK stage) before swapping it in as
   // c, whose ghosts it then fills: the stale dcdt value is never needed
   c->dcdt_stale = false;
+  // the pending update of the previous substep rides along beside this RK
+  // stage when it is the RK3 predecessor's (3 -> 1 -> 2 -> 3), the only case
+  // the captured graphs hold; any other is run first, on its own
+  int pend = c->ne_pending;
+  if (pend && pend != (rkiter == 1 ? 3 : rkiter - 1)) {
+    if (settle(c)) return 1;
+    pend = 0;
+  }
+  c->ne_pending = 0;
+  const bool np = c->pc.np > 0;
   if (c->use_graphs) {
-    const int r = (rkiter - 1) + 3 * c->cpar;
+    const int r = (rkiter - 1) + 3 * c->cpar + 6 * (pend ? 1 : 0);
     if (!c->graph_ok[r]) {
       // capturing records the launches and makes the host-side buffer swap
       hipGraph_t graph;
       IPG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
-      int rc = substep_launches(c, rkiter);
+      int rc = substep_launches(c, rkiter, pend, true);
       hipError_t e = hipStreamEndCapture(c->stream, &graph);
       if (rc) return rc;
       if (e != hipSuccess) { ipg_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return 1; }
@@ -503,9 +546,11 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
     c->p_ghosts_stale = true;
     c->xcol_stale = true;
     c->dcdt_stale = true;
-    return 0;
+  } else if (substep_launches(c, rkiter, pend, true)) {
+    return 1;
   }
-  return substep_launches(c, rkiter);
+  c->ne_pending = np ? rkiter : 0;
+  return 0;
 }
 
 int ipg_chkdt(ipg_ctx *c, double *dtmax) {
@@ -527,7 +572,7 @@ int ipg_post3d(ipg_ctx *c, int quantity, double *host) {
 }
 
 int ipg_sync(ipg_ctx *c) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return ipg_ibm_check(c);
 }
@@ -537,11 +582,12 @@ int ipg_substep_phases(ipg_ctx *c, int rkiter, int maxn, double *ms, const char 
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("substep_phases: rkiter must be 1..3"); return 1; }
   for (int q = 0; q < 64; q++)
     if (!c->ev[q]) IPG_HIP(hipEventCreate(&c->ev[q]));
+  if (settle(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   c->timing = true;
   c->nev = 0;
   IPG_HIP(hipEventRecord(c->ev[0], c->stream));
-  int rc = substep_launches(c, rkiter);
+  int rc = substep_launches(c, rkiter, 0, false);
   c->timing = false;
   if (rc) return rc;
   IPG_HIP(hipEventSynchronize(c->ev[c->nev]));
@@ -561,7 +607,7 @@ int ipg_substep_phases(ipg_ctx *c, int rkiter, int maxn, double *ms, const char 
 // overwrites dudt..dcdt only (every RK stage recomputes them), so it may be
 // called before any substep; captured graphs are rebuilt afterwards.
 int ipg_autotune(ipg_ctx *c, int *kb_out) {
-  if (!check_ctx(c)) return 1;
+  if (!ready(c)) return 1;
   for (int q = 0; q < 2; q++)
     if (!c->ev[q]) IPG_HIP(hipEventCreate(&c->ev[q]));
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -585,7 +631,7 @@ int ipg_autotune(ipg_ctx *c, int *kb_out) {
   }
   c->tune_rk_kb = best > 0 ? best : saved;
   c->dcdt_stale = false;  // the timed stages rewrote dcdt
-  for (int r = 0; r < 6; r++)
+  for (int r = 0; r < IPG_NGRAPH; r++)
     if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
   if (kb_out) *kb_out = c->tune_rk_kb;
   return 0;

@@ -628,7 +628,7 @@ int ipg_ibm_setup(ipg_ctx *c) {
   // old one are dropped
   const size_t nflag = 3 * (size_t)c->pc.np * (((size_t)bmx * bmy * bmz + 31) / 32);
   if (nflag > c->ibm_flg_n) {
-    for (int r = 0; r < 6; r++)
+    for (int r = 0; r < IPG_NGRAPH; r++)
       if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
     if (c->ibm_flg) (void)hipFree(c->ibm_flg);
     c->ibm_flg = nullptr;

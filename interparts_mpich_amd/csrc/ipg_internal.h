@@ -8,6 +8,7 @@
 #include "../../include/ipg.h"
 
 #define IPG_NB 9
+#define IPG_NGRAPH 12
 
 // Grid description passed by value to every field kernel.
 // Field layout in HBM: element (i,j,k), i in 0..nx+1, j in 0..ny+1,
@@ -84,13 +85,22 @@ struct ipg_ctx {
   double dt_host;
   Poisson ps2;
   bool solver_ready, markers_ready, use_graphs;
-  // one graph per (rkiter, buffer parity): the fused substep swaps the roles
-  // of the c and dcdt buffers (cnew = dcdt, src/main.f90:375), so the
-  // captured pointers alternate
-  hipGraphExec_t graph[6];
-  bool graph_ok[6];
+  // one graph per (rkiter, buffer parity, carried Newton-Euler update): the
+  // fused substep swaps the roles of the c and dcdt buffers (cnew = dcdt,
+  // src/main.f90:375), so the captured pointers alternate, and it may carry
+  // the previous substep's particle update on the side stream
+  hipGraphExec_t graph[IPG_NGRAPH];
+  bool graph_ok[IPG_NGRAPH];
   int cpar;                       // 1 when c and dcdt have swapped buffers an odd number of times
   bool dcdt_stale;                // dcdt's buffer holds the previous c; its value is c's
+  // The Newton-Euler update that ends a fused substep (src/main.f90:380-405)
+  // reads u, v, w and the particle state only; the next substep's RK stage
+  // touches neither, so the update is left pending (ne_pending = its rkiter)
+  // and runs on the side stream beside that RK stage.  Every entry point that
+  // reads or writes the particle state, dt or u, v, w runs it first.
+  int ne_pending;
+  hipStream_t side;
+  hipEvent_t fork_ev, join_ev;
   // kernel timing (ipg_substep_phases): an event is recorded after every
   // launch while timing is on
   bool timing;
