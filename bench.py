@@ -103,7 +103,9 @@ def run_ensemble(args):
     cases = ensemble_cases(args.ensemble)
     sims = []
     for c in cases:
-        sims.append(Simulation(c, use_graphs=True).initialize(initial_fields(c), place_spheres(c), markers_for(c)))
+        # many contexts overlap one another's streams: no side stream each
+        sims.append(Simulation(c, use_graphs=True, side_stream=args.side_stream == "on")
+                    .initialize(initial_fields(c), place_spheres(c), markers_for(c)))
     print(f"[bench] ensemble of {len(sims)} cases ready", file=sys.stderr, flush=True)
     for sim in sims:
         for s in range(args.warmup):
@@ -131,7 +133,7 @@ def run_ensemble(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded cases)",
         "config": {"workload": f"{len(sims)} cases, grids 64^3/96x96x64/128^3/128x128x192, 1-200 spheres, "
                                "D 8-12h, Re_bulk 1000-5000",
-                   "gcell_updates_per_s": round(cells / t_conc / 1e9, 4)},
+                   "gcell_updates_per_s": round(cells / t_conc / 1e9, 4), "side_stream": args.side_stream == "on"},
         "sequential_case_substeps_per_s": round(n / t_seq, 2),
         "speedup_vs_sequential": round(t_seq / t_conc, 3),
     }
@@ -154,6 +156,9 @@ def main():
                     help="only run a plain device-to-device copy of MIB MiB (calibrates the HBM byte counters of "
                          "a profiler pass; measures the practical copy bandwidth) and exit")
     ap.add_argument("--no-copy-probe", action="store_true", help="skip the copy-bandwidth child process")
+    ap.add_argument("--side-stream", choices=("on", "off"), default=None,
+                    help="run each substep's Newton-Euler update beside the next RK stage on a second stream "
+                         "(default: on for one case, off for an ensemble)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="NCASES",
                     help="instead of the 256^3 workload, advance NCASES independent small cases (uneven grids and "
                          "sphere counts) together and report aggregate case-substeps/s against running them one "
@@ -167,6 +172,8 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.side_stream is None:
+        args.side_stream = "off" if args.ensemble else "on"
     if args.ensemble:
         run_ensemble(args)
         return
@@ -178,7 +185,8 @@ def main():
     case = CONFIGS[args.config]
     fields, spheres, markers = initial_fields(case), place_spheres(case), markers_for(case)
     print(f"[bench] {case.name}: {case.itot}x{case.jtot}x{case.ktot}, np={case.np}, nl={case.nl}", file=sys.stderr, flush=True)
-    sim = Simulation(case, use_graphs=not args.no_graphs).initialize(fields, spheres, markers)
+    sim = Simulation(case, use_graphs=not args.no_graphs, side_stream=args.side_stream == "on").initialize(fields, spheres,
+                                                                                                        markers)
     rk_kb = None if args.no_autotune else sim.autotune()
     for s in range(args.warmup):
         sim.substep(s % 3 + 1)
@@ -237,7 +245,7 @@ def main():
         "data": "synthetic (seeded Poiseuille + 0.1 U_bulk noise, seeded sphere placement)",
         "config": {"workload": f"{case.name}: {case.itot}x{case.jtot}x{case.ktot} channel, {case.np} spheres "
                                f"D={case.diameter_h:g}h nl={case.nl}, Re_bulk={case.re_bulk:g}, FP64 path",
-                   "graphs": not args.no_graphs,
+                   "graphs": not args.no_graphs, "side_stream": args.side_stream == "on",
                    "rk_z_planes_per_workgroup": rk_kb if rk_kb is not None else "default",
                    "gcell_updates_per_s": round(value * case.cells / 1e9, 4)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -123,7 +123,8 @@ int ipg_intgr_nwtn_eulr(ipg_ctx *ctx, int rkiter);    /* src/intgr_nwtn_eulr.f90
 /* src/main.f90:268-414, whole substep.  Returns with the stream running.  The
  * closing Newton-Euler update (src/main.f90:380-405) is left pending and runs
  * beside the next substep's RK stage, or first thing in any other entry point
- * that reads or writes the particle state, dt or u, v, w (ipg_sync included). */
+ * that reads or writes the particle state, dt or u, v, w (ipg_sync included);
+ * with ipg_use_side_stream(ctx, 0) it closes the substep instead. */
 int ipg_substep(ipg_ctx *ctx, int rkiter);
 int ipg_chkdt(ipg_ctx *ctx, double *dtmax);           /* src/chkdt.f90:6 */
 int ipg_chkdiv(ipg_ctx *ctx, double *divtot, double *divmax); /* src/chkdiv.f90:7 */
@@ -143,6 +144,10 @@ int ipg_uniform(unsigned long long seed, unsigned long long stream, unsigned lon
 /* ---- runtime controls (no reference counterpart) --------------------- */
 int ipg_sync(ipg_ctx *ctx);                           /* finish all pending work and wait for it */
 int ipg_use_graphs(ipg_ctx *ctx, int on);             /* capture each substep as a hipGraph */
+/* run a substep's Newton-Euler update beside the next substep's RK stage on a
+ * second stream (default on); off: the update closes its substep.  Turn it off
+ * when many contexts share the GPU: their streams already overlap. */
+int ipg_use_side_stream(ipg_ctx *ctx, int on);
 /* time the RK stage for 8..64 z planes per workgroup and keep the fastest;
  * overwrites dudt..dcdt (recomputed by every stage): call between substeps */
 int ipg_autotune(ipg_ctx *ctx, int *kb);

@@ -83,6 +83,19 @@ int settle(ipg_ctx *c) {
 // state, dt or u, v, w
 bool ready(ipg_ctx *c) { return check_ctx(c) && settle(c) == 0; }
 
+// The side stream and its two events, created when a substep first carries
+// an update over (never during a capture).  Streams are spread over the
+// device's hardware queues in the order they are created; creating the side
+// stream with the context would put the main streams of many contexts on
+// every other queue.
+int ensure_side(ipg_ctx *c) {
+  if (c->side) return 0;
+  IPG_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  IPG_HIP(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+  IPG_HIP(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming));
+  return 0;
+}
+
 int release(ipg_ctx *c) {
   for (int r = 0; r < IPG_NGRAPH; r++)
     if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
@@ -195,13 +208,8 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
   derived(c);
   if (const char *e = std::getenv("IPG_TUNE_RK_KB")) c->tune_rk_kb = std::atoi(e);
   auto fail = [&](int code) { release(c); delete c; return code; };
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
-    ipg_set_error("stream create failed");
-    return fail(1);
-  }
+  c->side_on = true;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { ipg_set_error("stream create failed"); return fail(1); }
   const Grid &g = c->g;
   c->nfield = 16 + (size_t)g.sxy * (g.nz + 2) + 16;
   c->nfield = (c->nfield + 15) / 16 * 16;
@@ -524,14 +532,17 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
     pend = 0;
   }
   c->ne_pending = 0;
-  const bool np = c->pc.np > 0;
+  if (pend && ensure_side(c)) return 1;
+  // with the side stream off the update closes the substep, as in the
+  // reference (ipg_use_side_stream drops the graphs when it changes)
+  const bool defer = c->side_on && c->pc.np > 0;
   if (c->use_graphs) {
     const int r = (rkiter - 1) + 3 * c->cpar + 6 * (pend ? 1 : 0);
     if (!c->graph_ok[r]) {
       // capturing records the launches and makes the host-side buffer swap
       hipGraph_t graph;
       IPG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
-      int rc = substep_launches(c, rkiter, pend, true);
+      int rc = substep_launches(c, rkiter, pend, defer);
       hipError_t e = hipStreamEndCapture(c->stream, &graph);
       if (rc) return rc;
       if (e != hipSuccess) { ipg_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return 1; }
@@ -546,10 +557,10 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
     c->p_ghosts_stale = true;
     c->xcol_stale = true;
     c->dcdt_stale = true;
-  } else if (substep_launches(c, rkiter, pend, true)) {
+  } else if (substep_launches(c, rkiter, pend, defer)) {
     return 1;
   }
-  c->ne_pending = np ? rkiter : 0;
+  c->ne_pending = defer ? rkiter : 0;
   return 0;
 }
 
@@ -640,6 +651,18 @@ int ipg_autotune(ipg_ctx *c, int *kb_out) {
 int ipg_use_graphs(ipg_ctx *c, int on) {
   if (!check_ctx(c)) return 1;
   c->use_graphs = on != 0;
+  return 0;
+}
+
+// Many contexts sharing the GPU already overlap one another's streams; a
+// second stream each, joined by events, only adds waits to the hardware
+// queues they share.
+int ipg_use_side_stream(ipg_ctx *c, int on) {
+  if (!ready(c)) return 1;
+  if (c->side_on != (on != 0))
+    for (int r = 0; r < IPG_NGRAPH; r++)
+      if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
+  c->side_on = on != 0;
   return 0;
 }
 

@@ -99,6 +99,7 @@ struct ipg_ctx {
   // and runs on the side stream beside that RK stage.  Every entry point that
   // reads or writes the particle state, dt or u, v, w runs it first.
   int ne_pending;
+  bool side_on;                   // carry the update over to the side stream (ipg_use_side_stream)
   hipStream_t side;
   hipEvent_t fork_ev, join_ev;
   // kernel timing (ipg_substep_phases): an event is recorded after every

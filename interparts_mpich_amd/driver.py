@@ -22,7 +22,7 @@ def _ptr(a):
 
 
 class Simulation:
-    def __init__(self, case: Case, use_graphs=False):
+    def __init__(self, case: Case, use_graphs=False, side_stream=True):
         self.case = case
         self.lib = load()
         prm = _lib.IpgParams(itot=case.itot, jtot=case.jtot, ktot=case.ktot, np=case.np,
@@ -36,6 +36,8 @@ class Simulation:
         check(self.lib.ipg_init(C.byref(prm), C.byref(self.ctx)), "ipg_init")
         if use_graphs:
             check(self.lib.ipg_use_graphs(self.ctx, 1), "ipg_use_graphs")
+        if not side_stream:
+            check(self.lib.ipg_use_side_stream(self.ctx, 0), "ipg_use_side_stream")
         self.shape = (case.ktot + 2, case.jtot + 2, case.itot + 2)
         self.time = 0.0
         self.dt = 0.0

@@ -13,7 +13,7 @@ module mod_ipg_capi
             ipg_ibm, ipg_ibm_begin, ipg_eulr2lagr, ipg_lagrforces, ipg_lagr2eulr, ipg_bounduvw, ipg_boundp, &
             ipg_boundscal, ipg_fillps, ipg_solver2d, ipg_correc, ipg_pnew_update, ipg_intgr_over_sphere, &
             ipg_intgr_nwtn_eulr, ipg_substep, ipg_chkdt, ipg_chkdiv, ipg_uniform, ipg_sync, ipg_use_graphs, &
-            ipg_autotune, ipg_post3d
+            ipg_use_side_stream, ipg_autotune, ipg_post3d
 
   ! field selectors (enum of include/ipg.h)
   integer(c_int), parameter, public :: IPG_U = 0, IPG_V = 1, IPG_W = 2, IPG_PNEW = 3, IPG_C = 4, &
@@ -207,6 +207,11 @@ module mod_ipg_capi
       type(c_ptr), value :: ctx
     end function
     integer(c_int) function ipg_use_graphs(ctx, on) bind(C, name="ipg_use_graphs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function ipg_use_side_stream(ctx, on) bind(C, name="ipg_use_side_stream")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: on

@@ -218,6 +218,54 @@ def test_graph_replay_matches_eager():
             assert r <= 1e-13, (n, r)
 
 
+def test_pending_update_matches_sequential_order():
+    """ipg_substep leaves its Newton-Euler update pending: the next substep
+    runs it beside its RK stage when it is the RK3 successor (1 -> 2 -> 3 ->
+    1), first thing otherwise (a repeated or skipped stage), and every entry
+    that touches the particles, dt or u, v, w runs it before acting.  Here u
+    is rewritten and dt changed while an update is pending, and fluid-only
+    reads come in between; graphs on and off, and the side stream off, must
+    give the state of substeps each followed by ipg_sync (<= 1e-13: the IBM's
+    atomic sums are reordered from run to run)."""
+    case = CONFIGS["t32"]
+    fields, spheres, markers = initial_fields(case), place_spheres(case), markers_for(case)
+    ref = Simulation(case).initialize(fields, spheres, markers)
+    sims = [Simulation(case, use_graphs=True).initialize(fields, spheres, markers),
+            Simulation(case).initialize(fields, spheres, markers),
+            Simulation(case, use_graphs=True, side_stream=False).initialize(fields, spheres, markers)]
+    order = (1, 2, 3, 1, 1, 3, 2, 3, 1, 2)
+    for k, r in enumerate(order):
+        ref.substep(r)
+        ref.sync()
+        for s in sims:
+            s.substep(r)
+        if k == 3:
+            # fluid-only reads leave the update pending
+            for s in sims:
+                s.get("V")
+                s.chkdt()
+                s.chkdiv()
+        if k == 4:
+            # u rewritten while the update of substep 4 is pending: it must
+            # integrate the old u
+            u = ref.get("U") * 1.01
+            ref.put("U", u)
+            for s in sims:
+                s.put("U", u)
+        if k == 6:
+            # likewise dt
+            dt = ref.dt * 0.9
+            ref.set_dt(dt)
+            for s in sims:
+                s.set_dt(dt)
+    for s in sims:
+        s.sync()
+        for n in FLUID:
+            assert rel(s.get(n), ref.get(n)) <= 1e-13, n
+        for n, r in worst_part(s, ref).items():
+            assert r <= 1e-13, (n, r)
+
+
 def test_c1_100_steps():
     """The CPU-runnable reference case (64^3, 8 spheres, D = 8h) for 100 RK3
     steps: <= 1e-8 relative against the oracle."""
