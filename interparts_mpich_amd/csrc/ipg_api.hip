@@ -119,24 +119,25 @@ int release(ipg_ctx *c) {
 //    the forward x transform computes with the boundary values at the source;
 //  * boundp(p) (src/main.f90:361) is read at the source by correc; p's own
 //    ghosts are filled only when p is read back (ipg_field_get).
-// pend > 0: the previous substep's Newton-Euler update (its rkiter), run on
-// the side stream beside this substep's RK stage and joined before the IBM,
-// which needs the particles it moves.  defer: leave this substep's update
+// pend > 0: the previous substep's Newton-Euler update (its rkiter) and the
+// IBM neighbour lists of the positions it gives, run on the side stream
+// beside this substep's RK stage and joined before the IBM passes.  defer: leave this substep's update
 // pending instead of launching it at the end.
 int substep_launches(ipg_ctx *c, int rkiter, int pend, bool defer) {
   const bool np = c->pc.np > 0;
   if (pend) {
+    // the update, then the IBM's neighbour lists of the positions it gives
     IPG_HIP(hipEventRecord(c->fork_ev, c->stream));
     IPG_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
     std::swap(c->stream, c->side);
-    const int rc = ipg_launch_nwtn_eulr(c, pend);
+    const int rc = ipg_launch_nwtn_eulr(c, pend) || (np && ipg_launch_ibm_begin(c));
     std::swap(c->stream, c->side);
-    if (rc) return rc;
+    if (rc) return 1;
     IPG_HIP(hipEventRecord(c->join_ev, c->side));
   }
   if (ipg_launch_rk(c, rkiter, true)) return 1;
   if (pend) IPG_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
-  if (np && ipg_launch_ibm(c)) return 1;
+  if (np && (pend ? ipg_launch_ibm_passes(c) : ipg_launch_ibm(c))) return 1;
   if (ipg_launch_poisson_ex(c, rkiter)) return 1;
   if (ipg_launch_correc_fused(c, rkiter)) return 1;
   // cnew = dcdt (src/main.f90:375): the buffers swap roles, no copy; dcdt's
