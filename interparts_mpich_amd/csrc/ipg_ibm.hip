@@ -21,6 +21,7 @@
 // plane (src/interp_spread.f90:100 has no upper clamp), which never receives
 // dudt and keeps the last spread until the next one (planes K).
 #include "ipg_internal.h"
+#include "ipg_ibm_geom.h"
 #include <cmath>
 
 namespace {
@@ -30,25 +31,6 @@ constexpr int IBM_MPT = 2;        // markers per thread: nl <= 1024
 constexpr int IBM_MAXNB = 64;     // neighbour images per sphere
 constexpr int IBM_MAXNM = 4096;   // neighbour markers touching one (sphere, component) box
 
-// src/coordsfp.f90:49-68 / src/intgr_nwtn_eulr.f90:1451-1465 marker position
-// and the isout test; false when the marker is not kept.
-__device__ __forceinline__ bool marker_pos(const Grid &g, double xp, double yp, double zp, double ex, double ey, double ez,
-                                           double &xl, double &yl, double &zl) {
-  xl = xp + ex; yl = yp + ey; zl = zp + ez;
-  // a non-finite or far-away marker (a diverged run) is dropped rather than
-  // turned into an out-of-range index
-  if (!(fabs(xl) < 1e6 * g.lx && fabs(yl) < 1e6 * g.ly && fabs(zl) < 1e6 * g.lz)) return false;
-  double isx = 0.0, isy = 0.0;
-  if (xl < 0.0 + 0.5 * g.dx) isx = 1.0;
-  if (xl >= g.lx + 0.5 * g.dx) isx = -1.0;
-  if (yl < 0.0 + 0.5 * g.dy) isy = 1.0;
-  if (yl >= g.ly + 0.5 * g.dy) isy = -1.0;
-  const double cx = (xl + isx * g.lx - 0.0) * g.dxi;
-  if (nint_f(cx) < 1 || nint_f(cx) > g.nx) return false;
-  const double cy = (yl + isy * g.ly - 0.0) * g.dyi;
-  if (nint_f(cy) < 1 || nint_f(cy) > g.ny) return false;
-  return true;
-}
 
 struct IbmArgs {
   Grid g;
@@ -93,107 +75,6 @@ __device__ __forceinline__ int wrapany(int i, int n) {
   return r + 1;
 }
 
-// Cells [b0, b0+bn) per direction in unwrapped index space (z clipped to the
-// channel, 1..nz).
-struct Box { int bi0, bj0, bk0, bx, by, bz; };
-
-// The reach of a sphere: a box holding every stencil cell of its markers, of
-// any component (cells floor(X) - 1 .. floor(X) + 2 for a marker at X cells),
-// with one cell to spare at each end.  Used to find the spheres whose markers
-// may reach another sphere's boxes.
-__device__ __forceinline__ Box reach_box(const IbmArgs &A, double xp, double yp, double zp) {
-  const Grid &g = A.g;
-  Box b;
-  const int x0 = (int)floor((xp - A.radfp) * g.dxi), x1 = (int)floor((xp + A.radfp) * g.dxi);
-  const int y0 = (int)floor((yp - A.radfp) * g.dyi), y1 = (int)floor((yp + A.radfp) * g.dyi);
-  const int z0 = (int)floor((zp - A.radfp) * g.dzi), z1 = (int)floor((zp + A.radfp) * g.dzi);
-  b.bi0 = x0 - 2; b.bx = x1 + 3 - b.bi0 + 1;
-  b.bj0 = y0 - 2; b.by = y1 + 3 - b.bj0 + 1;
-  b.bk0 = max(z0 - 2, 1);
-  b.bz = max(min(z1 + 3, g.nz) - b.bk0 + 1, 0);
-  return b;
-}
-
-// First stencil cell of one direction over the markers of a sphere, [lo, hi].
-// A marker at X = x / h cells (x = centre + offset, offset in [-radfp,
-// radfp]) starts its stencil at floor(X) in a cell-centred direction and at
-// nint(X) - 1 in the component's own (face) direction.  Xa = (centre -
-// radfp) / h and Xb = (centre + radfp) / h bound X, but each of X, Xa and Xb
-// is rounded, and a rounding can push a floor or nint over an integer by one
-// (upward only for X, whose wrapped coordinate is positive); hence one cell
-// more than the exact bounds at each end:
-//   cell-centred: [floor(Xa) - 1, floor(Xb) + 1]
-//   face:         [floor(Xa + 1/2) - 2, floor(Xb + 1/2)]
-// so the stencils span at most ceil(2 radfp / h) + 5 cells (ibm_box_max).
-__device__ __forceinline__ void first_cell_range(double lo_pos, double hi_pos, double hinv, bool face, int &lo, int &hi) {
-  if (face) {
-    lo = (int)floor(lo_pos * hinv + 0.5) - 2;
-    hi = (int)floor(hi_pos * hinv + 0.5);
-  } else {
-    lo = (int)floor(lo_pos * hinv) - 1;
-    hi = (int)floor(hi_pos * hinv) + 1;
-  }
-}
-
-// The box of one (sphere, velocity component): the cells its markers'
-// stencils of that component can touch, z clipped to 1..nz (the planes above
-// kmax are the k1 plane, outside every box).  The first z plane of a stencil
-// is clamped to 1 (and to nz - 2 for u and v) as comp_stencil clamps it.
-__device__ __forceinline__ Box comp_box(const IbmArgs &A, double xp, double yp, double zp, int comp) {
-  const Grid &g = A.g;
-  int x0, x1, y0, y1, z0, z1;
-  first_cell_range(xp - A.radfp, xp + A.radfp, g.dxi, comp == 0, x0, x1);
-  first_cell_range(yp - A.radfp, yp + A.radfp, g.dyi, comp == 1, y0, y1);
-  first_cell_range(zp - A.radfp, zp + A.radfp, g.dzi, comp == 2, z0, z1);
-  if (comp != 2) { z0 = min(z0, g.nz - 2); z1 = min(z1, g.nz - 2); }
-  z0 = max(z0, 1);
-  z1 = max(z1, 1);
-  Box b;
-  b.bi0 = x0; b.bx = x1 + 2 - x0 + 1;
-  b.bj0 = y0; b.by = y1 + 2 - y0 + 1;
-  b.bk0 = z0; b.bz = max(min(z1 + 2, g.nz) - z0 + 1, 0);
-  return b;
-}
-
-// local box index of an unwrapped cell, -1 when outside the box
-__device__ __forceinline__ int box_at(const Box &b, int iu, int ju, int k) {
-  const int li = iu - b.bi0, lj = ju - b.bj0, lk = k - b.bk0;
-  if ((unsigned)li >= (unsigned)b.bx || (unsigned)lj >= (unsigned)b.by || (unsigned)lk >= (unsigned)b.bz) return -1;
-  return li + b.bx * (lj + b.by * lk);
-}
-
-// Marker stencil of one component in unwrapped indices: base cell (i0,j0,k0)
-// and the three weights per direction, in the reference's order
-// (src/interp_spread.f90:76-130).  Only the 9 weights the component uses are
-// evaluated; each is the same expression stencil() evaluates.
-// With WEIGHTS = false only the base cell is found.
-struct CompStencil { int i0, j0, k0; double wx[3], wy[3], wz[3]; };
-template <bool WEIGHTS = true>
-__device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl, double zl, int comp, CompStencil &c) {
-  const int sxw = (int)floor((xl - 0.5 * g.dx) / g.lx) * g.nx;
-  const int syw = (int)floor((yl - 0.5 * g.dy) / g.ly) * g.ny;
-  const double cx = xl * g.dxi - (double)sxw - 0.0;
-  const double cy = yl * g.dyi - (double)syw - 0.0;
-  const double cz = zl * g.dzi;
-  // component 0 sits on x faces: the x weights are the face (s) ones, y and z
-  // the half-shifted ones; likewise for 1 (y) and 2 (z)
-  if (comp == 0) {
-    c.i0 = nint_f(cx) - 1; c.j0 = (int)floor(cy); c.k0 = max(min((int)floor(cz), g.nz - 2), 1);
-  } else if (comp == 1) {
-    c.i0 = (int)floor(cx); c.j0 = nint_f(cy) - 1; c.k0 = max(min((int)floor(cz), g.nz - 2), 1);
-  } else {
-    c.i0 = (int)floor(cx); c.j0 = (int)floor(cy); c.k0 = max(nint_f(cz) - 1, 1);
-  }
-#pragma unroll
-  for (int q = 0; q < 3 && WEIGHTS; q++) {
-    c.wx[q] = roma_kernel(comp == 0 ? ((double)c.i0 + (double)q) - cx : (((double)c.i0 + (double)q) - 0.5) - cx);
-    c.wy[q] = roma_kernel(comp == 1 ? ((double)c.j0 + (double)q) - cy : (((double)c.j0 + (double)q) - 0.5) - cy);
-    c.wz[q] = roma_kernel(comp == 2 ? ((double)c.k0 + (double)q) - cz : (((double)c.k0 + (double)q) - 0.5) - cz);
-  }
-  c.i0 += sxw;
-  c.j0 += syw;
-}
-
 // One IBM pass for one (sphere, velocity component) workgroup of IBM_THREADS
 // threads; thread t owns markers t, t + IBM_THREADS, ... (at most IBM_MPT).
 // Each marker's position (src/coordsfp.f90:49-68), stencil base and nine
@@ -225,10 +106,10 @@ __global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int
   __shared__ int count;
   if (threadIdx.x == 0) count = 0;
   __syncthreads();
-  const Box bp = reach_box(A, A.px[p], A.py[p], A.pz[p]);
+  const Box bp = reach_box(A.g, A.radfp, A.px[p], A.py[p], A.pz[p]);
   for (int q = threadIdx.x; q < A.np; q += blockDim.x) {
     if (q == p) continue;
-    const Box bq = reach_box(A, A.px[q], A.py[q], A.pz[q]);
+    const Box bq = reach_box(A.g, A.radfp, A.px[q], A.py[q], A.pz[q]);
     if (!overlap(bp.bk0, bp.bz, bq.bk0, bq.bz)) continue;
     unsigned mx = 0u, my = 0u;
 #pragma unroll
@@ -302,7 +183,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   const double dt = *A.dtp, dti = 1.0 / dt;
   const double c0 = (A.dVlagr / A.dVeul) * dt;
   const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
-  Box b = comp_box(A, xp, yp, zp, comp);
+  Box b = comp_box(A.g, A.radfp, xp, yp, zp, comp);
   if (b.bx > A.bmx || b.by > A.bmy || b.bz > A.bmz) {
     // cannot happen for markers within radfp of the centre; kept as a check
     if (MODE == 0 && threadIdx.x == 0) atomicOr(A.err, 4);
@@ -597,12 +478,6 @@ size_t ibm_lds_bytes(int bmx, int bmy, int bmz) {
   return cells * sizeof(double) + 2 * ((cells + 31) / 32) * sizeof(unsigned) + ((size_t)bmy * bmz + bmx) * sizeof(int);
 }
 
-// Largest comp_box extent in a direction of spacing 1/hinv: the first stencil
-// cells span floor(Xb) - floor(Xa) + 2 (face: the same with Xa, Xb + 1/2) <=
-// ceil(Xb - Xa) + 2, and a stencil is 3 cells; Xb - Xa exceeds 2 radfp hinv by
-// roundings only, which an exact integer 2 radfp hinv would carry over to the
-// next integer, hence floor(.. + 1e-9) + 1 rather than ceil.
-int ibm_box_max(double radfp, double hinv) { return (int)std::floor(2.0 * radfp * hinv + 1e-9) + 1 + 5; }
 
 }  // namespace
 

@@ -129,15 +129,15 @@ void ipg_set_error(const std::string &msg);
     }                                                                                   \
   } while (0)
 
-// ---- device helpers --------------------------------------------------------
+// ---- host and device helpers ----------------------------------------------
 // Fortran nint: round half away from zero.
-__device__ __forceinline__ int nint_f(double x) { return (int)llround(x); }
-__device__ __forceinline__ double sqd(double x) { return x * x; }
+__host__ __device__ __forceinline__ int nint_f(double x) { return (int)llround(x); }
+__host__ __device__ __forceinline__ double sqd(double x) { return x * x; }
 
 // src/kernel.f90:7-25, Roma et al. 3-point regularized delta times h.
 // Both branches share one square root: the argument is selected first, so a
 // wavefront whose lanes fall in both branches evaluates one sqrt, not two.
-__device__ __forceinline__ double roma_kernel(double r) {
+__host__ __device__ __forceinline__ double roma_kernel(double r) {
   const double ar = fabs(r);
   const bool outer = ar > 0.5;
   const double s = sqrt(-(3.0 * sqd(outer ? 1.0 - ar : r)) + 1.0);
