@@ -249,7 +249,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
       (void)hipMemset(c->ps.q[q], 0, (size_t)np * sizeof(double));
     }
   }
-  size_t nred = 4096 + 3 * (size_t)np + 64;
+  const size_t nred = 4096;  // reduction scratch of chkdt (1024) and chkdiv (2048)
   if (hipMalloc(&c->red, nred * sizeof(double)) != hipSuccess || hipMalloc(&c->dscal, 16 * sizeof(double)) != hipSuccess) {
     ipg_set_error("out of device memory (scratch)"); return fail(1);
   }
@@ -513,7 +513,7 @@ int ipg_intgr_nwtn_eulr(ipg_ctx *c, int rkiter) {
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("intgr_nwtn_eulr: rkiter must be 1..3"); return 1; }
   if (ipg_launch_nwtn_eulr(c, rkiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
-  return 0;
+  return ipg_ibm_check(c);
 }
 
 int ipg_substep(ipg_ctx *c, int rkiter) {

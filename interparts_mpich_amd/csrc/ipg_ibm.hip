@@ -46,7 +46,7 @@ struct IbmArgs {
   const int *nbc;     // [np] neighbour image count
   int *nml;           // [np*3][IBM_MAXNM] neighbour markers touching the box: (q*nl + l)*25 + shift
   int *nmc;           // [np*3] their count
-  int *err;           // bits 1, 2: a neighbour list overflowed; 4: a stencil outside its box
+  int *err;           // device error word: bits 1, 2: a neighbour list overflowed; 4: a stencil outside its box
   unsigned *flg;      // [np*3][nflagmax] the box cells the own stencils touch, set by pass 0
   double *Kread, *Kspread, *Kzero;  // dwdtf(.,.,k1) planes
   const double *dtp;
@@ -603,9 +603,11 @@ int ipg_launch_ibm_totals(ipg_ctx *c, int pass) {
   return 0;
 }
 
-// A neighbour list that overflowed (more than 64 neighbour images: a cluster
-// far denser than random packing) or a stencil outside its sphere's box would
-// have dropped contributions: fail.
+// The device error word.  An IBM neighbour list that overflowed (more than 64
+// neighbour images: a cluster far denser than random packing), a stencil
+// outside its sphere's box, or a collision candidate list that overflowed
+// (more than 48 spheres within reach of one) would have dropped contributions:
+// fail.
 int ipg_ibm_check(ipg_ctx *c) {
   if (c->pc.np <= 0 || !c->ibm_err) return 0;
   int e = 0;
@@ -613,7 +615,8 @@ int ipg_ibm_check(ipg_ctx *c) {
   if (e) {
     ipg_set_error(e & 1   ? "ibm: more than 64 neighbour images around a sphere"
                   : e & 2 ? "ibm: more than 4096 neighbour markers touching one sphere's stencils"
-                          : "ibm: a marker stencil outside its sphere's box");
+                  : e & 4 ? "ibm: a marker stencil outside its sphere's box"
+                          : "collisions: more than 48 spheres within reach of one sphere");
     return 3;
   }
   return 0;

@@ -73,7 +73,7 @@ struct ipg_ctx {
   double *fl2;                    // [3][np*nl] marker forces after pass 1
   int *nbl, *nbc;                 // IBM neighbour images: [np][64] codes, [np] counts
   int *nml, *nmc;                 // IBM neighbour markers per (sphere, component): [3 np][4096], [3 np]
-  int *ibm_err;                   // set on the device when a neighbour list overflows
+  int *ibm_err;                   // device error word: IBM lists or boxes (1, 2, 4), collision candidates (8)
   unsigned *ibm_flg;              // per (sphere, component) box: cells its markers' stencils touch
   size_t ibm_flg_n;               // its length in words
   // particles

@@ -406,7 +406,7 @@ __global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc
   if (!live) return;
   wave_lds_sync();
   if (nc > MAXCAND) {
-    if (lane == 0) atomicOr(err, 1);
+    if (lane == 0) atomicOr(err, 8);
     nc = MAXCAND;
   }
   // the candidates' state, one lane per candidate
@@ -535,7 +535,7 @@ int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter) {
   const Motion state{q[IPG_PX], q[IPG_PY], q[IPG_PZ], q[IPG_PU], q[IPG_PV], q[IPG_PW], q[IPG_POMX], q[IPG_POMY], q[IPG_POMZ]};
   double *m = c->motion2;
   const Motion second{m, m + np, m + 2L * np, m + 3L * np, m + 4L * np, m + 5L * np, m + 6L * np, m + 7L * np, m + 8L * np};
-  int *err = (int *)(c->red + 3L * np);
+  int *err = c->ibm_err;  // device error word, bit 8: a candidate list overflowed
   const dim3 grid((np + COL_WAVES - 1) / COL_WAVES), blk(COL_WAVES * 64);
   hipLaunchKernelGGL(k_collide, grid, blk, 0, c->stream, c->g, c->pc, P, state, second, c->op, c->dscal, rkiter, 0,
                      c->ps.nbf, err);

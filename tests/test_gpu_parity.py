@@ -266,6 +266,25 @@ def test_pending_update_matches_sequential_order():
             assert r <= 1e-13, (n, r)
 
 
+def test_collision_candidate_overflow_is_reported():
+    """A sphere with more than 48 others within collision reach (2R(1 +
+    0.025)) cannot have them all in its candidate list; the collision pass
+    keeps 48, flags the overflow, and the Newton-Euler entry returns an error
+    instead of dropping contacts silently.  The crowd is set up from the host:
+    63 sphere centres inside a ball of radius 1.9 R around sphere 0."""
+    case = Case(name="crowd", itot=32, jtot=32, ktot=32, np=64, diameter_h=8.0, re_bulk=1000.0, seed=5)
+    R = case.radius
+    rng = np.random.default_rng(5)
+    d = rng.standard_normal((case.np, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d *= 1.9 * R * np.cbrt(rng.uniform(0.05, 1.0, case.np))[:, None]
+    d[0] = 0.0
+    xs, ys, zs = (0.5 + d[:, 0]) * case.lx, (0.5 + d[:, 1]) * case.ly, (0.5 + d[:, 2]) * case.lz
+    g = Simulation(case).initialize(initial_fields(case), (xs, ys, zs), markers_for(case))
+    with pytest.raises(RuntimeError, match="more than 48 spheres within reach"):
+        g.intgr_nwtn_eulr(1)
+
+
 def test_c1_100_steps():
     """The CPU-runnable reference case (64^3, 8 spheres, D = 8h) for 100 RK3
     steps: <= 1e-8 relative against the oracle."""
