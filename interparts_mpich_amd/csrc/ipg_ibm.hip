@@ -498,13 +498,14 @@ int ipg_ibm_setup(ipg_ctx *c) {
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  // the captured substeps hold the IBM launch arguments (marker radius, box
+  // sizes, dVlagr) by value: a new marker set-up drops them
+  for (int r = 0; r < IPG_NGRAPH; r++)
+    if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
   // the pass-0 cell flags; a larger box (a second set-up with a larger
-  // marker radius) takes a new buffer, and the captured graphs that hold the
-  // old one are dropped
+  // marker radius) takes a new buffer
   const size_t nflag = 3 * (size_t)c->pc.np * (((size_t)bmx * bmy * bmz + 31) / 32);
   if (nflag > c->ibm_flg_n) {
-    for (int r = 0; r < IPG_NGRAPH; r++)
-      if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
     if (c->ibm_flg) (void)hipFree(c->ibm_flg);
     c->ibm_flg = nullptr;
     c->ibm_flg_n = 0;

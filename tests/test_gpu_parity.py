@@ -266,6 +266,30 @@ def test_pending_update_matches_sequential_order():
             assert r <= 1e-13, (n, r)
 
 
+def test_marker_setup_between_steps_drops_captured_graphs():
+    """The captured substeps hold the IBM launch arguments (marker radius, box
+    sizes, dVlagr) by value: a second ipg_coordsfp with a smaller marker
+    radius must be seen by the replayed substeps as by eager ones."""
+    import ctypes as C
+    case = CONFIGS["t32"]
+    fields, spheres, markers = initial_fields(case), place_spheres(case), markers_for(case)
+    a = Simulation(case, use_graphs=True).initialize(fields, spheres, markers)
+    b = Simulation(case).initialize(fields, spheres, markers)
+    a.step(); b.step()
+    theta, phi, radfp = markers
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    for s in (a, b):
+        rc = s.lib.ipg_coordsfp(s.ctx, theta.ctypes.data_as(C.c_void_p), phi.ctypes.data_as(C.c_void_p),
+                                C.c_double(0.9 * radfp))
+        assert rc == 0
+    a.step(); b.step()
+    for n in FLUID:
+        assert rel(a.get(n), b.get(n)) <= 1e-13, n
+    for n, r in worst_part(a, b).items():
+        assert r <= 1e-13, (n, r)
+
+
 def test_collision_candidate_overflow_is_reported():
     """A sphere with more than 48 others within collision reach (2R(1 +
     0.025)) cannot have them all in its candidate list; the collision pass
