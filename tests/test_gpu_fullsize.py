@@ -10,6 +10,8 @@ where the oracle would take too long: properties that hold at any size.
 - The projection: after a full substep max |div u| <= 1e-10 (src/chkdiv.f90
   definition), the north-star bound for the FP64 path.
 - Graph replay equals the eager launch sequence to <= 1e-13 relative.
+- One time step at 256^3 with 1 000 spheres against the FP64 oracle (a few
+  seconds of the oracle on the host's cores): <= 1e-12 relative.
 """
 import numpy as np
 import pytest
@@ -66,9 +68,10 @@ def test_ibm_momentum_balance_and_divergence(name):
 
 
 def test_graph_replay_full_size():
-    """Four steps: each of the six graphs (rkiter x the parity of the c/dcdt
-    buffer swap) is captured once and replayed once; the result matches the
-    eager launches (IBM atomics reorder sums: <= 1e-11 after 12 substeps)."""
+    """Four steps: the graphs (rkiter x the parity of the c/dcdt buffer swap x
+    whether the previous substep's particle update is carried on the side
+    stream) are captured and replayed; the result matches the eager launches
+    (IBM atomics reorder sums: <= 1e-11 after 12 substeps)."""
     case = CONFIGS["c2_256"]
     a = make(case)
     b = make(case, graphs=True)
@@ -84,3 +87,28 @@ def test_graph_replay_full_size():
     assert np.array_equal(b.get("DCDT")[1:-1, 1:-1, 1:-1], b.get("C")[1:-1, 1:-1, 1:-1])
     for q in ("X", "U", "FXLTOT"):
         assert rel(b.pget(q), a.pget(q)) <= 1e-11, q
+
+
+def test_c2_one_step_matches_oracle():
+    """The headline case itself (256^3, 1 000 spheres, D = 12h): one time step
+    (three substeps, graphs on) against the FP64 CPU oracle on identical
+    seeded inputs, <= 1e-12 relative per field and particle quantity."""
+    from oracle.pyoracle import Oracle
+    case = CONFIGS["c2_256"]
+    fields, spheres, markers = initial_fields(case), place_spheres(case), markers_for(case)
+    g = Simulation(case, use_graphs=True).initialize(fields, spheres, markers)
+    o = Oracle(case).initialize(fields, spheres, markers)
+    print("[c2] oracle initialised", flush=True)
+    g.step()
+    o.step()
+    assert g.dt == o.dt or abs(g.dt - o.dt) <= 1e-14 * o.dt
+    for n in ("U", "V", "W", "PNEW", "C"):
+        r = rel(g.get(n), o.get(n))
+        print(n, r)
+        assert r <= 1e-12, (n, r)
+    for q in ("X", "Y", "Z", "U", "V", "W", "INTU", "INTV", "INTW", "FXLTOT", "FYLTOT", "FZLTOT"):
+        r = rel(g.pget(q), o.pget(q))
+        print(q, r)
+        assert r <= 1e-12, (q, r)
+    o.close()
+    g.close()
