@@ -26,6 +26,13 @@
 namespace {
 
 constexpr int FFT_THREADS = 256;
+// Plane order of a pass.  A pass reads best the planes its predecessor wrote
+// last, which the device's last-level cache still holds: the forward x pass
+// writes the spectrum from the bottom plane up, so the forward y pass starts
+// at the top; the Thomas back substitution ends at the bottom, so the inverse
+// y pass starts there and ends at the top, where the inverse x pass starts;
+// it ends at the bottom, where the projection starts (0.2 % of a substep).
+constexpr int Y_FWD_TOP_FIRST = 1, X_INV_TOP_FIRST = 1;
 constexpr int LDS_COMPLEX = 2048;  // complex elements per ping-pong buffer (32 KiB)
 
 struct Radices { int r[16]; int n; };
@@ -232,11 +239,12 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, const doubl
 // y transform of a tile of B x-modes of one z-plane, in place in spec.
 template <int N>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_y_t(Grid g, double2 *__restrict__ spec, long sp, int nmx,
-                                                         const double2 *__restrict__ tw, int sign) {
+                                                         const double2 *__restrict__ tw, int sign, int top_first) {
   constexpr int B = batch_of<N>();
   __shared__ double2 buf[2 * B * N];
   const int tiles = (nmx + B - 1) / B;
-  const int k = blockIdx.x / tiles, m0 = (blockIdx.x % tiles) * B;
+  const int kb = blockIdx.x / tiles, m0 = (blockIdx.x % tiles) * B;
+  const int k = top_first ? g.nz - 1 - kb : kb;
   double2 *plane = spec + (long)k * g.ny * sp;
   // every load of the tile in flight before the first LDS store
   constexpr int IT = (B * N + FFT_THREADS - 1) / FFT_THREADS;
@@ -265,11 +273,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_y_t(Grid g, double2 *__rest
 // with the overall 1/(nx*ny) normalisation.
 template <int N>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, double *__restrict__ p, const double2 *__restrict__ spec,
-                                                             long sp, const double2 *__restrict__ tw, double scale) {
+                                                             long sp, const double2 *__restrict__ tw, double scale, int top_first) {
   constexpr int B = xbatch_of<N>();
   __shared__ double2 buf[2 * B * N];
   const long pairs = (long)(g.ny / 2) * g.nz;
-  const long pair0 = (long)blockIdx.x * B;
+  const long pair0 = (long)(top_first ? gridDim.x - 1 - blockIdx.x : blockIdx.x) * B;
   constexpr int half = N / 2;
   // modes m > N/2 are the conjugates of N - m; every load of the block in
   // flight before the first LDS store
@@ -549,13 +557,13 @@ void launch_x_inv(const XArgs &a, hipStream_t s) {
   constexpr int B = xbatch_of<N>();
   const long pairs = (long)(a.g.ny / 2) * a.g.nz;
   hipLaunchKernelGGL(k_fft_x_inv_t<N>, dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g, a.p, a.spec,
-                     a.sp, a.tw, a.scale);
+                     a.sp, a.tw, a.scale, X_INV_TOP_FIRST);
 }
 template <int N>
-void launch_y(const Grid &g, double2 *spec, long sp, int nmx, const double2 *tw, int sign, hipStream_t s) {
+void launch_y(const Grid &g, double2 *spec, long sp, int nmx, const double2 *tw, int sign, int top_first, hipStream_t s) {
   constexpr int B = batch_of<N>();
   const int tiles = (nmx + B - 1) / B;
-  hipLaunchKernelGGL(k_fft_y_t<N>, dim3(g.nz * tiles), dim3(FFT_THREADS), 0, s, g, spec, sp, nmx, tw, sign);
+  hipLaunchKernelGGL(k_fft_y_t<N>, dim3(g.nz * tiles), dim3(FFT_THREADS), 0, s, g, spec, sp, nmx, tw, sign, top_first);
 }
 
 #define IPG_FFT_DISPATCH(n, CALL) \
@@ -672,7 +680,7 @@ int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter) {
   }
   ipg_mark(c, "fft_x_fwd");
   if (specialised(g.ny)) {
-#define CALLY(n) launch_y<n>(g, P.spec, P.sp, P.nmx, P.twy, -1, c->stream)
+#define CALLY(n) launch_y<n>(g, P.spec, P.sp, P.nmx, P.twy, -1, Y_FWD_TOP_FIRST, c->stream)
     IPG_FFT_DISPATCH(g.ny, CALLY)
 #undef CALLY
   } else {
@@ -685,7 +693,7 @@ int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter) {
                      P.a, P.b, P.c, P.xrt, P.yrt);
   ipg_mark(c, "thomas");
   if (specialised(g.ny)) {
-#define CALLY(n) launch_y<n>(g, P.spec, P.sp, P.nmx, P.twy, +1, c->stream)
+#define CALLY(n) launch_y<n>(g, P.spec, P.sp, P.nmx, P.twy, +1, 0, c->stream)
     IPG_FFT_DISPATCH(g.ny, CALLY)
 #undef CALLY
   } else {
