@@ -26,6 +26,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def thomas_words(case, Sw):
+    """FP64 words the Thomas solve moves: rows 1 .. K-48 (K = ktot-1) four times, the last 48
+    (the kernel's register tail, when K > 48) and row ktot twice, plus the multipliers d."""
+    K = case.ktot - 1
+    T = 48 if K > 48 else 0
+    return Sw * (4.0 * (K - T) + 2.0 * T + 2.0) / case.ktot + Sw * (case.jtot // 2 + 1) / case.jtot
+
+
 def kernel_bytes(case, name, rkiter):
     """Algorithmic HBM bytes of one launch of a kernel (FP64 words x 8).
     N cells; Sw = FP64 words of the half spectrum, 2 (nx/2+1) ny nz."""
@@ -36,9 +44,10 @@ def kernel_bytes(case, name, rkiter):
         "fillps": 4.0 * N,
         "fft_x_fwd": 3.0 * N + Sw,    # fillps fused: reads dudt dvdt dwdt, writes the half spectrum
         "fft_y_fwd": 2.0 * Sw,
-        # spectrum read + written in each sweep; d (one word per mode and plane, shared by the
-        # y-modes +-ky) written then read
-        "thomas": 4.0 * Sw + Sw * (case.jtot // 2 + 1) / case.jtot,
+        # spectrum read + written in each sweep, except the last 48 rows of each column (kept in
+        # registers between the sweeps: read once, written once) and the last row; d (one word per
+        # mode and plane, shared by the y-modes +-ky) written then read
+        "thomas": thomas_words(case, Sw),
         "fft_y_inv": 2.0 * Sw,
         "fft_x_inv": Sw + N,
         "correc": 9.0 * N,            # dudt dvdt dwdt p pnew -> u v w pnew (cnew = dcdt is a buffer swap)

@@ -422,6 +422,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv(Grid g, double *__res
 // bound by memory latency, so both sweeps are software-pipelined: the next TD
 // planes are loaded while the current TD are eliminated.
 constexpr int TD = 16;
+// rows per column kept in registers (a multiple of TD): 48 keeps two
+// wavefronts per SIMD (216 VGPRs); 64 would leave one
+constexpr int THOMAS_TAIL = 48;
 __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
                                                 const double *__restrict__ a, const double *__restrict__ b,
                                                 const double *__restrict__ cc, const double *__restrict__ xrt,
@@ -446,26 +449,32 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
   const long dbase = (long)pr * sp + m;               // the pair's row of d
   double *S = spec + part;                            // element (k, base): S[2 * (base + k plane)]
   const double xy = xrt[m] + yrt[my];
-  // forward elimination, k = 1 .. nz-1 (z = 1/(b + xy - a dprev))
+  // forward elimination, k = 1 .. K = nz-1 (z = 1/(b + xy - a dprev)).  The
+  // last THOMAS_TAIL rows are kept in registers, not stored, and the back
+  // substitution starts from them: those rows cross HBM once each way instead
+  // of twice.  Rows 1 .. kf go through memory.
+  const int K = nz - 1;
+  const bool tail = K > THOMAS_TAIL;
+  const int kf = tail ? K - THOMAS_TAIL : K;
   double dprev = 0.0, pp = 0.0;
   double vn[TD];
 #pragma unroll
-  for (int r = 0; r < TD; r++) vn[r] = 1 + r <= nz - 1 ? S[2 * (base + (long)r * plane)] : 0.0;
-  for (int k0 = 1; k0 <= nz - 1; k0 += TD) {
+  for (int r = 0; r < TD; r++) vn[r] = 1 + r <= kf ? S[2 * (base + (long)r * plane)] : 0.0;
+  for (int k0 = 1; k0 <= kf; k0 += TD) {
     double v[TD];
 #pragma unroll
     for (int r = 0; r < TD; r++) v[r] = vn[r];
-    if (k0 + TD <= nz - 1) {
+    if (k0 + TD <= kf) {
 #pragma unroll
       for (int r = 0; r < TD; r++) {
         const int k = k0 + TD + r;
-        vn[r] = k <= nz - 1 ? S[2 * (base + (long)(k - 1) * plane)] : 0.0;
+        vn[r] = k <= kf ? S[2 * (base + (long)(k - 1) * plane)] : 0.0;
       }
     }
 #pragma unroll
     for (int r = 0; r < TD; r++) {
       const int k = k0 + r;
-      if (k <= nz - 1) {
+      if (k <= kf) {
         const long q = base + (long)(k - 1) * plane;
         double z;
         if (k == 1) z = 1.0 / (b[1] + xy);
@@ -480,6 +489,26 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
       }
     }
   }
+  // rows kf+1 .. K (k >= 2) into registers
+  double wr[THOMAS_TAIL];
+  if (tail) {
+#pragma unroll
+    for (int bb = 0; bb < THOMAS_TAIL / TD; bb++) {
+      double v[TD];
+#pragma unroll
+      for (int r = 0; r < TD; r++) v[r] = S[2 * (base + (long)(kf + bb * TD + r) * plane)];
+#pragma unroll
+      for (int r = 0; r < TD; r++) {
+        const int k = kf + 1 + bb * TD + r;
+        const double z = 1.0 / ((b[k] + xy) - a[k] * dprev);
+        dprev = cc[k] * z;
+        if (side == 0) dtri[dbase + (long)(k - 1) * plane] = dprev;
+        const double w = (v[r] - a[k] * pp) * z;
+        wr[bb * TD + r] = w;
+        pp = w;
+      }
+    }
+  }
   {  // last row (singular kx = ky = 0 guard)
     const long q = base + (long)(nz - 1) * plane;
     const double z = (b[nz] + xy) - a[nz] * dprev;
@@ -490,15 +519,30 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
   // back substitution, k = nz-1 .. 1; side 1 reads the d that side 0 of this
   // wavefront stored
   __threadfence_block();
+  if (tail) {
+#pragma unroll
+    for (int bb = THOMAS_TAIL / TD - 1; bb >= 0; bb--) {
+      double d[TD];
+#pragma unroll
+      for (int r = 0; r < TD; r++) d[r] = dtri[dbase + (long)(kf + bb * TD + TD - 1 - r) * plane];
+#pragma unroll
+      for (int r = 0; r < TD; r++) {
+        const int k = kf + 1 + bb * TD + TD - 1 - r;
+        const double w = wr[bb * TD + TD - 1 - r] - d[r] * pp;
+        S[2 * (base + (long)(k - 1) * plane)] = w;
+        pp = w;
+      }
+    }
+  }
   double vb[TD], db[TD];
 #pragma unroll
   for (int r = 0; r < TD; r++) {
-    const int k = nz - 1 - r;
+    const int k = kf - r;
     const long q = base + (long)(k - 1) * plane;
     vb[r] = k >= 1 ? S[2 * q] : 0.0;
     db[r] = k >= 1 ? dtri[dbase + (long)(k - 1) * plane] : 0.0;
   }
-  for (int k0 = nz - 1; k0 >= 1; k0 -= TD) {
+  for (int k0 = kf; k0 >= 1; k0 -= TD) {
     double v[TD], d[TD];
 #pragma unroll
     for (int r = 0; r < TD; r++) { v[r] = vb[r]; d[r] = db[r]; }
