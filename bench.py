@@ -156,7 +156,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300, help="timed substeps")
     ap.add_argument("--warmup", type=int, default=30, help="untimed substeps")
-    ap.add_argument("--config", default="c2_256")
+    ap.add_argument("--config", default="ch256_1000sph")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default RK-stage launch shape")
     ap.add_argument("--cpu-substeps", type=int, default=12, help="oracle substeps in the CPU-baseline sample")

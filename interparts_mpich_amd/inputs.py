@@ -107,11 +107,11 @@ def nl_formula(itot, lx, radius):
 # The configurations the benchmark and the parity tests run (sizes of the
 # reference method's channel: lx = ly = lz = 1, flow along y, walls in z).
 CONFIGS = {
-    "c1_64": Case(name="c1_64", itot=64, jtot=64, ktot=64, np=8, diameter_h=8.0, ratiorho=1.5, re_bulk=2000.0,
+    "ch64_8sph": Case(name="ch64_8sph", itot=64, jtot=64, ktot=64, np=8, diameter_h=8.0, ratiorho=1.5, re_bulk=2000.0,
                   seed=1, steps=100),
-    "c2_256": Case(name="c2_256", itot=256, jtot=256, ktot=256, np=1000, diameter_h=12.0, ratiorho=1.5,
+    "ch256_1000sph": Case(name="ch256_1000sph", itot=256, jtot=256, ktot=256, np=1000, diameter_h=12.0, ratiorho=1.5,
                    re_bulk=5000.0, seed=2, steps=1000),
-    "c4_512": Case(name="c4_512", itot=512, jtot=512, ktot=512, np=8000, diameter_h=16.0, ratiorho=2.0,
+    "ch512_8000sph": Case(name="ch512_8000sph", itot=512, jtot=512, ktot=512, np=8000, diameter_h=16.0, ratiorho=2.0,
                    re_bulk=10000.0, seed=4, steps=200),
     # small parity cases
     "t16": Case(name="t16", itot=16, jtot=16, ktot=16, np=2, diameter_h=5.0, ratiorho=1.5, re_bulk=200.0, seed=7,

@@ -8,7 +8,7 @@ where the oracle would take too long: properties that hold at any size.
   clamps the w stencil to k >= 1 (src/interp_spread.f90:100) and the weight of
   the wall plane is dropped.
 - The projection: after a full substep max |div u| <= 1e-10 (src/chkdiv.f90
-  definition), the north-star bound for the FP64 path.
+  definition), the bound the FP64 path is held to.
 - Graph replay equals the eager launch sequence to <= 1e-13 relative.
 - One time step at 256^3 with 1 000 spheres against the FP64 oracle (a few
   seconds of the oracle on the host's cores): <= 1e-12 relative.
@@ -31,7 +31,7 @@ def make(case, graphs=False):
     return Simulation(case, use_graphs=graphs).initialize(initial_fields(case), place_spheres(case), markers_for(case))
 
 
-@pytest.mark.parametrize("name", ["c2_256", "c4_512"])
+@pytest.mark.parametrize("name", ["ch256_1000sph", "ch512_8000sph"])
 def test_ibm_momentum_balance_and_divergence(name):
     case = CONFIGS[name]
     g = make(case)
@@ -72,7 +72,7 @@ def test_graph_replay_full_size():
     whether the previous substep's particle update is carried on the side
     stream) are captured and replayed; the result matches the eager launches
     (IBM atomics reorder sums: <= 1e-11 after 12 substeps)."""
-    case = CONFIGS["c2_256"]
+    case = CONFIGS["ch256_1000sph"]
     a = make(case)
     b = make(case, graphs=True)
     for s in range(12):
@@ -89,16 +89,16 @@ def test_graph_replay_full_size():
         assert rel(b.pget(q), a.pget(q)) <= 1e-11, q
 
 
-def test_c2_one_step_matches_oracle():
+def test_ch256_one_step_matches_oracle():
     """The headline case itself (256^3, 1 000 spheres, D = 12h): one time step
     (three substeps, graphs on) against the FP64 CPU oracle on identical
     seeded inputs, <= 1e-12 relative per field and particle quantity."""
     from oracle.pyoracle import Oracle
-    case = CONFIGS["c2_256"]
+    case = CONFIGS["ch256_1000sph"]
     fields, spheres, markers = initial_fields(case), place_spheres(case), markers_for(case)
     g = Simulation(case, use_graphs=True).initialize(fields, spheres, markers)
     o = Oracle(case).initialize(fields, spheres, markers)
-    print("[c2] oracle initialised", flush=True)
+    print("[ch256_1000sph] oracle initialised", flush=True)
     g.step()
     o.step()
     assert g.dt == o.dt or abs(g.dt - o.dt) <= 1e-14 * o.dt

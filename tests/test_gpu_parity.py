@@ -46,7 +46,7 @@ def worst_part(g, o, names=PART):
     return {n: rel(g.pget(n), o.pget(n)) for n in names}
 
 
-@pytest.mark.parametrize("name", ["t16", "t24x16x32", "c1_64"])
+@pytest.mark.parametrize("name", ["t16", "t24x16x32", "ch64_8sph"])
 def test_initialize_parity(name):
     g, o = pair(CONFIGS[name])
     assert g.dt == o.dt
@@ -77,7 +77,7 @@ def test_rk_stage_bitwise(name, rkiter):
         assert np.array_equal(a, b), (n, rel(a, b))
 
 
-@pytest.mark.parametrize("name", ["t16", "t24x16x32", "c1_64"])
+@pytest.mark.parametrize("name", ["t16", "t24x16x32", "ch64_8sph"])
 def test_routine_sequence(name):
     """One substep routine by routine (src/main.f90:268-414), comparing after
     each call; each call starts from identical state on both sides."""
@@ -128,7 +128,7 @@ def test_routine_sequence(name):
             assert r <= 1e-12, (n, r)
 
 
-@pytest.mark.parametrize("name", ["t16", "t32", "t24x16x32", "c1_64"])
+@pytest.mark.parametrize("name", ["t16", "t32", "t24x16x32", "ch64_8sph"])
 def test_step_parity(name):
     """One time step (three substeps) through ipg_substep: <= 1e-12 relative."""
     g, o = pair(CONFIGS[name])
@@ -180,7 +180,7 @@ def test_step_parity_variants(name):
     assert abs(g.dt - o.dt) <= 1e-13 * o.dt
 
 
-@pytest.mark.parametrize("name", ["t32", "c1_64"])
+@pytest.mark.parametrize("name", ["t32", "ch64_8sph"])
 def test_split_ibm_entries_match_ibm(name):
     """The reference-shaped IBM entries (src/main.f90:281-332: eulr2lagr,
     complagrforces/updtlagrforces, lagr2eulr per ibmiter) called one by one
@@ -309,10 +309,10 @@ def test_collision_candidate_overflow_is_reported():
         g.intgr_nwtn_eulr(1)
 
 
-def test_c1_100_steps():
+def test_ch64_100_steps():
     """The CPU-runnable reference case (64^3, 8 spheres, D = 8h) for 100 RK3
     steps: <= 1e-8 relative against the oracle."""
-    case = CONFIGS["c1_64"]
+    case = CONFIGS["ch64_8sph"]
     g, o = pair(case)
     for s in range(case.steps):
         g.step(); o.step()
@@ -362,7 +362,7 @@ def sum_tolerance(o, dt):
     return 5.0 * fl, 5.0 * acc
 
 
-@pytest.mark.parametrize("name", ["t32", "c1_64"])
+@pytest.mark.parametrize("name", ["t32", "ch64_8sph"])
 def test_substep_sums_match_oracle(name):
     """rkp%fxltot.. and rkp%dudt.. (sumrk3, src/intgr_nwtn_eulr.f90:1586-1669)
     after two steps, to 1e-12 of the size of the summed terms."""
