@@ -26,12 +26,40 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def thomas_d_rows(case):
+    """Rows of the Thomas multipliers d stored and read back, summed over the (x-mode, |y-mode|)
+    columns: rows 1 .. the last row where d changes.  Between the end rows d follows one map
+    and reaches a fixed point; the solve sends the later rows' stores and loads to that row,
+    which stays in the caches.  The forward recurrence of src/solver.f90:90-135 is repeated here
+    with the same tables (src/initsolver.f90:17-56) to find the row per column."""
+    import numpy as np
+    nx, ny, nz = case.itot, case.jtot, case.ktot
+    dxi, dyi, dzi = nx / case.lx, ny / case.ly, nz / case.lz
+    f = np.arange(1, nx // 2, dtype=np.float64)
+    lx = np.concatenate(([0.0], -4.0 * dxi * dxi * np.sin(f * np.pi / nx) ** 2, [-4.0 * dxi * dxi]))
+    f = np.arange(1, ny // 2, dtype=np.float64)
+    ly = np.concatenate(([0.0], -4.0 * dyi * dyi * np.sin(f * np.pi / ny) ** 2, [-4.0 * dyi * dyi]))
+    xy = (lx[:, None] + ly[None, :]).ravel()
+    a = c = dzi * dzi
+    b = -(a + c)
+    K = nz - 1
+    dprev = c * (1.0 / ((b + a) + xy))            # row 1 (Neumann b)
+    last = np.ones(xy.shape, dtype=np.int64)
+    for k in range(2, K + 1):
+        dn = c * (1.0 / ((b + xy) - a * dprev))
+        last[dn != dprev] = k
+        dprev = dn
+    return float(last.sum())
+
+
 def thomas_words(case, Sw):
     """FP64 words the Thomas solve moves: rows 1 .. K-48 (K = ktot-1) four times, the last 48
-    (the kernel's register tail, when K > 48) and row ktot twice, plus the multipliers d."""
+    (the kernel's register tail, when K > 48) and row ktot twice, plus the multipliers d up to
+    the row where they stop changing, written then read (one word per x-mode, |y-mode| and
+    row, shared by the y-modes +-ky)."""
     K = case.ktot - 1
     T = 48 if K > 48 else 0
-    return Sw * (4.0 * (K - T) + 2.0 * T + 2.0) / case.ktot + Sw * (case.jtot // 2 + 1) / case.jtot
+    return Sw * (4.0 * (K - T) + 2.0 * T + 2.0) / case.ktot + 2.0 * thomas_d_rows(case)
 
 
 def kernel_bytes(case, name, rkiter):
@@ -46,7 +74,7 @@ def kernel_bytes(case, name, rkiter):
         "fft_y_fwd": 2.0 * Sw,
         # spectrum read + written in each sweep, except the last 48 rows of each column (kept in
         # registers between the sweeps: read once, written once) and the last row; d (one word per
-        # mode and plane, shared by the y-modes +-ky) written then read
+        # mode and plane, shared by the y-modes +-ky) written then read up to its fixed point
         "thomas": thomas_words(case, Sw),
         "fft_y_inv": 2.0 * Sw,
         "fft_x_inv": Sw + N,

@@ -151,7 +151,8 @@ int ipg_use_side_stream(ipg_ctx *ctx, int on);
 /* time the RK stage for 8..64 z planes per workgroup and keep the fastest;
  * overwrites dudt..dcdt (recomputed by every stage): call between substeps */
 int ipg_autotune(ipg_ctx *ctx, int *kb);
-/* HBM bytes one substep must move by the algorithmic model, for roofline */
+/* HBM bytes one substep must move by the algorithmic model, for roofline
+ * (after ipg_initsolver) */
 int ipg_substep_bytes(ipg_ctx *ctx, double *bytes);
 /* one eager substep with a HIP event after every kernel launch: per-launch
  * durations (ms) and kernel names, at most maxn, count in *n */

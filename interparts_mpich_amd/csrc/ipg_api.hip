@@ -670,17 +670,19 @@ int ipg_use_side_stream(ipg_ctx *c, int on) {
 // Algorithmic HBM bytes of one substep (FP64), the roofline numerator.
 int ipg_substep_bytes(ipg_ctx *c, double *bytes) {
   if (!check_ctx(c) || !bytes) return 1;
+  if (!c->solver_ready) { ipg_set_error("substep_bytes: ipg_initsolver has not been called"); return 1; }
   // FP64 words per substep, by kernel: RK stage 43/3 N (13, 17, 13 by stage),
   // forward x FFT with fillps 3 N + Sw, two y passes 2 Sw each, Thomas (the
   // spectrum read, written, re-read and re-written in rows 1 .. nz-1-48, once
   // each way in the last 48 rows, kept in registers, and in row nz; d shared
-  // by the y-modes +-ky, written and read) Sw (4 (K - 48) + 2 * 48 + 2) / nz +
-  // Sw (ny/2+1)/ny with K = nz-1 (4 Sw (nz-1)/nz + 2 Sw / nz when K <= 48),
-  // inverse x FFT Sw + N, correc 9 N; Sw = 2 (nx/2+1) ny nz.
+  // by the y-modes +-ky, written and read up to the row where it stops
+  // changing) Sw (4 (K - 48) + 2 * 48 + 2) / nz + 2 d_rows with K = nz-1
+  // (4 Sw (nz-1)/nz + 2 Sw / nz when K <= 48), inverse x FFT Sw + N, correc
+  // 9 N; Sw = 2 (nx/2+1) ny nz.
   const double N = (double)c->g.nx * c->g.ny * c->g.nz;
   const double Sw = 2.0 * (c->g.nx / 2 + 1) * (double)c->g.ny * c->g.nz;
   const int K = c->g.nz - 1, T = K > 48 ? 48 : 0;
-  const double thomas = Sw * (4.0 * (K - T) + 2.0 * T + 2.0) / c->g.nz + Sw * (double)(c->g.ny / 2 + 1) / c->g.ny;
+  const double thomas = Sw * (4.0 * (K - T) + 2.0 * T + 2.0) / c->g.nz + 2.0 * c->ps2.d_rows;
   const double fluid = (43.0 / 3.0 + 3.0 + 1.0 + 9.0) * N + (1.0 + 4.0 + 1.0) * Sw + thomas;
   // IBM: per pass and component the stencil shell (radfp +- 2.5 h) is read
   // once from dudt (+ G) and receives one atomic add; sphere integrals read

@@ -51,6 +51,7 @@ struct Poisson {
   double *xrt, *yrt;// eigenvalues by x-mode m (nmx) and y-mode my (ny)
   double2 *twx, *twy; // e^{-2 pi i t/n} tables
   int radx[16], nradx, rady[16], nrady;
+  double d_rows;    // rows of d stored and read back, over the (m, |my|) columns
 };
 
 struct ipg_ctx {

@@ -457,6 +457,12 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
   const bool tail = K > THOMAS_TAIL;
   const int kf = tail ? K - THOMAS_TAIL : K;
   double dprev = 0.0, pp = 0.0;
+  // d goes to row dlast, the last row where it changed: later rows hold the
+  // same value.  Between the end rows a, b, c do not depend on k, so d
+  // follows one monotone map and, rounded, reaches a fixed point it then
+  // keeps, in most columns within a few dozen rows; from there on its stores
+  // and loads hit one row, which the caches hold.
+  double *dlast = dtri + dbase;
   double vn[TD];
 #pragma unroll
   for (int r = 0; r < TD; r++) vn[r] = 1 + r <= kf ? S[2 * (base + (long)r * plane)] : 0.0;
@@ -479,10 +485,12 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
         double z;
         if (k == 1) z = 1.0 / (b[1] + xy);
         else z = 1.0 / ((b[k] + xy) - a[k] * dprev);
-        dprev = cc[k] * z;
+        const double dn = cc[k] * z;
+        if (dn != dprev) dlast = dtri + dbase + (long)(k - 1) * plane;
+        dprev = dn;
         // side 0's two parts store the same value; side 1 reads it back in
         // the back substitution of this wavefront
-        if (side == 0) dtri[dbase + (long)(k - 1) * plane] = dprev;
+        if (side == 0) *dlast = dprev;
         const double w = k == 1 ? v[r] * z : (v[r] - a[k] * pp) * z;
         S[2 * q] = w;
         pp = w;
@@ -501,8 +509,10 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
       for (int r = 0; r < TD; r++) {
         const int k = kf + 1 + bb * TD + r;
         const double z = 1.0 / ((b[k] + xy) - a[k] * dprev);
-        dprev = cc[k] * z;
-        if (side == 0) dtri[dbase + (long)(k - 1) * plane] = dprev;
+        const double dn = cc[k] * z;
+        if (dn != dprev) dlast = dtri + dbase + (long)(k - 1) * plane;
+        dprev = dn;
+        if (side == 0) *dlast = dprev;
         const double w = (v[r] - a[k] * pp) * z;
         wr[bb * TD + r] = w;
         pp = w;
@@ -524,7 +534,10 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
     for (int bb = THOMAS_TAIL / TD - 1; bb >= 0; bb--) {
       double d[TD];
 #pragma unroll
-      for (int r = 0; r < TD; r++) d[r] = dtri[dbase + (long)(kf + bb * TD + TD - 1 - r) * plane];
+      for (int r = 0; r < TD; r++) {
+        const double *q = dtri + dbase + (long)(kf + bb * TD + TD - 1 - r) * plane;
+        d[r] = *(q < dlast ? q : dlast);
+      }
 #pragma unroll
       for (int r = 0; r < TD; r++) {
         const int k = kf + 1 + bb * TD + TD - 1 - r;
@@ -540,7 +553,8 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
     const int k = kf - r;
     const long q = base + (long)(k - 1) * plane;
     vb[r] = k >= 1 ? S[2 * q] : 0.0;
-    db[r] = k >= 1 ? dtri[dbase + (long)(k - 1) * plane] : 0.0;
+    const double *qd = dtri + dbase + (long)(k - 1) * plane;
+    db[r] = k >= 1 ? *(qd < dlast ? qd : dlast) : 0.0;
   }
   for (int k0 = kf; k0 >= 1; k0 -= TD) {
     double v[TD], d[TD];
@@ -552,7 +566,8 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
         const int k = k0 - TD - r;
         const long q = base + (long)(k - 1) * plane;
         vb[r] = k >= 1 ? S[2 * q] : 0.0;
-        db[r] = k >= 1 ? dtri[dbase + (long)(k - 1) * plane] : 0.0;
+        const double *qd = dtri + dbase + (long)(k - 1) * plane;
+        db[r] = k >= 1 ? *(qd < dlast ? qd : dlast) : 0.0;
       }
     }
 #pragma unroll
@@ -676,6 +691,23 @@ int ipg_poisson_setup(ipg_ctx *c) {
     return 0;
   };
   if (up(&P.a, a) || up(&P.b, b) || up(&P.c, cc) || up(&P.xrt, xrt) || up(&P.yrt, yrt)) return 1;
+  // The Thomas solve stores a column's d up to the last row where it changes
+  // (the rows beyond go to that row).  Rows 2 .. nz-1 share a, b, c, so d is
+  // fixed from the first row that repeats it.  Counted for ipg_substep_bytes.
+  P.d_rows = 0.0;
+  for (int f = 0; f <= ny / 2; f++)
+    for (int m = 0; m < P.nmx; m++) {
+      const double xy = xrt[m] + yrt[f];
+      double dp = cc[1] * (1.0 / (b[1] + xy));
+      int last = 1;
+      for (int k = 2; k <= nz - 1; k++) {
+        const double dn = cc[k] * (1.0 / ((b[k] + xy) - a[k] * dp));
+        if (dn == dp) break;
+        last = k;
+        dp = dn;
+      }
+      P.d_rows += last;
+    }
   auto twid = [&](double2 **d, int n) -> int {
     std::vector<double2> h(n);
     for (int t = 0; t < n; t++) {
