@@ -463,6 +463,12 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
   // keeps, in most columns within a few dozen rows; from there on its stores
   // and loads hit one row, which the caches hold.
   double *dlast = dtri + dbase;
+  // Once d repeats in every lane of the wavefront (d(k) == d(k-1), k >= 2),
+  // every later row's z and d equal this row's bit for bit (ipg_poisson_setup
+  // gives rows 2 .. nz-1 one a, b, c): the division, the longest dependent
+  // step of the sweep, is skipped from there on.
+  bool conv = false;
+  double z = 0.0;
   double vn[TD];
 #pragma unroll
   for (int r = 0; r < TD; r++) vn[r] = 1 + r <= kf ? S[2 * (base + (long)r * plane)] : 0.0;
@@ -482,12 +488,13 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
       const int k = k0 + r;
       if (k <= kf) {
         const long q = base + (long)(k - 1) * plane;
-        double z;
-        if (k == 1) z = 1.0 / (b[1] + xy);
-        else z = 1.0 / ((b[k] + xy) - a[k] * dprev);
-        const double dn = cc[k] * z;
-        if (dn != dprev) dlast = dtri + dbase + (long)(k - 1) * plane;
-        dprev = dn;
+        if (!conv) {
+          z = k == 1 ? 1.0 / (b[1] + xy) : 1.0 / ((b[k] + xy) - a[k] * dprev);
+          const double dn = cc[k] * z;
+          if (dn != dprev) dlast = dtri + dbase + (long)(k - 1) * plane;
+          conv = __all(k >= 2 && dn == dprev);
+          dprev = dn;
+        }
         // side 0's two parts store the same value; side 1 reads it back in
         // the back substitution of this wavefront
         if (side == 0) *dlast = dprev;
@@ -508,10 +515,13 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
 #pragma unroll
       for (int r = 0; r < TD; r++) {
         const int k = kf + 1 + bb * TD + r;
-        const double z = 1.0 / ((b[k] + xy) - a[k] * dprev);
-        const double dn = cc[k] * z;
-        if (dn != dprev) dlast = dtri + dbase + (long)(k - 1) * plane;
-        dprev = dn;
+        if (!conv) {
+          z = 1.0 / ((b[k] + xy) - a[k] * dprev);
+          const double dn = cc[k] * z;
+          if (dn != dprev) dlast = dtri + dbase + (long)(k - 1) * plane;
+          conv = __all(dn == dprev);
+          dprev = dn;
+        }
         if (side == 0) *dlast = dprev;
         const double w = (v[r] - a[k] * pp) * z;
         wr[bb * TD + r] = w;
