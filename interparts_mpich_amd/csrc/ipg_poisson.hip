@@ -418,12 +418,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv(Grid g, double *__res
 // recurrence of the pivots and multipliers (both compute and store the same
 // values), which doubles the number of independent lanes over one thread per
 // pair.  Adjacent lanes hold the two parts of one complex value, so
-// every load and store is coalesced.  With one wavefront per SIMD the sweep is
-// bound by memory latency, so both sweeps are software-pipelined: the next TD
-// planes are loaded while the current TD are eliminated.
+// every load and store is coalesced.  With about one wavefront per SIMD the
+// sweep is bound by latency, of memory and of each row's dependent chain, so
+// both sweeps are software-pipelined (the next TD planes are loaded while the
+// current TD are eliminated) and the chain is kept short.
 constexpr int TD = 16;
 // rows per column kept in registers (a multiple of TD): 48 keeps two
-// wavefronts per SIMD (216 VGPRs); 64 would leave one
+// wavefronts per SIMD (221 VGPRs); 64 would leave one
 constexpr int THOMAS_TAIL = 48;
 __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
                                                 const double *__restrict__ a, const double *__restrict__ b,
