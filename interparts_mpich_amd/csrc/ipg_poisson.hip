@@ -682,6 +682,13 @@ int ipg_poisson_setup(ipg_ctx *c) {
   b[nz] = b[nz] + cc[nz];
   a[1] = 0.0;
   cc[nz] = 0.0;
+  // k_thomas reuses a row's z and d once d repeats, exact only while rows
+  // 2 .. nz-1 share a, b, c
+  for (int k = 3; k <= nz - 1; k++)
+    if (a[k] != a[2] || b[k] != b[2] || cc[k] != cc[2]) {
+      ipg_set_error("initsolver: the interior rows of the z tridiagonal system differ");
+      return 1;
+    }
   // src/initsolver.f90:41-56, in VFFTPACK's packed order ...
   std::vector<double> xp(nx + 1, 0.0), yp(ny + 1, 0.0);
   for (int i = 3; i <= nx; i += 2) { double s = std::sin((double)(i - 1) * pi / (2.0 * nx)); xp[i - 1] = -4.0 * dxi * dxi * (s * s); xp[i] = xp[i - 1]; }
