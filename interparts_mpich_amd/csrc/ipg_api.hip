@@ -173,6 +173,41 @@ int fill_xcol(ipg_ctx *c) {
   return 0;
 }
 
+// Captures the substep graph of key r = (rkiter - 1) + 3 parity + 6 carried
+// and instantiates it, leaving the host-side state (the c/dcdt buffer roles
+// and parity, the stale flags) as it was: capturing records the launches only.
+int capture_graph(ipg_ctx *c, int r) {
+  const int rkiter = r % 3 + 1, par = (r / 3) % 2, carried = r / 6;
+  double *fc = c->f[IPG_C], *fd = c->f[IPG_DCDT];
+  const int cpar = c->cpar;
+  const bool pgs = c->p_ghosts_stale, xcs = c->xcol_stale, dcs = c->dcdt_stale;
+  if (c->cpar != par) { std::swap(c->f[IPG_C], c->f[IPG_DCDT]); c->cpar ^= 1; }
+  const bool defer = c->side_on && c->pc.np > 0;
+  hipGraph_t graph;
+  IPG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
+  const int rc = substep_launches(c, rkiter, carried ? (rkiter == 1 ? 3 : rkiter - 1) : 0, defer);
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  c->f[IPG_C] = fc; c->f[IPG_DCDT] = fd; c->cpar = cpar;
+  c->p_ghosts_stale = pgs; c->xcol_stale = xcs; c->dcdt_stale = dcs;
+  if (rc) return rc;
+  if (e != hipSuccess) { ipg_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return 1; }
+  IPG_HIP(hipGraphInstantiate(&c->graph[r], graph, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(graph);
+  c->graph_ok[r] = true;
+  return 0;
+}
+
+// Every graph the RK3 sequence can replay, captured together the first time
+// one is needed, so that no later substep pays for a capture: 6 (rkiter,
+// parity) keys, and 6 more carrying the previous update when it is deferred.
+int capture_all_graphs(ipg_ctx *c) {
+  const bool defer = c->side_on && c->pc.np > 0;
+  if (defer && ensure_side(c)) return 1;
+  for (int r = 0; r < (defer ? IPG_NGRAPH : 6); r++)
+    if (!c->graph_ok[r] && capture_graph(c, r)) return 1;
+  return 0;
+}
+
 bool ghosted_field(int field) {
   return field == IPG_U || field == IPG_V || field == IPG_W || field == IPG_PNEW || field == IPG_C;
 }
@@ -296,7 +331,7 @@ int ipg_field_get(ipg_ctx *c, int field, double *host) {
   IPG_HIP(hipMemcpy2DAsync(host, (g.nx + 2) * sizeof(double), c->f[field] - 1, g.sx * sizeof(double),
                            (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyDeviceToHost, c->stream));
   IPG_HIP(hipStreamSynchronize(c->stream));
-  return 0;
+  return ipg_ibm_check(c);
 }
 
 int ipg_particles_put(ipg_ctx *c, int quantity, const double *host) {
@@ -314,7 +349,7 @@ int ipg_particles_get(ipg_ctx *c, int quantity, double *host) {
   if (c->pc.np == 0) return 0;
   IPG_HIP(hipMemcpyAsync(host, c->ps.q[quantity], c->pc.np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   IPG_HIP(hipStreamSynchronize(c->stream));
-  return 0;
+  return ipg_ibm_check(c);
 }
 
 int ipg_set_dt(ipg_ctx *c, double dt) {
@@ -539,21 +574,11 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
   const bool defer = c->side_on && c->pc.np > 0;
   if (c->use_graphs) {
     const int r = (rkiter - 1) + 3 * c->cpar + 6 * (pend ? 1 : 0);
-    if (!c->graph_ok[r]) {
-      // capturing records the launches and makes the host-side buffer swap
-      hipGraph_t graph;
-      IPG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
-      int rc = substep_launches(c, rkiter, pend, defer);
-      hipError_t e = hipStreamEndCapture(c->stream, &graph);
-      if (rc) return rc;
-      if (e != hipSuccess) { ipg_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return 1; }
-      IPG_HIP(hipGraphInstantiate(&c->graph[r], graph, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(graph);
-      c->graph_ok[r] = true;
-    } else {
-      std::swap(c->f[IPG_C], c->f[IPG_DCDT]);
-      c->cpar ^= 1;
-    }
+    if (!c->graph_ok[r] && capture_all_graphs(c)) return 1;
+    if (!c->graph_ok[r] && capture_graph(c, r)) return 1;
+    // the replay swaps the roles of c and dcdt as the captured launches did
+    std::swap(c->f[IPG_C], c->f[IPG_DCDT]);
+    c->cpar ^= 1;
     IPG_HIP(hipGraphLaunch(c->graph[r], c->stream));
     c->p_ghosts_stale = true;
     c->xcol_stale = true;
@@ -565,14 +590,18 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
   return 0;
 }
 
+// The entries whose results a driver writes out (chkdt, chkdiv, post3d, and
+// field_get / particles_get above) synchronise anyway; each also checks the
+// device error word, so no output is produced from a substep that dropped
+// IBM or collision contributions.
 int ipg_chkdt(ipg_ctx *c, double *dtmax) {
   if (!check_ctx(c) || !dtmax || fill_xcol(c)) return 1;
-  return ipg_launch_chkdt(c, dtmax);
+  return ipg_launch_chkdt(c, dtmax) || ipg_ibm_check(c) ? 3 : 0;
 }
 
 int ipg_chkdiv(ipg_ctx *c, double *tot, double *mx) {
   if (!check_ctx(c) || !tot || !mx || fill_xcol(c)) return 1;
-  return ipg_launch_chkdiv(c, tot, mx);
+  return ipg_launch_chkdiv(c, tot, mx) || ipg_ibm_check(c) ? 3 : 0;
 }
 
 int ipg_post3d(ipg_ctx *c, int quantity, double *host) {
@@ -580,7 +609,7 @@ int ipg_post3d(ipg_ctx *c, int quantity, double *host) {
   if (quantity < 0 || quantity > 5) { ipg_set_error("post3d: quantity must be 0..5"); return 1; }
   if (!c->solver_ready) { ipg_set_error("post3d: ipg_initsolver has not been called"); return 1; }
   if (fill_xcol(c)) return 1;
-  return ipg_launch_post3d(c, quantity, host);
+  return ipg_launch_post3d(c, quantity, host) || ipg_ibm_check(c) ? 3 : 0;
 }
 
 int ipg_sync(ipg_ctx *c) {
