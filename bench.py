@@ -62,6 +62,29 @@ def thomas_words(case, Sw):
     return Sw * (4.0 * (K - T) + 2.0 * T + 2.0) / case.ktot + 2.0 * thomas_d_rows(case)
 
 
+def kernel_bytes_mixed(case, name, rkiter, nref):
+    """Algorithmic HBM bytes of one launch of a kernel on the mixed path: FP64
+    fields (u v w c pnew dudt..dcdt, the right-hand side) at 8 B, the RK history,
+    the spectra and the pressure pieces at 4 B, the Thomas multipliers d at 8 B."""
+    N = float(case.itot * case.jtot * case.ktot)
+    Sw = 2.0 * (case.itot // 2 + 1) * case.jtot * case.ktot
+    d_rows = thomas_d_rows(case)
+    b = {
+        # u v w c pnew read, dudt..dcdt written (FP64); history 4 written / 4 read + 4 written / 4 read (FP32)
+        "rk_stage": 72.0 * N + {1: 16.0, 2: 32.0, 3: 16.0}[rkiter] * N,
+        "fft_x_fwd": 32.0 * N + 4.0 * Sw,      # dudt dvdt dwdt read, the right-hand side (FP64) and the spectrum stored
+        # refinement r: the right-hand side and r pieces read, the FP32 residual written, then transformed
+        "resid": (12.0 + 4.0 * max(1, nref)) * N if nref else None,
+        "resid_x_fwd": 4.0 * N + 4.0 * Sw,
+        "fft_y_fwd": 8.0 * Sw,
+        "thomas": 4.0 * (thomas_words(case, Sw) - 2.0 * d_rows) + 16.0 * d_rows,
+        "fft_y_inv": 8.0 * Sw,
+        "fft_x_inv": 4.0 * Sw + 4.0 * N,       # the spectrum read, one FP32 piece of p written
+        "correc": (64.0 + 4.0 * (nref + 1)) * N,  # dudt dvdt dwdt pnew and the pieces read; u v w pnew written
+    }
+    return b.get(name)
+
+
 def kernel_bytes(case, name, rkiter):
     """Algorithmic HBM bytes of one launch of a kernel (FP64 words x 8).
     N cells; Sw = FP64 words of the half spectrum, 2 (nx/2+1) ny nz."""
@@ -179,12 +202,106 @@ def run_ensemble(args):
     print(json.dumps(out))
 
 
+def run_profile(args):
+    """The workload under a profiler: no timing of its own, no graphs unless
+    asked for, a few substeps, a line per phase so the run is never silent."""
+    from interparts_mpich_amd.inputs import CONFIGS, initial_fields, place_spheres, markers_for
+    from interparts_mpich_amd.driver import Simulation
+    case = CONFIGS[args.config]
+    precision = 1 if args.precision == "mixed" else 0
+    print(f"[profile] {case.name} {'mixed' if precision else 'fp64'}: inputs", flush=True)
+    inputs = (initial_fields(case), place_spheres(case), markers_for(case))
+    sim = Simulation(case, use_graphs=not args.no_graphs, side_stream=args.side_stream == "on",
+                     precision=precision).initialize(*inputs)
+    print("[profile] initialised", flush=True)
+    for s in range(args.warmup + args.steps):
+        sim.substep(s % 3 + 1)
+    sim.sync()
+    print(f"[profile] {args.warmup + args.steps} substeps done", flush=True)
+    sim.close()
+
+
+def measure(case, inputs, args, precision):
+    """Substeps/s of one path over K timed substeps (after W warmup substeps),
+    then per-kernel durations with HIP events over K eager substeps."""
+    from interparts_mpich_amd.driver import Simulation
+    from interparts_mpich_amd import _lib
+    import numpy as np
+    fields, spheres, markers = inputs
+    name = "mixed" if precision else "fp64"
+    sim = Simulation(case, use_graphs=not args.no_graphs, side_stream=args.side_stream == "on",
+                     precision=precision).initialize(fields, spheres, markers)
+    rk_kb = None if args.no_autotune else sim.autotune()
+    for s in range(args.warmup):
+        sim.substep(s % 3 + 1)
+    sim.sync()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        sim.substep(s % 3 + 1)
+    sim.sync()
+    t1 = time.perf_counter()
+    res = {"value": args.steps / (t1 - t0), "ms_per_step": (t1 - t0) * 1e3 / args.steps,
+           "rk_kb": rk_kb if rk_kb is not None else "default", "nref": None}
+    print(f"[bench] {name}: {args.steps} substeps in {t1 - t0:.4f} s ({res['value']:.1f} substeps/s)", file=sys.stderr,
+          flush=True)
+    if precision:
+        res["nref"] = int(os.environ.get("IPG_MIXED_REFINE", "1"))
+        _, mdiv = sim.chkdiv()
+        res["accuracy"] = {"max_div_after_timed_substeps": mdiv}
+
+    # per-kernel durations over the same number of substeps, run as the timed
+    # ones were (graph replays beside the carried particle update)
+    lib = _lib.load()
+    maxn = 64
+    ms = (C.c_double * maxn)()
+    names = (C.c_char_p * maxn)()
+    n = C.c_int()
+    acc = {}
+    for s in range(args.steps):
+        rk = s % 3 + 1
+        _lib.check(lib.ipg_substep_phases(sim.ctx, rk, maxn, ms, names, C.byref(n)), "substep_phases")
+        for q in range(n.value):
+            key = names[q].decode()
+            a = acc.setdefault(key, {"ms": 0.0, "launches": 0, "bytes": 0.0})
+            a["ms"] += ms[q]
+            a["launches"] += 1
+            b = kernel_bytes_mixed(case, key, rk, res["nref"]) if precision else kernel_bytes(case, key, rk)
+            if b is not None:
+                a["bytes"] += b
+    sim.close()
+    per_substep = {k: v["ms"] / args.steps for k, v in acc.items()}
+    kernel_gbs = {k: round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) for k, v in acc.items() if v["bytes"] > 0}
+    dominant = max((k for k in acc if acc[k]["bytes"] > 0), key=lambda k: acc[k]["ms"])
+    d = acc[dominant]
+    avg_ms = d["ms"] / d["launches"]
+    bytes_per_launch = d["bytes"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    print(f"[bench] {name} per-substep kernel time (ms): " +
+          ", ".join(f"{k}={v:.4f}" for k, v in sorted(per_substep.items(), key=lambda kv: -kv[1])),
+          file=sys.stderr, flush=True)
+    res["roofline"] = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                       "timed_as": ("graph replays of the timed substeps, wall-clock marks on the kernel's stream"
+                                    if not args.no_graphs else "eager launches, HIP events on the kernel's stream"),
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                       "traffic": traffic_from_profile(dominant + ("_mixed" if precision else ""), case.name),
+                       "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch}
+    res["kernel_ms_per_substep"] = {k: round(v, 5) for k, v in per_substep.items()}
+    res["kernel_algorithmic_gbs"] = kernel_gbs
+    res["event_sum_ms_per_substep"] = round(sum(v["ms"] for v in acc.values()) / args.steps, 4)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300, help="timed substeps")
     ap.add_argument("--warmup", type=int, default=30, help="untimed substeps")
     ap.add_argument("--config", default="ch256_1000sph")
+    ap.add_argument("--precision", choices=("fp64", "mixed", "both"), default="both",
+                    help="path(s) to time: both prints the mixed path's line with the FP64 path's figures beside it")
+    ap.add_argument("--profile", action="store_true",
+                    help="for a profiler: initialise the case, then run --warmup and --steps substeps of the path "
+                         "--precision names (fp64 or mixed) and exit; prints one line per phase")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default RK-stage launch shape")
     ap.add_argument("--cpu-substeps", type=int, default=12, help="oracle substeps in the CPU-baseline sample")
@@ -214,89 +331,64 @@ def main():
     if args.ensemble:
         run_ensemble(args)
         return
+    if args.profile:
+        run_profile(args)
+        return
 
     from interparts_mpich_amd.inputs import CONFIGS, initial_fields, place_spheres, markers_for
-    from interparts_mpich_amd.driver import Simulation
-    from interparts_mpich_amd import _lib
 
     case = CONFIGS[args.config]
     fields, spheres, markers = initial_fields(case), place_spheres(case), markers_for(case)
     print(f"[bench] {case.name}: {case.itot}x{case.jtot}x{case.ktot}, np={case.np}, nl={case.nl}", file=sys.stderr, flush=True)
-    sim = Simulation(case, use_graphs=not args.no_graphs, side_stream=args.side_stream == "on").initialize(fields, spheres,
-                                                                                                        markers)
-    rk_kb = None if args.no_autotune else sim.autotune()
-    for s in range(args.warmup):
-        sim.substep(s % 3 + 1)
-    sim.sync()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        sim.substep(s % 3 + 1)
-    sim.sync()
-    t1 = time.perf_counter()
-    ms_per = (t1 - t0) * 1e3 / args.steps
-    value = args.steps / (t1 - t0)
-    print(f"[bench] {args.steps} substeps in {t1 - t0:.3f} s", file=sys.stderr, flush=True)
-
-    # per-kernel durations with HIP events over the same number of substeps
-    lib = _lib.load()
-    maxn = 64
-    ms = (C.c_double * maxn)()
-    names = (C.c_char_p * maxn)()
-    n = C.c_int()
-    acc = {}
-    for s in range(args.steps):
-        rk = s % 3 + 1
-        _lib.check(lib.ipg_substep_phases(sim.ctx, rk, maxn, ms, names, C.byref(n)), "substep_phases")
-        for q in range(n.value):
-            key = names[q].decode()
-            a = acc.setdefault(key, {"ms": 0.0, "launches": 0, "bytes": 0.0})
-            a["ms"] += ms[q]
-            a["launches"] += 1
-            b = kernel_bytes(case, key, rk)
-            if b is not None:
-                a["bytes"] += b
-    per_substep = {k: v["ms"] / args.steps for k, v in acc.items()}
-    kernel_gbs = {k: round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) for k, v in acc.items() if v["bytes"] > 0}
-    dominant = max((k for k in acc if acc[k]["bytes"] > 0), key=lambda k: acc[k]["ms"])
-    d = acc[dominant]
-    avg_ms = d["ms"] / d["launches"]
-    bytes_per_launch = d["bytes"] / d["launches"]
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    total_ms_events = sum(v["ms"] for v in acc.values()) / args.steps
-    print("[bench] per-substep kernel time (ms): " +
-          ", ".join(f"{k}={v:.4f}" for k, v in sorted(per_substep.items(), key=lambda kv: -kv[1])),
-          file=sys.stderr, flush=True)
-
+    runs = {}
+    for prec in (("fp64", "mixed") if args.precision == "both" else (args.precision,)):
+        runs[prec] = measure(case, (fields, spheres, markers), args, 1 if prec == "mixed" else 0)
+    # the line leads with the faster path; the other is reported beside it
+    head = max(runs.values(), key=lambda r: r["value"])
+    mixed = head is runs.get("mixed")
+    value = head["value"]
     out = {
-        "metric": "RK3 substeps/s (256^3, 1k spheres) FP64",
+        "metric": "RK3 substeps/s (256^3, 1k spheres) " + ("mixed precision" if mixed else "FP64"),
         "value": round(value, 3),
         "unit": "substeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per, 4),
+        "ms_per_step": round(head["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": ("mixed: u v w p dudt and particles f64, stencil and Poisson arithmetic f64, RK history and "
+                  "Poisson spectra stored f32, IBM f32" if mixed else "f64"),
         "data": "synthetic (seeded Poiseuille + 0.1 U_bulk noise, seeded sphere placement)",
         "config": {"workload": f"{case.name}: {case.itot}x{case.jtot}x{case.ktot} channel, {case.np} spheres "
-                               f"D={case.diameter_h:g}h nl={case.nl}, Re_bulk={case.re_bulk:g}, FP64 path",
+                               f"D={case.diameter_h:g}h nl={case.nl}, Re_bulk={case.re_bulk:g}, "
+                               + ("mixed-precision path" if mixed else "FP64 path"),
                    "graphs": not args.no_graphs, "side_stream": args.side_stream == "on",
-                   "rk_z_planes_per_workgroup": rk_kb if rk_kb is not None else "default",
+                   "rk_z_planes_per_workgroup": head["rk_kb"],
                    "gcell_updates_per_s": round(value * case.cells / 1e9, 4)},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile(dominant, case.name),
-                     "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch},
-        "kernel_ms_per_substep": {k: round(v, 5) for k, v in per_substep.items()},
-        "kernel_algorithmic_gbs": kernel_gbs,
-        "event_sum_ms_per_substep": round(total_ms_events, 4),
+        "roofline": head["roofline"],
+        "kernel_ms_per_substep": head["kernel_ms_per_substep"],
+        "kernel_algorithmic_gbs": head["kernel_algorithmic_gbs"],
+        "event_sum_ms_per_substep": head["event_sum_ms_per_substep"],
     }
-    sim.close()
+    if mixed:
+        out["config"]["poisson_refinement_sweeps"] = head["nref"]
+    other = [r for r in runs.values() if r is not head]
+    if other:
+        o = other[0]
+        key = "fp64" if mixed else "mixed"
+        out[key] = {k: o[k] for k in ("value", "ms_per_step", "nref", "roofline", "kernel_ms_per_substep",
+                                      "kernel_algorithmic_gbs", "event_sum_ms_per_substep") if k in o}
+        if "accuracy" in o:
+            out[key]["accuracy"] = o["accuracy"]
+        out["mixed_over_fp64"] = round(runs["mixed"]["value"] / runs["fp64"]["value"], 4)
+    if "accuracy" in head:
+        out["mixed_accuracy"] = head["accuracy"]
     if not args.no_copy_probe:
         copy_gbs = copy_bandwidth()
-        out["roofline"]["copy_gbs_measured"] = round(copy_gbs, 1) if copy_gbs else None
+        for r in runs.values():
+            r["roofline"]["copy_gbs_measured"] = round(copy_gbs, 1) if copy_gbs else None
 
     if not args.no_cpu_baseline:
         from oracle.pyoracle import Oracle

@@ -36,7 +36,15 @@ typedef struct ipg_params {
   int bc_uvw_top, bc_uvw_bot;    /* 0 'BC_NOSLIP', 1 'BC_FREESLIP' */
   int bc_c_top, bc_c_bot;        /* 0 'BC_DIRICHLET', 1 'BC_NEUMANN' */
   double bc_c_top_val, bc_c_bot_val;
-  int precision;                 /* 0 FP64 path, 1 mixed precision */
+  /* 0: FP64 path, every value and step FP64 in the reference's order.
+   * 1: mixed precision: u, v, w, pnew, c, dudt..dcdt, the particles, the
+   * sphere integrals and the IBM FP64; the RK history, the Poisson spectra
+   * and the pressure correction's pieces stored FP32 (all stencil and
+   * Poisson arithmetic FP64, with FP64-residual refinement of the solve).
+   * The environment variable IPG_MIXED_IBM=32 at init makes the IBM compute
+   * in FP32.  itot and jtot in {16, 24, 32, 48, 64, 96, 128, 192, 256, 384,
+   * 512}. */
+  int precision;
 } ipg_params;
 
 /* field selectors for ipg_field_put / ipg_field_get */
@@ -148,6 +156,9 @@ int ipg_use_graphs(ipg_ctx *ctx, int on);             /* capture each substep as
  * second stream (default on); off: the update closes its substep.  Turn it off
  * when many contexts share the GPU: their streams already overlap. */
 int ipg_use_side_stream(ipg_ctx *ctx, int on);
+/* mixed precision: FP64-residual refinement sweeps of the Poisson solve,
+ * 0..2 (default 1, or IPG_MIXED_REFINE at init) */
+int ipg_set_refinement(ipg_ctx *ctx, int nref);
 /* time the RK stage for 8..64 z planes per workgroup and keep the fastest;
  * overwrites dudt..dcdt (recomputed by every stage): call between substeps */
 int ipg_autotune(ipg_ctx *ctx, int *kb);

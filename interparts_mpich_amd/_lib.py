@@ -26,7 +26,7 @@ EXPORTS = ["ipg_init", "ipg_finalize", "ipg_last_error", "ipg_nl_formula", "ipg_
            "ipg_lagrforces", "ipg_lagr2eulr", "ipg_uniform", "ipg_bounduvw", "ipg_boundp",
            "ipg_boundscal", "ipg_fillps", "ipg_solver2d", "ipg_correc", "ipg_pnew_update",
            "ipg_intgr_over_sphere", "ipg_intgr_nwtn_eulr", "ipg_substep", "ipg_chkdt", "ipg_chkdiv", "ipg_post3d",
-           "ipg_sync", "ipg_use_graphs", "ipg_use_side_stream", "ipg_autotune", "ipg_substep_bytes",
+           "ipg_sync", "ipg_use_graphs", "ipg_use_side_stream", "ipg_set_refinement", "ipg_autotune", "ipg_substep_bytes",
            "ipg_substep_phases"]
 
 
@@ -93,6 +93,7 @@ def load():
         "ipg_sync": ([vp], ip),
         "ipg_use_graphs": ([vp, ip], ip),
         "ipg_use_side_stream": ([vp, ip], ip),
+        "ipg_set_refinement": ([vp, ip], ip),
         "ipg_autotune": ([vp, C.POINTER(C.c_int)], ip),
         "ipg_substep_bytes": ([vp, dp], ip),
         "ipg_substep_phases": ([vp, ip, ip, dp, C.POINTER(C.c_char_p), C.POINTER(ip)], ip),

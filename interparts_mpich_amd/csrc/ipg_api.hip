@@ -102,11 +102,12 @@ int release(ipg_ctx *c) {
   if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
   if (c->join_ev) (void)hipEventDestroy(c->join_ev);
   if (c->side) (void)hipStreamDestroy(c->side);
-  for (int q = 0; q < 64; q++)
+  for (int q = 0; q < IPG_NMARK; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
+  if (c->stamps) (void)hipFree(c->stamps);
   ipg_poisson_free(c);
   void *ptrs[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->nml, c->nmc, c->ibm_err, c->ibm_flg, c->op, c->motion2, c->red, c->dscal,
-                  c->ps.nbf, c->K1[0], c->K1[1], c->K1[2]};
+                  c->ps.nbf, c->K1[0], c->K1[1], c->K1[2], c->pbuf};
   for (void *q : ptrs) if (q) (void)hipFree(q);
   for (int q = 0; q < IPG_NPART; q++) if (c->ps.q[q]) (void)hipFree(c->ps.q[q]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -130,13 +131,17 @@ int substep_launches(ipg_ctx *c, int rkiter, int pend, bool defer) {
     IPG_HIP(hipEventRecord(c->fork_ev, c->stream));
     IPG_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
     std::swap(c->stream, c->side);
+    ipg_mark(c, nullptr);
     const int rc = ipg_launch_nwtn_eulr(c, pend) || (np && ipg_launch_ibm_begin(c));
     std::swap(c->stream, c->side);
     if (rc) return 1;
     IPG_HIP(hipEventRecord(c->join_ev, c->side));
   }
   if (ipg_launch_rk(c, rkiter, true)) return 1;
-  if (pend) IPG_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+  if (pend) {
+    IPG_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+    ipg_mark(c, nullptr);
+  }
   if (np && (pend ? ipg_launch_ibm_passes(c) : ipg_launch_ibm(c))) return 1;
   if (ipg_launch_poisson_ex(c, rkiter)) return 1;
   if (ipg_launch_correc_fused(c, rkiter)) return 1;
@@ -180,7 +185,7 @@ int capture_graph(ipg_ctx *c, int r) {
   const int rkiter = r % 3 + 1, par = (r / 3) % 2, carried = r / 6;
   double *fc = c->f[IPG_C], *fd = c->f[IPG_DCDT];
   const int cpar = c->cpar;
-  const bool pgs = c->p_ghosts_stale, xcs = c->xcol_stale, dcs = c->dcdt_stale;
+  const bool pgs = c->p_ghosts_stale, xcs = c->xcol_stale, dcs = c->dcdt_stale, pip = c->p_in_pieces;
   if (c->cpar != par) { std::swap(c->f[IPG_C], c->f[IPG_DCDT]); c->cpar ^= 1; }
   const bool defer = c->side_on && c->pc.np > 0;
   hipGraph_t graph;
@@ -188,7 +193,7 @@ int capture_graph(ipg_ctx *c, int r) {
   const int rc = substep_launches(c, rkiter, carried ? (rkiter == 1 ? 3 : rkiter - 1) : 0, defer);
   const hipError_t e = hipStreamEndCapture(c->stream, &graph);
   c->f[IPG_C] = fc; c->f[IPG_DCDT] = fd; c->cpar = cpar;
-  c->p_ghosts_stale = pgs; c->xcol_stale = xcs; c->dcdt_stale = dcs;
+  c->p_ghosts_stale = pgs; c->xcol_stale = xcs; c->dcdt_stale = dcs; c->p_in_pieces = pip;
   if (rc) return rc;
   if (e != hipSuccess) { ipg_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return 1; }
   IPG_HIP(hipGraphInstantiate(&c->graph[r], graph, nullptr, nullptr, 0));
@@ -205,6 +210,44 @@ int capture_all_graphs(ipg_ctx *c) {
   if (defer && ensure_side(c)) return 1;
   for (int r = 0; r < (defer ? IPG_NGRAPH : 6); r++)
     if (!c->graph_ok[r] && capture_graph(c, r)) return 1;
+  return 0;
+}
+
+// After a mixed-path substep p is the sum of its FP32 pieces and the P field
+// holds the Poisson right-hand side: the sum is formed before p is read.
+int fill_p(ipg_ctx *c) {
+  if (!c->p_in_pieces) return 0;
+  if (ipg_launch_p_combine(c)) return 1;
+  c->p_in_pieces = false;
+  c->p_ghosts_stale = true;
+  return 0;
+}
+
+// Host <-> HBM copy of one field in the reference's (0:i1,0:j1,0:k1) layout;
+// an FP32 field (the mixed path's RK history) is converted on the host.
+int field_copy(ipg_ctx *c, int field, const double *in, double *out) {
+  const Grid &g = c->g;
+  const size_t w = (size_t)g.nx + 2, rows = (size_t)(g.ny + 2) * (g.nz + 2);
+  if (!c->f32[field]) {
+    if (in) IPG_HIP(hipMemcpy2DAsync(c->f[field] - 1, g.sx * sizeof(double), in, w * sizeof(double), w * sizeof(double), rows,
+                                     hipMemcpyHostToDevice, c->stream));
+    else IPG_HIP(hipMemcpy2DAsync(out, w * sizeof(double), c->f[field] - 1, g.sx * sizeof(double), w * sizeof(double), rows,
+                                  hipMemcpyDeviceToHost, c->stream));
+    IPG_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+  }
+  std::vector<float> h(w * rows);
+  if (in) {
+    for (size_t q = 0; q < h.size(); q++) h[q] = (float)in[q];
+    IPG_HIP(hipMemcpy2DAsync(f32ptr(c, field) - 1, g.sx * sizeof(float), h.data(), w * sizeof(float), w * sizeof(float), rows,
+                             hipMemcpyHostToDevice, c->stream));
+    IPG_HIP(hipStreamSynchronize(c->stream));
+  } else {
+    IPG_HIP(hipMemcpy2DAsync(h.data(), w * sizeof(float), f32ptr(c, field) - 1, g.sx * sizeof(float), w * sizeof(float), rows,
+                             hipMemcpyDeviceToHost, c->stream));
+    IPG_HIP(hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < h.size(); q++) out[q] = (double)h[q];
+  }
   return 0;
 }
 
@@ -231,7 +274,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
     ipg_set_error("ipg_init: nl does not match the marker count of src/init.f90:129 (nl /= nlmax)");
     return 2;
   }
-  if (prm->precision != 0) { ipg_set_error("ipg_init: only precision 0 (FP64) is built"); return 1; }
+  if (prm->precision != 0 && prm->precision != 1) { ipg_set_error("ipg_init: precision must be 0 (FP64) or 1 (mixed)"); return 1; }
   if (prm->np > 0) {
     // the sphere-integral box (src/intgr_over_sphere.f90:117-130) is at most
     // 2 R / h + 6 cells wide; its cells are indexed with ten bits per axis
@@ -246,12 +289,28 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
   auto fail = [&](int code) { release(c); delete c; return code; };
   c->side_on = true;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { ipg_set_error("stream create failed"); return fail(1); }
+  c->main_stream = c->stream;
   const Grid &g = c->g;
   c->nfield = 16 + (size_t)g.sxy * (g.nz + 2) + 16;
   c->nfield = (c->nfield + 15) / 16 * 16;
   if (hipMalloc(&c->slab, c->nfield * IPG_NFIELDS * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory (fields)"); return fail(1); }
   if (hipMemset(c->slab, 0, c->nfield * IPG_NFIELDS * sizeof(double)) != hipSuccess) { ipg_set_error("memset failed"); return fail(1); }
   for (int q = 0; q < IPG_NFIELDS; q++) c->f[q] = c->slab + (size_t)q * c->nfield + 16;
+  if (prm->precision == 1) {
+    c->mixed = true;
+    // the IBM stays FP64 unless IPG_MIXED_IBM=32: its FP32 form spreads with
+    // FP32 LDS atomic adds, which stall where the FP64 ones do not (2.5x
+    // slower passes at 256^3)
+    if (const char *e = std::getenv("IPG_MIXED_IBM")) c->ibm32 = std::atoi(e) == 32;
+    c->nref = 1;
+    if (const char *e = std::getenv("IPG_MIXED_REFINE")) c->nref = std::max(0, std::min(IPG_MAX_REFINE, std::atoi(e)));
+    // the RK history is stored FP32, each field in its own slot
+    for (int q : {IPG_DURK, IPG_DVRK, IPG_DWRK, IPG_DCRK}) c->f32[q] = true;
+    const size_t n32 = (size_t)(IPG_MAX_REFINE + 2) * c->nfield;
+    if (hipMalloc(&c->pbuf, n32 * sizeof(float)) != hipSuccess) { ipg_set_error("out of device memory (pressure pieces)"); return fail(1); }
+    if (hipMemset(c->pbuf, 0, n32 * sizeof(float)) != hipSuccess) { ipg_set_error("memset failed"); return fail(1); }
+    for (int r = 0; r <= IPG_MAX_REFINE + 1; r++) c->pp[r] = c->pbuf + (size_t)r * c->nfield + 16;
+  }
   const size_t ncell = (size_t)g.nx * g.ny * g.nz, nplane = (size_t)g.nx * g.ny;
   const int np = prm->np;
   if (np > 0) {
@@ -306,20 +365,17 @@ int ipg_finalize(ipg_ctx *c) {
 int ipg_field_put(ipg_ctx *c, int field, const double *host) {
   if (!ready(c)) return 1;
   if (field < 0 || field >= IPG_NFIELDS || !host) { ipg_set_error("field_put: bad field"); return 1; }
-  if (field == IPG_P) c->p_ghosts_stale = false;
+  if (field == IPG_P) { c->p_ghosts_stale = false; c->p_in_pieces = false; }
   if (ghosted_field(field) && fill_xcol(c)) return 1;
   if (field == IPG_C && fill_dcdt(c)) return 1;
   if (field == IPG_DCDT) c->dcdt_stale = false;
-  const Grid &g = c->g;
-  IPG_HIP(hipMemcpy2DAsync(c->f[field] - 1, g.sx * sizeof(double), host, (g.nx + 2) * sizeof(double),
-                           (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyHostToDevice, c->stream));
-  IPG_HIP(hipStreamSynchronize(c->stream));
-  return 0;
+  return field_copy(c, field, host, nullptr);
 }
 
 int ipg_field_get(ipg_ctx *c, int field, double *host) {
   if (!check_ctx(c)) return 1;
   if (field < 0 || field >= IPG_NFIELDS || !host) { ipg_set_error("field_get: bad field"); return 1; }
+  if (field == IPG_P && fill_p(c)) return 1;
   if (field == IPG_P && c->p_ghosts_stale) {
     // the fused substep leaves p's ghosts to the reader: boundp(p) now
     if (ipg_launch_bound(c, BND_P)) return 1;
@@ -327,10 +383,7 @@ int ipg_field_get(ipg_ctx *c, int field, double *host) {
   }
   if (ghosted_field(field) && fill_xcol(c)) return 1;
   if (field == IPG_DCDT && fill_dcdt(c)) return 1;
-  const Grid &g = c->g;
-  IPG_HIP(hipMemcpy2DAsync(host, (g.nx + 2) * sizeof(double), c->f[field] - 1, g.sx * sizeof(double),
-                           (g.nx + 2) * sizeof(double), (size_t)(g.ny + 2) * (g.nz + 2), hipMemcpyDeviceToHost, c->stream));
-  IPG_HIP(hipStreamSynchronize(c->stream));
+  if (field_copy(c, field, nullptr, host)) return 1;
   return ipg_ibm_check(c);
 }
 
@@ -488,7 +541,7 @@ int ipg_bounduvw(ipg_ctx *c, int which) {
 }
 
 int ipg_boundp(ipg_ctx *c, int which) {
-  if (!ready(c) || fill_xcol(c)) return 1;
+  if (!ready(c) || fill_xcol(c) || (which != 0 && fill_p(c))) return 1;
   if (ipg_launch_bound(c, which == 0 ? BND_PNEW : BND_P)) return 1;
   if (which != 0) c->p_ghosts_stale = false;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -505,21 +558,24 @@ int ipg_boundscal(ipg_ctx *c) {
 int ipg_fillps(ipg_ctx *c, int rkiter) {
   if (!ready(c)) return 1;
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("fillps: rkiter must be 1..3"); return 1; }
+  c->p_in_pieces = false;
   if (ipg_launch_fillps(c, rkiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
 
 int ipg_solver2d(ipg_ctx *c) {
-  if (!ready(c)) return 1;
+  if (!ready(c) || fill_p(c)) return 1;
   if (!c->solver_ready) { ipg_set_error("solver2d: ipg_initsolver has not been called"); return 1; }
   if (ipg_launch_poisson(c)) return 1;
+  // the mixed path solves into FP32 pieces: p = their sum, as the field
+  if (c->mixed && fill_p(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
 
 int ipg_correc(ipg_ctx *c, int rkiter) {
-  if (!ready(c)) return 1;
+  if (!ready(c) || fill_p(c)) return 1;
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("correc: rkiter must be 1..3"); return 1; }
   if (ipg_launch_correc(c, rkiter)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
@@ -527,7 +583,7 @@ int ipg_correc(ipg_ctx *c, int rkiter) {
 }
 
 int ipg_pnew_update(ipg_ctx *c) {
-  if (!ready(c) || fill_dcdt(c)) return 1;
+  if (!ready(c) || fill_dcdt(c) || fill_p(c)) return 1;
   if (ipg_launch_pnew(c)) return 1;
   if (ipg_launch_bound(c, BND_UVW_NEW | BND_PNEW | BND_C)) return 1;
   c->xcol_stale = false;
@@ -583,6 +639,7 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
     c->p_ghosts_stale = true;
     c->xcol_stale = true;
     c->dcdt_stale = true;
+    c->p_in_pieces = c->mixed;
   } else if (substep_launches(c, rkiter, pend, defer)) {
     return 1;
   }
@@ -618,26 +675,88 @@ int ipg_sync(ipg_ctx *c) {
   return ipg_ibm_check(c);
 }
 
+// One substep run as ipg_substep runs it (the previous substep's pending
+// update carried beside the RK stage; replayed from a captured graph when
+// graphs are on) with a timing marker after every launch: per-launch
+// durations (ms) and kernel names.  A launch is timed from the marker before
+// it on its own stream.  In a replay the markers are wall-clock stamps,
+// captured into a timing copy of the substep's graph.
 int ipg_substep_phases(ipg_ctx *c, int rkiter, int maxn, double *ms, const char **names, int *n) {
   if (!check_ctx(c) || !ms || !names || !n) return 1;
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("substep_phases: rkiter must be 1..3"); return 1; }
-  for (int q = 0; q < 64; q++)
+  if (!c->solver_ready) { ipg_set_error("substep_phases: ipg_initsolver has not been called"); return 1; }
+  if (c->pc.np > 0 && !c->markers_ready) { ipg_set_error("substep_phases: ipg_coordsfp has not been called"); return 1; }
+  for (int q = 0; q < IPG_NMARK; q++)
     if (!c->ev[q]) IPG_HIP(hipEventCreate(&c->ev[q]));
-  if (settle(c)) return 1;
+  if (!c->stamps) IPG_HIP(hipMalloc(&c->stamps, IPG_NMARK * sizeof(unsigned long long)));
+  c->dcdt_stale = false;
+  int pend = c->ne_pending;
+  if (pend && pend != (rkiter == 1 ? 3 : rkiter - 1)) {
+    if (settle(c)) return 1;
+    pend = 0;
+  }
+  c->ne_pending = 0;
+  if (pend && ensure_side(c)) return 1;
+  const bool defer = c->side_on && c->pc.np > 0;
   IPG_HIP(hipStreamSynchronize(c->stream));
   c->timing = true;
+  c->stamping = c->use_graphs;
   c->nev = 0;
-  IPG_HIP(hipEventRecord(c->ev[0], c->stream));
-  int rc = substep_launches(c, rkiter, 0, false);
-  c->timing = false;
-  if (rc) return rc;
-  IPG_HIP(hipEventSynchronize(c->ev[c->nev]));
-  int k = c->nev < maxn ? c->nev : maxn;
-  for (int q = 0; q < k; q++) {
+  c->evtag[0] = 0;
+  int rc = 0;
+  if (c->use_graphs) {
+    // a timing copy of this substep's graph, replayed once
+    double *fc = c->f[IPG_C], *fd = c->f[IPG_DCDT];
+    const int cpar = c->cpar;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    IPG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
+    ipg_stamp(c, 0);
+    rc = substep_launches(c, rkiter, pend, defer);
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    c->f[IPG_C] = fc; c->f[IPG_DCDT] = fd; c->cpar = cpar;
+    c->timing = c->stamping = false;
+    if (rc) return rc;
+    if (e != hipSuccess) { ipg_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return 1; }
+    IPG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    std::swap(c->f[IPG_C], c->f[IPG_DCDT]);
+    c->cpar ^= 1;
+    IPG_HIP(hipGraphLaunch(exec, c->stream));
+    IPG_HIP(hipStreamSynchronize(c->stream));
+    (void)hipGraphExecDestroy(exec);
+    c->p_ghosts_stale = c->xcol_stale = c->dcdt_stale = true;
+    c->p_in_pieces = c->mixed;
+  } else {
+    IPG_HIP(hipEventRecord(c->ev[0], c->stream));
+    rc = substep_launches(c, rkiter, pend, defer);
+    c->timing = false;
+    if (rc) return rc;
+    IPG_HIP(hipStreamSynchronize(c->stream));
+  }
+  c->ne_pending = defer ? rkiter : 0;
+  std::vector<unsigned long long> st;
+  double rate_khz = 0.0;
+  if (c->use_graphs) {
+    st.resize(c->nev + 1);
+    IPG_HIP(hipMemcpy(st.data(), c->stamps, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    int dev = 0, r = 0;
+    IPG_HIP(hipGetDevice(&dev));
+    IPG_HIP(hipDeviceGetAttribute(&r, hipDeviceAttributeWallClockRate, dev));
+    rate_khz = r;
+  }
+  int k = 0;
+  for (int q = 0; q < c->nev && k < maxn; q++) {
+    if (!c->evname[q]) continue;  // a begin marker
+    const int e = q + 1;
+    int b = e - 1;
+    while (b > 0 && c->evtag[b] != c->evtag[e]) b--;
     float t = 0.f;
-    IPG_HIP(hipEventElapsedTime(&t, c->ev[q], c->ev[q + 1]));
-    ms[q] = t;
-    names[q] = c->evname[q];
+    if (c->use_graphs) t = (float)((double)(st[e] - st[b]) / rate_khz);
+    else IPG_HIP(hipEventElapsedTime(&t, c->ev[b], c->ev[e]));
+    ms[k] = t;
+    names[k] = c->evname[q];
+    k++;
   }
   *n = k;
   return 0;
@@ -696,7 +815,19 @@ int ipg_use_side_stream(ipg_ctx *c, int on) {
   return 0;
 }
 
-// Algorithmic HBM bytes of one substep (FP64), the roofline numerator.
+int ipg_set_refinement(ipg_ctx *c, int nref) {
+  if (!ready(c)) return 1;
+  if (!c->mixed) { ipg_set_error("set_refinement: the context is not mixed precision"); return 1; }
+  if (nref < 0 || nref > IPG_MAX_REFINE) { ipg_set_error("set_refinement: nref must be 0..2"); return 1; }
+  if (fill_p(c)) return 1;
+  if (nref != c->nref)
+    for (int r = 0; r < IPG_NGRAPH; r++)
+      if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
+  c->nref = nref;
+  return 0;
+}
+
+// Algorithmic HBM bytes of one substep, the roofline numerator.
 int ipg_substep_bytes(ipg_ctx *c, double *bytes) {
   if (!check_ctx(c) || !bytes) return 1;
   if (!c->solver_ready) { ipg_set_error("substep_bytes: ipg_initsolver has not been called"); return 1; }
@@ -721,6 +852,23 @@ int ipg_substep_bytes(ipg_ctx *c, double *bytes) {
   const double ibm = np * 3.0 * 3.0 * 3.0 * shell;
   const double box = np * 3.0 * std::pow(2.0 * R / h + 5.0, 3.0);
   *bytes = 8.0 * (fluid + ibm + box);
+  if (c->mixed) {
+    // mixed path, bytes: RK stage 72 N FP64 (u v w c pnew read, dudt..dcdt
+    // written) + the FP32 history (4 written, 4 read and written, 4 read by
+    // stage) 64/3 N; the first forward x pass reads dudt dvdt dwdt and stores
+    // the right-hand side (32 N) and the FP32 spectrum (4 Sw); refinement r
+    // reads the right-hand side and r pieces and stores the FP32 residual,
+    // which its forward x pass reads before storing the spectrum; per solve
+    // two y passes (16 Sw), Thomas (its spectrum words at 4 B, d at 8 B) and
+    // the inverse x pass (4 Sw + 4 N); correc reads dudt dvdt dwdt, pnew and
+    // the nref+1 pieces, writes u v w pnew
+    const int ns = c->nref + 1;
+    double b = (72.0 + 64.0 / 3.0) * N + 32.0 * N + 4.0 * Sw;
+    for (int r = 1; r <= c->nref; r++) b += (8.0 + 4.0 * r + 4.0) * N + 4.0 * N + 4.0 * Sw;
+    b += ns * (16.0 * Sw + 4.0 * (thomas - 2.0 * c->ps2.d_rows) + 16.0 * c->ps2.d_rows + 4.0 * Sw + 4.0 * N);
+    b += (64.0 + 4.0 * ns) * N;
+    *bytes = b + 8.0 * (ibm + box);
+  }
   return 0;
 }
 

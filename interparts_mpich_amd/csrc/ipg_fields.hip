@@ -27,14 +27,19 @@ constexpr int RTX = 64, RTY = 8, RNT = RTX * RTY;
 constexpr int LW = RTX + 2, LH = RTY + 2, LPL = LW * LH;
 constexpr int NLD = (LPL + RNT - 1) / RNT;
 
-template <int STAGE>
+//
+// RT is the storage type of the RK history durk..dcrk: double, or float in
+// the mixed path, where the history is rounded to FP32 when stored and every
+// term is still computed in FP64 (the kernel is bound by HBM traffic, not by
+// its FP64 arithmetic).
+template <int STAGE, typename RT>
 __global__ __launch_bounds__(RNT) void k_rk(Grid g, const double *__restrict__ U, const double *__restrict__ V,
                                             const double *__restrict__ W, const double *__restrict__ C,
                                             const double *__restrict__ P, double *__restrict__ dudt,
                                             double *__restrict__ dvdt, double *__restrict__ dwdt,
-                                            double *__restrict__ dcdt, double *__restrict__ durk,
-                                            double *__restrict__ dvrk, double *__restrict__ dwrk,
-                                            double *__restrict__ dcrk, const double *__restrict__ dtp, int ntx,
+                                            double *__restrict__ dcdt, RT *__restrict__ durk,
+                                            RT *__restrict__ dvrk, RT *__restrict__ dwrk,
+                                            RT *__restrict__ dcrk, const double *__restrict__ dtp, int ntx,
                                             int nty, int kb, int wrapx) {
   __shared__ double sm[2][5][LPL];
   const int nblk = gridDim.x, L = blockIdx.x;
@@ -180,14 +185,14 @@ __global__ __launch_bounds__(RNT) void k_rk(Grid g, const double *__restrict__ U
       dvdt[c] = dv0 + f1 * ay;
       dwdt[c] = dw0 + f1 * az;
       dcdt[c] = Cc + f1 * ac;
-      durk[c] = ax; dvrk[c] = ay; dwrk[c] = az; dcrk[c] = ac;
+      durk[c] = (RT)ax; dvrk[c] = (RT)ay; dwrk[c] = (RT)az; dcrk[c] = (RT)ac;
     } else {
       const double ru = durk[c], rv = dvrk[c], rw = dwrk[c], rc = dcrk[c];
       dudt[c] = du0 + (f2 * ax - f3 * ru);
       dvdt[c] = dv0 + (f2 * ay - f3 * rv);
       dwdt[c] = dw0 + (f2 * az - f3 * rw);
       dcdt[c] = Cc + (f2 * ac - f3 * rc);
-      if (STAGE == 2) { durk[c] = ax; dvrk[c] = ay; dwrk[c] = az; dcrk[c] = ac; }
+      if (STAGE == 2) { durk[c] = (RT)ax; dvrk[c] = (RT)ay; dwrk[c] = (RT)az; dcrk[c] = (RT)ac; }
     }
     // the k-1 values of the next plane
     Ukm = Uc; Vkm = Vc; Wkm = Wc; Wip_km = Wip; Wjp_km = Wjp; Ckm = Cc;
@@ -296,6 +301,83 @@ __global__ __launch_bounds__(BX *BY) void k_correc(Grid g, const double *__restr
   v[c] = dvdt[c] - fj * (pj1 - pc);
   w[c] = dwdt[c] - fk * (pk1 - pc);
   if (FUSED) pnew[c] = pnew[c] + pc;
+}
+
+// The mixed path's pressure correction is the sum of NP FP32 pieces (the
+// FP32-stored solve and its FP64-residual refinements), each exact in FP64:
+// p = (p0 + p1) + p2 in FP64 wherever p is read.
+struct Pieces { const float *q[IPG_MAX_REFINE + 1]; };
+template <int NP>
+__device__ __forceinline__ double psum(const Pieces &P, long c) {
+  double s = (double)P.q[0][c];
+#pragma unroll
+  for (int r = 1; r < NP; r++) s += (double)P.q[r][c];
+  return s;
+}
+
+// k_correc<true> with p read as the sum of its pieces (src/correc.f90:11-42,
+// src/main.f90:370), the same periodic and top-wall source values.
+template <int NP>
+__global__ __launch_bounds__(BX *BY) void k_correc_pieces(Grid g, const double *__restrict__ dudt, const double *__restrict__ dvdt,
+                                                          const double *__restrict__ dwdt, Pieces P, double *__restrict__ u,
+                                                          double *__restrict__ v, double *__restrict__ w,
+                                                          double *__restrict__ pnew, const double *__restrict__ dtp, double rkab) {
+  const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx || j > ny) return;
+  const double factor = rkab * *dtp;
+  const double fi = factor * g.dxi, fj = factor * g.dyi, fk = factor * g.dzi;
+  const long c = g.at(i, j, k);
+  const double pc = psum<NP>(P, c);
+  const double pi1 = psum<NP>(P, i == nx ? c - (nx - 1) : c + 1);
+  const double pj1 = psum<NP>(P, j == ny ? c - (long)(ny - 1) * g.sx : c + g.sx);
+  const double pk1 = k == nz ? pc : psum<NP>(P, c + g.sxy);
+  u[c] = dudt[c] - fi * (pi1 - pc);
+  v[c] = dvdt[c] - fj * (pj1 - pc);
+  w[c] = dwdt[c] - fk * (pk1 - pc);
+  pnew[c] = pnew[c] + pc;
+}
+
+// The mixed path's refinement residual r = rhs - L p, p the sum of its NP
+// pieces so far, in FP64, stored FP32.  L is the operator the Poisson solve
+// inverts: the periodic 3-point second differences in x and y, and in z the
+// Neumann rows of src/initsolver.f90:17-30 (p(0) = p(1), p(kmax+1) =
+// p(kmax)); the pieces' ghosts are never filled, so the neighbours are taken
+// at the source.
+// Each thread marches RZ planes up its column, keeping p below, at and above
+// the current plane in registers, so that every plane of the pieces is read
+// from HBM once; the in-plane neighbours come from the caches.
+constexpr int RZ = 16;
+template <int NP>
+__global__ __launch_bounds__(BX *BY) void k_resid(Grid g, Pieces P, const double *__restrict__ rhs, float *__restrict__ r) {
+  const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1;
+  const int k0 = blockIdx.z * RZ + 1, k1 = min(k0 + RZ - 1, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i > nx || j > ny) return;
+  const long ce = i == nx ? -(long)(nx - 1) : 1, cw = i == 1 ? (long)(nx - 1) : -1;
+  const long cn = j == ny ? -(long)(ny - 1) * g.sx : g.sx, cs = j == 1 ? (long)(ny - 1) * g.sx : -g.sx;
+  long c = g.at(i, j, k0);
+  double pc = psum<NP>(P, c);
+  double pd = k0 == 1 ? pc : psum<NP>(P, c - g.sxy);
+  for (int k = k0; k <= k1; k++, c += g.sxy) {
+    const double pu = k == nz ? pc : psum<NP>(P, c + g.sxy);
+    const double pe = psum<NP>(P, c + ce), pw = psum<NP>(P, c + cw);
+    const double pn = psum<NP>(P, c + cn), ps = psum<NP>(P, c + cs);
+    const double lap = ((pe - pc) - (pc - pw)) * (g.dxi * g.dxi) + ((pn - pc) - (pc - ps)) * (g.dyi * g.dyi) +
+                       ((pu - pc) - (pc - pd)) * (g.dzi * g.dzi);
+    r[c] = (float)(rhs[c] - lap);
+    pd = pc;
+    pc = pu;
+  }
+}
+
+// p = the sum of its pieces on the interior (before p is read as a field)
+template <int NP>
+__global__ __launch_bounds__(BX *BY) void k_p_combine(Grid g, Pieces P, double *__restrict__ p) {
+  const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long c = g.at(i, j, k);
+  p[c] = psum<NP>(P, c);
 }
 
 __global__ __launch_bounds__(BX *BY) void k_pnew(Grid g, const double *__restrict__ p, double *__restrict__ pnew,
@@ -420,7 +502,14 @@ __global__ __launch_bounds__(BX *BY) void k_post3d(Grid g, int q, const double *
   out[(long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (k - 1))] = r;
 }
 
+// one lane stores the device's wall clock (constant rate) into t[slot]
+__global__ void k_stamp(unsigned long long *t, int slot) {
+  if (threadIdx.x == 0) t[slot] = wall_clock64();
+}
+
 }  // namespace
+
+void ipg_stamp(ipg_ctx *c, int slot) { hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, c->stamps, slot); }
 
 static dim3 cell_grid(const Grid &g) { return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, g.nz); }
 
@@ -437,13 +526,23 @@ int ipg_launch_rk(ipg_ctx *c, int rkiter, bool wrapx) {
   }
   const int ntz = (g.nz + kb - 1) / kb;
   dim3 grid(ntx * nty * ntz), blk(RNT);
-#define IPG_RK_LAUNCH(S)                                                                                               \
-  hipLaunchKernelGGL(k_rk<S>, grid, blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],         \
-                     f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK],         \
-                     f[IPG_DCRK], c->dscal, ntx, nty, kb, wrapx ? 1 : 0)
-  if (rkiter == 1) IPG_RK_LAUNCH(1);
-  else if (rkiter == 2) IPG_RK_LAUNCH(2);
-  else IPG_RK_LAUNCH(3);
+#define IPG_RK_LAUNCH(S, RT, RK)                                                                                       \
+  hipLaunchKernelGGL((k_rk<S, RT>), grid, blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],   \
+                     f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], RK(IPG_DURK), RK(IPG_DVRK), RK(IPG_DWRK),     \
+                     RK(IPG_DCRK), c->dscal, ntx, nty, kb, wrapx ? 1 : 0)
+#define RK64(q) f[q]
+#define RK32(q) f32ptr(c, q)
+  if (c->mixed) {
+    if (rkiter == 1) IPG_RK_LAUNCH(1, float, RK32);
+    else if (rkiter == 2) IPG_RK_LAUNCH(2, float, RK32);
+    else IPG_RK_LAUNCH(3, float, RK32);
+  } else {
+    if (rkiter == 1) IPG_RK_LAUNCH(1, double, RK64);
+    else if (rkiter == 2) IPG_RK_LAUNCH(2, double, RK64);
+    else IPG_RK_LAUNCH(3, double, RK64);
+  }
+#undef RK64
+#undef RK32
 #undef IPG_RK_LAUNCH
   IPG_HIP(hipGetLastError());
   ipg_mark(c, "rk_stage");
@@ -496,8 +595,51 @@ int ipg_launch_correc(ipg_ctx *c, int rkiter) {
   return 0;
 }
 
+static Pieces pieces_of(ipg_ctx *c) {
+  Pieces P{};
+  for (int r = 0; r <= IPG_MAX_REFINE; r++) P.q[r] = c->pp[r];
+  return P;
+}
+
+int ipg_launch_p_combine(ipg_ctx *c) {
+  const Grid &g = c->g;
+  const Pieces P = pieces_of(c);
+  if (c->nref == 0) hipLaunchKernelGGL(k_p_combine<1>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, P, c->f[IPG_P]);
+  else if (c->nref == 1) hipLaunchKernelGGL(k_p_combine<2>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, P, c->f[IPG_P]);
+  else hipLaunchKernelGGL(k_p_combine<3>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, P, c->f[IPG_P]);
+  IPG_HIP(hipGetLastError());
+  ipg_mark(c, "p_combine");
+  return 0;
+}
+
+int ipg_launch_resid(ipg_ctx *c, int npieces) {
+  const Grid &g = c->g;
+  const Pieces P = pieces_of(c);
+  float *r = c->pp[IPG_MAX_REFINE + 1];
+  const dim3 grid((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + RZ - 1) / RZ);
+  if (npieces == 1) hipLaunchKernelGGL(k_resid<1>, grid, dim3(BX, BY), 0, c->stream, g, P, c->f[IPG_P], r);
+  else hipLaunchKernelGGL(k_resid<2>, grid, dim3(BX, BY), 0, c->stream, g, P, c->f[IPG_P], r);
+  IPG_HIP(hipGetLastError());
+  ipg_mark(c, "resid");
+  return 0;
+}
+
 int ipg_launch_correc_fused(ipg_ctx *c, int rkiter) {
   const Grid &g = c->g;
+  if (c->mixed) {
+    const Pieces P = pieces_of(c);
+#define IPG_CORREC_PIECES(NP)                                                                                          \
+  hipLaunchKernelGGL(k_correc_pieces<NP>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT], \
+                     c->f[IPG_DWDT], P, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->dscal,              \
+                     c->pc.rkab[rkiter - 1])
+    if (c->nref == 0) IPG_CORREC_PIECES(1);
+    else if (c->nref == 1) IPG_CORREC_PIECES(2);
+    else IPG_CORREC_PIECES(3);
+#undef IPG_CORREC_PIECES
+    IPG_HIP(hipGetLastError());
+    ipg_mark(c, "correc");
+    return 0;
+  }
   hipLaunchKernelGGL(k_correc<true>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT],
                      c->f[IPG_DWDT], c->f[IPG_P], c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->f[IPG_DCDT],
                      c->f[IPG_C], c->dscal, c->pc.rkab[rkiter - 1]);

@@ -39,8 +39,8 @@ struct IbmArgs {
   int bmx, bmy, bmz;  // largest box extent per direction (LDS sizing)
   const double *px, *py, *pz, *pu, *pv, *pw, *pomx, *pomy, *pomz, *pphi;
   const double *mk;   // [3][nl] marker offsets from the sphere centre
-  double *fl;         // [3][np*nl] marker forces fxl, fyl, fzl written by this launch
-  const double *flp;  // [3][np*nl] marker forces of the previous pass (read; another buffer)
+  void *fl;           // [3][np*nl] marker forces fxl, fyl, fzl written by this launch (T, the launch's type)
+  const void *flp;    // [3][np*nl] marker forces of the previous pass (read; another buffer)
   double *fld[3];     // dudt dvdt dwdt
   const int *nbl;     // [np][IBM_MAXNB] neighbour images: q*25 + (sx+2)*5 + (sy+2)
   const int *nbc;     // [np] neighbour image count
@@ -136,9 +136,9 @@ __global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int
 
 // Stencil of marker l of sphere q, seen through the periodic shift (sxs nx,
 // sys ny) in the unwrapped frame of box b; false when it misses the box.
-template <bool WEIGHTS = true>
+template <bool WEIGHTS = true, typename T = double>
 __device__ __forceinline__ bool neighbour_stencil(const IbmArgs &A, const Box &b, int comp, int qs, int l, int sxs, int sys,
-                                                  CompStencil &c) {
+                                                  CompStencilT<T> &c) {
   const Grid &g = A.g;
   const double xq = A.px[qs], yq = A.py[qs], zq = A.pz[qs];
   // a stencil spans less than two cells either side of its marker
@@ -149,7 +149,7 @@ __device__ __forceinline__ bool neighbour_stencil(const IbmArgs &A, const Box &b
     return false;
   double xl, yl, zl;
   if (!marker_pos(g, xq, yq, zq, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return false;
-  comp_stencil<WEIGHTS>(g, xl, yl, zl, comp, c);
+  comp_stencil<WEIGHTS, T>(g, xl, yl, zl, comp, c);
   c.i0 += sxs * g.nx;
   c.j0 += sys * g.ny;
   return !(c.i0 + 2 < b.bi0 || c.i0 >= b.bi0 + b.bx || c.j0 + 2 < b.bj0 || c.j0 >= b.bj0 + b.by ||
@@ -170,9 +170,17 @@ __device__ __forceinline__ bool neighbour_stencil(const IbmArgs &A, const Box &b
 // src/interp_spread.f90:132-154 (((f phi_x) phi_y) phi_z); the force update
 // src/forcing.f90:63-65 / 255-257; spreads src/interp_spread.f90:329-355
 // ((((F c0) phi_x) phi_y) phi_z, c0 = dVlagr / dVeul dt).
-template <int MODE>
+//
+// T is the arithmetic type of the weights, the gathers, the spreads and the
+// LDS box: double (FP64 path) or float (mixed path; positions, stencil base
+// cells and the offsets the weights take are still formed in FP64).  With
+// float, MODE 2 sums only the spread S in the box and adds it to the FP64
+// dudt as it stores, so that dudt is never rounded to FP32.
+template <int MODE, typename T>
 __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_ibm(IbmArgs A, int pass) {
-  extern __shared__ double box[];
+  constexpr bool F64 = sizeof(T) == sizeof(double);
+  extern __shared__ __align__(16) unsigned char ibm_lds[];
+  T *box = reinterpret_cast<T *>(ibm_lds);
   const Grid &g = A.g;
   const int p = blockIdx.x / 3, comp = blockIdx.x % 3;
   const long M = (long)A.np * A.nl;
@@ -180,8 +188,9 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     const long nplane = (long)g.nx * g.ny;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nplane; t += (long)gridDim.x * blockDim.x) A.Kzero[t] = 0.0;
   }
-  const double dt = *A.dtp, dti = 1.0 / dt;
-  const double c0 = (A.dVlagr / A.dVeul) * dt;
+  const double dt = *A.dtp;
+  const T dti = (T)(1.0 / dt);
+  const T c0 = (T)((A.dVlagr / A.dVeul) * dt);
   const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
   Box b = comp_box(A.g, A.radfp, xp, yp, zp, comp);
   if (b.bx > A.bmx || b.by > A.bmy || b.bz > A.bmz) {
@@ -198,10 +207,10 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   int *rowoff = (int *)(taken + nflagmax);
   int *icol = rowoff + A.bmy * A.bmz;
   double *F = A.fld[comp];
-  double *fl = A.fl + comp * M;
+  T *fl = (T *)A.fl + comp * M;
   // the previous pass's forces: a buffer no workgroup of this launch writes,
   // as neighbours read them while their owners compute the new ones
-  const double *flp = A.flp + comp * M;
+  const T *flp = (const T *)A.flp + comp * M;
   const long m0 = (long)p * A.nl;
   // Own marker q of this thread: stencil, LDS index e0 of the stencil's first
   // cell, how many of its z planes lie in the box (the rest is the k1 plane),
@@ -209,20 +218,22 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   // rather than kept live across the launch.  The box holds the stencil in x
   // and y, and in z up to kmax (comp_box); a stencil that were not held is
   // reported (error bit 4) and left out rather than index outside the box.
-  auto own = [&](int q, CompStencil &c, int &e0, int &nzin, double &ub) -> bool {
+  auto own = [&](int q, CompStencilT<T> &c, int &e0, int &nzin, T &ub) -> bool {
     const int l = (int)threadIdx.x + q * IBM_THREADS;
     double xl, yl, zl;
     if (l >= A.nl || !marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return false;
-    comp_stencil(g, xl, yl, zl, comp, c);
+    comp_stencil<true, T>(g, xl, yl, zl, comp, c);
     e0 = box_at(b, c.i0, c.j0, c.k0);
     if (e0 < 0 || box_at(b, c.i0 + 2, c.j0 + 2, c.k0) < 0) {
       if (MODE == 0) atomicOr(A.err, 4);
       return false;
     }
     nzin = min(3, b.bk0 + b.bz - c.k0);
-    if (comp == 0) ub = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
-    else if (comp == 1) ub = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
-    else ub = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
+    double u;
+    if (comp == 0) u = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
+    else if (comp == 1) u = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
+    else u = A.pw[p] + A.pomx[p] * (yl - yp) - A.pomy[p] * (xl - xp);
+    ub = (T)u;
     return true;
   };
   // the cells the own stencils touch: flagged by pass 0, read back by the
@@ -238,7 +249,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   }
   for (int t = threadIdx.x; t < b.bx; t += blockDim.x) icol[t] = wrapany(b.bi0 + t, g.nx) - 1;
   if (MODE >= 1)
-    for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = 0.0;
+    for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = (T)0;
   __shared__ int lcount;
   if (MODE == 0 && threadIdx.x == 0) lcount = 0;
   __syncthreads();
@@ -253,7 +264,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     if (li >= b.bx) { li -= b.bx; row++; }
   };
   constexpr int CH = 8;
-  if (MODE >= 1) {
+  if (MODE == 1 || (MODE == 2 && F64)) {
     // dudt on the cells the own stencils touch (flagged by pass 0), added
     // into the box together with the spreads below, so that its loads overlap
     // them: the box ends as dudt + S, the reference's dudtf, with the terms
@@ -274,7 +285,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
       }
 #pragma unroll
       for (int u = 0; u < CH; u++)
-        if ((fm >> u) & 1u) unsafeAtomicAdd(&box[t0 + u * IBM_THREADS], v[u]);
+        if ((fm >> u) & 1u) unsafeAtomicAdd(&box[t0 + u * IBM_THREADS], (T)v[u]);
     }
   }
   const long lbase = (long)blockIdx.x * IBM_MAXNM;
@@ -307,9 +318,9 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
       const int code = A.nml[lbase + idx];
       const int mq = code / 25, sh = code % 25;
       const int qs = mq / A.nl, l = mq - qs * A.nl;
-      CompStencil c;
-      if (!neighbour_stencil(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c)) continue;
-      const double fc = flp[mq] * c0;
+      CompStencilT<T> c;
+      if (!neighbour_stencil<true, T>(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c)) continue;
+      const T fc = flp[mq] * c0;
 #pragma unroll
       for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -327,19 +338,19 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   // the previous pass into S (after the neighbours' part, so that the first
   // marker's stencil, kept in registers for the gather below, is not live
   // across the neighbour loop).
-  CompStencil cs0{};
+  CompStencilT<T> cs0{};
   int e00 = 0, nz00 = 0;
-  double ub00 = 0.0;
+  T ub00 = (T)0;
   bool have0 = false;
 #pragma unroll
   for (int q = 0; q < IBM_MPT; q++) {
-    CompStencil c;
+    CompStencilT<T> c;
     int e0, nzin;
-    double ub;
+    T ub;
     if (!own(q, c, e0, nzin, ub)) continue;
     if (q == 0 && MODE <= 1) { cs0 = c; e00 = e0; nz00 = nzin; ub00 = ub; have0 = true; }
     const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
-    const double fc = MODE >= 1 ? flp[m] * c0 : 0.0;
+    const T fc = MODE >= 1 ? flp[m] * c0 : (T)0;
 #pragma unroll
     for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -364,7 +375,11 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     // dudt = dudtf = fl(dudt + S) (src/main.f90:332-334), once per cell
     int li = li0, row = row0;
     for (int t = threadIdx.x; t < nbox; t += IBM_THREADS) {
-      if (flagged(t) && !((taken[t >> 5] >> (t & 31)) & 1u)) F[(long)rowoff[row] + icol[li]] = box[t];
+      if (flagged(t) && !((taken[t >> 5] >> (t & 31)) & 1u)) {
+        const long x = (long)rowoff[row] + icol[li];
+        if (F64) F[x] = (double)box[t];
+        else F[x] = F[x] + (double)box[t];
+      }
       advance(li, row);
     }
     return;
@@ -385,16 +400,16 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
       for (int u = 0; u < CH; u++) {
         const int t = t0 + u * IBM_THREADS;
-        if (t < nbox) box[t] = v[u];
+        if (t < nbox) box[t] = (T)v[u];
       }
     }
     __syncthreads();
   }
 #pragma unroll
   for (int q = 0; q < IBM_MPT; q++) {
-    CompStencil c;
+    CompStencilT<T> c;
     int e0, nzin;
-    double ub;
+    T ub;
     if (q == 0 && MODE <= 1) {
       if (!have0) continue;
       c = cs0; e0 = e00; nzin = nz00; ub = ub00;
@@ -402,7 +417,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
       continue;
     }
     const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
-    double acc = 0.0;
+    T acc = (T)0;
 #pragma unroll
     for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -419,25 +434,27 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
         for (int j = 0; j < 3; j++)
 #pragma unroll
           for (int i = 0; i < 3; i++)
-            if (k >= nzin) acc = acc + A.Kread[kplane(g, c.i0 + i, c.j0 + j)] * c.wx[i] * c.wy[j] * c.wz[k];
+            if (k >= nzin) acc = acc + (T)A.Kread[kplane(g, c.i0 + i, c.j0 + j)] * c.wx[i] * c.wy[j] * c.wz[k];
     }
-    const double fnew = MODE == 0 ? (ub - acc) * dti : flp[m] + (ub - acc) * dti;
+    const T fnew = MODE == 0 ? (ub - acc) * dti : flp[m] + (ub - acc) * dti;
     fl[m] = fnew;
     if (nzin < 3) {
-      const double fc = fnew * c0;
+      const T fc = fnew * c0;
 #pragma unroll
       for (int k = 1; k < 3; k++)
 #pragma unroll
         for (int j = 0; j < 3; j++)
 #pragma unroll
           for (int i = 0; i < 3; i++)
-            if (k >= nzin) unsafeAtomicAdd(&A.Kspread[kplane(g, c.i0 + i, c.j0 + j)], fc * c.wx[i] * c.wy[j] * c.wz[k]);
+            if (k >= nzin)
+              unsafeAtomicAdd(&A.Kspread[kplane(g, c.i0 + i, c.j0 + j)], (double)(fc * c.wx[i] * c.wy[j] * c.wz[k]));
     }
   }
 }
 
 // Per-sphere force and torque sums after the last pass, src/forcing.f90:258-287,
 // one block per sphere.
+template <typename T>
 __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, double *fyt, double *fzt, double *tqx, double *tqy,
                                                     double *tqz, double *tqt) {
   const Grid &g = A.g;
@@ -449,7 +466,8 @@ __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, doub
     double xl, yl, zl;
     if (!marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
     const long m = (long)p * A.nl + l;
-    const double fx = A.fl[m], fy = A.fl[M + m], fz = A.fl[2 * M + m];
+    const T *f = (const T *)A.fl;
+    const double fx = f[m], fy = f[M + m], fz = f[2 * M + m];
     s[0] += fx; s[1] += fy; s[2] += fz;
     s[3] += (yl - yp) * fz - (zl - zp) * fy;
     s[4] += (zl - zp) * fx - (xl - xp) * fz;
@@ -473,9 +491,9 @@ __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, doub
 
 // LDS of one k_ibm workgroup: the box, its two cell masks, the row offsets
 // and the wrapped column indices.
-size_t ibm_lds_bytes(int bmx, int bmy, int bmz) {
+size_t ibm_lds_bytes(int bmx, int bmy, int bmz, size_t esize) {
   const size_t cells = (size_t)bmx * bmy * bmz;
-  return cells * sizeof(double) + 2 * ((cells + 31) / 32) * sizeof(unsigned) + ((size_t)bmy * bmz + bmx) * sizeof(int);
+  return cells * esize + 2 * ((cells + 31) / 32) * sizeof(unsigned) + ((size_t)bmy * bmz + bmx) * sizeof(int);
 }
 
 
@@ -487,7 +505,7 @@ size_t ibm_lds_bytes(int bmx, int bmy, int bmz) {
 int ipg_ibm_setup(ipg_ctx *c) {
   const int bmx = ibm_box_max(c->pc.radfp, c->g.dxi), bmy = ibm_box_max(c->pc.radfp, c->g.dyi);
   const int bmz = ibm_box_max(c->pc.radfp, c->g.dzi);
-  const size_t lds = ibm_lds_bytes(bmx, bmy, bmz);
+  const size_t lds = ibm_lds_bytes(bmx, bmy, bmz, c->ibm32 ? sizeof(float) : sizeof(double));
   if (lds > 160 * 1024) { ipg_set_error("ibm: sphere box exceeds the 160 KiB LDS of a CU"); return 1; }
   // the box's row offsets are kept as 32-bit integers
   if ((double)c->g.sxy * (c->g.nz + 2) >= 2147483647.0) { ipg_set_error("ibm: field larger than 2^31 cells"); return 1; }
@@ -495,9 +513,15 @@ int ipg_ibm_setup(ipg_ctx *c) {
   // own periodic image
   if (bmx + 1 > c->g.nx || bmy + 1 > c->g.ny) { ipg_set_error("ibm: sphere box wider than the domain"); return 1; }
   if (c->pc.nl > IBM_THREADS * IBM_MPT) { ipg_set_error("ibm: more than 1024 markers per sphere"); return 1; }
-  IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (c->ibm32) {
+    IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<0, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<1, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<2, float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  } else {
+    IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<0, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<1, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<2, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
   // the captured substeps hold the IBM launch arguments (marker radius, box
   // sizes, dVlagr) by value: a new marker set-up drops them
   for (int r = 0; r < IPG_NGRAPH; r++)
@@ -526,7 +550,7 @@ bool ibm_args(ipg_ctx *c, IbmArgs &A, size_t &lds) {
   A.bmx = ibm_box_max(c->pc.radfp, c->g.dxi);
   A.bmy = ibm_box_max(c->pc.radfp, c->g.dyi);
   A.bmz = ibm_box_max(c->pc.radfp, c->g.dzi);
-  lds = ibm_lds_bytes(A.bmx, A.bmy, A.bmz);
+  lds = ibm_lds_bytes(A.bmx, A.bmy, A.bmz, c->ibm32 ? sizeof(float) : sizeof(double));
   double **q = c->ps.q;
   A.px = q[IPG_PX]; A.py = q[IPG_PY]; A.pz = q[IPG_PZ]; A.pu = q[IPG_PU]; A.pv = q[IPG_PV]; A.pw = q[IPG_PW];
   A.pomx = q[IPG_POMX]; A.pomy = q[IPG_POMY]; A.pomz = q[IPG_POMZ]; A.pphi = q[IPG_PPHI];
@@ -568,8 +592,14 @@ int ipg_launch_ibm_gather(ipg_ctx *c, int pass) {
   A.Kspread = K[(pass + 1) % 3];
   A.Kzero = pass == 0 ? nullptr : K[pass - 1];
   static const char *names[3] = {"ibm_pass0", "ibm_pass1", "ibm_pass2"};
-  if (pass == 0) hipLaunchKernelGGL(k_ibm<0>, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, pass);
-  else hipLaunchKernelGGL(k_ibm<1>, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, pass);
+  const dim3 grid(3 * A.np), blk(IBM_THREADS);
+  if (c->ibm32) {
+    if (pass == 0) hipLaunchKernelGGL((k_ibm<0, float>), grid, blk, lds, c->stream, A, pass);
+    else hipLaunchKernelGGL((k_ibm<1, float>), grid, blk, lds, c->stream, A, pass);
+  } else {
+    if (pass == 0) hipLaunchKernelGGL((k_ibm<0, double>), grid, blk, lds, c->stream, A, pass);
+    else hipLaunchKernelGGL((k_ibm<1, double>), grid, blk, lds, c->stream, A, pass);
+  }
   ipg_mark(c, names[pass]);
   IPG_HIP(hipGetLastError());
   return 0;
@@ -583,7 +613,8 @@ int ipg_launch_ibm_final(ipg_ctx *c) {
   if (!ibm_args(c, A, lds)) return 1;
   A.Kzero = c->K1[2];
   A.flp = c->fl;
-  hipLaunchKernelGGL(k_ibm<2>, dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, 2);
+  if (c->ibm32) hipLaunchKernelGGL((k_ibm<2, float>), dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, 2);
+  else hipLaunchKernelGGL((k_ibm<2, double>), dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, 2);
   ipg_mark(c, "ibm_spread");
   IPG_HIP(hipGetLastError());
   return 0;
@@ -597,7 +628,8 @@ int ipg_launch_ibm_totals(ipg_ctx *c, int pass) {
   if (!ibm_args(c, A, lds)) return 1;
   if (pass == 1) A.fl = c->fl2;
   double **q = c->ps.q;
-  hipLaunchKernelGGL(k_ibm_totals, dim3(A.np), dim3(256), 0, c->stream, A, q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
+  hipLaunchKernelGGL(c->ibm32 ? k_ibm_totals<float> : k_ibm_totals<double>, dim3(A.np), dim3(256), 0, c->stream, A,
+                     q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
                      q[IPG_PTORQX], q[IPG_PTORQY], q[IPG_PTORQZ], q[IPG_PTORQTHETA]);
   ipg_mark(c, "ibm_totals");
   IPG_HIP(hipGetLastError());

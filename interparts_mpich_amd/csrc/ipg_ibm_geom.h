@@ -100,10 +100,15 @@ __host__ __device__ __forceinline__ int box_at(const Box &b, int iu, int ju, int
 // and the three weights per direction, in the reference's order
 // (src/interp_spread.f90:76-130).  Only the 9 weights the component uses are
 // evaluated; each is the same expression stencil() evaluates.
-// With WEIGHTS = false only the base cell is found.
-struct CompStencil { int i0, j0, k0; double wx[3], wy[3], wz[3]; };
-template <bool WEIGHTS = true>
-__host__ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl, double zl, int comp, CompStencil &c) {
+// With WEIGHTS = false only the base cell is found.  T is the type of the
+// weights: the offsets r are formed in FP64 either way and rounded to T for
+// the kernel (FP32 in the mixed path).
+template <typename T>
+struct CompStencilT { int i0, j0, k0; T wx[3], wy[3], wz[3]; };
+using CompStencil = CompStencilT<double>;
+template <bool WEIGHTS = true, typename T = double>
+__host__ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl, double zl, int comp,
+                                                      CompStencilT<T> &c) {
   const int sxw = (int)floor((xl - 0.5 * g.dx) / g.lx) * g.nx;
   const int syw = (int)floor((yl - 0.5 * g.dy) / g.ly) * g.ny;
   const double cx = xl * g.dxi - (double)sxw - 0.0;
@@ -120,9 +125,9 @@ __host__ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, 
   }
 #pragma unroll
   for (int q = 0; q < 3 && WEIGHTS; q++) {
-    c.wx[q] = roma_kernel(comp == 0 ? ((double)c.i0 + (double)q) - cx : (((double)c.i0 + (double)q) - 0.5) - cx);
-    c.wy[q] = roma_kernel(comp == 1 ? ((double)c.j0 + (double)q) - cy : (((double)c.j0 + (double)q) - 0.5) - cy);
-    c.wz[q] = roma_kernel(comp == 2 ? ((double)c.k0 + (double)q) - cz : (((double)c.k0 + (double)q) - 0.5) - cz);
+    c.wx[q] = roma_kernel((T)(comp == 0 ? ((double)c.i0 + (double)q) - cx : (((double)c.i0 + (double)q) - 0.5) - cx));
+    c.wy[q] = roma_kernel((T)(comp == 1 ? ((double)c.j0 + (double)q) - cy : (((double)c.j0 + (double)q) - 0.5) - cy));
+    c.wz[q] = roma_kernel((T)(comp == 2 ? ((double)c.k0 + (double)q) - cz : (((double)c.k0 + (double)q) - 0.5) - cz));
   }
   c.i0 += sxw;
   c.j0 += syw;

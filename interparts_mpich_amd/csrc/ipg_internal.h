@@ -9,6 +9,7 @@
 
 #define IPG_NB 9
 #define IPG_NGRAPH 12
+#define IPG_NMARK 96
 
 // Grid description passed by value to every field kernel.
 // Field layout in HBM: element (i,j,k), i in 0..nx+1, j in 0..ny+1,
@@ -45,7 +46,7 @@ struct PartState {
 struct Poisson {
   int nmx;          // nx/2+1 complex x-modes
   long sp;          // spectrum row pitch (complex elements)
-  double2 *spec;    // [nz][ny][sp] complex spectrum
+  double2 *spec;    // [nz][ny][sp] complex spectrum (float2 in the mixed path)
   double *dtri;     // [nz][ny][sp] Thomas multipliers d
   double *a, *b, *c;// [nz+1] tridiagonal coefficients (1-based)
   double *xrt, *yrt;// eigenvalues by x-mode m (nmx) and y-mode my (ny)
@@ -53,6 +54,14 @@ struct Poisson {
   int radx[16], nradx, rady[16], nrady;
   double d_rows;    // rows of d stored and read back, over the (m, |my|) columns
 };
+
+// Mixed precision (ipg_params.precision = 1).  Kept FP64: u, v, w, pnew, c,
+// the stage increments dudt..dcdt, the particles, the sphere integrals, the
+// IBM (unless IPG_MIXED_IBM=32) and every arithmetic step of the stencil and
+// the Poisson passes.  Stored FP32: the RK history durk..dcrk, the Poisson
+// spectra between passes, the pressure correction as a sum of FP32 pieces
+// (the solve and its FP64-residual refinements) and the residual.
+constexpr int IPG_MAX_REFINE = 2;
 
 struct ipg_ctx {
   ipg_params prm;
@@ -62,6 +71,13 @@ struct ipg_ctx {
   size_t nfield;                  // doubles per field allocation
   double *slab;                   // one allocation for all fields
   double *f[IPG_NFIELDS];         // field base pointers (see Grid)
+  bool mixed;                     // precision 1
+  bool ibm32;                     // the IBM computes in FP32 (mixed path with IPG_MIXED_IBM=32)
+  bool f32[IPG_NFIELDS];          // field stored FP32 (its slot read as float, same indexing)
+  int nref;                       // FP64-residual refinement sweeps of the mixed Poisson solve (0..2)
+  float *pbuf;                    // their allocation
+  float *pp[IPG_MAX_REFINE + 2];  // the pressure correction's FP32 pieces 0..nref, then the refinement residual (field indexing)
+  bool p_in_pieces;               // p is the sum of the pieces; the P field holds the right-hand side
   // IBM
   double *K1[3];                  // dwdtf(.,.,k1) planes (nx*ny), src/interp_spread.f90:100 (no upper clamp)
   bool ibm_lds_set;
@@ -103,19 +119,34 @@ struct ipg_ctx {
   bool side_on;                   // carry the update over to the side stream (ipg_use_side_stream)
   hipStream_t side;
   hipEvent_t fork_ev, join_ev;
-  // kernel timing (ipg_substep_phases): an event is recorded after every
-  // launch while timing is on
-  bool timing;
+  // kernel timing (ipg_substep_phases): a marker after every launch while
+  // timing is on -- a HIP event on eager launches, or, in a substep replayed
+  // from a captured graph, a one-lane kernel storing the device's wall clock
+  // (events recorded during a capture are not replayed); begin markers (no
+  // name) where a stream starts after waiting on the other
+  bool timing, stamping;
   int nev;
-  hipEvent_t ev[64];
-  const char *evname[64];
+  hipEvent_t ev[IPG_NMARK];
+  const char *evname[IPG_NMARK];
+  int evtag[IPG_NMARK];           // 0: the main stream, 1: the side stream
+  unsigned long long *stamps;     // [IPG_NMARK] wall-clock marks
+  hipStream_t main_stream;        // the context's stream (c->stream is swapped while the side stream is filled)
 };
 
-// record a timing event named for the kernel just launched
+// an FP32 field's base pointer: the same slot and indexing as f[q], float
+// elements (element -1, the ghost i = 0, exists)
+inline float *f32ptr(ipg_ctx *c, int q) { return (float *)(c->slab + (size_t)q * c->nfield) + 16; }
+
+// a wall-clock mark into stamps[slot] on the context's current stream
+void ipg_stamp(ipg_ctx *c, int slot);
+// record a timing marker named for the kernel just launched (name null: a
+// begin marker)
 inline void ipg_mark(ipg_ctx *c, const char *name) {
-  if (!c->timing || c->nev >= 63) return;
-  (void)hipEventRecord(c->ev[c->nev + 1], c->stream);
+  if (!c->timing || c->nev >= IPG_NMARK - 1) return;
+  if (c->stamping) ipg_stamp(c, c->nev + 1);
+  else (void)hipEventRecord(c->ev[c->nev + 1], c->stream);
   c->evname[c->nev] = name;
+  c->evtag[c->nev + 1] = c->stream == c->main_stream ? 0 : 1;
   c->nev++;
 }
 
@@ -145,6 +176,23 @@ __host__ __device__ __forceinline__ double roma_kernel(double r) {
   const double w = outer ? (1.0 / 6.0) * ((5.0 - 3.0 * ar) - s) : (1.0 / 3.0) * (1.0 + s);
   return ar > 1.5 ? 0.0 : w;
 }
+// The same kernel in FP32 (the mixed path's IBM), with the hardware square
+// root (1 ulp) on the device rather than the correctly rounded sequence.
+__host__ __device__ __forceinline__ float sqrt_f32(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+__host__ __device__ __forceinline__ float roma_kernel(float r) {
+  const float ar = fabsf(r);
+  const bool outer = ar > 0.5f;
+  const float x = outer ? 1.0f - ar : r;
+  const float s = sqrt_f32(-(3.0f * (x * x)) + 1.0f);
+  const float w = outer ? (1.0f / 6.0f) * ((5.0f - 3.0f * ar) - s) : (1.0f / 3.0f) * (1.0f + s);
+  return ar > 1.5f ? 0.0f : w;
+}
 
 // periodic wrap of an index in [-1, n+2] onto 1..n
 __device__ __forceinline__ int wrapi(int i, int n) {
@@ -161,6 +209,8 @@ int ipg_launch_bound(ipg_ctx *c, int mask);
 int ipg_launch_fillps(ipg_ctx *c, int rkiter);
 int ipg_launch_correc(ipg_ctx *c, int rkiter);
 int ipg_launch_pnew(ipg_ctx *c);
+int ipg_launch_p_combine(ipg_ctx *c);
+int ipg_launch_resid(ipg_ctx *c, int npieces);
 int ipg_launch_chkdt(ipg_ctx *c, double *dtmax);
 int ipg_launch_chkdiv(ipg_ctx *c, double *tot, double *mx);
 int ipg_poisson_setup(ipg_ctx *c);

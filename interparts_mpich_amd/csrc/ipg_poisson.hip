@@ -176,23 +176,47 @@ __device__ __forceinline__ double rhs_at(const Grid &g, const double *__restrict
   return (wk - wkm) * dtidzi + (dv[c] - dv[cjm]) * dtidyi + (du[c] - du[cim]) * dtidxi;
 }
 
+// Spectrum storage between the passes: double2 (FP64 path) or float2 (mixed
+// path, rounded when stored).  Registers and LDS always hold FP64.
+__device__ __forceinline__ double2 ld2(const double2 *p, long i) { return p[i]; }
+__device__ __forceinline__ double2 ld2(const float2 *p, long i) {
+  const float2 v = p[i];
+  return make_double2((double)v.x, (double)v.y);
+}
+__device__ __forceinline__ void st2(double2 *p, long i, double2 v) { p[i] = v; }
+template <typename TS> __device__ __forceinline__ TS zero2();
+template <> __device__ __forceinline__ double2 zero2<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ __forceinline__ float2 zero2<float2>() { return make_float2(0.0f, 0.0f); }
+__device__ __forceinline__ void st2(float2 *p, long i, double2 v) { p[i] = make_float2((float)v.x, (float)v.y); }
+
+// Input of the forward x pass.
+//   XSRC_P:   the right-hand side in p's interior (ipg_fillps wrote it);
+//   XSRC_DIV: the right-hand side of src/fillps.f90 computed on the fly from
+//             dudt, dvdt, dwdt, and stored to rhs_out (FP64) when given;
+//   XSRC_R32: an FP32 field in p's indexing (the mixed path's refinement
+//             residual, k_resid).
+enum { XSRC_P = 0, XSRC_DIV = 1, XSRC_R32 = 2 };
+struct XSrc {
+  const double *p;
+  const double *du, *dv, *dw, *dtp;
+  double rkab;
+  double *rhs_out;
+  const float *r32;
+};
+
 // Forward x transform: rows (j, j+1) of plane k form one complex sequence;
-// the two half spectra are split after the transform.  Input is p's interior
-// (fused = 0) or the divergence of dudt computed on the fly (fused = 1).
-// spec[(k*ny + j)*sp + m], m = 0..nx/2.
-template <int N>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, const double *__restrict__ p, double2 *__restrict__ spec,
-                                                             long sp, const double2 *__restrict__ tw, int fused,
-                                                             const double *__restrict__ du, const double *__restrict__ dv,
-                                                             const double *__restrict__ dw, const double *__restrict__ dtp,
-                                                             double rkab) {
+// the two half spectra are split after the transform.  The input is chosen by
+// SRC (XSRC_*).  spec[(k*ny + j)*sp + m], m = 0..nx/2, stored as TS.
+template <int N, typename TS, int SRC>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, XSrc src, TS *__restrict__ spec, long sp,
+                                                             const double2 *__restrict__ tw) {
   constexpr int B = xbatch_of<N>();
   __shared__ double2 buf[2 * B * N];
   const long pairs = (long)(g.ny / 2) * g.nz;
   const long pair0 = (long)blockIdx.x * B;
   double dtidxi = 0.0, dtidyi = 0.0, dtidzi = 0.0;
-  if (fused) {
-    const double dti = 1.0 / (rkab * *dtp);
+  if (SRC == XSRC_DIV) {
+    const double dti = 1.0 / (src.rkab * *src.dtp);
     dtidxi = dti * g.dxi; dtidyi = dti * g.dyi; dtidzi = dti * g.dzi;
   }
   // the loads of CHK rows' worth of elements in flight before their LDS stores
@@ -208,9 +232,16 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, const doubl
       if (t < B * N && pr < pairs) {
         const int k = (int)(pr / (g.ny / 2)) + 1, j = 2 * (int)(pr % (g.ny / 2)) + 1;
         const long c = g.at(i + 1, j, k);
-        if (fused) v[u] = make_double2(rhs_at(g, du, dv, dw, c, i + 1, j, k, dtidxi, dtidyi, dtidzi),
-                                       rhs_at(g, du, dv, dw, c + g.sx, i + 1, j + 1, k, dtidxi, dtidyi, dtidzi));
-        else v[u] = make_double2(p[c], p[c + g.sx]);
+        if (SRC == XSRC_DIV) {
+          const double r0 = rhs_at(g, src.du, src.dv, src.dw, c, i + 1, j, k, dtidxi, dtidyi, dtidzi);
+          const double r1 = rhs_at(g, src.du, src.dv, src.dw, c + g.sx, i + 1, j + 1, k, dtidxi, dtidyi, dtidzi);
+          if (src.rhs_out) { src.rhs_out[c] = r0; src.rhs_out[c + g.sx] = r1; }
+          v[u] = make_double2(r0, r1);
+        } else if (SRC == XSRC_R32) {
+          v[u] = make_double2((double)src.r32[c], (double)src.r32[c + g.sx]);
+        } else {
+          v[u] = make_double2(src.p[c], src.p[c + g.sx]);
+        }
       }
     }
 #pragma unroll
@@ -231,48 +262,51 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, const doubl
     double2 za = x[b * N + m];
     double2 zb = x[b * N + (N - m) % N];
     zb.y = -zb.y;
-    spec[((long)k * g.ny + j) * sp + m] = make_double2(0.5 * (za.x + zb.x), 0.5 * (za.y + zb.y));
-    spec[((long)k * g.ny + j + 1) * sp + m] = make_double2(0.5 * (za.y - zb.y), -0.5 * (za.x - zb.x));
+    st2(spec, ((long)k * g.ny + j) * sp + m, make_double2(0.5 * (za.x + zb.x), 0.5 * (za.y + zb.y)));
+    st2(spec, ((long)k * g.ny + j + 1) * sp + m, make_double2(0.5 * (za.y - zb.y), -0.5 * (za.x - zb.x)));
   }
 }
 
 // y transform of a tile of B x-modes of one z-plane, in place in spec.
-template <int N>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_y_t(Grid g, double2 *__restrict__ spec, long sp, int nmx,
+template <int N, typename TS>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_y_t(Grid g, TS *__restrict__ spec, long sp, int nmx,
                                                          const double2 *__restrict__ tw, int sign, int top_first) {
   constexpr int B = batch_of<N>();
   __shared__ double2 buf[2 * B * N];
   const int tiles = (nmx + B - 1) / B;
   const int kb = blockIdx.x / tiles, m0 = (blockIdx.x % tiles) * B;
   const int k = top_first ? g.nz - 1 - kb : kb;
-  double2 *plane = spec + (long)k * g.ny * sp;
+  TS *plane = spec + (long)k * g.ny * sp;
   // every load of the tile in flight before the first LDS store
+  // (held in the storage type until stored to LDS: a conversion next to its
+  // load would wait for it)
   constexpr int IT = (B * N + FFT_THREADS - 1) / FFT_THREADS;
-  double2 v[IT];
+  TS v[IT];
 #pragma unroll
   for (int u = 0; u < IT; u++) {
     const int t = threadIdx.x + u * FFT_THREADS;
     const int jj = t / B, b = t % B;
-    v[u] = (t < B * N && m0 + b < nmx) ? plane[(long)jj * sp + m0 + b] : make_double2(0.0, 0.0);
+    v[u] = (t < B * N && m0 + b < nmx) ? plane[(long)jj * sp + m0 + b] : zero2<TS>();
   }
 #pragma unroll
   for (int u = 0; u < IT; u++) {
     const int t = threadIdx.x + u * FFT_THREADS;
-    if (t < B * N) buf[t] = v[u];
+    if (t < B * N) buf[t] = ld2(v, u);
   }
   __syncthreads();
   double2 *x = buf, *y = buf + B * N;
   fft_stages<N, B, true, 0, 1>(x, y, tw, sign);
   for (int t = threadIdx.x; t < B * N; t += FFT_THREADS) {
     const int jj = t / B, b = t % B;
-    if (m0 + b < nmx) plane[(long)jj * sp + m0 + b] = x[jj * B + b];
+    if (m0 + b < nmx) st2(plane, (long)jj * sp + m0 + b, x[jj * B + b]);
   }
 }
 
 // Inverse x transform of two half spectra (rows j, j+1) into p's interior,
-// with the overall 1/(nx*ny) normalisation.
-template <int N>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, double *__restrict__ p, const double2 *__restrict__ spec,
+// with the overall 1/(nx*ny) normalisation; p stored as TP (double, or a
+// float piece of the mixed path's pressure correction).
+template <int N, typename TS, typename TP>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, TP *__restrict__ p, const TS *__restrict__ spec,
                                                              long sp, const double2 *__restrict__ tw, double scale, int top_first) {
   constexpr int B = xbatch_of<N>();
   __shared__ double2 buf[2 * B * N];
@@ -282,17 +316,17 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, double *__r
   // modes m > N/2 are the conjugates of N - m; every load of the block in
   // flight before the first LDS store
   constexpr int IT = (B * N + FFT_THREADS - 1) / FFT_THREADS;
-  double2 va[IT], vb[IT];
+  TS va[IT], vb[IT];
 #pragma unroll
   for (int u = 0; u < IT; u++) {
     const int t = threadIdx.x + u * FFT_THREADS;
     const int b = t / N, m = t % N;
     const long pr = pair0 + b;
-    va[u] = make_double2(0.0, 0.0);
-    vb[u] = make_double2(0.0, 0.0);
+    va[u] = zero2<TS>();
+    vb[u] = zero2<TS>();
     if (t < B * N && pr < pairs) {
       const int k = (int)(pr / (g.ny / 2)), j = 2 * (int)(pr % (g.ny / 2));
-      const double2 *ra = spec + ((long)k * g.ny + j) * sp;
+      const TS *ra = spec + ((long)k * g.ny + j) * sp;
       const int mm = m <= half ? m : N - m;
       va[u] = ra[mm];
       vb[u] = ra[sp + mm];
@@ -302,7 +336,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, double *__r
   for (int u = 0; u < IT; u++) {
     const int t = threadIdx.x + u * FFT_THREADS;
     const int m = t % N;
-    double2 xa = va[u], xb = vb[u];
+    double2 xa = ld2(va, u), xb = ld2(vb, u);
     if (m > half) { xa.y = -xa.y; xb.y = -xb.y; }
     if (t < B * N) buf[t] = make_double2(xa.x - xb.y, xa.y + xb.x);
   }
@@ -315,8 +349,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, double *__r
     if (pr >= pairs) continue;
     const int k = (int)(pr / (g.ny / 2)) + 1, j = 2 * (int)(pr % (g.ny / 2)) + 1;
     const long c = g.at(i + 1, j, k);
-    p[c] = x[b * N + i].x * scale;
-    p[c + g.sx] = x[b * N + i].y * scale;
+    p[c] = (TP)(x[b * N + i].x * scale);
+    p[c + g.sx] = (TP)(x[b * N + i].y * scale);
   }
 }
 
@@ -426,10 +460,16 @@ constexpr int TD = 16;
 // rows per column kept in registers (a multiple of TD): 48 keeps two
 // wavefronts per SIMD (221 VGPRs); 64 would leave one
 constexpr int THOMAS_TAIL = 48;
-__global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
+// TS is the spectrum's storage type (double, or float in the mixed path, whose
+// sweeps still run in FP64).  pin_zero (mixed path): the singular kx = ky = 0
+// column takes the branch z == 0 -> p(kmax) = 0 whatever its last pivot, so
+// that the solve's free constant is never large beside the FP32 pieces it
+// lands in (the constant does not reach the velocity).
+template <typename TS>
+__global__ __launch_bounds__(256) void k_thomas(Grid g, TS *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
                                                 const double *__restrict__ a, const double *__restrict__ b,
                                                 const double *__restrict__ cc, const double *__restrict__ xrt,
-                                                const double *__restrict__ yrt) {
+                                                const double *__restrict__ yrt, int pin_zero) {
   // lane u: part (bit 0), x-mode m = 16 mb + (bits 1-4), side (bit 5), and
   // the y-mode pair pr (u >> 6 = pr nmb + mb).  The y-modes pr and ny - pr
   // have the same eigenvalue (both fold to |ky|), hence the same pivots: the
@@ -448,7 +488,7 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
   const long plane = (long)g.ny * sp;                 // complex elements per z plane
   const long base = (long)my * sp + m;                // complex index in plane 0
   const long dbase = (long)pr * sp + m;               // the pair's row of d
-  double *S = spec + part;                            // element (k, base): S[2 * (base + k plane)]
+  TS *S = spec + part;                                // element (k, base): S[2 * (base + k plane)]
   const double xy = xrt[m] + yrt[my];
   // forward elimination, k = 1 .. K = nz-1 (z = 1/(b + xy - a dprev)).  The
   // last THOMAS_TAIL rows are kept in registers, not stored, and the back
@@ -470,9 +510,11 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
   // step of the sweep, is skipped from there on.
   bool conv = false;
   double z = 0.0;
-  double vn[TD];
+  // prefetched rows are held in the storage type and converted where they
+  // are used, so that a load in flight is not waited for by its conversion
+  TS vn[TD];
 #pragma unroll
-  for (int r = 0; r < TD; r++) vn[r] = 1 + r <= kf ? S[2 * (base + (long)r * plane)] : 0.0;
+  for (int r = 0; r < TD; r++) vn[r] = 1 + r <= kf ? S[2 * (base + (long)r * plane)] : (TS)0;
   for (int k0 = 1; k0 <= kf; k0 += TD) {
     double v[TD];
 #pragma unroll
@@ -481,7 +523,7 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
 #pragma unroll
       for (int r = 0; r < TD; r++) {
         const int k = k0 + TD + r;
-        vn[r] = k <= kf ? S[2 * (base + (long)(k - 1) * plane)] : 0.0;
+        vn[r] = k <= kf ? S[2 * (base + (long)(k - 1) * plane)] : (TS)0;
       }
     }
 #pragma unroll
@@ -500,7 +542,7 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
         // the back substitution of this wavefront
         if (side == 0) *dlast = dprev;
         const double w = k == 1 ? v[r] * z : (v[r] - a[k] * pp) * z;
-        S[2 * q] = w;
+        S[2 * q] = (TS)w;
         pp = w;
       }
     }
@@ -510,7 +552,7 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
   if (tail) {
 #pragma unroll
     for (int bb = 0; bb < THOMAS_TAIL / TD; bb++) {
-      double v[TD];
+      TS v[TD];
 #pragma unroll
       for (int r = 0; r < TD; r++) v[r] = S[2 * (base + (long)(kf + bb * TD + r) * plane)];
 #pragma unroll
@@ -534,8 +576,8 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
     const long q = base + (long)(nz - 1) * plane;
     const double z = (b[nz] + xy) - a[nz] * dprev;
     const double v = S[2 * q];
-    pp = z != 0.0 ? (v - a[nz] * pp) / z : 0.0;
-    S[2 * q] = pp;
+    pp = z != 0.0 && !(pin_zero && m == 0 && my == 0) ? (v - a[nz] * pp) / z : 0.0;
+    S[2 * q] = (TS)pp;
   }
   // back substitution, k = nz-1 .. 1; side 1 reads the d that side 0 of this
   // wavefront stored
@@ -553,17 +595,18 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
       for (int r = 0; r < TD; r++) {
         const int k = kf + 1 + bb * TD + TD - 1 - r;
         const double w = wr[bb * TD + TD - 1 - r] - d[r] * pp;
-        S[2 * (base + (long)(k - 1) * plane)] = w;
+        S[2 * (base + (long)(k - 1) * plane)] = (TS)w;
         pp = w;
       }
     }
   }
-  double vb[TD], db[TD];
+  TS vb[TD];
+  double db[TD];
 #pragma unroll
   for (int r = 0; r < TD; r++) {
     const int k = kf - r;
     const long q = base + (long)(k - 1) * plane;
-    vb[r] = k >= 1 ? S[2 * q] : 0.0;
+    vb[r] = k >= 1 ? S[2 * q] : (TS)0;
     const double *qd = dtri + dbase + (long)(k - 1) * plane;
     db[r] = k >= 1 ? *(qd < dlast ? qd : dlast) : 0.0;
   }
@@ -576,7 +619,7 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
       for (int r = 0; r < TD; r++) {
         const int k = k0 - TD - r;
         const long q = base + (long)(k - 1) * plane;
-        vb[r] = k >= 1 ? S[2 * q] : 0.0;
+        vb[r] = k >= 1 ? S[2 * q] : (TS)0;
         const double *qd = dtri + dbase + (long)(k - 1) * plane;
         db[r] = k >= 1 ? *(qd < dlast ? qd : dlast) : 0.0;
       }
@@ -587,7 +630,7 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, double *__restrict__ spe
       if (k >= 1) {
         const long q = base + (long)(k - 1) * plane;
         const double w = v[r] - d[r] * pp;
-        S[2 * q] = w;
+        S[2 * q] = (TS)w;
         pp = w;
       }
     }
@@ -610,30 +653,36 @@ bool specialised(int n) {
   return false;
 }
 
+template <typename T> struct ElemOf;
+template <> struct ElemOf<double2> { using type = double; };
+template <> struct ElemOf<float2> { using type = float; };
+
+// Launch arguments of the specialised x passes; TS the spectrum storage.
+template <typename TS>
 struct XArgs {
-  Grid g; double *p; double2 *spec; long sp; const double2 *tw; int fused;
-  const double *du, *dv, *dw, *dtp; double rkab; double scale;
+  Grid g; XSrc src; TS *spec; long sp; const double2 *tw; double scale;
 };
 
-template <int N>
-void launch_x_fwd(const XArgs &a, hipStream_t s) {
+template <int N, typename TS, int SRC>
+void launch_x_fwd(const XArgs<TS> &a, hipStream_t s) {
   constexpr int B = xbatch_of<N>();
   const long pairs = (long)(a.g.ny / 2) * a.g.nz;
-  hipLaunchKernelGGL(k_fft_x_fwd_t<N>, dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g, a.p, a.spec,
-                     a.sp, a.tw, a.fused, a.du, a.dv, a.dw, a.dtp, a.rkab);
+  hipLaunchKernelGGL((k_fft_x_fwd_t<N, TS, SRC>), dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g,
+                     a.src, a.spec, a.sp, a.tw);
 }
-template <int N>
-void launch_x_inv(const XArgs &a, hipStream_t s) {
+template <int N, typename TS, typename TP>
+void launch_x_inv(const XArgs<TS> &a, TP *p, hipStream_t s) {
   constexpr int B = xbatch_of<N>();
   const long pairs = (long)(a.g.ny / 2) * a.g.nz;
-  hipLaunchKernelGGL(k_fft_x_inv_t<N>, dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g, a.p, a.spec,
-                     a.sp, a.tw, a.scale, X_INV_TOP_FIRST);
+  hipLaunchKernelGGL((k_fft_x_inv_t<N, TS, TP>), dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g, p,
+                     a.spec, a.sp, a.tw, a.scale, X_INV_TOP_FIRST);
 }
-template <int N>
-void launch_y(const Grid &g, double2 *spec, long sp, int nmx, const double2 *tw, int sign, int top_first, hipStream_t s) {
+template <int N, typename TS>
+void launch_y(const Grid &g, TS *spec, long sp, int nmx, const double2 *tw, int sign, int top_first, hipStream_t s) {
   constexpr int B = batch_of<N>();
   const int tiles = (nmx + B - 1) / B;
-  hipLaunchKernelGGL(k_fft_y_t<N>, dim3(g.nz * tiles), dim3(FFT_THREADS), 0, s, g, spec, sp, nmx, tw, sign, top_first);
+  hipLaunchKernelGGL((k_fft_y_t<N, TS>), dim3(g.nz * tiles), dim3(FFT_THREADS), 0, s, g, spec, sp, nmx, tw, sign,
+                     top_first);
 }
 
 #define IPG_FFT_DISPATCH(n, CALL) \
@@ -663,6 +712,10 @@ int ipg_poisson_setup(ipg_ctx *c) {
   }
   if (g.nx > LDS_COMPLEX || g.ny > LDS_COMPLEX || (g.ny % 2) != 0 || g.nz < 3) {
     ipg_set_error("solver2d: unsupported grid (need itot, jtot <= 2048, jtot even, ktot >= 3)");
+    return 1;
+  }
+  if (c->mixed && !(specialised(g.nx) && specialised(g.ny))) {
+    ipg_set_error("solver2d: the mixed path needs itot and jtot in {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512}");
     return 1;
   }
   P.nradx = rx.n; P.nrady = ry.n;
@@ -750,58 +803,129 @@ int ipg_poisson_free(ipg_ctx *c) {
 
 // fused_rkiter > 0: the x forward pass computes the right-hand side of
 // fillps (src/fillps.f90) itself from dudt/dvdt/dwdt instead of reading p.
+namespace {
+
+XSrc xsrc_of(ipg_ctx *c, int fused_rkiter) {
+  XSrc x{};
+  x.p = c->f[IPG_P];
+  x.du = c->f[IPG_DUDT]; x.dv = c->f[IPG_DVDT]; x.dw = c->f[IPG_DWDT]; x.dtp = c->dscal;
+  x.rkab = fused_rkiter > 0 ? c->pc.rkab[fused_rkiter - 1] : 1.0;
+  return x;
+}
+
+// The y pass, Thomas, inverse y pass and inverse x pass of one solve, from
+// the spectrum the forward x pass left; p out as TP.
+template <typename TS, typename TP>
+int solve_rest(ipg_ctx *c, const XArgs<TS> &xa, TP *pout, int pin_zero) {
+  const Grid &g = c->g;
+  Poisson &P = c->ps2;
+  TS *spec = xa.spec;
+#define CALLY(n) launch_y<n, TS>(g, spec, P.sp, P.nmx, P.twy, -1, Y_FWD_TOP_FIRST, c->stream)
+  IPG_FFT_DISPATCH(g.ny, CALLY)
+#undef CALLY
+  ipg_mark(c, "fft_y_fwd");
+  // lanes: 64 per (block of 16 x-modes, y-mode pair pr = 0..ny/2)
+  const long nthomas = 64L * ((P.nmx + 15) / 16) * (g.ny / 2 + 1);
+  using TE = typename ElemOf<TS>::type;
+  hipLaunchKernelGGL(k_thomas<TE>, dim3((unsigned)((nthomas + 255) / 256)), dim3(256), 0, c->stream, g, (TE *)spec,
+                     P.dtri, P.sp, P.nmx, P.a, P.b, P.c, P.xrt, P.yrt, pin_zero);
+  ipg_mark(c, "thomas");
+#define CALLY(n) launch_y<n, TS>(g, spec, P.sp, P.nmx, P.twy, +1, 0, c->stream)
+  IPG_FFT_DISPATCH(g.ny, CALLY)
+#undef CALLY
+  ipg_mark(c, "fft_y_inv");
+#define CALLX(n) launch_x_inv<n, TS, TP>(xa, pout, c->stream)
+  IPG_FFT_DISPATCH(g.nx, CALLX)
+#undef CALLX
+  ipg_mark(c, "fft_x_inv");
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+// Mixed path: one solve with the spectra stored FP32 between the passes
+// (every pass computes in FP64), then nref refinements: the FP64 residual of
+// the pieces so far, rhs - L p, solved the same way.  The right-hand side is
+// kept FP64 in the P field (the fused forward pass stores it as it goes), and
+// p is left as the sum of its FP32 pieces, which the projection reads.
+int poisson_mixed(ipg_ctx *c, int fused_rkiter) {
+  const Grid &g = c->g;
+  Poisson &P = c->ps2;
+  XArgs<float2> xa{g, xsrc_of(c, fused_rkiter), (float2 *)P.spec, P.sp, P.twx, 1.0 / ((double)g.nx * (double)g.ny)};
+  if (fused_rkiter > 0) {
+    xa.src.rhs_out = c->f[IPG_P];
+#define CALLX(n) launch_x_fwd<n, float2, XSRC_DIV>(xa, c->stream)
+    IPG_FFT_DISPATCH(g.nx, CALLX)
+#undef CALLX
+  } else {
+#define CALLX(n) launch_x_fwd<n, float2, XSRC_P>(xa, c->stream)
+    IPG_FFT_DISPATCH(g.nx, CALLX)
+#undef CALLX
+  }
+  ipg_mark(c, "fft_x_fwd");
+  if (solve_rest<float2, float>(c, xa, c->pp[0], 1)) return 1;
+  for (int r = 1; r <= c->nref; r++) {
+    // the FP64 residual of the pieces so far, rounded to FP32 (it is about a
+    // millionth of the right-hand side), then solved like the right-hand side
+    if (ipg_launch_resid(c, r)) return 1;
+    XArgs<float2> xr = xa;
+    xr.src.rhs_out = nullptr;
+    xr.src.r32 = c->pp[IPG_MAX_REFINE + 1];
+#define CALLX(n) launch_x_fwd<n, float2, XSRC_R32>(xr, c->stream)
+    IPG_FFT_DISPATCH(g.nx, CALLX)
+#undef CALLX
+    ipg_mark(c, "resid_x_fwd");
+    if (solve_rest<float2, float>(c, xr, c->pp[r], 1)) return 1;
+  }
+  c->p_in_pieces = true;
+  return 0;
+}
+
+}  // namespace
+
+// fused_rkiter > 0: the x forward pass computes the right-hand side of
+// fillps (src/fillps.f90) itself from dudt/dvdt/dwdt instead of reading p.
 int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter) {
+  if (c->mixed) return poisson_mixed(c, fused_rkiter);
   const Grid &g = c->g;
   Poisson &P = c->ps2;
   Radices rx, ry;
   rx.n = P.nradx; ry.n = P.nrady;
   for (int q = 0; q < rx.n; q++) rx.r[q] = P.radx[q];
   for (int q = 0; q < ry.n; q++) ry.r[q] = P.rady[q];
-  XArgs xa{g, c->f[IPG_P], P.spec, P.sp, P.twx, fused_rkiter > 0 ? 1 : 0, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT],
-           c->dscal, fused_rkiter > 0 ? c->pc.rkab[fused_rkiter - 1] : 1.0, 1.0 / ((double)g.nx * (double)g.ny)};
+  XArgs<double2> xa{g, xsrc_of(c, fused_rkiter), P.spec, P.sp, P.twx, 1.0 / ((double)g.nx * (double)g.ny)};
+  if (specialised(g.nx) && specialised(g.ny)) {
+    if (fused_rkiter > 0) {
+#define CALLX(n) launch_x_fwd<n, double2, XSRC_DIV>(xa, c->stream)
+      IPG_FFT_DISPATCH(g.nx, CALLX)
+#undef CALLX
+    } else {
+#define CALLX(n) launch_x_fwd<n, double2, XSRC_P>(xa, c->stream)
+      IPG_FFT_DISPATCH(g.nx, CALLX)
+#undef CALLX
+    }
+    ipg_mark(c, "fft_x_fwd");
+    return solve_rest<double2, double>(c, xa, c->f[IPG_P], 0);
+  }
+  // a length without a compile-time plan: the runtime-plan kernels
   const long pairs = (long)(g.ny / 2) * g.nz;
   const int RPx = LDS_COMPLEX / g.nx;
   const int gx = (int)((pairs + RPx - 1) / RPx);
   const int TM = LDS_COMPLEX / g.ny;
   const int gy = g.nz * ((P.nmx + TM - 1) / TM);
-  if (specialised(g.nx)) {
-#define CALLX(n) launch_x_fwd<n>(xa, c->stream)
-    IPG_FFT_DISPATCH(g.nx, CALLX)
-#undef CALLX
-  } else {
-    hipLaunchKernelGGL(k_fft_x_fwd, dim3(gx), dim3(FFT_THREADS), 0, c->stream, g, c->f[IPG_P], P.spec, P.sp, rx, P.twx,
-                       xa.fused, xa.du, xa.dv, xa.dw, xa.dtp, xa.rkab);
-  }
+  const int fused = fused_rkiter > 0 ? 1 : 0;
+  hipLaunchKernelGGL(k_fft_x_fwd, dim3(gx), dim3(FFT_THREADS), 0, c->stream, g, c->f[IPG_P], P.spec, P.sp, rx, P.twx,
+                     fused, xa.src.du, xa.src.dv, xa.src.dw, xa.src.dtp, xa.src.rkab);
   ipg_mark(c, "fft_x_fwd");
-  if (specialised(g.ny)) {
-#define CALLY(n) launch_y<n>(g, P.spec, P.sp, P.nmx, P.twy, -1, Y_FWD_TOP_FIRST, c->stream)
-    IPG_FFT_DISPATCH(g.ny, CALLY)
-#undef CALLY
-  } else {
-    hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, -1);
-  }
+  hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, -1);
   ipg_mark(c, "fft_y_fwd");
-  // lanes: 64 per (block of 16 x-modes, y-mode pair pr = 0..ny/2)
   const long nthomas = 64L * ((P.nmx + 15) / 16) * (g.ny / 2 + 1);
-  hipLaunchKernelGGL(k_thomas, dim3((unsigned)((nthomas + 255) / 256)), dim3(256), 0, c->stream, g, (double *)P.spec, P.dtri, P.sp, P.nmx,
-                     P.a, P.b, P.c, P.xrt, P.yrt);
+  hipLaunchKernelGGL(k_thomas<double>, dim3((unsigned)((nthomas + 255) / 256)), dim3(256), 0, c->stream, g,
+                     (double *)P.spec, P.dtri, P.sp, P.nmx, P.a, P.b, P.c, P.xrt, P.yrt, 0);
   ipg_mark(c, "thomas");
-  if (specialised(g.ny)) {
-#define CALLY(n) launch_y<n>(g, P.spec, P.sp, P.nmx, P.twy, +1, 0, c->stream)
-    IPG_FFT_DISPATCH(g.ny, CALLY)
-#undef CALLY
-  } else {
-    hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, +1);
-  }
+  hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, +1);
   ipg_mark(c, "fft_y_inv");
-  if (specialised(g.nx)) {
-#define CALLX(n) launch_x_inv<n>(xa, c->stream)
-    IPG_FFT_DISPATCH(g.nx, CALLX)
-#undef CALLX
-  } else {
-    hipLaunchKernelGGL(k_fft_x_inv, dim3(gx), dim3(FFT_THREADS), 0, c->stream, g, c->f[IPG_P], P.spec, P.sp, rx, P.twx,
-                       xa.scale);
-  }
+  hipLaunchKernelGGL(k_fft_x_inv, dim3(gx), dim3(FFT_THREADS), 0, c->stream, g, c->f[IPG_P], P.spec, P.sp, rx, P.twx,
+                     xa.scale);
   ipg_mark(c, "fft_x_inv");
   IPG_HIP(hipGetLastError());
   return 0;

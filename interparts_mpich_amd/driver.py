@@ -22,8 +22,11 @@ def _ptr(a):
 
 
 class Simulation:
-    def __init__(self, case: Case, use_graphs=False, side_stream=True):
+    def __init__(self, case: Case, use_graphs=False, side_stream=True, precision=0, nref=None):
+        """precision 0: the FP64 path; 1: mixed precision (ipg_params.precision),
+        with nref FP64-residual refinement sweeps of the Poisson solve."""
         self.case = case
+        self.precision = precision
         self.lib = load()
         prm = _lib.IpgParams(itot=case.itot, jtot=case.jtot, ktot=case.ktot, np=case.np,
                              nl=case.nl if case.np > 0 else 0, lx=case.lx, ly=case.ly, lz=case.lz,
@@ -31,11 +34,13 @@ class Simulation:
                              ratiorho=case.ratiorho, particle_rotation=int(case.particle_rotation),
                              bc_uvw_top=case.bc_uvw_top, bc_uvw_bot=case.bc_uvw_bot, bc_c_top=case.bc_c_top,
                              bc_c_bot=case.bc_c_bot, bc_c_top_val=case.bc_c_top_val,
-                             bc_c_bot_val=case.bc_c_bot_val, precision=0)
+                             bc_c_bot_val=case.bc_c_bot_val, precision=precision)
         self.ctx = C.c_void_p()
         check(self.lib.ipg_init(C.byref(prm), C.byref(self.ctx)), "ipg_init")
         if use_graphs:
             check(self.lib.ipg_use_graphs(self.ctx, 1), "ipg_use_graphs")
+        if nref is not None:
+            check(self.lib.ipg_set_refinement(self.ctx, nref), "ipg_set_refinement")
         if not side_stream:
             check(self.lib.ipg_use_side_stream(self.ctx, 0), "ipg_use_side_stream")
         self.shape = (case.ktot + 2, case.jtot + 2, case.itot + 2)
