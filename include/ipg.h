@@ -136,6 +136,16 @@ int ipg_intgr_nwtn_eulr(ipg_ctx *ctx, int rkiter);    /* src/intgr_nwtn_eulr.f90
 int ipg_substep(ipg_ctx *ctx, int rkiter);
 int ipg_chkdt(ipg_ctx *ctx, double *dtmax);           /* src/chkdt.f90:6 */
 int ipg_chkdiv(ipg_ctx *ctx, double *divtot, double *divmax); /* src/chkdiv.f90:7 */
+/* ipg_chkdiv's check (src/chkdiv.f90:23-39) on caller-owned device arrays u,
+ * v, w in the reference's real(8) (0:i1,0:j1,0:k1) layout, i fastest, for a
+ * grid of at most 512^3 cells (itot*jtot*ktot <= 134217728; larger grids
+ * return 2).  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc):
+ * it allocates nothing, its partial sums living in a fixed 256 KiB array on
+ * each device.  It runs on the null stream and returns when done; calls that
+ * overlap on one GPU share that array and race. */
+int ipg_chkdiv_noalloc(const double *u, const double *v, const double *w, int itot, int jtot, int ktot, double lx,
+                       double ly, double lz, double *divtot, double *divmax);
 
 /* ---- output ---------------------------------------------------------- */
 /* post3d quantities (src/output.f90:997-1030, vorticity src/post.f90:15-61) at

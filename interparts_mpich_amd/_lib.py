@@ -25,7 +25,8 @@ EXPORTS = ["ipg_init", "ipg_finalize", "ipg_last_error", "ipg_nl_formula", "ipg_
            "ipg_initparticles", "ipg_coordsfp", "ipg_rk", "ipg_ibm", "ipg_ibm_begin", "ipg_eulr2lagr",
            "ipg_lagrforces", "ipg_lagr2eulr", "ipg_uniform", "ipg_bounduvw", "ipg_boundp",
            "ipg_boundscal", "ipg_fillps", "ipg_solver2d", "ipg_correc", "ipg_pnew_update",
-           "ipg_intgr_over_sphere", "ipg_intgr_nwtn_eulr", "ipg_substep", "ipg_chkdt", "ipg_chkdiv", "ipg_post3d",
+           "ipg_intgr_over_sphere", "ipg_intgr_nwtn_eulr", "ipg_substep", "ipg_chkdt", "ipg_chkdiv", "ipg_chkdiv_noalloc",
+           "ipg_post3d",
            "ipg_sync", "ipg_use_graphs", "ipg_use_side_stream", "ipg_set_refinement", "ipg_autotune", "ipg_substep_bytes",
            "ipg_substep_phases"]
 
@@ -89,6 +90,7 @@ def load():
         "ipg_substep": ([vp, ip], ip),
         "ipg_chkdt": ([vp, dp], ip),
         "ipg_chkdiv": ([vp, dp, dp], ip),
+        "ipg_chkdiv_noalloc": ([vp, vp, vp, ip, ip, ip, d, d, d, dp, dp], ip),
         "ipg_post3d": ([vp, ip, vp], ip),
         "ipg_sync": ([vp], ip),
         "ipg_use_graphs": ([vp, ip], ip),

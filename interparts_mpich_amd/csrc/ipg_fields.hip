@@ -4,6 +4,7 @@
 // FP64, with the reference's operation order kept term by term (the library
 // is compiled with -ffp-contract=off so no multiply-add is fused).
 #include "ipg_internal.h"
+#include <mutex>
 
 namespace {
 
@@ -691,5 +692,91 @@ int ipg_launch_post3d(ipg_ctx *c, int q, double *host) {
   IPG_HIP(hipGetLastError());
   IPG_HIP(hipMemcpyAsync(host, out, (size_t)g.nx * g.ny * g.nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   IPG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ---- divergence check without allocation ------------------------------------
+// ipg_chkdiv_noalloc: src/chkdiv.f90:23-39 (k_chkdiv, as ipg_chkdiv launches
+// it) on caller-owned device arrays.  Its per-block partial sums go to a
+// file-scope device array rather than to a context's scratch, so a call
+// allocates nothing.  Each GPU holds its own copy of the array, at its own
+// address, found once per device with hipGetSymbolAddress; host code uses that
+// address only.  Every partial the final reduction reads is written by the
+// same call, so nothing needs clearing; calls that overlap on one GPU share
+// the array and race.
+namespace {
+
+constexpr int NOALLOC_BLOCKS = 16384;            // per-block partials the array holds
+constexpr long NOALLOC_CELLS_PER_BLOCK = 8192;   // 256 threads, at most 32 cells each
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+// [2 b], [2 b + 1]: block b's total and maximum; the last pair: the result
+static __device__ __align__(256) double scratch[2 * NOALLOC_BLOCKS + 2];
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+// one block: thread t reduces partials t, t + 256, ... in order, then a tree
+__global__ __launch_bounds__(256) void k_chkdiv_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
+  __shared__ double s1[256], s2[256];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < nb; q += 256) { a += part[2 * q]; b = fmax(b, part[2 * q + 1]); }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s1[threadIdx.x] += s1[threadIdx.x + h];
+      s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = s1[0]; out[1] = s2[0]; }
+}
+
+}  // namespace
+
+extern "C" int ipg_chkdiv_noalloc(const double *u, const double *v, const double *w, int itot, int jtot, int ktot, double lx,
+                                  double ly, double lz, double *divtot, double *divmax) {
+  if (!u || !v || !w || !divtot || !divmax) { ipg_set_error("chkdiv_noalloc: null argument"); return 1; }
+  if (itot < 1 || jtot < 1 || ktot < 1 || !(lx > 0.0) || !(ly > 0.0) || !(lz > 0.0)) {
+    ipg_set_error("chkdiv_noalloc: bad grid");
+    return 1;
+  }
+  const long cells = (long)itot * jtot * ktot;
+  const long nb = (cells + NOALLOC_CELLS_PER_BLOCK - 1) / NOALLOC_CELLS_PER_BLOCK;
+  if (nb > NOALLOC_BLOCKS) {
+    ipg_set_error("chkdiv_noalloc: " + std::to_string(cells) + " cells exceed its maximum of " +
+                  std::to_string((long)NOALLOC_BLOCKS * NOALLOC_CELLS_PER_BLOCK) + " (512^3)");
+    return 2;
+  }
+  int dev = 0;
+  IPG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) { ipg_set_error("chkdiv_noalloc: device index out of range"); return 1; }
+  double *part;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void *p = nullptr;
+      IPG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      g_scratch_ptr[dev] = (double *)p;
+    }
+    part = g_scratch_ptr[dev];
+  }
+  // the reference's (0:i1,0:j1,0:k1) layout, i fastest: Grid indexing with
+  // the row pitch itot + 2 from element (1, 0, 0)
+  Grid g{};
+  g.nx = itot; g.ny = jtot; g.nz = ktot;
+  g.sx = (long)itot + 2;
+  g.sxy = g.sx * ((long)jtot + 2);
+  g.dxi = itot / lx; g.dyi = jtot / ly; g.dzi = ktot / lz;
+  hipLaunchKernelGGL(k_chkdiv, dim3((unsigned)nb), dim3(256), 0, 0, g, u + 1, v + 1, w + 1, part);
+  IPG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_chkdiv_final, dim3(1), dim3(256), 0, 0, part, (int)nb, part + 2 * NOALLOC_BLOCKS);
+  IPG_HIP(hipGetLastError());
+  double r[2];
+  IPG_HIP(hipMemcpy(r, part + 2 * NOALLOC_BLOCKS, sizeof(r), hipMemcpyDeviceToHost));
+  *divtot = r[0];
+  *divmax = r[1];
   return 0;
 }
