@@ -20,13 +20,21 @@
  * reference's own kernel.f90 compiled into oracle/_ref; the real FFT vs
  * numpy.fft and the reference's VFFTPACK (fft.f) compiled into oracle/_ref;
  * stencils vs closed forms on polynomial / trigonometric fields; Poisson vs a
- * dense least-squares solve on tiny grids; IBM interp/spread adjointness and
- * partition of unity; sphere integrals vs 4/3 pi R^3 and rigid fields;
- * Newton-Euler vs the free-fall closed form; collisions vs the closed-form
- * spring-dashpot force; chkdt vs the dtmax known answer of src/terminate.txt.
- * Not pinned by any reference output (parity unpinned beyond those checks):
- * the full substep sequencing of src/main.f90:268-414 and the 9x contact
- * multiplicity of src/intgr_nwtn_eulr.f90:358-499 at one rank.
+ * dense least-squares solve on tiny grids; the substep sequence vs the RK3
+ * polynomial on a z-eigenmode and an exactly cancelling pressure-only
+ * substep; ghost rules as the boundary conditions they state; IBM stencils
+ * vs exact interpolation of linear fields on each staggered grid, spreading
+ * vs force and first-moment conservation (with the top-wall k1 plane
+ * counted), interp/spread adjointness; marker positions, rigid-body
+ * velocities and pass-0 forces vs closed forms; sphere integrals vs
+ * 4/3 pi R^3 and rigid fields; Newton-Euler vs the free-fall closed form;
+ * collisions vs the closed-form spring-dashpot force; chkdt vs the dtmax
+ * known answer of src/terminate.txt.  tests/test_oracle_mutations.py plants
+ * plausible slips in copies of this file and checks that a pin fails.
+ * Restated from the reference's code with no output or closed form to pin
+ * them (parity unpinned beyond the reading): the three IBM passes and the
+ * dudtf copies of src/main.f90:286-338, and the 9x contact multiplicity of
+ * src/intgr_nwtn_eulr.f90:358-499 at one rank.
  */
 #include <math.h>
 #include <stdint.h>

@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(_HERE, "liboracle.so")
+# IPG_ORACLE_LIB: another build of the oracle (tests/test_oracle_mutations.py
+# plants mistakes in copies of it and checks that the pins fail)
+LIB = os.environ.get("IPG_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 REF = os.path.join(_HERE, "_ref", "libref.so")
 
 FIELDS = ["U", "V", "W", "PNEW", "C", "DUDT", "DVDT", "DWDT", "DCDT", "DURK", "DVRK", "DWRK", "DCRK", "P",
