@@ -3,6 +3,7 @@
 // the substep sequence of src/main.f90:268-414, optionally captured as one
 // hipGraph per RK stage.
 #include "ipg_internal.h"
+#include "ipg_ibm_geom.h"
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -104,12 +105,10 @@ int release(ipg_ctx *c) {
   if (c->side) (void)hipStreamDestroy(c->side);
   for (int q = 0; q < IPG_NMARK; q++)
     if (c->ev[q]) { (void)hipEventDestroy(c->ev[q]); c->ev[q] = nullptr; }
-  if (c->stamps) (void)hipFree(c->stamps);
   ipg_poisson_free(c);
-  void *ptrs[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->nml, c->nmc, c->ibm_err, c->ibm_flg, c->op, c->motion2, c->red, c->dscal,
-                  c->ps.nbf, c->K1[0], c->K1[1], c->K1[2], c->pbuf};
-  for (void *q : ptrs) if (q) (void)hipFree(q);
-  for (int q = 0; q < IPG_NPART; q++) if (c->ps.q[q]) (void)hipFree(c->ps.q[q]);
+  void *alloc[IPG_MAX_ALLOCS];
+  const int nalloc = ipg_ctx_allocations(c, alloc);
+  for (int q = 0; q < nalloc; q++) (void)hipFree(alloc[q]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   return 0;
 }
@@ -322,9 +321,9 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
     if (hipMalloc(&c->mk, 3 * (size_t)prm->nl * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->fl, 3 * M * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->fl2, 3 * M * sizeof(double)) != hipSuccess ||
-        hipMalloc(&c->nbl, (size_t)np * 64 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->nbl, (size_t)np * IBM_MAXNB * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->nbc, (size_t)np * sizeof(int)) != hipSuccess ||
-        hipMalloc(&c->nml, (size_t)np * 3 * 4096 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->nml, (size_t)np * 3 * IBM_MAXNM * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->nmc, (size_t)np * 3 * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->ibm_err, sizeof(int)) != hipSuccess ||
         hipMalloc(&c->op, 24 * (size_t)np * sizeof(double)) != hipSuccess ||

@@ -26,11 +26,6 @@
 
 namespace {
 
-constexpr int IBM_THREADS = 512;  // threads per (sphere, component) workgroup
-constexpr int IBM_MPT = 2;        // markers per thread: nl <= 1024
-constexpr int IBM_MAXNB = 64;     // neighbour images per sphere
-constexpr int IBM_MAXNM = 4096;   // neighbour markers touching one (sphere, component) box
-
 
 struct IbmArgs {
   Grid g;
@@ -52,29 +47,6 @@ struct IbmArgs {
   const double *dtp;
 };
 
-// offset in a k1 plane (nx*ny) of the unwrapped stencil cell (i, j).  A
-// marker lies within radfp of its sphere's centre, which is inside the
-// domain, and a box is narrower than the domain, so i is in [1-nx, 2nx] and
-// one conditional shift per direction wraps it.
-__device__ __forceinline__ long kplane(const Grid &g, int i, int j) {
-  int r = i - 1, s = j - 1;
-  r += r < 0 ? g.nx : 0;
-  r -= r >= g.nx ? g.nx : 0;
-  s += s < 0 ? g.ny : 0;
-  s -= s >= g.ny ? g.ny : 0;
-  return (long)r + (long)g.nx * s;
-}
-
-// periodic wrap of any unwrapped index onto 1..n
-__device__ __forceinline__ int wrapany(int i, int n) {
-  int r = i - 1;
-  if ((unsigned)r >= (unsigned)n) {
-    r %= n;
-    if (r < 0) r += n;
-  }
-  return r + 1;
-}
-
 // One IBM pass for one (sphere, velocity component) workgroup of IBM_THREADS
 // threads; thread t owns markers t, t + IBM_THREADS, ... (at most IBM_MPT).
 // Each marker's position (src/coordsfp.f90:49-68), stencil base and nine
@@ -92,9 +64,6 @@ __device__ __forceinline__ int wrapany(int i, int n) {
 // because its spread goes straight into dudt, which other spheres may still
 // be gathering from.
 
-// a box [a0, a0+an) and a box [b0 + s n, b0 + s n + bn) overlap
-__device__ __forceinline__ bool overlap(int a0, int an, int b0, int bn) { return a0 < b0 + bn && b0 < a0 + an; }
-
 // Neighbour images of each sphere, once per substep: every (q, sx, sy) with
 // q's box, shifted by (sx nx, sy ny), overlapping p's box; q != p (a box
 // never meets its own image: it is narrower than the domain).  One block per
@@ -110,15 +79,9 @@ __global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int
   for (int q = threadIdx.x; q < A.np; q += blockDim.x) {
     if (q == p) continue;
     const Box bq = reach_box(A.g, A.radfp, A.px[q], A.py[q], A.pz[q]);
-    if (!overlap(bp.bk0, bp.bz, bq.bk0, bq.bz)) continue;
-    unsigned mx = 0u, my = 0u;
-#pragma unroll
-    for (int sh = -2; sh <= 2; sh++) {
-      if (overlap(bp.bi0, bp.bx, bq.bi0 + sh * g.nx, bq.bx)) mx |= 1u << (sh + 2);
-      if (overlap(bp.bj0, bp.by, bq.bj0 + sh * g.ny, bq.by)) my |= 1u << (sh + 2);
-    }
+    unsigned mx, my;
+    if (!image_masks(g, bp, bq, mx, my)) continue;
     const int n = __popc(mx) * __popc(my);
-    if (n == 0) continue;
     int slot = atomicAdd(&count, n);
     for (int sx = 0; sx < 5; sx++)
       for (int sy = 0; sy < 5; sy++)
@@ -139,21 +102,8 @@ __global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int
 template <bool WEIGHTS = true, typename T = double>
 __device__ __forceinline__ bool neighbour_stencil(const IbmArgs &A, const Box &b, int comp, int qs, int l, int sxs, int sys,
                                                   CompStencilT<T> &c) {
-  const Grid &g = A.g;
-  const double xq = A.px[qs], yq = A.py[qs], zq = A.pz[qs];
-  // a stencil spans less than two cells either side of its marker
-  const double xs = (xq + A.mk[l]) * g.dxi + sxs * g.nx, ys = (yq + A.mk[A.nl + l]) * g.dyi + sys * g.ny;
-  const double zs = (zq + A.mk[2 * A.nl + l]) * g.dzi;
-  if (xs < b.bi0 - 3.0 || xs > b.bi0 + b.bx + 2.0 || ys < b.bj0 - 3.0 || ys > b.bj0 + b.by + 2.0 ||
-      zs < b.bk0 - 3.0 || zs > b.bk0 + b.bz + 2.0)
-    return false;
-  double xl, yl, zl;
-  if (!marker_pos(g, xq, yq, zq, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return false;
-  comp_stencil<WEIGHTS, T>(g, xl, yl, zl, comp, c);
-  c.i0 += sxs * g.nx;
-  c.j0 += sys * g.ny;
-  return !(c.i0 + 2 < b.bi0 || c.i0 >= b.bi0 + b.bx || c.j0 + 2 < b.bj0 || c.j0 >= b.bj0 + b.by ||
-           c.k0 + 2 < b.bk0 || c.k0 >= b.bk0 + b.bz);
+  return image_stencil<WEIGHTS, T>(A.g, b, comp, A.px[qs], A.py[qs], A.pz[qs], A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l],
+                                   sxs, sys, c);
 }
 
 // One IBM launch for one (sphere, velocity component) workgroup.
@@ -199,13 +149,13 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     b.bx = min(b.bx, A.bmx); b.by = min(b.by, A.bmy); b.bz = min(b.bz, A.bmz);
   }
   const int nbox = b.bx * b.by * b.bz, bxy = b.bx * b.by;
-  const int nflag = (nbox + 31) >> 5, nflagmax = (A.bmx * A.bmy * A.bmz + 31) >> 5;
-  unsigned *flag = (unsigned *)(box + A.bmx * A.bmy * A.bmz);
-  unsigned *taken = flag + nflagmax;  // MODE 2: cells a lower-numbered sphere owns
-  // field offset of a box cell = rowoff[row] + icol[li], row = lj + by lk:
-  // the periodic wrap and the index arithmetic once per row and column
-  int *rowoff = (int *)(taken + nflagmax);
-  int *icol = rowoff + A.bmy * A.bmz;
+  const IbmLds L = ibm_lds_layout(A.bmx, A.bmy, A.bmz, sizeof(T));
+  const int nflag = (nbox + 31) >> 5, nflagmax = L.nflagmax;
+  unsigned *flag = reinterpret_cast<unsigned *>(ibm_lds + L.flag);
+  unsigned *taken = reinterpret_cast<unsigned *>(ibm_lds + L.taken);  // MODE 2: cells a lower-numbered sphere owns
+  // field offset of a box cell = rowoff[row] + icol[li] (box_rowoff, box_icol)
+  int *rowoff = reinterpret_cast<int *>(ibm_lds + L.rowoff);
+  int *icol = reinterpret_cast<int *>(ibm_lds + L.icol);
   double *F = A.fld[comp];
   T *fl = (T *)A.fl + comp * M;
   // the previous pass's forces: a buffer no workgroup of this launch writes,
@@ -223,12 +173,10 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     double xl, yl, zl;
     if (l >= A.nl || !marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return false;
     comp_stencil<true, T>(g, xl, yl, zl, comp, c);
-    e0 = box_at(b, c.i0, c.j0, c.k0);
-    if (e0 < 0 || box_at(b, c.i0 + 2, c.j0 + 2, c.k0) < 0) {
+    if (!own_cells(b, c.i0, c.j0, c.k0, e0, nzin)) {
       if (MODE == 0) atomicOr(A.err, 4);
       return false;
     }
-    nzin = min(3, b.bk0 + b.bz - c.k0);
     double u;
     if (comp == 0) u = A.pu[p] + A.pomy[p] * (zl - zp) - A.pomz[p] * (yl - yp);
     else if (comp == 1) u = A.pv[p] + A.pomz[p] * (xl - xp) - A.pomx[p] * (zl - zp);
@@ -243,33 +191,23 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     flag[t] = MODE == 0 ? 0u : gflag[t];
     if (MODE == 2) taken[t] = 0u;
   }
-  for (int t = threadIdx.x; t < b.by * b.bz; t += blockDim.x) {
-    const int lk = t / b.by, lj = t - lk * b.by;
-    rowoff[t] = (int)(g.sx * wrapany(b.bj0 + lj, g.ny) + g.sxy * (b.bk0 + lk));
-  }
-  for (int t = threadIdx.x; t < b.bx; t += blockDim.x) icol[t] = wrapany(b.bi0 + t, g.nx) - 1;
+  for (int t = threadIdx.x; t < b.by * b.bz; t += blockDim.x) rowoff[t] = box_rowoff(g, b, t);
+  for (int t = threadIdx.x; t < b.bx; t += blockDim.x) icol[t] = box_icol(g, b, t);
   if (MODE >= 1)
     for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = (T)0;
   __shared__ int lcount;
   if (MODE == 0 && threadIdx.x == 0) lcount = 0;
   __syncthreads();
   auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
-  // cell t = li + bx row of the box walks by IBM_THREADS per step: (li, row)
-  // advance by (rb, qb) with a carry instead of a division per cell
-  const int qb = IBM_THREADS / b.bx, rb = IBM_THREADS - qb * b.bx;
-  const int row0 = (int)threadIdx.x / b.bx, li0 = (int)threadIdx.x - row0 * b.bx;
-  auto advance = [&](int &li, int &row) {
-    li += rb;
-    row += qb;
-    if (li >= b.bx) { li -= b.bx; row++; }
-  };
-  constexpr int CH = 8;
+  // the box cells of this thread: t = threadIdx.x + n IBM_THREADS
+  const BoxWalk w0(b.bx, IBM_THREADS, (int)threadIdx.x);
+  constexpr int CH = IBM_CH;
   if (MODE == 1 || (MODE == 2 && F64)) {
     // dudt on the cells the own stencils touch (flagged by pass 0), added
     // into the box together with the spreads below, so that its loads overlap
     // them: the box ends as dudt + S, the reference's dudtf, with the terms
     // summed in another order (as the LDS atomics already order S's terms)
-    int li = li0, row = row0;
+    BoxWalk w = w0;
     for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * IBM_THREADS) {
       double v[CH];
       unsigned fm = 0u;
@@ -278,10 +216,10 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
         const int t = t0 + u * IBM_THREADS;
         v[u] = 0.0;
         if (t < nbox && flagged(t)) {
-          v[u] = F[(long)rowoff[row] + icol[li]];
+          v[u] = F[(long)rowoff[w.row] + icol[w.li]];
           fm |= 1u << u;
         }
-        advance(li, row);
+        w.advance();
       }
 #pragma unroll
       for (int u = 0; u < CH; u++)
@@ -373,29 +311,29 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   }
   if (MODE == 2) {
     // dudt = dudtf = fl(dudt + S) (src/main.f90:332-334), once per cell
-    int li = li0, row = row0;
+    BoxWalk w = w0;
     for (int t = threadIdx.x; t < nbox; t += IBM_THREADS) {
       if (flagged(t) && !((taken[t >> 5] >> (t & 31)) & 1u)) {
-        const long x = (long)rowoff[row] + icol[li];
+        const long x = (long)rowoff[w.row] + icol[w.li];
         if (F64) F[x] = (double)box[t];
         else F[x] = F[x] + (double)box[t];
       }
-      advance(li, row);
+      w.advance();
     }
     return;
   }
   if (MODE == 0) {
     // stage dudt on the flagged cells, in chunks of 8 cells per thread so
     // that the loads of a chunk are all in flight together
-    int li = li0, row = row0;
+    BoxWalk w = w0;
     for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * IBM_THREADS) {
       double v[CH];
 #pragma unroll
       for (int u = 0; u < CH; u++) {
         const int t = t0 + u * IBM_THREADS;
         v[u] = 0.0;
-        if (t < nbox && flagged(t)) v[u] = F[(long)rowoff[row] + icol[li]];
-        advance(li, row);
+        if (t < nbox && flagged(t)) v[u] = F[(long)rowoff[w.row] + icol[w.li]];
+        w.advance();
       }
 #pragma unroll
       for (int u = 0; u < CH; u++) {
@@ -489,12 +427,8 @@ __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, doub
   }
 }
 
-// LDS of one k_ibm workgroup: the box, its two cell masks, the row offsets
-// and the wrapped column indices.
-size_t ibm_lds_bytes(int bmx, int bmy, int bmz, size_t esize) {
-  const size_t cells = (size_t)bmx * bmy * bmz;
-  return cells * esize + 2 * ((cells + 31) / 32) * sizeof(unsigned) + ((size_t)bmy * bmz + bmx) * sizeof(int);
-}
+// dynamic LDS of one k_ibm workgroup (ibm_lds_layout)
+size_t ibm_lds_bytes(int bmx, int bmy, int bmz, size_t esize) { return ibm_lds_layout(bmx, bmy, bmz, esize).bytes; }
 
 
 }  // namespace
@@ -506,7 +440,10 @@ int ipg_ibm_setup(ipg_ctx *c) {
   const int bmx = ibm_box_max(c->pc.radfp, c->g.dxi), bmy = ibm_box_max(c->pc.radfp, c->g.dyi);
   const int bmz = ibm_box_max(c->pc.radfp, c->g.dzi);
   const size_t lds = ibm_lds_bytes(bmx, bmy, bmz, c->ibm32 ? sizeof(float) : sizeof(double));
-  if (lds > 160 * 1024) { ipg_set_error("ibm: sphere box exceeds the 160 KiB LDS of a CU"); return 1; }
+  // with the kernels' own static LDS (pass 0's list counter)
+  hipFuncAttributes fa{};
+  IPG_HIP(hipFuncGetAttributes(&fa, c->ibm32 ? (const void *)k_ibm<0, float> : (const void *)k_ibm<0, double>));
+  if (lds + fa.sharedSizeBytes > 160 * 1024) { ipg_set_error("ibm: sphere box exceeds the 160 KiB LDS of a CU"); return 1; }
   // the box's row offsets are kept as 32-bit integers
   if ((double)c->g.sxy * (c->g.nz + 2) >= 2147483647.0) { ipg_set_error("ibm: field larger than 2^31 cells"); return 1; }
   // the reach box (one cell wider than a component's box) must not meet its

@@ -139,3 +139,133 @@ __host__ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, 
 // roundings only, which an exact integer 2 radfp hinv would carry over to the
 // next integer, hence floor(.. + 1e-9) + 1 rather than ceil.
 inline int ibm_box_max(double radfp, double hinv) { return (int)std::floor(2.0 * radfp * hinv + 1e-9) + 1 + 5; }
+
+// ---------------------------------------------------------------------------
+// Index arithmetic of the k_ibm workgroups (ipg_ibm.hip), host and device, so
+// that a host emulation of every thread checks each LDS and global index the
+// kernel forms.
+
+constexpr int IBM_THREADS = 512;  // threads per (sphere, component) workgroup
+constexpr int IBM_MPT = 2;        // markers per thread: nl <= IBM_THREADS * IBM_MPT
+constexpr int IBM_MAXNB = 64;     // neighbour images per sphere
+constexpr int IBM_MAXNM = 4096;   // neighbour markers touching one (sphere, component) box
+constexpr int IBM_CH = 8;         // cells per thread whose loads one staging chunk keeps in flight
+
+// periodic wrap of any unwrapped index onto 1..n
+__host__ __device__ __forceinline__ int wrapany(int i, int n) {
+  int r = i - 1;
+  if ((unsigned)r >= (unsigned)n) {
+    r %= n;
+    if (r < 0) r += n;
+  }
+  return r + 1;
+}
+
+// offset in a k1 plane (nx*ny) of the unwrapped stencil cell (i, j).  A
+// marker lies within radfp of its sphere's centre, which is inside the
+// domain, and a box is narrower than the domain, so i is in [1-nx, 2nx] and
+// one conditional shift per direction wraps it.
+__host__ __device__ __forceinline__ long kplane(const Grid &g, int i, int j) {
+  int r = i - 1, s = j - 1;
+  r += r < 0 ? g.nx : 0;
+  r -= r >= g.nx ? g.nx : 0;
+  s += s < 0 ? g.ny : 0;
+  s -= s >= g.ny ? g.ny : 0;
+  return (long)r + (long)g.nx * s;
+}
+
+// a box [a0, a0+an) and a box [b0 + s n, b0 + s n + bn) overlap
+__host__ __device__ __forceinline__ bool overlap(int a0, int an, int b0, int bn) { return a0 < b0 + bn && b0 < a0 + an; }
+
+// The periodic images (sx nx, sy ny), sx, sy in -2..2, of sphere q's reach box
+// bq that overlap sphere p's reach box bp: bit sx+2 of mx, bit sy+2 of my
+// (an image is a pair of bits); false when none does.
+__host__ __device__ __forceinline__ bool image_masks(const Grid &g, const Box &bp, const Box &bq, unsigned &mx, unsigned &my) {
+  mx = 0u;
+  my = 0u;
+  if (!overlap(bp.bk0, bp.bz, bq.bk0, bq.bz)) return false;
+#pragma unroll
+  for (int sh = -2; sh <= 2; sh++) {
+    if (overlap(bp.bi0, bp.bx, bq.bi0 + sh * g.nx, bq.bx)) mx |= 1u << (sh + 2);
+    if (overlap(bp.bj0, bp.by, bq.bj0 + sh * g.ny, bq.by)) my |= 1u << (sh + 2);
+  }
+  return mx != 0u && my != 0u;
+}
+
+// LDS of one k_ibm workgroup, byte offsets from the start of its dynamic LDS:
+// the box (bmx*bmy*bmz cells of esize bytes), two cell masks of one bit per
+// cell (the cells the own stencils touch, and the cells a lower-numbered
+// sphere owns), the field offsets of the box rows and the wrapped columns.
+struct IbmLds { size_t box, flag, taken, rowoff, icol, bytes; int nflagmax; };
+__host__ __device__ __forceinline__ IbmLds ibm_lds_layout(int bmx, int bmy, int bmz, size_t esize) {
+  IbmLds L;
+  const size_t cells = (size_t)bmx * bmy * bmz;
+  L.nflagmax = (int)((cells + 31) / 32);
+  L.box = 0;
+  L.flag = cells * esize;
+  L.taken = L.flag + (size_t)L.nflagmax * sizeof(unsigned);
+  L.rowoff = L.taken + (size_t)L.nflagmax * sizeof(unsigned);
+  L.icol = L.rowoff + (size_t)bmy * bmz * sizeof(int);
+  L.bytes = L.icol + (size_t)bmx * sizeof(int);
+  return L;
+}
+
+// Field offset of box cell li + bx (lj + by lk) = rowoff(row) + icol(li), row
+// = lj + by lk, from a field base pointer at cell (1, 0, 0) (Grid::at): the
+// periodic wrap and the multiplications once per row and column.
+__host__ __device__ __forceinline__ int box_rowoff(const Grid &g, const Box &b, int row) {
+  const int lk = row / b.by, lj = row - lk * b.by;
+  return (int)(g.sx * wrapany(b.bj0 + lj, g.ny) + g.sxy * (b.bk0 + lk));
+}
+__host__ __device__ __forceinline__ int box_icol(const Grid &g, const Box &b, int li) { return wrapany(b.bi0 + li, g.nx) - 1; }
+
+// Cell t = li + bx row of a box, walked by a stride of `threads` cells from
+// t = tid: (li, row) advance by (rb, qb) with a carry instead of a division
+// per cell.
+struct BoxWalk {
+  int li, row, rb, qb, bx;
+  __host__ __device__ __forceinline__ BoxWalk(int bx_, int threads, int tid) : bx(bx_) {
+    qb = threads / bx;
+    rb = threads - qb * bx;
+    row = tid / bx;
+    li = tid - row * bx;
+  }
+  __host__ __device__ __forceinline__ void advance() {
+    li += rb;
+    row += qb;
+    if (li >= bx) { li -= bx; row++; }
+  }
+};
+
+// A marker's own stencil in its sphere's box b: the LDS index e0 of its first
+// cell and how many of its z planes lie in the box (the rest are above kmax:
+// the k1 plane).  False when the stencil is not held in x and y, or its first
+// plane is not in the box (cannot happen for markers within radfp of the
+// centre; the kernel reports it).
+__host__ __device__ __forceinline__ bool own_cells(const Box &b, int i0, int j0, int k0, int &e0, int &nzin) {
+  e0 = box_at(b, i0, j0, k0);
+  if (e0 < 0 || box_at(b, i0 + 2, j0 + 2, k0) < 0) return false;
+  nzin = imin(3, b.bk0 + b.bz - k0);
+  return true;
+}
+
+// Stencil of the marker at offset (ex, ey, ez) from sphere q's centre (xq,
+// yq, zq), seen through the periodic shift (sxs nx, sys ny) in the unwrapped
+// frame of box b; false when it misses the box.
+template <bool WEIGHTS = true, typename T = double>
+__host__ __device__ __forceinline__ bool image_stencil(const Grid &g, const Box &b, int comp, double xq, double yq, double zq,
+                                                      double ex, double ey, double ez, int sxs, int sys, CompStencilT<T> &c) {
+  // a stencil spans less than two cells either side of its marker
+  const double xs = (xq + ex) * g.dxi + sxs * g.nx, ys = (yq + ey) * g.dyi + sys * g.ny;
+  const double zs = (zq + ez) * g.dzi;
+  if (xs < b.bi0 - 3.0 || xs > b.bi0 + b.bx + 2.0 || ys < b.bj0 - 3.0 || ys > b.bj0 + b.by + 2.0 ||
+      zs < b.bk0 - 3.0 || zs > b.bk0 + b.bz + 2.0)
+    return false;
+  double xl, yl, zl;
+  if (!marker_pos(g, xq, yq, zq, ex, ey, ez, xl, yl, zl)) return false;
+  comp_stencil<WEIGHTS, T>(g, xl, yl, zl, comp, c);
+  c.i0 += sxs * g.nx;
+  c.j0 += sys * g.ny;
+  return !(c.i0 + 2 < b.bi0 || c.i0 >= b.bi0 + b.bx || c.j0 + 2 < b.bj0 || c.j0 >= b.bj0 + b.by ||
+           c.k0 + 2 < b.bk0 || c.k0 >= b.bk0 + b.bz);
+}

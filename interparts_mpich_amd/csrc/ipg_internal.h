@@ -133,6 +133,22 @@ struct ipg_ctx {
   hipStream_t main_stream;        // the context's stream (c->stream is swapped while the side stream is filled)
 };
 
+// Every device allocation a context holds outside the Poisson plan (which
+// ipg_poisson_free releases), each listed once: release() frees exactly
+// these, so none is freed twice.
+constexpr int IPG_MAX_ALLOCS = 32 + IPG_NPART;
+inline int ipg_ctx_allocations(const ipg_ctx *c, void **out) {
+  void *const one[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->nml, c->nmc, c->ibm_err, c->ibm_flg,
+                       c->op, c->motion2, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2], c->pbuf, c->stamps};
+  static_assert(sizeof(one) / sizeof(one[0]) + IPG_NPART <= IPG_MAX_ALLOCS, "IPG_MAX_ALLOCS");
+  int n = 0;
+  for (void *q : one)
+    if (q) out[n++] = q;
+  for (int q = 0; q < IPG_NPART; q++)
+    if (c->ps.q[q]) out[n++] = c->ps.q[q];
+  return n;
+}
+
 // an FP32 field's base pointer: the same slot and indexing as f[q], float
 // elements (element -1, the ghost i = 0, exists)
 inline float *f32ptr(ipg_ctx *c, int q) { return (float *)(c->slab + (size_t)q * c->nfield) + 16; }
