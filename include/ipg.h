@@ -77,6 +77,10 @@ int ipg_finalize(ipg_ctx *ctx);
 const char *ipg_last_error(void);
 /* nl = nint((pi/3)(12((radius-0.3/dxi)dxi)^2+1)), src/init.f90:127-133 */
 int ipg_nl_formula(int itot, double lx, double radius);
+/* kerneltest (src/kernel.f90:43-63), the start-up check of src/main.f90:142-145:
+ * the integral of kernel(x) kernel(y) kernel(z) over a 51^3 grid of spacing
+ * 1/12, in the reference's summation order.  Host only; needs no context. */
+int ipg_kerneltest(double *sum);
 
 /* ---- state transfer (host <-> HBM) ----------------------------------- */
 /* host: real(8) (0:i1,0:j1,0:k1) Fortran order, (itot+2)(jtot+2)(ktot+2) values */
@@ -119,6 +123,13 @@ int ipg_ibm_begin(ipg_ctx *ctx);
 int ipg_eulr2lagr(ipg_ctx *ctx, int ibmiter);
 int ipg_lagrforces(ipg_ctx *ctx, int ibmiter);
 int ipg_lagr2eulr(ipg_ctx *ctx, int ibmiter);
+/* updtintermediatevel (src/forcing.f90:393-513, called after every IBM pass,
+ * src/main.f90:315): the values it appends to IBM_forcing.out, forcextot,
+ * forceytot, forceztot = the sums over the spheres of fxltot.. after each pass
+ * of the last IBM block: tot[3 ibmiter + component], 9 values.  Its other
+ * file, meanforces, holds the mean of forcex.., arrays the reference never
+ * writes (zero).  Returns 3 when the device error word is set. */
+int ipg_ibm_forces(ipg_ctx *ctx, double *tot);
 int ipg_bounduvw(ipg_ctx *ctx, int which);            /* src/bound.f90:11; which 0 = unew.., 1 = dudt.. */
 int ipg_boundp(ipg_ctx *ctx, int which);              /* src/bound.f90:82; which 0 = pnew, 1 = p */
 int ipg_boundscal(ipg_ctx *ctx);                      /* src/bound.f90:109 on cnew */
@@ -148,9 +159,12 @@ int ipg_chkdiv_noalloc(const double *u, const double *v, const double *w, int it
                        double ly, double lz, double *divtot, double *divmax);
 
 /* ---- output ---------------------------------------------------------- */
-/* post3d quantities (src/output.f90:997-1030, vorticity src/post.f90:15-61) at
- * the cell centres, interior only, i fastest (itot*jtot*ktot values): */
-enum { IPG_POST_PRE = 0, IPG_POST_VEZ, IPG_POST_CON, IPG_POST_VOX, IPG_POST_VOY, IPG_POST_VOZ };
+/* post3d quantities (src/output.f90:997-1074, vorticity src/post.f90:15-61) at
+ * the cell centres, interior only, i fastest (itot*jtot*ktot values).
+ * IPG_POST_SOL is the file 'sol': 1 minus the pressure-cell phase indicator of
+ * src/phase_indicator.f90:11-559 with itype = 1 (src/output.f90:1068-1074);
+ * IPG_POST_VOZ is computed by the reference and not written. */
+enum { IPG_POST_PRE = 0, IPG_POST_VEZ, IPG_POST_CON, IPG_POST_VOX, IPG_POST_VOY, IPG_POST_VOZ, IPG_POST_SOL };
 int ipg_post3d(ipg_ctx *ctx, int quantity, double *host);
 
 /* ---- seeded inputs (replaces random_number in src/init.f90:287-305,

@@ -20,10 +20,11 @@ PARTS = ["X", "Y", "Z", "THETA", "PHI", "U", "V", "W", "OMX", "OMY", "OMZ", "OMT
 PQ = {n: i for i, n in enumerate(PARTS)}
 
 # every symbol include/ipg.h declares
-EXPORTS = ["ipg_init", "ipg_finalize", "ipg_last_error", "ipg_nl_formula", "ipg_field_put", "ipg_field_get",
+EXPORTS = ["ipg_init", "ipg_finalize", "ipg_last_error", "ipg_nl_formula", "ipg_kerneltest", "ipg_field_put",
+           "ipg_field_get",
            "ipg_particles_put", "ipg_particles_get", "ipg_set_dt", "ipg_get_dt", "ipg_initsolver",
            "ipg_initparticles", "ipg_coordsfp", "ipg_rk", "ipg_ibm", "ipg_ibm_begin", "ipg_eulr2lagr",
-           "ipg_lagrforces", "ipg_lagr2eulr", "ipg_uniform", "ipg_bounduvw", "ipg_boundp",
+           "ipg_lagrforces", "ipg_lagr2eulr", "ipg_ibm_forces", "ipg_uniform", "ipg_bounduvw", "ipg_boundp",
            "ipg_boundscal", "ipg_fillps", "ipg_solver2d", "ipg_correc", "ipg_pnew_update",
            "ipg_intgr_over_sphere", "ipg_intgr_nwtn_eulr", "ipg_substep", "ipg_chkdt", "ipg_chkdiv", "ipg_chkdiv_noalloc",
            "ipg_post3d",
@@ -62,6 +63,7 @@ def load():
         "ipg_finalize": ([vp], ip),
         "ipg_last_error": ([], C.c_char_p),
         "ipg_nl_formula": ([ip, d, d], ip),
+        "ipg_kerneltest": ([dp], ip),
         "ipg_field_put": ([vp, ip, vp], ip),
         "ipg_field_get": ([vp, ip, vp], ip),
         "ipg_particles_put": ([vp, ip, vp], ip),
@@ -77,6 +79,7 @@ def load():
         "ipg_eulr2lagr": ([vp, ip], ip),
         "ipg_lagrforces": ([vp, ip], ip),
         "ipg_lagr2eulr": ([vp, ip], ip),
+        "ipg_ibm_forces": ([vp, dp], ip),
         "ipg_uniform": ([C.c_ulonglong, C.c_ulonglong, C.c_ulonglong, C.c_longlong, vp], ip),
         "ipg_bounduvw": ([vp, ip], ip),
         "ipg_boundp": ([vp, ip], ip),

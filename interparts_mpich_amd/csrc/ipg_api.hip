@@ -263,6 +263,27 @@ int ipg_nl_formula(int itot, double lx, double radius) {
   return (int)std::lround((pi / 3.0) * (12.0 * sq((radius - retrac) * dxi) + 1.0));
 }
 
+// src/kernel.f90:43-63 (kerneltest): the kernel's integral over a 51^3 grid
+// of spacing 1/12, summed in the reference's order (k, j, i loops, each term
+// ((((1/12)(1/12))(1/12)) kernel(x)) kernel(y)) kernel(z)); close to 1.
+// The kernel is the same function the IBM launches evaluate, run on the host.
+int ipg_kerneltest(double *sum) {
+  if (!sum) { ipg_set_error("kerneltest: null argument"); return 1; }
+  double s = 0.0;
+  for (int k = -25; k <= 25; k++) {
+    const double cz = k / 12.0;
+    for (int j = -25; j <= 25; j++) {
+      const double cy = j / 12.0;
+      for (int i = -25; i <= 25; i++) {
+        const double cx = i / 12.0;
+        s = s + ((((1.0 / 12.0) * (1.0 / 12.0)) * (1.0 / 12.0)) * roma_kernel(cx)) * roma_kernel(cy) * roma_kernel(cz);
+      }
+    }
+  }
+  *sum = s;
+  return 0;
+}
+
 int ipg_init(const ipg_params *prm, ipg_ctx **out) {
   if (!prm || !out) { ipg_set_error("ipg_init: null argument"); return 1; }
   *out = nullptr;
@@ -326,6 +347,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
         hipMalloc(&c->nml, (size_t)np * 3 * IBM_MAXNM * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->nmc, (size_t)np * 3 * sizeof(int)) != hipSuccess ||
         hipMalloc(&c->ibm_err, sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->ibm_psum, 9 * (size_t)np * IBM_WAVES * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->op, 24 * (size_t)np * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->motion2, 9 * (size_t)np * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->ps.nbf, (size_t)np * IPG_NB * sizeof(int)) != hipSuccess) {
@@ -336,6 +358,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
     (void)hipMemset(c->nbc, 0, (size_t)np * sizeof(int));
     (void)hipMemset(c->nmc, 0, (size_t)np * 3 * sizeof(int));
     (void)hipMemset(c->ibm_err, 0, sizeof(int));
+    (void)hipMemset(c->ibm_psum, 0, 9 * (size_t)np * IBM_WAVES * sizeof(double));
     (void)hipMemset(c->ps.nbf, 0, (size_t)np * IPG_NB * sizeof(int));
     for (int q = 0; q < IPG_NPART; q++) {
       if (hipMalloc(&c->ps.q[q], (size_t)np * sizeof(double)) != hipSuccess) { ipg_set_error("out of device memory"); return fail(1); }
@@ -506,6 +529,20 @@ int ipg_lagrforces(ipg_ctx *c, int ibmiter) {
   return 0;
 }
 
+// updtintermediatevel (src/forcing.f90:393-513): what it writes to
+// IBM_forcing.out, the totals over the spheres of fxltot, fyltot, fzltot after
+// each IBM pass of the last IBM block (fused substep or split entries).
+int ipg_ibm_forces(ipg_ctx *c, double *tot) {
+  if (!check_ctx(c) || !tot) return 1;
+  for (int q = 0; q < 9; q++) tot[q] = 0.0;
+  if (c->pc.np <= 0) return 0;
+  double *d = c->red;  // reduction scratch, free between the checks that use it
+  if (ipg_launch_ibm_forces(c, d)) return 1;
+  IPG_HIP(hipMemcpyAsync(tot, d, 9 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  IPG_HIP(hipStreamSynchronize(c->stream));
+  return ipg_ibm_check(c) ? 3 : 0;
+}
+
 int ipg_lagr2eulr(ipg_ctx *c, int ibmiter) {
   if (!ready(c)) return 1;
   if (ibmiter < 0 || ibmiter > 2) { ipg_set_error("lagr2eulr: ibmiter must be 0..2"); return 1; }
@@ -662,8 +699,10 @@ int ipg_chkdiv(ipg_ctx *c, double *tot, double *mx) {
 
 int ipg_post3d(ipg_ctx *c, int quantity, double *host) {
   if (!check_ctx(c) || !host) return 1;
-  if (quantity < 0 || quantity > 5) { ipg_set_error("post3d: quantity must be 0..5"); return 1; }
+  if (quantity < 0 || quantity > IPG_POST_SOL) { ipg_set_error("post3d: quantity must be 0..6"); return 1; }
   if (!c->solver_ready) { ipg_set_error("post3d: ipg_initsolver has not been called"); return 1; }
+  // the phase indicator reads the particle positions and neighbour flags
+  if (quantity == IPG_POST_SOL && !ready(c)) return 1;
   if (fill_xcol(c)) return 1;
   return ipg_launch_post3d(c, quantity, host) || ipg_ibm_check(c) ? 3 : 0;
 }

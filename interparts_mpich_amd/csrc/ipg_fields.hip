@@ -687,8 +687,12 @@ int ipg_launch_chkdiv(ipg_ctx *c, double *tot, double *mx) {
 int ipg_launch_post3d(ipg_ctx *c, int q, double *host) {
   const Grid &g = c->g;
   double *out = (double *)c->ps2.spec;
-  hipLaunchKernelGGL(k_post3d, cell_grid(g), dim3(BX, BY), 0, c->stream, g, q, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W],
-                     c->f[IPG_PNEW], c->f[IPG_C], out);
+  if (q == IPG_POST_SOL) {
+    if (ipg_launch_phase_sol(c, out)) return 1;
+  } else {
+    hipLaunchKernelGGL(k_post3d, cell_grid(g), dim3(BX, BY), 0, c->stream, g, q, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W],
+                       c->f[IPG_PNEW], c->f[IPG_C], out);
+  }
   IPG_HIP(hipGetLastError());
   IPG_HIP(hipMemcpyAsync(host, out, (size_t)g.nx * g.ny * g.nz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   IPG_HIP(hipStreamSynchronize(c->stream));

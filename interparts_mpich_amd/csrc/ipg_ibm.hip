@@ -44,6 +44,7 @@ struct IbmArgs {
   int *err;           // device error word: bits 1, 2: a neighbour list overflowed; 4: a stencil outside its box
   unsigned *flg;      // [np*3][nflagmax] the box cells the own stencils touch, set by pass 0
   double *Kread, *Kspread, *Kzero;  // dwdtf(.,.,k1) planes
+  double *psum;       // [3 passes][3 np][IBM_WAVES]: per-wavefront sums of the pass's new marker forces
   const double *dtp;
 };
 
@@ -343,6 +344,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     }
     __syncthreads();
   }
+  double fsum = 0.0;
 #pragma unroll
   for (int q = 0; q < IBM_MPT; q++) {
     CompStencilT<T> c;
@@ -376,6 +378,7 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     }
     const T fnew = MODE == 0 ? (ub - acc) * dti : flp[m] + (ub - acc) * dti;
     fl[m] = fnew;
+    fsum += (double)fnew;
     if (nzin < 3) {
       const T fc = fnew * c0;
 #pragma unroll
@@ -388,6 +391,35 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
               unsafeAtomicAdd(&A.Kspread[kplane(g, c.i0 + i, c.j0 + j)], (double)(fc * c.wx[i] * c.wy[j] * c.wz[k]));
     }
   }
+  // this pass's force total of the sphere's markers (src/forcing.f90:66 /
+  // 258, before the factor dVlagr), one partial sum per wavefront, for the
+  // per-pass output of updtintermediatevel (k_ibm_force_sums)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) fsum += __shfl_xor(fsum, o, 64);
+  if ((threadIdx.x & 63) == 0) A.psum[((long)pass * gridDim.x + blockIdx.x) * IBM_WAVES + (threadIdx.x >> 6)] = fsum;
+}
+
+// src/forcing.f90:440-458 (updtintermediatevel): forcextot, forceytot,
+// forceztot, the sums over the spheres of fxltot.. (the marker-force sums
+// times dVlagr, src/forcing.f90:66-77 / 258-275), for each of the three passes
+// of the last IBM block: out[3 pass + comp].  One block per (pass,
+// component); the order of the sums is fixed, so the result is reproducible.
+__global__ __launch_bounds__(256) void k_ibm_force_sums(const double *__restrict__ psum, int np, double dVlagr,
+                                                        double *__restrict__ out) {
+  const int pass = blockIdx.x / 3, comp = blockIdx.x % 3;
+  double s = 0.0;
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    const double *w = psum + ((long)pass * 3 * np + 3 * p + comp) * IBM_WAVES;
+    double t = 0.0;
+    for (int q = 0; q < IBM_WAVES; q++) t += w[q];
+    s += t * dVlagr;
+  }
+  __shared__ double red[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // Per-sphere force and torque sums after the last pass, src/forcing.f90:258-287,
@@ -494,6 +526,7 @@ bool ibm_args(ipg_ctx *c, IbmArgs &A, size_t &lds) {
   A.mk = c->mk; A.fl = c->fl;
   A.fld[0] = c->f[IPG_DUDT]; A.fld[1] = c->f[IPG_DVDT]; A.fld[2] = c->f[IPG_DWDT];
   A.nbl = c->nbl; A.nbc = c->nbc; A.nml = c->nml; A.nmc = c->nmc; A.err = c->ibm_err; A.flg = c->ibm_flg;
+  A.psum = c->ibm_psum;
   A.dtp = c->dscal;
   return true;
 }
@@ -553,6 +586,14 @@ int ipg_launch_ibm_final(ipg_ctx *c) {
   if (c->ibm32) hipLaunchKernelGGL((k_ibm<2, float>), dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, 2);
   else hipLaunchKernelGGL((k_ibm<2, double>), dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, 2);
   ipg_mark(c, "ibm_spread");
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
+
+// The per-pass force totals of the last IBM block into out9 (device memory).
+int ipg_launch_ibm_forces(ipg_ctx *c, double *out9) {
+  if (c->pc.np <= 0) return 0;
+  hipLaunchKernelGGL(k_ibm_force_sums, dim3(9), dim3(256), 0, c->stream, c->ibm_psum, c->pc.np, c->pc.dVlagr, out9);
   IPG_HIP(hipGetLastError());
   return 0;
 }

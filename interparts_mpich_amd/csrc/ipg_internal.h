@@ -92,6 +92,7 @@ struct ipg_ctx {
   int *nml, *nmc;                 // IBM neighbour markers per (sphere, component): [3 np][4096], [3 np]
   int *ibm_err;                   // device error word: IBM lists or boxes (1, 2, 4), collision candidates (8)
   unsigned *ibm_flg;              // per (sphere, component) box: cells its markers' stencils touch
+  double *ibm_psum;               // [3 passes][3 np][IBM_WAVES] per-wavefront sums of each pass's marker forces
   size_t ibm_flg_n;               // its length in words
   // particles
   PartState ps;
@@ -139,7 +140,8 @@ struct ipg_ctx {
 constexpr int IPG_MAX_ALLOCS = 32 + IPG_NPART;
 inline int ipg_ctx_allocations(const ipg_ctx *c, void **out) {
   void *const one[] = {c->slab, c->mk, c->fl, c->fl2, c->nbl, c->nbc, c->nml, c->nmc, c->ibm_err, c->ibm_flg,
-                       c->op, c->motion2, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2], c->pbuf, c->stamps};
+                       c->ibm_psum, c->op, c->motion2, c->red, c->dscal, c->ps.nbf, c->K1[0], c->K1[1], c->K1[2],
+                       c->pbuf, c->stamps};
   static_assert(sizeof(one) / sizeof(one[0]) + IPG_NPART <= IPG_MAX_ALLOCS, "IPG_MAX_ALLOCS");
   int n = 0;
   for (void *q : one)
@@ -240,8 +242,10 @@ int ipg_launch_ibm_passes(ipg_ctx *c);
 int ipg_launch_ibm_gather(ipg_ctx *c, int pass);
 int ipg_launch_ibm_final(ipg_ctx *c);
 int ipg_launch_ibm_totals(ipg_ctx *c, int pass);
+int ipg_launch_ibm_forces(ipg_ctx *c, double *out9);
 int ipg_ibm_check(ipg_ctx *c);
 int ipg_launch_post3d(ipg_ctx *c, int q, double *host);
+int ipg_launch_phase_sol(ipg_ctx *c, double *out);
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type);
 int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter);
 int ipg_launch_nbflags(ipg_ctx *c);

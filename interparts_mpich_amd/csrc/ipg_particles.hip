@@ -9,6 +9,7 @@
 //    1076-1111.
 // All arithmetic is FP64 in the reference's operation order.
 #include "ipg_internal.h"
+#include <algorithm>
 
 namespace {
 
@@ -502,7 +503,95 @@ PQ make_pq(ipg_ctx *c) {
   return P;
 }
 
+// Phase indicator of the pressure cells, src/phase_indicator.f90:11-559 with
+// itype = 1 as post3d calls it (src/output.f90:1068-1074): gamp(i,j,k) is the
+// largest, over the spheres and the periodic images their neighbour flags
+// name, of 1 when the cell's upper corner lies within radius - 2 dx of the
+// centre (lines 437-438), else sum1/sum2 of the eight corners' signed
+// distances (lines 440-457).  post3d passes one array for gamu, gamv, gamw,
+// upart, vpart and wpart, which nothing reads, so only gamp is formed.  One
+// block per sphere; the per-cell maximum is an atomic max on the bit
+// patterns (the values are >= 0, whose bit patterns order as they do), so
+// the result is independent of the order.  A cell whose nearest point lies
+// beyond the radius has every signed distance negative and a ratio of
+// exactly 0, which cannot raise the maximum: it is skipped.
+__global__ __launch_bounds__(256) void k_phase_gamp(Grid g, PartConst pc, PQ P, const int *__restrict__ nbf,
+                                                    unsigned long long *gamp) {
+  const int p = blockIdx.x;
+  const double R = pc.radius;
+  const double dx = 1.0 / g.dxi, dy = 1.0 / g.dyi, dz = 1.0 / g.dzi;
+  const double radin = R - 2.0 * dx, radin2 = radin * radin;
+  for (int nb = 0; nb < IPG_NB; nb++) {
+    if (nb > 0 && nbf[p * IPG_NB + nb] != 1) continue;
+    // lines 248-258 (images set up at 83-119): on one rank every neighbour
+    // is the rank itself, so the image of flag nb sits in slot nb + 4 (mod 8)
+    int nbrecv = nb;
+    if (nb > 0) { nbrecv = nb + 4; if (nbrecv > 8) nbrecv -= 8; }
+    double X = P.q[IPG_PX][p], Y = P.q[IPG_PY][p];
+    const double Z = P.q[IPG_PZ][p];
+    image_shift(g, nbrecv, X, Y);
+    const double cxc = X - 0.0, cyc = Y - 0.0, czc = Z;
+    int ilow = nint_f((cxc - R) * g.dxi - 2.0), ihigh = nint_f((cxc + R) * g.dxi + 2.0);
+    int jlow = nint_f((cyc - R) * g.dyi - 2.0), jhigh = nint_f((cyc + R) * g.dyi + 2.0);
+    int klow = nint_f((czc - R) * g.dzi - 2.0), khigh = nint_f((czc + R) * g.dzi + 2.0);
+    ilow = max(ilow, 1); jlow = max(jlow, 1); klow = max(klow, 1);
+    ihigh = min(ihigh, g.nx); jhigh = min(jhigh, g.ny); khigh = min(khigh, g.nz);
+    const int ni = ihigh - ilow + 1, nj = jhigh - jlow + 1, nk = khigh - klow + 1;
+    if (ni <= 0 || nj <= 0 || nk <= 0) continue;
+    const int nij = ni * nj, tot = nij * nk;
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+      const int kk = t / nij, r = t - kk * nij, jj = r / ni, ii = r - jj * ni;
+      const int i = ilow + ii, j = jlow + jj, k = klow + kk;
+      const double zmin = (double)(k - 1) * dz, zplus = (double)k * dz;
+      const double ymin = 0.0 + (double)(j - 1) * dy, yplus = 0.0 + (double)j * dy;
+      const double xmin = 0.0 + (double)(i - 1) * dx, xplus = 0.0 + (double)i * dx;
+      const double ex = xmin > X ? xmin - X : (xplus < X ? X - xplus : 0.0);
+      const double ey = ymin > Y ? ymin - Y : (yplus < Y ? Y - yplus : 0.0);
+      const double ez = zmin > Z ? zmin - Z : (zplus < Z ? Z - zplus : 0.0);
+      if (!(ex * ex + ey * ey + ez * ez <= R * R * (1.0 + 1e-9))) continue;
+      const double dzp2 = sqd(zplus - Z), dyp2 = sqd(yplus - Y), dxp2 = sqd(xplus - X);
+      const double dist2 = dxp2 + dyp2 + dzp2;
+      double v;
+      if (dist2 < radin2) {
+        v = 1.0;
+      } else {
+        const double dxm2 = sqd(xmin - X), dym2 = sqd(ymin - Y), dzm2 = sqd(zmin - Z);
+        double sg[8];
+        sg[0] = R - sqrt(dxm2 + dym2 + dzm2); sg[1] = R - sqrt(dxm2 + dyp2 + dzm2);
+        sg[2] = R - sqrt(dxp2 + dyp2 + dzm2); sg[3] = R - sqrt(dxp2 + dym2 + dzm2);
+        sg[4] = R - sqrt(dxm2 + dym2 + dzp2); sg[5] = R - sqrt(dxm2 + dyp2 + dzp2);
+        sg[6] = R - sqrt(dxp2 + dyp2 + dzp2); sg[7] = R - sqrt(dxp2 + dym2 + dzp2);
+        double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int cp = 0; cp < 8; cp++) { if (sg[cp] > 0.0) s1 = s1 + sg[cp]; s2 = s2 + fabs(sg[cp]); }
+        v = s1 / s2;
+      }
+      if (v > 0.0)
+        atomicMax(&gamp[(long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (k - 1))],
+                  (unsigned long long)__double_as_longlong(v));
+    }
+  }
+}
+
+// sol = 1 - gamp (src/phase_indicator.f90:497), in place over the interior
+__global__ void k_phase_sol(double *a, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) a[t] = 1.0 - a[t];
+}
+
 }  // namespace
+
+// The 'sol' quantity of post3d into out (interior, i fastest, nx*ny*nz).
+int ipg_launch_phase_sol(ipg_ctx *c, double *out) {
+  const Grid &g = c->g;
+  const long n = (long)g.nx * g.ny * g.nz;
+  IPG_HIP(hipMemsetAsync(out, 0, n * sizeof(double), c->stream));
+  if (c->pc.np > 0)
+    hipLaunchKernelGGL(k_phase_gamp, dim3(c->pc.np), dim3(256), 0, c->stream, g, c->pc, make_pq(c), c->ps.nbf,
+                       (unsigned long long *)out);
+  hipLaunchKernelGGL(k_phase_sol, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, c->stream, out, n);
+  IPG_HIP(hipGetLastError());
+  return 0;
+}
 
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type) {
   if (c->pc.np <= 0) return 0;

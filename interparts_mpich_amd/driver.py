@@ -152,9 +152,17 @@ class Simulation:
         check(self.lib.ipg_chkdiv(self.ctx, C.byref(a), C.byref(b)), "chkdiv")
         return a.value, b.value
 
+    def ibm_forces(self):
+        """updtintermediatevel's IBM_forcing.out values (src/forcing.f90:440-458)
+        of the last IBM block: (3 passes, 3 components) sums over the spheres
+        of fxltot, fyltot, fzltot."""
+        a = np.zeros(9)
+        check(self.lib.ipg_ibm_forces(self.ctx, a.ctypes.data_as(C.POINTER(C.c_double))), "ibm_forces")
+        return a.reshape(3, 3)
+
     def post3d(self, name):
-        """One post3d quantity (pre vez con vox voy voz) as a (ktot, jtot, itot) array."""
-        q = ("pre", "vez", "con", "vox", "voy", "voz").index(name)
+        """One post3d quantity (pre vez con vox voy voz sol) as a (ktot, jtot, itot) array."""
+        q = ("pre", "vez", "con", "vox", "voy", "voz", "sol").index(name)
         a = np.empty((self.case.ktot, self.case.jtot, self.case.itot))
         check(self.lib.ipg_post3d(self.ctx, q, _ptr(a)), f"post3d {name}")
         return a

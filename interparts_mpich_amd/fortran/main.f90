@@ -3,7 +3,9 @@
 ! MI355X library.  Inputs: parameters.inp (&PARAMETERS of src/param.f90),
 ! run.nml (&GRIDSIZE: the compile-time sizes of src/gridsize.f90, and
 ! &RUNCONTROL), lagrangianforcepoints2 when np > 0, fld%07d when begin > 0.
-! Outputs: timestep.txt (src/main.f90:425-431); info_1dstat.out and partvisu%07d.bin every iout1d
+! Outputs: timestep.txt (src/main.f90:425-431); meanforces and IBM_forcing.out,
+! a line each per IBM pass (updtintermediatevel, src/main.f90:315; run.nml
+! ibm_log = .false. leaves them out); info_1dstat.out and partvisu%07d.bin every iout1d
 ! steps, partstate%07d every ioutchk steps with particles, the post3d fields
 ! name%07d_3d.bin every iout2d steps (src/main.f90:443-472); fld%07d
 ! (src/loadd.f90 format) and partstate%07d when the run stops (step nstep,
@@ -32,7 +34,7 @@ program interpt
   use mod_loadpart
   use mod_output
   implicit none
-  real :: dtmax, maxerror, avererror, t1
+  real :: dtmax, maxerror, avererror, t1, sumk
   integer :: ibmiter, u, last
   integer(8) :: cs, c0, c1, crate
   integer(c_int) :: kbtune
@@ -46,7 +48,12 @@ program interpt
   end if
   if (begin == 0) then
     call init_ics
-    if (np > 0) call initparticles
+    if (np > 0) then
+      ! src/main.f90:142-145
+      call ipg_ok(ipg_kerneltest(sumk), 'kerneltest')
+      write(6, *) 'Integral over kernel = ', sumk
+      call initparticles
+    end if
     time = 0.
     call bounduvw(FLD_NEW)
     call boundscal
@@ -88,6 +95,13 @@ program interpt
       if (fused_substep) then
         ! the whole substep below as one launch sequence (one hipGraph per stage)
         call ipg_ok(ipg_substep(ctx, int(rkiter, c_int)), 'substep')
+        ! updtintermediatevel's lines of the substep's three IBM passes
+        if (np > 0 .and. ibm_log) then
+          do ibmiter = 0, 2
+            curiter = ibmiter
+            call updtintermediatevel(istep)
+          end do
+        end if
         cycle
       end if
       if (rkiter == 1) call rk1
@@ -103,6 +117,7 @@ program interpt
             call updtlagrforces(maxerror, avererror)
           end if
           call lagr2eulr
+          if (ibm_log) call updtintermediatevel(istep)
         end do
       end if
       call bounduvw(FLD_DT)

@@ -13,7 +13,7 @@ module mod_ipg_capi
             ipg_ibm, ipg_ibm_begin, ipg_eulr2lagr, ipg_lagrforces, ipg_lagr2eulr, ipg_bounduvw, ipg_boundp, &
             ipg_boundscal, ipg_fillps, ipg_solver2d, ipg_correc, ipg_pnew_update, ipg_intgr_over_sphere, &
             ipg_intgr_nwtn_eulr, ipg_substep, ipg_chkdt, ipg_chkdiv, ipg_uniform, ipg_sync, ipg_use_graphs, &
-            ipg_use_side_stream, ipg_autotune, ipg_post3d
+            ipg_use_side_stream, ipg_autotune, ipg_post3d, ipg_kerneltest, ipg_ibm_forces
 
   ! field selectors (enum of include/ipg.h)
   integer(c_int), parameter, public :: IPG_U = 0, IPG_V = 1, IPG_W = 2, IPG_PNEW = 3, IPG_C = 4, &
@@ -29,7 +29,7 @@ module mod_ipg_capi
        IPG_PRKDWDT = 36, IPG_NPART = 37
   ! post3d quantities
   integer(c_int), parameter, public :: IPG_POST_PRE = 0, IPG_POST_VEZ = 1, IPG_POST_CON = 2, IPG_POST_VOX = 3, &
-       IPG_POST_VOY = 4, IPG_POST_VOZ = 5
+       IPG_POST_VOY = 4, IPG_POST_VOZ = 5, IPG_POST_SOL = 6
 
   type, bind(C) :: ipg_params
     integer(c_int) :: itot, jtot, ktot, np, nl
@@ -133,6 +133,15 @@ module mod_ipg_capi
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: ibmiter
+    end function
+    integer(c_int) function ipg_ibm_forces(ctx, tot) bind(C, name="ipg_ibm_forces")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: tot(3, 0:2)
+    end function
+    integer(c_int) function ipg_kerneltest(sum) bind(C, name="ipg_kerneltest")
+      import :: c_int, c_double
+      real(c_double), intent(out) :: sum
     end function
     integer(c_int) function ipg_bounduvw(ctx, which) bind(C, name="ipg_bounduvw")
       import :: c_int, c_ptr

@@ -39,7 +39,9 @@ module mod_param
   integer(8) :: seed = 1
   real(8) :: perturb = 0.1
   logical :: use_graphs = .true., fused_substep = .true.
-  namelist /runcontrol/ begin, nstep, seed, perturb, use_graphs, fused_substep
+  ! the per-pass lines of updtintermediatevel (src/forcing.f90:430-468)
+  logical :: ibm_log = .true.
+  namelist /runcontrol/ begin, nstep, seed, perturb, use_graphs, fused_substep, ibm_log
   integer :: nl = 0
   integer :: imax, jmax, kmax, i1, j1, k1
   real(8), parameter :: pi = acos(-1.d0)
@@ -289,6 +291,7 @@ contains
 end module mod_interp_spread
 
 module mod_forcing
+  use mod_param, only: itot, jtot, kmax
   use mod_common
   use mod_interp_spread, only: curiter
   implicit none
@@ -304,6 +307,28 @@ contains
     call ipg_ok(ipg_lagrforces(ctx, int(curiter, c_int)), 'updtlagrforces')
     maxerror = 0.
     avererror = 0.
+  end subroutine
+  ! src/forcing.f90:393-513, after IBM pass curiter: one line appended to
+  ! meanforces (istep, time/tref and the mean of forcex, forcey, forcez over
+  ! the interior, arrays the reference never writes: zero) and one to
+  ! IBM_forcing.out (istep, time and the sums over the spheres of fxltot,
+  ! fyltot, fzltot after the pass).  The velocity update the routine once
+  ! made is commented out in the reference.
+  subroutine updtintermediatevel(istep)
+    integer, intent(in) :: istep
+    real(8), parameter :: tref = 1.                   ! src/init.f90:102
+    real(8) :: tot(3, 0:2), sumfx, sumfy, sumfz
+    integer :: u
+    call ipg_ok(ipg_ibm_forces(ctx, tot), 'updtintermediatevel')
+    sumfx = 0. / (1. * itot * jtot * kmax)
+    sumfy = sumfx
+    sumfz = sumfx
+    open(newunit=u, file='meanforces', position='append')
+    write(u, '(I8,4E16.8)') istep, time / tref, sumfx, sumfy, sumfz
+    close(u)
+    open(newunit=u, file='IBM_forcing.out', position='append')
+    write(u, '(I8,4E16.8)') istep, time, tot(1, curiter), tot(2, curiter), tot(3, curiter)
+    close(u)
   end subroutine
 end module mod_forcing
 
@@ -476,10 +501,11 @@ end module mod_loadpart
 
 ! src/output.f90: post1d (13-593), whose files are info_1dstat.out and, through
 ! outputpart, partvisu%07d.bin (its profile statistics are computed and not
-! written), and post3d (997-1030):
-! pre, vez, con, vox, voy as name%07d_3d.bin, the interior in Fortran order
-! (decomp_2d_write_every with no skipping), computed on the device.  The
-! phase indicator 'sol' of post3d (src/phase_indicator.f90) is not built.
+! written), and post3d (997-1074):
+! pre, vez, con, vox, voy and the phase indicator sol
+! (src/phase_indicator.f90:11-559, itype = 1) as name%07d_3d.bin, the
+! interior in Fortran order (decomp_2d_write_every with no skipping),
+! computed on the device.
 module mod_output
   use mod_param
   use mod_common
@@ -524,13 +550,14 @@ contains
   subroutine post3d(nr)
     integer, intent(in) :: nr
     character(len=7) :: istepchar
-    character(len=3), parameter :: names(5) = ['pre', 'vez', 'con', 'vox', 'voy']
-    integer(c_int), parameter :: sel(5) = [IPG_POST_PRE, IPG_POST_VEZ, IPG_POST_CON, IPG_POST_VOX, IPG_POST_VOY]
+    character(len=3), parameter :: names(6) = ['pre', 'vez', 'con', 'vox', 'voy', 'sol']
+    integer(c_int), parameter :: sel(6) = [IPG_POST_PRE, IPG_POST_VEZ, IPG_POST_CON, IPG_POST_VOX, IPG_POST_VOY, &
+         IPG_POST_SOL]
     real(8), allocatable :: var(:, :, :)
     integer :: u, q
     allocate(var(imax, jmax, kmax))
     write(istepchar, '(i7.7)') nr
-    do q = 1, 5
+    do q = 1, 6
       call ipg_ok(ipg_post3d(ctx, sel(q), var), 'post3d')
       open(newunit=u, file=names(q)//istepchar//'_3d.bin', access='stream', form='unformatted', status='replace', &
            action='write')

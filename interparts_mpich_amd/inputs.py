@@ -370,7 +370,8 @@ def ensemble_cases(n=64, seed=5):
     return out
 
 
-POST3D = ("pre", "vez", "con", "vox", "voy")
+# the files post3d writes (src/output.f90:1015-1074), in its order
+POST3D = ("pre", "vez", "con", "vox", "voy", "sol")
 
 
 def read_post3d(path, case):

@@ -86,6 +86,7 @@ typedef struct {
   /* time */
   double dt, time;
   double forceytot, diag_fsum[3];
+  double forcetot[3][3]; /* updtintermediatevel's forcextot.. after each IBM pass */
 } orc_state;
 
 static inline size_t I1(const orc_state *s, int i, int j, int k) {
@@ -681,6 +682,17 @@ void orc_lagrforces(orc_state *s, int ibmiter) {
     s->torqx[p] = tx * s->dVlagr; s->torqy[p] = ty * s->dVlagr; s->torqz[p] = tz * s->dVlagr;
     s->torqtheta[p] = (s->torqy[p] * cos(s->pphi[p])) - (s->torqx[p] * sin(s->pphi[p]));
   }
+  /* src/forcing.f90:440-458 (updtintermediatevel, called after the pass's
+   * spread, which does not change fxltot): the sums over the spheres */
+  double t[3] = {0.0, 0.0, 0.0};
+  for (int p = 0; p < s->np; p++) { t[0] = t[0] + s->fxltot[p]; t[1] = t[1] + s->fyltot[p]; t[2] = t[2] + s->fzltot[p]; }
+  if (ibmiter >= 0 && ibmiter <= 2)
+    for (int c = 0; c < 3; c++) s->forcetot[ibmiter][c] = t[c];
+}
+
+/* the nine values of orc_lagrforces' per-pass sums, out[3 ibmiter + comp] */
+void orc_ibm_forces(orc_state *s, double *out) {
+  for (int q = 0; q < 9; q++) out[q] = s->forcetot[q / 3][q % 3];
 }
 
 /* src/interp_spread.f90:230-474 lagr2eulr */
@@ -864,10 +876,13 @@ void orc_chkdiv(orc_state *s, double *tot, double *mx) {
   *tot = divtot; *mx = divmax;
 }
 
-/* src/output.f90:997-1030 (post3d) and src/post.f90:15-61 (vorticity): one
+void orc_phase_sol(orc_state *s, double *out);
+
+/* src/output.f90:997-1074 (post3d) and src/post.f90:15-61 (vorticity): one
  * cell-centred output quantity, interior only, i fastest.
- * q: 0 pre, 1 vez, 2 con, 3 vox, 4 voy, 5 voz */
+ * q: 0 pre, 1 vez, 2 con, 3 vox, 4 voy, 5 voz, 6 sol (orc_phase_sol) */
 void orc_post3d(orc_state *s, int q, double *out) {
+  if (q == 6) { orc_phase_sol(s, out); return; }
   const double *ux = s->u, *uy = s->v, *uz = s->w;
   const double dxi = s->dxi, dyi = s->dyi, dzi = s->dzi;
 #define A(f, i, j, k) (f)[I1(s, (i), (j), (k))]
@@ -1095,6 +1110,63 @@ void orc_intgr_over_sphere(orc_state *s, int type) {
     else if (type == 2) { s->intu[p] = t[0]; s->intv[p] = t[1]; s->intw[p] = t[2]; }
     else { s->intu[p] = t[0]; s->intv[p] = t[1]; s->intw[p] = t[2]; }
   }
+}
+
+/* 'sol' of post3d (src/output.f90:1068-1074): 1 - gamp, the pressure-cell
+ * phase indicator of src/phase_indicator.f90:11-559 with itype = 1.  post3d
+ * passes one array for gamu, gamv, gamw, upart, vpart and wpart, which the
+ * output never reads, so only the pressure loop (422-490) is restated; its
+ * itype = 0 branch (459-486) changes nothing with itype = 1.  Images: lines
+ * 248-258 with every neighbour the rank itself (slot nb + 4 mod 8, shifted
+ * as set up at 83-119, image_shift); boundleftmyid = boundfrontmyid = 0. */
+void orc_phase_sol(orc_state *s, double *out) {
+  const size_t n = (size_t)s->nx * s->ny * s->nz;
+  double dx = 1.0 / s->dxi, dy = 1.0 / s->dyi, dz = 1.0 / s->dzi;
+  double R = s->radius, radin = R - 2.0 * dx, radin2 = sq(radin);
+  for (size_t q = 0; q < n; q++) out[q] = 0.0;
+  for (int p = 0; p < s->np; p++)
+    for (int nb = 0; nb < NB; nb++) {
+      if (!(nb == 0 || s->nbf[p * NB + nb] == 1)) continue;
+      int nbrecv = nb;
+      if (nb > 0) { nbrecv = nb + 4; if (nbrecv > 8) nbrecv -= 8; }
+      double X = s->px[p], Y = s->py[p], Z = s->pz[p];
+      image_shift(s, nbrecv, &X, &Y);
+      double cxc = X - 0.0, cyc = Y - 0.0, czc = Z;
+      int ilow = nint_f((cxc - R) * s->dxi - 2.0), ihigh = nint_f((cxc + R) * s->dxi + 2.0);
+      int jlow = nint_f((cyc - R) * s->dyi - 2.0), jhigh = nint_f((cyc + R) * s->dyi + 2.0);
+      int klow = nint_f((czc - R) * s->dzi - 2.0), khigh = nint_f((czc + R) * s->dzi + 2.0);
+      if (ilow < 1) ilow = 1; if (jlow < 1) jlow = 1; if (klow < 1) klow = 1;
+      if (ihigh > s->nx) ihigh = s->nx; if (jhigh > s->ny) jhigh = s->ny; if (khigh > s->nz) khigh = s->nz;
+      for (int k = klow; k <= khigh; k++) {
+        double zmin = (double)(k - 1) * dz, zplus = (double)k * dz;
+        double dzp2 = sq(zplus - Z);
+        for (int j = jlow; j <= jhigh; j++) {
+          double ymin = 0.0 + (double)(j - 1) * dy, yplus = 0.0 + (double)j * dy;
+          double dyp2 = sq(yplus - Y);
+          for (int i = ilow; i <= ihigh; i++) {
+            double xmin = 0.0 + (double)(i - 1) * dx, xplus = 0.0 + (double)i * dx;
+            double dxp2 = sq(xplus - X);
+            double dist2 = dxp2 + dyp2 + dzp2;
+            double *g = &out[(size_t)(i - 1) + (size_t)s->nx * ((size_t)(j - 1) + (size_t)s->ny * (size_t)(k - 1))];
+            if (dist2 < radin2) {
+              *g = 1.0;
+            } else {
+              double dxm2 = sq(xmin - X), dym2 = sq(ymin - Y), dzm2 = sq(zmin - Z);
+              double sg[8];
+              sg[0] = R - sqrt(dxm2 + dym2 + dzm2); sg[1] = R - sqrt(dxm2 + dyp2 + dzm2);
+              sg[2] = R - sqrt(dxp2 + dyp2 + dzm2); sg[3] = R - sqrt(dxp2 + dym2 + dzm2);
+              sg[4] = R - sqrt(dxm2 + dym2 + dzp2); sg[5] = R - sqrt(dxm2 + dyp2 + dzp2);
+              sg[6] = R - sqrt(dxp2 + dyp2 + dzp2); sg[7] = R - sqrt(dxp2 + dym2 + dzp2);
+              double s1 = 0.0, s2 = 0.0;
+              for (int cp = 0; cp < 8; cp++) { if (sg[cp] > 0.0) s1 = s1 + sg[cp]; s2 = s2 + fabs(sg[cp]); }
+              /* gamp = max(sum1/sum2, gamp) (line 457) */
+              if (s1 / s2 > *g) *g = s1 / s2;
+            }
+          }
+        }
+      }
+    }
+  for (size_t q = 0; q < n; q++) out[q] = 1.0 - out[q];  /* line 497 */
 }
 
 /* src/collisions.f90:222-438 lubrication: only the squeeze force f1 reaches

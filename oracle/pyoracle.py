@@ -69,6 +69,7 @@ def load():
         "orc_correc": ([vp, ip], None),
         "orc_pnew_update": ([vp], None),
         "orc_chkdiv": ([vp, dp, dp], None),
+        "orc_ibm_forces": ([vp, vp], None),
         "orc_post3d": ([vp, C.c_int, vp], None),
         "orc_chkdt": ([vp, dp], None),
         "orc_update_nb_flags": ([vp], None),
@@ -175,10 +176,16 @@ class Oracle:
         return d.value
 
     def post3d(self, name):
-        q = ("pre", "vez", "con", "vox", "voy", "voz").index(name)
+        q = ("pre", "vez", "con", "vox", "voy", "voz", "sol").index(name)
         a = np.empty((self.case.ktot, self.case.jtot, self.case.itot))
         self.lib.orc_post3d(self.s, q, a.ctypes.data_as(C.c_void_p))
         return a
+
+    def ibm_forces(self):
+        """(3 passes, 3 components): forcextot.. of updtintermediatevel."""
+        a = np.zeros(9)
+        self.lib.orc_ibm_forces(self.s, a.ctypes.data_as(C.c_void_p))
+        return a.reshape(3, 3)
 
     def chkdiv(self):
         a, b = C.c_double(), C.c_double()

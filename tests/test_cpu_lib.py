@@ -46,3 +46,18 @@ def test_uniform_matches_host_generator():
         ref = uniform(seed, stream, np.arange(first, first + count, dtype=np.uint64))
         assert np.array_equal(out, ref)
     assert lib.ipg_uniform(1, 0, 0, -1, None) != 0
+
+
+def test_kerneltest_matches_reference():
+    """ipg_kerneltest (the start-up print of src/main.f90:142-145, host only):
+    the reference's kernel.f90 value, bit for bit (the golden vector written
+    by the reference compiled here), and the oracle's."""
+    import json
+    from oracle.pyoracle import load as oracle_load
+    lib = _lib.load()
+    s = ctypes.c_double()
+    assert lib.ipg_kerneltest(ctypes.byref(s)) == 0
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_vectors.json")))["kerneltest"]
+    assert s.value == gold
+    assert s.value == oracle_load().orc_kerneltest()
+    assert lib.ipg_kerneltest(None) != 0

@@ -88,15 +88,40 @@ def test_driver_steps_match_oracle(tmp_path, fused):
     """Three steps with a dt update after every step (ioutchk = 1): the fused
     substep and the routine-by-routine loop (rk1.., eulr2lagr/complagrforces/
     lagr2eulr, bounduvw, fillps, solver2d, correc, intgr_nwtn_eulr) both match
-    the oracle to <= 1e-11 relative."""
-    from oracle.pyoracle import Oracle
+    the oracle to <= 1e-11 relative.  The start-up print of kerneltest
+    (src/main.f90:142-145) and the per-pass lines of updtintermediatevel
+    (src/forcing.f90:430-468: meanforces, IBM_forcing.out) match it too, to
+    the files' eight digits."""
+    from oracle.pyoracle import Oracle, load as oracle_load
     needs_exe()
     case = dataclasses.replace(CONFIGS["t32"], ioutchk=1)
     d = write_case_dir(str(tmp_path), case, nstep=3, fused_substep=fused)
-    assert run(d).returncode == 0
+    r = run(d)
+    assert r.returncode == 0
+    kline = [l for l in r.stdout.splitlines() if "Integral over kernel" in l]
+    assert len(kline) == 1 and abs(float(kline[0].split("=")[1]) - oracle_load().orc_kerneltest()) <= 1e-14
     o = Oracle(case).initialize(initial_fields(case), place_spheres(case), markers_for(case))
+    want = []   # (istep, time, fx, fy, fz) per IBM pass, as the reference appends them
     for _ in range(3):
-        o.step()
+        # Oracle.step, with updtintermediatevel's values after each substep
+        o.istep += 1
+        o.time += o.dt
+        for rkiter in (1, 2, 3):
+            o.substep(rkiter)
+            for tot in o.ibm_forces():
+                want.append((o.istep, o.time, *tot))
+        o.divergence = o.chkdiv()
+        o.set_dt(0.75 * o.chkdt())
+    want = np.array(want)
+    got = np.loadtxt(os.path.join(d, "IBM_forcing.out"), ndmin=2)
+    assert got.shape == (27, 5), got.shape
+    assert np.array_equal(got[:, 0], want[:, 0])
+    assert np.allclose(got[:, 1], want[:, 1], rtol=1e-7, atol=0)
+    scale = float(np.max(np.abs(want[:, 2:])))
+    print("IBM_forcing.out max |difference|", float(np.max(np.abs(got[:, 2:] - want[:, 2:]))), "of", scale)
+    assert np.max(np.abs(got[:, 2:] - want[:, 2:])) <= 1e-7 * scale
+    mean = np.loadtxt(os.path.join(d, "meanforces"), ndmin=2)
+    assert mean.shape == (27, 5) and np.array_equal(mean[:, 0], want[:, 0]) and np.all(mean[:, 2:] == 0.0)
     f, info = read_fld(fld_path(d, 3), case)
     assert info[1] == 3.0
     for n in FLUID:

@@ -343,7 +343,7 @@ def test_post3d_matches_oracle():
     g, o = pair(case)
     for _ in range(2):
         g.step(); o.step()
-    for n in ("pre", "vez", "con", "vox", "voy", "voz"):
+    for n in ("pre", "vez", "con", "vox", "voy", "voz", "sol"):
         r = rel(g.post3d(n), o.post3d(n)) if np.any(o.post3d(n)) else float(np.max(np.abs(g.post3d(n))))
         assert r <= 1e-11, (n, r)
     for n in FLUID:

@@ -10,7 +10,9 @@
   96, 128; nz = 192, where the singular kx = ky = 0 Thomas pivot of
   src/solver.f90:114-125 is not exactly zero);
 - ensemble cases advanced concurrently equal each case advanced alone;
-- the device error word stops every output entry after a dropped contribution.
+- the device error word stops every output entry after a dropped contribution;
+- post3d's phase indicator 'sol' (src/phase_indicator.f90:11-559, itype = 1)
+  on spheres across the periodic edges and corners and at both walls.
 """
 import numpy as np
 import pytest
@@ -190,3 +192,58 @@ def test_error_word_stops_outputs():
     for call in (g.chkdiv, g.chkdt, lambda: g.get("U"), lambda: g.post3d("pre"), lambda: g.pget("U")):
         with pytest.raises(RuntimeError, match="more than"):
             call()
+
+
+def test_phase_indicator_matches_oracle():
+    """'sol' of post3d bit for bit against the oracle, on the seeded 32^3
+    placement and on spheres placed across the x and y periodic edges, across
+    a corner of the periodic plane and against both walls (images through the
+    neighbour flags, boxes clamped at the walls); <= 1e-12 after a step, as
+    the positions then differ by rounding."""
+    from interparts_mpich_amd.inputs import CONFIGS
+    edge = Case(name="edges", itot=32, jtot=32, ktot=32, np=5, diameter_h=8.0, ratiorho=1.5, re_bulk=1000.0, seed=9)
+    h, R = edge.lx / edge.itot, edge.radius
+    edge_spheres = (np.array([0.2 * R, 0.5, edge.lx - 0.3 * R, 0.3, 0.7]),
+                    np.array([0.5, edge.ly - 0.1 * R, 0.4 * R, 0.25, 0.75]),
+                    np.array([0.5, 0.4, 0.6, R + 0.05 * h, edge.lz - R - 0.05 * h]))
+    for case, spheres in ((CONFIGS["t32"], None), (edge, edge_spheres)):
+        spheres = spheres if spheres is not None else place_spheres(case)
+        fields, markers = initial_fields(case), markers_for(case)
+        print(f"[case {case.name}] sol", flush=True)
+        g = Simulation(case).initialize(fields, spheres, markers)
+        o = Oracle(case).initialize(fields, spheres, markers)
+        a, b = g.post3d("sol"), o.post3d("sol")
+        print(case.name, "cells inside", int(np.sum(b == 0.0)), "cut", int(np.sum((b > 0) & (b < 1))))
+        assert np.any(b < 1.0)
+        assert np.array_equal(a, b)
+        g.step(); o.step()
+        d = float(np.max(np.abs(g.post3d("sol") - o.post3d("sol"))))
+        print(case.name, "after a step", d)
+        assert d <= 1e-12, d
+        g.close(); o.close()
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_ibm_pass_forces_match_oracle(graphs):
+    """updtintermediatevel's values (src/forcing.f90:440-458): the sums over
+    the spheres of fxltot, fyltot, fzltot after each of the three IBM passes,
+    from the fused substep (graph replays or eager launches), against the
+    oracle after every substep of two steps; the last pass's sums equal the
+    sums of the per-sphere totals the library keeps."""
+    from interparts_mpich_amd.inputs import CONFIGS
+    case = CONFIGS["t32"]
+    fields, spheres, markers = initial_fields(case), place_spheres(case), markers_for(case)
+    g = Simulation(case, use_graphs=graphs).initialize(fields, spheres, markers)
+    o = Oracle(case).initialize(fields, spheres, markers)
+    for s in range(6):
+        rk = s % 3 + 1
+        g.substep(rk); o.substep(rk)
+        a, b = g.ibm_forces(), o.ibm_forces()
+        scale = max(float(np.max(np.abs(b))), 1e-300)
+        d = float(np.max(np.abs(a - b)))
+        print("substep", s + 1, "pass totals", b[:, 1], "max |difference|", d, "of", scale)
+        assert np.all(b != 0.0)
+        assert d <= 1e-12 * scale, (s, d, scale)
+        for c, n in enumerate(("FXLTOT", "FYLTOT", "FZLTOT")):
+            t = g.pget(n)
+            assert abs(a[2, c] - sum(float(x) for x in t)) <= 1e-13 * scale

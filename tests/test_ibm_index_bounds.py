@@ -134,8 +134,8 @@ static void neighbours(Ctx &C) {
   }
 }
 
-// One launch of k_ibm<MODE> for workgroup (p, comp), phase by phase.
-static void block(Ctx &C, int MODE, int p, int comp) {
+// One launch of k_ibm<MODE> (IBM pass `pass`) for workgroup (p, comp), phase by phase.
+static void block(Ctx &C, int MODE, int pass, int p, int comp) {
   const CaseData &cs = *C.cs;
   const Grid &g = C.g;
   Globals &G = *C.G;
@@ -395,6 +395,11 @@ static void block(Ctx &C, int MODE, int p, int comp) {
       const long m = (long)p * cs.nl + tid + q * NT;
       CHECK(m < M, "force store %ld", m);
     }
+  // the per-wavefront force sums: psum[pass][3 np][waves]
+  for (int tid = 0; tid < NT; tid += 64) {
+    const long at = ((long)pass * 3 * cs.np + blk) * (NT / 64) + tid / 64;
+    CHECK(at >= 0 && at < 9L * cs.np * (NT / 64) && NT % 64 == 0, "psum index %ld", at);
+  }
 }
 
 // Every neighbour marker whose stencil touches box (p, comp), through any
@@ -518,8 +523,9 @@ int main(int argc, char **argv) {
           for (int q = 0; q < 3; q++) G.writes[q].assign(G.nfield, 0);
           Ctx C{&cs, g, {bm[0], bm[1], bm[2]}, NT, MPT, esize, L.nflagmax, &G};
           neighbours(C);
-          for (int MODE : {0, 1, 1, 2})
-            for (int blk = 0; blk < 3 * cs.np; blk++) block(C, MODE, blk / 3, blk % 3);
+          const int modes[4] = {0, 1, 1, 2}, passes[4] = {0, 1, 2, 2};
+          for (int launch = 0; launch < 4; launch++)
+            for (int blk = 0; blk < 3 * cs.np; blk++) block(C, modes[launch], passes[launch], blk / 3, blk % 3);
           if (!cs.dense) {
             CHECK(C.overflowed == 0, "a neighbour list overflowed");
             ownership(C);

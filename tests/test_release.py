@@ -29,7 +29,7 @@ int main() {
   uintptr_t next = 0x1000;
   auto give = [&](auto *&p) { p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(next); next += 0x100; };
   give(c.slab); give(c.mk); give(c.fl); give(c.fl2); give(c.nbl); give(c.nbc); give(c.nml); give(c.nmc);
-  give(c.ibm_err); give(c.ibm_flg); give(c.op); give(c.motion2); give(c.red); give(c.dscal); give(c.ps.nbf);
+  give(c.ibm_err); give(c.ibm_flg); give(c.ibm_psum); give(c.op); give(c.motion2); give(c.red); give(c.dscal); give(c.ps.nbf);
   for (auto &k : c.K1) give(k);
   give(c.pbuf); give(c.stamps);
   for (auto &q : c.ps.q) give(q);
