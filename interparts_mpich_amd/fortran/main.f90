@@ -6,9 +6,9 @@
 ! Outputs: timestep.txt (src/main.f90:425-431); meanforces and IBM_forcing.out,
 ! a line each per IBM pass (updtintermediatevel, src/main.f90:315; run.nml
 ! ibm_log = .false. leaves them out); info_1dstat.out and partvisu%07d.bin every iout1d
-! steps, partstate%07d every ioutchk steps with particles, the post3d fields
+! steps, partstate%07d and allpartdata%07d (src/loadd.f90:121-692) every ioutchk steps with particles, the post3d fields
 ! name%07d_3d.bin every iout2d steps (src/main.f90:443-472); fld%07d
-! (src/loadd.f90 format) and partstate%07d when the run stops (step nstep,
+! (src/loadd.f90 format), partstate%07d and allpartdata%07d when the run stops (step nstep,
 ! t_end or tmax hours, src/main.f90:498-512).  nstep is the last step index, as
 ! in `do istep = begin+1,nstep` (src/main.f90:252).
 program interpt
@@ -38,6 +38,8 @@ program interpt
   integer :: ibmiter, u, last
   integer(8) :: cs, c0, c1, crate
   integer(c_int) :: kbtune
+  character(len=7) :: istepchar
+  logical :: have_partstate
   real, parameter :: tmax = 23.5   ! hours, src/main.f90:140
 
   call system_clock(cs, crate)
@@ -60,7 +62,15 @@ program interpt
     call chkdt(dtmax)
     dt = 0.5 * dtmax
   else
-    if (np > 0) call loadpart(0, begin)
+    if (np > 0) then
+      write(istepchar, '(i7.7)') begin
+      inquire(file='partstate'//istepchar, exist=have_partstate)
+      if (have_partstate) then
+        call loadpart(0, begin)
+      else
+        call loadpart_ref(0, begin)
+      end if
+    end if
     call loadflds(0, begin)
     call chkdt(dtmax)
     write(6, *) 'nr steps at beginning simulation = ', begin
@@ -141,7 +151,10 @@ program interpt
     end if
     last = istep
     if (mod(istep, iout1d) == 0) call post1d(time, istep)
-    if (np > 0 .and. mod(istep, ioutchk) == 0) call loadpart(1, last)
+    if (np > 0 .and. mod(istep, ioutchk) == 0) then
+      call loadpart(1, last)
+      call loadpart_ref(1, last)
+    end if
     if (mod(istep, iout2d) == 0) call post3d(istep)
     ! src/main.f90:498-512: after tmax hours, at step nstep or at t_end, write
     ! the particle and field state and leave the loop
@@ -163,6 +176,9 @@ contains
   subroutine dump(nr)
     integer, intent(inout) :: nr
     call loadflds(1, nr)
-    if (np > 0) call loadpart(1, nr)
+    if (np > 0) then
+      call loadpart(1, nr)
+      call loadpart_ref(1, nr)
+    end if
   end subroutine dump
 end program interpt

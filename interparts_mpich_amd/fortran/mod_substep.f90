@@ -456,10 +456,10 @@ end module mod_loadflds
 ! quantities of type particle kept on the device (x y z theta phi u v w omx
 ! omy omz omtheta intu intv intw intomx intomy intomz colfx colfy colfz coltx
 ! colty coltz fxltot fyltot fzltot torqx torqy torqz torqtheta), each as np
-! FP64 values, quantity after quantity, then (time, istep, dt).  It stands in
-! for allpartdata%07d of src/loadd.f90:121-692, whose contact-history arrays
-! (dx.., psi, firstc) have no counterpart here: the tangential contact force
-! they feed never reaches the state (src/collisions.f90:199-205).
+! FP64 values, quantity after quantity, then (time, istep, dt): the complete
+! state, from which a restart continues exactly.  The reference's own file,
+! allpartdata%07d of src/loadd.f90:121-692, is written beside it
+! (loadpart_ref) and read when no partstate%07d is present.
 module mod_loadpart
   use mod_param
   use mod_common
@@ -497,6 +497,110 @@ contains
       close(u)
     end if
   end subroutine loadpart
+
+  ! allpartdata%07d of src/loadd.f90:121-692 on one rank.  A particle is a
+  ! record of type particle_restart (lines 123-138), nrec = 31 + 11 nqmax
+  ! reals, nqmax = min(np + 2, 15) (src/param.f90:100): x .. intomz (18),
+  ! flag1, flag2, colfx .. coltz (6), fxltot, fyltot, fzltot, then the contact
+  ! history dx, dy, dz, dxt, dyt, dzt, dut, dvt, dwt, psi (nqmax each), qmax,
+  ! idp and firstc (nqmax).  The reference moves skip = 29 + 11 nqmax reals
+  ! per particle over that record array (lines 140-142, 206, 690), so its file
+  ! holds the first (29 + 11 nqmax) np reals of the array and the last 2 np are
+  ! never written; this writes the same bytes, the particles in id order (the
+  ! order of the masters on one rank).  Slots the reference leaves unassigned
+  ! (flag1, flag2, dut, dvt, dwt, psi) are 0, and so are the contact history,
+  ! qmax and firstc, which this build does not keep (the tangential force they
+  ! feed never reaches the state, src/collisions.f90:199-205).
+  ! Reading follows lines 190-339: each particle takes the first
+  ! 24 + 10 nqmax reals of its record in the layout of type particle_old
+  ! (src/common.f90:74-88), which has no flag1, flag2: colfx, colfy, colfz,
+  ! coltx, colty, coltz are read from the record's flag1, flag2, colfx, colfy,
+  ! colfz, coltx slots, as the reference reads them; fxltot.. are not read.
+  ! Where the missing tail has taken a record's idp, the record's position
+  ! gives it (the reference would abort there); a particle whose state lies
+  ! in the missing tail cannot be read, and the run stops, as the reference's
+  ! does.
+  subroutine loadpart_ref(in, nr)
+    integer, intent(in) :: in
+    integer, intent(inout) :: nr
+    character(len=7) :: istepchar
+    real(8), allocatable :: rec(:), q(:, :)
+    integer :: nq, nrec, nfile, u, p, m, idp, base, ios
+    integer(8) :: fsize
+    logical, allocatable :: seen(:)
+    nq = min(np + 2, 15)
+    nrec = 31 + 11 * nq
+    nfile = 29 + 11 * nq
+    allocate(rec(nrec * np), q(np, 0:IPG_NPART - 1), seen(np))
+    rec = 0.
+    q = 0.
+    write(istepchar, '(i7.7)') nr
+    if (in == 0) then
+      open(newunit=u, file='allpartdata'//istepchar, access='stream', form='unformatted', status='old', &
+           action='read', iostat=ios)
+      if (ios /= 0) error stop 'allpartdata: file not found'
+      inquire(unit=u, size=fsize)
+      if (fsize /= 8_8 * nfile * np) then
+        write(6, *) 'allpartdata: ', fsize, ' bytes, expected ', 8_8 * nfile * np
+        error stop 'allpartdata: wrong size for np'
+      end if
+      read(u) rec(1:nfile * np)
+      close(u)
+      seen = .false.
+      do p = 1, np
+        base = (p - 1) * nrec
+        ! the state a particle takes: reals 1 .. 24 + 10 nqmax of its record
+        if (base + 24 + 10 * nq > nfile * np) then
+          write(6, *) 'Fatal error in initialisation of particle positions: allpartdata holds no state for', &
+               ' the particle of record ', p
+          error stop 1
+        end if
+        idp = p
+        if (base + 29 + 10 * nq + 2 <= nfile * np) idp = nint(rec(base + 29 + 10 * nq + 2))
+        if (idp < 1 .or. idp > np) error stop 'allpartdata: particle id out of range'
+        if (seen(idp)) error stop 'allpartdata: particle id repeated'
+        seen(idp) = .true.
+        ! src/loadd.f90:194-205
+        if (rec(base + 1) < 0. .or. rec(base + 1) > lx .or. rec(base + 2) < 0. .or. rec(base + 2) > ly .or. &
+            rec(base + 3) < 0. .or. rec(base + 3) > lz) then
+          write(6, *) 'Fatal error in initialisation of particle positions - '
+          write(6, *) 'particle outside the domain!'
+          error stop 1
+        end if
+        do m = 0, 17
+          q(idp, m) = rec(base + 1 + m)
+        end do
+        do m = 0, 5
+          q(idp, IPG_PCOLFX + m) = rec(base + 19 + m)
+        end do
+      end do
+      call ipg_ok(ipg_initparticles(ctx, q(:, IPG_PX), q(:, IPG_PY), q(:, IPG_PZ)), 'ipg_initparticles')
+      do m = 0, IPG_NPART - 1
+        call ipg_ok(ipg_particles_put(ctx, int(m, c_int), q(:, m)), 'particles_put')
+      end do
+    else
+      do m = 0, IPG_NPART - 1
+        call ipg_ok(ipg_particles_get(ctx, int(m, c_int), q(:, m)), 'particles_get')
+      end do
+      do p = 1, np
+        base = (p - 1) * nrec
+        do m = 0, 17
+          rec(base + 1 + m) = q(p, m)
+        end do
+        do m = 0, 5
+          rec(base + 21 + m) = q(p, IPG_PCOLFX + m)
+        end do
+        do m = 0, 2
+          rec(base + 27 + m) = q(p, IPG_PFXLTOT + m)
+        end do
+        rec(base + 29 + 10 * nq + 2) = 1. * p
+      end do
+      open(newunit=u, file='allpartdata'//istepchar, access='stream', form='unformatted', status='replace', &
+           action='write')
+      write(u) rec(1:nfile * np)
+      close(u)
+    end if
+  end subroutine loadpart_ref
 end module mod_loadpart
 
 ! src/output.f90: post1d (13-593), whose files are info_1dstat.out and, through

@@ -348,6 +348,39 @@ def read_partstate(path, case):
     return {q: a[m * n:(m + 1) * n] for m, q in enumerate(PARTS)}, tuple(a[-3:])
 
 
+# the 29 leading reals of a record of type particle_restart (src/loadd.f90:123-138)
+ALLPART_FIELDS = ("X", "Y", "Z", "THETA", "PHI", "U", "V", "W", "OMX", "OMY", "OMZ", "OMTHETA", "INTU", "INTV",
+                  "INTW", "INTOMX", "INTOMY", "INTOMZ", "FLAG1", "FLAG2", "COLFX", "COLFY", "COLFZ", "COLTX", "COLTY",
+                  "COLTZ", "FXLTOT", "FYLTOT", "FZLTOT")
+
+
+def allpartdata_layout(npart):
+    """(nqmax, reals per record, reals per particle in the file) of
+    allpartdata%07d: nqmax = min(np + 2, 15) (src/param.f90:100), a record of
+    type particle_restart is 31 + 11 nqmax reals, and the reference moves
+    skip = 29 + 11 nqmax per particle (src/loadd.f90:140-142)."""
+    nq = min(npart + 2, 15)
+    return nq, 31 + 11 * nq, 29 + 11 * nq
+
+
+def read_allpartdata(path, case):
+    """allpartdata%07d as an (np, record) array of the reference's record
+    array; the missing tail (the last 2 np reals, never written) is NaN."""
+    nq, nrec, nfile = allpartdata_layout(case.np)
+    a = np.fromfile(path, dtype="<f8")
+    if a.size != nfile * case.np:
+        raise ValueError(f"{path}: {a.size} values, expected {nfile * case.np}")
+    rec = np.full(nrec * case.np, np.nan)
+    rec[:a.size] = a
+    return rec.reshape(case.np, nrec)
+
+
+def allpartdata_idp_slot(npart):
+    """index of idp in a record (after the 29 reals, 10 nqmax history, qmax)"""
+    nq = min(npart + 2, 15)
+    return 29 + 10 * nq + 1
+
+
 def ensemble_cases(n=64, seed=5):
     """The batched ensemble: n independent cases with their own seeds.  Grids
     from {64^3, 96x96x64, 128^3, 128x128x192}; D uniform in [8h, 12h]; Re_bulk

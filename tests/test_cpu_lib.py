@@ -61,3 +61,16 @@ def test_kerneltest_matches_reference():
     assert s.value == gold
     assert s.value == oracle_load().orc_kerneltest()
     assert lib.ipg_kerneltest(None) != 0
+
+
+def test_allpartdata_layout():
+    """allpartdata%07d sizes (src/loadd.f90:123-142, src/param.f90:100): the
+    record of type particle_restart, 29 reals, 10 history arrays of nqmax,
+    qmax, idp and firstc(nqmax), and the skip the reference moves per
+    particle, two reals short of it."""
+    from interparts_mpich_amd.inputs import ALLPART_FIELDS, allpartdata_idp_slot, allpartdata_layout
+    assert len(ALLPART_FIELDS) == 29
+    for npart, nq in ((1, 3), (6, 8), (13, 15), (1000, 15)):
+        q, nrec, nfile = allpartdata_layout(npart)
+        assert q == nq and nrec == 29 + 10 * nq + 2 + nq and nfile == (27 + 10 * nq) + (2 + nq)
+        assert allpartdata_idp_slot(npart) == 29 + 10 * nq + 1

@@ -138,6 +138,52 @@ def test_driver_steps_match_oracle(tmp_path, fused):
 
 
 @pytest.mark.gpu
+def test_driver_allpartdata(tmp_path):
+    """allpartdata%07d (src/loadd.f90:121-692) as the reference writes it: np
+    records of type particle_restart (31 + 11 nqmax reals) of which the file
+    holds the first (29 + 11 nqmax) np reals (skip, lines 140-142), x .. intomz,
+    colf.., colt.., fxltot.. of each sphere in their slots, idp where the file
+    reaches it.  Read back with no partstate%07d present (begin = 2, no step),
+    each sphere takes the leading reals of its record in the layout of
+    particle_old, as the reference's reader does: positions, velocities and
+    integrals unchanged, colfx..coltz from the slots flag1, flag2, colfx,
+    colfy, colfz, coltx; the idp lost to the missing tail comes from the
+    record's position."""
+    from interparts_mpich_amd.inputs import ALLPART_FIELDS, allpartdata_idp_slot, allpartdata_layout, read_allpartdata
+    needs_exe()
+    case = CONFIGS["t32"]
+    d = write_case_dir(str(tmp_path), case, nstep=2)
+    assert run(d).returncode == 0
+    nq, nrec, nfile = allpartdata_layout(case.np)
+    path = os.path.join(d, "allpartdata0000002")
+    assert os.path.getsize(path) == 8 * nfile * case.np
+    rec = read_allpartdata(path, case)
+    q, _ = read_partstate(os.path.join(d, "partstate0000002"), case)
+    for m, name in enumerate(ALLPART_FIELDS):
+        if name.startswith("FLAG"):
+            assert np.all(rec[:, m] == 0.0)
+        else:
+            assert np.array_equal(rec[:, m], q[name]), name
+    ids = rec[:, allpartdata_idp_slot(case.np)]
+    kept = ~np.isnan(ids)
+    assert np.array_equal(ids[kept], np.arange(1, case.np + 1)[kept]) and not kept[-1]
+    # restart from the reference's file alone, with a distinct value in every
+    # slot from flag1 to fzltot so that the slots the reader takes show
+    os.remove(os.path.join(d, "partstate0000002"))
+    rec[:, 18:29] = 1000.0 * np.arange(1, case.np + 1)[:, None] + np.arange(18, 29)[None, :]
+    rec.reshape(-1)[:nfile * case.np].astype("<f8").tofile(path)
+    write_case_dir(d, case, nstep=2, begin=2)
+    assert run(d).returncode == 0
+    q2, _ = read_partstate(os.path.join(d, "partstate0000002"), case)
+    for name in ALLPART_FIELDS[:18]:
+        assert np.array_equal(q2[name], q[name]), name
+    for m, name in enumerate(("COLFX", "COLFY", "COLFZ", "COLTX", "COLTY", "COLTZ")):
+        assert np.array_equal(q2[name], rec[:, 18 + m]), name
+    for name in ("FXLTOT", "FYLTOT", "FZLTOT"):
+        assert np.all(q2[name] == 0.0), name
+
+
+@pytest.mark.gpu
 def test_driver_restart_round_trip(tmp_path):
     """begin = 2, nstep = 0 reads fld0000002 / partstate0000002 and writes them
     back unchanged (src/loadd.f90 read and write paths)."""
