@@ -29,7 +29,11 @@
  * velocities and pass-0 forces vs closed forms; sphere integrals vs
  * 4/3 pi R^3 and rigid fields; Newton-Euler vs the free-fall closed form;
  * collisions vs the closed-form spring-dashpot force; chkdt vs the dtmax
- * known answer of src/terminate.txt.  tests/test_oracle_mutations.py plants
+ * known answer of src/terminate.txt; the phase indicator vs the corner
+ * classification of every cell against every periodic image, a cut cell's
+ * closed form, second-order volume convergence and translation.  The
+ * per-pass force sums of updtintermediatevel are sums of the per-sphere
+ * totals pinned above.  tests/test_oracle_mutations.py plants
  * plausible slips in copies of this file and checks that a pin fails.
  * Restated from the reference's code with no output or closed form to pin
  * them (parity unpinned beyond the reading): the three IBM passes and the
