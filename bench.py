@@ -218,6 +218,14 @@ def run_profile(args):
         sim.substep(s % 3 + 1)
     sim.sync()
     print(f"[profile] {args.warmup + args.steps} substeps done", flush=True)
+    if args.checks:
+        # the dt and divergence checks of every ioutchk-th step, repeated
+        for name, fn in (("chkdt", sim.chkdt), ("chkdiv", sim.chkdiv)):
+            t0 = time.perf_counter()
+            for _ in range(args.checks):
+                fn()
+            print(f"[profile] {name}: {(time.perf_counter() - t0) / args.checks * 1e3:.3f} ms per call "
+                  f"(with its synchronisation and copy)", flush=True)
     sim.close()
 
 
@@ -302,6 +310,8 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="for a profiler: initialise the case, then run --warmup and --steps substeps of the path "
                          "--precision names (fp64 or mixed) and exit; prints one line per phase")
+    ap.add_argument("--checks", type=int, default=0, metavar="N",
+                    help="with --profile: then run chkdt and chkdiv N times each")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default RK-stage launch shape")
     ap.add_argument("--cpu-substeps", type=int, default=12, help="oracle substeps in the CPU-baseline sample")

@@ -4,7 +4,9 @@
 // FP64, with the reference's operation order kept term by term (the library
 // is compiled with -ffp-contract=off so no multiply-add is fused).
 #include "ipg_internal.h"
+#include <algorithm>
 #include <mutex>
+#include <vector>
 
 namespace {
 
@@ -411,37 +413,86 @@ __device__ __forceinline__ double chkdt_dt2(double u2, double v2, double w2, dou
   if (fabs(1.0 * velo) > 1.e-12) return sqrt(3.0) / (sqrt(u2) * dxi + sqrt(v2) * dyi + sqrt(w2) * dzi);
   return 99999.0;
 }
-__global__ __launch_bounds__(256) void k_chkdt(Grid g, const double *__restrict__ U, const double *__restrict__ V,
-                                                const double *__restrict__ W, double *__restrict__ part) {
+
+// src/chkdt.f90:6-271 as a march in z: a workgroup takes a 64 x 4 (i, j)
+// tile through kb planes, one column per lane.  A cell's u and v values at
+// k+1 and its w values at k-1 are carried in registers between planes, and
+// its i and j neighbours are in the cache lines the tile's other lanes load,
+// so each field crosses HBM about once (a grid-stride form re-read the
+// planes either side: 3.7 times the bytes).  At each cell the three dt
+// limits of the u, v and w points, each from the velocity interpolated to
+// that point, in the reference's order of terms.
+__global__ __launch_bounds__(256) void k_chkdt_march(Grid g, const double *__restrict__ U, const double *__restrict__ V,
+                                                      const double *__restrict__ W, double *__restrict__ part, int kb) {
   __shared__ double sm[4];
-  const long n = (long)g.nx * g.ny * g.nz;
-  double m = 899999.0;
+  const int i = blockIdx.x * 64 + (int)(threadIdx.x & 63) + 1, j = blockIdx.y * 4 + (int)(threadIdx.x >> 6) + 1;
+  const int k0 = blockIdx.z * kb + 1, k1 = min(k0 + kb - 1, g.nz);
   const double dxi = g.dxi, dyi = g.dyi, dzi = g.dzi;
-  const double dxi2 = dxi * dxi + dyi * dyi + dzi * dzi;
-  const double dt1 = 1.65 / (4.0 * g.visc * dxi2);
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    int i = (int)(t % g.nx) + 1, j = (int)((t / g.nx) % g.ny) + 1, k = (int)(t / ((long)g.nx * g.ny)) + 1;
-    long c = g.at(i, j, k), sx = g.sx, sxy = g.sxy;
-    double u2 = sqd(U[c]);
-    double v2 = sqd(0.25 * (V[c] + V[c - sx] + V[c + 1] + V[c + 1 - sx]));
-    double w2 = sqd(0.25 * (W[c] + W[c - sxy] + W[c + 1] + W[c + 1 - sxy]));
-    m = fmin(m, fmin(dt1, chkdt_dt2(u2, v2, w2, dxi, dyi, dzi)));
-    u2 = sqd(0.25 * (U[c] + U[c + sx] + U[c - 1 + sx] + U[c - 1]));
-    v2 = sqd(V[c]);
-    w2 = sqd(0.25 * (W[c] + W[c + sx] + W[c + sx - sxy] + W[c - sxy]));
-    m = fmin(m, fmin(dt1, chkdt_dt2(u2, v2, w2, dxi, dyi, dzi)));
-    u2 = sqd(0.25 * (U[c] + U[c - 1] + U[c - 1 + sxy] + U[c + sxy]));
-    v2 = sqd(0.25 * (V[c] + V[c - sx] + V[c - sx + sxy] + V[c + sxy]));
-    w2 = sqd(W[c]);
-    m = fmin(m, fmin(dt1, chkdt_dt2(u2, v2, w2, dxi, dyi, dzi)));
+  const double dt1 = 1.65 / (4.0 * g.visc * (dxi * dxi + dyi * dyi + dzi * dzi));
+  double m = 899999.0;
+  if (i <= g.nx && j <= g.ny && k0 <= k1) {
+    const long sx = g.sx, sxy = g.sxy;
+    long c = g.at(i, j, k0);
+    // plane k: u(i,j), u(i-1,j), v(i,j), v(i,j-1); plane k-1: w(i,j), w(i+1,j), w(i,j+1)
+    double Uc = U[c], Um = U[c - 1], Vc = V[c], Vm = V[c - sx];
+    double Wc1 = W[c - sxy], Wp1 = W[c + 1 - sxy], Wj1 = W[c + sx - sxy];
+    for (int k = k0; k <= k1; k++, c += sxy) {
+      const double Ujp = U[c + sx], Umjp = U[c - 1 + sx], Vip = V[c + 1], Vipjm = V[c + 1 - sx];
+      const double Wc = W[c], Wp = W[c + 1], Wj = W[c + sx];
+      const double Uk1 = U[c + sxy], Umk1 = U[c - 1 + sxy], Vk1 = V[c + sxy], Vmk1 = V[c - sx + sxy];
+      double u2 = sqd(Uc);
+      double v2 = sqd(0.25 * (Vc + Vm + Vip + Vipjm));
+      double w2 = sqd(0.25 * (Wc + Wc1 + Wp + Wp1));
+      m = fmin(m, fmin(dt1, chkdt_dt2(u2, v2, w2, dxi, dyi, dzi)));
+      u2 = sqd(0.25 * (Uc + Ujp + Umjp + Um));
+      v2 = sqd(Vc);
+      w2 = sqd(0.25 * (Wc + Wj + Wj1 + Wc1));
+      m = fmin(m, fmin(dt1, chkdt_dt2(u2, v2, w2, dxi, dyi, dzi)));
+      u2 = sqd(0.25 * (Uc + Um + Umk1 + Uk1));
+      v2 = sqd(0.25 * (Vc + Vm + Vmk1 + Vk1));
+      w2 = sqd(Wc);
+      m = fmin(m, fmin(dt1, chkdt_dt2(u2, v2, w2, dxi, dyi, dzi)));
+      Uc = Uk1; Um = Umk1; Vc = Vk1; Vm = Vmk1;
+      Wc1 = Wc; Wp1 = Wp; Wj1 = Wj;
+    }
   }
   m = wave_min(m);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     double r = sm[0];
-    for (int q = 1; q < (int)(blockDim.x >> 6); q++) r = fmin(r, sm[q]);
-    part[blockIdx.x] = r;
+    for (int q = 1; q < 4; q++) r = fmin(r, sm[q]);
+    part[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = r;
+  }
+}
+
+// src/chkdiv.f90:23-39 as the same z march (k_chkdiv below is the
+// grid-stride form, used on caller-owned arrays): w(k-1) carried, the i-1 and
+// j-1 neighbours from the tile's cache lines.
+__global__ __launch_bounds__(256) void k_chkdiv_march(Grid g, const double *__restrict__ U, const double *__restrict__ V,
+                                                       const double *__restrict__ W, double *__restrict__ part, int kb) {
+  __shared__ double s1[4], s2[4];
+  const int i = blockIdx.x * 64 + (int)(threadIdx.x & 63) + 1, j = blockIdx.y * 4 + (int)(threadIdx.x >> 6) + 1;
+  const int k0 = blockIdx.z * kb + 1, k1 = min(k0 + kb - 1, g.nz);
+  double tot = 0.0, mx = 0.0;
+  if (i <= g.nx && j <= g.ny && k0 <= k1) {
+    long c = g.at(i, j, k0);
+    double Wm = W[c - g.sxy];
+    for (int k = k0; k <= k1; k++, c += g.sxy) {
+      const double Wc = W[c], Vc = V[c], Vm = V[c - g.sx], Uc = U[c], Um = U[c - 1];
+      const double div = (Wc - Wm) * g.dzi + (Vc - Vm) * g.dyi + (Uc - Um) * g.dxi;
+      tot += div;
+      mx = fmax(mx, fabs(div));
+      Wm = Wc;
+    }
+  }
+  tot = wave_sum(tot); mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) { s1[threadIdx.x >> 6] = tot; s2[threadIdx.x >> 6] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    part[2 * b] = ((s1[0] + s1[1]) + (s1[2] + s1[3]));
+    part[2 * b + 1] = fmax(fmax(s2[0], s2[1]), fmax(s2[2], s2[3]));
   }
 }
 
@@ -656,12 +707,26 @@ int ipg_launch_pnew(ipg_ctx *c) {
   return 0;
 }
 
+// The grid of the z-marching checks: 64 x 4 (i, j) tiles, and enough z
+// blocks of kb planes to reach about 1024 workgroups (at most 1024, the
+// partials the reduction scratch holds).
+static dim3 march_grid(const Grid &g, int &kb) {
+  const int bx = (g.nx + 63) / 64, by = (g.ny + 3) / 4;
+  const int nkb = std::max(1, std::min(g.nz, 1024 / std::max(1, bx * by)));
+  kb = (g.nz + nkb - 1) / nkb;
+  return dim3(bx, by, (g.nz + kb - 1) / kb);
+}
+
 int ipg_launch_chkdt(ipg_ctx *c, double *dtmax) {
-  const int nb = 1024;
-  hipLaunchKernelGGL(k_chkdt, dim3(nb), dim3(256), 0, c->stream, c->g, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->red);
+  int kb;
+  const dim3 grid = march_grid(c->g, kb);
+  const int nb = (int)(grid.x * grid.y * grid.z);
+  if (nb > 2048) { ipg_set_error("chkdt: grid too wide for the reduction scratch"); return 1; }
+  hipLaunchKernelGGL(k_chkdt_march, grid, dim3(256), 0, c->stream, c->g, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->red,
+                     kb);
   IPG_HIP(hipGetLastError());
-  double h[nb];
-  IPG_HIP(hipMemcpyAsync(h, c->red, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> h(nb);
+  IPG_HIP(hipMemcpyAsync(h.data(), c->red, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   IPG_HIP(hipStreamSynchronize(c->stream));
   double m = h[0];
   for (int q = 1; q < nb; q++) m = h[q] < m ? h[q] : m;
@@ -670,11 +735,15 @@ int ipg_launch_chkdt(ipg_ctx *c, double *dtmax) {
 }
 
 int ipg_launch_chkdiv(ipg_ctx *c, double *tot, double *mx) {
-  const int nb = 1024;
-  hipLaunchKernelGGL(k_chkdiv, dim3(nb), dim3(256), 0, c->stream, c->g, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->red);
+  int kb;
+  const dim3 grid = march_grid(c->g, kb);
+  const int nb = (int)(grid.x * grid.y * grid.z);
+  if (nb > 2048) { ipg_set_error("chkdiv: grid too wide for the reduction scratch"); return 1; }
+  hipLaunchKernelGGL(k_chkdiv_march, grid, dim3(256), 0, c->stream, c->g, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->red,
+                     kb);
   IPG_HIP(hipGetLastError());
-  double h[2 * nb];
-  IPG_HIP(hipMemcpyAsync(h, c->red, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> h(2 * (size_t)nb);
+  IPG_HIP(hipMemcpyAsync(h.data(), c->red, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   IPG_HIP(hipStreamSynchronize(c->stream));
   double a = 0.0, b = 0.0;
   for (int q = 0; q < nb; q++) { a += h[2 * q]; b = h[2 * q + 1] > b ? h[2 * q + 1] : b; }
