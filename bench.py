@@ -154,49 +154,85 @@ def copy_bandwidth(mib=1024):
 
 
 def run_ensemble(args):
-    """Batched ensemble: every case keeps its own context, stream and captured
-    per-stage graphs; one round launches every case's substep before waiting,
-    so independent small cases overlap on the GPU.  Compared with the same
-    cases advanced one at a time."""
+    """Batched ensemble.  Three ways to advance the same cases, timed one after
+    another over the same number of substeps:
+    - fused: one batch (ipg_batch_*), every kernel of a substep stage merged
+      over all cases, one captured graph per substep, launch shapes chosen by
+      the batch's tuner (the headline);
+    - concurrent: each case its own context, stream and captured graphs, one
+      round launching every case's substep before waiting;
+    - sequential: the cases one after another."""
     from interparts_mpich_amd.inputs import ensemble_cases, initial_fields, place_spheres, markers_for
-    from interparts_mpich_amd.driver import Simulation
+    from interparts_mpich_amd.driver import Simulation, Batch
     cases = ensemble_cases(args.ensemble)
+    precision = 1 if args.precision == "mixed" else 0
     sims = []
     for c in cases:
         # many contexts overlap one another's streams: no side stream each
-        sims.append(Simulation(c, use_graphs=True, side_stream=args.side_stream == "on")
+        sims.append(Simulation(c, use_graphs=True, side_stream=args.side_stream == "on", precision=precision)
                     .initialize(initial_fields(c), place_spheres(c), markers_for(c)))
     print(f"[bench] ensemble of {len(sims)} cases ready", file=sys.stderr, flush=True)
+    steps = max(3, args.steps // 3 * 3)
+    warm = max(3, args.warmup // 3 * 3)
     for sim in sims:
-        for s in range(args.warmup):
+        for s in range(warm):
             sim.substep(s % 3 + 1)
         sim.sync()
     t0 = time.perf_counter()
     for sim in sims:
-        for s in range(args.steps):
+        for s in range(steps):
             sim.substep(s % 3 + 1)
         sim.sync()
     t_seq = time.perf_counter() - t0
+    print(f"[bench] sequential: {len(sims) * steps / t_seq:.1f} case-substeps/s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    for s in range(args.steps):
+    for s in range(steps):
         for sim in sims:
             sim.substep(s % 3 + 1)
     for sim in sims:
         sim.sync()
     t_conc = time.perf_counter() - t0
-    n = len(sims) * args.steps
-    cells = sum(c.cells for c in cases) * args.steps
+    print(f"[bench] concurrent contexts: {len(sims) * steps / t_conc:.1f} case-substeps/s", file=sys.stderr, flush=True)
+    batch = Batch(sims)
+    tuned = 0 if args.no_autotune else batch.autotune()
+    for s in range(warm):
+        batch.substep(s % 3 + 1)
+    batch.sync()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        batch.substep(s % 3 + 1)
+    batch.sync()
+    t_fused = time.perf_counter() - t0
+    nl, npers = batch.launches(1)
+    phases = {}
+    for rk in (1, 2, 3):
+        for name, ms in batch.phases(rk):
+            key = name.split("#")[0]
+            phases[key] = phases.get(key, 0.0) + ms / 3.0
+    batch.sync()
+    print(f"[bench] fused batch: {len(sims) * steps / t_fused:.1f} case-substeps/s, {nl} merged launches per substep "
+          f"({npers} persistent)", file=sys.stderr, flush=True)
+    print("[bench] fused batch per-substep stage time (ms, eager): " +
+          ", ".join(f"{k}={v:.3f}" for k, v in sorted(phases.items(), key=lambda kv: -kv[1])), file=sys.stderr, flush=True)
+    n = len(sims) * steps
+    cells = sum(c.cells for c in cases) * steps
     out = {
-        "metric": f"ensemble case-substeps/s ({len(sims)} cases) FP64", "value": round(n / t_conc, 2),
-        "unit": "case-substeps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(t_conc * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded cases)",
-        "config": {"workload": f"{len(sims)} cases, grids 64^3/96x96x64/128^3/128x128x192, 1-200 spheres, "
+        "metric": f"ensemble case-substeps/s ({len(sims)} cases) " + ("mixed precision" if precision else "FP64"),
+        "value": round(n / t_fused, 2),
+        "unit": "case-substeps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": round(t_fused * 1e3 / steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "mixed" if precision else "f64", "data": "synthetic (seeded cases)",
+        "config": {"workload": f"{len(sims)} cases, grids 64^3/96x96x64/128^3/128x128x192, 8-200 spheres, "
                                "D 8-12h, Re_bulk 1000-5000",
-                   "gcell_updates_per_s": round(cells / t_conc / 1e9, 4), "side_stream": args.side_stream == "on"},
+                   "gcell_updates_per_s": round(cells / t_fused / 1e9, 4),
+                   "merged_launches_per_substep": nl, "persistent_launches": npers, "tuner_substeps": tuned},
+        "concurrent_contexts_case_substeps_per_s": round(n / t_conc, 2),
         "sequential_case_substeps_per_s": round(n / t_seq, 2),
-        "speedup_vs_sequential": round(t_seq / t_conc, 3),
+        "speedup_vs_sequential": round(t_seq / t_fused, 3),
+        "speedup_vs_concurrent_contexts": round(t_conc / t_fused, 3),
+        "fused_stage_ms_per_substep": {k: round(v, 4) for k, v in phases.items()},
     }
+    batch.close()
     for sim in sims:
         sim.close()
     print(json.dumps(out))
@@ -306,7 +342,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=30, help="untimed substeps")
     ap.add_argument("--config", default="ch256_1000sph")
     ap.add_argument("--precision", choices=("fp64", "mixed", "both"), default="both",
-                    help="path(s) to time: both prints the mixed path's line with the FP64 path's figures beside it")
+                    help="path(s) to time: both prints the mixed path's line with the FP64 path's figures beside it "
+                         "(with --ensemble: fp64 unless mixed)")
     ap.add_argument("--profile", action="store_true",
                     help="for a profiler: initialise the case, then run --warmup and --steps substeps of the path "
                          "--precision names (fp64 or mixed) and exit; prints one line per phase")
@@ -325,8 +362,8 @@ def main():
                          "(default: on for one case, off for an ensemble)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="NCASES",
                     help="instead of the 256^3 workload, advance NCASES independent small cases (uneven grids and "
-                         "sphere counts) together and report aggregate case-substeps/s against running them one "
-                         "after another")
+                         "sphere counts) together in one batch of merged launches and report aggregate "
+                         "case-substeps/s against running them as concurrent contexts and one after another")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "

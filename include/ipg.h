@@ -193,6 +193,45 @@ int ipg_substep_bytes(ipg_ctx *ctx, double *bytes);
  * durations (ms) and kernel names, at most maxn, count in *n */
 int ipg_substep_phases(ipg_ctx *ctx, int rkiter, int maxn, double *ms, const char **names, int *n);
 
+/* ---- batched ensemble (no reference counterpart: the reference runs one
+ * case per MPI job) ----------------------------------------------------
+ * A batch advances independent cases -- contexts with their own grids, sphere
+ * counts, parameters and precision -- through each substep together: every
+ * kernel of a substep stage runs once for all cases (once per FFT length),
+ * over a merged index space in which each case owns its blocks and reads its
+ * own launch arguments; a substep of all cases is one captured hipGraph.  A
+ * case's results are those of ipg_substep on the case alone (bit for bit
+ * where ipg_substep is deterministic; the spreads' atomics order their sums
+ * freely).  The cases stay usable on their own: entry points on a case are
+ * ordered after the batch's work on it, and the batch's after theirs.  The
+ * cases must start every batch substep at the same c/dcdt buffer parity (the
+ * same number of substeps run, or an even difference), the Newton-Euler
+ * update closes each substep, and itot, jtot must have compiled FFT plans
+ * (16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512). */
+typedef struct ipg_batch ipg_batch;
+int ipg_batch_create(ipg_ctx *const *cases, int ncases, ipg_batch **out);
+int ipg_batch_destroy(ipg_batch *batch);
+/* one RK3 substep (rkiter 1..3) of every case */
+int ipg_batch_substep(ipg_batch *batch, int rkiter);
+/* finish the batch's work and check every case's device error word */
+int ipg_batch_sync(ipg_batch *batch);
+/* replay each substep from a captured graph (default on) */
+int ipg_batch_use_graphs(ipg_batch *batch, int on);
+/* The batch's tuner: times whole RK3 steps with every case's RK-stage launch
+ * shape at 16, 32 and 64 z planes per workgroup and keeps the fastest, then,
+ * for every merged launch, one workgroup per block against persistent grids of
+ * 1, 2 and 4 workgroups per CU striding over the blocks, and keeps the fastest
+ * per launch.  The timed steps advance every case (results do not depend on
+ * the launch shapes): call it at a time-step boundary (after rkiter 3);
+ * *substeps gets the number of substeps it ran (a multiple of 3). */
+int ipg_batch_autotune(ipg_batch *batch, int *substeps);
+/* merged launches per substep of stage rkiter (after its first substep or the
+ * tuner) and how many of them run as persistent grids */
+int ipg_batch_launches(ipg_batch *batch, int rkiter, int *nlaunch, int *npersistent);
+/* one eager substep with a HIP event after every merged launch: per-launch
+ * durations (ms) and stage names, at most maxn, count in *n */
+int ipg_batch_phases(ipg_batch *batch, int rkiter, int maxn, double *ms, const char **names, int *n);
+
 #ifdef __cplusplus
 }
 #endif

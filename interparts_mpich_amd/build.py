@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-HIP_SOURCES = ["ipg_api.hip", "ipg_fields.hip", "ipg_poisson.hip", "ipg_ibm.hip", "ipg_particles.hip"]
+HIP_SOURCES = ["ipg_api.hip", "ipg_fields.hip", "ipg_poisson.hip", "ipg_ibm.hip", "ipg_particles.hip", "ipg_ensemble.hip"]
 LIB = os.path.join(HERE, "libipg.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -31,9 +31,19 @@ def _run(cmd, cwd=None):
 
 
 def build_hip():
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    """Each translation unit compiled on its own (in parallel), then linked."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    jobs = []
+    for s in HIP_SOURCES:
+        obj = os.path.join(objdir, s.replace(".hip", ".o"))
+        jobs.append(([HIPCC] + cflags + ["-c", os.path.join(CSRC, s), "-o", obj], obj))
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
+        list(ex.map(lambda j: _run(j[0]), jobs))
     tmp = LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + srcs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for _, o in jobs])
     os.replace(tmp, LIB)
     return LIB
 

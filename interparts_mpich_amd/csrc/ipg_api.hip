@@ -256,6 +256,48 @@ bool ghosted_field(int field) {
 
 }  // namespace
 
+// ---- a context's part in a batch (ipg_ensemble.hip) -------------------------
+
+// Records the launches of one substep of c as the merged launches of a batch
+// will run it: the substep of RK stage rkiter that starts from buffer parity
+// par, with the Newton-Euler update closing it (no side stream).  Like a
+// capture, recording leaves the host-side state as it was.
+int ipg_batch_record(ipg_ctx *c, int rkiter, int par, IpgRec *rec) {
+  double *fc = c->f[IPG_C], *fd = c->f[IPG_DCDT];
+  const int cpar = c->cpar;
+  const bool pgs = c->p_ghosts_stale, xcs = c->xcol_stale, dcs = c->dcdt_stale, pip = c->p_in_pieces;
+  if (c->cpar != par) { std::swap(c->f[IPG_C], c->f[IPG_DCDT]); c->cpar ^= 1; }
+  c->rec = rec;
+  const int rc = substep_launches(c, rkiter, 0, false);
+  c->rec = nullptr;
+  c->f[IPG_C] = fc; c->f[IPG_DCDT] = fd; c->cpar = cpar;
+  c->p_ghosts_stale = pgs; c->xcol_stale = xcs; c->dcdt_stale = dcs; c->p_in_pieces = pip;
+  return rc;
+}
+
+// Before a merged substep: c is set up, and an update a fused substep of its
+// own left pending has run.
+int ipg_batch_enter(ipg_ctx *c) {
+  if (!check_ctx(c)) return 1;
+  if (!c->solver_ready) { ipg_set_error("batch: ipg_initsolver has not been called on a case"); return 1; }
+  if (c->pc.np > 0 && !c->markers_ready) { ipg_set_error("batch: ipg_coordsfp has not been called on a case"); return 1; }
+  if (settle(c)) return 1;
+  // the substep rewrites dcdt's interior before swapping it in as c
+  c->dcdt_stale = false;
+  return 0;
+}
+
+// After a merged substep: the host-side state its launches leave, as after a
+// graph replay of the context's own substep.
+void ipg_batch_leave(ipg_ctx *c) {
+  std::swap(c->f[IPG_C], c->f[IPG_DCDT]);
+  c->cpar ^= 1;
+  c->p_ghosts_stale = true;
+  c->xcol_stale = true;
+  c->dcdt_stale = true;
+  c->p_in_pieces = c->mixed;
+}
+
 extern "C" {
 
 int ipg_nl_formula(int itot, double lx, double radius) {
@@ -446,6 +488,7 @@ int ipg_initsolver(ipg_ctx *c) {
   if (c->solver_ready) return 0;
   if (ipg_poisson_setup(c)) return 1;
   c->solver_ready = true;
+  c->gen++;
   return 0;
 }
 
@@ -829,6 +872,7 @@ int ipg_autotune(ipg_ctx *c, int *kb_out) {
   }
   c->tune_rk_kb = best > 0 ? best : saved;
   c->dcdt_stale = false;  // the timed stages rewrote dcdt
+  c->gen++;
   for (int r = 0; r < IPG_NGRAPH; r++)
     if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
   if (kb_out) *kb_out = c->tune_rk_kb;
@@ -858,9 +902,11 @@ int ipg_set_refinement(ipg_ctx *c, int nref) {
   if (!c->mixed) { ipg_set_error("set_refinement: the context is not mixed precision"); return 1; }
   if (nref < 0 || nref > IPG_MAX_REFINE) { ipg_set_error("set_refinement: nref must be 0..2"); return 1; }
   if (fill_p(c)) return 1;
-  if (nref != c->nref)
+  if (nref != c->nref) {
+    c->gen++;
     for (int r = 0; r < IPG_NGRAPH; r++)
       if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
+  }
   c->nref = nref;
   return 0;
 }

@@ -1,0 +1,455 @@
+// ipg_ensemble.hip -- a batch of independent cases advanced through each
+// substep together (ipg_batch_* in include/ipg.h).
+//
+// A batch records one substep of every case: each launch the case's own
+// substep would issue becomes a record (its body, grid, block, dynamic LDS and
+// argument struct) tagged with its substep stage.  Records of one stage are
+// merged across cases into one launch per kernel (per FFT length, whose plans
+// are compile-time): k_batch<K> over a merged index space in which each case
+// owns a run of blocks and reads its own argument struct (ipg_batch.h).  The
+// merged launches of a substep -- about 20 for any number of cases, against
+// about 20 per case -- are captured as one hipGraph per (RK stage, buffer
+// parity).  Records are made once per key and kept on the device; a case's
+// configuration counter (ipg_ctx::gen) moving makes the batch record again.
+#include "ipg_internal.h"
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+struct IpgRecEntry {
+  BatchFns fns;
+  dim3 grid, block;
+  size_t lds;
+  std::vector<unsigned char> args;
+  std::string stage;
+};
+struct IpgRec {
+  std::vector<IpgRecEntry> e;
+};
+
+int ipg_rec_add(IpgRec *r, BatchFns fns, dim3 grid, dim3 block, size_t lds, const void *args, size_t argsz,
+                const char *stage) {
+  const unsigned char *a = (const unsigned char *)args;
+  r->e.push_back(IpgRecEntry{fns, grid, block, lds, std::vector<unsigned char>(a, a + argsz), stage ? stage : "?"});
+  return 0;
+}
+
+namespace {
+
+constexpr int NKEY = 6;  // (rkiter - 1) + 3 parity
+
+// One merged launch: the cases holding a record of one stage with one body.
+struct Merged {
+  BatchFns fns;
+  dim3 block;
+  size_t lds;
+  int total;            // merged blocks (a multiple of 8)
+  int ncase;
+  size_t off_args, off_seg, off_of;  // in the plan's device table
+  std::string stage;    // stage#occurrence
+  std::string tune_key; // stage#occurrence|body: the tuner's choice is kept per key
+};
+
+struct Plan {
+  std::vector<Merged> L;
+  void *dev = nullptr;  // argument structs, segments and block->segment tables of every launch
+  hipGraphExec_t graph = nullptr;
+};
+
+std::string hexptr(const void *p) {
+  char b[32];
+  std::snprintf(b, sizeof b, "%p", p);
+  return b;
+}
+
+}  // namespace
+
+struct ipg_batch {
+  std::vector<ipg_ctx *> cs;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<hipEvent_t> case_ev;
+  std::vector<hipEvent_t> tev;  // timing events
+  bool use_graphs = true;
+  int ncu = 256;
+  Plan plan[NKEY];
+  std::map<std::string, int> persist;  // tune_key -> persistent workgroups per CU (0: one per block)
+  int persist_default = 0;             // for a launch the tuner has not seen
+  std::map<const void *, size_t> lds_set;  // the dynamic-LDS limit each merged kernel has been given
+  std::vector<unsigned> gen;           // the cases' configuration counters the plans were recorded at
+};
+
+namespace {
+
+void drop_plan(Plan &P) {
+  if (P.graph) (void)hipGraphExecDestroy(P.graph);
+  if (P.dev) (void)hipFree(P.dev);
+  P = Plan{};
+}
+
+// true when no case has changed its configuration since the plans were recorded
+bool gen_current(const ipg_batch *b) {
+  if (b->gen.size() != b->cs.size()) return false;
+  for (size_t i = 0; i < b->cs.size(); i++)
+    if (b->gen[i] != b->cs[i]->gen) return false;
+  return true;
+}
+
+// Records one substep of every case for key r and merges the records.
+int build_plan(ipg_batch *b, int r) {
+  const int rkiter = r % 3 + 1, par = r / 3;
+  const int n = (int)b->cs.size();
+  drop_plan(b->plan[r]);
+  std::vector<IpgRec> rec(n);
+  for (int i = 0; i < n; i++)
+    if (ipg_batch_record(b->cs[i], rkiter, par, &rec[i])) return 1;
+  // stage#occurrence keys per case; the longest sequence gives the order, and
+  // every other case's sequence must be a subsequence of it (a case without
+  // spheres has no IBM or particle stages)
+  std::vector<std::map<std::string, int>> at(n);
+  std::vector<std::vector<std::string>> keys(n);
+  int ref = 0;
+  for (int i = 0; i < n; i++) {
+    std::map<std::string, int> occ;
+    for (size_t j = 0; j < rec[i].e.size(); j++) {
+      const std::string k = rec[i].e[j].stage + "#" + std::to_string(occ[rec[i].e[j].stage]++);
+      keys[i].push_back(k);
+      at[i][k] = (int)j;
+    }
+    if (keys[i].size() > keys[ref].size()) ref = i;
+  }
+  std::map<std::string, int> pos;
+  for (size_t j = 0; j < keys[ref].size(); j++) pos[keys[ref][j]] = (int)j;
+  for (int i = 0; i < n; i++) {
+    int last = -1;
+    for (const std::string &k : keys[i]) {
+      auto it = pos.find(k);
+      if (it == pos.end() || it->second <= last) {
+        ipg_set_error("batch: the cases' substeps do not run the same stages in the same order (" + k + ")");
+        return 1;
+      }
+      last = it->second;
+    }
+  }
+  Plan &P = b->plan[r];
+  std::vector<unsigned char> blob;
+  auto align = [&](size_t a) { blob.resize((blob.size() + a - 1) / a * a); };
+  auto put = [&](const void *p, size_t nb) {
+    const unsigned char *q = (const unsigned char *)p;
+    blob.insert(blob.end(), q, q + nb);
+  };
+  for (const std::string &k : keys[ref]) {
+    // the cases holding stage k, grouped by body (kernel), in case order
+    std::vector<std::pair<const void *, std::vector<int>>> groups;
+    for (int i = 0; i < n; i++) {
+      auto it = at[i].find(k);
+      if (it == at[i].end()) continue;
+      const void *fn = rec[i].e[it->second].fns.one;
+      auto g = std::find_if(groups.begin(), groups.end(), [&](const auto &x) { return x.first == fn; });
+      if (g == groups.end()) groups.push_back({fn, {i}});
+      else g->second.push_back(i);
+    }
+    for (const auto &grp : groups) {
+      const IpgRecEntry &e0 = rec[grp.second[0]].e[at[grp.second[0]][k]];
+      Merged M{};
+      M.fns = e0.fns;
+      M.block = e0.block;
+      M.lds = 0;
+      M.ncase = (int)grp.second.size();
+      M.stage = k;
+      M.tune_key = k + "|" + hexptr(e0.fns.one);
+      const size_t argsz = e0.args.size();
+      align(16);
+      M.off_args = blob.size();
+      for (int i : grp.second) {
+        const IpgRecEntry &e = rec[i].e[at[i][k]];
+        if (e.args.size() != argsz || e.block.x != M.block.x || e.block.y != M.block.y || e.block.z != M.block.z) {
+          ipg_set_error("batch: records of one kernel differ in their block or argument size");
+          return 1;
+        }
+        put(e.args.data(), argsz);
+        M.lds = std::max(M.lds, e.lds);
+      }
+      align(16);
+      M.off_seg = blob.size();
+      long running = 0;
+      std::vector<int> of;
+      for (int s = 0; s < M.ncase; s++) {
+        const IpgRecEntry &e = rec[grp.second[s]].e[at[grp.second[s]][k]];
+        const BatchSeg sg{(int)running, (int)e.grid.x, (int)e.grid.y, (int)e.grid.z};
+        put(&sg, sizeof sg);
+        const long cnt = (long)e.grid.x * e.grid.y * e.grid.z;
+        const long run8 = (cnt + 7) / 8 * 8;
+        for (long t = 0; t < run8 / 8; t++) of.push_back(s);
+        running += run8;
+        if (running >= (1L << 30)) { ipg_set_error("batch: a merged launch of more than 2^30 blocks"); return 1; }
+      }
+      M.total = (int)running;
+      align(16);
+      M.off_of = blob.size();
+      put(of.data(), of.size() * sizeof(int));
+      if (M.lds > 0)
+        for (const void *fn : {M.fns.one, M.fns.persist})
+          if (b->lds_set[fn] < M.lds) {
+            IPG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)M.lds));
+            b->lds_set[fn] = M.lds;
+          }
+      P.L.push_back(M);
+    }
+  }
+  IPG_HIP(hipMalloc(&P.dev, std::max<size_t>(blob.size(), 16)));
+  IPG_HIP(hipMemcpy(P.dev, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// The merged launches of plan P on the batch's stream; with timing, an event
+// after each (b->tev[q + 1], b->tev[0] before the first).
+int run_plan(ipg_batch *b, Plan &P, bool timing) {
+  if (timing) {
+    while (b->tev.size() < P.L.size() + 1) {
+      hipEvent_t e;
+      IPG_HIP(hipEventCreate(&e));
+      b->tev.push_back(e);
+    }
+    IPG_HIP(hipEventRecord(b->tev[0], b->stream));
+  }
+  for (size_t q = 0; q < P.L.size(); q++) {
+    const Merged &M = P.L[q];
+    const char *base = (const char *)P.dev;
+    const void *args = base + M.off_args;
+    BatchMap m{(const BatchSeg *)(base + M.off_seg), (const int *)(base + M.off_of), M.total};
+    auto it = b->persist.find(M.tune_key);
+    const int f = it == b->persist.end() ? b->persist_default : it->second;
+    unsigned grid = (unsigned)M.total;
+    const void *fn = M.fns.one;
+    if (f > 0) {
+      const long cap = std::max<long>(8, (long)f * b->ncu / 8 * 8);
+      if (cap < M.total) { grid = (unsigned)cap; fn = M.fns.persist; }
+    }
+    void *kp[] = {(void *)&args, (void *)&m};
+    IPG_HIP(hipLaunchKernel(fn, dim3(grid), M.block, kp, M.lds, b->stream));
+    if (timing) IPG_HIP(hipEventRecord(b->tev[q + 1], b->stream));
+  }
+  return 0;
+}
+
+int ensure_plan(ipg_batch *b, int r) {
+  if (!gen_current(b)) {
+    // a case changed: every key records again
+    IPG_HIP(hipStreamSynchronize(b->stream));
+    for (int q = 0; q < NKEY; q++) drop_plan(b->plan[q]);
+    b->gen.clear();
+    for (ipg_ctx *c : b->cs) b->gen.push_back(c->gen);
+  }
+  if (b->plan[r].dev) return 0;
+  IPG_HIP(hipStreamSynchronize(b->stream));
+  return build_plan(b, r);
+}
+
+int capture(ipg_batch *b, Plan &P) {
+  hipGraph_t graph;
+  IPG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = run_plan(b, P, false);
+  const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+  if (rc) return rc;
+  if (e != hipSuccess) { ipg_set_error(std::string("batch graph capture: ") + hipGetErrorString(e)); return 1; }
+  IPG_HIP(hipGraphInstantiate(&P.graph, graph, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(graph);
+  return 0;
+}
+
+void drop_graphs(ipg_batch *b) {
+  for (Plan &P : b->plan)
+    if (P.graph) { (void)hipGraphExecDestroy(P.graph); P.graph = nullptr; }
+}
+
+// One substep of every case; timing: eager, with an event after every merged
+// launch (the plan of the substep is returned in *plan).
+int substep(ipg_batch *b, int rkiter, bool timing, Plan **plan) {
+  if (rkiter < 1 || rkiter > 3) { ipg_set_error("batch: rkiter must be 1..3"); return 1; }
+  const int n = (int)b->cs.size();
+  for (int i = 0; i < n; i++)
+    if (ipg_batch_enter(b->cs[i])) return 1;
+  const int par = b->cs[0]->cpar;
+  for (int i = 1; i < n; i++)
+    if (b->cs[i]->cpar != par) {
+      ipg_set_error("batch: the cases are at different c/dcdt buffer parities (an odd difference in substeps run)");
+      return 1;
+    }
+  const int r = (rkiter - 1) + 3 * par;
+  if (ensure_plan(b, r)) return 1;
+  Plan &P = b->plan[r];
+  // the batch's work on a case follows the case's own
+  for (int i = 0; i < n; i++) {
+    IPG_HIP(hipEventRecord(b->case_ev[i], b->cs[i]->stream));
+    IPG_HIP(hipStreamWaitEvent(b->stream, b->case_ev[i], 0));
+  }
+  if (b->use_graphs && !timing) {
+    if (!P.graph && capture(b, P)) return 1;
+    IPG_HIP(hipGraphLaunch(P.graph, b->stream));
+  } else if (run_plan(b, P, timing)) {
+    return 1;
+  }
+  for (int i = 0; i < n; i++) ipg_batch_leave(b->cs[i]);
+  // and the case's later work follows the batch's
+  IPG_HIP(hipEventRecord(b->done, b->stream));
+  for (int i = 0; i < n; i++) IPG_HIP(hipStreamWaitEvent(b->cs[i]->stream, b->done, 0));
+  if (plan) *plan = &P;
+  return 0;
+}
+
+// Per-launch times (ms) of the last timed substep of plan P.
+int launch_times(ipg_batch *b, const Plan &P, std::vector<float> &t) {
+  IPG_HIP(hipEventSynchronize(b->tev[P.L.size()]));
+  t.assign(P.L.size(), 0.f);
+  for (size_t q = 0; q < P.L.size(); q++) IPG_HIP(hipEventElapsedTime(&t[q], b->tev[q], b->tev[q + 1]));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ipg_batch_create(ipg_ctx *const *cases, int ncases, ipg_batch **out) {
+  if (!cases || ncases < 1 || !out) { ipg_set_error("batch_create: bad arguments"); return 1; }
+  *out = nullptr;
+  for (int i = 0; i < ncases; i++)
+    for (int j = 0; j < i; j++)
+      if (!cases[i] || cases[i] == cases[j]) { ipg_set_error("batch_create: a null or repeated case"); return 1; }
+  ipg_batch *b = new ipg_batch();
+  b->cs.assign(cases, cases + ncases);
+  auto fail = [&]() { ipg_batch_destroy(b); return 1; };
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    ipg_set_error("batch_create: device query failed");
+    return fail();
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+    ipg_set_error("batch_create: stream or event create failed");
+    return fail();
+  }
+  b->case_ev.assign(ncases, nullptr);
+  for (int i = 0; i < ncases; i++)
+    if (hipEventCreateWithFlags(&b->case_ev[i], hipEventDisableTiming) != hipSuccess) {
+      ipg_set_error("batch_create: event create failed");
+      return fail();
+    }
+  *out = b;
+  return 0;
+}
+
+int ipg_batch_destroy(ipg_batch *b) {
+  if (!b) return 0;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (Plan &P : b->plan) drop_plan(P);
+  for (hipEvent_t e : b->case_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->tev) (void)hipEventDestroy(e);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int ipg_batch_substep(ipg_batch *b, int rkiter) {
+  if (!b) { ipg_set_error("batch: null batch"); return 1; }
+  return substep(b, rkiter, false, nullptr);
+}
+
+int ipg_batch_sync(ipg_batch *b) {
+  if (!b) { ipg_set_error("batch: null batch"); return 1; }
+  IPG_HIP(hipStreamSynchronize(b->stream));
+  for (ipg_ctx *c : b->cs)
+    if (ipg_sync(c)) return 3;
+  return 0;
+}
+
+int ipg_batch_use_graphs(ipg_batch *b, int on) {
+  if (!b) { ipg_set_error("batch: null batch"); return 1; }
+  b->use_graphs = on != 0;
+  return 0;
+}
+
+int ipg_batch_autotune(ipg_batch *b, int *substeps) {
+  if (!b) { ipg_set_error("batch: null batch"); return 1; }
+  int count = 0;
+  std::vector<float> t;
+  // the RK stage's z planes per workgroup, the same for every case, timed
+  // over one RK3 step each
+  int best_kb = 0;
+  float best_ms = 0.f;
+  for (int kb = 16; kb <= 64; kb *= 2) {
+    for (ipg_ctx *c : b->cs) {
+      const int k = std::min(kb, c->g.nz);
+      if (c->tune_rk_kb != k) { c->tune_rk_kb = k; c->gen++; }
+    }
+    float ms = 0.f;
+    for (int rk = 1; rk <= 3; rk++) {
+      Plan *P = nullptr;
+      if (substep(b, rk, true, &P) || launch_times(b, *P, t)) return 1;
+      count++;
+      for (size_t q = 0; q < P->L.size(); q++)
+        if (P->L[q].stage.rfind("rk_stage#", 0) == 0) ms += t[q];
+    }
+    if (best_kb == 0 || ms < best_ms) { best_kb = kb; best_ms = ms; }
+  }
+  for (ipg_ctx *c : b->cs) {
+    const int k = std::min(best_kb, c->g.nz);
+    if (c->tune_rk_kb != k) { c->tune_rk_kb = k; c->gen++; }
+  }
+  // per merged launch: one workgroup per block, or a persistent grid of f
+  // workgroups per CU
+  std::map<std::string, std::pair<int, float>> best;  // tune_key -> (f, ms)
+  b->persist.clear();
+  for (int f : {0, 1, 2, 4}) {
+    b->persist_default = f;
+    for (int rk = 1; rk <= 3; rk++) {
+      Plan *P = nullptr;
+      if (substep(b, rk, true, &P) || launch_times(b, *P, t)) return 1;
+      count++;
+      for (size_t q = 0; q < P->L.size(); q++) {
+        const std::string &key = P->L[q].tune_key;
+        auto it = best.find(key);
+        if (it == best.end() || t[q] < it->second.second) best[key] = {f, t[q]};
+      }
+    }
+  }
+  b->persist_default = 0;
+  for (const auto &kv : best) b->persist[kv.first] = kv.second.first;
+  drop_graphs(b);
+  if (substeps) *substeps = count;
+  return 0;
+}
+
+int ipg_batch_launches(ipg_batch *b, int rkiter, int *nlaunch, int *npersistent) {
+  if (!b || !nlaunch || rkiter < 1 || rkiter > 3) { ipg_set_error("batch_launches: bad arguments"); return 1; }
+  const int par = b->cs[0]->cpar;
+  const Plan &P = b->plan[(rkiter - 1) + 3 * par];
+  *nlaunch = (int)P.L.size();
+  int np = 0;
+  for (const Merged &M : P.L) {
+    auto it = b->persist.find(M.tune_key);
+    const int f = it == b->persist.end() ? b->persist_default : it->second;
+    if (f > 0 && std::max<long>(8, (long)f * b->ncu / 8 * 8) < M.total) np++;
+  }
+  if (npersistent) *npersistent = np;
+  return 0;
+}
+
+int ipg_batch_phases(ipg_batch *b, int rkiter, int maxn, double *ms, const char **names, int *n) {
+  if (!b || !ms || !names || !n) { ipg_set_error("batch_phases: bad arguments"); return 1; }
+  Plan *P = nullptr;
+  std::vector<float> t;
+  if (substep(b, rkiter, true, &P) || launch_times(b, *P, t)) return 1;
+  int k = 0;
+  for (size_t q = 0; q < P->L.size() && k < maxn; q++, k++) {
+    ms[k] = t[q];
+    names[k] = P->L[q].stage.c_str();
+  }
+  *n = k;
+  return 0;
+}
+
+}  // extern "C"

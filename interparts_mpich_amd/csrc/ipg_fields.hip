@@ -36,16 +36,16 @@ constexpr int NLD = (LPL + RNT - 1) / RNT;
 // term is still computed in FP64 (the kernel is bound by HBM traffic, not by
 // its FP64 arithmetic).
 template <int STAGE, typename RT>
-__global__ __launch_bounds__(RNT) void k_rk(Grid g, const double *__restrict__ U, const double *__restrict__ V,
-                                            const double *__restrict__ W, const double *__restrict__ C,
-                                            const double *__restrict__ P, double *__restrict__ dudt,
-                                            double *__restrict__ dvdt, double *__restrict__ dwdt,
-                                            double *__restrict__ dcdt, RT *__restrict__ durk,
-                                            RT *__restrict__ dvrk, RT *__restrict__ dwrk,
-                                            RT *__restrict__ dcrk, const double *__restrict__ dtp, int ntx,
-                                            int nty, int kb, int wrapx) {
+__device__ __forceinline__ void rk_body(const Grid g, const double *__restrict__ U, const double *__restrict__ V,
+                                        const double *__restrict__ W, const double *__restrict__ C,
+                                        const double *__restrict__ P, double *__restrict__ dudt,
+                                        double *__restrict__ dvdt, double *__restrict__ dwdt,
+                                        double *__restrict__ dcdt, RT *__restrict__ durk,
+                                        RT *__restrict__ dvrk, RT *__restrict__ dwrk,
+                                        RT *__restrict__ dcrk, const double *__restrict__ dtp, int ntx,
+                                        int nty, int kb, int wrapx, const VB vb) {
   __shared__ double sm[2][5][LPL];
-  const int nblk = gridDim.x, L = blockIdx.x;
+  const int nblk = vb.gx, L = vb.x;
   const int t = (nblk % 8 == 0) ? (L % 8) * (nblk / 8) + L / 8 : L;
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
   const int i0 = tx * RTX + 1, j0 = ty * RTY + 1;
@@ -207,6 +207,27 @@ __global__ __launch_bounds__(RNT) void k_rk(Grid g, const double *__restrict__ U
   }
 }
 
+template <typename RT>
+struct RkArgs {
+  Grid g;
+  const double *U, *V, *W, *C, *P;
+  double *dudt, *dvdt, *dwdt, *dcdt;
+  RT *durk, *dvrk, *dwrk, *dcrk;
+  const double *dtp;
+  int ntx, nty, kb, wrapx;
+};
+template <int STAGE, typename RT>
+struct RkK {
+  using Args = RkArgs<RT>;
+  static constexpr int THREADS = RNT, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) {
+    rk_body<STAGE, RT>(a.g, a.U, a.V, a.W, a.C, a.P, a.dudt, a.dvdt, a.dwdt, a.dcdt, a.durk, a.dvrk, a.dwrk, a.dcrk,
+                       a.dtp, a.ntx, a.nty, a.kb, a.wrapx, v);
+  }
+};
+template <int STAGE, typename RT>
+__global__ __launch_bounds__(RNT) void k_rk(RkArgs<RT>) { run_single<RkK<STAGE, RT>>(); }
+
 // Ghost fill for up to five fields in one launch: src/bound.f90:11-80
 // (bounduvw), 82-107 (boundp), 109-171 (boundscal) followed by updthalos
 // (173-221) on one rank.  Each ghost takes its final value directly: the z
@@ -219,11 +240,12 @@ enum { KIND_UV = 0, KIND_W = 1, KIND_P = 2, KIND_C = 3 };
 // reads them.
 struct BoundJob { double *f[5]; int kind[5]; int nf; int xcol; };
 
-__global__ void k_bound(Grid g, BoundJob job) {
+constexpr int BND_THREADS = 256;
+__device__ __forceinline__ void bound_body(const Grid &g, const BoundJob &job, const VB vb) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int nplane = (nx + 2) * (ny + 2), nrow = (nx + 2) * nz, ncol = job.xcol ? ny * nz : 0, nwall = nx * ny;
   const int total = 2 * nplane + 2 * nrow + 2 * ncol + nwall;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (int t = vb.x * BND_THREADS + threadIdx.x; t < total; t += vb.gx * BND_THREADS) {
     int i, j, k; bool wallplane = false;
     int r = t;
     if (r < 2 * nplane) { k = r < nplane ? 0 : nz + 1; r = r < nplane ? r : r - nplane; i = r % (nx + 2); j = r / (nx + 2); }
@@ -263,6 +285,14 @@ __global__ void k_bound(Grid g, BoundJob job) {
   }
 }
 
+struct BoundArgs { Grid g; BoundJob job; };
+struct BoundK {
+  using Args = BoundArgs;
+  static constexpr int THREADS = BND_THREADS, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) { bound_body(a.g, a.job, v); }
+};
+__global__ __launch_bounds__(BND_THREADS) void k_bound(BoundArgs) { run_single<BoundK>(); }
+
 // src/fillps.f90:8-53
 __global__ __launch_bounds__(BX *BY) void k_fillps(Grid g, const double *__restrict__ dudt, const double *__restrict__ dvdt,
                                                    const double *__restrict__ dwdt, double *__restrict__ p,
@@ -285,12 +315,12 @@ __global__ __launch_bounds__(BX *BY) void k_fillps(Grid g, const double *__restr
 // measured: the scattered x-ghost writes and the divergent boundary lanes
 // cost as much as the separate ghost-fill launch.)
 template <bool FUSED>
-__global__ __launch_bounds__(BX *BY) void k_correc(Grid g, const double *__restrict__ dudt, const double *__restrict__ dvdt,
-                                                   const double *__restrict__ dwdt, const double *__restrict__ p,
-                                                   double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
-                                                   double *__restrict__ pnew, const double *__restrict__ dcdt,
-                                                   double *__restrict__ cnew, const double *__restrict__ dtp, double rkab) {
-  const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+__device__ __forceinline__ void correc_body(const Grid g, const double *__restrict__ dudt, const double *__restrict__ dvdt,
+                                            const double *__restrict__ dwdt, const double *__restrict__ p,
+                                            double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+                                            double *__restrict__ pnew, const double *__restrict__ dtp, double rkab,
+                                            const VB vb) {
+  const int i = vb.x * BX + threadIdx.x + 1, j = vb.y * BY + threadIdx.y + 1, k = vb.z + 1;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny) return;
   const double factor = rkab * *dtp;
@@ -305,6 +335,24 @@ __global__ __launch_bounds__(BX *BY) void k_correc(Grid g, const double *__restr
   w[c] = dwdt[c] - fk * (pk1 - pc);
   if (FUSED) pnew[c] = pnew[c] + pc;
 }
+
+struct CorrecArgs {
+  Grid g;
+  const double *dudt, *dvdt, *dwdt, *p;
+  double *u, *v, *w, *pnew;
+  const double *dtp;
+  double rkab;
+};
+template <bool FUSED>
+struct CorrecK {
+  using Args = CorrecArgs;
+  static constexpr int THREADS = BX * BY, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) {
+    correc_body<FUSED>(a.g, a.dudt, a.dvdt, a.dwdt, a.p, a.u, a.v, a.w, a.pnew, a.dtp, a.rkab, v);
+  }
+};
+template <bool FUSED>
+__global__ __launch_bounds__(BX *BY) void k_correc(CorrecArgs) { run_single<CorrecK<FUSED>>(); }
 
 // The mixed path's pressure correction is the sum of NP FP32 pieces (the
 // FP32-stored solve and its FP64-residual refinements), each exact in FP64:
@@ -321,11 +369,12 @@ __device__ __forceinline__ double psum(const Pieces &P, long c) {
 // k_correc<true> with p read as the sum of its pieces (src/correc.f90:11-42,
 // src/main.f90:370), the same periodic and top-wall source values.
 template <int NP>
-__global__ __launch_bounds__(BX *BY) void k_correc_pieces(Grid g, const double *__restrict__ dudt, const double *__restrict__ dvdt,
-                                                          const double *__restrict__ dwdt, Pieces P, double *__restrict__ u,
-                                                          double *__restrict__ v, double *__restrict__ w,
-                                                          double *__restrict__ pnew, const double *__restrict__ dtp, double rkab) {
-  const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1, k = blockIdx.z + 1;
+__device__ __forceinline__ void correc_pieces_body(const Grid g, const double *__restrict__ dudt,
+                                                   const double *__restrict__ dvdt, const double *__restrict__ dwdt,
+                                                   const Pieces P, double *__restrict__ u, double *__restrict__ v,
+                                                   double *__restrict__ w, double *__restrict__ pnew,
+                                                   const double *__restrict__ dtp, double rkab, const VB vb) {
+  const int i = vb.x * BX + threadIdx.x + 1, j = vb.y * BY + threadIdx.y + 1, k = vb.z + 1;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny) return;
   const double factor = rkab * *dtp;
@@ -341,6 +390,25 @@ __global__ __launch_bounds__(BX *BY) void k_correc_pieces(Grid g, const double *
   pnew[c] = pnew[c] + pc;
 }
 
+struct CorrecPiecesArgs {
+  Grid g;
+  const double *dudt, *dvdt, *dwdt;
+  Pieces P;
+  double *u, *v, *w, *pnew;
+  const double *dtp;
+  double rkab;
+};
+template <int NP>
+struct CorrecPiecesK {
+  using Args = CorrecPiecesArgs;
+  static constexpr int THREADS = BX * BY, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) {
+    correc_pieces_body<NP>(a.g, a.dudt, a.dvdt, a.dwdt, a.P, a.u, a.v, a.w, a.pnew, a.dtp, a.rkab, v);
+  }
+};
+template <int NP>
+__global__ __launch_bounds__(BX *BY) void k_correc_pieces(CorrecPiecesArgs) { run_single<CorrecPiecesK<NP>>(); }
+
 // The mixed path's refinement residual r = rhs - L p, p the sum of its NP
 // pieces so far, in FP64, stored FP32.  L is the operator the Poisson solve
 // inverts: the periodic 3-point second differences in x and y, and in z the
@@ -352,9 +420,10 @@ __global__ __launch_bounds__(BX *BY) void k_correc_pieces(Grid g, const double *
 // from HBM once; the in-plane neighbours come from the caches.
 constexpr int RZ = 16;
 template <int NP>
-__global__ __launch_bounds__(BX *BY) void k_resid(Grid g, Pieces P, const double *__restrict__ rhs, float *__restrict__ r) {
-  const int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1;
-  const int k0 = blockIdx.z * RZ + 1, k1 = min(k0 + RZ - 1, g.nz);
+__device__ __forceinline__ void resid_body(const Grid g, const Pieces P, const double *__restrict__ rhs,
+                                           float *__restrict__ r, const VB vb) {
+  const int i = vb.x * BX + threadIdx.x + 1, j = vb.y * BY + threadIdx.y + 1;
+  const int k0 = vb.z * RZ + 1, k1 = min(k0 + RZ - 1, g.nz);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (i > nx || j > ny) return;
   const long ce = i == nx ? -(long)(nx - 1) : 1, cw = i == 1 ? (long)(nx - 1) : -1;
@@ -373,6 +442,16 @@ __global__ __launch_bounds__(BX *BY) void k_resid(Grid g, Pieces P, const double
     pc = pu;
   }
 }
+
+struct ResidArgs { Grid g; Pieces P; const double *rhs; float *r; };
+template <int NP>
+struct ResidK {
+  using Args = ResidArgs;
+  static constexpr int THREADS = BX * BY, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) { resid_body<NP>(a.g, a.P, a.rhs, a.r, v); }
+};
+template <int NP>
+__global__ __launch_bounds__(BX *BY) void k_resid(ResidArgs) { run_single<ResidK<NP>>(); }
 
 // p = the sum of its pieces on the interior (before p is read as a field)
 template <int NP>
@@ -577,27 +656,25 @@ int ipg_launch_rk(ipg_ctx *c, int rkiter, bool wrapx) {
     while (kb > 4 && (long)ntx * nty * ((g.nz + kb - 1) / kb) < 2048) kb /= 2;
   }
   const int ntz = (g.nz + kb - 1) / kb;
-  dim3 grid(ntx * nty * ntz), blk(RNT);
-#define IPG_RK_LAUNCH(S, RT, RK)                                                                                       \
-  hipLaunchKernelGGL((k_rk<S, RT>), grid, blk, 0, c->stream, g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW],   \
-                     f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT], f[IPG_DCDT], RK(IPG_DURK), RK(IPG_DVRK), RK(IPG_DWRK),     \
-                     RK(IPG_DCRK), c->dscal, ntx, nty, kb, wrapx ? 1 : 0)
-#define RK64(q) f[q]
-#define RK32(q) f32ptr(c, q)
+  const dim3 grid(ntx * nty * ntz), blk(RNT);
+  const int wx = wrapx ? 1 : 0;
+  int rc;
   if (c->mixed) {
-    if (rkiter == 1) IPG_RK_LAUNCH(1, float, RK32);
-    else if (rkiter == 2) IPG_RK_LAUNCH(2, float, RK32);
-    else IPG_RK_LAUNCH(3, float, RK32);
+    float *r[4] = {f32ptr(c, IPG_DURK), f32ptr(c, IPG_DVRK), f32ptr(c, IPG_DWRK), f32ptr(c, IPG_DCRK)};
+    const RkArgs<float> a{g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW], f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT],
+                          f[IPG_DCDT], r[0], r[1], r[2], r[3], c->dscal, ntx, nty, kb, wx};
+    if (rkiter == 1) rc = ipg_go<RkK<1, float>>(c, k_rk<1, float>, grid, blk, 0, a, "rk_stage");
+    else if (rkiter == 2) rc = ipg_go<RkK<2, float>>(c, k_rk<2, float>, grid, blk, 0, a, "rk_stage");
+    else rc = ipg_go<RkK<3, float>>(c, k_rk<3, float>, grid, blk, 0, a, "rk_stage");
   } else {
-    if (rkiter == 1) IPG_RK_LAUNCH(1, double, RK64);
-    else if (rkiter == 2) IPG_RK_LAUNCH(2, double, RK64);
-    else IPG_RK_LAUNCH(3, double, RK64);
+    const RkArgs<double> a{g, f[IPG_U], f[IPG_V], f[IPG_W], f[IPG_C], f[IPG_PNEW], f[IPG_DUDT], f[IPG_DVDT], f[IPG_DWDT],
+                           f[IPG_DCDT], f[IPG_DURK], f[IPG_DVRK], f[IPG_DWRK], f[IPG_DCRK], c->dscal, ntx, nty, kb, wx};
+    if (rkiter == 1) rc = ipg_go<RkK<1, double>>(c, k_rk<1, double>, grid, blk, 0, a, "rk_stage");
+    else if (rkiter == 2) rc = ipg_go<RkK<2, double>>(c, k_rk<2, double>, grid, blk, 0, a, "rk_stage");
+    else rc = ipg_go<RkK<3, double>>(c, k_rk<3, double>, grid, blk, 0, a, "rk_stage");
   }
-#undef RK64
-#undef RK32
-#undef IPG_RK_LAUNCH
+  if (rc) return rc;
   IPG_HIP(hipGetLastError());
-  ipg_mark(c, "rk_stage");
   return 0;
 }
 
@@ -622,9 +699,8 @@ int ipg_launch_bound(ipg_ctx *c, int mask) {
   if (total > 2000000000L) { ipg_set_error("bound: grid too large"); return 1; }
   int blocks = (int)((total + 255) / 256);
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_bound, dim3(blocks), dim3(256), 0, c->stream, g, job);
+  if (ipg_go<BoundK>(c, k_bound, dim3(blocks), dim3(BND_THREADS), 0, BoundArgs{g, job}, "bound")) return 1;
   IPG_HIP(hipGetLastError());
-  ipg_mark(c, "bound");
   return 0;
 }
 
@@ -639,11 +715,10 @@ int ipg_launch_fillps(ipg_ctx *c, int rkiter) {
 
 int ipg_launch_correc(ipg_ctx *c, int rkiter) {
   const Grid &g = c->g;
-  hipLaunchKernelGGL(k_correc<false>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT],
-                     c->f[IPG_DWDT], c->f[IPG_P], c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->f[IPG_DCDT],
-                     c->f[IPG_C], c->dscal, c->pc.rkab[rkiter - 1]);
+  const CorrecArgs a{g, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT], c->f[IPG_P], c->f[IPG_U], c->f[IPG_V],
+                     c->f[IPG_W], c->f[IPG_PNEW], c->dscal, c->pc.rkab[rkiter - 1]};
+  if (ipg_go<CorrecK<false>>(c, k_correc<false>, cell_grid(g), dim3(BX, BY), 0, a, "correc")) return 1;
   IPG_HIP(hipGetLastError());
-  ipg_mark(c, "correc");
   return 0;
 }
 
@@ -669,34 +744,30 @@ int ipg_launch_resid(ipg_ctx *c, int npieces) {
   const Pieces P = pieces_of(c);
   float *r = c->pp[IPG_MAX_REFINE + 1];
   const dim3 grid((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (g.nz + RZ - 1) / RZ);
-  if (npieces == 1) hipLaunchKernelGGL(k_resid<1>, grid, dim3(BX, BY), 0, c->stream, g, P, c->f[IPG_P], r);
-  else hipLaunchKernelGGL(k_resid<2>, grid, dim3(BX, BY), 0, c->stream, g, P, c->f[IPG_P], r);
+  const ResidArgs a{g, P, c->f[IPG_P], r};
+  const int rc = npieces == 1 ? ipg_go<ResidK<1>>(c, k_resid<1>, grid, dim3(BX, BY), 0, a, "resid")
+                              : ipg_go<ResidK<2>>(c, k_resid<2>, grid, dim3(BX, BY), 0, a, "resid");
+  if (rc) return rc;
   IPG_HIP(hipGetLastError());
-  ipg_mark(c, "resid");
   return 0;
 }
 
 int ipg_launch_correc_fused(ipg_ctx *c, int rkiter) {
   const Grid &g = c->g;
+  int rc;
   if (c->mixed) {
-    const Pieces P = pieces_of(c);
-#define IPG_CORREC_PIECES(NP)                                                                                          \
-  hipLaunchKernelGGL(k_correc_pieces<NP>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT], \
-                     c->f[IPG_DWDT], P, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->dscal,              \
-                     c->pc.rkab[rkiter - 1])
-    if (c->nref == 0) IPG_CORREC_PIECES(1);
-    else if (c->nref == 1) IPG_CORREC_PIECES(2);
-    else IPG_CORREC_PIECES(3);
-#undef IPG_CORREC_PIECES
-    IPG_HIP(hipGetLastError());
-    ipg_mark(c, "correc");
-    return 0;
+    const CorrecPiecesArgs a{g, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT], pieces_of(c), c->f[IPG_U], c->f[IPG_V],
+                             c->f[IPG_W], c->f[IPG_PNEW], c->dscal, c->pc.rkab[rkiter - 1]};
+    if (c->nref == 0) rc = ipg_go<CorrecPiecesK<1>>(c, k_correc_pieces<1>, cell_grid(g), dim3(BX, BY), 0, a, "correc");
+    else if (c->nref == 1) rc = ipg_go<CorrecPiecesK<2>>(c, k_correc_pieces<2>, cell_grid(g), dim3(BX, BY), 0, a, "correc");
+    else rc = ipg_go<CorrecPiecesK<3>>(c, k_correc_pieces<3>, cell_grid(g), dim3(BX, BY), 0, a, "correc");
+  } else {
+    const CorrecArgs a{g, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT], c->f[IPG_P], c->f[IPG_U], c->f[IPG_V],
+                       c->f[IPG_W], c->f[IPG_PNEW], c->dscal, c->pc.rkab[rkiter - 1]};
+    rc = ipg_go<CorrecK<true>>(c, k_correc<true>, cell_grid(g), dim3(BX, BY), 0, a, "correc");
   }
-  hipLaunchKernelGGL(k_correc<true>, cell_grid(g), dim3(BX, BY), 0, c->stream, g, c->f[IPG_DUDT], c->f[IPG_DVDT],
-                     c->f[IPG_DWDT], c->f[IPG_P], c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], c->f[IPG_PNEW], c->f[IPG_DCDT],
-                     c->f[IPG_C], c->dscal, c->pc.rkab[rkiter - 1]);
+  if (rc) return rc;
   IPG_HIP(hipGetLastError());
-  ipg_mark(c, "correc");
   return 0;
 }
 

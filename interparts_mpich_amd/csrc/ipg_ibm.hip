@@ -70,14 +70,15 @@ struct IbmArgs {
 // never meets its own image: it is narrower than the domain).  One block per
 // sphere, one thread per partner; the order of the entries is immaterial, as
 // the spreads they feed are sums.
-__global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int *nbc, int *err) {
+constexpr int IBM_NB_THREADS = 256;
+__device__ __forceinline__ void ibm_neighbours_body(const IbmArgs &A, int *nbl, int *nbc, int *err, const VB vb) {
   const Grid &g = A.g;
-  const int p = blockIdx.x;
+  const int p = vb.x;
   __shared__ int count;
   if (threadIdx.x == 0) count = 0;
   __syncthreads();
   const Box bp = reach_box(A.g, A.radfp, A.px[p], A.py[p], A.pz[p]);
-  for (int q = threadIdx.x; q < A.np; q += blockDim.x) {
+  for (int q = threadIdx.x; q < A.np; q += IBM_NB_THREADS) {
     if (q == p) continue;
     const Box bq = reach_box(A.g, A.radfp, A.px[q], A.py[q], A.pz[q]);
     unsigned mx, my;
@@ -97,6 +98,14 @@ __global__ __launch_bounds__(256) void k_ibm_neighbours(IbmArgs A, int *nbl, int
     if (count > IBM_MAXNB) atomicOr(err, 1);
   }
 }
+
+struct IbmNbArgs { IbmArgs A; int *nbl, *nbc, *err; };
+struct IbmNbK {
+  using Args = IbmNbArgs;
+  static constexpr int THREADS = IBM_NB_THREADS, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) { ibm_neighbours_body(a.A, a.nbl, a.nbc, a.err, v); }
+};
+__global__ __launch_bounds__(IBM_NB_THREADS) void k_ibm_neighbours(IbmNbArgs) { run_single<IbmNbK>(); }
 
 // Stencil of marker l of sphere q, seen through the periodic shift (sxs nx,
 // sys ny) in the unwrapped frame of box b; false when it misses the box.
@@ -128,16 +137,16 @@ __device__ __forceinline__ bool neighbour_stencil(const IbmArgs &A, const Box &b
 // float, MODE 2 sums only the spread S in the box and adds it to the FP64
 // dudt as it stores, so that dudt is never rounded to FP32.
 template <int MODE, typename T>
-__global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_ibm(IbmArgs A, int pass) {
+__device__ __forceinline__ void ibm_body(const IbmArgs &A, int pass, const VB vb) {
   constexpr bool F64 = sizeof(T) == sizeof(double);
   extern __shared__ __align__(16) unsigned char ibm_lds[];
   T *box = reinterpret_cast<T *>(ibm_lds);
   const Grid &g = A.g;
-  const int p = blockIdx.x / 3, comp = blockIdx.x % 3;
+  const int p = vb.x / 3, comp = vb.x % 3;
   const long M = (long)A.np * A.nl;
   if (A.Kzero) {
     const long nplane = (long)g.nx * g.ny;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nplane; t += (long)gridDim.x * blockDim.x) A.Kzero[t] = 0.0;
+    for (long t = vb.x * (long)IBM_THREADS + threadIdx.x; t < nplane; t += (long)vb.gx * IBM_THREADS) A.Kzero[t] = 0.0;
   }
   const double dt = *A.dtp;
   const T dti = (T)(1.0 / dt);
@@ -187,15 +196,15 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   };
   // the cells the own stencils touch: flagged by pass 0, read back by the
   // later launches (the markers do not move inside the IBM block)
-  unsigned *gflag = A.flg + (long)blockIdx.x * nflagmax;
-  for (int t = threadIdx.x; t < nflag; t += blockDim.x) {
+  unsigned *gflag = A.flg + (long)vb.x * nflagmax;
+  for (int t = threadIdx.x; t < nflag; t += IBM_THREADS) {
     flag[t] = MODE == 0 ? 0u : gflag[t];
     if (MODE == 2) taken[t] = 0u;
   }
-  for (int t = threadIdx.x; t < b.by * b.bz; t += blockDim.x) rowoff[t] = box_rowoff(g, b, t);
-  for (int t = threadIdx.x; t < b.bx; t += blockDim.x) icol[t] = box_icol(g, b, t);
+  for (int t = threadIdx.x; t < b.by * b.bz; t += IBM_THREADS) rowoff[t] = box_rowoff(g, b, t);
+  for (int t = threadIdx.x; t < b.bx; t += IBM_THREADS) icol[t] = box_icol(g, b, t);
   if (MODE >= 1)
-    for (int t = threadIdx.x; t < nbox; t += blockDim.x) box[t] = (T)0;
+    for (int t = threadIdx.x; t < nbox; t += IBM_THREADS) box[t] = (T)0;
   __shared__ int lcount;
   if (MODE == 0 && threadIdx.x == 0) lcount = 0;
   __syncthreads();
@@ -227,14 +236,14 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
         if ((fm >> u) & 1u) unsafeAtomicAdd(&box[t0 + u * IBM_THREADS], (T)v[u]);
     }
   }
-  const long lbase = (long)blockIdx.x * IBM_MAXNM;
+  const long lbase = (long)vb.x * IBM_MAXNM;
   if (MODE == 0) {
     // list the neighbours' markers whose stencils touch this box; the
     // markers do not move inside the IBM block, so the three launches that
     // spread them reuse the list
     const int nn = A.nbc[p];
     const int tot = nn * A.nl;
-    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < tot; idx += IBM_THREADS) {
       const int n = idx / A.nl, l = idx - n * A.nl;
       const int code = A.nbl[(long)p * IBM_MAXNB + n];
       const int qs = code / 25, sh = code % 25;
@@ -252,8 +261,8 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     }
   } else {
     // the listed neighbour markers' contributions to this box
-    const int cnt = A.nmc[blockIdx.x];
-    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+    const int cnt = A.nmc[vb.x];
+    for (int idx = threadIdx.x; idx < cnt; idx += IBM_THREADS) {
       const int code = A.nml[lbase + idx];
       const int mq = code / 25, sh = code % 25;
       const int qs = mq / A.nl, l = mq - qs * A.nl;
@@ -304,9 +313,9 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   }
   __syncthreads();
   if (MODE == 0) {
-    for (int t = threadIdx.x; t < nflag; t += blockDim.x) gflag[t] = flag[t];
+    for (int t = threadIdx.x; t < nflag; t += IBM_THREADS) gflag[t] = flag[t];
     if (threadIdx.x == 0) {
-      A.nmc[blockIdx.x] = min(lcount, IBM_MAXNM);
+      A.nmc[vb.x] = min(lcount, IBM_MAXNM);
       if (lcount > IBM_MAXNM) atomicOr(A.err, 2);
     }
   }
@@ -396,7 +405,19 @@ __global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))
   // per-pass output of updtintermediatevel (k_ibm_force_sums)
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) fsum += __shfl_xor(fsum, o, 64);
-  if ((threadIdx.x & 63) == 0) A.psum[((long)pass * gridDim.x + blockIdx.x) * IBM_WAVES + (threadIdx.x >> 6)] = fsum;
+  if ((threadIdx.x & 63) == 0) A.psum[((long)pass * vb.gx + vb.x) * IBM_WAVES + (threadIdx.x >> 6)] = fsum;
+}
+
+struct IbmPassArgs { IbmArgs A; int pass; };
+template <int MODE, typename T>
+struct IbmK {
+  using Args = IbmPassArgs;
+  static constexpr int THREADS = IBM_THREADS, WPE = 4;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) { ibm_body<MODE, T>(a.A, a.pass, v); }
+};
+template <int MODE, typename T>
+__global__ __launch_bounds__(IBM_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_ibm(IbmPassArgs) {
+  run_single<IbmK<MODE, T>>();
 }
 
 // src/forcing.f90:440-458 (updtintermediatevel): forcextot, forceytot,
@@ -424,15 +445,16 @@ __global__ __launch_bounds__(256) void k_ibm_force_sums(const double *__restrict
 
 // Per-sphere force and torque sums after the last pass, src/forcing.f90:258-287,
 // one block per sphere.
+constexpr int IBM_TOT_THREADS = 256;
 template <typename T>
-__global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, double *fyt, double *fzt, double *tqx, double *tqy,
-                                                    double *tqz, double *tqt) {
+__device__ __forceinline__ void ibm_totals_body(const IbmArgs &A, double *fxt, double *fyt, double *fzt, double *tqx,
+                                                double *tqy, double *tqz, double *tqt, const VB vb) {
   const Grid &g = A.g;
-  const int p = blockIdx.x;
+  const int p = vb.x;
   const long M = (long)A.np * A.nl;
   const double xp = A.px[p], yp = A.py[p], zp = A.pz[p];
   double s[6] = {0, 0, 0, 0, 0, 0};
-  for (int l = threadIdx.x; l < A.nl; l += blockDim.x) {
+  for (int l = threadIdx.x; l < A.nl; l += IBM_TOT_THREADS) {
     double xl, yl, zl;
     if (!marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) continue;
     const long m = (long)p * A.nl + l;
@@ -452,12 +474,24 @@ __global__ __launch_bounds__(256) void k_ibm_totals(IbmArgs A, double *fxt, doub
   __syncthreads();
   if (threadIdx.x == 0) {
     double r[6];
-    for (int q = 0; q < 6; q++) { r[q] = 0.0; for (int w = 0; w < (int)(blockDim.x >> 6); w++) r[q] += sh[q][w]; }
+    for (int q = 0; q < 6; q++) { r[q] = 0.0; for (int w = 0; w < IBM_TOT_THREADS / 64; w++) r[q] += sh[q][w]; }
     fxt[p] = r[0] * A.dVlagr; fyt[p] = r[1] * A.dVlagr; fzt[p] = r[2] * A.dVlagr;
     tqx[p] = r[3] * A.dVlagr; tqy[p] = r[4] * A.dVlagr; tqz[p] = r[5] * A.dVlagr;
     tqt[p] = (tqy[p] * cos(A.pphi[p])) - (tqx[p] * sin(A.pphi[p]));
   }
 }
+
+struct IbmTotArgs { IbmArgs A; double *fxt, *fyt, *fzt, *tqx, *tqy, *tqz, *tqt; };
+template <typename T>
+struct IbmTotK {
+  using Args = IbmTotArgs;
+  static constexpr int THREADS = IBM_TOT_THREADS, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) {
+    ibm_totals_body<T>(a.A, a.fxt, a.fyt, a.fzt, a.tqx, a.tqy, a.tqz, a.tqt, v);
+  }
+};
+template <typename T>
+__global__ __launch_bounds__(IBM_TOT_THREADS) void k_ibm_totals(IbmTotArgs) { run_single<IbmTotK<T>>(); }
 
 // dynamic LDS of one k_ibm workgroup (ibm_lds_layout)
 size_t ibm_lds_bytes(int bmx, int bmy, int bmz, size_t esize) { return ibm_lds_layout(bmx, bmy, bmz, esize).bytes; }
@@ -492,7 +526,9 @@ int ipg_ibm_setup(ipg_ctx *c) {
     IPG_HIP(hipFuncSetAttribute((const void *)k_ibm<2, double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   // the captured substeps hold the IBM launch arguments (marker radius, box
-  // sizes, dVlagr) by value: a new marker set-up drops them
+  // sizes, dVlagr) by value: a new marker set-up drops them (and a batch's
+  // merged launches, which hold them too)
+  c->gen++;
   for (int r = 0; r < IPG_NGRAPH; r++)
     if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
   // the pass-0 cell flags; a larger box (a second set-up with a larger
@@ -540,8 +576,9 @@ int ipg_launch_ibm_begin(ipg_ctx *c) {
   if (c->pc.np <= 0) return 0;
   IbmArgs A; size_t lds;
   if (!ibm_args(c, A, lds)) return 1;
-  hipLaunchKernelGGL(k_ibm_neighbours, dim3(A.np), dim3(256), 0, c->stream, A, c->nbl, c->nbc, c->ibm_err);
-  ipg_mark(c, "ibm_neighbours");
+  if (ipg_go<IbmNbK>(c, k_ibm_neighbours, dim3(A.np), dim3(IBM_NB_THREADS), 0, IbmNbArgs{A, c->nbl, c->nbc, c->ibm_err},
+                     "ibm_neighbours"))
+    return 1;
   IPG_HIP(hipGetLastError());
   return 0;
 }
@@ -563,14 +600,16 @@ int ipg_launch_ibm_gather(ipg_ctx *c, int pass) {
   A.Kzero = pass == 0 ? nullptr : K[pass - 1];
   static const char *names[3] = {"ibm_pass0", "ibm_pass1", "ibm_pass2"};
   const dim3 grid(3 * A.np), blk(IBM_THREADS);
+  const IbmPassArgs a{A, pass};
+  int rc;
   if (c->ibm32) {
-    if (pass == 0) hipLaunchKernelGGL((k_ibm<0, float>), grid, blk, lds, c->stream, A, pass);
-    else hipLaunchKernelGGL((k_ibm<1, float>), grid, blk, lds, c->stream, A, pass);
+    if (pass == 0) rc = ipg_go<IbmK<0, float>>(c, k_ibm<0, float>, grid, blk, lds, a, names[pass]);
+    else rc = ipg_go<IbmK<1, float>>(c, k_ibm<1, float>, grid, blk, lds, a, names[pass]);
   } else {
-    if (pass == 0) hipLaunchKernelGGL((k_ibm<0, double>), grid, blk, lds, c->stream, A, pass);
-    else hipLaunchKernelGGL((k_ibm<1, double>), grid, blk, lds, c->stream, A, pass);
+    if (pass == 0) rc = ipg_go<IbmK<0, double>>(c, k_ibm<0, double>, grid, blk, lds, a, names[pass]);
+    else rc = ipg_go<IbmK<1, double>>(c, k_ibm<1, double>, grid, blk, lds, a, names[pass]);
   }
-  ipg_mark(c, names[pass]);
+  if (rc) return rc;
   IPG_HIP(hipGetLastError());
   return 0;
 }
@@ -583,9 +622,10 @@ int ipg_launch_ibm_final(ipg_ctx *c) {
   if (!ibm_args(c, A, lds)) return 1;
   A.Kzero = c->K1[2];
   A.flp = c->fl;
-  if (c->ibm32) hipLaunchKernelGGL((k_ibm<2, float>), dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, 2);
-  else hipLaunchKernelGGL((k_ibm<2, double>), dim3(3 * A.np), dim3(IBM_THREADS), lds, c->stream, A, 2);
-  ipg_mark(c, "ibm_spread");
+  const IbmPassArgs a{A, 2};
+  const int rc = c->ibm32 ? ipg_go<IbmK<2, float>>(c, k_ibm<2, float>, dim3(3 * A.np), dim3(IBM_THREADS), lds, a, "ibm_spread")
+                          : ipg_go<IbmK<2, double>>(c, k_ibm<2, double>, dim3(3 * A.np), dim3(IBM_THREADS), lds, a, "ibm_spread");
+  if (rc) return rc;
   IPG_HIP(hipGetLastError());
   return 0;
 }
@@ -606,10 +646,11 @@ int ipg_launch_ibm_totals(ipg_ctx *c, int pass) {
   if (!ibm_args(c, A, lds)) return 1;
   if (pass == 1) A.fl = c->fl2;
   double **q = c->ps.q;
-  hipLaunchKernelGGL(c->ibm32 ? k_ibm_totals<float> : k_ibm_totals<double>, dim3(A.np), dim3(256), 0, c->stream, A,
-                     q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT],
-                     q[IPG_PTORQX], q[IPG_PTORQY], q[IPG_PTORQZ], q[IPG_PTORQTHETA]);
-  ipg_mark(c, "ibm_totals");
+  const IbmTotArgs a{A, q[IPG_PFXLTOT], q[IPG_PFYLTOT], q[IPG_PFZLTOT], q[IPG_PTORQX], q[IPG_PTORQY], q[IPG_PTORQZ],
+                     q[IPG_PTORQTHETA]};
+  const int rc = c->ibm32 ? ipg_go<IbmTotK<float>>(c, k_ibm_totals<float>, dim3(A.np), dim3(IBM_TOT_THREADS), 0, a, "ibm_totals")
+                          : ipg_go<IbmTotK<double>>(c, k_ibm_totals<double>, dim3(A.np), dim3(IBM_TOT_THREADS), 0, a, "ibm_totals");
+  if (rc) return rc;
   IPG_HIP(hipGetLastError());
   return 0;
 }

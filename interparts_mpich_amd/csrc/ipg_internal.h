@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <string>
 #include "../../include/ipg.h"
+#include "ipg_batch.h"
 
 #define IPG_NB 9
 #define IPG_NGRAPH 12
@@ -132,6 +133,13 @@ struct ipg_ctx {
   int evtag[IPG_NMARK];           // 0: the main stream, 1: the side stream
   unsigned long long *stamps;     // [IPG_NMARK] wall-clock marks
   hipStream_t main_stream;        // the context's stream (c->stream is swapped while the side stream is filled)
+  // set while a batch records this context's substep: launches are recorded
+  // for the batch's merged launches instead of being issued
+  struct IpgRec *rec;
+  // counts the changes that alter launch arguments or shapes (marker set-up,
+  // solver set-up, the RK-stage launch shape, the refinement sweeps): a batch
+  // re-records its merged launches when a case's count moves
+  unsigned gen;
 };
 
 // Every device allocation a context holds outside the Poisson plan (which
@@ -167,6 +175,31 @@ inline void ipg_mark(ipg_ctx *c, const char *name) {
   c->evtag[c->nev + 1] = c->stream == c->main_stream ? 0 : 1;
   c->nev++;
 }
+
+// A launch of this context recorded by a batch: the merged kernels of its body
+// (k_batch<K> and its persistent form), its own grid, block and dynamic LDS,
+// a copy of its argument struct, and the substep stage it belongs to.
+int ipg_rec_add(struct IpgRec *r, BatchFns fns, dim3 grid, dim3 block, size_t lds, const void *args, size_t argsz,
+                const char *stage);
+
+// Launches kernel body K for context c: its single-context kernel `one` on the
+// context's stream, marked for timing as `stage`, or, while a batch records
+// the context, a record of the launch.
+#if defined(__HIPCC__)
+template <class K>
+int ipg_go(ipg_ctx *c, void (*one)(typename K::Args), dim3 grid, dim3 block, size_t lds, const typename K::Args &a,
+           const char *stage) {
+  if (c->rec) return ipg_rec_add(c->rec, batch_kernels<K>(), grid, block, lds, &a, sizeof(a), stage);
+  hipLaunchKernelGGL(one, grid, block, lds, c->stream, a);
+  ipg_mark(c, stage);
+  return 0;
+}
+#endif
+
+// a context's part in a batch (ipg_api.hip)
+int ipg_batch_record(ipg_ctx *c, int rkiter, int par, struct IpgRec *rec);
+int ipg_batch_enter(ipg_ctx *c);
+void ipg_batch_leave(ipg_ctx *c);
 
 // error plumbing
 void ipg_set_error(const std::string &msg);

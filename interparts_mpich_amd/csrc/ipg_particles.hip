@@ -46,10 +46,10 @@ __device__ __forceinline__ void image_shift(const Grid &g, int nbrecv, double &x
 // queue is then worked through densely, so that the eight corner square roots
 // are evaluated only by the lanes that need them.
 constexpr int SPH_THREADS = 512, SPH_SLICE = 16 * SPH_THREADS;
-__global__ __launch_bounds__(SPH_THREADS) void k_sphere_int(Grid g, PartConst pc, const double *__restrict__ U,
-                                                            const double *__restrict__ V, const double *__restrict__ W, PQ P,
-                                                            const int *__restrict__ nbf, int type, double *op_save) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void sphere_int_body(const Grid &g, const PartConst &pc, const double *__restrict__ U,
+                                                const double *__restrict__ V, const double *__restrict__ W, const PQ &P,
+                                                const int *__restrict__ nbf, int type, double *op_save, const VB vb) {
+  const int p = vb.x;
   __shared__ double acc[IPG_NB][6];
   __shared__ double red[6][SPH_THREADS / 64];
   __shared__ int cut[SPH_SLICE];
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(SPH_THREADS) void k_sphere_int(Grid g, PartConst pc
     __syncthreads();
     if (threadIdx.x < 6) {
       double v = 0.0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); w++) v += red[threadIdx.x][w];
+      for (int w = 0; w < SPH_THREADS / 64; w++) v += red[threadIdx.x][w];
       acc[nbrecv][threadIdx.x] = v;
     }
     __syncthreads();
@@ -189,6 +189,25 @@ __global__ __launch_bounds__(SPH_THREADS) void k_sphere_int(Grid g, PartConst pc
       P.q[IPG_PINTOMX + (q - 3)][p] = t;
     }
   }
+}
+
+struct SphereIntArgs {
+  Grid g; PartConst pc;
+  const double *U, *V, *W;
+  PQ P;
+  const int *nbf;
+  int type;
+  double *op_save;
+};
+struct SphereIntK {
+  using Args = SphereIntArgs;
+  static constexpr int THREADS = SPH_THREADS, WPE = 6;  // three blocks per CU
+  static __device__ __forceinline__ void run(const Args &a, const VB v) {
+    sphere_int_body(a.g, a.pc, a.U, a.V, a.W, a.P, a.nbf, a.type, a.op_save, v);
+  }
+};
+__global__ __launch_bounds__(SPH_THREADS) __attribute__((amdgpu_waves_per_eu(SphereIntK::WPE))) void k_sphere_int(SphereIntArgs) {
+  run_single<SphereIntK>();
 }
 
 // src/collisions.f90:222-438, normal squeeze force only (lines 430-432)
@@ -363,16 +382,16 @@ __device__ __forceinline__ void wave_lds_sync() {
 // (a different buffer), so no wavefront reads a state another one is
 // updating; the last pass also wraps the position and sets the neighbour
 // flags (src/intgr_nwtn_eulr.f90:702-714, 1076-1111).
-__global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc, PQ P, Motion src, Motion dst,
-                                                            double *__restrict__ op, const double *__restrict__ dtp,
-                                                            int rkiter, int last, int *nbf, int *err) {
+__device__ __forceinline__ void collide_body(const Grid &g, const PartConst &pc, const PQ &P, const Motion &src,
+                                             const Motion &dst, double *__restrict__ op, const double *__restrict__ dtp,
+                                             int rkiter, int last, int *nbf, int *err, const VB vb) {
   __shared__ double cx[COL_CHUNK], cy[COL_CHUNK], cz[COL_CHUNK];
   __shared__ int cand[COL_WAVES][MAXCAND];
   __shared__ double cd[COL_WAVES][MAXCAND][9];
   __shared__ double terms[COL_WAVES][64][6];
   __shared__ int tuse[COL_WAVES][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int p = blockIdx.x * COL_WAVES + wv;
+  const int p = vb.x * COL_WAVES + wv;
   const int np = pc.np;
   const double *X = src.x, *Y = src.y, *Zq = src.z, *Uq = src.u, *Vq = src.v, *Wq = src.w;
   const bool live = p < np;
@@ -383,7 +402,7 @@ __global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc
   for (int c0 = 0; c0 < np; c0 += COL_CHUNK) {
     const int n = min(COL_CHUNK, np - c0);
     __syncthreads();
-    for (int t = threadIdx.x; t < n; t += blockDim.x) { cx[t] = X[c0 + t]; cy[t] = Y[c0 + t]; cz[t] = Zq[c0 + t]; }
+    for (int t = threadIdx.x; t < n; t += COL_WAVES * 64) { cx[t] = X[c0 + t]; cy[t] = Y[c0 + t]; cz[t] = Zq[c0 + t]; }
     __syncthreads();
     if (!live) continue;
     for (int q0 = 0; q0 < n; q0 += 64) {
@@ -491,6 +510,21 @@ __global__ __launch_bounds__(COL_WAVES * 64) void k_collide(Grid g, PartConst pc
   if (last) wrap_flags_one(g, pc, P, nbf, 1, p);
 }
 
+struct CollideArgs {
+  Grid g; PartConst pc; PQ P; Motion src, dst;
+  double *op; const double *dtp;
+  int rkiter, last;
+  int *nbf, *err;
+};
+struct CollideK {
+  using Args = CollideArgs;
+  static constexpr int THREADS = COL_WAVES * 64, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) {
+    collide_body(a.g, a.pc, a.P, a.src, a.dst, a.op, a.dtp, a.rkiter, a.last, a.nbf, a.err, v);
+  }
+};
+__global__ __launch_bounds__(COL_WAVES * 64) void k_collide(CollideArgs) { run_single<CollideK>(); }
+
 __global__ void k_wrap_flags(Grid g, PartConst pc, PQ P, int *nbf, int wrap) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= pc.np) return;
@@ -595,8 +629,8 @@ int ipg_launch_phase_sol(ipg_ctx *c, double *out) {
 
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type) {
   if (c->pc.np <= 0) return 0;
-  hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(SPH_THREADS), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V],
-                     c->f[IPG_W], make_pq(c), c->ps.nbf, type, (double *)nullptr);
+  const SphereIntArgs a{c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], make_pq(c), c->ps.nbf, type, nullptr};
+  hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(SPH_THREADS), 0, c->stream, a);
   IPG_HIP(hipGetLastError());
   return 0;
 }
@@ -617,21 +651,20 @@ int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter) {
   const int np = c->pc.np;
   if (np <= 0) return 0;
   const PQ P = make_pq(c);
-  hipLaunchKernelGGL(k_sphere_int, dim3(np), dim3(SPH_THREADS), 0, c->stream, c->g, c->pc, c->f[IPG_U], c->f[IPG_V],
-                     c->f[IPG_W], P, c->ps.nbf, 23, c->op);
-  ipg_mark(c, "sphere_int");
+  const SphereIntArgs sa{c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], P, c->ps.nbf, 23, c->op};
+  if (ipg_go<SphereIntK>(c, k_sphere_int, dim3(np), dim3(SPH_THREADS), 0, sa, "sphere_int")) return 1;
   double **q = c->ps.q;
   const Motion state{q[IPG_PX], q[IPG_PY], q[IPG_PZ], q[IPG_PU], q[IPG_PV], q[IPG_PW], q[IPG_POMX], q[IPG_POMY], q[IPG_POMZ]};
   double *m = c->motion2;
   const Motion second{m, m + np, m + 2L * np, m + 3L * np, m + 4L * np, m + 5L * np, m + 6L * np, m + 7L * np, m + 8L * np};
   int *err = c->ibm_err;  // device error word, bit 8: a candidate list overflowed
   const dim3 grid((np + COL_WAVES - 1) / COL_WAVES), blk(COL_WAVES * 64);
-  hipLaunchKernelGGL(k_collide, grid, blk, 0, c->stream, c->g, c->pc, P, state, second, c->op, c->dscal, rkiter, 0,
-                     c->ps.nbf, err);
-  ipg_mark(c, "collide_update");
-  hipLaunchKernelGGL(k_collide, grid, blk, 0, c->stream, c->g, c->pc, P, second, state, c->op, c->dscal, rkiter, 1,
-                     c->ps.nbf, err);
-  ipg_mark(c, "collide_update");
+  if (ipg_go<CollideK>(c, k_collide, grid, blk, 0, CollideArgs{c->g, c->pc, P, state, second, c->op, c->dscal, rkiter, 0,
+                                                             c->ps.nbf, err}, "collide_update"))
+    return 1;
+  if (ipg_go<CollideK>(c, k_collide, grid, blk, 0, CollideArgs{c->g, c->pc, P, second, state, c->op, c->dscal, rkiter, 1,
+                                                             c->ps.nbf, err}, "collide_update"))
+    return 1;
   IPG_HIP(hipGetLastError());
   return 0;
 }

@@ -208,12 +208,12 @@ struct XSrc {
 // the two half spectra are split after the transform.  The input is chosen by
 // SRC (XSRC_*).  spec[(k*ny + j)*sp + m], m = 0..nx/2, stored as TS.
 template <int N, typename TS, int SRC>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, XSrc src, TS *__restrict__ spec, long sp,
-                                                             const double2 *__restrict__ tw) {
+__device__ __forceinline__ void fft_x_fwd_body(const Grid g, const XSrc src, TS *__restrict__ spec, long sp,
+                                               const double2 *__restrict__ tw, const VB vb) {
   constexpr int B = xbatch_of<N>();
   __shared__ double2 buf[2 * B * N];
   const long pairs = (long)(g.ny / 2) * g.nz;
-  const long pair0 = (long)blockIdx.x * B;
+  const long pair0 = (long)vb.x * B;
   double dtidxi = 0.0, dtidyi = 0.0, dtidzi = 0.0;
   if (SRC == XSRC_DIV) {
     const double dti = 1.0 / (src.rkab * *src.dtp);
@@ -267,14 +267,27 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(Grid g, XSrc src, T
   }
 }
 
+template <typename TS>
+struct XFwdArgs { Grid g; XSrc src; TS *spec; long sp; const double2 *tw; };
+template <int N, typename TS, int SRC>
+struct XFwdK {
+  using Args = XFwdArgs<TS>;
+  static constexpr int THREADS = FFT_THREADS, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) {
+    fft_x_fwd_body<N, TS, SRC>(a.g, a.src, a.spec, a.sp, a.tw, v);
+  }
+};
+template <int N, typename TS, int SRC>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd_t(XFwdArgs<TS>) { run_single<XFwdK<N, TS, SRC>>(); }
+
 // y transform of a tile of B x-modes of one z-plane, in place in spec.
 template <int N, typename TS>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_y_t(Grid g, TS *__restrict__ spec, long sp, int nmx,
-                                                         const double2 *__restrict__ tw, int sign, int top_first) {
+__device__ __forceinline__ void fft_y_body(const Grid g, TS *__restrict__ spec, long sp, int nmx,
+                                           const double2 *__restrict__ tw, int sign, int top_first, const VB vb) {
   constexpr int B = batch_of<N>();
   __shared__ double2 buf[2 * B * N];
   const int tiles = (nmx + B - 1) / B;
-  const int kb = blockIdx.x / tiles, m0 = (blockIdx.x % tiles) * B;
+  const int kb = vb.x / tiles, m0 = (vb.x % tiles) * B;
   const int k = top_first ? g.nz - 1 - kb : kb;
   TS *plane = spec + (long)k * g.ny * sp;
   // every load of the tile in flight before the first LDS store
@@ -302,16 +315,30 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_y_t(Grid g, TS *__restrict_
   }
 }
 
+template <typename TS>
+struct YArgs { Grid g; TS *spec; long sp; int nmx; const double2 *tw; int sign, top_first; };
+template <int N, typename TS>
+struct YK {
+  using Args = YArgs<TS>;
+  static constexpr int THREADS = FFT_THREADS, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) {
+    fft_y_body<N, TS>(a.g, a.spec, a.sp, a.nmx, a.tw, a.sign, a.top_first, v);
+  }
+};
+template <int N, typename TS>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_y_t(YArgs<TS>) { run_single<YK<N, TS>>(); }
+
 // Inverse x transform of two half spectra (rows j, j+1) into p's interior,
 // with the overall 1/(nx*ny) normalisation; p stored as TP (double, or a
 // float piece of the mixed path's pressure correction).
 template <int N, typename TS, typename TP>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, TP *__restrict__ p, const TS *__restrict__ spec,
-                                                             long sp, const double2 *__restrict__ tw, double scale, int top_first) {
+__device__ __forceinline__ void fft_x_inv_body(const Grid g, TP *__restrict__ p, const TS *__restrict__ spec, long sp,
+                                               const double2 *__restrict__ tw, double scale, int top_first,
+                                               const VB vblk) {
   constexpr int B = xbatch_of<N>();
   __shared__ double2 buf[2 * B * N];
   const long pairs = (long)(g.ny / 2) * g.nz;
-  const long pair0 = (long)(top_first ? gridDim.x - 1 - blockIdx.x : blockIdx.x) * B;
+  const long pair0 = (long)(top_first ? vblk.gx - 1 - vblk.x : vblk.x) * B;
   constexpr int half = N / 2;
   // modes m > N/2 are the conjugates of N - m; every load of the block in
   // flight before the first LDS store
@@ -353,6 +380,19 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(Grid g, TP *__restr
     p[c + g.sx] = (TP)(x[b * N + i].y * scale);
   }
 }
+
+template <typename TS, typename TP>
+struct XInvArgs { Grid g; TP *p; const TS *spec; long sp; const double2 *tw; double scale; int top_first; };
+template <int N, typename TS, typename TP>
+struct XInvK {
+  using Args = XInvArgs<TS, TP>;
+  static constexpr int THREADS = FFT_THREADS, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) {
+    fft_x_inv_body<N, TS, TP>(a.g, a.p, a.spec, a.sp, a.tw, a.scale, a.top_first, v);
+  }
+};
+template <int N, typename TS, typename TP>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_x_inv_t(XInvArgs<TS, TP>) { run_single<XInvK<N, TS, TP>>(); }
 
 // ---- generic kernels (runtime plan) -----------------------------------------
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_x_fwd(Grid g, const double *__restrict__ p, double2 *__restrict__ spec,
@@ -465,17 +505,18 @@ constexpr int THOMAS_TAIL = 48;
 // column takes the branch z == 0 -> p(kmax) = 0 whatever its last pivot, so
 // that the solve's free constant is never large beside the FP32 pieces it
 // lands in (the constant does not reach the velocity).
+constexpr int THOMAS_THREADS = 256;
 template <typename TS>
-__global__ __launch_bounds__(256) void k_thomas(Grid g, TS *__restrict__ spec, double *__restrict__ dtri, long sp, int nmx,
-                                                const double *__restrict__ a, const double *__restrict__ b,
-                                                const double *__restrict__ cc, const double *__restrict__ xrt,
-                                                const double *__restrict__ yrt, int pin_zero) {
+__device__ __forceinline__ void thomas_body(const Grid g, TS *__restrict__ spec, double *__restrict__ dtri, long sp,
+                                            int nmx, const double *__restrict__ a, const double *__restrict__ b,
+                                            const double *__restrict__ cc, const double *__restrict__ xrt,
+                                            const double *__restrict__ yrt, int pin_zero, const VB vblk) {
   // lane u: part (bit 0), x-mode m = 16 mb + (bits 1-4), side (bit 5), and
   // the y-mode pair pr (u >> 6 = pr nmb + mb).  The y-modes pr and ny - pr
   // have the same eigenvalue (both fold to |ky|), hence the same pivots: the
   // two sides of a wavefront share one row of multipliers d, written by side
   // 0 and read by both.
-  const long u = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long u = vblk.x * (long)THOMAS_THREADS + threadIdx.x;
   const int nmb = (nmx + 15) / 16;
   const int part = (int)(u & 1), ml = (int)((u >> 1) & 15), side = (int)((u >> 5) & 1);
   const long grp = u >> 6;
@@ -637,6 +678,23 @@ __global__ __launch_bounds__(256) void k_thomas(Grid g, TS *__restrict__ spec, d
   }
 }
 
+template <typename TS>
+struct ThomasArgs {
+  Grid g; TS *spec; double *dtri; long sp; int nmx;
+  const double *a, *b, *cc, *xrt, *yrt;
+  int pin_zero;
+};
+template <typename TS>
+struct ThomasK {
+  using Args = ThomasArgs<TS>;
+  static constexpr int THREADS = THOMAS_THREADS, WPE = 0;
+  static __device__ __forceinline__ void run(const Args &a, const VB v) {
+    thomas_body<TS>(a.g, a.spec, a.dtri, a.sp, a.nmx, a.a, a.b, a.cc, a.xrt, a.yrt, a.pin_zero, v);
+  }
+};
+template <typename TS>
+__global__ __launch_bounds__(THOMAS_THREADS) void k_thomas(ThomasArgs<TS>) { run_single<ThomasK<TS>>(); }
+
 bool plan_radices(int n, Radices &rd) {
   rd.n = 0;
   int m = n;
@@ -664,40 +722,44 @@ struct XArgs {
 };
 
 template <int N, typename TS, int SRC>
-void launch_x_fwd(const XArgs<TS> &a, hipStream_t s) {
+int launch_x_fwd(ipg_ctx *c, const XArgs<TS> &a, const char *stage) {
   constexpr int B = xbatch_of<N>();
   const long pairs = (long)(a.g.ny / 2) * a.g.nz;
-  hipLaunchKernelGGL((k_fft_x_fwd_t<N, TS, SRC>), dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g,
-                     a.src, a.spec, a.sp, a.tw);
+  return ipg_go<XFwdK<N, TS, SRC>>(c, k_fft_x_fwd_t<N, TS, SRC>, dim3((unsigned)((pairs + B - 1) / B)),
+                                   dim3(FFT_THREADS), 0, XFwdArgs<TS>{a.g, a.src, a.spec, a.sp, a.tw}, stage);
 }
 template <int N, typename TS, typename TP>
-void launch_x_inv(const XArgs<TS> &a, TP *p, hipStream_t s) {
+int launch_x_inv(ipg_ctx *c, const XArgs<TS> &a, TP *p) {
   constexpr int B = xbatch_of<N>();
   const long pairs = (long)(a.g.ny / 2) * a.g.nz;
-  hipLaunchKernelGGL((k_fft_x_inv_t<N, TS, TP>), dim3((unsigned)((pairs + B - 1) / B)), dim3(FFT_THREADS), 0, s, a.g, p,
-                     a.spec, a.sp, a.tw, a.scale, X_INV_TOP_FIRST);
+  return ipg_go<XInvK<N, TS, TP>>(c, k_fft_x_inv_t<N, TS, TP>, dim3((unsigned)((pairs + B - 1) / B)),
+                                  dim3(FFT_THREADS), 0,
+                                  XInvArgs<TS, TP>{a.g, p, a.spec, a.sp, a.tw, a.scale, X_INV_TOP_FIRST}, "fft_x_inv");
 }
 template <int N, typename TS>
-void launch_y(const Grid &g, TS *spec, long sp, int nmx, const double2 *tw, int sign, int top_first, hipStream_t s) {
+int launch_y(ipg_ctx *c, TS *spec, int sign, int top_first, const char *stage) {
   constexpr int B = batch_of<N>();
-  const int tiles = (nmx + B - 1) / B;
-  hipLaunchKernelGGL((k_fft_y_t<N, TS>), dim3(g.nz * tiles), dim3(FFT_THREADS), 0, s, g, spec, sp, nmx, tw, sign,
-                     top_first);
+  const Poisson &P = c->ps2;
+  const int tiles = (P.nmx + B - 1) / B;
+  return ipg_go<YK<N, TS>>(c, k_fft_y_t<N, TS>, dim3(c->g.nz * tiles), dim3(FFT_THREADS), 0,
+                           YArgs<TS>{c->g, spec, P.sp, P.nmx, P.twy, sign, top_first}, stage);
 }
 
-#define IPG_FFT_DISPATCH(n, CALL) \
-  switch (n) {                    \
-    case 16: CALL(16); break;     \
-    case 24: CALL(24); break;     \
-    case 32: CALL(32); break;     \
-    case 48: CALL(48); break;     \
-    case 64: CALL(64); break;     \
-    case 96: CALL(96); break;     \
-    case 128: CALL(128); break;   \
-    case 192: CALL(192); break;   \
-    case 256: CALL(256); break;   \
-    case 384: CALL(384); break;   \
-    case 512: CALL(512); break;   \
+// rc = the launch of the plan of length n (one of the specialised lengths)
+#define IPG_FFT_DISPATCH(n, CALL)        \
+  switch (n) {                           \
+    case 16: rc = CALL(16); break;       \
+    case 24: rc = CALL(24); break;       \
+    case 32: rc = CALL(32); break;       \
+    case 48: rc = CALL(48); break;       \
+    case 64: rc = CALL(64); break;       \
+    case 96: rc = CALL(96); break;       \
+    case 128: rc = CALL(128); break;     \
+    case 192: rc = CALL(192); break;     \
+    case 256: rc = CALL(256); break;     \
+    case 384: rc = CALL(384); break;     \
+    case 512: rc = CALL(512); break;     \
+    default: rc = 1; break;              \
   }
 
 }  // namespace
@@ -820,24 +882,26 @@ int solve_rest(ipg_ctx *c, const XArgs<TS> &xa, TP *pout, int pin_zero) {
   const Grid &g = c->g;
   Poisson &P = c->ps2;
   TS *spec = xa.spec;
-#define CALLY(n) launch_y<n, TS>(g, spec, P.sp, P.nmx, P.twy, -1, Y_FWD_TOP_FIRST, c->stream)
+  int rc = 0;
+#define CALLY(n) launch_y<n, TS>(c, spec, -1, Y_FWD_TOP_FIRST, "fft_y_fwd")
   IPG_FFT_DISPATCH(g.ny, CALLY)
 #undef CALLY
-  ipg_mark(c, "fft_y_fwd");
+  if (rc) return rc;
   // lanes: 64 per (block of 16 x-modes, y-mode pair pr = 0..ny/2)
   const long nthomas = 64L * ((P.nmx + 15) / 16) * (g.ny / 2 + 1);
   using TE = typename ElemOf<TS>::type;
-  hipLaunchKernelGGL(k_thomas<TE>, dim3((unsigned)((nthomas + 255) / 256)), dim3(256), 0, c->stream, g, (TE *)spec,
-                     P.dtri, P.sp, P.nmx, P.a, P.b, P.c, P.xrt, P.yrt, pin_zero);
-  ipg_mark(c, "thomas");
-#define CALLY(n) launch_y<n, TS>(g, spec, P.sp, P.nmx, P.twy, +1, 0, c->stream)
+  const ThomasArgs<TE> ta{g, (TE *)spec, P.dtri, P.sp, P.nmx, P.a, P.b, P.c, P.xrt, P.yrt, pin_zero};
+  if (ipg_go<ThomasK<TE>>(c, k_thomas<TE>, dim3((unsigned)((nthomas + THOMAS_THREADS - 1) / THOMAS_THREADS)),
+                          dim3(THOMAS_THREADS), 0, ta, "thomas"))
+    return 1;
+#define CALLY(n) launch_y<n, TS>(c, spec, +1, 0, "fft_y_inv")
   IPG_FFT_DISPATCH(g.ny, CALLY)
 #undef CALLY
-  ipg_mark(c, "fft_y_inv");
-#define CALLX(n) launch_x_inv<n, TS, TP>(xa, pout, c->stream)
+  if (rc) return rc;
+#define CALLX(n) launch_x_inv<n, TS, TP>(c, xa, pout)
   IPG_FFT_DISPATCH(g.nx, CALLX)
 #undef CALLX
-  ipg_mark(c, "fft_x_inv");
+  if (rc) return rc;
   IPG_HIP(hipGetLastError());
   return 0;
 }
@@ -851,17 +915,18 @@ int poisson_mixed(ipg_ctx *c, int fused_rkiter) {
   const Grid &g = c->g;
   Poisson &P = c->ps2;
   XArgs<float2> xa{g, xsrc_of(c, fused_rkiter), (float2 *)P.spec, P.sp, P.twx, 1.0 / ((double)g.nx * (double)g.ny)};
+  int rc = 0;
   if (fused_rkiter > 0) {
     xa.src.rhs_out = c->f[IPG_P];
-#define CALLX(n) launch_x_fwd<n, float2, XSRC_DIV>(xa, c->stream)
+#define CALLX(n) launch_x_fwd<n, float2, XSRC_DIV>(c, xa, "fft_x_fwd")
     IPG_FFT_DISPATCH(g.nx, CALLX)
 #undef CALLX
   } else {
-#define CALLX(n) launch_x_fwd<n, float2, XSRC_P>(xa, c->stream)
+#define CALLX(n) launch_x_fwd<n, float2, XSRC_P>(c, xa, "fft_x_fwd")
     IPG_FFT_DISPATCH(g.nx, CALLX)
 #undef CALLX
   }
-  ipg_mark(c, "fft_x_fwd");
+  if (rc) return rc;
   if (solve_rest<float2, float>(c, xa, c->pp[0], 1)) return 1;
   for (int r = 1; r <= c->nref; r++) {
     // the FP64 residual of the pieces so far, rounded to FP32 (it is about a
@@ -870,10 +935,10 @@ int poisson_mixed(ipg_ctx *c, int fused_rkiter) {
     XArgs<float2> xr = xa;
     xr.src.rhs_out = nullptr;
     xr.src.r32 = c->pp[IPG_MAX_REFINE + 1];
-#define CALLX(n) launch_x_fwd<n, float2, XSRC_R32>(xr, c->stream)
+#define CALLX(n) launch_x_fwd<n, float2, XSRC_R32>(c, xr, "resid_x_fwd")
     IPG_FFT_DISPATCH(g.nx, CALLX)
 #undef CALLX
-    ipg_mark(c, "resid_x_fwd");
+    if (rc) return rc;
     if (solve_rest<float2, float>(c, xr, c->pp[r], 1)) return 1;
   }
   c->p_in_pieces = true;
@@ -894,19 +959,25 @@ int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter) {
   for (int q = 0; q < ry.n; q++) ry.r[q] = P.rady[q];
   XArgs<double2> xa{g, xsrc_of(c, fused_rkiter), P.spec, P.sp, P.twx, 1.0 / ((double)g.nx * (double)g.ny)};
   if (specialised(g.nx) && specialised(g.ny)) {
+    int rc = 0;
     if (fused_rkiter > 0) {
-#define CALLX(n) launch_x_fwd<n, double2, XSRC_DIV>(xa, c->stream)
+#define CALLX(n) launch_x_fwd<n, double2, XSRC_DIV>(c, xa, "fft_x_fwd")
       IPG_FFT_DISPATCH(g.nx, CALLX)
 #undef CALLX
     } else {
-#define CALLX(n) launch_x_fwd<n, double2, XSRC_P>(xa, c->stream)
+#define CALLX(n) launch_x_fwd<n, double2, XSRC_P>(c, xa, "fft_x_fwd")
       IPG_FFT_DISPATCH(g.nx, CALLX)
 #undef CALLX
     }
-    ipg_mark(c, "fft_x_fwd");
+    if (rc) return rc;
     return solve_rest<double2, double>(c, xa, c->f[IPG_P], 0);
   }
-  // a length without a compile-time plan: the runtime-plan kernels
+  // a length without a compile-time plan: the runtime-plan kernels (not in a
+  // batch, whose merged launches are those of the compiled plans)
+  if (c->rec) {
+    ipg_set_error("batch: itot and jtot must be in {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512}");
+    return 1;
+  }
   const long pairs = (long)(g.ny / 2) * g.nz;
   const int RPx = LDS_COMPLEX / g.nx;
   const int gx = (int)((pairs + RPx - 1) / RPx);
@@ -919,8 +990,9 @@ int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter) {
   hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, -1);
   ipg_mark(c, "fft_y_fwd");
   const long nthomas = 64L * ((P.nmx + 15) / 16) * (g.ny / 2 + 1);
-  hipLaunchKernelGGL(k_thomas<double>, dim3((unsigned)((nthomas + 255) / 256)), dim3(256), 0, c->stream, g,
-                     (double *)P.spec, P.dtri, P.sp, P.nmx, P.a, P.b, P.c, P.xrt, P.yrt, 0);
+  const ThomasArgs<double> ta{g, (double *)P.spec, P.dtri, P.sp, P.nmx, P.a, P.b, P.c, P.xrt, P.yrt, 0};
+  hipLaunchKernelGGL(k_thomas<double>, dim3((unsigned)((nthomas + THOMAS_THREADS - 1) / THOMAS_THREADS)),
+                     dim3(THOMAS_THREADS), 0, c->stream, ta);
   ipg_mark(c, "thomas");
   hipLaunchKernelGGL(k_fft_y, dim3(gy), dim3(FFT_THREADS), 0, c->stream, g, P.spec, P.sp, P.nmx, ry, P.twy, +1);
   ipg_mark(c, "fft_y_inv");

@@ -229,3 +229,55 @@ class Simulation:
             self.set_dt(0.75 * self.chkdt())
         else:
             self.sync()
+
+
+class Batch:
+    """A batch of Simulations advanced through each substep together
+    (ipg_batch_*): one merged launch per kernel and stage for all cases, one
+    captured graph per substep.  The simulations stay usable on their own."""
+
+    def __init__(self, sims, use_graphs=True):
+        self.sims = list(sims)
+        self.lib = load()
+        arr = (C.c_void_p * len(self.sims))(*[s.ctx for s in self.sims])
+        self.handle = C.c_void_p()
+        check(self.lib.ipg_batch_create(arr, len(self.sims), C.byref(self.handle)), "batch_create")
+        if not use_graphs:
+            check(self.lib.ipg_batch_use_graphs(self.handle, 0), "batch_use_graphs")
+
+    def close(self):
+        if self.handle:
+            self.lib.ipg_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def substep(self, rkiter):
+        check(self.lib.ipg_batch_substep(self.handle, rkiter), "batch_substep")
+
+    def sync(self):
+        check(self.lib.ipg_batch_sync(self.handle), "batch_sync")
+
+    def autotune(self):
+        """The batch's tuner (ipg_batch_autotune); returns the substeps it ran."""
+        n = C.c_int()
+        check(self.lib.ipg_batch_autotune(self.handle, C.byref(n)), "batch_autotune")
+        return n.value
+
+    def launches(self, rkiter):
+        """(merged launches, persistent ones) of the next substep of stage rkiter."""
+        n, p = C.c_int(), C.c_int()
+        check(self.lib.ipg_batch_launches(self.handle, rkiter, C.byref(n), C.byref(p)), "batch_launches")
+        return n.value, p.value
+
+    def phases(self, rkiter, maxn=128):
+        """One eager substep with per-merged-launch times: [(stage, ms)]."""
+        ms = (C.c_double * maxn)()
+        names = (C.c_char_p * maxn)()
+        n = C.c_int()
+        check(self.lib.ipg_batch_phases(self.handle, rkiter, maxn, ms, names, C.byref(n)), "batch_phases")
+        return [(names[q].decode(), ms[q]) for q in range(n.value)]

@@ -37,7 +37,8 @@ def test_planted_index_slip_is_caught(tmp_path, what, old, new):
     csrc = tmp_path / "interparts_mpich_amd" / "csrc"
     csrc.mkdir(parents=True)
     (tmp_path / "include").mkdir()
-    shutil.copy(os.path.join(ROOT, "interparts_mpich_amd", "csrc", "ipg_internal.h"), csrc)
+    for h in ("ipg_internal.h", "ipg_batch.h"):
+        shutil.copy(os.path.join(ROOT, "interparts_mpich_amd", "csrc", h), csrc)
     shutil.copy(os.path.join(ROOT, "include", "ipg.h"), tmp_path / "include")
     (csrc / "ipg_ibm_geom.h").write_text(src.replace(old, new))
     (tmp_path / "drv.cpp").write_text(DRIVER)
