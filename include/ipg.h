@@ -217,13 +217,19 @@ int ipg_batch_substep(ipg_batch *batch, int rkiter);
 int ipg_batch_sync(ipg_batch *batch);
 /* replay each substep from a captured graph (default on) */
 int ipg_batch_use_graphs(ipg_batch *batch, int on);
+/* split the cases into ngroups groups of about equal cell counts, each group's
+ * merged launches on a stream of its own, side by side in the substep's graph
+ * (default 1; the tuner chooses); ipg_batch_groups reads the number back */
+int ipg_batch_set_groups(ipg_batch *batch, int ngroups);
+int ipg_batch_groups(ipg_batch *batch, int *ngroups);
 /* The batch's tuner: times whole RK3 steps with every case's RK-stage launch
  * shape at 16, 32 and 64 z planes per workgroup and keeps the fastest, then,
  * for every merged launch, one workgroup per block against persistent grids of
  * 1, 2 and 4 workgroups per CU striding over the blocks, and keeps the fastest
- * per launch.  The timed steps advance every case (results do not depend on
- * the launch shapes): call it at a time-step boundary (after rkiter 3);
- * *substeps gets the number of substeps it ran (a multiple of 3). */
+ * per launch, then 1, 2 and 4 stream groups on graph replays of whole steps.
+ * The timed steps advance every case (results do not depend on the launch
+ * shapes): call it at a time-step boundary (after rkiter 3); *substeps gets
+ * the number of substeps it ran (a multiple of 3). */
 int ipg_batch_autotune(ipg_batch *batch, int *substeps);
 /* merged launches per substep of stage rkiter (after its first substep or the
  * tuner) and how many of them run as persistent grids */

@@ -11,6 +11,8 @@
 // about 20 per case -- are captured as one hipGraph per (RK stage, buffer
 // parity).  Records are made once per key and kept on the device; a case's
 // configuration counter (ipg_ctx::gen) moving makes the batch record again.
+// The cases may be split into groups on streams of their own, whose merged
+// launches run side by side inside the one graph.
 #include "ipg_internal.h"
 #include <algorithm>
 #include <map>
@@ -54,7 +56,6 @@ struct Merged {
 struct Plan {
   std::vector<Merged> L;
   void *dev = nullptr;  // argument structs, segments and block->segment tables of every launch
-  hipGraphExec_t graph = nullptr;
 };
 
 std::string hexptr(const void *p) {
@@ -65,15 +66,28 @@ std::string hexptr(const void *p) {
 
 }  // namespace
 
+// The cases are split into groups, each with its own stream and merged
+// launches: the groups' substeps run side by side, so that the latency-bound
+// stages of one group (the IBM passes, the Thomas solve, the sphere integrals)
+// overlap the bandwidth-bound stages of another.  One group is one stream of
+// merged launches; the tuner chooses the number of groups.
+struct Group {
+  std::vector<int> idx;          // the cases it holds, in batch order
+  hipStream_t stream = nullptr;  // the batch's stream for group 0
+  hipEvent_t joined = nullptr;   // its substep done (another group's stream)
+  Plan plan[NKEY];
+};
+
 struct ipg_batch {
   std::vector<ipg_ctx *> cs;
   hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;
+  hipEvent_t done = nullptr, fork = nullptr;
   std::vector<hipEvent_t> case_ev;
   std::vector<hipEvent_t> tev;  // timing events
   bool use_graphs = true;
   int ncu = 256;
-  Plan plan[NKEY];
+  std::vector<Group> groups;
+  hipGraphExec_t graph[NKEY] = {};
   std::map<std::string, int> persist;  // tune_key -> persistent workgroups per CU (0: one per block)
   int persist_default = 0;             // for a launch the tuner has not seen
   std::map<const void *, size_t> lds_set;  // the dynamic-LDS limit each merged kernel has been given
@@ -83,9 +97,51 @@ struct ipg_batch {
 namespace {
 
 void drop_plan(Plan &P) {
-  if (P.graph) (void)hipGraphExecDestroy(P.graph);
   if (P.dev) (void)hipFree(P.dev);
   P = Plan{};
+}
+
+void drop_graphs(ipg_batch *b) {
+  for (hipGraphExec_t &g : b->graph)
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+void drop_groups(ipg_batch *b) {
+  drop_graphs(b);
+  for (Group &G : b->groups) {
+    for (Plan &P : G.plan) drop_plan(P);
+    if (G.stream && G.stream != b->stream) (void)hipStreamDestroy(G.stream);
+    if (G.joined) (void)hipEventDestroy(G.joined);
+  }
+  b->groups.clear();
+}
+
+// ng groups of about equal cell counts: cases by decreasing size, each to the
+// group holding the fewest cells so far
+int make_groups(ipg_batch *b, int ng) {
+  IPG_HIP(hipDeviceSynchronize());
+  drop_groups(b);
+  const int n = (int)b->cs.size();
+  ng = std::max(1, std::min(ng, n));
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  auto cells = [&](int i) { const Grid &g = b->cs[i]->g; return (double)g.nx * g.ny * g.nz; };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cells(x) > cells(y); });
+  std::vector<double> load(ng, 0.0);
+  b->groups.resize(ng);
+  for (int i : order) {
+    const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    b->groups[g].idx.push_back(i);
+    load[g] += cells(i);
+  }
+  for (int g = 0; g < ng; g++) {
+    Group &G = b->groups[g];
+    std::sort(G.idx.begin(), G.idx.end());
+    if (g == 0) G.stream = b->stream;
+    else IPG_HIP(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
+    IPG_HIP(hipEventCreateWithFlags(&G.joined, hipEventDisableTiming));
+  }
+  return 0;
 }
 
 // true when no case has changed its configuration since the plans were recorded
@@ -96,14 +152,15 @@ bool gen_current(const ipg_batch *b) {
   return true;
 }
 
-// Records one substep of every case for key r and merges the records.
-int build_plan(ipg_batch *b, int r) {
+// Records one substep of every case of group G for key r and merges the
+// records.
+int build_plan(ipg_batch *b, Group &G, int r) {
   const int rkiter = r % 3 + 1, par = r / 3;
-  const int n = (int)b->cs.size();
-  drop_plan(b->plan[r]);
+  const int n = (int)G.idx.size();
+  drop_plan(G.plan[r]);
   std::vector<IpgRec> rec(n);
   for (int i = 0; i < n; i++)
-    if (ipg_batch_record(b->cs[i], rkiter, par, &rec[i])) return 1;
+    if (ipg_batch_record(b->cs[G.idx[i]], rkiter, par, &rec[i])) return 1;
   // stage#occurrence keys per case; the longest sequence gives the order, and
   // every other case's sequence must be a subsequence of it (a case without
   // spheres has no IBM or particle stages)
@@ -132,7 +189,7 @@ int build_plan(ipg_batch *b, int r) {
       last = it->second;
     }
   }
-  Plan &P = b->plan[r];
+  Plan &P = G.plan[r];
   std::vector<unsigned char> blob;
   auto align = [&](size_t a) { blob.resize((blob.size() + a - 1) / a * a); };
   auto put = [&](const void *p, size_t nb) {
@@ -203,16 +260,16 @@ int build_plan(ipg_batch *b, int r) {
   return 0;
 }
 
-// The merged launches of plan P on the batch's stream; with timing, an event
-// after each (b->tev[q + 1], b->tev[0] before the first).
-int run_plan(ipg_batch *b, Plan &P, bool timing) {
+// The merged launches of plan P on stream s; with timing, an event after
+// each (b->tev[t0 + q + 1], b->tev[t0] before the first).
+int run_plan(ipg_batch *b, Plan &P, hipStream_t s, bool timing, size_t t0 = 0) {
   if (timing) {
-    while (b->tev.size() < P.L.size() + 1) {
+    while (b->tev.size() < t0 + P.L.size() + 1) {
       hipEvent_t e;
       IPG_HIP(hipEventCreate(&e));
       b->tev.push_back(e);
     }
-    IPG_HIP(hipEventRecord(b->tev[0], b->stream));
+    IPG_HIP(hipEventRecord(b->tev[t0], s));
   }
   for (size_t q = 0; q < P.L.size(); q++) {
     const Merged &M = P.L[q];
@@ -228,45 +285,68 @@ int run_plan(ipg_batch *b, Plan &P, bool timing) {
       if (cap < M.total) { grid = (unsigned)cap; fn = M.fns.persist; }
     }
     void *kp[] = {(void *)&args, (void *)&m};
-    IPG_HIP(hipLaunchKernel(fn, dim3(grid), M.block, kp, M.lds, b->stream));
-    if (timing) IPG_HIP(hipEventRecord(b->tev[q + 1], b->stream));
+    IPG_HIP(hipLaunchKernel(fn, dim3(grid), M.block, kp, M.lds, s));
+    if (timing) IPG_HIP(hipEventRecord(b->tev[t0 + q + 1], s));
   }
   return 0;
 }
 
-int ensure_plan(ipg_batch *b, int r) {
+int ensure_plans(ipg_batch *b, int r) {
+  if (b->groups.empty() && make_groups(b, 1)) return 1;
   if (!gen_current(b)) {
     // a case changed: every key records again
-    IPG_HIP(hipStreamSynchronize(b->stream));
-    for (int q = 0; q < NKEY; q++) drop_plan(b->plan[q]);
+    IPG_HIP(hipDeviceSynchronize());
+    drop_graphs(b);
+    for (Group &G : b->groups)
+      for (Plan &P : G.plan) drop_plan(P);
     b->gen.clear();
     for (ipg_ctx *c : b->cs) b->gen.push_back(c->gen);
   }
-  if (b->plan[r].dev) return 0;
-  IPG_HIP(hipStreamSynchronize(b->stream));
-  return build_plan(b, r);
+  for (Group &G : b->groups)
+    if (!G.plan[r].dev) {
+      IPG_HIP(hipDeviceSynchronize());
+      if (build_plan(b, G, r)) return 1;
+    }
+  return 0;
 }
 
-int capture(ipg_batch *b, Plan &P) {
+// The groups' merged launches of key r: group 0 on the batch's stream, the
+// others forked from it and joined back.
+int run_groups(ipg_batch *b, int r) {
+  if (b->groups.size() > 1) IPG_HIP(hipEventRecord(b->fork, b->stream));
+  for (size_t g = 1; g < b->groups.size(); g++) IPG_HIP(hipStreamWaitEvent(b->groups[g].stream, b->fork, 0));
+  for (Group &G : b->groups)
+    if (run_plan(b, G.plan[r], G.stream, false)) return 1;
+  for (size_t g = 1; g < b->groups.size(); g++) {
+    IPG_HIP(hipEventRecord(b->groups[g].joined, b->groups[g].stream));
+    IPG_HIP(hipStreamWaitEvent(b->stream, b->groups[g].joined, 0));
+  }
+  return 0;
+}
+
+int capture(ipg_batch *b, int r) {
   hipGraph_t graph;
   IPG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = run_plan(b, P, false);
+  const int rc = run_groups(b, r);
   const hipError_t e = hipStreamEndCapture(b->stream, &graph);
   if (rc) return rc;
   if (e != hipSuccess) { ipg_set_error(std::string("batch graph capture: ") + hipGetErrorString(e)); return 1; }
-  IPG_HIP(hipGraphInstantiate(&P.graph, graph, nullptr, nullptr, 0));
+  IPG_HIP(hipGraphInstantiate(&b->graph[r], graph, nullptr, nullptr, 0));
   (void)hipGraphDestroy(graph);
   return 0;
 }
 
-void drop_graphs(ipg_batch *b) {
-  for (Plan &P : b->plan)
-    if (P.graph) { (void)hipGraphExecDestroy(P.graph); P.graph = nullptr; }
-}
+// A merged launch of a timed substep and the event recorded before it (the
+// next event follows it).
+struct Timed {
+  const Merged *m;
+  size_t ev;
+};
 
-// One substep of every case; timing: eager, with an event after every merged
-// launch (the plan of the substep is returned in *plan).
-int substep(ipg_batch *b, int rkiter, bool timing, Plan **plan) {
+// One substep of every case.  With timed: eager, the groups one after
+// another on the batch's stream, an event after every merged launch, and
+// *timed gets the launches in that order.
+int substep(ipg_batch *b, int rkiter, std::vector<Timed> *timed) {
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("batch: rkiter must be 1..3"); return 1; }
   const int n = (int)b->cs.size();
   for (int i = 0; i < n; i++)
@@ -278,32 +358,60 @@ int substep(ipg_batch *b, int rkiter, bool timing, Plan **plan) {
       return 1;
     }
   const int r = (rkiter - 1) + 3 * par;
-  if (ensure_plan(b, r)) return 1;
-  Plan &P = b->plan[r];
+  if (ensure_plans(b, r)) return 1;
   // the batch's work on a case follows the case's own
   for (int i = 0; i < n; i++) {
     IPG_HIP(hipEventRecord(b->case_ev[i], b->cs[i]->stream));
     IPG_HIP(hipStreamWaitEvent(b->stream, b->case_ev[i], 0));
   }
-  if (b->use_graphs && !timing) {
-    if (!P.graph && capture(b, P)) return 1;
-    IPG_HIP(hipGraphLaunch(P.graph, b->stream));
-  } else if (run_plan(b, P, timing)) {
+  if (timed) {
+    size_t t0 = 0;
+    timed->clear();
+    for (Group &G : b->groups) {
+      if (run_plan(b, G.plan[r], b->stream, true, t0)) return 1;
+      for (size_t q = 0; q < G.plan[r].L.size(); q++) timed->push_back(Timed{&G.plan[r].L[q], t0 + q});
+      t0 += G.plan[r].L.size() + 1;
+    }
+  } else if (b->use_graphs) {
+    if (!b->graph[r] && capture(b, r)) return 1;
+    IPG_HIP(hipGraphLaunch(b->graph[r], b->stream));
+  } else if (run_groups(b, r)) {
     return 1;
   }
   for (int i = 0; i < n; i++) ipg_batch_leave(b->cs[i]);
   // and the case's later work follows the batch's
   IPG_HIP(hipEventRecord(b->done, b->stream));
   for (int i = 0; i < n; i++) IPG_HIP(hipStreamWaitEvent(b->cs[i]->stream, b->done, 0));
-  if (plan) *plan = &P;
   return 0;
 }
 
-// Per-launch times (ms) of the last timed substep of plan P.
-int launch_times(ipg_batch *b, const Plan &P, std::vector<float> &t) {
-  IPG_HIP(hipEventSynchronize(b->tev[P.L.size()]));
-  t.assign(P.L.size(), 0.f);
-  for (size_t q = 0; q < P.L.size(); q++) IPG_HIP(hipEventElapsedTime(&t[q], b->tev[q], b->tev[q + 1]));
+// Per-launch times (ms) of a timed substep.
+int launch_times(ipg_batch *b, const std::vector<Timed> &timed, std::vector<float> &t) {
+  IPG_HIP(hipStreamSynchronize(b->stream));
+  t.assign(timed.size(), 0.f);
+  for (size_t q = 0; q < timed.size(); q++)
+    IPG_HIP(hipEventElapsedTime(&t[q], b->tev[timed[q].ev], b->tev[timed[q].ev + 1]));
+  return 0;
+}
+
+// ms per substep of one RK3 step replayed from the graphs (after one
+// untimed step, which captures them)
+int time_step(ipg_batch *b, float &ms, int &count) {
+  for (int rk = 1; rk <= 3; rk++, count++)
+    if (substep(b, rk, nullptr)) return 1;
+  if (b->tev.size() < 2) {
+    hipEvent_t e[2];
+    IPG_HIP(hipEventCreate(&e[0]));
+    IPG_HIP(hipEventCreate(&e[1]));
+    b->tev.insert(b->tev.end(), e, e + 2);
+  }
+  IPG_HIP(hipEventRecord(b->tev[0], b->stream));
+  for (int rk = 1; rk <= 3; rk++, count++)
+    if (substep(b, rk, nullptr)) return 1;
+  IPG_HIP(hipEventRecord(b->tev[1], b->stream));
+  IPG_HIP(hipEventSynchronize(b->tev[1]));
+  IPG_HIP(hipEventElapsedTime(&ms, b->tev[0], b->tev[1]));
+  ms /= 3.f;
   return 0;
 }
 
@@ -326,7 +434,8 @@ int ipg_batch_create(ipg_ctx *const *cases, int ncases, ipg_batch **out) {
     return fail();
   }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
     ipg_set_error("batch_create: stream or event create failed");
     return fail();
   }
@@ -336,18 +445,20 @@ int ipg_batch_create(ipg_ctx *const *cases, int ncases, ipg_batch **out) {
       ipg_set_error("batch_create: event create failed");
       return fail();
     }
+  if (make_groups(b, 1)) return fail();
   *out = b;
   return 0;
 }
 
 int ipg_batch_destroy(ipg_batch *b) {
   if (!b) return 0;
-  if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (Plan &P : b->plan) drop_plan(P);
+  if (b->stream) (void)hipDeviceSynchronize();
+  drop_groups(b);
   for (hipEvent_t e : b->case_ev)
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : b->tev) (void)hipEventDestroy(e);
   if (b->done) (void)hipEventDestroy(b->done);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
@@ -355,7 +466,7 @@ int ipg_batch_destroy(ipg_batch *b) {
 
 int ipg_batch_substep(ipg_batch *b, int rkiter) {
   if (!b) { ipg_set_error("batch: null batch"); return 1; }
-  return substep(b, rkiter, false, nullptr);
+  return substep(b, rkiter, nullptr);
 }
 
 int ipg_batch_sync(ipg_batch *b) {
@@ -372,10 +483,19 @@ int ipg_batch_use_graphs(ipg_batch *b, int on) {
   return 0;
 }
 
+int ipg_batch_set_groups(ipg_batch *b, int ngroups) {
+  if (!b || ngroups < 1) { ipg_set_error("batch_set_groups: bad arguments"); return 1; }
+  if ((int)b->groups.size() == std::min(ngroups, (int)b->cs.size())) return 0;
+  return make_groups(b, ngroups);
+}
+
 int ipg_batch_autotune(ipg_batch *b, int *substeps) {
   if (!b) { ipg_set_error("batch: null batch"); return 1; }
   int count = 0;
   std::vector<float> t;
+  std::vector<Timed> timed;
+  const int ng0 = (int)b->groups.size();
+  if (ng0 != 1 && make_groups(b, 1)) return 1;
   // the RK stage's z planes per workgroup, the same for every case, timed
   // over one RK3 step each
   int best_kb = 0;
@@ -386,12 +506,10 @@ int ipg_batch_autotune(ipg_batch *b, int *substeps) {
       if (c->tune_rk_kb != k) { c->tune_rk_kb = k; c->gen++; }
     }
     float ms = 0.f;
-    for (int rk = 1; rk <= 3; rk++) {
-      Plan *P = nullptr;
-      if (substep(b, rk, true, &P) || launch_times(b, *P, t)) return 1;
-      count++;
-      for (size_t q = 0; q < P->L.size(); q++)
-        if (P->L[q].stage.rfind("rk_stage#", 0) == 0) ms += t[q];
+    for (int rk = 1; rk <= 3; rk++, count++) {
+      if (substep(b, rk, &timed) || launch_times(b, timed, t)) return 1;
+      for (size_t q = 0; q < timed.size(); q++)
+        if (timed[q].m->stage.rfind("rk_stage#", 0) == 0) ms += t[q];
     }
     if (best_kb == 0 || ms < best_ms) { best_kb = kb; best_ms = ms; }
   }
@@ -405,12 +523,10 @@ int ipg_batch_autotune(ipg_batch *b, int *substeps) {
   b->persist.clear();
   for (int f : {0, 1, 2, 4}) {
     b->persist_default = f;
-    for (int rk = 1; rk <= 3; rk++) {
-      Plan *P = nullptr;
-      if (substep(b, rk, true, &P) || launch_times(b, *P, t)) return 1;
-      count++;
-      for (size_t q = 0; q < P->L.size(); q++) {
-        const std::string &key = P->L[q].tune_key;
+    for (int rk = 1; rk <= 3; rk++, count++) {
+      if (substep(b, rk, &timed) || launch_times(b, timed, t)) return 1;
+      for (size_t q = 0; q < timed.size(); q++) {
+        const std::string &key = timed[q].m->tune_key;
         auto it = best.find(key);
         if (it == best.end() || t[q] < it->second.second) best[key] = {f, t[q]};
       }
@@ -419,6 +535,15 @@ int ipg_batch_autotune(ipg_batch *b, int *substeps) {
   b->persist_default = 0;
   for (const auto &kv : best) b->persist[kv.first] = kv.second.first;
   drop_graphs(b);
+  // the number of stream groups, timed on graph replays of whole steps
+  int best_ng = 1;
+  float best_step = 0.f;
+  for (int ng = 1; ng <= std::min(4, (int)b->cs.size()); ng *= 2) {
+    float ms = 0.f;
+    if (make_groups(b, ng) || time_step(b, ms, count)) return 1;
+    if (ng == 1 || ms < best_step) { best_ng = ng; best_step = ms; }
+  }
+  if ((int)b->groups.size() != best_ng && make_groups(b, best_ng)) return 1;
   if (substeps) *substeps = count;
   return 0;
 }
@@ -426,27 +551,34 @@ int ipg_batch_autotune(ipg_batch *b, int *substeps) {
 int ipg_batch_launches(ipg_batch *b, int rkiter, int *nlaunch, int *npersistent) {
   if (!b || !nlaunch || rkiter < 1 || rkiter > 3) { ipg_set_error("batch_launches: bad arguments"); return 1; }
   const int par = b->cs[0]->cpar;
-  const Plan &P = b->plan[(rkiter - 1) + 3 * par];
-  *nlaunch = (int)P.L.size();
-  int np = 0;
-  for (const Merged &M : P.L) {
-    auto it = b->persist.find(M.tune_key);
-    const int f = it == b->persist.end() ? b->persist_default : it->second;
-    if (f > 0 && std::max<long>(8, (long)f * b->ncu / 8 * 8) < M.total) np++;
-  }
+  int nl = 0, np = 0;
+  for (const Group &G : b->groups)
+    for (const Merged &M : G.plan[(rkiter - 1) + 3 * par].L) {
+      nl++;
+      auto it = b->persist.find(M.tune_key);
+      const int f = it == b->persist.end() ? b->persist_default : it->second;
+      if (f > 0 && std::max<long>(8, (long)f * b->ncu / 8 * 8) < M.total) np++;
+    }
+  *nlaunch = nl;
   if (npersistent) *npersistent = np;
+  return 0;
+}
+
+int ipg_batch_groups(ipg_batch *b, int *ngroups) {
+  if (!b || !ngroups) { ipg_set_error("batch_groups: bad arguments"); return 1; }
+  *ngroups = (int)b->groups.size();
   return 0;
 }
 
 int ipg_batch_phases(ipg_batch *b, int rkiter, int maxn, double *ms, const char **names, int *n) {
   if (!b || !ms || !names || !n) { ipg_set_error("batch_phases: bad arguments"); return 1; }
-  Plan *P = nullptr;
+  std::vector<Timed> timed;
   std::vector<float> t;
-  if (substep(b, rkiter, true, &P) || launch_times(b, *P, t)) return 1;
+  if (substep(b, rkiter, &timed) || launch_times(b, timed, t)) return 1;
   int k = 0;
-  for (size_t q = 0; q < P->L.size() && k < maxn; q++, k++) {
+  for (size_t q = 0; q < timed.size() && k < maxn; q++, k++) {
     ms[k] = t[q];
-    names[k] = P->L[q].stage.c_str();
+    names[k] = timed[q].m->stage.c_str();
   }
   *n = k;
   return 0;

@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include "../../include/ipg.h"
 #include "ipg_batch.h"
 
@@ -185,12 +186,15 @@ int ipg_rec_add(struct IpgRec *r, BatchFns fns, dim3 grid, dim3 block, size_t ld
 // Launches kernel body K for context c: its single-context kernel `one` on the
 // context's stream, marked for timing as `stage`, or, while a batch records
 // the context, a record of the launch.
+// `one` is the single-context kernel (taking the argument struct), or a
+// callable that launches it on the stream it is given (for a kernel that
+// keeps its arguments as separate parameters).
 #if defined(__HIPCC__)
-template <class K>
-int ipg_go(ipg_ctx *c, void (*one)(typename K::Args), dim3 grid, dim3 block, size_t lds, const typename K::Args &a,
-           const char *stage) {
+template <class K, class One>
+int ipg_go(ipg_ctx *c, One one, dim3 grid, dim3 block, size_t lds, const typename K::Args &a, const char *stage) {
   if (c->rec) return ipg_rec_add(c->rec, batch_kernels<K>(), grid, block, lds, &a, sizeof(a), stage);
-  hipLaunchKernelGGL(one, grid, block, lds, c->stream, a);
+  if constexpr (std::is_invocable_v<One, hipStream_t>) one(c->stream);
+  else hipLaunchKernelGGL(one, grid, block, lds, c->stream, a);
   ipg_mark(c, stage);
   return 0;
 }

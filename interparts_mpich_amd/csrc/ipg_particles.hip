@@ -46,149 +46,21 @@ __device__ __forceinline__ void image_shift(const Grid &g, int nbrecv, double &x
 // queue is then worked through densely, so that the eight corner square roots
 // are evaluated only by the lanes that need them.
 constexpr int SPH_THREADS = 512, SPH_SLICE = 16 * SPH_THREADS;
+// the body of k_sphere_int for a batch's merged launches
+template <int TYPE>
 __device__ __forceinline__ void sphere_int_body(const Grid &g, const PartConst &pc, const double *__restrict__ U,
                                                 const double *__restrict__ V, const double *__restrict__ W, const PQ &P,
-                                                const int *__restrict__ nbf, int type, double *op_save, const VB vb) {
+                                                const int *__restrict__ nbf, double *op_save, const VB vb) {
   const int p = vb.x;
-  __shared__ double acc[IPG_NB][6];
-  __shared__ double red[6][SPH_THREADS / 64];
-  __shared__ int cut[SPH_SLICE];
-  __shared__ int ncut;
-  if (threadIdx.x < IPG_NB * 6) acc[threadIdx.x / 6][threadIdx.x % 6] = 0.0;
-  __syncthreads();
-  const double R = pc.radius, dVeul = pc.dVeul;
-  const double dx = 1.0 / g.dxi, dy = 1.0 / g.dyi, dz = 1.0 / g.dzi;
-  const double radin = R - 2.0 * dx, radin2 = radin * radin;
-  const bool t1 = type == 1, t2 = type == 2 || type == 23, t3 = type == 3 || type == 23;
-  for (int nb = 0; nb < IPG_NB; nb++) {
-    if (nb > 0 && nbf[p * IPG_NB + nb] != 1) continue;
-    int nbrecv = nb;
-    if (nb > 0) { nbrecv = nb + 4; if (nbrecv > 8) nbrecv -= 8; }
-    double X = P.q[IPG_PX][p], Y = P.q[IPG_PY][p];
-    const double Z = P.q[IPG_PZ][p];
-    image_shift(g, nbrecv, X, Y);
-    const double cxc = X - 0.0, cyc = Y - 0.0, czc = Z;
-    int ilow = nint_f((cxc - R) * g.dxi - 2.0), ihigh = nint_f((cxc + R) * g.dxi + 2.0);
-    int jlow = nint_f((cyc - R) * g.dyi - 2.0), jhigh = nint_f((cyc + R) * g.dyi + 2.0);
-    int klow = nint_f((czc - R) * g.dzi - 2.0), khigh = nint_f((czc + R) * g.dzi + 2.0);
-    ilow = max(ilow, 1); jlow = max(jlow, 1); klow = max(klow, 1);
-    ihigh = min(ihigh, g.nx); jhigh = min(jhigh, g.ny); khigh = min(khigh, g.nz);
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    const int ni = ihigh - ilow + 1, nj = jhigh - jlow + 1, nk = khigh - klow + 1;
-    if (ni > 0 && nj > 0 && nk > 0) {
-      const int nij = ni * nj, tot = nij * nk;
-      // a thread's cell t = ii + ni (jj + nj kk) advances by SPH_THREADS per
-      // step: (ii, jj, kk) by (si, sj, sk) with carries
-      const int sk = SPH_THREADS / nij, r0 = SPH_THREADS - sk * nij, sj = r0 / ni, si = r0 - sj * ni;
-      for (int comp = 0; comp < 3; comp++) {
-        const double *F = comp == 0 ? U : (comp == 1 ? V : W);
-        // corner coordinates of cell (i, j, k) for this component's grid
-        auto corners = [&](int i, int j, int k, double &xmin, double &xplus, double &ymin, double &yplus, double &zmin,
-                           double &zplus) {
-          if (comp == 2) { zmin = ((double)k - 0.5) * dz; zplus = ((double)k + 0.5) * dz; }
-          else { zmin = (double)(k - 1) * dz; zplus = (double)k * dz; }
-          if (comp == 1) { ymin = 0.0 + ((double)j - 0.5) * dy; yplus = 0.0 + ((double)j + 0.5) * dy; }
-          else { ymin = 0.0 + (double)(j - 1) * dy; yplus = 0.0 + (double)j * dy; }
-          if (comp == 0) { xmin = 0.0 + ((double)i - 0.5) * dx; xplus = 0.0 + ((double)i + 0.5) * dx; }
-          else { xmin = 0.0 + (double)(i - 1) * dx; xplus = 0.0 + (double)i * dx; }
-        };
-        // src/intgr_over_sphere.f90:131-300 contribution of one cell
-        auto add = [&](int i, int j, int k, double fval, bool full, double wfrac) {
-          if (t1) s[comp] += full ? dVeul : wfrac * dVeul;
-          if (t2) s[comp] += full ? fval * dVeul : fval * wfrac * dVeul;
-          if (t3) {
-            const double rx = 0.0 + ((double)i - 0.5) * dx - X;
-            const double ry = 0.0 + ((double)j - 0.5) * dy - Y;
-            const double rz = ((double)k - 0.5) * dz - Z;
-            if (comp == 0) {
-              s[4] += full ? rz * fval * dVeul : rz * fval * wfrac * dVeul;
-              s[5] -= full ? ry * fval * dVeul : ry * fval * wfrac * dVeul;
-            } else if (comp == 1) {
-              s[3] -= full ? rz * fval * dVeul : rz * fval * wfrac * dVeul;
-              s[5] += full ? rx * fval * dVeul : rx * fval * wfrac * dVeul;
-            } else {
-              s[3] += full ? ry * fval * dVeul : ry * fval * wfrac * dVeul;
-              s[4] -= full ? rx * fval * dVeul : rx * fval * wfrac * dVeul;
-            }
-          }
-        };
-        int t = threadIdx.x;
-        int kk = t / nij, ii = t - kk * nij, jj = ii / ni;
-        ii -= jj * ni;
-        for (int t0 = 0; t0 < tot; t0 += SPH_SLICE) {
-          if (threadIdx.x == 0) ncut = 0;
-          __syncthreads();
-          const int tend = min(t0 + SPH_SLICE, tot);
-          for (; t < tend; t += SPH_THREADS) {
-            const int i = ilow + ii, j = jlow + jj, k = klow + kk;
-            double xmin, xplus, ymin, yplus, zmin, zplus;
-            corners(i, j, k, xmin, xplus, ymin, yplus, zmin, zplus);
-            // a cell whose nearest point lies beyond R has every corner
-            // outside: its weight is exactly 0 and it adds nothing
-            const double ex = xmin > X ? xmin - X : (xplus < X ? X - xplus : 0.0);
-            const double ey = ymin > Y ? ymin - Y : (yplus < Y ? Y - yplus : 0.0);
-            const double ez = zmin > Z ? zmin - Z : (zplus < Z ? Z - zplus : 0.0);
-            if (ex * ex + ey * ey + ez * ez <= R * R * (1.0 + 1e-9)) {
-              const double dist2 = sqd(xplus - X) + sqd(yplus - Y) + sqd(zplus - Z);
-              if (dist2 < radin2) add(i, j, k, F[g.at(i, j, k)], true, 1.0);
-              else cut[atomicAdd(&ncut, 1)] = ii | (jj << 10) | (kk << 20);
-            }
-            ii += si; jj += sj; kk += sk;
-            if (ii >= ni) { ii -= ni; jj++; }
-            if (jj >= nj) { jj -= nj; kk++; }
-          }
-          __syncthreads();
-          const int n = ncut;
-          for (int q = threadIdx.x; q < n; q += SPH_THREADS) {
-            // box cell packed ten bits per axis (boxes are narrower than
-            // 1024 cells, checked at set-up)
-            const int e = cut[q];
-            const int i = ilow + (e & 1023), j = jlow + ((e >> 10) & 1023), k = klow + (e >> 20);
-            double xmin, xplus, ymin, yplus, zmin, zplus;
-            corners(i, j, k, xmin, xplus, ymin, yplus, zmin, zplus);
-            const double dzp2 = sqd(zplus - Z), dyp2 = sqd(yplus - Y), dxp2 = sqd(xplus - X);
-            const double dxm2 = sqd(xmin - X), dym2 = sqd(ymin - Y), dzm2 = sqd(zmin - Z);
-            double sg[8];
-            sg[0] = R - sqrt(dxm2 + dym2 + dzm2); sg[1] = R - sqrt(dxm2 + dyp2 + dzm2);
-            sg[2] = R - sqrt(dxp2 + dyp2 + dzm2); sg[3] = R - sqrt(dxp2 + dym2 + dzm2);
-            sg[4] = R - sqrt(dxm2 + dym2 + dzp2); sg[5] = R - sqrt(dxm2 + dyp2 + dzp2);
-            sg[6] = R - sqrt(dxp2 + dyp2 + dzp2); sg[7] = R - sqrt(dxp2 + dym2 + dzp2);
-            double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-            for (int cp = 0; cp < 8; cp++) { if (sg[cp] > 0.0) s1 = s1 + sg[cp]; s2 = s2 + fabs(sg[cp]); }
-            add(i, j, k, F[g.at(i, j, k)], false, s1 / s2);
-          }
-          __syncthreads();
-        }
-      }
-    }
-    for (int q = 0; q < 6; q++) {
-      double v = s[q];
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-      double v = 0.0;
-      for (int w = 0; w < SPH_THREADS / 64; w++) v += red[threadIdx.x][w];
-      acc[nbrecv][threadIdx.x] = v;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x < 6) {
-    // src/intgr_over_sphere.f90:564-599: sum of the nine slots in order
-    double t = 0.0;
-    for (int nb = 0; nb < IPG_NB; nb++) t = t + acc[nb][threadIdx.x];
-    const int q = threadIdx.x;
-    // with op_save: the integrals at the start of intgr_nwtn_eulr
-    // (src/intgr_nwtn_eulr.f90:77-118, op slots 12..17), before they change
-    if (op_save) op_save[(long)(12 + q) * pc.np + p] = P.q[IPG_PINTU + q][p];
-    if (q < 3) {
-      if (t1 || t2) P.q[IPG_PINTU + q][p] = t;
-    } else if (t3) {
-      P.q[IPG_PINTOMX + (q - 3)][p] = t;
-    }
-  }
+  constexpr int type = TYPE;
+#include "ipg_sphere_int_body.inc"
+}
+
+__global__ __launch_bounds__(SPH_THREADS) void k_sphere_int(Grid g, PartConst pc, const double *__restrict__ U,
+                                                            const double *__restrict__ V, const double *__restrict__ W, PQ P,
+                                                            const int *__restrict__ nbf, int type, double *op_save) {
+  const int p = blockIdx.x;
+#include "ipg_sphere_int_body.inc"
 }
 
 struct SphereIntArgs {
@@ -196,18 +68,30 @@ struct SphereIntArgs {
   const double *U, *V, *W;
   PQ P;
   const int *nbf;
-  int type;
   double *op_save;
 };
+// TYPE (1, 2, 3 or 23) at compile time: each kind of integral its own code
+template <int TYPE>
 struct SphereIntK {
   using Args = SphereIntArgs;
-  static constexpr int THREADS = SPH_THREADS, WPE = 6;  // three blocks per CU
+  static constexpr int THREADS = SPH_THREADS, WPE = 6;  // three blocks per CU, as k_sphere_int
   static __device__ __forceinline__ void run(const Args &a, const VB v) {
-    sphere_int_body(a.g, a.pc, a.U, a.V, a.W, a.P, a.nbf, a.type, a.op_save, v);
+    sphere_int_body<TYPE>(a.g, a.pc, a.U, a.V, a.W, a.P, a.nbf, a.op_save, v);
   }
 };
-__global__ __launch_bounds__(SPH_THREADS) __attribute__((amdgpu_waves_per_eu(SphereIntK::WPE))) void k_sphere_int(SphereIntArgs) {
-  run_single<SphereIntK>();
+int launch_sphere_int(ipg_ctx *c, const SphereIntArgs &a, int type, dim3 grid, const char *stage) {
+  const dim3 blk(SPH_THREADS);
+  auto one = [&](hipStream_t s) {
+    hipLaunchKernelGGL(k_sphere_int, grid, blk, 0, s, a.g, a.pc, a.U, a.V, a.W, a.P, a.nbf, type, a.op_save);
+  };
+  switch (type) {
+    case 1: return ipg_go<SphereIntK<1>>(c, one, grid, blk, 0, a, stage);
+    case 2: return ipg_go<SphereIntK<2>>(c, one, grid, blk, 0, a, stage);
+    case 3: return ipg_go<SphereIntK<3>>(c, one, grid, blk, 0, a, stage);
+    case 23: return ipg_go<SphereIntK<23>>(c, one, grid, blk, 0, a, stage);
+  }
+  ipg_set_error("intgr_over_sphere: type must be 1, 2, 3 or 23");
+  return 1;
 }
 
 // src/collisions.f90:222-438, normal squeeze force only (lines 430-432)
@@ -629,8 +513,8 @@ int ipg_launch_phase_sol(ipg_ctx *c, double *out) {
 
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type) {
   if (c->pc.np <= 0) return 0;
-  const SphereIntArgs a{c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], make_pq(c), c->ps.nbf, type, nullptr};
-  hipLaunchKernelGGL(k_sphere_int, dim3(c->pc.np), dim3(SPH_THREADS), 0, c->stream, a);
+  const SphereIntArgs a{c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], make_pq(c), c->ps.nbf, nullptr};
+  if (launch_sphere_int(c, a, type, dim3(c->pc.np), "sphere_int")) return 1;
   IPG_HIP(hipGetLastError());
   return 0;
 }
@@ -651,8 +535,8 @@ int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter) {
   const int np = c->pc.np;
   if (np <= 0) return 0;
   const PQ P = make_pq(c);
-  const SphereIntArgs sa{c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], P, c->ps.nbf, 23, c->op};
-  if (ipg_go<SphereIntK>(c, k_sphere_int, dim3(np), dim3(SPH_THREADS), 0, sa, "sphere_int")) return 1;
+  const SphereIntArgs sa{c->g, c->pc, c->f[IPG_U], c->f[IPG_V], c->f[IPG_W], P, c->ps.nbf, c->op};
+  if (launch_sphere_int(c, sa, 23, dim3(np), "sphere_int")) return 1;
   double **q = c->ps.q;
   const Motion state{q[IPG_PX], q[IPG_PY], q[IPG_PZ], q[IPG_PU], q[IPG_PV], q[IPG_PW], q[IPG_POMX], q[IPG_POMY], q[IPG_POMZ]};
   double *m = c->motion2;

@@ -259,6 +259,15 @@ class Batch:
     def substep(self, rkiter):
         check(self.lib.ipg_batch_substep(self.handle, rkiter), "batch_substep")
 
+    def set_groups(self, n):
+        """Split the cases into n stream groups (ipg_batch_set_groups)."""
+        check(self.lib.ipg_batch_set_groups(self.handle, n), "batch_set_groups")
+
+    def groups(self):
+        n = C.c_int()
+        check(self.lib.ipg_batch_groups(self.handle, C.byref(n)), "batch_groups")
+        return n.value
+
     def sync(self):
         check(self.lib.ipg_batch_sync(self.handle), "batch_sync")
 

@@ -78,6 +78,31 @@ def test_batch_matches_cases_alone(graphs):
     b.close()
 
 
+@pytest.mark.parametrize("ngroups", [2, 3])
+def test_batch_stream_groups(ngroups):
+    """The cases split into stream groups whose merged launches run side by
+    side in one graph, regrouped between steps."""
+    cases = cases_small()
+    bat = [make(c) for c in cases]
+    alone = [make(c) for c in cases]
+    b = Batch(bat)
+    b.set_groups(ngroups)
+    assert b.groups() == ngroups
+    for step in range(2):
+        if step == 1:
+            b.set_groups(ngroups - 1)
+        for rk in (1, 2, 3):
+            b.substep(rk)
+            for s in alone:
+                s.substep(rk)
+    b.sync()
+    for x, y in zip(bat, alone):
+        w = worst(x, y)
+        print(f"[batch in {ngroups} groups vs alone] {x.case.name}: {w:.3e}", flush=True)
+        assert w <= 1e-12, (x.case.name, w)
+    b.close()
+
+
 def test_batch_with_per_case_checks_between_steps():
     """chkdiv, chkdt and set_dt on each case between batch steps: per-case
     entry points and the batch's merged launches stay ordered."""
@@ -115,7 +140,7 @@ def test_batch_tuner_keeps_results():
     nsub = b.autotune()
     assert nsub % 3 == 0 and nsub > 0
     n, npers = b.launches(1)
-    print(f"[tuner] {nsub} substeps; {npers} of {n} merged launches persistent", flush=True)
+    print(f"[tuner] {nsub} substeps; {npers} of {n} merged launches persistent; {b.groups()} stream groups", flush=True)
     for q in range(nsub + 6):
         rk = q % 3 + 1
         if q >= nsub:
