@@ -66,8 +66,19 @@ void derived(ipg_ctx *c) {
   pc.dVeul = 1.0 / (g.dxi * g.dyi * g.dzi);
 }
 
+// Every entry point on a context passes here: work it enqueues on the
+// context's stream follows the batch substeps already issued on the case
+// (their completion event), and the call is counted, so that the next batch
+// substep waits for the context's stream only when something was enqueued
+// there since the last one.
 bool check_ctx(ipg_ctx *c) {
   if (!c) { ipg_set_error("null context"); return false; }
+  c->calls++;
+  if (c->batch_pending) {
+    c->batch_pending = false;
+    const hipError_t e = hipStreamWaitEvent(c->stream, c->batch_ev, 0);
+    if (e != hipSuccess) { ipg_set_error(std::string("batch wait: ") + hipGetErrorString(e)); return false; }
+  }
   return true;
 }
 
@@ -276,11 +287,12 @@ int ipg_batch_record(ipg_ctx *c, int rkiter, int par, IpgRec *rec) {
 }
 
 // Before a merged substep: c is set up, and an update a fused substep of its
-// own left pending has run.
+// own left pending has run (on its stream, which then counts as used).
 int ipg_batch_enter(ipg_ctx *c) {
-  if (!check_ctx(c)) return 1;
+  if (!c) { ipg_set_error("null context"); return 1; }
   if (!c->solver_ready) { ipg_set_error("batch: ipg_initsolver has not been called on a case"); return 1; }
   if (c->pc.np > 0 && !c->markers_ready) { ipg_set_error("batch: ipg_coordsfp has not been called on a case"); return 1; }
+  if (c->ne_pending && !check_ctx(c)) return 1;
   if (settle(c)) return 1;
   // the substep rewrites dcdt's interior before swapping it in as c
   c->dcdt_stale = false;
@@ -288,8 +300,11 @@ int ipg_batch_enter(ipg_ctx *c) {
 }
 
 // After a merged substep: the host-side state its launches leave, as after a
-// graph replay of the context's own substep.
-void ipg_batch_leave(ipg_ctx *c) {
+// graph replay of the context's own substep; done is recorded after the
+// merged launches, and the context's next entry point waits for it.
+void ipg_batch_leave(ipg_ctx *c, hipEvent_t done) {
+  c->batch_ev = done;
+  c->batch_pending = true;
   std::swap(c->f[IPG_C], c->f[IPG_DCDT]);
   c->cpar ^= 1;
   c->p_ghosts_stale = true;
@@ -420,6 +435,7 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
 
 int ipg_finalize(ipg_ctx *c) {
   if (!c) return 0;
+  (void)check_ctx(c);  // after the batch substeps issued on the case
   (void)hipStreamSynchronize(c->stream);
   release(c);
   delete c;

@@ -83,6 +83,7 @@ struct ipg_batch {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr, fork = nullptr;
   std::vector<hipEvent_t> case_ev;
+  std::vector<unsigned long> seen;  // each case's entry-point count when the batch last waited for its stream
   std::vector<hipEvent_t> tev;  // timing events
   bool use_graphs = true;
   int ncu = 256;
@@ -359,10 +360,14 @@ int substep(ipg_batch *b, int rkiter, std::vector<Timed> *timed) {
     }
   const int r = (rkiter - 1) + 3 * par;
   if (ensure_plans(b, r)) return 1;
-  // the batch's work on a case follows the case's own
+  // the batch's work on a case follows the case's own, where there was any
+  // since the batch last waited for it
   for (int i = 0; i < n; i++) {
-    IPG_HIP(hipEventRecord(b->case_ev[i], b->cs[i]->stream));
+    ipg_ctx *c = b->cs[i];
+    if (c->calls == b->seen[i]) continue;
+    IPG_HIP(hipEventRecord(b->case_ev[i], c->stream));
     IPG_HIP(hipStreamWaitEvent(b->stream, b->case_ev[i], 0));
+    b->seen[i] = c->calls;
   }
   if (timed) {
     size_t t0 = 0;
@@ -378,10 +383,10 @@ int substep(ipg_batch *b, int rkiter, std::vector<Timed> *timed) {
   } else if (run_groups(b, r)) {
     return 1;
   }
-  for (int i = 0; i < n; i++) ipg_batch_leave(b->cs[i]);
-  // and the case's later work follows the batch's
+  // and the case's later work follows the batch's: the next entry point on
+  // the case makes its stream wait for this event
   IPG_HIP(hipEventRecord(b->done, b->stream));
-  for (int i = 0; i < n; i++) IPG_HIP(hipStreamWaitEvent(b->cs[i]->stream, b->done, 0));
+  for (int i = 0; i < n; i++) ipg_batch_leave(b->cs[i], b->done);
   return 0;
 }
 
@@ -440,6 +445,8 @@ int ipg_batch_create(ipg_ctx *const *cases, int ncases, ipg_batch **out) {
     return fail();
   }
   b->case_ev.assign(ncases, nullptr);
+  // (unsigned long)-1: wait for every case's stream before the first substep
+  b->seen.assign(ncases, (unsigned long)-1);
   for (int i = 0; i < ncases; i++)
     if (hipEventCreateWithFlags(&b->case_ev[i], hipEventDisableTiming) != hipSuccess) {
       ipg_set_error("batch_create: event create failed");
@@ -453,6 +460,9 @@ int ipg_batch_create(ipg_ctx *const *cases, int ncases, ipg_batch **out) {
 int ipg_batch_destroy(ipg_batch *b) {
   if (!b) return 0;
   if (b->stream) (void)hipDeviceSynchronize();
+  // the cases no longer wait for the batch's event (its work is done)
+  for (ipg_ctx *c : b->cs)
+    if (c->batch_pending && c->batch_ev == b->done) c->batch_pending = false;
   drop_groups(b);
   for (hipEvent_t e : b->case_ev)
     if (e) (void)hipEventDestroy(e);

@@ -141,6 +141,11 @@ struct ipg_ctx {
   // solver set-up, the RK-stage launch shape, the refinement sweeps): a batch
   // re-records its merged launches when a case's count moves
   unsigned gen;
+  // entry-point calls on the context (check_ctx), and the batch's completion
+  // event its stream must wait for before its next work (batch_pending)
+  unsigned long calls;
+  hipEvent_t batch_ev;
+  bool batch_pending;
 };
 
 // Every device allocation a context holds outside the Poisson plan (which
@@ -203,7 +208,7 @@ int ipg_go(ipg_ctx *c, One one, dim3 grid, dim3 block, size_t lds, const typenam
 // a context's part in a batch (ipg_api.hip)
 int ipg_batch_record(ipg_ctx *c, int rkiter, int par, struct IpgRec *rec);
 int ipg_batch_enter(ipg_ctx *c);
-void ipg_batch_leave(ipg_ctx *c);
+void ipg_batch_leave(ipg_ctx *c, hipEvent_t done);
 
 // error plumbing
 void ipg_set_error(const std::string &msg);

@@ -129,6 +129,30 @@ def test_batch_with_per_case_checks_between_steps():
     b.close()
 
 
+def test_batch_interleaved_with_cases_stepping_alone():
+    """A batch step, then every case steps on its own (ipg_substep on its own
+    stream), then a batch step again: each kind of work waits for the other."""
+    cases = cases_small()[:4]
+    bat = [make(c) for c in cases]
+    alone = [make(c) for c in cases]
+    b = Batch(bat)
+    for step in range(3):
+        for rk in (1, 2, 3):
+            if step == 1:
+                for s in bat:
+                    s.substep(rk)
+            else:
+                b.substep(rk)
+            for s in alone:
+                s.substep(rk)
+    b.sync()
+    for x, y in zip(bat, alone):
+        w = worst(x, y)
+        print(f"[batch and own steps vs alone] {x.case.name}: {w:.3e}", flush=True)
+        assert w <= 1e-12, (x.case.name, w)
+    b.close()
+
+
 def test_batch_tuner_keeps_results():
     """The tuner times whole RK3 steps (RK launch shapes, persistent grids);
     the steps it runs are ordinary substeps, and the tuned graphs replay the
