@@ -14,6 +14,9 @@ module mod_ipg_capi
             ipg_boundscal, ipg_fillps, ipg_solver2d, ipg_correc, ipg_pnew_update, ipg_intgr_over_sphere, &
             ipg_intgr_nwtn_eulr, ipg_substep, ipg_chkdt, ipg_chkdiv, ipg_uniform, ipg_sync, ipg_use_graphs, &
             ipg_use_side_stream, ipg_autotune, ipg_post3d, ipg_kerneltest, ipg_ibm_forces
+  ! the batched ensemble (no reference counterpart: many cases through each
+  ! substep together, include/ipg.h)
+  public :: ipg_batch_create, ipg_batch_destroy, ipg_batch_substep, ipg_batch_sync, ipg_batch_autotune
 
   ! field selectors (enum of include/ipg.h)
   integer(c_int), parameter, public :: IPG_U = 0, IPG_V = 1, IPG_W = 2, IPG_PNEW = 3, IPG_C = 4, &
@@ -229,6 +232,30 @@ module mod_ipg_capi
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), intent(out) :: kb
+    end function
+    integer(c_int) function ipg_batch_create(cases, ncases, batch) bind(C, name="ipg_batch_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: cases(*)
+      integer(c_int), value :: ncases
+      type(c_ptr), intent(out) :: batch
+    end function
+    integer(c_int) function ipg_batch_destroy(batch) bind(C, name="ipg_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+    integer(c_int) function ipg_batch_substep(batch, rkiter) bind(C, name="ipg_batch_substep")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: rkiter
+    end function
+    integer(c_int) function ipg_batch_sync(batch) bind(C, name="ipg_batch_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+    integer(c_int) function ipg_batch_autotune(batch, substeps) bind(C, name="ipg_batch_autotune")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), intent(out) :: substeps
     end function
   end interface
 
