@@ -204,7 +204,7 @@ def run_ensemble(args):
     batch.sync()
     t_fused = time.perf_counter() - t0
     nl, npers = batch.launches(1)
-    ngroups = batch.groups()
+    ngroups, carry = batch.groups(), batch.carry()
     phases = {}
     for rk in (1, 2, 3):
         for name, ms in batch.phases(rk):
@@ -212,7 +212,8 @@ def run_ensemble(args):
             phases[key] = phases.get(key, 0.0) + ms / 3.0
     batch.sync()
     print(f"[bench] fused batch: {len(sims) * steps / t_fused:.1f} case-substeps/s, {nl} merged launches per substep "
-          f"({npers} persistent) in {ngroups} stream groups", file=sys.stderr, flush=True)
+          f"({npers} persistent) in {ngroups} stream groups, updates carried beside the RK stage: {carry}",
+          file=sys.stderr, flush=True)
     print("[bench] fused batch per-substep stage time (ms, eager): " +
           ", ".join(f"{k}={v:.3f}" for k, v in sorted(phases.items(), key=lambda kv: -kv[1])), file=sys.stderr, flush=True)
     n = len(sims) * steps
@@ -227,6 +228,7 @@ def run_ensemble(args):
                                "D 8-12h, Re_bulk 1000-5000",
                    "gcell_updates_per_s": round(cells / t_fused / 1e9, 4),
                    "merged_launches_per_substep": nl, "persistent_launches": npers, "stream_groups": ngroups,
+                   "updates_beside_rk_stage": carry,
                    "tuner_substeps": tuned},
         "concurrent_contexts_case_substeps_per_s": round(n / t_conc, 2),
         "sequential_case_substeps_per_s": round(n / t_seq, 2),

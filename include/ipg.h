@@ -207,7 +207,8 @@ int ipg_substep_phases(ipg_ctx *ctx, int rkiter, int maxn, double *ms, const cha
  * cases must start every batch substep at the same c/dcdt buffer parity (the
  * same number of substeps run, or an even difference), the Newton-Euler
  * update closes each substep, and itot, jtot must have compiled FFT plans
- * (16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512). */
+ * (16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512).  The batch refers to
+ * its cases: destroy it before any of them is finalized. */
 typedef struct ipg_batch ipg_batch;
 int ipg_batch_create(ipg_ctx *const *cases, int ncases, ipg_batch **out);
 int ipg_batch_destroy(ipg_batch *batch);
@@ -222,11 +223,19 @@ int ipg_batch_use_graphs(ipg_batch *batch, int on);
  * (default 1; the tuner chooses); ipg_batch_groups reads the number back */
 int ipg_batch_set_groups(ipg_batch *batch, int ngroups);
 int ipg_batch_groups(ipg_batch *batch, int *ngroups);
+/* on (default): each case's Newton-Euler update is left to the next batch
+ * substep, which runs it on a side stream beside its RK stage, as ipg_substep
+ * does with its side stream; off: the update closes its substep.  Either way
+ * the next entry point on a case that needs the update runs it first.  The
+ * tuner chooses; ipg_batch_carry reads the setting back */
+int ipg_batch_set_carry(ipg_batch *batch, int on);
+int ipg_batch_carry(ipg_batch *batch, int *on);
 /* The batch's tuner: times whole RK3 steps with every case's RK-stage launch
  * shape at 16, 32 and 64 z planes per workgroup and keeps the fastest, then,
  * for every merged launch, one workgroup per block against persistent grids of
  * 1, 2 and 4 workgroups per CU striding over the blocks, and keeps the fastest
- * per launch, then 1, 2 and 4 stream groups on graph replays of whole steps.
+ * per launch, then 1, 2 and 4 stream groups and the carried updates on or
+ * off, on graph replays of whole steps.
  * The timed steps advance every case (results do not depend on the launch
  * shapes): call it at a time-step boundary (after rkiter 3); *substeps gets
  * the number of substeps it ran (a multiple of 3). */

@@ -30,7 +30,8 @@ EXPORTS = ["ipg_init", "ipg_finalize", "ipg_last_error", "ipg_nl_formula", "ipg_
            "ipg_post3d",
            "ipg_sync", "ipg_use_graphs", "ipg_use_side_stream", "ipg_set_refinement", "ipg_autotune", "ipg_substep_bytes",
            "ipg_substep_phases", "ipg_batch_create", "ipg_batch_destroy", "ipg_batch_substep", "ipg_batch_sync",
-           "ipg_batch_use_graphs", "ipg_batch_set_groups", "ipg_batch_groups", "ipg_batch_autotune", "ipg_batch_launches", "ipg_batch_phases"]
+           "ipg_batch_use_graphs", "ipg_batch_set_groups", "ipg_batch_groups", "ipg_batch_set_carry", "ipg_batch_carry",
+           "ipg_batch_autotune", "ipg_batch_launches", "ipg_batch_phases"]
 
 
 class IpgParams(C.Structure):
@@ -110,6 +111,8 @@ def load():
         "ipg_batch_use_graphs": ([vp, ip], ip),
         "ipg_batch_set_groups": ([vp, ip], ip),
         "ipg_batch_groups": ([vp, C.POINTER(ip)], ip),
+        "ipg_batch_set_carry": ([vp, ip], ip),
+        "ipg_batch_carry": ([vp, C.POINTER(ip)], ip),
         "ipg_batch_autotune": ([vp, C.POINTER(ip)], ip),
         "ipg_batch_launches": ([vp, ip, C.POINTER(ip), C.POINTER(ip)], ip),
         "ipg_batch_phases": ([vp, ip, ip, dp, C.POINTER(C.c_char_p), C.POINTER(ip)], ip),

@@ -136,21 +136,29 @@ int release(ipg_ctx *c) {
 // pending instead of launching it at the end.
 int substep_launches(ipg_ctx *c, int rkiter, int pend, bool defer) {
   const bool np = c->pc.np > 0;
+  // (a batch recording the launches keeps the fork and the join as marks of
+  // its own: the records of the side stream, and where the main one joins it)
   if (pend) {
     // the update, then the IBM's neighbour lists of the positions it gives
-    IPG_HIP(hipEventRecord(c->fork_ev, c->stream));
-    IPG_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    if (!c->rec) {
+      IPG_HIP(hipEventRecord(c->fork_ev, c->stream));
+      IPG_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    }
     std::swap(c->stream, c->side);
     ipg_mark(c, nullptr);
     const int rc = ipg_launch_nwtn_eulr(c, pend) || (np && ipg_launch_ibm_begin(c));
     std::swap(c->stream, c->side);
     if (rc) return 1;
-    IPG_HIP(hipEventRecord(c->join_ev, c->side));
+    if (!c->rec) IPG_HIP(hipEventRecord(c->join_ev, c->side));
   }
   if (ipg_launch_rk(c, rkiter, true)) return 1;
   if (pend) {
-    IPG_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
-    ipg_mark(c, nullptr);
+    if (c->rec) {
+      ipg_rec_join(c->rec);
+    } else {
+      IPG_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+      ipg_mark(c, nullptr);
+    }
   }
   if (np && (pend ? ipg_launch_ibm_passes(c) : ipg_launch_ibm(c))) return 1;
   if (ipg_launch_poisson_ex(c, rkiter)) return 1;
@@ -271,29 +279,44 @@ bool ghosted_field(int field) {
 
 // Records the launches of one substep of c as the merged launches of a batch
 // will run it: the substep of RK stage rkiter that starts from buffer parity
-// par, with the Newton-Euler update closing it (no side stream).  Like a
-// capture, recording leaves the host-side state as it was.
-int ipg_batch_record(ipg_ctx *c, int rkiter, int par, IpgRec *rec) {
+// par.  With defer, a case with spheres leaves its Newton-Euler update to
+// the next substep, which, when carried, runs the previous substep's update on
+// a side stream beside its RK stage, as ipg_substep does; without, the update
+// closes the substep.  Like a capture, recording leaves the host-side state
+// as it was.
+int ipg_batch_record(ipg_ctx *c, int rkiter, int par, bool carried, bool defer, IpgRec *rec) {
   double *fc = c->f[IPG_C], *fd = c->f[IPG_DCDT];
   const int cpar = c->cpar;
   const bool pgs = c->p_ghosts_stale, xcs = c->xcol_stale, dcs = c->dcdt_stale, pip = c->p_in_pieces;
   if (c->cpar != par) { std::swap(c->f[IPG_C], c->f[IPG_DCDT]); c->cpar ^= 1; }
+  const bool np = c->pc.np > 0;
   c->rec = rec;
-  const int rc = substep_launches(c, rkiter, 0, false);
+  const int rc = substep_launches(c, rkiter, carried && np ? (rkiter == 1 ? 3 : rkiter - 1) : 0, defer && np);
   c->rec = nullptr;
   c->f[IPG_C] = fc; c->f[IPG_DCDT] = fd; c->cpar = cpar;
   c->p_ghosts_stale = pgs; c->xcol_stale = xcs; c->dcdt_stale = dcs; c->p_in_pieces = pip;
   return rc;
 }
 
-// Before a merged substep: c is set up, and an update a fused substep of its
-// own left pending has run (on its stream, which then counts as used).
-int ipg_batch_enter(ipg_ctx *c) {
+// Whether c's pending Newton-Euler update is the one a merged substep of RK
+// stage rkiter carries (the update of its RK3 predecessor).
+bool ipg_batch_carries(const ipg_ctx *c, int rkiter) {
+  return c->pc.np <= 0 || c->ne_pending == (rkiter == 1 ? 3 : rkiter - 1);
+}
+
+// Before a merged substep: c is set up, and, unless the merged substep
+// carries it, an update left pending has run (on its stream, which then
+// counts as used).
+int ipg_batch_enter(ipg_ctx *c, bool carried) {
   if (!c) { ipg_set_error("null context"); return 1; }
   if (!c->solver_ready) { ipg_set_error("batch: ipg_initsolver has not been called on a case"); return 1; }
   if (c->pc.np > 0 && !c->markers_ready) { ipg_set_error("batch: ipg_coordsfp has not been called on a case"); return 1; }
-  if (c->ne_pending && !check_ctx(c)) return 1;
-  if (settle(c)) return 1;
+  if (carried) {
+    c->ne_pending = 0;  // the merged substep runs it
+  } else {
+    if (c->ne_pending && !check_ctx(c)) return 1;
+    if (settle(c)) return 1;
+  }
   // the substep rewrites dcdt's interior before swapping it in as c
   c->dcdt_stale = false;
   return 0;
@@ -302,9 +325,12 @@ int ipg_batch_enter(ipg_ctx *c) {
 // After a merged substep: the host-side state its launches leave, as after a
 // graph replay of the context's own substep; done is recorded after the
 // merged launches, and the context's next entry point waits for it.
-void ipg_batch_leave(ipg_ctx *c, hipEvent_t done) {
+void ipg_batch_leave(ipg_ctx *c, int rkiter, bool deferred, hipEvent_t done) {
   c->batch_ev = done;
   c->batch_pending = true;
+  // a deferred update is left to the next merged substep (or to the next
+  // entry point on the case that needs it)
+  c->ne_pending = deferred && c->pc.np > 0 ? rkiter : 0;
   std::swap(c->f[IPG_C], c->f[IPG_DCDT]);
   c->cpar ^= 1;
   c->p_ghosts_stale = true;

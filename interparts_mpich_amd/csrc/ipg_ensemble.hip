@@ -25,25 +25,37 @@ struct IpgRecEntry {
   size_t lds;
   std::vector<unsigned char> args;
   std::string stage;
+  bool side;  // issued on the side stream (the carried Newton-Euler update)
+  bool join;  // not a launch: the main stream waits for the side stream here
 };
 struct IpgRec {
   std::vector<IpgRecEntry> e;
 };
 
 int ipg_rec_add(IpgRec *r, BatchFns fns, dim3 grid, dim3 block, size_t lds, const void *args, size_t argsz,
-                const char *stage) {
+                const char *stage, bool side) {
   const unsigned char *a = (const unsigned char *)args;
-  r->e.push_back(IpgRecEntry{fns, grid, block, lds, std::vector<unsigned char>(a, a + argsz), stage ? stage : "?"});
+  r->e.push_back(IpgRecEntry{fns, grid, block, lds, std::vector<unsigned char>(a, a + argsz), stage ? stage : "?",
+                             side, false});
   return 0;
+}
+
+void ipg_rec_join(IpgRec *r) {
+  r->e.push_back(IpgRecEntry{BatchFns{nullptr, nullptr}, dim3(1), dim3(1), 0, {}, "join", false, true});
 }
 
 namespace {
 
-constexpr int NKEY = 6;  // (rkiter - 1) + 3 parity
+// (rkiter - 1) + 3 parity + 6 carried: the substep of RK stage rkiter from
+// c/dcdt buffer parity `parity`, carrying the previous substep's Newton-Euler
+// update beside its RK stage or not
+constexpr int NKEY = 12;
 
-// One merged launch: the cases holding a record of one stage with one body.
+// One merged launch: the cases holding a record of one stage with one body
+// (or, with join set, the point where the main stream waits for the side).
 struct Merged {
   BatchFns fns;
+  bool side, join;
   dim3 block;
   size_t lds;
   int total;            // merged blocks (a multiple of 8)
@@ -74,7 +86,9 @@ std::string hexptr(const void *p) {
 struct Group {
   std::vector<int> idx;          // the cases it holds, in batch order
   hipStream_t stream = nullptr;  // the batch's stream for group 0
+  hipStream_t side = nullptr;    // the carried updates beside the RK stage
   hipEvent_t joined = nullptr;   // its substep done (another group's stream)
+  hipEvent_t fork = nullptr, side_done = nullptr;
   Plan plan[NKEY];
 };
 
@@ -93,6 +107,11 @@ struct ipg_batch {
   int persist_default = 0;             // for a launch the tuner has not seen
   std::map<const void *, size_t> lds_set;  // the dynamic-LDS limit each merged kernel has been given
   std::vector<unsigned> gen;           // the cases' configuration counters the plans were recorded at
+  // defer each case's Newton-Euler update to the next substep, which carries
+  // it on a side stream beside its RK stage (the tuner's choice); off, the
+  // update closes its substep
+  bool carry = true;
+  bool carry_planned = true;           // the carry mode the plans were recorded in
 };
 
 namespace {
@@ -112,7 +131,9 @@ void drop_groups(ipg_batch *b) {
   for (Group &G : b->groups) {
     for (Plan &P : G.plan) drop_plan(P);
     if (G.stream && G.stream != b->stream) (void)hipStreamDestroy(G.stream);
-    if (G.joined) (void)hipEventDestroy(G.joined);
+    if (G.side) (void)hipStreamDestroy(G.side);
+    for (hipEvent_t e : {G.joined, G.fork, G.side_done})
+      if (e) (void)hipEventDestroy(e);
   }
   b->groups.clear();
 }
@@ -140,7 +161,10 @@ int make_groups(ipg_batch *b, int ng) {
     std::sort(G.idx.begin(), G.idx.end());
     if (g == 0) G.stream = b->stream;
     else IPG_HIP(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
+    IPG_HIP(hipStreamCreateWithFlags(&G.side, hipStreamNonBlocking));
     IPG_HIP(hipEventCreateWithFlags(&G.joined, hipEventDisableTiming));
+    IPG_HIP(hipEventCreateWithFlags(&G.fork, hipEventDisableTiming));
+    IPG_HIP(hipEventCreateWithFlags(&G.side_done, hipEventDisableTiming));
   }
   return 0;
 }
@@ -156,12 +180,13 @@ bool gen_current(const ipg_batch *b) {
 // Records one substep of every case of group G for key r and merges the
 // records.
 int build_plan(ipg_batch *b, Group &G, int r) {
-  const int rkiter = r % 3 + 1, par = r / 3;
+  const int rkiter = r % 3 + 1, par = (r / 3) % 2;
+  const bool carried = r >= 6;
   const int n = (int)G.idx.size();
   drop_plan(G.plan[r]);
   std::vector<IpgRec> rec(n);
   for (int i = 0; i < n; i++)
-    if (ipg_batch_record(b->cs[G.idx[i]], rkiter, par, &rec[i])) return 1;
+    if (ipg_batch_record(b->cs[G.idx[i]], rkiter, par, carried, b->carry, &rec[i])) return 1;
   // stage#occurrence keys per case; the longest sequence gives the order, and
   // every other case's sequence must be a subsequence of it (a case without
   // spheres has no IBM or particle stages)
@@ -212,6 +237,15 @@ int build_plan(ipg_batch *b, Group &G, int r) {
       const IpgRecEntry &e0 = rec[grp.second[0]].e[at[grp.second[0]][k]];
       Merged M{};
       M.fns = e0.fns;
+      M.side = e0.side;
+      M.join = e0.join;
+      M.stage = k;
+      for (int i : grp.second)
+        if (rec[i].e[at[i][k]].side != e0.side) {
+          ipg_set_error("batch: the cases run a stage on different streams (" + k + ")");
+          return 1;
+        }
+      if (M.join) { P.L.push_back(M); continue; }
       M.block = e0.block;
       M.lds = 0;
       M.ncase = (int)grp.second.size();
@@ -261,9 +295,28 @@ int build_plan(ipg_batch *b, Group &G, int r) {
   return 0;
 }
 
-// The merged launches of plan P on stream s; with timing, an event after
-// each (b->tev[t0 + q + 1], b->tev[t0] before the first).
-int run_plan(ipg_batch *b, Plan &P, hipStream_t s, bool timing, size_t t0 = 0) {
+// The merged launches of plan P for group G: the main launches on stream s,
+// the side launches forked onto G.side and joined back where the plan says.
+// With timing, every launch goes on s in the plan's order with an event after
+// each (b->tev[t0 + q + 1], b->tev[t0] before the first; joins take none).
+int launch_merged(ipg_batch *b, const Plan &P, const Merged &M, hipStream_t s) {
+  const char *base = (const char *)P.dev;
+  const void *args = base + M.off_args;
+  BatchMap m{(const BatchSeg *)(base + M.off_seg), (const int *)(base + M.off_of), M.total};
+  auto it = b->persist.find(M.tune_key);
+  const int f = it == b->persist.end() ? b->persist_default : it->second;
+  unsigned grid = (unsigned)M.total;
+  const void *fn = M.fns.one;
+  if (f > 0) {
+    const long cap = std::max<long>(8, (long)f * b->ncu / 8 * 8);
+    if (cap < M.total) { grid = (unsigned)cap; fn = M.fns.persist; }
+  }
+  void *kp[] = {(void *)&args, (void *)&m};
+  IPG_HIP(hipLaunchKernel(fn, dim3(grid), M.block, kp, M.lds, s));
+  return 0;
+}
+
+int run_plan(ipg_batch *b, Group &G, const Plan &P, hipStream_t s, bool timing, size_t t0 = 0) {
   if (timing) {
     while (b->tev.size() < t0 + P.L.size() + 1) {
       hipEvent_t e;
@@ -271,31 +324,43 @@ int run_plan(ipg_batch *b, Plan &P, hipStream_t s, bool timing, size_t t0 = 0) {
       b->tev.push_back(e);
     }
     IPG_HIP(hipEventRecord(b->tev[t0], s));
-  }
-  for (size_t q = 0; q < P.L.size(); q++) {
-    const Merged &M = P.L[q];
-    const char *base = (const char *)P.dev;
-    const void *args = base + M.off_args;
-    BatchMap m{(const BatchSeg *)(base + M.off_seg), (const int *)(base + M.off_of), M.total};
-    auto it = b->persist.find(M.tune_key);
-    const int f = it == b->persist.end() ? b->persist_default : it->second;
-    unsigned grid = (unsigned)M.total;
-    const void *fn = M.fns.one;
-    if (f > 0) {
-      const long cap = std::max<long>(8, (long)f * b->ncu / 8 * 8);
-      if (cap < M.total) { grid = (unsigned)cap; fn = M.fns.persist; }
+    size_t q = 0;
+    for (const Merged &M : P.L) {
+      if (M.join) continue;
+      if (launch_merged(b, P, M, s)) return 1;
+      IPG_HIP(hipEventRecord(b->tev[t0 + ++q], s));
     }
-    void *kp[] = {(void *)&args, (void *)&m};
-    IPG_HIP(hipLaunchKernel(fn, dim3(grid), M.block, kp, M.lds, s));
-    if (timing) IPG_HIP(hipEventRecord(b->tev[t0 + q + 1], s));
+    return 0;
+  }
+  bool forked = false;
+  for (const Merged &M : P.L) {
+    if (M.join) {
+      if (forked) {
+        IPG_HIP(hipEventRecord(G.side_done, G.side));
+        IPG_HIP(hipStreamWaitEvent(s, G.side_done, 0));
+        forked = false;
+      }
+      continue;
+    }
+    if (M.side && !forked) {
+      IPG_HIP(hipEventRecord(G.fork, s));
+      IPG_HIP(hipStreamWaitEvent(G.side, G.fork, 0));
+      forked = true;
+    }
+    if (launch_merged(b, P, M, M.side ? G.side : s)) return 1;
+  }
+  if (forked) {
+    IPG_HIP(hipEventRecord(G.side_done, G.side));
+    IPG_HIP(hipStreamWaitEvent(s, G.side_done, 0));
   }
   return 0;
 }
 
 int ensure_plans(ipg_batch *b, int r) {
   if (b->groups.empty() && make_groups(b, 1)) return 1;
-  if (!gen_current(b)) {
-    // a case changed: every key records again
+  if (!gen_current(b) || b->carry_planned != b->carry) {
+    // a case changed, or the carry mode: every key records again
+    b->carry_planned = b->carry;
     IPG_HIP(hipDeviceSynchronize());
     drop_graphs(b);
     for (Group &G : b->groups)
@@ -317,7 +382,7 @@ int run_groups(ipg_batch *b, int r) {
   if (b->groups.size() > 1) IPG_HIP(hipEventRecord(b->fork, b->stream));
   for (size_t g = 1; g < b->groups.size(); g++) IPG_HIP(hipStreamWaitEvent(b->groups[g].stream, b->fork, 0));
   for (Group &G : b->groups)
-    if (run_plan(b, G.plan[r], G.stream, false)) return 1;
+    if (run_plan(b, G, G.plan[r], G.stream, false)) return 1;
   for (size_t g = 1; g < b->groups.size(); g++) {
     IPG_HIP(hipEventRecord(b->groups[g].joined, b->groups[g].stream));
     IPG_HIP(hipStreamWaitEvent(b->stream, b->groups[g].joined, 0));
@@ -350,15 +415,24 @@ struct Timed {
 int substep(ipg_batch *b, int rkiter, std::vector<Timed> *timed) {
   if (rkiter < 1 || rkiter > 3) { ipg_set_error("batch: rkiter must be 1..3"); return 1; }
   const int n = (int)b->cs.size();
-  for (int i = 0; i < n; i++)
-    if (ipg_batch_enter(b->cs[i])) return 1;
   const int par = b->cs[0]->cpar;
   for (int i = 1; i < n; i++)
     if (b->cs[i]->cpar != par) {
       ipg_set_error("batch: the cases are at different c/dcdt buffer parities (an odd difference in substeps run)");
       return 1;
     }
-  const int r = (rkiter - 1) + 3 * par;
+  // the previous merged substep's Newton-Euler updates ride along beside this
+  // RK stage when every case with spheres still has its RK3 predecessor's
+  // pending; otherwise each pending one runs first, on its case's stream
+  bool carried = false, all = true;
+  for (ipg_ctx *c : b->cs) {
+    all = all && ipg_batch_carries(c, rkiter);
+    carried = carried || c->pc.np > 0;
+  }
+  carried = carried && all && b->carry;
+  for (int i = 0; i < n; i++)
+    if (ipg_batch_enter(b->cs[i], carried)) return 1;
+  const int r = (rkiter - 1) + 3 * par + (carried ? 6 : 0);
   if (ensure_plans(b, r)) return 1;
   // the batch's work on a case follows the case's own, where there was any
   // since the batch last waited for it
@@ -373,9 +447,11 @@ int substep(ipg_batch *b, int rkiter, std::vector<Timed> *timed) {
     size_t t0 = 0;
     timed->clear();
     for (Group &G : b->groups) {
-      if (run_plan(b, G.plan[r], b->stream, true, t0)) return 1;
-      for (size_t q = 0; q < G.plan[r].L.size(); q++) timed->push_back(Timed{&G.plan[r].L[q], t0 + q});
-      t0 += G.plan[r].L.size() + 1;
+      if (run_plan(b, G, G.plan[r], b->stream, true, t0)) return 1;
+      size_t q = 0;
+      for (const Merged &M : G.plan[r].L)
+        if (!M.join) timed->push_back(Timed{&M, t0 + q++});
+      t0 += q + 1;
     }
   } else if (b->use_graphs) {
     if (!b->graph[r] && capture(b, r)) return 1;
@@ -386,7 +462,7 @@ int substep(ipg_batch *b, int rkiter, std::vector<Timed> *timed) {
   // and the case's later work follows the batch's: the next entry point on
   // the case makes its stream wait for this event
   IPG_HIP(hipEventRecord(b->done, b->stream));
-  for (int i = 0; i < n; i++) ipg_batch_leave(b->cs[i], b->done);
+  for (int i = 0; i < n; i++) ipg_batch_leave(b->cs[i], rkiter, b->carry, b->done);
   return 0;
 }
 
@@ -554,16 +630,40 @@ int ipg_batch_autotune(ipg_batch *b, int *substeps) {
     if (ng == 1 || ms < best_step) { best_ng = ng; best_step = ms; }
   }
   if ((int)b->groups.size() != best_ng && make_groups(b, best_ng)) return 1;
+  // the Newton-Euler updates carried beside the next RK stage, or closing
+  // their substeps
+  float carry_ms[2] = {0.f, 0.f};
+  for (int on = 1; on >= 0; on--) {
+    b->carry = on != 0;
+    if (time_step(b, carry_ms[on], count)) return 1;
+  }
+  b->carry = carry_ms[1] <= carry_ms[0];
   if (substeps) *substeps = count;
+  return 0;
+}
+
+int ipg_batch_set_carry(ipg_batch *b, int on) {
+  if (!b) { ipg_set_error("batch: null batch"); return 1; }
+  b->carry = on != 0;
+  return 0;
+}
+
+int ipg_batch_carry(ipg_batch *b, int *on) {
+  if (!b || !on) { ipg_set_error("batch_carry: bad arguments"); return 1; }
+  *on = b->carry ? 1 : 0;
   return 0;
 }
 
 int ipg_batch_launches(ipg_batch *b, int rkiter, int *nlaunch, int *npersistent) {
   if (!b || !nlaunch || rkiter < 1 || rkiter > 3) { ipg_set_error("batch_launches: bad arguments"); return 1; }
   const int par = b->cs[0]->cpar;
+  // the plan of the next substep of stage rkiter: carrying updates, if built
+  int r = (rkiter - 1) + 3 * par + 6;
+  if (b->groups.empty() || !b->groups[0].plan[r].dev) r -= 6;
   int nl = 0, np = 0;
   for (const Group &G : b->groups)
-    for (const Merged &M : G.plan[(rkiter - 1) + 3 * par].L) {
+    for (const Merged &M : G.plan[r].L) {
+      if (M.join) continue;
       nl++;
       auto it = b->persist.find(M.tune_key);
       const int f = it == b->persist.end() ? b->persist_default : it->second;

@@ -186,7 +186,9 @@ inline void ipg_mark(ipg_ctx *c, const char *name) {
 // (k_batch<K> and its persistent form), its own grid, block and dynamic LDS,
 // a copy of its argument struct, and the substep stage it belongs to.
 int ipg_rec_add(struct IpgRec *r, BatchFns fns, dim3 grid, dim3 block, size_t lds, const void *args, size_t argsz,
-                const char *stage);
+                const char *stage, bool side);
+// where the main stream of a recorded substep waits for its side stream
+void ipg_rec_join(struct IpgRec *r);
 
 // Launches kernel body K for context c: its single-context kernel `one` on the
 // context's stream, marked for timing as `stage`, or, while a batch records
@@ -197,7 +199,7 @@ int ipg_rec_add(struct IpgRec *r, BatchFns fns, dim3 grid, dim3 block, size_t ld
 #if defined(__HIPCC__)
 template <class K, class One>
 int ipg_go(ipg_ctx *c, One one, dim3 grid, dim3 block, size_t lds, const typename K::Args &a, const char *stage) {
-  if (c->rec) return ipg_rec_add(c->rec, batch_kernels<K>(), grid, block, lds, &a, sizeof(a), stage);
+  if (c->rec) return ipg_rec_add(c->rec, batch_kernels<K>(), grid, block, lds, &a, sizeof(a), stage, c->stream != c->main_stream);
   if constexpr (std::is_invocable_v<One, hipStream_t>) one(c->stream);
   else hipLaunchKernelGGL(one, grid, block, lds, c->stream, a);
   ipg_mark(c, stage);
@@ -206,9 +208,10 @@ int ipg_go(ipg_ctx *c, One one, dim3 grid, dim3 block, size_t lds, const typenam
 #endif
 
 // a context's part in a batch (ipg_api.hip)
-int ipg_batch_record(ipg_ctx *c, int rkiter, int par, struct IpgRec *rec);
-int ipg_batch_enter(ipg_ctx *c);
-void ipg_batch_leave(ipg_ctx *c, hipEvent_t done);
+int ipg_batch_record(ipg_ctx *c, int rkiter, int par, bool carried, bool defer, struct IpgRec *rec);
+bool ipg_batch_carries(const ipg_ctx *c, int rkiter);
+int ipg_batch_enter(ipg_ctx *c, bool carried);
+void ipg_batch_leave(ipg_ctx *c, int rkiter, bool deferred, hipEvent_t done);
 
 // error plumbing
 void ipg_set_error(const std::string &msg);

@@ -268,6 +268,16 @@ class Batch:
         check(self.lib.ipg_batch_groups(self.handle, C.byref(n)), "batch_groups")
         return n.value
 
+    def set_carry(self, on):
+        """Carry each substep's Newton-Euler update beside the next RK stage
+        (ipg_batch_set_carry)."""
+        check(self.lib.ipg_batch_set_carry(self.handle, 1 if on else 0), "batch_set_carry")
+
+    def carry(self):
+        n = C.c_int()
+        check(self.lib.ipg_batch_carry(self.handle, C.byref(n)), "batch_carry")
+        return bool(n.value)
+
     def sync(self):
         check(self.lib.ipg_batch_sync(self.handle), "batch_sync")
 

@@ -55,12 +55,15 @@ def cases_small():
             dataclasses.replace(CONFIGS["t32"], name="t32_seed9", seed=9, re_bulk=700.0)]
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_batch_matches_cases_alone(graphs):
+@pytest.mark.parametrize("graphs,carry", [(True, True), (False, True), (True, False)])
+def test_batch_matches_cases_alone(graphs, carry):
+    """With carry on (the default) each substep's Newton-Euler update runs
+    beside the next substep's RK stage; off, it closes its substep."""
     cases = cases_small()
     bat = [make(c) for c in cases]
     alone = [make(c) for c in cases]
     b = Batch(bat, use_graphs=graphs)
+    b.set_carry(carry)
     for step in range(2):
         for rk in (1, 2, 3):
             b.substep(rk)
@@ -164,7 +167,8 @@ def test_batch_tuner_keeps_results():
     nsub = b.autotune()
     assert nsub % 3 == 0 and nsub > 0
     n, npers = b.launches(1)
-    print(f"[tuner] {nsub} substeps; {npers} of {n} merged launches persistent; {b.groups()} stream groups", flush=True)
+    print(f"[tuner] {nsub} substeps; {npers} of {n} merged launches persistent; {b.groups()} stream groups; "
+          f"carried updates {b.carry()}", flush=True)
     for q in range(nsub + 6):
         rk = q % 3 + 1
         if q >= nsub:
@@ -196,6 +200,30 @@ def test_batch_mixed_precision():
         print(f"[mixed batch vs alone] {x.case.name}: {w:.3e}, max|div| {mdiv:.2e}", flush=True)
         assert w <= 1e-6, (x.case.name, w)
         assert mdiv <= 1e-10
+    b.close()
+
+
+def test_batch_of_fp64_and_mixed_cases():
+    """FP64 and mixed-precision cases in one batch: the FP64 substep's stages
+    are a subsequence of the mixed one's (which adds the refinement solve),
+    and each stage merges the cases of each body (FP64 and FP32 spectra apart)."""
+    cases = cases_small()[:4]
+    precs = [0, 1, 0, 1]
+    bat = [make(c, precision=p) for c, p in zip(cases, precs)]
+    alone = [make(c, precision=p) for c, p in zip(cases, precs)]
+    b = Batch(bat)
+    for step in range(2):
+        for rk in (1, 2, 3):
+            b.substep(rk)
+            for s in alone:
+                s.substep(rk)
+    b.sync()
+    n, _ = b.launches(1)
+    print(f"[fp64 + mixed batch] {n} merged launches per substep", flush=True)
+    for x, y, p in zip(bat, alone, precs):
+        w = worst(x, y)
+        print(f"[fp64 + mixed batch vs alone] {x.case.name} precision {p}: {w:.3e}", flush=True)
+        assert w <= (1e-6 if p else 1e-12), (x.case.name, w)
     b.close()
 
 
