@@ -105,6 +105,25 @@ __host__ __device__ __forceinline__ int box_at(const Box &b, int iu, int ju, int
 // the kernel (FP32 in the mixed path).
 template <typename T>
 struct CompStencilT { int i0, j0, k0; T wx[3], wy[3], wz[3]; };
+
+// The three Roma weights (src/kernel.f90:7-25) of a stencil direction whose
+// middle point lies within half a cell of the marker (|r1| <= 1/2, the
+// outer points at r1 -+ 1): the outer branch's 1 - |r| of either outer point
+// is -+r1, so all three take the one square root sqrt(1 - 3 r1^2) that the
+// middle point's inner branch takes.  The weights are the kernel's, each
+// formula as roma_kernel writes it; only the outer points' square-root
+// argument is r1^2 rather than (1 - |r|)^2 rounded on its own, a difference
+// in the last bit.  Off-centre stencils (clamped at a wall) take roma_kernel
+// per point.
+__host__ __device__ __forceinline__ double roma_root(double x) { return sqrt(x); }
+__host__ __device__ __forceinline__ float roma_root(float x) { return sqrt_f32(x); }
+template <typename T>
+__host__ __device__ __forceinline__ void roma3(T r0, T r1, T r2, T *w) {
+  const T s = roma_root(-((T)3 * (r1 * r1)) + (T)1);
+  w[1] = ((T)1 / (T)3) * ((T)1 + s);
+  w[0] = ((T)1 / (T)6) * (((T)5 - (T)3 * (T)fabs(r0)) - s);
+  w[2] = ((T)1 / (T)6) * (((T)5 - (T)3 * (T)fabs(r2)) - s);
+}
 using CompStencil = CompStencilT<double>;
 template <bool WEIGHTS = true, typename T = double>
 __host__ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl, double zl, int comp,
@@ -123,11 +142,23 @@ __host__ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, 
   } else {
     c.i0 = (int)floor(cx); c.j0 = (int)floor(cy); c.k0 = imax(nint_f(cz) - 1, 1);
   }
+  if (WEIGHTS) {
+    T rx[3], ry[3], rz[3];
 #pragma unroll
-  for (int q = 0; q < 3 && WEIGHTS; q++) {
-    c.wx[q] = roma_kernel((T)(comp == 0 ? ((double)c.i0 + (double)q) - cx : (((double)c.i0 + (double)q) - 0.5) - cx));
-    c.wy[q] = roma_kernel((T)(comp == 1 ? ((double)c.j0 + (double)q) - cy : (((double)c.j0 + (double)q) - 0.5) - cy));
-    c.wz[q] = roma_kernel((T)(comp == 2 ? ((double)c.k0 + (double)q) - cz : (((double)c.k0 + (double)q) - 0.5) - cz));
+    for (int q = 0; q < 3; q++) {
+      rx[q] = (T)(comp == 0 ? ((double)c.i0 + (double)q) - cx : (((double)c.i0 + (double)q) - 0.5) - cx);
+      ry[q] = (T)(comp == 1 ? ((double)c.j0 + (double)q) - cy : (((double)c.j0 + (double)q) - 0.5) - cy);
+      rz[q] = (T)(comp == 2 ? ((double)c.k0 + (double)q) - cz : (((double)c.k0 + (double)q) - 0.5) - cz);
+    }
+    // x and y are periodic: their stencils are always centred on the marker
+    roma3(rx[0], rx[1], rx[2], c.wx);
+    roma3(ry[0], ry[1], ry[2], c.wy);
+    if (fabs((double)rz[1]) <= 0.5) {
+      roma3(rz[0], rz[1], rz[2], c.wz);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 3; q++) c.wz[q] = roma_kernel(rz[q]);
+    }
   }
   c.i0 += sxw;
   c.j0 += syw;
