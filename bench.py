@@ -316,6 +316,24 @@ def measure(case, inputs, args, precision):
             b = kernel_bytes_mixed(case, key, rk, res["nref"]) if precision else kernel_bytes(case, key, rk)
             if b is not None:
                 a["bytes"] += b
+    # the dominant kernel again with no update beside it (the substep's own
+    # stream only), so its duration alone is reported beside the overlapped one
+    alone = None
+    if args.side_stream == "on" and case.np > 0:
+        _lib.check(lib.ipg_use_side_stream(sim.ctx, 0), "use_side_stream")
+        alone = {}
+        nalone = min(args.steps, 60)
+        for s in range(nalone):
+            rk = s % 3 + 1
+            _lib.check(lib.ipg_substep_phases(sim.ctx, rk, maxn, ms, names, C.byref(n)), "substep_phases")
+            for q in range(n.value):
+                key = names[q].decode()
+                b = kernel_bytes_mixed(case, key, rk, res["nref"]) if precision else kernel_bytes(case, key, rk)
+                if b is not None:
+                    a = alone.setdefault(key, [0.0, 0, 0.0])
+                    a[0] += ms[q]
+                    a[1] += 1
+                    a[2] += b
     sim.close()
     per_substep = {k: v["ms"] / args.steps for k, v in acc.items()}
     kernel_gbs = {k: round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) for k, v in acc.items() if v["bytes"] > 0}
@@ -333,6 +351,14 @@ def measure(case, inputs, args, precision):
                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                        "traffic": traffic_from_profile(dominant + ("_mixed" if precision else ""), case.name),
                        "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch}
+    if alone and dominant in alone:
+        ams, alaunch, abytes = alone[dominant]
+        agbs = abytes / alaunch / (ams / alaunch * 1e-3) / 1e9
+        res["roofline"]["without_side_stream"] = {
+            "avg_launch_ms": round(ams / alaunch, 5), "achieved": round(agbs, 1),
+            "frac": round(agbs / HBM_PEAK_GBS, 4),
+            "note": "the same kernel over further substeps with the Newton-Euler update in its own substep "
+                    "(no second stream), so it has the chip to itself"}
     res["kernel_ms_per_substep"] = {k: round(v, 5) for k, v in per_substep.items()}
     res["kernel_algorithmic_gbs"] = kernel_gbs
     res["event_sum_ms_per_substep"] = round(sum(v["ms"] for v in acc.values()) / args.steps, 4)
