@@ -178,10 +178,39 @@ __device__ __forceinline__ void ibm_body(const IbmArgs &A, int pass, const VB vb
   // rather than kept live across the launch.  The box holds the stencil in x
   // and y, and in z up to kmax (comp_box); a stencil that were not held is
   // reported (error bit 4) and left out rather than index outside the box.
+  // The own markers' offsets and previous-pass forces, loaded first so that
+  // their latency overlaps the set-up below rather than following the
+  // staging (indexed by the unrolled q only: registers).
+  double mkx[IBM_MPT], mky[IBM_MPT], mkz[IBM_MPT];
+  T fprev[IBM_MPT];
+#pragma unroll
+  for (int q = 0; q < IBM_MPT; q++) {
+    const int l = (int)threadIdx.x + q * IBM_THREADS;
+    mkx[q] = mky[q] = mkz[q] = 0.0;
+    fprev[q] = (T)0;
+    if (l < A.nl) {
+      mkx[q] = A.mk[l]; mky[q] = A.mk[A.nl + l]; mkz[q] = A.mk[2 * A.nl + l];
+      if (MODE >= 1) fprev[q] = flp[m0 + l];
+    }
+  }
+  // likewise the count of listed neighbour markers and this thread's first
+  // entry (the list holds IBM_MAXNM >= IBM_THREADS entries, so the load is in
+  // bounds whatever the count)
+  static_assert(IBM_MAXNM >= IBM_THREADS, "first neighbour-list entry per thread");
+  int nbcnt = 0, code0 = 0;
+  if (MODE >= 1) {
+    nbcnt = A.nmc[vb.x];
+    code0 = A.nml[(long)vb.x * IBM_MAXNM + threadIdx.x];
+  } else {
+    // pass 0: the sphere's neighbour images, and the image of this thread's
+    // first candidate marker (entry threadIdx.x / nl of the list)
+    nbcnt = A.nbc[p];
+    code0 = A.nbl[(long)p * IBM_MAXNB + min((int)threadIdx.x / A.nl, IBM_MAXNB - 1)];
+  }
   auto own = [&](int q, CompStencilT<T> &c, int &e0, int &nzin, T &ub) -> bool {
     const int l = (int)threadIdx.x + q * IBM_THREADS;
     double xl, yl, zl;
-    if (l >= A.nl || !marker_pos(g, xp, yp, zp, A.mk[l], A.mk[A.nl + l], A.mk[2 * A.nl + l], xl, yl, zl)) return false;
+    if (l >= A.nl || !marker_pos(g, xp, yp, zp, mkx[q], mky[q], mkz[q], xl, yl, zl)) return false;
     comp_stencil<true, T>(g, xl, yl, zl, comp, c);
     if (!own_cells(b, c.i0, c.j0, c.k0, e0, nzin)) {
       if (MODE == 0) atomicOr(A.err, 4);
@@ -209,6 +238,17 @@ __device__ __forceinline__ void ibm_body(const IbmArgs &A, int pass, const VB vb
   if (MODE == 0 && threadIdx.x == 0) lcount = 0;
   __syncthreads();
   auto flagged = [&](int t) { return (flag[t >> 5] >> (t & 31)) & 1u; };
+  // the operands of this thread's first listed neighbour marker (its
+  // sphere's centre, its offsets, its previous force), loaded before the
+  // staging so that their latency overlaps it
+  double nbq[3] = {0.0, 0.0, 0.0}, nbe[3] = {0.0, 0.0, 0.0};
+  T nbf = (T)0;
+  if (MODE >= 1 && (int)threadIdx.x < nbcnt) {
+    const int mq = code0 / 25, qs = mq / A.nl, l = mq - qs * A.nl;
+    nbq[0] = A.px[qs]; nbq[1] = A.py[qs]; nbq[2] = A.pz[qs];
+    nbe[0] = A.mk[l]; nbe[1] = A.mk[A.nl + l]; nbe[2] = A.mk[2 * A.nl + l];
+    nbf = flp[mq];
+  }
   // the box cells of this thread: t = threadIdx.x + n IBM_THREADS
   const BoxWalk w0(b.bx, IBM_THREADS, (int)threadIdx.x);
   constexpr int CH = IBM_CH;
@@ -239,16 +279,25 @@ __device__ __forceinline__ void ibm_body(const IbmArgs &A, int pass, const VB vb
   const long lbase = (long)vb.x * IBM_MAXNM;
   if (MODE == 0) {
     // list the neighbours' markers whose stencils touch this box; the
-    // markers do not move inside the IBM block, so the three launches that
-    // spread them reuse the list
-    const int nn = A.nbc[p];
-    const int tot = nn * A.nl;
+    // markers do not move inside the IBM block, so the three launches
+    // that spread them reuse the list.  A thread's first candidate takes
+    // the image loaded first, and, from the first image, the offsets of
+    // its own marker (the same marker number).
+    const int tot = nbcnt * A.nl;
     for (int idx = threadIdx.x; idx < tot; idx += IBM_THREADS) {
       const int n = idx / A.nl, l = idx - n * A.nl;
-      const int code = A.nbl[(long)p * IBM_MAXNB + n];
+      const bool first = idx == (int)threadIdx.x;
+      const int code = first ? code0 : A.nbl[(long)p * IBM_MAXNB + n];
       const int qs = code / 25, sh = code % 25;
+      double ex, ey, ez;
+      if (first && n == 0) {
+        ex = mkx[0]; ey = mky[0]; ez = mkz[0];
+      } else {
+        ex = A.mk[l]; ey = A.mk[A.nl + l]; ez = A.mk[2 * A.nl + l];
+      }
       CompStencil c;
-      const bool keep = neighbour_stencil<false>(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c);
+      const bool keep = image_stencil<false, double>(g, b, comp, A.px[qs], A.py[qs], A.pz[qs], ex, ey, ez, sh / 5 - 2,
+                                                     sh % 5 - 2, c);
       // one LDS atomic per wavefront
       const unsigned long long msk = __ballot(keep);
       if (!keep) continue;
@@ -261,14 +310,23 @@ __device__ __forceinline__ void ibm_body(const IbmArgs &A, int pass, const VB vb
     }
   } else {
     // the listed neighbour markers' contributions to this box
-    const int cnt = A.nmc[vb.x];
-    for (int idx = threadIdx.x; idx < cnt; idx += IBM_THREADS) {
-      const int code = A.nml[lbase + idx];
+    for (int idx = threadIdx.x; idx < nbcnt; idx += IBM_THREADS) {
+      const bool first = idx == (int)threadIdx.x;
+      const int code = first ? code0 : A.nml[lbase + idx];
       const int mq = code / 25, sh = code % 25;
       const int qs = mq / A.nl, l = mq - qs * A.nl;
+      double xq, yq, zq, ex, ey, ez;
+      T f;
+      if (first) {
+        xq = nbq[0]; yq = nbq[1]; zq = nbq[2]; ex = nbe[0]; ey = nbe[1]; ez = nbe[2]; f = nbf;
+      } else {
+        xq = A.px[qs]; yq = A.py[qs]; zq = A.pz[qs];
+        ex = A.mk[l]; ey = A.mk[A.nl + l]; ez = A.mk[2 * A.nl + l];
+        f = flp[mq];
+      }
       CompStencilT<T> c;
-      if (!neighbour_stencil<true, T>(A, b, comp, qs, l, sh / 5 - 2, sh % 5 - 2, c)) continue;
-      const T fc = flp[mq] * c0;
+      if (!image_stencil<true, T>(g, b, comp, xq, yq, zq, ex, ey, ez, sh / 5 - 2, sh % 5 - 2, c)) continue;
+      const T fc = f * c0;
 #pragma unroll
       for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -297,8 +355,7 @@ __device__ __forceinline__ void ibm_body(const IbmArgs &A, int pass, const VB vb
     T ub;
     if (!own(q, c, e0, nzin, ub)) continue;
     if (q == 0 && MODE <= 1) { cs0 = c; e00 = e0; nz00 = nzin; ub00 = ub; have0 = true; }
-    const long m = m0 + (int)threadIdx.x + q * IBM_THREADS;
-    const T fc = MODE >= 1 ? flp[m] * c0 : (T)0;
+    const T fc = MODE >= 1 ? fprev[q] * c0 : (T)0;
 #pragma unroll
     for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -312,13 +369,8 @@ __device__ __forceinline__ void ibm_body(const IbmArgs &A, int pass, const VB vb
         }
   }
   __syncthreads();
-  if (MODE == 0) {
+  if (MODE == 0)
     for (int t = threadIdx.x; t < nflag; t += IBM_THREADS) gflag[t] = flag[t];
-    if (threadIdx.x == 0) {
-      A.nmc[vb.x] = min(lcount, IBM_MAXNM);
-      if (lcount > IBM_MAXNM) atomicOr(A.err, 2);
-    }
-  }
   if (MODE == 2) {
     // dudt = dudtf = fl(dudt + S) (src/main.f90:332-334), once per cell
     BoxWalk w = w0;
@@ -333,8 +385,8 @@ __device__ __forceinline__ void ibm_body(const IbmArgs &A, int pass, const VB vb
     return;
   }
   if (MODE == 0) {
-    // stage dudt on the flagged cells, in chunks of 8 cells per thread so
-    // that the loads of a chunk are all in flight together
+    // stage dudt on the flagged cells, in chunks of IBM_CH cells per thread
+    // so that the loads of a chunk are all in flight together
     BoxWalk w = w0;
     for (int t0 = threadIdx.x; t0 < nbox; t0 += CH * IBM_THREADS) {
       double v[CH];
@@ -352,6 +404,10 @@ __device__ __forceinline__ void ibm_body(const IbmArgs &A, int pass, const VB vb
       }
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+      A.nmc[vb.x] = min(lcount, IBM_MAXNM);
+      if (lcount > IBM_MAXNM) atomicOr(A.err, 2);
+    }
   }
   double fsum = 0.0;
 #pragma unroll
@@ -385,7 +441,7 @@ __device__ __forceinline__ void ibm_body(const IbmArgs &A, int pass, const VB vb
           for (int i = 0; i < 3; i++)
             if (k >= nzin) acc = acc + (T)A.Kread[kplane(g, c.i0 + i, c.j0 + j)] * c.wx[i] * c.wy[j] * c.wz[k];
     }
-    const T fnew = MODE == 0 ? (ub - acc) * dti : flp[m] + (ub - acc) * dti;
+    const T fnew = MODE == 0 ? (ub - acc) * dti : fprev[q] + (ub - acc) * dti;
     fl[m] = fnew;
     fsum += (double)fnew;
     if (nzin < 3) {

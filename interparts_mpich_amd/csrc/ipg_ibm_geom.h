@@ -180,7 +180,7 @@ constexpr int IBM_THREADS = 512;  // threads per (sphere, component) workgroup
 constexpr int IBM_MPT = 2;        // markers per thread: nl <= IBM_THREADS * IBM_MPT
 constexpr int IBM_MAXNB = 64;     // neighbour images per sphere
 constexpr int IBM_MAXNM = 4096;   // neighbour markers touching one (sphere, component) box
-constexpr int IBM_CH = 8;         // cells per thread whose loads one staging chunk keeps in flight
+constexpr int IBM_CH = 12;        // cells per thread whose loads one staging chunk keeps in flight
 constexpr int IBM_WAVES = IBM_THREADS / 64;  // wavefronts per workgroup
 
 // periodic wrap of any unwrapped index onto 1..n

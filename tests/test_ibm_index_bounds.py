@@ -245,7 +245,14 @@ static void block(Ctx &C, int MODE, int pass, int p, int comp) {
           w.advance();
         }
     }
+    // passes 1, 2 and the final spread load each thread's first list entry
+    // early, before the count is known: in bounds for every thread
+    if (MODE >= 1) CHECK(lbase + tid < (long)G.nml.size() && tid < IBM_MAXNM, "early nml read %ld", lbase + tid);
     if (MODE == 0) {
+      // pass 0 loads the image of each thread's first candidate early
+      const long at0 = (long)p * IBM_MAXNB + std::min(tid / cs.nl, IBM_MAXNB - 1);
+      CHECK(at0 >= 0 && at0 < (long)G.nbl.size(), "early nbl read %ld", at0);
+      if (tid < G.nbc[p] * cs.nl) CHECK(G.nbl_w[at0], "early nbl read %ld before written", at0);
       CHECK(G.nbc[p] >= 0 && G.nbc[p] <= IBM_MAXNB, "nbc %d", G.nbc[p]);
       const int tot = G.nbc[p] * cs.nl;
       for (int idx = tid; idx < tot; idx += NT) {
