@@ -128,8 +128,12 @@ using CompStencil = CompStencilT<double>;
 template <bool WEIGHTS = true, typename T = double>
 __host__ __device__ __forceinline__ void comp_stencil(const Grid &g, double xl, double yl, double zl, int comp,
                                                       CompStencilT<T> &c) {
-  const int sxw = (int)floor((xl - 0.5 * g.dx) / g.lx) * g.nx;
-  const int syw = (int)floor((yl - 0.5 * g.dy) / g.ly) * g.ny;
+  // floor(a / L) is 0 for 0 <= a < L without the division: there a / L < 1
+  // exactly, and its rounding stays below 1 (a <= L's predecessor, so a / L
+  // <= 1 - 2^-53); a division only for markers beyond a periodic edge
+  const double ax = xl - 0.5 * g.dx, ay = yl - 0.5 * g.dy;
+  const int sxw = ax >= 0.0 && ax < g.lx ? 0 : (int)floor(ax / g.lx) * g.nx;
+  const int syw = ay >= 0.0 && ay < g.ly ? 0 : (int)floor(ay / g.ly) * g.ny;
   const double cx = xl * g.dxi - (double)sxw - 0.0;
   const double cy = yl * g.dyi - (double)syw - 0.0;
   const double cz = zl * g.dzi;

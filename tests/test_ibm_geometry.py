@@ -133,3 +133,21 @@ def test_every_stencil_lies_in_its_box(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout[-2000:]
     assert "bad 0" in r.stdout
+
+
+def test_periodic_shift_needs_no_division_inside_the_domain():
+    """comp_stencil takes the periodic shift floor((x - dx/2) / L) as 0 for 0 <=
+    x - dx/2 < L without dividing: there the correctly rounded quotient stays
+    below 1.  Checked at the quotient's worst case (the largest double below
+    L), at random points, for random and power-of-two L, and at L itself,
+    where the shift is 1 (outside the range, as for negative x - dx/2, the
+    division is taken)."""
+    import numpy as np
+    rng = np.random.default_rng(7)
+    lens = np.concatenate([rng.uniform(0.05, 50.0, 4000), 2.0 ** np.arange(-6, 7), [1.0, 0.75, 1.5, 3.0, np.pi]])
+    for L in lens:
+        below = np.nextafter(L, 0.0)
+        a = np.concatenate([[0.0, below, np.nextafter(below, 0.0)], rng.uniform(0.0, L, 64), L * (1.0 - 2.0 ** -np.arange(40, 54))])
+        a = a[(a >= 0.0) & (a < L)]
+        assert np.all(np.floor(a / L) == 0.0), L
+        assert np.floor(L / L) == 1.0

@@ -1,7 +1,8 @@
 """The IBM kernels build without scratch (no register spills) and keep the
 register count that lets two 512-thread workgroups share a CU, as reported by
 the compiler for gfx950 (-Rpass-analysis=kernel-resource-usage) on
-interparts_mpich_amd/csrc/ipg_ibm.hip with the library's flags.
+interparts_mpich_amd/csrc/ipg_ibm.hip with the library's flags; the sphere
+integrals (ipg_particles.hip) likewise, with three 512-thread blocks per CU.
 
 Both IBM anomalies on record (a NaN in one GPU run, a fault of a 384-thread
 variant) came from builds of k_ibm that spilled to scratch; builds that
@@ -18,9 +19,9 @@ from interparts_mpich_amd import build
 SRC = os.path.join(build.CSRC, "ipg_ibm.hip")
 
 
-def _usage(tmp_path):
+def _usage(tmp_path, src=SRC):
     flags = [f for f in build.HIP_FLAGS if f != "-shared"]
-    r = subprocess.run([build.HIPCC] + flags + ["-c", SRC, "-o", str(tmp_path / "ibm.o"),
+    r = subprocess.run([build.HIPCC] + flags + ["-c", src, "-o", str(tmp_path / "k.o"),
                                                 "-Rpass-analysis=kernel-resource-usage"],
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=900)
     assert r.returncode == 0, r.stdout[-3000:]
@@ -53,3 +54,17 @@ def test_ibm_kernels_do_not_spill(tmp_path):
     for k, v in ibm.items():
         if re.search(r"k_ibmILi[012]EdE", k) or re.search(r"k_batch_wpeI.*IbmKILi[012]EdE", k):
             assert v["VGPRs"] <= 128, f"{k}: {v}"
+
+
+def test_sphere_integrals_do_not_spill(tmp_path):
+    """k_sphere_int and the batch's k_batch_wpe<SphereIntK<type>>: no scratch,
+    at most 80 VGPRs (6 wavefronts per SIMD: three 512-thread blocks per CU)."""
+    if not os.path.exists(build.HIPCC):
+        pytest.skip("needs hipcc")
+    kernels = _usage(tmp_path, os.path.join(build.CSRC, "ipg_particles.hip"))
+    sph = {k: v for k, v in kernels.items() if "k_sphere_int" in k or re.search(r"k_batch_wpeI.*SphereIntK", k)}
+    assert len(sph) >= 5, sorted(kernels)
+    for k, v in sorted(sph.items()):
+        print(k, v)
+        assert v.get("ScratchSize", -1) == 0, f"{k} spills: {v}"
+        assert v["VGPRs"] <= 80, f"{k}: {v}"
