@@ -280,6 +280,7 @@ def measure(case, inputs, args, precision):
     sim = Simulation(case, use_graphs=not args.no_graphs, side_stream=args.side_stream == "on",
                      precision=precision).initialize(fields, spheres, markers)
     rk_kb = None if args.no_autotune else sim.autotune()
+    sph_blocks = sim.tuning()[1]
     for s in range(args.warmup):
         sim.substep(s % 3 + 1)
     sim.sync()
@@ -289,7 +290,8 @@ def measure(case, inputs, args, precision):
     sim.sync()
     t1 = time.perf_counter()
     res = {"value": args.steps / (t1 - t0), "ms_per_step": (t1 - t0) * 1e3 / args.steps,
-           "rk_kb": rk_kb if rk_kb is not None else "default", "nref": None}
+           "rk_kb": rk_kb if rk_kb is not None else "default", "nref": None,
+           "sphere_int_blocks": sph_blocks if sph_blocks > 0 else "one per sphere"}
     print(f"[bench] {name}: {args.steps} substeps in {t1 - t0:.4f} s ({res['value']:.1f} substeps/s)", file=sys.stderr,
           flush=True)
     if precision:
@@ -443,6 +445,7 @@ def main():
                                + ("mixed-precision path" if mixed else "FP64 path"),
                    "graphs": not args.no_graphs, "side_stream": args.side_stream == "on",
                    "rk_z_planes_per_workgroup": head["rk_kb"],
+                   "sphere_integral_blocks": head["sphere_int_blocks"],
                    "gcell_updates_per_s": round(value * case.cells / 1e9, 4)},
         "roofline": head["roofline"],
         "kernel_ms_per_substep": head["kernel_ms_per_substep"],

@@ -186,6 +186,10 @@ int ipg_set_refinement(ipg_ctx *ctx, int nref);
 /* time the RK stage for 8..64 z planes per workgroup and keep the fastest;
  * overwrites dudt..dcdt (recomputed by every stage): call between substeps */
 int ipg_autotune(ipg_ctx *ctx, int *kb);
+/* the launch shapes in use: RK z planes per workgroup (0: chosen per grid),
+ * sphere-integral blocks (0: one per sphere; with the side stream on and more
+ * spheres than CUs, one per CU striding over them, or IPG_SPH_GRID at init) */
+int ipg_tuning(ipg_ctx *ctx, int *rk_kb, int *sphere_blocks);
 /* HBM bytes one substep must move by the algorithmic model, for roofline
  * (after ipg_initsolver) */
 int ipg_substep_bytes(ipg_ctx *ctx, double *bytes);

@@ -389,9 +389,16 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
   c->prm = *prm;
   derived(c);
   if (const char *e = std::getenv("IPG_TUNE_RK_KB")) c->tune_rk_kb = std::atoi(e);
+  c->sph_grid = -1;
+  if (const char *e = std::getenv("IPG_SPH_GRID")) c->sph_grid = std::atoi(e);
   auto fail = [&](int code) { release(c); delete c; return code; };
   c->side_on = true;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { ipg_set_error("stream create failed"); return fail(1); }
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      c->ncu = 0;
+  }
   c->main_stream = c->stream;
   const Grid &g = c->g;
   c->nfield = 16 + (size_t)g.sxy * (g.nz + 2) + 16;
@@ -918,6 +925,13 @@ int ipg_autotune(ipg_ctx *c, int *kb_out) {
   for (int r = 0; r < IPG_NGRAPH; r++)
     if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
   if (kb_out) *kb_out = c->tune_rk_kb;
+  return 0;
+}
+
+int ipg_tuning(ipg_ctx *c, int *rk_kb, int *sphere_blocks) {
+  if (!check_ctx(c)) return 1;
+  if (rk_kb) *rk_kb = c->tune_rk_kb;
+  if (sphere_blocks) *sphere_blocks = ipg_sphere_blocks(c) < c->pc.np ? ipg_sphere_blocks(c) : 0;
   return 0;
 }
 

@@ -87,6 +87,9 @@ struct ipg_ctx {
   bool xcol_stale;                // nor the x-ghost columns of u, v, w, pnew, c
   // launch shape: z planes per RK-stage workgroup (IPG_TUNE_RK_KB at init; 0 = chosen per grid)
   int tune_rk_kb;
+  int sph_grid;                   // sphere-integral blocks: -1 automatic (ipg_sphere_blocks), 0 one per sphere,
+                                  // n > 0: n blocks (IPG_SPH_GRID at init)
+  int ncu;                        // compute units of the device
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
   double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl (after passes 0 and 2)
   double *fl2;                    // [3][np*nl] marker forces after pass 1
@@ -292,6 +295,7 @@ int ipg_ibm_check(ipg_ctx *c);
 int ipg_launch_post3d(ipg_ctx *c, int q, double *host);
 int ipg_launch_phase_sol(ipg_ctx *c, double *out);
 int ipg_launch_sphere_integrals(ipg_ctx *c, int type);
+int ipg_sphere_blocks(const ipg_ctx *c);
 int ipg_launch_nwtn_eulr(ipg_ctx *c, int rkiter);
 int ipg_launch_nbflags(ipg_ctx *c);
 

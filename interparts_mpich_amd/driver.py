@@ -176,6 +176,13 @@ class Simulation:
         check(self.lib.ipg_autotune(self.ctx, C.byref(kb)), "autotune")
         return kb.value
 
+    def tuning(self):
+        """(RK z planes per workgroup, sphere-integral blocks) in use
+        (ipg_tuning; 0: chosen per grid, one block per sphere)."""
+        kb, sb = C.c_int(), C.c_int()
+        check(self.lib.ipg_tuning(self.ctx, C.byref(kb), C.byref(sb)), "tuning")
+        return kb.value, sb.value
+
     def sync(self):
         check(self.lib.ipg_sync(self.ctx), "sync")
 

@@ -58,7 +58,9 @@ def test_ibm_kernels_do_not_spill(tmp_path):
 
 def test_sphere_integrals_do_not_spill(tmp_path):
     """k_sphere_int and the batch's k_batch_wpe<SphereIntK<type>>: no scratch,
-    at most 80 VGPRs (6 wavefronts per SIMD: three 512-thread blocks per CU)."""
+    at most 80 VGPRs (6 wavefronts per SIMD: three 512-thread blocks per CU);
+    the persistent k_sphere_int_persist, launched with one block per CU: no
+    scratch."""
     if not os.path.exists(build.HIPCC):
         pytest.skip("needs hipcc")
     kernels = _usage(tmp_path, os.path.join(build.CSRC, "ipg_particles.hip"))
@@ -67,4 +69,5 @@ def test_sphere_integrals_do_not_spill(tmp_path):
     for k, v in sorted(sph.items()):
         print(k, v)
         assert v.get("ScratchSize", -1) == 0, f"{k} spills: {v}"
-        assert v["VGPRs"] <= 80, f"{k}: {v}"
+        if "persist" not in k:
+            assert v["VGPRs"] <= 80, f"{k}: {v}"
