@@ -390,6 +390,8 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
   derived(c);
   if (const char *e = std::getenv("IPG_TUNE_RK_KB")) c->tune_rk_kb = std::atoi(e);
   c->sph_grid = -1;
+  c->poisson_split = true;
+  if (const char *e = std::getenv("IPG_POISSON_SPLIT")) c->poisson_split = std::atoi(e) != 0;
   if (const char *e = std::getenv("IPG_SPH_GRID")) c->sph_grid = std::atoi(e);
   auto fail = [&](int code) { release(c); delete c; return code; };
   c->side_on = true;

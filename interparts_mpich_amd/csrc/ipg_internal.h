@@ -90,6 +90,7 @@ struct ipg_ctx {
   int sph_grid;                   // sphere-integral blocks: -1 automatic (ipg_sphere_blocks), 0 one per sphere,
                                   // n > 0: n blocks (IPG_SPH_GRID at init)
   int ncu;                        // compute units of the device
+  bool poisson_split;             // y passes and Thomas in two x-mode ranges on two streams (IPG_POISSON_SPLIT=0: off)
   double *mk;                     // [3][nl] marker offsets (radfp sin cos ...)
   double *fl;                     // [3][np*nl] marker forces fxl,fyl,fzl (after passes 0 and 2)
   double *fl2;                    // [3][np*nl] marker forces after pass 1

@@ -737,12 +737,13 @@ int launch_x_inv(ipg_ctx *c, const XArgs<TS> &a, TP *p) {
                                   XInvArgs<TS, TP>{a.g, p, a.spec, a.sp, a.tw, a.scale, X_INV_TOP_FIRST}, "fft_x_inv");
 }
 template <int N, typename TS>
-int launch_y(ipg_ctx *c, TS *spec, int sign, int top_first, const char *stage) {
+int launch_y(ipg_ctx *c, TS *spec, int sign, int top_first, const char *stage, int nmx = -1) {
   constexpr int B = batch_of<N>();
   const Poisson &P = c->ps2;
-  const int tiles = (P.nmx + B - 1) / B;
+  if (nmx < 0) nmx = P.nmx;
+  const int tiles = (nmx + B - 1) / B;
   return ipg_go<YK<N, TS>>(c, k_fft_y_t<N, TS>, dim3(c->g.nz * tiles), dim3(FFT_THREADS), 0,
-                           YArgs<TS>{c->g, spec, P.sp, P.nmx, P.twy, sign, top_first}, stage);
+                           YArgs<TS>{c->g, spec, P.sp, nmx, P.twy, sign, top_first}, stage);
 }
 
 // rc = the launch of the plan of length n (one of the specialised lengths)
@@ -881,23 +882,54 @@ template <typename TS, typename TP>
 int solve_rest(ipg_ctx *c, const XArgs<TS> &xa, TP *pout, int pin_zero) {
   const Grid &g = c->g;
   Poisson &P = c->ps2;
-  TS *spec = xa.spec;
   int rc = 0;
-#define CALLY(n) launch_y<n, TS>(c, spec, -1, Y_FWD_TOP_FIRST, "fft_y_fwd")
-  IPG_FFT_DISPATCH(g.ny, CALLY)
-#undef CALLY
-  if (rc) return rc;
-  // lanes: 64 per (block of 16 x-modes, y-mode pair pr = 0..ny/2)
-  const long nthomas = 64L * ((P.nmx + 15) / 16) * (g.ny / 2 + 1);
   using TE = typename ElemOf<TS>::type;
-  const ThomasArgs<TE> ta{g, (TE *)spec, P.dtri, P.sp, P.nmx, P.a, P.b, P.c, P.xrt, P.yrt, pin_zero};
-  if (ipg_go<ThomasK<TE>>(c, k_thomas<TE>, dim3((unsigned)((nthomas + THOMAS_THREADS - 1) / THOMAS_THREADS)),
-                          dim3(THOMAS_THREADS), 0, ta, "thomas"))
-    return 1;
-#define CALLY(n) launch_y<n, TS>(c, spec, +1, 0, "fft_y_inv")
-  IPG_FFT_DISPATCH(g.ny, CALLY)
+  // The y passes and the Thomas solve of x-modes [m0, m0 + nm): every mode's
+  // arithmetic is its own, so a range runs the same kernels on the spectrum,
+  // multipliers and eigenvalues offset by m0.  (Only the mixed path's pin of
+  // the singular mode names the mode: it goes with the range holding m = 0.)
+  auto chain = [&](int m0, int nm) -> int {
+    TS *spec = xa.spec + m0;
+#define CALLY(n) launch_y<n, TS>(c, spec, -1, Y_FWD_TOP_FIRST, "fft_y_fwd", nm)
+    IPG_FFT_DISPATCH(g.ny, CALLY)
 #undef CALLY
-  if (rc) return rc;
+    if (rc) return rc;
+    if (c->stream == c->main_stream && m0 == 0 && nm < P.nmx) {
+      // the second range's chain starts once this range's forward y pass
+      // is done (below)
+      IPG_HIP(hipEventRecord(c->fork_ev, c->stream));
+    }
+    // lanes: 64 per (block of 16 x-modes, y-mode pair pr = 0..ny/2)
+    const long nthomas = 64L * ((nm + 15) / 16) * (g.ny / 2 + 1);
+    const ThomasArgs<TE> ta{g, (TE *)spec, P.dtri + m0, P.sp, nm, P.a, P.b, P.c, P.xrt + m0, P.yrt, m0 == 0 ? pin_zero : 0};
+    if (ipg_go<ThomasK<TE>>(c, k_thomas<TE>, dim3((unsigned)((nthomas + THOMAS_THREADS - 1) / THOMAS_THREADS)),
+                            dim3(THOMAS_THREADS), 0, ta, "thomas"))
+      return 1;
+#define CALLY(n) launch_y<n, TS>(c, spec, +1, 0, "fft_y_inv", nm)
+    IPG_FFT_DISPATCH(g.ny, CALLY)
+#undef CALLY
+    return rc;
+  };
+  // Two ranges of x-modes, the second on the side stream one y pass behind
+  // the first, so that each range's Thomas solve (latency-bound: about one
+  // wavefront per SIMD at 256^3) runs beside the other range's y pass
+  // (bandwidth-bound).  Not while a batch records the context or its launches
+  // are being timed one by one, nor before the side stream exists.
+  const bool split = c->poisson_split && c->side && !c->rec && !c->timing && P.nmx >= 32;
+  if (!split) {
+    if (chain(0, P.nmx)) return 1;
+  } else {
+    const int nma = (P.nmx / 2 + 15) / 16 * 16;
+    hipStream_t main = c->stream;
+    if (chain(0, nma)) return 1;
+    IPG_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    c->stream = c->side;
+    rc = chain(nma, P.nmx - nma);
+    if (!rc && hipEventRecord(c->join_ev, c->side) != hipSuccess) rc = 1;
+    c->stream = main;
+    if (rc) { ipg_set_error("solver2d: the second mode range failed to launch"); return 1; }
+    IPG_HIP(hipStreamWaitEvent(main, c->join_ev, 0));
+  }
 #define CALLX(n) launch_x_inv<n, TS, TP>(c, xa, pout)
   IPG_FFT_DISPATCH(g.nx, CALLX)
 #undef CALLX
