@@ -1,4 +1,5 @@
-"""The launch shapes of the sphere integrals leave the arithmetic unchanged.
+"""The launch shapes of the sphere integrals and of the Poisson solve leave the
+arithmetic unchanged.
 
 The sphere integrals (src/intgr_over_sphere.f90:12-625) run one block per
 sphere or, on a grid smaller than the spheres, as blocks striding over them
@@ -109,3 +110,33 @@ def test_default_striding_grid_matches_one_block_per_sphere():
     for n, r in worst(b, a).items():
         assert r <= 1e-12, (n, r)
     a.close(); b.close()
+
+
+def test_poisson_in_two_mode_ranges_matches_one():
+    """The Poisson solve's y passes and Thomas solve in two x-mode ranges on
+    two streams (the default once the side stream exists) against one range
+    on one stream (IPG_POISSON_SPLIT=0): each mode's arithmetic is the same,
+    so three steps agree to the IBM spreads' rounding (FP64: 1e-13; mixed,
+    where such differences can flip an FP32 rounding: 1e-6, as two mixed runs
+    are held)."""
+    case = CONFIGS["ch64_8sph"]
+    inputs = (initial_fields(case), place_spheres(case), markers_for(case))
+    for precision in (0, 1):
+        print(f"[case {case.name}] precision {precision}", flush=True)
+        a = make(case, inputs, use_graphs=True, precision=precision)
+        old = os.environ.get("IPG_POISSON_SPLIT")
+        os.environ["IPG_POISSON_SPLIT"] = "0"
+        try:
+            b = make(case, inputs, use_graphs=True, precision=precision)
+        finally:
+            if old is None:
+                os.environ.pop("IPG_POISSON_SPLIT", None)
+            else:
+                os.environ["IPG_POISSON_SPLIT"] = old
+        for q in range(9):
+            rk = q % 3 + 1
+            a.substep(rk); b.substep(rk)
+        a.sync(); b.sync()
+        for n, r in worst(a, b).items():
+            assert r <= (1e-13 if precision == 0 else 1e-6), (precision, n, r)
+        a.close(); b.close()
