@@ -13,7 +13,7 @@ module mod_ipg_capi
             ipg_ibm, ipg_ibm_begin, ipg_eulr2lagr, ipg_lagrforces, ipg_lagr2eulr, ipg_bounduvw, ipg_boundp, &
             ipg_boundscal, ipg_fillps, ipg_solver2d, ipg_correc, ipg_pnew_update, ipg_intgr_over_sphere, &
             ipg_intgr_nwtn_eulr, ipg_substep, ipg_chkdt, ipg_chkdiv, ipg_uniform, ipg_sync, ipg_use_graphs, &
-            ipg_use_side_stream, ipg_autotune, ipg_post3d, ipg_kerneltest, ipg_ibm_forces
+            ipg_use_side_stream, ipg_autotune, ipg_tuning, ipg_post3d, ipg_kerneltest, ipg_ibm_forces
   ! the batched ensemble (no reference counterpart: many cases through each
   ! substep together, include/ipg.h)
   public :: ipg_batch_create, ipg_batch_destroy, ipg_batch_substep, ipg_batch_sync, ipg_batch_autotune
@@ -232,6 +232,11 @@ module mod_ipg_capi
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), intent(out) :: kb
+    end function
+    integer(c_int) function ipg_tuning(ctx, rk_kb, sphere_blocks) bind(C, name="ipg_tuning")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: rk_kb, sphere_blocks
     end function
     integer(c_int) function ipg_batch_create(cases, ncases, batch) bind(C, name="ipg_batch_create")
       import :: c_int, c_ptr
