@@ -62,11 +62,17 @@ def thomas_words(case, Sw):
     return Sw * (4.0 * (K - T) + 2.0 * T + 2.0) / case.ktot + 2.0 * thomas_d_rows(case)
 
 
-def kernel_bytes_mixed(case, name, rkiter, nref):
+def kernel_bytes_mixed(case, name, rkiter, nref, solve=32):
     """Algorithmic HBM bytes of one launch of a kernel on the mixed path: FP64
-    fields (u v w c pnew dudt..dcdt, the right-hand side) at 8 B, the RK history,
-    the spectra and the pressure pieces at 4 B, the Thomas multipliers d at 8 B."""
+    fields (u v w c pnew dudt..dcdt, the right-hand side) at 8 B, the RK history
+    at 4 B; with the FP32-stored solve (solve 32) the spectra and the pressure
+    pieces at 4 B and the Thomas multipliers d at 8 B, with the FP64 solve
+    (solve 64) every Poisson and projection launch as on the FP64 path."""
     N = float(case.itot * case.jtot * case.ktot)
+    if solve == 64:
+        if name == "rk_stage":
+            return 72.0 * N + {1: 16.0, 2: 32.0, 3: 16.0}[rkiter] * N
+        return kernel_bytes(case, name, rkiter)
     Sw = 2.0 * (case.itot // 2 + 1) * case.jtot * case.ktot
     d_rows = thomas_d_rows(case)
     b = {
@@ -83,6 +89,15 @@ def kernel_bytes_mixed(case, name, rkiter, nref):
         "correc": (64.0 + 4.0 * (nref + 1)) * N,  # dudt dvdt dwdt pnew and the pieces read; u v w pnew written
     }
     return b.get(name)
+
+
+def mixed_dtype(solve):
+    """What the mixed path stores and computes in FP32, by its Poisson solve."""
+    if solve == 64:
+        return ("mixed: RK history stored f32; u v w p dudt, the Poisson solve, the IBM and the particles f64, "
+                "stencil arithmetic f64")
+    return ("mixed: RK history, Poisson spectra and the pressure correction's pieces stored f32 (one FP64-residual "
+            "refinement); u v w dudt, the IBM and the particles f64, stencil and Poisson arithmetic f64")
 
 
 def kernel_bytes(case, name, rkiter):
@@ -169,7 +184,8 @@ def run_ensemble(args):
     sims = []
     for c in cases:
         # many contexts overlap one another's streams: no side stream each
-        sims.append(Simulation(c, use_graphs=True, side_stream=args.side_stream == "on", precision=precision)
+        sims.append(Simulation(c, use_graphs=True, side_stream=args.side_stream == "on", precision=precision,
+                               solve=args.mixed_solve if precision else None)
                     .initialize(initial_fields(c), place_spheres(c), markers_for(c)))
     print(f"[bench] ensemble of {len(sims)} cases ready", file=sys.stderr, flush=True)
     steps = max(3, args.steps // 3 * 3)
@@ -252,7 +268,7 @@ def run_profile(args):
     print(f"[profile] {case.name} {'mixed' if precision else 'fp64'}: inputs", flush=True)
     inputs = (initial_fields(case), place_spheres(case), markers_for(case))
     sim = Simulation(case, use_graphs=not args.no_graphs, side_stream=args.side_stream == "on",
-                     precision=precision).initialize(*inputs)
+                     precision=precision, solve=args.mixed_solve if precision else None).initialize(*inputs)
     print("[profile] initialised", flush=True)
     for s in range(args.warmup + args.steps):
         sim.substep(s % 3 + 1)
@@ -269,16 +285,18 @@ def run_profile(args):
     sim.close()
 
 
-def measure(case, inputs, args, precision):
+def measure(case, inputs, args, precision, solve=None):
     """Substeps/s of one path over K timed substeps (after W warmup substeps),
-    then per-kernel durations with HIP events over K eager substeps."""
+    then per-kernel durations with HIP events over K eager substeps.  solve
+    (mixed path): 64 the FP64 Poisson solve, 32 the FP32-stored solve with
+    FP64-residual refinement."""
     from interparts_mpich_amd.driver import Simulation
     from interparts_mpich_amd import _lib
     import numpy as np
     fields, spheres, markers = inputs
-    name = "mixed" if precision else "fp64"
+    name = ("mixed" + ("_fp64_solve" if solve == 64 else "")) if precision else "fp64"
     sim = Simulation(case, use_graphs=not args.no_graphs, side_stream=args.side_stream == "on",
-                     precision=precision).initialize(fields, spheres, markers)
+                     precision=precision, solve=solve if precision else None).initialize(fields, spheres, markers)
     rk_kb = None if args.no_autotune else sim.autotune()
     sph_blocks = sim.tuning()[1]
     for s in range(args.warmup):
@@ -295,7 +313,8 @@ def measure(case, inputs, args, precision):
     print(f"[bench] {name}: {args.steps} substeps in {t1 - t0:.4f} s ({res['value']:.1f} substeps/s)", file=sys.stderr,
           flush=True)
     if precision:
-        res["nref"] = int(os.environ.get("IPG_MIXED_REFINE", "1"))
+        res["solve"] = solve
+        res["nref"] = int(os.environ.get("IPG_MIXED_REFINE", "1")) if solve == 32 else None
         _, mdiv = sim.chkdiv()
         res["accuracy"] = {"max_div_after_timed_substeps": mdiv}
 
@@ -315,7 +334,7 @@ def measure(case, inputs, args, precision):
             a = acc.setdefault(key, {"ms": 0.0, "launches": 0, "bytes": 0.0})
             a["ms"] += ms[q]
             a["launches"] += 1
-            b = kernel_bytes_mixed(case, key, rk, res["nref"]) if precision else kernel_bytes(case, key, rk)
+            b = kernel_bytes_mixed(case, key, rk, res["nref"], solve) if precision else kernel_bytes(case, key, rk)
             if b is not None:
                 a["bytes"] += b
     # the dominant kernel again with no update beside it (the substep's own
@@ -330,7 +349,7 @@ def measure(case, inputs, args, precision):
             _lib.check(lib.ipg_substep_phases(sim.ctx, rk, maxn, ms, names, C.byref(n)), "substep_phases")
             for q in range(n.value):
                 key = names[q].decode()
-                b = kernel_bytes_mixed(case, key, rk, res["nref"]) if precision else kernel_bytes(case, key, rk)
+                b = kernel_bytes_mixed(case, key, rk, res["nref"], solve) if precision else kernel_bytes(case, key, rk)
                 if b is not None:
                     a = alone.setdefault(key, [0.0, 0, 0.0])
                     a[0] += ms[q]
@@ -351,7 +370,7 @@ def measure(case, inputs, args, precision):
                        "timed_as": ("graph replays of the timed substeps, wall-clock marks on the kernel's stream"
                                     if not args.no_graphs else "eager launches, HIP events on the kernel's stream"),
                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                       "traffic": traffic_from_profile(dominant + ("_mixed" if precision else ""), case.name),
+                       "traffic": traffic_from_profile(dominant + ("_" + name if precision else ""), case.name),
                        "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch}
     if alone and dominant in alone:
         ams, alaunch, abytes = alone[dominant]
@@ -376,6 +395,9 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed", "both"), default="both",
                     help="path(s) to time: both prints the mixed path's line with the FP64 path's figures beside it "
                          "(with --ensemble: fp64 unless mixed)")
+    ap.add_argument("--mixed-solve", type=int, choices=(32, 64), default=32,
+                    help="the mixed path's Poisson solve: 32 FP32-stored spectra with FP64-residual refinement (the "
+                         "library's default), 64 the FP64 path's solve (timed beside the others with --precision both)")
     ap.add_argument("--profile", action="store_true",
                     help="for a profiler: initialise the case, then run --warmup and --steps substeps of the path "
                          "--precision names (fp64 or mixed) and exit; prints one line per phase")
@@ -420,11 +442,14 @@ def main():
     fields, spheres, markers = initial_fields(case), place_spheres(case), markers_for(case)
     print(f"[bench] {case.name}: {case.itot}x{case.jtot}x{case.ktot}, np={case.np}, nl={case.nl}", file=sys.stderr, flush=True)
     runs = {}
-    for prec in (("fp64", "mixed") if args.precision == "both" else (args.precision,)):
-        runs[prec] = measure(case, (fields, spheres, markers), args, 1 if prec == "mixed" else 0)
-    # the line leads with the faster path; the other is reported beside it
-    head = max(runs.values(), key=lambda r: r["value"])
-    mixed = head is runs.get("mixed")
+    for prec in (("fp64", "mixed", "mixed_fp64_solve") if args.precision == "both" else (args.precision,)):
+        solve = None if prec == "fp64" else 64 if prec == "mixed_fp64_solve" else args.mixed_solve
+        runs[prec] = measure(case, (fields, spheres, markers), args, 0 if prec == "fp64" else 1, solve)
+    # the line leads with the FP64 path (the workload the metric is quoted
+    # on) when it ran; the mixed path (its FP32-stored solve with refinement)
+    # and the mixed path with the FP64 solve are reported beside it
+    head = runs["fp64"] if "fp64" in runs else runs[args.precision]
+    mixed = head is not runs.get("fp64")
     value = head["value"]
     out = {
         "metric": "RK3 substeps/s (256^3, 1k spheres) " + ("mixed precision" if mixed else "FP64"),
@@ -437,8 +462,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("mixed: u v w p dudt and particles f64, stencil and Poisson arithmetic f64, RK history and "
-                  "Poisson spectra stored f32, IBM f32" if mixed else "f64"),
+        "dtype": (mixed_dtype(head["solve"]) if mixed else "f64"),
         "data": "synthetic (seeded Poiseuille + 0.1 U_bulk noise, seeded sphere placement)",
         "config": {"workload": f"{case.name}: {case.itot}x{case.jtot}x{case.ktot} channel, {case.np} spheres "
                                f"D={case.diameter_h:g}h nl={case.nl}, Re_bulk={case.re_bulk:g}, "
@@ -453,16 +477,21 @@ def main():
         "event_sum_ms_per_substep": head["event_sum_ms_per_substep"],
     }
     if mixed:
+        out["config"]["poisson_solve"] = head["solve"]
         out["config"]["poisson_refinement_sweeps"] = head["nref"]
-    other = [r for r in runs.values() if r is not head]
-    if other:
-        o = other[0]
-        key = "fp64" if mixed else "mixed"
-        out[key] = {k: o[k] for k in ("value", "ms_per_step", "nref", "roofline", "kernel_ms_per_substep",
+    for key, o in runs.items():
+        if o is head:
+            continue
+        out[key] = {k: o[k] for k in ("value", "ms_per_step", "solve", "nref", "roofline", "kernel_ms_per_substep",
                                       "kernel_algorithmic_gbs", "event_sum_ms_per_substep") if k in o}
+        if "solve" in o:
+            out[key]["dtype"] = mixed_dtype(o["solve"])
         if "accuracy" in o:
             out[key]["accuracy"] = o["accuracy"]
-        out["mixed_over_fp64"] = round(runs["mixed"]["value"] / runs["fp64"]["value"], 4)
+    if "fp64" in runs:
+        for key in ("mixed", "mixed_fp64_solve"):
+            if key in runs:
+                out[key + "_over_fp64"] = round(runs[key]["value"] / runs["fp64"]["value"], 4)
     if "accuracy" in head:
         out["mixed_accuracy"] = head["accuracy"]
     if not args.no_copy_probe:

@@ -180,8 +180,14 @@ int ipg_use_graphs(ipg_ctx *ctx, int on);             /* capture each substep as
  * second stream (default on); off: the update closes its substep.  Turn it off
  * when many contexts share the GPU: their streams already overlap. */
 int ipg_use_side_stream(ipg_ctx *ctx, int on);
-/* mixed precision: FP64-residual refinement sweeps of the Poisson solve,
- * 0..2 (default 1, or IPG_MIXED_REFINE at init) */
+/* mixed precision: the Poisson solve, 32 (default) with its spectra stored
+ * FP32 between the passes, a pressure correction in FP32 pieces and
+ * FP64-residual refinement, or 64 the FP64 path's solve (IPG_MIXED_SOLVE=64
+ * at init) */
+int ipg_set_mixed_solve(ipg_ctx *ctx, int bits);
+int ipg_mixed_solve(ipg_ctx *ctx, int *bits);         /* 64, 32, or 0 for an FP64 context */
+/* mixed precision, FP32-stored solve: FP64-residual refinement sweeps, 0..2
+ * (default 1, or IPG_MIXED_REFINE at init) */
 int ipg_set_refinement(ipg_ctx *ctx, int nref);
 /* time the RK stage for 8..64 z planes per workgroup and keep the fastest;
  * overwrites dudt..dcdt (recomputed by every stage): call between substeps */

@@ -28,7 +28,7 @@ EXPORTS = ["ipg_init", "ipg_finalize", "ipg_last_error", "ipg_nl_formula", "ipg_
            "ipg_boundscal", "ipg_fillps", "ipg_solver2d", "ipg_correc", "ipg_pnew_update",
            "ipg_intgr_over_sphere", "ipg_intgr_nwtn_eulr", "ipg_substep", "ipg_chkdt", "ipg_chkdiv", "ipg_chkdiv_noalloc",
            "ipg_post3d",
-           "ipg_sync", "ipg_use_graphs", "ipg_use_side_stream", "ipg_set_refinement", "ipg_autotune", "ipg_tuning",
+           "ipg_sync", "ipg_use_graphs", "ipg_use_side_stream", "ipg_set_mixed_solve", "ipg_mixed_solve", "ipg_set_refinement", "ipg_autotune", "ipg_tuning",
            "ipg_substep_bytes",
            "ipg_substep_phases", "ipg_batch_create", "ipg_batch_destroy", "ipg_batch_substep", "ipg_batch_sync",
            "ipg_batch_use_graphs", "ipg_batch_set_groups", "ipg_batch_groups", "ipg_batch_set_carry", "ipg_batch_carry",
@@ -102,6 +102,8 @@ def load():
         "ipg_use_graphs": ([vp, ip], ip),
         "ipg_use_side_stream": ([vp, ip], ip),
         "ipg_set_refinement": ([vp, ip], ip),
+        "ipg_set_mixed_solve": ([vp, ip], ip),
+        "ipg_mixed_solve": ([vp, C.POINTER(C.c_int)], ip),
         "ipg_autotune": ([vp, C.POINTER(C.c_int)], ip),
         "ipg_tuning": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], ip),
         "ipg_substep_bytes": ([vp, dp], ip),

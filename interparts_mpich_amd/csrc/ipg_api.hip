@@ -336,7 +336,7 @@ void ipg_batch_leave(ipg_ctx *c, int rkiter, bool deferred, hipEvent_t done) {
   c->p_ghosts_stale = true;
   c->xcol_stale = true;
   c->dcdt_stale = true;
-  c->p_in_pieces = c->mixed;
+  c->p_in_pieces = solve_in_pieces(c);
 }
 
 extern "C" {
@@ -414,6 +414,12 @@ int ipg_init(const ipg_params *prm, ipg_ctx **out) {
     // FP32 LDS atomic adds, which stall where the FP64 ones do not (2.5x
     // slower passes at 256^3)
     if (const char *e = std::getenv("IPG_MIXED_IBM")) c->ibm32 = std::atoi(e) == 32;
+    // the Poisson solve stores its spectra FP32 and refines with FP64
+    // residuals unless IPG_MIXED_SOLVE=64 (the FP64 path's solve, which is
+    // faster: the FP32-stored solve needs a second solve to keep max |div u|
+    // under 1e-10, and the two take 0.86 against 0.43 ms at 256^3)
+    c->solve32 = true;
+    if (const char *e = std::getenv("IPG_MIXED_SOLVE")) c->solve32 = std::atoi(e) != 64;
     c->nref = 1;
     if (const char *e = std::getenv("IPG_MIXED_REFINE")) c->nref = std::max(0, std::min(IPG_MAX_REFINE, std::atoi(e)));
     // the RK history is stored FP32, each field in its own slot
@@ -698,8 +704,8 @@ int ipg_solver2d(ipg_ctx *c) {
   if (!ready(c) || fill_p(c)) return 1;
   if (!c->solver_ready) { ipg_set_error("solver2d: ipg_initsolver has not been called"); return 1; }
   if (ipg_launch_poisson(c)) return 1;
-  // the mixed path solves into FP32 pieces: p = their sum, as the field
-  if (c->mixed && fill_p(c)) return 1;
+  // the FP32-stored solve leaves FP32 pieces: p = their sum, as the field
+  if (solve_in_pieces(c) && fill_p(c)) return 1;
   IPG_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -769,7 +775,7 @@ int ipg_substep(ipg_ctx *c, int rkiter) {
     c->p_ghosts_stale = true;
     c->xcol_stale = true;
     c->dcdt_stale = true;
-    c->p_in_pieces = c->mixed;
+    c->p_in_pieces = solve_in_pieces(c);
   } else if (substep_launches(c, rkiter, pend, defer)) {
     return 1;
   }
@@ -858,7 +864,7 @@ int ipg_substep_phases(ipg_ctx *c, int rkiter, int maxn, double *ms, const char 
     IPG_HIP(hipStreamSynchronize(c->stream));
     (void)hipGraphExecDestroy(exec);
     c->p_ghosts_stale = c->xcol_stale = c->dcdt_stale = true;
-    c->p_in_pieces = c->mixed;
+    c->p_in_pieces = solve_in_pieces(c);
   } else {
     IPG_HIP(hipEventRecord(c->ev[0], c->stream));
     rc = substep_launches(c, rkiter, pend, defer);
@@ -969,6 +975,27 @@ int ipg_set_refinement(ipg_ctx *c, int nref) {
   return 0;
 }
 
+int ipg_set_mixed_solve(ipg_ctx *c, int bits) {
+  if (!ready(c)) return 1;
+  if (!c->mixed) { ipg_set_error("set_mixed_solve: the context is not mixed precision"); return 1; }
+  if (bits != 32 && bits != 64) { ipg_set_error("set_mixed_solve: bits must be 32 or 64"); return 1; }
+  // p as the FP64 field before the solve that wrote it changes form
+  if (fill_p(c)) return 1;
+  if ((bits == 32) != c->solve32) {
+    c->gen++;
+    for (int r = 0; r < IPG_NGRAPH; r++)
+      if (c->graph_ok[r]) { (void)hipGraphExecDestroy(c->graph[r]); c->graph_ok[r] = false; }
+  }
+  c->solve32 = bits == 32;
+  return 0;
+}
+
+int ipg_mixed_solve(ipg_ctx *c, int *bits) {
+  if (!check_ctx(c) || !bits) return 1;
+  *bits = !c->mixed ? 0 : c->solve32 ? 32 : 64;
+  return 0;
+}
+
 // Algorithmic HBM bytes of one substep, the roofline numerator.
 int ipg_substep_bytes(ipg_ctx *c, double *bytes) {
   if (!check_ctx(c) || !bytes) return 1;
@@ -994,8 +1021,13 @@ int ipg_substep_bytes(ipg_ctx *c, double *bytes) {
   const double ibm = np * 3.0 * 3.0 * 3.0 * shell;
   const double box = np * 3.0 * std::pow(2.0 * R / h + 5.0, 3.0);
   *bytes = 8.0 * (fluid + ibm + box);
-  if (c->mixed) {
-    // mixed path, bytes: RK stage 72 N FP64 (u v w c pnew read, dudt..dcdt
+  if (c->mixed && !c->solve32) {
+    // mixed path with the FP64 solve: the FP64 figure less the RK history's
+    // 4 B per word (4 writes at stage 1, 4 reads and 4 writes at stage 2, 4
+    // reads at stage 3: 16/3 words per cell and stage)
+    *bytes -= 4.0 * (16.0 / 3.0) * N;
+  } else if (c->mixed) {
+    // mixed path with the FP32-stored solve, bytes: RK stage 72 N FP64 (u v w c pnew read, dudt..dcdt
     // written) + the FP32 history (4 written, 4 read and written, 4 read by
     // stage) 64/3 N; the first forward x pass reads dudt dvdt dwdt and stores
     // the right-hand side (32 N) and the FP32 spectrum (4 Sw); refinement r

@@ -770,7 +770,7 @@ int ipg_launch_resid(ipg_ctx *c, int npieces) {
 int ipg_launch_correc_fused(ipg_ctx *c, int rkiter) {
   const Grid &g = c->g;
   int rc;
-  if (c->mixed) {
+  if (solve_in_pieces(c)) {
     const CorrecPiecesArgs a{g, c->f[IPG_DUDT], c->f[IPG_DVDT], c->f[IPG_DWDT], pieces_of(c), c->f[IPG_U], c->f[IPG_V],
                              c->f[IPG_W], c->f[IPG_PNEW], c->dscal, c->pc.rkab[rkiter - 1]};
     if (c->nref == 0) rc = ipg_go<CorrecPiecesK<1>>(c, k_correc_pieces<1>, cell_grid(g), dim3(BX, BY), 0, a, "correc");

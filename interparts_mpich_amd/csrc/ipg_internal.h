@@ -76,7 +76,10 @@ struct ipg_ctx {
   bool mixed;                     // precision 1
   bool ibm32;                     // the IBM computes in FP32 (mixed path with IPG_MIXED_IBM=32)
   bool f32[IPG_NFIELDS];          // field stored FP32 (its slot read as float, same indexing)
-  int nref;                       // FP64-residual refinement sweeps of the mixed Poisson solve (0..2)
+  bool solve32;                   // mixed path: the Poisson solve stores its spectra FP32 and refines with FP64
+                                  // residuals (the default); else it is the FP64 path's solve (IPG_MIXED_SOLVE=64,
+                                  // ipg_set_mixed_solve)
+  int nref;                       // FP64-residual refinement sweeps of the FP32-stored Poisson solve (0..2)
   float *pbuf;                    // their allocation
   float *pp[IPG_MAX_REFINE + 2];  // the pressure correction's FP32 pieces 0..nref, then the refinement residual (field indexing)
   bool p_in_pieces;               // p is the sum of the pieces; the P field holds the right-hand side
@@ -171,6 +174,9 @@ inline int ipg_ctx_allocations(const ipg_ctx *c, void **out) {
 
 // an FP32 field's base pointer: the same slot and indexing as f[q], float
 // elements (element -1, the ghost i = 0, exists)
+// The pressure correction comes as FP32 pieces (the mixed path's FP32-stored
+// solve with its refinements) rather than as the FP64 P field.
+inline bool solve_in_pieces(const ipg_ctx *c) { return c->mixed && c->solve32; }
 inline float *f32ptr(ipg_ctx *c, int q) { return (float *)(c->slab + (size_t)q * c->nfield) + 16; }
 
 // a wall-clock mark into stamps[slot] on the context's current stream

@@ -982,7 +982,7 @@ int poisson_mixed(ipg_ctx *c, int fused_rkiter) {
 // fused_rkiter > 0: the x forward pass computes the right-hand side of
 // fillps (src/fillps.f90) itself from dudt/dvdt/dwdt instead of reading p.
 int ipg_launch_poisson_ex(ipg_ctx *c, int fused_rkiter) {
-  if (c->mixed) return poisson_mixed(c, fused_rkiter);
+  if (solve_in_pieces(c)) return poisson_mixed(c, fused_rkiter);
   const Grid &g = c->g;
   Poisson &P = c->ps2;
   Radices rx, ry;

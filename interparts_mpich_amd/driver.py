@@ -22,9 +22,11 @@ def _ptr(a):
 
 
 class Simulation:
-    def __init__(self, case: Case, use_graphs=False, side_stream=True, precision=0, nref=None):
-        """precision 0: the FP64 path; 1: mixed precision (ipg_params.precision),
-        with nref FP64-residual refinement sweeps of the Poisson solve."""
+    def __init__(self, case: Case, use_graphs=False, side_stream=True, precision=0, nref=None, solve=None):
+        """precision 0: the FP64 path; 1: mixed precision (ipg_params.precision).
+        solve (mixed only): 32, the Poisson solve with FP32-stored spectra and
+        nref FP64-residual refinement sweeps (the library's default), or 64,
+        the FP64 path's solve; giving nref alone selects 32."""
         self.case = case
         self.precision = precision
         self.lib = load()
@@ -39,6 +41,10 @@ class Simulation:
         check(self.lib.ipg_init(C.byref(prm), C.byref(self.ctx)), "ipg_init")
         if use_graphs:
             check(self.lib.ipg_use_graphs(self.ctx, 1), "ipg_use_graphs")
+        if solve is None and nref is not None:
+            solve = 32
+        if solve is not None:
+            check(self.lib.ipg_set_mixed_solve(self.ctx, solve), "ipg_set_mixed_solve")
         if nref is not None:
             check(self.lib.ipg_set_refinement(self.ctx, nref), "ipg_set_refinement")
         if not side_stream:

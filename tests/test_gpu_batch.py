@@ -29,9 +29,10 @@ def rel(a, b):
     return float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300)
 
 
-def make(case, precision=0, graphs=True):
-    print(f"[case {case.name}] {case.itot}x{case.jtot}x{case.ktot} np={case.np} precision={precision}", flush=True)
-    return Simulation(case, use_graphs=graphs, side_stream=False, precision=precision).initialize(
+def make(case, precision=0, graphs=True, solve=None):
+    print(f"[case {case.name}] {case.itot}x{case.jtot}x{case.ktot} np={case.np} precision={precision} solve={solve}",
+          flush=True)
+    return Simulation(case, use_graphs=graphs, side_stream=False, precision=precision, solve=solve).initialize(
         initial_fields(case), place_spheres(case), markers_for(case) if case.np > 0 else None)
 
 
@@ -183,10 +184,11 @@ def test_batch_tuner_keeps_results():
     b.close()
 
 
-def test_batch_mixed_precision():
+@pytest.mark.parametrize("solve", [64, 32])
+def test_batch_mixed_precision(solve):
     cases = cases_small()[:4]
-    bat = [make(c, precision=1) for c in cases]
-    alone = [make(c, precision=1) for c in cases]
+    bat = [make(c, precision=1, solve=solve) for c in cases]
+    alone = [make(c, precision=1, solve=solve) for c in cases]
     b = Batch(bat)
     for step in range(2):
         for rk in (1, 2, 3):
@@ -204,13 +206,15 @@ def test_batch_mixed_precision():
 
 
 def test_batch_of_fp64_and_mixed_cases():
-    """FP64 and mixed-precision cases in one batch: the FP64 substep's stages
-    are a subsequence of the mixed one's (which adds the refinement solve),
-    and each stage merges the cases of each body (FP64 and FP32 spectra apart)."""
+    """FP64 and mixed-precision cases in one batch, the mixed ones with each
+    Poisson solve: the FP64 substep's stages are a subsequence of the mixed
+    one's (whose FP32-stored solve adds the refinement solve), and each stage
+    merges the cases of each body (FP64 and FP32 spectra apart)."""
     cases = cases_small()[:4]
     precs = [0, 1, 0, 1]
-    bat = [make(c, precision=p) for c, p in zip(cases, precs)]
-    alone = [make(c, precision=p) for c, p in zip(cases, precs)]
+    solves = [None, 64, None, 32]
+    bat = [make(c, precision=p, solve=v) for c, p, v in zip(cases, precs, solves)]
+    alone = [make(c, precision=p, solve=v) for c, p, v in zip(cases, precs, solves)]
     b = Batch(bat)
     for step in range(2):
         for rk in (1, 2, 3):

@@ -1,11 +1,11 @@
 """The mixed-precision path (ipg_params.precision = 1) against the FP64 oracle
 and the FP64 GPU path, on identical seeded inputs.
 
-Mixed precision keeps u, v, w, pnew, c, dudt..dcdt, the particles and the
-sphere integrals in FP64, stores the RK history, the Poisson spectra and the
-pressure correction's pieces in FP32 (the stencil and Poisson arithmetic stay
-FP64), refines the Poisson solve with FP64 residuals, and computes the IBM in
-FP32.  The bars, as the metric states them per result:
+Mixed precision keeps u, v, w, pnew, c, dudt..dcdt, the particles, the
+sphere integrals and the IBM in FP64 and stores the RK history in FP32.  Its
+Poisson solve either stores the spectra and the pressure correction's pieces
+in FP32 (the arithmetic stays FP64) and refines with FP64 residuals (solve
+32, the default) or is the FP64 path's (solve 64); the tests run both.  The bars, as the metric states them per result:
 
 - max |div u| (src/chkdiv.f90:23-39) <= 1e-10 after every substep;
 - velocity L2 relative error against the FP64 oracle <= 1e-6 after the
@@ -13,9 +13,8 @@ FP32.  The bars, as the metric states them per result:
 - particle-velocity error against the oracle, printed with the others (held
   here to <= 1e-5 of the largest particle speed).
 
-Runs of the mixed path differ from one another at FP32 rounding (the IBM sums
-with FP32 atomics in any order), so comparisons between two mixed runs hold to
-1e-6, not to FP64 rounding.
+Comparisons between two mixed runs are held to 1e-6, the FP32 storage's
+rounding, not to FP64 rounding.
 """
 import numpy as np
 import pytest
@@ -54,10 +53,14 @@ def inputs(case):
     return initial_fields(case), place_spheres(case), markers_for(case)
 
 
-def test_mixed_ch64_100_steps_against_fp64_oracle():
+SOLVES = [64, 32]
+
+
+@pytest.mark.parametrize("solve", SOLVES)
+def test_mixed_ch64_100_steps_against_fp64_oracle(solve):
     case = CONFIGS["ch64_8sph"]
     fields, spheres, markers = inputs(case)
-    m = Simulation(case, use_graphs=True, precision=1).initialize(fields, spheres, markers)
+    m = Simulation(case, use_graphs=True, precision=1, solve=solve).initialize(fields, spheres, markers)
     o = Oracle(case).initialize(fields, spheres, markers)
     worst_div = 0.0
     for s in range(case.steps):
@@ -72,41 +75,43 @@ def test_mixed_ch64_100_steps_against_fp64_oracle():
                   f"{float(np.min(np.minimum(o.pget('Z'), case.lz - o.pget('Z')))) / case.radius - 1.0:.4f}",
                   flush=True)
     l2, pe = vel_l2(m, o), part_err(m, o)
-    print(f"ch64_8sph mixed after {case.steps} steps: max|div| {worst_div:.3e} (worst check), velocity L2 {l2:.3e}, "
+    print(f"ch64_8sph mixed (solve {solve}) after {case.steps} steps: max|div| {worst_div:.3e} (worst check), velocity L2 {l2:.3e}, "
           f"particle velocity {pe:.3e}")
     assert l2 <= 1e-6, l2
     assert pe <= 1e-5, pe
 
 
+@pytest.mark.parametrize("solve", SOLVES)
 @pytest.mark.parametrize("name", ["t32", "ch64_8sph"])
-def test_mixed_one_step_against_fp64_gpu(name):
+def test_mixed_one_step_against_fp64_gpu(name, solve):
     case = CONFIGS[name]
     fields, spheres, markers = inputs(case)
-    m = Simulation(case, precision=1).initialize(fields, spheres, markers)
+    m = Simulation(case, precision=1, solve=solve).initialize(fields, spheres, markers)
     d = Simulation(case).initialize(fields, spheres, markers)
     m.step(); d.step()
     _, mdiv = m.chkdiv()
     l2, pe = vel_l2(m, d), part_err(m, d)
-    print(name, "max|div|", mdiv, "velocity L2", l2, "particle velocity", pe)
+    print(name, f"solve {solve}", "max|div|", mdiv, "velocity L2", l2, "particle velocity", pe)
     assert mdiv <= 1e-10
     assert l2 <= 1e-7, l2
     assert pe <= 1e-6, pe
-    # the pressure level is the solve's free constant, which the mixed path
-    # pins at p(kmax) = 0 for the mean mode: compared without its mean
+    # the pressure level is the solve's free constant, which the FP32-stored
+    # solve pins at p(kmax) = 0 for the mean mode: compared without its mean
     pm, pd = m.get("PNEW")[1:-1, 1:-1, 1:-1], d.get("PNEW")[1:-1, 1:-1, 1:-1]
     r = rel(pm - pm.mean(), pd - pd.mean())
     print(name, "pnew (less its mean)", r)
     assert r <= 1e-5, r
 
 
-def test_mixed_split_entries_match_fused():
+@pytest.mark.parametrize("solve", SOLVES)
+def test_mixed_split_entries_match_fused(solve):
     """The reference-shaped entries one by one (rk, IBM, bounduvw, fillps,
     solver2d, boundp, correc, pnew_update, intgr_nwtn_eulr) on the mixed path
-    equal its fused substep to FP32 rounding of the IBM sums."""
+    equal its fused substep."""
     case = CONFIGS["t32"]
     fields, spheres, markers = inputs(case)
-    a = Simulation(case, precision=1).initialize(fields, spheres, markers)
-    b = Simulation(case, precision=1, side_stream=False).initialize(fields, spheres, markers)
+    a = Simulation(case, precision=1, solve=solve).initialize(fields, spheres, markers)
+    b = Simulation(case, precision=1, side_stream=False, solve=solve).initialize(fields, spheres, markers)
     for rk in (1, 2, 3):
         a.rk(rk); a.ibm(); a.bounduvw(1); a.fillps(rk); a.solver2d(); a.boundp(1); a.correc(rk); a.pnew_update()
         a.intgr_nwtn_eulr(rk)
@@ -122,11 +127,12 @@ def test_mixed_split_entries_match_fused():
     assert r <= 1e-5, r
 
 
-def test_mixed_graph_replay_matches_eager():
+@pytest.mark.parametrize("solve", SOLVES)
+def test_mixed_graph_replay_matches_eager(solve):
     case = CONFIGS["t32"]
     fields, spheres, markers = inputs(case)
-    a = Simulation(case, precision=1).initialize(fields, spheres, markers)
-    b = Simulation(case, use_graphs=True, precision=1).initialize(fields, spheres, markers)
+    a = Simulation(case, precision=1, solve=solve).initialize(fields, spheres, markers)
+    b = Simulation(case, use_graphs=True, precision=1, solve=solve).initialize(fields, spheres, markers)
     for _ in range(3):
         a.step(); b.step()
         assert vel_l2(b, a) <= 1e-6
@@ -174,3 +180,70 @@ def test_mixed_fullsize_divergence_and_refinement(name):
     assert divs[0] > 1e-10
     assert divs[1] <= 1e-10, divs
     assert divs[2] <= 1e-10, divs
+
+
+def test_mixed_fp64_solve_is_the_fp64_solve():
+    """With the FP64 solve, the mixed path's fillps and solver2d are the FP64
+    path's: from the same dudt, dvdt, dwdt, p bit for bit."""
+    case = CONFIGS["ch64_8sph"]
+    fields, spheres, markers = inputs(case)
+    d = Simulation(case).initialize(fields, spheres, markers)
+    m = Simulation(case, precision=1, solve=64).initialize(fields, spheres, markers)
+    rng = np.random.default_rng(5)
+    for n in ("DUDT", "DVDT", "DWDT"):
+        a = rng.standard_normal(d.shape)
+        d.put(n, a)
+        m.put(n, a)
+    for s in (d, m):
+        s.bounduvw(1)
+        s.fillps(2)
+        s.solver2d()
+    assert np.array_equal(m.get("P"), d.get("P"))
+    d.close()
+    m.close()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_mixed_fp64_solve_first_substep_against_fp64(graphs):
+    """With the FP64 solve, the mixed path's first RK substep only writes the
+    FP32 history (it reads none) and runs the FP64 path's arithmetic: its
+    fields and particles equal the FP64 path's to rounding (the RK stage is a
+    kernel compiled apart, for FP32 history stores); the second substep reads
+    the history and differs at FP32 rounding.  Also the solve selection, its
+    refusals, and switching it between substeps."""
+    import ctypes as C
+    case = CONFIGS["ch64_8sph"]
+    fields, spheres, markers = inputs(case)
+    d = Simulation(case, use_graphs=graphs).initialize(fields, spheres, markers)
+    m = Simulation(case, use_graphs=graphs, precision=1).initialize(fields, spheres, markers)
+    bits = C.c_int()
+    assert m.lib.ipg_mixed_solve(m.ctx, C.byref(bits)) == 0 and bits.value == 32   # the default
+    assert m.lib.ipg_set_mixed_solve(m.ctx, 64) == 0
+    assert m.lib.ipg_mixed_solve(m.ctx, C.byref(bits)) == 0 and bits.value == 64
+    assert d.lib.ipg_mixed_solve(d.ctx, C.byref(bits)) == 0 and bits.value == 0
+    assert d.lib.ipg_set_mixed_solve(d.ctx, 64) != 0        # not a mixed context
+    assert m.lib.ipg_set_mixed_solve(m.ctx, 16) != 0        # not 32 or 64
+    d.substep(1); m.substep(1)
+    d.sync(); m.sync()
+    worst = max(rel(m.get(n)[1:-1, 1:-1, 1:-1], d.get(n)[1:-1, 1:-1, 1:-1]) for n in FLUID)
+    pw = max(rel(m.pget(n), d.pget(n)) for n in PVEL + ("X", "Y", "Z"))
+    print(f"graphs {graphs}: first substep, fields {worst:.3e}, particles {pw:.3e}", flush=True)
+    assert worst <= 1e-12, worst
+    assert pw <= 1e-12, pw
+    d.substep(2); m.substep(2)
+    d.sync(); m.sync()
+    l2 = vel_l2(m, d)
+    print(f"graphs {graphs}: second substep velocity L2 vs FP64 {l2:.3e}", flush=True)
+    assert l2 <= 1e-7, l2
+    # switching the solve between substeps: 32 from here on, then back
+    assert m.lib.ipg_set_mixed_solve(m.ctx, 32) == 0
+    assert m.lib.ipg_mixed_solve(m.ctx, C.byref(bits)) == 0 and bits.value == 32
+    d.substep(3); m.substep(3)
+    d.sync(); m.sync()
+    assert m.chkdiv()[1] <= 1e-10
+    assert vel_l2(m, d) <= 1e-6
+    assert m.lib.ipg_set_mixed_solve(m.ctx, 64) == 0
+    d.substep(1); m.substep(1)
+    d.sync(); m.sync()
+    assert m.chkdiv()[1] <= 1e-10
+    assert vel_l2(m, d) <= 1e-6
