@@ -370,7 +370,7 @@ def measure(case, inputs, args, precision, solve=None):
                        "timed_as": ("graph replays of the timed substeps, wall-clock marks on the kernel's stream"
                                     if not args.no_graphs else "eager launches, HIP events on the kernel's stream"),
                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                       "traffic": traffic_from_profile(dominant + ("_" + name if precision else ""), case.name),
+                       "traffic": traffic_from_profile(dominant + ("_mixed" if precision else ""), case.name),
                        "avg_launch_ms": round(avg_ms, 5), "bytes_per_launch": bytes_per_launch}
     if alone and dominant in alone:
         ams, alaunch, abytes = alone[dominant]
