@@ -13,7 +13,8 @@ module mod_ipg_capi
             ipg_ibm, ipg_ibm_begin, ipg_eulr2lagr, ipg_lagrforces, ipg_lagr2eulr, ipg_bounduvw, ipg_boundp, &
             ipg_boundscal, ipg_fillps, ipg_solver2d, ipg_correc, ipg_pnew_update, ipg_intgr_over_sphere, &
             ipg_intgr_nwtn_eulr, ipg_substep, ipg_chkdt, ipg_chkdiv, ipg_uniform, ipg_sync, ipg_use_graphs, &
-            ipg_use_side_stream, ipg_autotune, ipg_tuning, ipg_post3d, ipg_kerneltest, ipg_ibm_forces
+            ipg_use_side_stream, ipg_autotune, ipg_tuning, ipg_post3d, ipg_kerneltest, ipg_ibm_forces, &
+            ipg_set_mixed_solve, ipg_set_refinement
   ! the batched ensemble (no reference counterpart: many cases through each
   ! substep together, include/ipg.h)
   public :: ipg_batch_create, ipg_batch_destroy, ipg_batch_substep, ipg_batch_sync, ipg_batch_autotune
@@ -227,6 +228,17 @@ module mod_ipg_capi
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: on
+    end function
+    ! mixed precision: the Poisson solve (32 FP32-stored with refinement, 64 FP64)
+    integer(c_int) function ipg_set_mixed_solve(ctx, bits) bind(C, name="ipg_set_mixed_solve")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: bits
+    end function
+    integer(c_int) function ipg_set_refinement(ctx, nref) bind(C, name="ipg_set_refinement")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nref
     end function
     integer(c_int) function ipg_autotune(ctx, kb) bind(C, name="ipg_autotune")
       import :: c_int, c_ptr

@@ -41,7 +41,11 @@ module mod_param
   logical :: use_graphs = .true., fused_substep = .true.
   ! the per-pass lines of updtintermediatevel (src/forcing.f90:430-468)
   logical :: ibm_log = .true.
-  namelist /runcontrol/ begin, nstep, seed, perturb, use_graphs, fused_substep, ibm_log
+  ! arithmetic (no reference counterpart): precision 0 FP64, 1 mixed (FP32 RK
+  ! history; Poisson solve mixed_solve = 32, FP32-stored spectra with nref
+  ! FP64-residual refinement sweeps, or 64, the FP64 solve)
+  integer :: precision = 0, mixed_solve = 32, nref = 1
+  namelist /runcontrol/ begin, nstep, seed, perturb, use_graphs, fused_substep, ibm_log, precision, mixed_solve, nref
   integer :: nl = 0
   integer :: imax, jmax, kmax, i1, j1, k1
   real(8), parameter :: pi = acos(-1.d0)
@@ -104,8 +108,12 @@ contains
     prm%bc_c_top = merge(1, 0, trim(bc_c_top_type) == 'BC_NEUMANN')
     prm%bc_c_bot = merge(1, 0, trim(bc_c_bot_type) == 'BC_NEUMANN')
     prm%bc_c_top_val = bc_c_top_val; prm%bc_c_bot_val = bc_c_bot_val
-    prm%precision = 0
+    prm%precision = precision
     call ipg_ok(ipg_init(prm, ctx), 'ipg_init')
+    if (precision == 1) then
+      call ipg_ok(ipg_set_mixed_solve(ctx, int(mixed_solve, c_int)), 'set_mixed_solve')
+      if (mixed_solve == 32) call ipg_ok(ipg_set_refinement(ctx, int(nref, c_int)), 'set_refinement')
+    end if
     if (use_graphs) call ipg_ok(ipg_use_graphs(ctx, 1), 'ipg_use_graphs')
     allocate(hbuf(0:i1, 0:j1, 0:k1))
     s = ' '

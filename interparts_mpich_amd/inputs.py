@@ -287,15 +287,18 @@ def write_namelist(path, case, iout2d=100000):
     return path
 
 
-def write_run_nml(path, case, nstep, begin=0, use_graphs=True, fused_substep=True, t_end=1.0e30):
+def write_run_nml(path, case, nstep, begin=0, use_graphs=True, fused_substep=True, t_end=1.0e30, precision=0,
+                  mixed_solve=32):
     """run.nml of the Fortran driver: &GRIDSIZE (the compile-time sizes of
-    src/gridsize.f90:6-26) and &RUNCONTROL (start step, step count, seed)."""
+    src/gridsize.f90:6-26) and &RUNCONTROL (start step, step count, seed; the
+    arithmetic: precision 0 FP64, 1 mixed with its Poisson solve mixed_solve)."""
     tf = {True: ".true.", False: ".false."}
     with open(path, "w") as f:
         f.write(f"&GRIDSIZE itot = {case.itot}, jtot = {case.jtot}, ktot = {case.ktot}, np = {case.np}, "
                 f"nlmax = {case.nl if case.np > 0 else 0}, t_end = {t_end!r} /\n")
         f.write(f"&RUNCONTROL begin = {begin}, nstep = {nstep}, seed = {case.seed}, perturb = {case.perturb!r}, "
-                f"use_graphs = {tf[use_graphs]}, fused_substep = {tf[fused_substep]} /\n")
+                f"use_graphs = {tf[use_graphs]}, fused_substep = {tf[fused_substep]}, precision = {precision}, "
+                f"mixed_solve = {mixed_solve} /\n")
     return path
 
 

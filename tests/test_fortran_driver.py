@@ -238,3 +238,32 @@ def test_driver_output_files(tmp_path):
     info = np.loadtxt(os.path.join(d, "info_1dstat.out"))
     assert info.shape == (8,) and info[0] == 2.0 and info[3] == case.ktot
     assert abs(info[2] - case.visc) <= 1e-7 * case.visc
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mixed_solve", [32, 64])
+def test_driver_mixed_precision(tmp_path, mixed_solve):
+    """The Fortran driver with precision = 1 in &RUNCONTROL (the mixed path:
+    FP32 RK history; the FP32-stored Poisson solve with one FP64-residual
+    refinement, or the FP64 solve): three steps of the 32^3 case with a dt
+    update after every step, the fields within 1e-6 of the FP64 oracle's."""
+    from oracle.pyoracle import Oracle
+    needs_exe()
+    case = dataclasses.replace(CONFIGS["t32"], ioutchk=1)
+    d = write_case_dir(str(tmp_path), case, nstep=3, precision=1, mixed_solve=mixed_solve)
+    r = run(d)
+    assert r.returncode == 0
+    o = Oracle(case).initialize(initial_fields(case), place_spheres(case), markers_for(case))
+    for _ in range(3):
+        o.step()
+    f, info = read_fld(fld_path(d, 3), case)
+    assert info[1] == 3.0
+    for n in FLUID:
+        a, b = f[n], o.get(n)[1:-1, 1:-1, 1:-1]
+        if n == "PNEW":
+            # the pressure level is the solve's free constant, which the
+            # FP32-stored solve pins at p(kmax) = 0 for the mean mode
+            a, b = a - a.mean(), b - b.mean()
+        e = rel(a, b)
+        print(f"mixed solve {mixed_solve}: {n} {e:.3e}", flush=True)
+        assert e <= 1e-6, n
